@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Throughput of the Strided State UniFrac hot path on one MI355X.
+
+Workload (default): random binary tree with 100,000 tips (Exp(0.1) lengths,
+FP32-rounded), 16,384 samples at 1% tip density with Zipf(1.1) tip popularity,
+lognormal counts -- the "16k samples" configuration.  One step computes the
+full 16,384 x 16,384 unweighted matrix (exact int8-MFMA Gram) and the full
+weighted_normalized matrix (FP32 VALU pair tiles with FP64 merges), both with
+the table and tree already resident in HBM and the matrices left in HBM.
+
+metric: pair-node updates/sec = sum over both methods of P * N / t, with
+P = n(n-1)/2 and N the nodes the method processes after shear/collapse.
+
+Prints one JSON line (rank 0, N=1).  Extra fields: per-kernel HIP-event
+timings, the max relative error of both matrices against the FP64 oracle on a
+48-sample subset, the plain-FP32 unweighted kernel for comparison, a roofline
+entry for the dominant kernel and the CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "unifrac-binaries_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 (vector/matrix) peak
+PEAK_INT8_TOPS = 5000.0     # MI355X published dense INT8 MFMA peak, TOP/s
+FLOPS_PER_PAIR_NODE_W = 4   # sub, abs, fused multiply-add
+OPS_PER_PAIR_NODE_GRAM = 8  # 4 int8 digit passes x one multiply-add
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def sub_table(table, idx):
+    """CSR restricted to the samples idx (re-indexed), empty observations dropped."""
+    obs_ids, sample_ids, indptr, indices, data = table
+    remap = -np.ones(len(sample_ids), np.int64)
+    remap[idx] = np.arange(len(idx))
+    keep = remap[indices] >= 0
+    rows = np.repeat(np.arange(len(obs_ids)), np.diff(indptr.astype(np.int64)))
+    r, c, v = rows[keep], remap[indices[keep]], data[keep]
+    order = np.lexsort((c, r))
+    r, c, v = r[order], c[order], v[order]
+    used, counts = np.unique(r, return_counts=True)
+    ip = np.zeros(used.size + 1, np.uint32)
+    np.cumsum(counts, out=ip[1:])
+    return ([obs_ids[i] for i in used], [sample_ids[i] for i in idx], ip, c.astype(np.uint32), v)
+
+
+def max_rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    off = ~np.eye(a.shape[0], dtype=bool)
+    d = np.abs(a - b)[off]
+    s = np.abs(b)[off]
+    return float(np.max(np.where(s > 0, d / np.where(s > 0, s, 1), d)))
+
+
+def cpu_baseline(tree, table, methods, budget_s):
+    """Reference CPU code (oracle/_ref) on stripe sub-ranges; extrapolated to the full matrix."""
+    import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    otree = O.Tree(tree[0].astype(np.uint8), tree[1], tree[2])
+    otab = O.Table(*table)
+    n = otab.n_samples
+    S = (n + 1) // 2
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    total_time, per = 0.0, {}
+    for method in methods:
+        def run(k):
+            t0 = time.perf_counter()
+            if kind == "reference":
+                O.ref_stripes(otree, otab, method, fp32=True, start=0, stop=k)
+            else:
+                O.stripes(otree, otab, method, start=0, stop=k, threads=threads)
+            return time.perf_counter() - t0
+        t1 = run(1)
+        if t1 * 4 < budget_s / len(methods):
+            k2 = 3
+            t2 = run(k2)
+            slope = max((t2 - t1) / (k2 - 1), 1e-9)
+            intercept = max(t1 - slope, 0.0)
+        else:
+            slope, intercept = t1, 0.0
+        full = intercept + S * slope
+        per[method] = {"stripe_s": slope, "setup_s": intercept, "full_matrix_s": full}
+        total_time += full
+        log(f"cpu baseline {method}: {slope:.3f} s/stripe, setup {intercept:.2f} s, full {full:.0f} s")
+    return kind, threads, total_time, per
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="unw_100k_16384")
+    ap.add_argument("--samples", type=int, default=None)
+    ap.add_argument("--tips", type=int, default=None)
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of reference CPU work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 unweighted kernel")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is a design note, not an implemented path", file=sys.stderr)
+        sys.exit(2)
+
+    import ssu
+    import synth
+
+    if not ssu.device_available():
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    t0 = time.perf_counter()
+    tree, table = synth.workload(args.workload, samples=args.samples, tips=args.tips)
+    n = len(table[1])
+    log(f"workload {args.workload}: {len(tree[0]) // 2} nodes, {n} samples, nnz {table[4].size} "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+    inp = ssu.Inputs(*tree, *table)
+    plans = {
+        "unweighted": ssu.Plan(inp, "unweighted"),
+        "weighted_normalized": ssu.Plan(inp, "weighted_normalized"),
+    }
+    for name, p in plans.items():
+        log(f"plan {name}: main kernel {p.main_kernel()}, pair-nodes {p.pair_nodes():.4g}")
+    for _ in range(args.warmup):
+        for p in plans.values():
+            p.execute()
+    log("warmup done")
+    times = {k: [] for k in plans}
+    t_start = time.perf_counter()
+    for step in range(args.steps):
+        for name, p in plans.items():
+            p.execute()
+            times[name].append(p.timings())
+        log(f"step {step + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1000.0 * elapsed / args.steps
+    pair_nodes = {k: p.pair_nodes() for k, p in plans.items()}
+    value = sum(pair_nodes.values()) / (elapsed / args.steps)
+
+    def avg(name, key):
+        return float(np.mean([t[key] for t in times[name]]))
+
+    kernels = {}
+    for name, p in plans.items():
+        kernels[name] = {"kernel": p.main_kernel(), "main_ms": avg(name, "main"), "embed_ms": avg(name, "embed"),
+                         "prep_ms": avg(name, "prep"), "combine_ms": avg(name, "combine"),
+                         "device_total_ms": avg(name, "total"),
+                         "pair_node_per_s": pair_nodes[name] / (avg(name, "total") / 1000.0)}
+    # dominant kernel: the one with the largest main-kernel time
+    dom = max(kernels, key=lambda k: kernels[k]["main_ms"])
+    dk = kernels[dom]
+    if dk["kernel"] == "k_gram_i8":
+        achieved = OPS_PER_PAIR_NODE_GRAM * pair_nodes[dom] / (dk["main_ms"] / 1000.0) / 1e12
+        roofline = {"kernel": dk["kernel"], "bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS,
+                    "unit": "TOP/s", "frac": achieved / PEAK_INT8_TOPS, "traffic": None}
+    else:
+        achieved = FLOPS_PER_PAIR_NODE_W * pair_nodes[dom] / (dk["main_ms"] / 1000.0) / 1e12
+        roofline = {"kernel": dk["kernel"], "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                    "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None}
+    g = kernels["unweighted"]
+    g_ops = OPS_PER_PAIR_NODE_GRAM * pair_nodes["unweighted"] / (g["main_ms"] / 1000.0) / 1e12
+    kernels["unweighted"]["mfma_int8_tops"] = g_ops
+    kernels["unweighted"]["mfma_frac"] = g_ops / PEAK_INT8_TOPS
+    prof = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(prof):
+        tr = json.load(open(prof)).get(f"{args.workload}:{dk['kernel']}")
+        if tr and not args.samples and not args.tips:
+            roofline["traffic"] = tr
+
+    # accuracy against the FP64 oracle on a sample subset
+    import oracle as O
+    rng = np.random.default_rng(7)
+    idx = np.sort(rng.choice(n, size=min(48, n), replace=False))
+    sub = sub_table(table, idx)
+    otree = O.Tree(tree[0].astype(np.uint8), tree[1], tree[2])
+    errors = {}
+    for name, p in plans.items():
+        m = p.matrix()
+        ref = O.matrix(otree, O.Table(*sub), name)
+        errors[name] = max_rel_err(m[np.ix_(idx, idx)], ref)
+        del m
+    log(f"max rel err vs FP64 oracle: {errors}")
+
+    plain = None
+    if not args.no_plain:
+        pp = ssu.Plan(inp, "unweighted", unweighted_kernel=1)
+        pp.execute()
+        pp.execute()
+        pt = pp.timings()
+        plain = {"kernel": pp.main_kernel(), "main_ms": pt["main"], "device_total_ms": pt["total"],
+                 "gram_speedup_main": pt["main"] / kernels["unweighted"]["main_ms"],
+                 "gram_speedup_total": pt["total"] / kernels["unweighted"]["device_total_ms"]}
+        pp.close()
+        log(f"plain FP32 unweighted kernel: {pt['main']:.1f} ms main")
+    for p in plans.values():
+        p.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        kind, threads, t_full, per = cpu_baseline(tree, table, list(plans), args.cpu_budget)
+        cpu = {"value": sum(pair_nodes.values()) / t_full, "unit": "pair-node updates/s", "cores": threads,
+               "kind": kind,
+               "sample": f"stripes [0,1) and [0,3) of both matrices (FP32 reference kernels, OpenMP), "
+                         f"extrapolated linearly to all {(n + 1) // 2} stripes",
+               "per_method": per}
+
+    out = {
+        "metric": "pair-node updates/sec, 16k samples (unweighted, weighted); max rel err vs FP64",
+        "value": value,
+        "unit": "pair-node updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8 MFMA (unweighted, exact) + fp32/fp64 (weighted)",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: random binary tree {len(tree[0]) // 4 + 1} tips, "
+                               f"{n} samples, unweighted + weighted_normalized, full n x n matrices",
+                   "samples": n, "tree_nodes": len(tree[0]) // 2, "nnz": int(table[4].size),
+                   "methods": list(plans)},
+        "max_rel_err_vs_fp64": errors,
+        "kernels": kernels,
+        "plain_fp32_unweighted": plain,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,311 @@
+/*
+ * unifrac-binaries_amd -- C ABI of libssu.so, the MI355X (gfx950) Strided
+ * State UniFrac / Faith PD library.
+ *
+ * Drop-in contract: the type layouts, enum values and entry points below are
+ * those of biocore/unifrac-binaries src/api.hpp and src/status_enum.hpp, so a
+ * caller built against the reference header links against this library
+ * unchanged.  Each declaration names the reference declaration it replaces.
+ * Declarations after "Extensions" have no counterpart in the reference.
+ */
+#ifndef SSU_AMD_H
+#define SSU_AMD_H
+
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+#define SSU_EXTERN extern "C"
+#else
+#define SSU_EXTERN
+#endif
+
+/* src/api.hpp:22-23 */
+#define PARTIAL_MAGIC "SSU-PARTIAL-01"
+#define PARTIAL_MAGIC_V2 0x088ABA02
+
+/* src/status_enum.hpp:4-6 -- numeric values are part of the ABI */
+typedef enum compute_status {okay = 0, tree_missing, table_missing, table_empty, unknown_method,
+                             table_and_tree_do_not_overlap, output_error, invalid_method, grouping_missing} ComputeStatus;
+typedef enum io_status {read_okay = 0, write_okay, open_error, read_error, magic_incompatible, bad_header,
+                        unexpected_end, write_error} IOStatus;
+typedef enum merge_status {merge_okay = 0, incomplete_stripe_set, sample_id_consistency, square_mismatch,
+                           partials_mismatch, stripes_overlap} MergeStatus;
+
+/* src/api.hpp:40-46: condensed upper triangle, scipy squareform order */
+typedef struct mat {
+    unsigned int n_samples;
+    unsigned int cf_size;
+    bool is_upper_triangle;
+    double* condensed_form;
+    char** sample_ids;
+} mat_t;
+
+/* src/api.hpp:54-59 */
+typedef struct mat_full_fp64 {
+    uint32_t n_samples;
+    uint32_t flags;   /* 0, or (fd & 0x0fff) | 0x1000 when the matrix is an mmap'd file */
+    double* matrix;
+    char** sample_ids;
+} mat_full_fp64_t;
+
+/* src/api.hpp:67-72 */
+typedef struct mat_full_fp32 {
+    uint32_t n_samples;
+    uint32_t flags;
+    float* matrix;
+    char** sample_ids;
+} mat_full_fp32_t;
+
+/* src/api.hpp:82-86 */
+typedef struct results_vec {
+    unsigned int n_samples;
+    double* values;
+    char** sample_ids;
+} r_vec;
+
+/* src/api.hpp:99-107: stripes[s - stripe_start][k] = d(k, (k + s + 1) mod n_samples) */
+typedef struct partial_mat {
+    uint32_t n_samples;
+    char** sample_ids;
+    double** stripes;
+    uint32_t stripe_start;
+    uint32_t stripe_stop;
+    uint32_t stripe_total;
+    bool is_upper_triangle;
+} partial_mat_t;
+
+/* src/api.hpp:122-132 */
+typedef struct partial_dyn_mat {
+    uint32_t n_samples;
+    char** sample_ids;
+    uint64_t* offsets;
+    double** stripes;
+    uint32_t stripe_start;
+    uint32_t stripe_stop;
+    uint32_t stripe_total;
+    bool is_upper_triangle;
+    char* filename;
+} partial_dyn_mat_t;
+
+/* src/api.hpp:145-154: observation-major CSR */
+typedef struct support_biom {
+    char** obs_ids;
+    char** sample_ids;
+    uint32_t* indices;
+    uint32_t* indptr;
+    double* data;
+    int n_obs;
+    int n_samples;
+    int nnz;
+} support_biom_t;
+
+/* src/api.hpp:163-168: balanced parentheses, per-parenthesis lengths and names */
+typedef struct support_bptree {
+    bool* structure;
+    double* lengths;
+    char** names;
+    int n_parens;
+} support_bptree_t;
+
+/* src/api.hpp:173-175 */
+typedef struct opaque_bptree {
+    int dummy;
+} opaque_bptree_t;
+
+/* ------------------------------------------------------------------ trees */
+SSU_EXTERN void ssu_set_random_seed(unsigned int new_seed);                                          /* api.hpp:28 */
+SSU_EXTERN IOStatus read_bptree_opaque(const char* tree_filename, opaque_bptree_t** tree_data);      /* api.hpp:188 */
+SSU_EXTERN void load_bptree_opaque(const char* newick, opaque_bptree_t** tree_data);                /* api.hpp:191 */
+SSU_EXTERN void convert_bptree_opaque(const support_bptree_t* in_tree, opaque_bptree_t** tree_data); /* api.hpp:194 */
+SSU_EXTERN void destroy_bptree_opaque(opaque_bptree_t** tree_data);                                  /* api.hpp:203 */
+SSU_EXTERN int get_bptree_opaque_els(opaque_bptree_t* tree_data);                                    /* api.hpp:206 */
+
+/* ------------------------------------------------------------- destructors */
+SSU_EXTERN void destroy_mat(mat_t** result);                          /* api.hpp:196 */
+SSU_EXTERN void destroy_mat_full_fp64(mat_full_fp64_t** result);      /* api.hpp:197 */
+SSU_EXTERN void destroy_mat_full_fp32(mat_full_fp32_t** result);      /* api.hpp:198 */
+SSU_EXTERN void destroy_partial_mat(partial_mat_t** result);          /* api.hpp:199 */
+SSU_EXTERN void destroy_partial_dyn_mat(partial_dyn_mat_t** result);  /* api.hpp:200 */
+SSU_EXTERN void destroy_results_vec(r_vec** result);                  /* api.hpp:201 */
+
+/* ------------------------------------------------------ condensed, from files */
+SSU_EXTERN ComputeStatus one_off_v3(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                                    bool variance_adjust, double alpha, bool bypass_tips, bool normalize_sample_counts,
+                                    unsigned int n_substeps, mat_t** result);                        /* api.hpp:228 */
+SSU_EXTERN ComputeStatus one_off_wtree_v3(const char* biom_filename, const opaque_bptree_t* tree_data,
+                                          const char* unifrac_method, bool variance_adjust, double alpha,
+                                          bool bypass_tips, bool normalize_sample_counts, unsigned int n_substeps,
+                                          mat_t** result);                                           /* api.hpp:233 */
+SSU_EXTERN ComputeStatus one_off(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                                 bool variance_adjust, double alpha, bool bypass_tips, unsigned int n_substeps,
+                                 mat_t** result);                                                    /* api.hpp:238 */
+SSU_EXTERN ComputeStatus one_off_wtree(const char* biom_filename, const opaque_bptree_t* tree_data,
+                                       const char* unifrac_method, bool variance_adjust, double alpha,
+                                       bool bypass_tips, unsigned int n_substeps, mat_t** result);  /* api.hpp:242 */
+
+/* ----------------------------------------------------- full matrix, in memory */
+SSU_EXTERN ComputeStatus one_off_matrix_inmem_v3(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                                                 const char* unifrac_method, bool variance_adjust, double alpha,
+                                                 bool bypass_tips, bool normalize_sample_counts,
+                                                 unsigned int n_substeps, unsigned int subsample_depth,
+                                                 bool subsample_with_replacement, const char* mmap_dir,
+                                                 mat_full_fp64_t** result);                          /* api.hpp:268 */
+SSU_EXTERN ComputeStatus one_off_matrix_inmem_v2(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                                                 const char* unifrac_method, bool variance_adjust, double alpha,
+                                                 bool bypass_tips, unsigned int n_substeps,
+                                                 unsigned int subsample_depth, bool subsample_with_replacement,
+                                                 const char* mmap_dir, mat_full_fp64_t** result);    /* api.hpp:275 */
+SSU_EXTERN ComputeStatus one_off_inmem(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                                       const char* unifrac_method, bool variance_adjust, double alpha,
+                                       bool bypass_tips, unsigned int n_substeps, mat_full_fp64_t** result); /* api.hpp:282 */
+SSU_EXTERN ComputeStatus one_off_matrix_inmem_fp32_v3(const support_biom_t* table_data,
+                                                      const support_bptree_t* tree_data, const char* unifrac_method,
+                                                      bool variance_adjust, double alpha, bool bypass_tips,
+                                                      bool normalize_sample_counts, unsigned int n_substeps,
+                                                      unsigned int subsample_depth, bool subsample_with_replacement,
+                                                      const char* mmap_dir, mat_full_fp32_t** result); /* api.hpp:307 */
+SSU_EXTERN ComputeStatus one_off_matrix_inmem_fp32_v2(const support_biom_t* table_data,
+                                                      const support_bptree_t* tree_data, const char* unifrac_method,
+                                                      bool variance_adjust, double alpha, bool bypass_tips,
+                                                      unsigned int n_substeps, unsigned int subsample_depth,
+                                                      bool subsample_with_replacement, const char* mmap_dir,
+                                                      mat_full_fp32_t** result);                     /* api.hpp:314 */
+SSU_EXTERN ComputeStatus one_off_inmem_fp32(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                                            const char* unifrac_method, bool variance_adjust, double alpha,
+                                            bool bypass_tips, unsigned int n_substeps,
+                                            mat_full_fp32_t** result);                               /* api.hpp:321 */
+
+/* -------------------------------------------------------- full matrix, files */
+SSU_EXTERN ComputeStatus one_off_matrix_v3(const char* biom_filename, const char* tree_filename,
+                                           const char* unifrac_method, bool variance_adjust, double alpha,
+                                           bool bypass_tips, bool normalize_sample_counts, unsigned int n_substeps,
+                                           unsigned int subsample_depth, bool subsample_with_replacement,
+                                           const char* mmap_dir, mat_full_fp64_t** result);          /* api.hpp:348 */
+SSU_EXTERN ComputeStatus one_off_matrix_v3t(const char* biom_filename, const opaque_bptree_t* tree_data,
+                                            const char* unifrac_method, bool variance_adjust, double alpha,
+                                            bool bypass_tips, bool normalize_sample_counts, unsigned int n_substeps,
+                                            unsigned int subsample_depth, bool subsample_with_replacement,
+                                            const char* mmap_dir, mat_full_fp64_t** result);         /* api.hpp:355 */
+SSU_EXTERN ComputeStatus one_off_matrix_v2(const char* biom_filename, const char* tree_filename,
+                                           const char* unifrac_method, bool variance_adjust, double alpha,
+                                           bool bypass_tips, unsigned int n_substeps, unsigned int subsample_depth,
+                                           bool subsample_with_replacement, const char* mmap_dir,
+                                           mat_full_fp64_t** result);                                /* api.hpp:362 */
+SSU_EXTERN ComputeStatus one_off_matrix_v2t(const char* biom_filename, const opaque_bptree_t* tree_data,
+                                            const char* unifrac_method, bool variance_adjust, double alpha,
+                                            bool bypass_tips, unsigned int n_substeps, unsigned int subsample_depth,
+                                            bool subsample_with_replacement, const char* mmap_dir,
+                                            mat_full_fp64_t** result);                               /* api.hpp:369 */
+SSU_EXTERN ComputeStatus one_off_matrix(const char* biom_filename, const char* tree_filename,
+                                        const char* unifrac_method, bool variance_adjust, double alpha,
+                                        bool bypass_tips, unsigned int n_substeps, const char* mmap_dir,
+                                        mat_full_fp64_t** result);                                   /* api.hpp:375 */
+SSU_EXTERN ComputeStatus one_off_matrix_fp32_v3(const char* biom_filename, const char* tree_filename,
+                                                const char* unifrac_method, bool variance_adjust, double alpha,
+                                                bool bypass_tips, bool normalize_sample_counts,
+                                                unsigned int n_substeps, unsigned int subsample_depth,
+                                                bool subsample_with_replacement, const char* mmap_dir,
+                                                mat_full_fp32_t** result);                           /* api.hpp:404 */
+SSU_EXTERN ComputeStatus one_off_matrix_fp32_v3t(const char* biom_filename, const opaque_bptree_t* tree_data,
+                                                 const char* unifrac_method, bool variance_adjust, double alpha,
+                                                 bool bypass_tips, bool normalize_sample_counts,
+                                                 unsigned int n_substeps, unsigned int subsample_depth,
+                                                 bool subsample_with_replacement, const char* mmap_dir,
+                                                 mat_full_fp32_t** result);                          /* api.hpp:411 */
+SSU_EXTERN ComputeStatus one_off_matrix_fp32_v2(const char* biom_filename, const char* tree_filename,
+                                                const char* unifrac_method, bool variance_adjust, double alpha,
+                                                bool bypass_tips, unsigned int n_substeps,
+                                                unsigned int subsample_depth, bool subsample_with_replacement,
+                                                const char* mmap_dir, mat_full_fp32_t** result);     /* api.hpp:418 */
+SSU_EXTERN ComputeStatus one_off_matrix_fp32_v2t(const char* biom_filename, const opaque_bptree_t* tree_data,
+                                                 const char* unifrac_method, bool variance_adjust, double alpha,
+                                                 bool bypass_tips, unsigned int n_substeps,
+                                                 unsigned int subsample_depth, bool subsample_with_replacement,
+                                                 const char* mmap_dir, mat_full_fp32_t** result);    /* api.hpp:425 */
+SSU_EXTERN ComputeStatus one_off_matrix_fp32(const char* biom_filename, const char* tree_filename,
+                                             const char* unifrac_method, bool variance_adjust, double alpha,
+                                             bool bypass_tips, unsigned int n_substeps, const char* mmap_dir,
+                                             mat_full_fp32_t** result);                              /* api.hpp:432 */
+
+/* --------------------------------------------------------------- dense pair */
+SSU_EXTERN ComputeStatus one_dense_pair_v3t(unsigned int n_obs, const char** obs_ids, const double* sample1,
+                                            const double* sample2, const opaque_bptree_t* tree_data,
+                                            const char* unifrac_method, bool variance_adjust, double alpha,
+                                            bool bypass_tips, bool normalize_sample_counts, double* result); /* api.hpp:456 */
+SSU_EXTERN ComputeStatus one_dense_pair_v3(unsigned int n_obs, const char** obs_ids, const double* sample1,
+                                           const double* sample2, const support_bptree_t* tree_data,
+                                           const char* unifrac_method, bool variance_adjust, double alpha,
+                                           bool bypass_tips, bool normalize_sample_counts, double* result);  /* api.hpp:462 */
+SSU_EXTERN ComputeStatus one_dense_pair_v2t(unsigned int n_obs, const char** obs_ids, const double* sample1,
+                                            const double* sample2, const opaque_bptree_t* tree_data,
+                                            const char* unifrac_method, bool variance_adjust, double alpha,
+                                            bool bypass_tips, double* result);                       /* api.hpp:469 */
+SSU_EXTERN ComputeStatus one_dense_pair_v2(unsigned int n_obs, const char** obs_ids, const double* sample1,
+                                           const double* sample2, const support_bptree_t* tree_data,
+                                           const char* unifrac_method, bool variance_adjust, double alpha,
+                                           bool bypass_tips, double* result);                        /* api.hpp:475 */
+
+/* ----------------------------------------------------------------- Faith PD */
+SSU_EXTERN ComputeStatus faith_pd_one_off(const char* biom_filename, const char* tree_filename,
+                                          r_vec** result);                                           /* api.hpp:492 */
+
+/* ------------------------------------------------------------------ partial */
+SSU_EXTERN ComputeStatus partial_v3(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                                    bool variance_adjust, double alpha, bool bypass_tips, bool normalize_sample_count,
+                                    unsigned int n_substeps, unsigned int stripe_start, unsigned int stripe_stop,
+                                    partial_mat_t** result);                                         /* api.hpp:805 */
+SSU_EXTERN ComputeStatus partial(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                                 bool variance_adjust, double alpha, bool bypass_tips, unsigned int n_substeps,
+                                 unsigned int stripe_start, unsigned int stripe_stop,
+                                 partial_mat_t** result);                                            /* api.hpp:811 */
+
+/* ------------------------------------------------------------ text writers */
+SSU_EXTERN IOStatus write_mat(const char* filename, mat_t* result);                              /* api.hpp:649 */
+SSU_EXTERN IOStatus write_mat_from_matrix(const char* filename, mat_full_fp64_t* result);        /* api.hpp:688 */
+SSU_EXTERN IOStatus write_mat_from_matrix_fp32(const char* filename, mat_full_fp32_t* result);   /* api.hpp:691 */
+SSU_EXTERN IOStatus write_vec(const char* filename, r_vec* result);                              /* api.hpp:763 */
+
+/* ==================================================================
+ * Extensions (no counterpart in the reference)
+ * ================================================================== */
+
+/* Faith PD on in-memory inputs, same semantics as faith_pd_one_off
+ * (samples with total < 1 are dropped first). */
+SSU_EXTERN ComputeStatus ssu_faith_pd_inmem(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                                            r_vec** result);
+
+/* Device-resident plan: inputs uploaded once to HBM, then executed any
+ * number of times; the n x n result stays in device memory until copied.
+ * For benchmarking and for callers that chain GPU work on the matrix. */
+typedef struct ssu_plan ssu_plan_t;
+SSU_EXTERN ComputeStatus ssu_plan_create_inmem(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                                               const char* unifrac_method, bool variance_adjust, double alpha,
+                                               bool bypass_tips, bool normalize_sample_counts,
+                                               int unweighted_kernel, ssu_plan_t** plan);
+SSU_EXTERN ComputeStatus ssu_plan_execute(ssu_plan_t* plan);
+/* element size of the result: 8 (fp64 method names) or 4 */
+SSU_EXTERN int ssu_plan_result_bytes(const ssu_plan_t* plan);
+SSU_EXTERN ComputeStatus ssu_plan_copy_matrix(const ssu_plan_t* plan, void* host_out);
+/* last execute(): milliseconds per phase (embed, prep, main kernel, combine, total) from HIP events */
+SSU_EXTERN int ssu_plan_timings(const ssu_plan_t* plan, double* ms, int cap);
+SSU_EXTERN uint64_t ssu_plan_pair_nodes(const ssu_plan_t* plan);   /* n(n-1)/2 * processed nodes */
+SSU_EXTERN const char* ssu_plan_main_kernel(const ssu_plan_t* plan);
+SSU_EXTERN void ssu_plan_destroy(ssu_plan_t** plan);
+
+/* Host-side tree handling only (no device): sheared + collapsed postorder
+ * arrays of tree_data restricted to obs_ids.  Arrays hold n_parens/2 entries.
+ * Returns the number of non-root nodes, or -1. */
+SSU_EXTERN int ssu_postorder(const support_bptree_t* tree_data, int n_obs, const char* const* obs_ids,
+                             int32_t* parent, double* length, int32_t* is_leaf, int32_t* obs_row);
+/* Newick parse only: fills structure / lengths (n_parens entries) and copies
+ * names into names_buf[i] (name_cap bytes each).  Returns n_parens or -n_parens if cap is too small. */
+SSU_EXTERN int ssu_newick_to_bp(const char* newick, int cap, uint8_t* structure, double* lengths, char** names_buf,
+                                int name_cap);
+/* 1 if a HIP device is visible */
+SSU_EXTERN int ssu_device_available(void);
+SSU_EXTERN const char* ssu_last_error(void);
+
+#endif /* SSU_AMD_H */

@@ -1,0 +1,269 @@
+"""Parity of the HIP path (through the C ABI of libssu.so) with the oracle,
+the reference's golden vectors and the reference-written vectors.
+
+Tolerances (stated per test):
+  * FP64 method names: relative 1e-11 against the FP64 oracle
+    (the unweighted Gram is exact up to one FP64 rounding per K-chunk)
+  * FP32 method names: relative 1e-6 per matrix entry against the FP64 oracle
+  * Faith PD and unit-length unweighted counts: bit-exact
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_err, stripes_of
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_device(ssu):
+    if not ssu.device_available():
+        pytest.fail("no HIP device visible: the HIP path cannot run (no CPU fallback exists)")
+
+
+def inputs_from(ssu, tree, table):
+    st, le, nm = tree
+    return ssu.Inputs(st, le, nm, *table)
+
+
+def oracle_matrix(O, tree, table, method, **kw):
+    st, le, nm = tree
+    return O.matrix(O.Tree(np.asarray(st, np.uint8), le, nm), O.Table(*table), method, **kw)
+
+
+@pytest.fixture(scope="module")
+def test_fixture(ssu, ref_tables, ref_trees):
+    t = ref_tables["test"]
+    st, le, nm = ssu.newick_to_bp(ref_trees["test"])
+    return ssu.Inputs(st, le, nm, t["obs_ids"], t["sample_ids"], t["indptr"], t["indices"], t["data"])
+
+
+GOLDEN_KEYS = ["unweighted", "unweighted_bypass", "unweighted_unnormalized", "weighted_unnormalized",
+               "weighted_normalized", "generalized_a1", "generalized_a0", "generalized_a05",
+               "vaw_weighted_normalized"]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("key", GOLDEN_KEYS)
+def test_goldens_through_c_api(ssu, goldens, test_fixture, key, prec):
+    g = goldens[key]
+    method = g["method"] + ("_fp64" if prec == "fp64" else "")
+    say(key, prec)
+    st, m = ssu.matrix_inmem(test_fixture, method, variance_adjust=g["vaw"], alpha=g["alpha"],
+                             bypass_tips=g["bypass_tips"])
+    assert st == ssu.OKAY, ssu.last_error()
+    # src/test_helper.hpp:81-93: absolute 1e-6
+    assert np.max(np.abs(stripes_of(m) - np.array(g["stripes"]))) < 1e-6, g["cite"]
+
+
+def test_faith_goldens_through_c_api(ssu, goldens, ref_tables, ref_trees):
+    t = ref_tables["test"]
+    for tree_key, gold in [("test", "faith_test_tre"), ("test_faith", "faith_test_faith_tre"),
+                           ("test_faith_shear", "faith_test_faith_shear_tre")]:
+        say("faith", tree_key)
+        st, le, nm = ssu.newick_to_bp(ref_trees[tree_key])
+        inp = ssu.Inputs(st, le, nm, t["obs_ids"], t["sample_ids"], t["indptr"], t["indices"], t["data"])
+        s, vals, ids = ssu.faith_pd_inmem(inp)
+        assert s == ssu.OKAY
+        assert np.array_equal(vals, np.array(goldens[gold]["values"], float))
+
+
+def test_crawford_through_c_api(ssu, O, ref_tables, ref_trees, ref_expected):
+    t = ref_tables["crawford"]
+    st, le, nm = ssu.newick_to_bp(ref_trees["crawford"])
+    inp = ssu.Inputs(st, le, nm, t["obs_ids"], t["sample_ids"], t["indptr"], t["indices"], t["data"])
+    for key, vaw in [("crawford_unweighted_fp64", False), ("crawford_unweighted_fp64_vaw", True)]:
+        say(key)
+        s, m = ssu.matrix_inmem(inp, "unweighted_fp64", variance_adjust=vaw)
+        assert s == ssu.OKAY
+        exp = ref_expected[key]
+        idx = [t["sample_ids"].index(x) for x in exp["ids"]]
+        assert rel_err(m[np.ix_(idx, idx)], np.array(exp["matrix"])) < 1e-12
+    say("crawford faith")
+    s, vals, ids = ssu.faith_pd_inmem(inp)
+    exp = dict(ref_expected["crawford_faith"])
+    for sid, v in zip(ids, vals):
+        assert "%.16g" % v == exp[sid]   # ci/crawford_test.sh:40-44 (md5 of all digits)
+
+
+def _case(synth, tips, n, dens, seed, zipf=None, counts="lognormal"):
+    tree = synth.random_tree(tips, seed)
+    table = synth.random_table(tips, n, dens, seed + 1, zipf, counts)
+    return tree, table
+
+
+PARITY = [
+    # method, kwargs
+    ("unweighted", {}),
+    ("unweighted_unnormalized", {}),
+    ("unweighted", {"bypass_tips": True}),
+    ("weighted_normalized", {}),
+    ("weighted_normalized", {"normalize": False}),
+    ("weighted_unnormalized", {}),
+    ("weighted_normalized", {"bypass_tips": True}),
+    ("generalized", {"alpha": 0.5}),
+    ("generalized", {"alpha": 0.0}),
+    ("generalized", {"alpha": 1.0}),
+    ("unweighted", {"variance_adjust": True}),
+    ("unweighted_unnormalized", {"variance_adjust": True}),
+    ("weighted_normalized", {"variance_adjust": True}),
+    ("weighted_unnormalized", {"variance_adjust": True}),
+    ("generalized", {"alpha": 0.5, "variance_adjust": True}),
+]
+TOL = {"fp64": 1e-11, "fp32": 1e-6}
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("method,kw", PARITY)
+def test_parity_vs_oracle(ssu, O, synth, method, kw, prec):
+    tree, table = _case(synth, 700, 201, 0.04, 101, zipf=1.1)
+    say("parity", method, kw, prec)
+    s, m = ssu.matrix_inmem(inputs_from(ssu, tree, table), method + ("_fp64" if prec == "fp64" else ""), **kw)
+    assert s == ssu.OKAY, ssu.last_error()
+    ref = oracle_matrix(O, tree, table, method, **kw)
+    err = rel_err(m, ref)
+    say("max rel err", err)
+    assert err <= TOL[prec]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 127, 128, 129, 300])
+def test_sample_count_edges(ssu, O, synth, n):
+    tree, table = _case(synth, 150, n, 0.1, 500 + n)
+    for method in ("unweighted_fp64", "weighted_normalized_fp64", "weighted_unnormalized"):
+        say("edge n", n, method)
+        s, m = ssu.matrix_inmem(inputs_from(ssu, tree, table), method)
+        assert s == ssu.OKAY, ssu.last_error()
+        base = method.replace("_fp64", "")
+        ref = oracle_matrix(O, tree, table, base)
+        assert rel_err(m, ref) <= TOL["fp64" if method.endswith("fp64") else "fp32"]
+
+
+def test_unit_lengths_bit_exact(ssu, O, synth):
+    """All lengths 1: unweighted counts are integers, the Gram is exact, ratios correctly rounded."""
+    st, le, nm = synth.random_tree(900, 77)
+    ones = np.where(st, 1.0, 0.0)
+    table = synth.random_table(900, 260, 0.03, 78)
+    inp = ssu.Inputs(st, ones, nm, *table)
+    for method in ("unweighted_unnormalized_fp64", "unweighted_fp64"):
+        say("unit", method)
+        s, m = ssu.matrix_inmem(inp, method)
+        ref = oracle_matrix(O, (st, ones, nm), table, method.replace("_fp64", ""))
+        assert s == ssu.OKAY and np.array_equal(m, ref)
+    s, m32 = ssu.matrix_inmem(inp, "unweighted_unnormalized")
+    ref_u = oracle_matrix(O, (st, ones, nm), table, "unweighted_unnormalized").astype(np.float32)
+    assert s == ssu.OKAY and np.array_equal(m32, ref_u)
+
+
+def test_faith_bit_exact_vs_oracle(ssu, O, synth):
+    tree, table = _case(synth, 3000, 400, 0.02, 901, zipf=1.1)
+    say("faith random")
+    s, vals, ids = ssu.faith_pd_inmem(inputs_from(ssu, tree, table))
+    st, le, nm = tree
+    ref = O.faith_pd(O.Tree(st.astype(np.uint8), le, nm), O.Table(*table))
+    assert s == ssu.OKAY and np.array_equal(vals, ref)
+
+
+def test_empty_sample_and_zero_lengths(ssu, O, synth):
+    st, le, nm = synth.random_tree(60, 3)
+    obs, sids, ip, ix, d = synth.random_table(60, 10, 0.2, 4)
+    d = d.copy()
+    d[ix == 3] = 0.0  # sample 3 keeps explicit zeros only
+    table = (obs, sids, ip, ix, d)
+    for method in ("unweighted_fp64", "weighted_normalized_fp64", "weighted_unnormalized_fp64"):
+        say("empty sample", method)
+        s, m = ssu.matrix_inmem(ssu.Inputs(st, le, nm, *table), method)
+        ref = oracle_matrix(O, (st, le, nm), table, method.replace("_fp64", ""))
+        assert s == ssu.OKAY and rel_err(m, ref) <= 1e-11
+    zero = np.zeros_like(le)
+    s, m = ssu.matrix_inmem(ssu.Inputs(st, zero, nm, *table), "unweighted_fp64")
+    off = ~np.eye(10, dtype=bool)
+    assert s == ssu.OKAY and np.all(np.isnan(m[off]))   # 0/0, as the reference computes it
+
+
+def test_reference_vectors_on_gpu(ssu):
+    """tests/golden/ref_vectors.npz holds outputs of the reference implementation."""
+    z = np.load(os.path.join(GOLDEN, "ref_vectors.npz"), allow_pickle=False)
+    cases = sorted(k[:-len("__out")] for k in z.files if k.endswith("__out"))
+    for c in cases:
+        prec = c.split("__")[-1]
+        meta = z[c + "__meta"]
+        method = str(z[c + "__method"])
+        inp = ssu.Inputs(z[c + "__structure"].astype(bool), z[c + "__lengths"], list(z[c + "__names"]),
+                         list(z[c + "__obs_ids"]), list(z[c + "__sample_ids"]), z[c + "__indptr"],
+                         z[c + "__indices"], z[c + "__data"])
+        say("ref vectors", c)
+        s, m = ssu.matrix_inmem(inp, method + "_fp64", variance_adjust=bool(meta[0]), alpha=float(meta[1]),
+                                bypass_tips=bool(meta[2]), normalize=bool(meta[3]))
+        assert s == ssu.OKAY
+        got = stripes_of(m, z[c + "__out"].shape[0])
+        if prec == "fp64":
+            assert rel_err(got, z[c + "__out"]) < 1e-11, c
+        else:
+            # the reference's own FP32 path vs our FP64: bounded by the reference's FP32 error
+            assert rel_err(got, z[c + "__out"]) < 1e-4, c
+
+
+def test_plain_fp32_unweighted_kernel_matches_gram(ssu, O, synth):
+    tree, table = _case(synth, 500, 300, 0.05, 61, zipf=1.1)
+    inp = inputs_from(ssu, tree, table)
+    say("plain vs gram")
+    p_gram = ssu.Plan(inp, "unweighted", unweighted_kernel=0)
+    p_plain = ssu.Plan(inp, "unweighted", unweighted_kernel=1)
+    p_gram.execute()
+    p_plain.execute()
+    assert p_gram.main_kernel() == "k_gram_i8" and p_plain.main_kernel() == "k_pairs"
+    ref = oracle_matrix(O, tree, table, "unweighted")
+    assert rel_err(p_gram.matrix(), ref) <= 1e-6
+    assert rel_err(p_plain.matrix(), ref) <= 1e-5
+    p_gram.close()
+    p_plain.close()
+
+
+def test_file_entry_points(ssu, O, ref_trees, ref_tables, goldens, tmp_path):
+    L = ssu.lib()
+    tre = tmp_path / "test.tre"
+    tre.write_text(ref_trees["test"])
+    biom = os.path.join(GOLDEN, "test.biom").encode()
+    say("one_off file")
+    res = C.POINTER(ssu.Mat)()
+    st = L.one_off(biom, str(tre).encode(), b"unweighted", False, C.c_double(1.0), False, 1, C.byref(res))
+    assert st == ssu.OKAY, ssu.last_error()
+    cf = np.ctypeslib.as_array(res.contents.condensed_form, shape=(res.contents.cf_size,)).copy()
+    L.destroy_mat(C.byref(res))
+    assert np.max(np.abs(cf - np.array(goldens["condensed_unweighted"]["values"]))) < 1e-5
+    say("faith file")
+    rv = C.POINTER(ssu.RVec)()
+    assert L.faith_pd_one_off(biom, str(tre).encode(), C.byref(rv)) == ssu.OKAY
+    vals = np.ctypeslib.as_array(rv.contents.values, shape=(rv.contents.n_samples,)).copy()
+    L.destroy_results_vec(C.byref(rv))
+    assert np.array_equal(vals, np.array(goldens["faith_test_tre"]["values"], float))
+    say("partial file")
+    pm = C.POINTER(ssu.PartialMat)()
+    assert L.partial_v3(biom, str(tre).encode(), b"unweighted_fp64", False, C.c_double(1.0), False, True, 1, 1, 3,
+                        C.byref(pm)) == ssu.OKAY
+    g = np.array(goldens["unweighted"]["stripes"])
+    for s in range(1, 3):
+        row = np.ctypeslib.as_array(pm.contents.stripes[s - 1], shape=(6,))
+        assert np.max(np.abs(row - g[s])) < 1e-6
+    L.destroy_partial_mat(C.byref(pm))
+    say("dense pair")
+    d = goldens["dense_pair_unweighted"]
+    op = C.c_void_p()
+    L.load_bptree_opaque(ref_trees["test"].encode(), C.byref(op))
+    ids = (C.c_char_p * 5)(*[x.encode() for x in d["obs_ids"]])
+    s1 = (C.c_double * 5)(*d["sample1"])
+    s3 = (C.c_double * 5)(*d["sample3"])
+    out = C.c_double()
+    assert L.one_dense_pair_v2t(5, ids, s1, s3, op, b"unweighted", False, C.c_double(1.0), False,
+                                C.byref(out)) == ssu.OKAY
+    assert abs(out.value - d["exp13"]) < 1e-5
+    L.destroy_bptree_opaque(C.byref(op))

@@ -1,0 +1,943 @@
+// C ABI of libssu.so (include/ssu.h): the reference's src/api.hpp entry points
+// over the gfx950 device pipeline.
+#include "ssu.h"
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <unistd.h>
+
+#include <charconv>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ssu_biom.hpp"
+#include "ssu_engine.hpp"
+#include "ssu_table.hpp"
+#include "ssu_tree.hpp"
+
+using ssu::ComputeSpec;
+using ssu::Method;
+using ssu::PhyloTree;
+using ssu::Plan;
+using ssu::Table;
+
+namespace {
+
+thread_local std::string g_last_error;
+std::mt19937_64 g_rng(5489u);
+
+constexpr uint32_t MMAP_FD_MASK = 0x0fff;  // src/api.cpp:28-29
+constexpr uint32_t MMAP_FLAG = 0x1000;
+
+void set_error(const std::string& e) {
+    g_last_error = e;
+    if (!e.empty()) fprintf(stderr, "ssu: %s\n", e.c_str());
+}
+
+// src/api.cpp:55-88 (SET_METHOD): bare names run in FP32, "_fp64" in FP64
+bool parse_method(const char* name, Method& m, bool& fp64) {
+    if (!name) return false;
+    std::string s(name);
+    fp64 = false;
+    auto strip = [&](const char* suffix) {
+        const size_t l = strlen(suffix);
+        if (s.size() > l && s.compare(s.size() - l, l, suffix) == 0) { s.resize(s.size() - l); return true; }
+        return false;
+    };
+    if (strip("_fp64")) fp64 = true;
+    else strip("_fp32");
+    if (s == "unweighted") m = Method::unweighted;
+    else if (s == "weighted_normalized") m = Method::weighted_normalized;
+    else if (s == "weighted_unnormalized") m = Method::weighted_unnormalized;
+    else if (s == "unweighted_unnormalized") m = Method::unweighted_unnormalized;
+    else if (s == "generalized") m = Method::generalized;
+    else return false;
+    return true;
+}
+
+char** dup_ids(const std::vector<std::string>& ids) {
+    char** out = (char**)malloc(sizeof(char*) * (ids.empty() ? 1 : ids.size()));
+    for (size_t i = 0; i < ids.size(); ++i) out[i] = strdup(ids[i].c_str());
+    return out;
+}
+
+// src/api.cpp:94-100 (VALIDATE_TREE_TABLE)
+ComputeStatus validate(const Table& t, const PhyloTree& tree) {
+    if (t.n_samples == 0 || t.n_obs == 0) return table_empty;
+    if (!tree.valid()) return tree_missing;
+    const auto tips = tree.tip_names();
+    for (const auto& id : t.obs_ids)
+        if (!tips.count(id)) return table_and_tree_do_not_overlap;
+    return okay;
+}
+
+// src/api.cpp:90-92 (SYNC_TREE_TABLE) followed by the postorder walk
+ssu::PostorderNodes synced_nodes(const Table& t, const PhyloTree& tree) {
+    std::unordered_set<std::string> keep(t.obs_ids.begin(), t.obs_ids.end());
+    PhyloTree s = tree.shear(keep).collapse();
+    return ssu::postorder_nodes(s, t.obs_index);
+}
+
+ComputeSpec make_spec(Method m, bool fp64, bool vaw, double alpha, bool bypass, bool normalize) {
+    ComputeSpec s;
+    s.method = m;
+    s.fp64 = fp64;
+    s.vaw = vaw;
+    s.alpha = alpha;
+    s.bypass_tips = bypass;
+    s.normalize = normalize;
+    return s;
+}
+
+// Full n x n matrix of the given element type in host memory.
+template <class TOut>
+ComputeStatus run_full(const Table& t, const PhyloTree& tree, const ComputeSpec& spec, TOut* out) {
+    std::string err;
+    ssu::PostorderNodes nodes = synced_nodes(t, tree);
+    std::unique_ptr<Plan> plan = Plan::create(t, nodes, spec, &err);
+    if (!plan || !plan->execute(&err)) { set_error(err); return output_error; }
+    const size_t nn = (size_t)t.n_samples * t.n_samples;
+    if (plan->out_fp64() == (sizeof(TOut) == sizeof(double))) {
+        if (!plan->copy_matrix(out, &err)) { set_error(err); return output_error; }
+    } else if (plan->out_fp64()) {
+        std::vector<double> tmp(nn);
+        if (!plan->copy_matrix(tmp.data(), &err)) { set_error(err); return output_error; }
+        for (size_t i = 0; i < nn; ++i) out[i] = (TOut)tmp[i];
+    } else {
+        std::vector<float> tmp(nn);
+        if (!plan->copy_matrix(tmp.data(), &err)) { set_error(err); return output_error; }
+        for (size_t i = 0; i < nn; ++i) out[i] = (TOut)tmp[i];
+    }
+    return okay;
+}
+
+// src/api.cpp:270-327 (initialize_mat_full_no_biom_T): malloc, or an unlinked mmap'd file under mmap_dir
+template <class TReal, class TMat>
+bool init_full(TMat*& r, const std::vector<std::string>& ids, const char* mmap_dir) {
+    r = (TMat*)malloc(sizeof(TMat));
+    r->n_samples = (uint32_t)ids.size();
+    r->flags = 0;
+    r->sample_ids = dup_ids(ids);
+    const uint64_t n = r->n_samples;
+    const uint64_t msize = sizeof(TReal) * n * n;
+    if (mmap_dir && mmap_dir[0] == 0) mmap_dir = nullptr;
+    if (!mmap_dir) {
+        r->matrix = (TReal*)malloc(msize ? msize : sizeof(TReal));
+    } else {
+        std::string tmpl = std::string(mmap_dir) + "/su_mmap_XXXXXX";
+        int fd = mkostemp(&tmpl[0], O_NOATIME);
+        r->matrix = nullptr;
+        if (fd >= 0) {
+            unlink(tmpl.c_str());
+            if (ftruncate(fd, (off_t)msize) == 0) {
+                void* p = mmap(nullptr, msize ? msize : 1, PROT_READ | PROT_WRITE, MAP_SHARED | MAP_NORESERVE, fd, 0);
+                if (p != MAP_FAILED) {
+                    r->matrix = (TReal*)p;
+                    r->flags = ((uint32_t)fd & MMAP_FD_MASK) | MMAP_FLAG;
+                } else {
+                    close(fd);
+                }
+            } else {
+                close(fd);
+            }
+        }
+    }
+    return r->matrix != nullptr;
+}
+
+template <class TReal, class TMat>
+void destroy_full(TMat** result) {
+    if (!result || !*result) return;
+    TMat* r = *result;
+    for (uint32_t i = 0; i < r->n_samples; ++i) free(r->sample_ids[i]);
+    free(r->sample_ids);
+    if (r->matrix) {
+        if ((r->flags & MMAP_FLAG) == 0) {
+            free(r->matrix);
+        } else {
+            const uint64_t n = r->n_samples;
+            munmap(r->matrix, (sizeof(TReal) * n * n) > 0 ? sizeof(TReal) * n * n : 1);
+            close((int)(r->flags & MMAP_FD_MASK));
+        }
+    }
+    free(r);
+    *result = nullptr;
+}
+
+// Rarefaction: samples below the depth are dropped, the rest drawn down to
+// exactly `depth` counts; observations left empty are dropped
+// (src/biom_subsampled.cpp:288-321).  The random stream is this library's
+// own (std::mt19937_64), so only the statistics match the reference.
+Table subsample(const Table& t, uint32_t depth, bool with_replacement) {
+    std::vector<int64_t> keep(t.n_samples, -1);
+    std::vector<std::string> sids;
+    for (uint32_t s = 0; s < t.n_samples; ++s)
+        if (t.totals[s] >= depth) { keep[s] = (int64_t)sids.size(); sids.push_back(t.sample_ids[s]); }
+    // per kept sample: list of (obs, count)
+    std::vector<std::vector<std::pair<uint32_t, double>>> cols(sids.size());
+    for (uint32_t i = 0; i < t.n_obs; ++i)
+        for (uint32_t j = t.indptr[i]; j < t.indptr[i + 1]; ++j)
+            if (keep[t.indices[j]] >= 0 && t.data[j] > 0) cols[keep[t.indices[j]]].push_back({i, t.data[j]});
+    std::vector<std::vector<std::pair<uint32_t, double>>> drawn(sids.size());
+    for (size_t s = 0; s < cols.size(); ++s) {
+        auto& c = cols[s];
+        std::vector<double> out(c.size(), 0.0);
+        if (with_replacement) {
+            std::vector<double> w(c.size());
+            for (size_t q = 0; q < c.size(); ++q) w[q] = c[q].second;
+            std::discrete_distribution<size_t> dist(w.begin(), w.end());
+            for (uint32_t d = 0; d < depth; ++d) out[dist(g_rng)] += 1.0;
+        } else {
+            std::vector<uint64_t> remaining(c.size());
+            uint64_t total = 0;
+            for (size_t q = 0; q < c.size(); ++q) { remaining[q] = (uint64_t)std::floor(c[q].second); total += remaining[q]; }
+            for (uint32_t d = 0; d < depth && total > 0; ++d) {
+                uint64_t r = std::uniform_int_distribution<uint64_t>(0, total - 1)(g_rng);
+                size_t q = 0;
+                while (r >= remaining[q]) { r -= remaining[q]; ++q; }
+                remaining[q]--; total--; out[q] += 1.0;
+            }
+        }
+        for (size_t q = 0; q < c.size(); ++q)
+            if (out[q] > 0) drawn[s].push_back({c[q].first, out[q]});
+    }
+    std::vector<std::vector<std::pair<uint32_t, double>>> rows(t.n_obs);
+    for (size_t s = 0; s < drawn.size(); ++s)
+        for (auto& e : drawn[s]) rows[e.first].push_back({(uint32_t)s, e.second});
+    std::vector<std::string> oids;
+    std::vector<uint32_t> ip{0}, ix;
+    std::vector<double> d;
+    for (uint32_t i = 0; i < t.n_obs; ++i) {
+        if (rows[i].empty()) continue;
+        oids.push_back(t.obs_ids[i]);
+        for (auto& e : rows[i]) { ix.push_back(e.first); d.push_back(e.second); }
+        ip.push_back((uint32_t)ix.size());
+    }
+    return Table::own(std::move(oids), std::move(sids), std::move(ip), std::move(ix), std::move(d));
+}
+
+// src/api.cpp:638-698 (one_off_matrix_T) + :700-714 (optional subsampling)
+template <class TReal, class TMat>
+ComputeStatus matrix_common(const Table& t0, const PhyloTree& tree, const char* method_name, bool vaw, double alpha,
+                            bool bypass, bool normalize, unsigned subsample_depth, bool with_replacement,
+                            const char* mmap_dir, TMat** result) {
+    Method m;
+    bool fp64;
+    if (!parse_method(method_name, m, fp64)) return unknown_method;
+    const Table* tp = &t0;
+    Table sub;
+    if (subsample_depth > 0) {
+        sub = subsample(t0, subsample_depth, with_replacement);
+        if (sub.n_samples == 0 || sub.n_obs == 0) return table_empty;
+        tp = &sub;
+    }
+    const Table& t = *tp;
+    if (*result == nullptr) {
+        if (!init_full<TReal, TMat>(*result, t.sample_ids, mmap_dir)) {
+            set_error("result allocation failed");
+            return output_error;
+        }
+    }
+    return run_full<TReal>(t, tree, make_spec(m, fp64, vaw, alpha, bypass, normalize), (*result)->matrix);
+}
+
+template <class TReal, class TMat>
+ComputeStatus matrix_inmem(const support_biom_t* td, const support_bptree_t* tr, const char* method_name, bool vaw,
+                           double alpha, bool bypass, bool normalize, unsigned depth, bool with_repl,
+                           const char* mmap_dir, TMat** result, bool want_fp64) {
+    // src/api.cpp:775-856: method must match the result precision
+    Method m;
+    bool fp64;
+    if (!parse_method(method_name, m, fp64)) return unknown_method;
+    if (fp64 != want_fp64) return invalid_method;
+    if (!td || !tr || td->n_samples <= 0 || td->n_obs <= 0) return table_empty;
+    Table t = Table::borrow(td->obs_ids, td->sample_ids, td->indptr, td->indices, td->data, (uint32_t)td->n_obs,
+                            (uint32_t)td->n_samples);
+    PhyloTree tree = PhyloTree::from_arrays(tr->structure, tr->lengths, tr->names, tr->n_parens);
+    ComputeStatus v = validate(t, tree);
+    if (v != okay) return v;
+    return matrix_common<TReal, TMat>(t, tree, method_name, vaw, alpha, bypass, normalize, depth, with_repl, mmap_dir,
+                                      result);
+}
+
+bool file_exists(const char* f) {
+    std::ifstream in(f);
+    return in.good();
+}
+
+std::string read_file(const char* f) {
+    std::ifstream in(f);
+    std::stringstream ss;
+    ss << in.rdbuf();
+    return ss.str();
+}
+
+ComputeStatus load_table(const char* biom_filename, Table& t) {
+    if (!biom_filename || !file_exists(biom_filename)) return table_missing;
+    std::string err;
+    if (!ssu::load_biom(biom_filename, t, &err)) {
+        set_error(err);
+        return table_missing;
+    }
+    return okay;
+}
+
+template <class TReal, class TMat>
+ComputeStatus matrix_files(const char* biom_filename, const PhyloTree* tree_in, const char* tree_filename,
+                           const char* method_name, bool vaw, double alpha, bool bypass, bool normalize, unsigned depth,
+                           bool with_repl, const char* mmap_dir, TMat** result) {
+    if (!tree_in) {
+        if (!biom_filename || !file_exists(biom_filename)) return table_missing;
+        if (!tree_filename || !file_exists(tree_filename)) return tree_missing;
+    } else if (!biom_filename || !file_exists(biom_filename)) {
+        return table_missing;
+    }
+    PhyloTree parsed;
+    if (!tree_in) parsed = PhyloTree::from_newick(read_file(tree_filename).c_str());
+    const PhyloTree& tree = tree_in ? *tree_in : parsed;
+    Table t;
+    ComputeStatus st = load_table(biom_filename, t);
+    if (st != okay) return st;
+    st = validate(t, tree);
+    if (st != okay) return st;
+    return matrix_common<TReal, TMat>(t, tree, method_name, vaw, alpha, bypass, normalize, depth, with_repl, mmap_dir,
+                                      result);
+}
+
+// Condensed upper triangle (src/api.cpp:492-525, one_off_inmem_cpp), computed from the full matrix.
+ComputeStatus condensed_common(const Table& t, const PhyloTree& tree, const char* method_name, bool vaw, double alpha,
+                               bool bypass, bool normalize, mat_t** result) {
+    Method m;
+    bool fp64;
+    if (!parse_method(method_name, m, fp64)) return unknown_method;
+    const uint64_t n = t.n_samples;
+    std::vector<double> full(n * n);
+    ComputeStatus st = run_full<double>(t, tree, make_spec(m, fp64, vaw, alpha, bypass, normalize), full.data());
+    if (st != okay) return st;
+    mat_t* r = (mat_t*)malloc(sizeof(mat_t));
+    r->n_samples = (unsigned)n;
+    r->cf_size = (unsigned)(n * (n - 1) / 2);
+    r->is_upper_triangle = true;
+    r->sample_ids = dup_ids(t.sample_ids);
+    r->condensed_form = (double*)malloc(sizeof(double) * (r->cf_size ? r->cf_size : 1));
+    uint64_t c = 0;
+    for (uint64_t i = 0; i < n; ++i)
+        for (uint64_t j = i + 1; j < n; ++j) r->condensed_form[c++] = full[i * n + j];
+    *result = r;
+    return okay;
+}
+
+ComputeStatus condensed_files(const char* biom_filename, const PhyloTree* tree_in, const char* tree_filename,
+                              const char* method_name, bool vaw, double alpha, bool bypass, bool normalize,
+                              mat_t** result) {
+    if (!biom_filename || !file_exists(biom_filename)) return table_missing;
+    if (!tree_in && (!tree_filename || !file_exists(tree_filename))) return tree_missing;
+    PhyloTree parsed;
+    if (!tree_in) parsed = PhyloTree::from_newick(read_file(tree_filename).c_str());
+    const PhyloTree& tree = tree_in ? *tree_in : parsed;
+    Table t;
+    ComputeStatus st = load_table(biom_filename, t);
+    if (st != okay) return st;
+    st = validate(t, tree);
+    if (st != okay) return st;
+    return condensed_common(t, tree, method_name, vaw, alpha, bypass, normalize, result);
+}
+
+// src/api.cpp:723-773 (one_dense_pair): element [0][1] of the 2-sample matrix
+ComputeStatus dense_pair(unsigned n_obs, const char** obs_ids, const double* s1, const double* s2,
+                         const PhyloTree& tree, const char* method_name, bool vaw, double alpha, bool bypass,
+                         bool normalize, double* result) {
+    Method m;
+    bool fp64;
+    if (!parse_method(method_name, m, fp64)) return unknown_method;
+    Table t = Table::dense_pair(obs_ids, s1, s2, n_obs);
+    ComputeStatus st = validate(t, tree);
+    if (st != okay) return st;
+    double full[4];
+    st = run_full<double>(t, tree, make_spec(m, fp64, vaw, alpha, bypass, normalize), full);
+    if (st == okay) *result = full[1];
+    return st;
+}
+
+ComputeStatus faith_common(const Table& t0, const PhyloTree& tree, r_vec** result) {
+    // src/api.cpp:569-604: drop zero-count samples (total < 1), shear, Faith PD
+    if (t0.n_samples == 0 || t0.n_obs == 0) return table_empty;
+    Table nz = t0.filtered(1.0);
+    const bool degenerate = nz.n_samples == 0 || nz.n_obs == 0;
+    if (degenerate) fprintf(stderr, "WARNING: All samples had zero counts. Forcing zero result.\n");
+    else if (nz.n_samples != t0.n_samples || nz.n_obs != t0.n_obs)
+        fprintf(stderr, "WARNING: Some samples had zero counts and were filtered out.\n");
+    const Table& t = degenerate ? t0 : nz;
+    r_vec* r = (r_vec*)malloc(sizeof(r_vec));
+    r->n_samples = t.n_samples;
+    r->values = (double*)calloc(t.n_samples ? t.n_samples : 1, sizeof(double));
+    r->sample_ids = dup_ids(t.sample_ids);
+    if (!degenerate) {
+        std::string err;
+        ssu::PostorderNodes nodes = synced_nodes(t, tree);
+        std::unique_ptr<Plan> plan = Plan::create(t, nodes, ComputeSpec{}, &err);
+        if (!plan || !plan->faith(r->values, &err)) {
+            set_error(err);
+            r_vec* rr = r;
+            destroy_results_vec(&rr);
+            return output_error;
+        }
+    }
+    *result = r;
+    return okay;
+}
+
+}  // namespace
+
+// ===================================================================== trees
+
+void ssu_set_random_seed(unsigned int new_seed) { g_rng.seed(new_seed); }
+
+IOStatus read_bptree_opaque(const char* tree_filename, opaque_bptree_t** tree_data) {
+    if (!tree_data) return unexpected_end;
+    if (!tree_filename || !file_exists(tree_filename)) return open_error;
+    *tree_data = (opaque_bptree_t*)new PhyloTree(PhyloTree::from_newick(read_file(tree_filename).c_str()));
+    return read_okay;
+}
+
+void load_bptree_opaque(const char* newick, opaque_bptree_t** tree_data) {
+    *tree_data = (opaque_bptree_t*)new PhyloTree(PhyloTree::from_newick(newick));
+}
+
+void convert_bptree_opaque(const support_bptree_t* in, opaque_bptree_t** tree_data) {
+    *tree_data = (opaque_bptree_t*)new PhyloTree(
+        PhyloTree::from_arrays(in->structure, in->lengths, in->names, in->n_parens));
+}
+
+void destroy_bptree_opaque(opaque_bptree_t** tree_data) {
+    if (tree_data) {
+        delete (PhyloTree*)(*tree_data);
+        *tree_data = nullptr;
+    }
+}
+
+// src/api.cpp:166-172: the number of distinct tip names
+int get_bptree_opaque_els(opaque_bptree_t* tree_data) {
+    return tree_data ? (int)((PhyloTree*)tree_data)->n_tips() : 0;
+}
+
+// ================================================================ destructors
+
+void destroy_mat(mat_t** result) {
+    if (!result || !*result) return;
+    for (unsigned i = 0; i < (*result)->n_samples; ++i) free((*result)->sample_ids[i]);
+    free((*result)->sample_ids);
+    free((*result)->condensed_form);
+    free(*result);
+    *result = nullptr;
+}
+
+void destroy_mat_full_fp64(mat_full_fp64_t** result) { destroy_full<double, mat_full_fp64_t>(result); }
+void destroy_mat_full_fp32(mat_full_fp32_t** result) { destroy_full<float, mat_full_fp32_t>(result); }
+
+void destroy_partial_mat(partial_mat_t** result) {
+    if (!result || !*result) return;
+    partial_mat_t* r = *result;
+    for (uint32_t i = 0; i < r->n_samples; ++i) free(r->sample_ids[i]);
+    free(r->sample_ids);
+    for (uint32_t i = 0; i < r->stripe_stop - r->stripe_start; ++i) free(r->stripes[i]);
+    free(r->stripes);
+    free(r);
+    *result = nullptr;
+}
+
+void destroy_partial_dyn_mat(partial_dyn_mat_t** result) {
+    if (!result || !*result) return;
+    partial_dyn_mat_t* r = *result;
+    for (uint32_t i = 0; i < r->n_samples; ++i) free(r->sample_ids[i]);
+    free(r->sample_ids);
+    if (r->stripes)
+        for (uint32_t i = 0; i < r->stripe_stop - r->stripe_start; ++i) free(r->stripes[i]);
+    free(r->stripes);
+    free(r->offsets);
+    free(r->filename);
+    free(r);
+    *result = nullptr;
+}
+
+void destroy_results_vec(r_vec** result) {
+    if (!result || !*result) return;
+    for (unsigned i = 0; i < (*result)->n_samples; ++i) free((*result)->sample_ids[i]);
+    free((*result)->sample_ids);
+    free((*result)->values);
+    free(*result);
+    *result = nullptr;
+}
+
+// =================================================================== one_off
+
+ComputeStatus one_off_v3(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                         bool variance_adjust, double alpha, bool bypass_tips, bool normalize_sample_counts,
+                         unsigned int, mat_t** result) {
+    return condensed_files(biom_filename, nullptr, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips,
+                           normalize_sample_counts, result);
+}
+
+ComputeStatus one_off_wtree_v3(const char* biom_filename, const opaque_bptree_t* tree_data, const char* unifrac_method,
+                               bool variance_adjust, double alpha, bool bypass_tips, bool normalize_sample_counts,
+                               unsigned int, mat_t** result) {
+    if (!tree_data) return tree_missing;
+    return condensed_files(biom_filename, (const PhyloTree*)tree_data, nullptr, unifrac_method, variance_adjust, alpha,
+                           bypass_tips, normalize_sample_counts, result);
+}
+
+ComputeStatus one_off(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                      bool variance_adjust, double alpha, bool bypass_tips, unsigned int n_substeps, mat_t** result) {
+    return one_off_v3(biom_filename, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips, true,
+                      n_substeps, result);
+}
+
+ComputeStatus one_off_wtree(const char* biom_filename, const opaque_bptree_t* tree_data, const char* unifrac_method,
+                            bool variance_adjust, double alpha, bool bypass_tips, unsigned int n_substeps,
+                            mat_t** result) {
+    return one_off_wtree_v3(biom_filename, tree_data, unifrac_method, variance_adjust, alpha, bypass_tips, true,
+                            n_substeps, result);
+}
+
+// ============================================================ in-memory matrix
+
+ComputeStatus one_off_matrix_inmem_v3(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                                      const char* unifrac_method, bool variance_adjust, double alpha, bool bypass_tips,
+                                      bool normalize_sample_counts, unsigned int, unsigned int subsample_depth,
+                                      bool subsample_with_replacement, const char* mmap_dir,
+                                      mat_full_fp64_t** result) {
+    return matrix_inmem<double, mat_full_fp64_t>(table_data, tree_data, unifrac_method, variance_adjust, alpha,
+                                                 bypass_tips, normalize_sample_counts, subsample_depth,
+                                                 subsample_with_replacement, mmap_dir, result, true);
+}
+
+ComputeStatus one_off_matrix_inmem_v2(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                                      const char* unifrac_method, bool variance_adjust, double alpha, bool bypass_tips,
+                                      unsigned int n_substeps, unsigned int subsample_depth,
+                                      bool subsample_with_replacement, const char* mmap_dir,
+                                      mat_full_fp64_t** result) {
+    return one_off_matrix_inmem_v3(table_data, tree_data, unifrac_method, variance_adjust, alpha, bypass_tips, true,
+                                   n_substeps, subsample_depth, subsample_with_replacement, mmap_dir, result);
+}
+
+ComputeStatus one_off_inmem(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                            const char* unifrac_method, bool variance_adjust, double alpha, bool bypass_tips,
+                            unsigned int n_substeps, mat_full_fp64_t** result) {
+    return one_off_matrix_inmem_v3(table_data, tree_data, unifrac_method, variance_adjust, alpha, bypass_tips, true,
+                                   n_substeps, 0, true, nullptr, result);
+}
+
+ComputeStatus one_off_matrix_inmem_fp32_v3(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                                           const char* unifrac_method, bool variance_adjust, double alpha,
+                                           bool bypass_tips, bool normalize_sample_counts, unsigned int,
+                                           unsigned int subsample_depth, bool subsample_with_replacement,
+                                           const char* mmap_dir, mat_full_fp32_t** result) {
+    return matrix_inmem<float, mat_full_fp32_t>(table_data, tree_data, unifrac_method, variance_adjust, alpha,
+                                                bypass_tips, normalize_sample_counts, subsample_depth,
+                                                subsample_with_replacement, mmap_dir, result, false);
+}
+
+ComputeStatus one_off_matrix_inmem_fp32_v2(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                                           const char* unifrac_method, bool variance_adjust, double alpha,
+                                           bool bypass_tips, unsigned int n_substeps, unsigned int subsample_depth,
+                                           bool subsample_with_replacement, const char* mmap_dir,
+                                           mat_full_fp32_t** result) {
+    return one_off_matrix_inmem_fp32_v3(table_data, tree_data, unifrac_method, variance_adjust, alpha, bypass_tips,
+                                        true, n_substeps, subsample_depth, subsample_with_replacement, mmap_dir,
+                                        result);
+}
+
+ComputeStatus one_off_inmem_fp32(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                                 const char* unifrac_method, bool variance_adjust, double alpha, bool bypass_tips,
+                                 unsigned int n_substeps, mat_full_fp32_t** result) {
+    return one_off_matrix_inmem_fp32_v3(table_data, tree_data, unifrac_method, variance_adjust, alpha, bypass_tips,
+                                        true, n_substeps, 0, true, nullptr, result);
+}
+
+// ================================================================ file matrix
+
+ComputeStatus one_off_matrix_v3(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                                bool variance_adjust, double alpha, bool bypass_tips, bool normalize_sample_counts,
+                                unsigned int, unsigned int subsample_depth, bool subsample_with_replacement,
+                                const char* mmap_dir, mat_full_fp64_t** result) {
+    return matrix_files<double, mat_full_fp64_t>(biom_filename, nullptr, tree_filename, unifrac_method,
+                                                 variance_adjust, alpha, bypass_tips, normalize_sample_counts,
+                                                 subsample_depth, subsample_with_replacement, mmap_dir, result);
+}
+
+ComputeStatus one_off_matrix_v3t(const char* biom_filename, const opaque_bptree_t* tree_data,
+                                 const char* unifrac_method, bool variance_adjust, double alpha, bool bypass_tips,
+                                 bool normalize_sample_counts, unsigned int, unsigned int subsample_depth,
+                                 bool subsample_with_replacement, const char* mmap_dir, mat_full_fp64_t** result) {
+    if (!tree_data) return tree_missing;
+    return matrix_files<double, mat_full_fp64_t>(biom_filename, (const PhyloTree*)tree_data, nullptr, unifrac_method,
+                                                 variance_adjust, alpha, bypass_tips, normalize_sample_counts,
+                                                 subsample_depth, subsample_with_replacement, mmap_dir, result);
+}
+
+ComputeStatus one_off_matrix_v2(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                                bool variance_adjust, double alpha, bool bypass_tips, unsigned int n_substeps,
+                                unsigned int subsample_depth, bool subsample_with_replacement, const char* mmap_dir,
+                                mat_full_fp64_t** result) {
+    return one_off_matrix_v3(biom_filename, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips, true,
+                             n_substeps, subsample_depth, subsample_with_replacement, mmap_dir, result);
+}
+
+ComputeStatus one_off_matrix_v2t(const char* biom_filename, const opaque_bptree_t* tree_data,
+                                 const char* unifrac_method, bool variance_adjust, double alpha, bool bypass_tips,
+                                 unsigned int n_substeps, unsigned int subsample_depth,
+                                 bool subsample_with_replacement, const char* mmap_dir, mat_full_fp64_t** result) {
+    return one_off_matrix_v3t(biom_filename, tree_data, unifrac_method, variance_adjust, alpha, bypass_tips, true,
+                              n_substeps, subsample_depth, subsample_with_replacement, mmap_dir, result);
+}
+
+ComputeStatus one_off_matrix(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                             bool variance_adjust, double alpha, bool bypass_tips, unsigned int n_substeps,
+                             const char* mmap_dir, mat_full_fp64_t** result) {
+    return one_off_matrix_v3(biom_filename, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips, true,
+                             n_substeps, 0, true, mmap_dir, result);
+}
+
+ComputeStatus one_off_matrix_fp32_v3(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                                     bool variance_adjust, double alpha, bool bypass_tips,
+                                     bool normalize_sample_counts, unsigned int, unsigned int subsample_depth,
+                                     bool subsample_with_replacement, const char* mmap_dir,
+                                     mat_full_fp32_t** result) {
+    return matrix_files<float, mat_full_fp32_t>(biom_filename, nullptr, tree_filename, unifrac_method,
+                                                variance_adjust, alpha, bypass_tips, normalize_sample_counts,
+                                                subsample_depth, subsample_with_replacement, mmap_dir, result);
+}
+
+ComputeStatus one_off_matrix_fp32_v3t(const char* biom_filename, const opaque_bptree_t* tree_data,
+                                      const char* unifrac_method, bool variance_adjust, double alpha,
+                                      bool bypass_tips, bool normalize_sample_counts, unsigned int,
+                                      unsigned int subsample_depth, bool subsample_with_replacement,
+                                      const char* mmap_dir, mat_full_fp32_t** result) {
+    if (!tree_data) return tree_missing;
+    return matrix_files<float, mat_full_fp32_t>(biom_filename, (const PhyloTree*)tree_data, nullptr, unifrac_method,
+                                                variance_adjust, alpha, bypass_tips, normalize_sample_counts,
+                                                subsample_depth, subsample_with_replacement, mmap_dir, result);
+}
+
+ComputeStatus one_off_matrix_fp32_v2(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                                     bool variance_adjust, double alpha, bool bypass_tips, unsigned int n_substeps,
+                                     unsigned int subsample_depth, bool subsample_with_replacement,
+                                     const char* mmap_dir, mat_full_fp32_t** result) {
+    return one_off_matrix_fp32_v3(biom_filename, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips,
+                                  true, n_substeps, subsample_depth, subsample_with_replacement, mmap_dir, result);
+}
+
+ComputeStatus one_off_matrix_fp32_v2t(const char* biom_filename, const opaque_bptree_t* tree_data,
+                                      const char* unifrac_method, bool variance_adjust, double alpha,
+                                      bool bypass_tips, unsigned int n_substeps, unsigned int subsample_depth,
+                                      bool subsample_with_replacement, const char* mmap_dir,
+                                      mat_full_fp32_t** result) {
+    return one_off_matrix_fp32_v3t(biom_filename, tree_data, unifrac_method, variance_adjust, alpha, bypass_tips, true,
+                                   n_substeps, subsample_depth, subsample_with_replacement, mmap_dir, result);
+}
+
+ComputeStatus one_off_matrix_fp32(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                                  bool variance_adjust, double alpha, bool bypass_tips, unsigned int n_substeps,
+                                  const char* mmap_dir, mat_full_fp32_t** result) {
+    return one_off_matrix_fp32_v3(biom_filename, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips,
+                                  true, n_substeps, 0, true, mmap_dir, result);
+}
+
+// ================================================================ dense pair
+
+ComputeStatus one_dense_pair_v3t(unsigned int n_obs, const char** obs_ids, const double* sample1,
+                                 const double* sample2, const opaque_bptree_t* tree_data, const char* unifrac_method,
+                                 bool variance_adjust, double alpha, bool bypass_tips, bool normalize_sample_counts,
+                                 double* result) {
+    if (!tree_data) return tree_missing;
+    return dense_pair(n_obs, obs_ids, sample1, sample2, *(const PhyloTree*)tree_data, unifrac_method,
+                      variance_adjust, alpha, bypass_tips, normalize_sample_counts, result);
+}
+
+ComputeStatus one_dense_pair_v3(unsigned int n_obs, const char** obs_ids, const double* sample1,
+                                const double* sample2, const support_bptree_t* tree_data, const char* unifrac_method,
+                                bool variance_adjust, double alpha, bool bypass_tips, bool normalize_sample_counts,
+                                double* result) {
+    if (!tree_data) return tree_missing;
+    PhyloTree tree = PhyloTree::from_arrays(tree_data->structure, tree_data->lengths, tree_data->names,
+                                            tree_data->n_parens);
+    return dense_pair(n_obs, obs_ids, sample1, sample2, tree, unifrac_method, variance_adjust, alpha, bypass_tips,
+                      normalize_sample_counts, result);
+}
+
+ComputeStatus one_dense_pair_v2t(unsigned int n_obs, const char** obs_ids, const double* sample1,
+                                 const double* sample2, const opaque_bptree_t* tree_data, const char* unifrac_method,
+                                 bool variance_adjust, double alpha, bool bypass_tips, double* result) {
+    return one_dense_pair_v3t(n_obs, obs_ids, sample1, sample2, tree_data, unifrac_method, variance_adjust, alpha,
+                              bypass_tips, true, result);
+}
+
+ComputeStatus one_dense_pair_v2(unsigned int n_obs, const char** obs_ids, const double* sample1,
+                                const double* sample2, const support_bptree_t* tree_data, const char* unifrac_method,
+                                bool variance_adjust, double alpha, bool bypass_tips, double* result) {
+    return one_dense_pair_v3(n_obs, obs_ids, sample1, sample2, tree_data, unifrac_method, variance_adjust, alpha,
+                             bypass_tips, true, result);
+}
+
+// ================================================================== Faith PD
+
+ComputeStatus faith_pd_one_off(const char* biom_filename, const char* tree_filename, r_vec** result) {
+    if (!biom_filename || !file_exists(biom_filename)) return table_missing;
+    if (!tree_filename || !file_exists(tree_filename)) return tree_missing;
+    PhyloTree tree = PhyloTree::from_newick(read_file(tree_filename).c_str());
+    Table t;
+    ComputeStatus st = load_table(biom_filename, t);
+    if (st != okay) return st;
+    st = validate(t, tree);
+    if (st != okay) return st;
+    return faith_common(t, tree, result);
+}
+
+ComputeStatus ssu_faith_pd_inmem(const support_biom_t* td, const support_bptree_t* tr, r_vec** result) {
+    if (!td || !tr || td->n_samples <= 0 || td->n_obs <= 0) return table_empty;
+    Table t = Table::borrow(td->obs_ids, td->sample_ids, td->indptr, td->indices, td->data, (uint32_t)td->n_obs,
+                            (uint32_t)td->n_samples);
+    PhyloTree tree = PhyloTree::from_arrays(tr->structure, tr->lengths, tr->names, tr->n_parens);
+    ComputeStatus st = validate(t, tree);
+    if (st != okay) return st;
+    return faith_common(t, tree, result);
+}
+
+// =================================================================== partial
+
+// src/api.cpp:527-567: stripes [stripe_start, stripe_stop) of the full matrix
+ComputeStatus partial_v3(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                         bool variance_adjust, double alpha, bool bypass_tips, bool normalize_sample_count,
+                         unsigned int, unsigned int stripe_start, unsigned int stripe_stop, partial_mat_t** result) {
+    if (!biom_filename || !file_exists(biom_filename)) return table_missing;
+    if (!tree_filename || !file_exists(tree_filename)) return tree_missing;
+    Method m;
+    bool fp64;
+    if (!parse_method(unifrac_method, m, fp64)) return unknown_method;
+    PhyloTree tree = PhyloTree::from_newick(read_file(tree_filename).c_str());
+    Table t;
+    ComputeStatus st = load_table(biom_filename, t);
+    if (st != okay) return st;
+    st = validate(t, tree);
+    if (st != okay) return st;
+    const uint32_t n = t.n_samples;
+    const uint32_t total = (n + 1) / 2;
+    if (stripe_stop > total) {
+        set_error("stopping stripe is out-of-bounds, max " + std::to_string(total));
+        return output_error;
+    }
+    if (stripe_stop <= stripe_start) stripe_stop = total;  // src/api.cpp:456-457 (set_tasks)
+    std::string err;
+    ssu::PostorderNodes nodes = synced_nodes(t, tree);
+    std::unique_ptr<Plan> plan =
+        Plan::create(t, nodes, make_spec(m, fp64, variance_adjust, alpha, bypass_tips, normalize_sample_count), &err);
+    if (!plan || !plan->execute(&err)) { set_error(err); return output_error; }
+    partial_mat_t* r = (partial_mat_t*)malloc(sizeof(partial_mat_t));
+    r->n_samples = n;
+    r->sample_ids = dup_ids(t.sample_ids);
+    r->stripe_start = stripe_start;
+    r->stripe_stop = stripe_stop;
+    r->stripe_total = total;
+    r->is_upper_triangle = true;
+    const uint32_t rows = stripe_stop - stripe_start;
+    r->stripes = (double**)malloc(sizeof(double*) * (rows ? rows : 1));
+    for (uint32_t i = 0; i < rows; ++i) r->stripes[i] = (double*)malloc(sizeof(double) * (n ? n : 1));
+    if (!plan->copy_stripes(stripe_start, stripe_stop, r->stripes, &err)) {
+        set_error(err);
+        destroy_partial_mat(&r);
+        return output_error;
+    }
+    *result = r;
+    return okay;
+}
+
+ComputeStatus partial(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
+                      bool variance_adjust, double alpha, bool bypass_tips, unsigned int n_substeps,
+                      unsigned int stripe_start, unsigned int stripe_stop, partial_mat_t** result) {
+    return partial_v3(biom_filename, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips, true,
+                      n_substeps, stripe_start, stripe_stop, result);
+}
+
+// ============================================================= text writers
+
+// src/api.cpp:1711-1757 (write_mat): tab-separated square matrix with a header row
+IOStatus write_mat(const char* output_filename, mat_t* result) {
+    FILE* f = fopen(output_filename, "w");
+    if (!f) return open_error;
+    const uint64_t n = result->n_samples;
+    const uint64_t comb_N = n * (n - 1) / 2;
+    for (uint64_t i = 0; i < n; ++i) fprintf(f, "\t%s", result->sample_ids[i]);
+    fprintf(f, "\n");
+    for (uint64_t i = 0; i < n; ++i) {
+        fprintf(f, "%s", result->sample_ids[i]);
+        for (uint64_t j = 0; j < n; ++j) {
+            double v = 0.0;
+            if (i < j) { const uint64_t ni = n - i; v = result->condensed_form[comb_N - ni * (ni - 1) / 2 + (j - i - 1)]; }
+            else if (i > j) { const uint64_t nj = n - j; v = result->condensed_form[comb_N - nj * (nj - 1) / 2 + (i - j - 1)]; }
+            fprintf(f, "\t%.16g", v);
+        }
+        fprintf(f, "\n");
+    }
+    fclose(f);
+    return write_okay;
+}
+
+// src/api.cpp:1640-1775 (write_mat_from_matrix_txt_T): header of ids, then one row per sample,
+// values in std::to_chars fixed (shortest round-trip) form
+template <class TMat>
+IOStatus write_full_txt(const char* output_filename, TMat* result) {
+    FILE* f = fopen(output_filename, "w");
+    if (!f) return write_error;
+    const uint64_t n = result->n_samples;
+    std::string line;
+    for (uint64_t i = 0; i < n; ++i) { line += '\t'; line += result->sample_ids[i]; }
+    line += '\n';
+    IOStatus rc = write_okay;
+    if (fwrite(line.data(), line.size(), 1, f) != 1) rc = write_error;
+    char buf[400];
+    for (uint64_t i = 0; i < n && rc == write_okay; ++i) {
+        line.assign(result->sample_ids[i]);
+        for (uint64_t j = 0; j < n; ++j) {
+            line += '\t';
+            auto r = std::to_chars(buf, buf + sizeof(buf), result->matrix[i * n + j], std::chars_format::fixed);
+            line.append(buf, r.ptr);
+        }
+        line += '\n';
+        if (fwrite(line.data(), line.size(), 1, f) != 1) rc = write_error;
+    }
+    fclose(f);
+    return rc;
+}
+
+IOStatus write_mat_from_matrix(const char* output_filename, mat_full_fp64_t* result) {
+    return write_full_txt(output_filename, result);
+}
+
+IOStatus write_mat_from_matrix_fp32(const char* output_filename, mat_full_fp32_t* result) {
+    return write_full_txt(output_filename, result);
+}
+
+// src/api.cpp:1759-1775 (write_vec): "#SampleID\tfaith_pd" then one line per sample
+IOStatus write_vec(const char* output_filename, r_vec* result) {
+    FILE* f = fopen(output_filename, "w");
+    if (!f) return open_error;
+    fprintf(f, "#SampleID\tfaith_pd\n");
+    for (unsigned i = 0; i < result->n_samples; ++i) fprintf(f, "%s\t%.16g\n", result->sample_ids[i], result->values[i]);
+    fclose(f);
+    return write_okay;
+}
+
+// ================================================================ extensions
+
+struct ssu_plan {
+    Table table;
+    std::unique_ptr<Plan> plan;
+};
+
+ComputeStatus ssu_plan_create_inmem(const support_biom_t* td, const support_bptree_t* tr, const char* method_name,
+                                    bool variance_adjust, double alpha, bool bypass_tips, bool normalize_sample_counts,
+                                    int unweighted_kernel, ssu_plan_t** out) {
+    Method m;
+    bool fp64;
+    if (!parse_method(method_name, m, fp64)) return unknown_method;
+    if (!td || !tr || td->n_samples <= 0 || td->n_obs <= 0) return table_empty;
+    std::unique_ptr<ssu_plan> p(new ssu_plan);
+    p->table = Table::borrow(td->obs_ids, td->sample_ids, td->indptr, td->indices, td->data, (uint32_t)td->n_obs,
+                             (uint32_t)td->n_samples);
+    PhyloTree tree = PhyloTree::from_arrays(tr->structure, tr->lengths, tr->names, tr->n_parens);
+    ComputeStatus st = validate(p->table, tree);
+    if (st != okay) return st;
+    ComputeSpec spec = make_spec(m, fp64, variance_adjust, alpha, bypass_tips, normalize_sample_counts);
+    spec.unweighted_kernel = unweighted_kernel;
+    std::string err;
+    ssu::PostorderNodes nodes = synced_nodes(p->table, tree);
+    p->plan = Plan::create(p->table, nodes, spec, &err);
+    if (!p->plan) { set_error(err); return output_error; }
+    *out = p.release();
+    return okay;
+}
+
+ComputeStatus ssu_plan_execute(ssu_plan_t* plan) {
+    if (!plan) return output_error;
+    std::string err;
+    if (!plan->plan->execute(&err)) { set_error(err); return output_error; }
+    return okay;
+}
+
+int ssu_plan_result_bytes(const ssu_plan_t* plan) { return plan && plan->plan->out_fp64() ? 8 : 4; }
+
+ComputeStatus ssu_plan_copy_matrix(const ssu_plan_t* plan, void* host_out) {
+    std::string err;
+    if (!plan || !plan->plan->copy_matrix(host_out, &err)) { set_error(err); return output_error; }
+    return okay;
+}
+
+int ssu_plan_timings(const ssu_plan_t* plan, double* ms, int cap) {
+    if (!plan) return 0;
+    const float* t = plan->plan->timings();
+    int k = 0;
+    for (; k < ssu::PH_COUNT && k < cap; ++k) ms[k] = t[k];
+    return k;
+}
+
+uint64_t ssu_plan_pair_nodes(const ssu_plan_t* plan) {
+    if (!plan) return 0;
+    const uint64_t n = plan->plan->n_samples();
+    return n * (n - 1) / 2 * plan->plan->processed_nodes();
+}
+
+const char* ssu_plan_main_kernel(const ssu_plan_t* plan) { return plan ? plan->plan->main_kernel() : ""; }
+
+void ssu_plan_destroy(ssu_plan_t** plan) {
+    if (plan) {
+        delete *plan;
+        *plan = nullptr;
+    }
+}
+
+int ssu_postorder(const support_bptree_t* tr, int n_obs, const char* const* obs_ids, int32_t* parent, double* length,
+                  int32_t* is_leaf, int32_t* obs_row) {
+    PhyloTree tree = PhyloTree::from_arrays(tr->structure, tr->lengths, tr->names, tr->n_parens);
+    if (!tree.valid()) return -1;
+    std::unordered_set<std::string> keep;
+    std::unordered_map<std::string, uint32_t> idx;
+    for (int i = 0; i < n_obs; ++i) { keep.insert(obs_ids[i]); idx[obs_ids[i]] = (uint32_t)i; }
+    PhyloTree s = tree.shear(keep).collapse();
+    ssu::PostorderNodes p = ssu::postorder_nodes(s, idx);
+    for (uint32_t k = 0; k < p.m; ++k) {
+        parent[k] = p.parent[k];
+        length[k] = p.length[k];
+        is_leaf[k] = p.is_leaf[k];
+        obs_row[k] = p.obs_row[k];
+    }
+    return (int)p.m;
+}
+
+int ssu_newick_to_bp(const char* newick, int cap, uint8_t* structure, double* lengths, char** names_buf,
+                     int name_cap) {
+    PhyloTree t = PhyloTree::from_newick(newick);
+    const int n = (int)t.nparens();
+    if (n > cap) return -n;
+    for (int i = 0; i < n; ++i) {
+        structure[i] = t.open[i];
+        lengths[i] = t.length[i];
+        strncpy(names_buf[i], t.name[i].c_str(), name_cap - 1);
+        names_buf[i][name_cap - 1] = 0;
+    }
+    return n;
+}
+
+int ssu_device_available(void) {
+    std::string err;
+    return ssu::device_available(&err) ? 1 : 0;
+}
+
+const char* ssu_last_error(void) { return g_last_error.c_str(); }

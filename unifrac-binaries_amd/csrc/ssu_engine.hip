@@ -1,0 +1,430 @@
+// Host orchestration of the device pipeline: schedule building, uploads,
+// buffer management and the per-method kernel sequence.
+#include "ssu_engine.hpp"
+#include "ssu_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+namespace ssu {
+
+namespace {
+
+#define SSU_HIP(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            if (err) *err = std::string(#call) + " failed: " + hipGetErrorString(e_);        \
+            return false;                                                                    \
+        }                                                                                    \
+    } while (0)
+
+size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevMem() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    bool ensure(size_t b, std::string* err) {
+        if (b <= bytes && p) return true;
+        release();
+        if (b == 0) b = 16;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            p = nullptr;
+            if (err) *err = "hipMalloc(" + std::to_string(b) + " bytes) failed: " + hipGetErrorString(e);
+            return false;
+        }
+        bytes = b;
+        return true;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+template <class T>
+bool upload(DevMem& d, const std::vector<T>& h, hipStream_t st, std::string* err) {
+    if (!d.ensure(h.size() * sizeof(T), err)) return false;
+    if (!h.empty()) SSU_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return true;
+}
+
+bool upload_raw(DevMem& d, const void* h, size_t bytes, hipStream_t st, std::string* err) {
+    if (!d.ensure(bytes, err)) return false;
+    if (bytes) SSU_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, st));
+    return true;
+}
+
+}  // namespace
+
+bool device_available(std::string* err) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        if (err) *err = std::string("no HIP device available (") + hipGetErrorString(e) + ")";
+        return false;
+    }
+    return true;
+}
+
+struct PlanImpl {
+    ComputeSpec spec;
+    uint32_t n = 0, m = 0;
+    size_t ldn = 0;
+    bool gram = false;
+    bool bad_lengths = false;
+
+    // host schedule
+    std::vector<int32_t> leaf_k, leaf_row;
+    std::vector<int32_t> level_nodes;
+    std::vector<uint32_t> level_off;
+    std::vector<int32_t> child_ptr, child_idx;
+    std::vector<int32_t> knodes;     // nodes entering the pair accumulation
+    std::vector<double> klen_d;
+    std::vector<float> klen_f;
+    std::vector<double> len_all;     // every non-root node, postorder (Faith PD)
+    // exact Gram
+    std::vector<int32_t> col_node;
+    std::vector<int8_t> digits;      // [4][Kp]
+    std::vector<GramChunk> chunks;
+    size_t Kp = 0, n_pad = 0;
+
+    hipStream_t st = nullptr;
+    hipEvent_t ev[PH_COUNT + 1] = {};
+    float times[PH_COUNT] = {};
+    std::vector<double> totals;
+
+    DevMem d_indptr, d_indices, d_data, d_totals, d_leaf_k, d_leaf_row, d_level_nodes, d_child_ptr, d_child_idx;
+    DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
+        d_faith, d_stripes;
+
+    ~PlanImpl() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    bool build_gram_columns();
+    bool embed(DevMem& dst, bool normalize, std::string* err);
+};
+
+// Exact Gram column order.  Every non-zero length is an integer times a power
+// of two, |l| = M * 2^lsb with M odd.  Lengths are grouped into buckets that
+// share one scale 2^lsb_min; inside a bucket each length is the integer
+// M' = |l| / 2^lsb_min < 2^bits, cut into 7-bit digits (int8 MFMA operands,
+// four per pass).  Buckets greedily absorb smaller lengths while the digit
+// pass count does not grow, so FP32 lengths (24-bit mantissas) spanning many
+// binades need only a handful of buckets.
+bool PlanImpl::build_gram_columns() {
+    struct Item { int32_t k; int e; int lsb; uint64_t mant; bool neg; };
+    std::vector<Item> items;
+    for (size_t q = 0; q < knodes.size(); ++q) {
+        const double l = spec.fp64 ? klen_d[q] : (double)klen_f[q];
+        if (l == 0.0) continue;
+        if (!std::isfinite(l)) return false;
+        int e = 0;
+        const double f = std::frexp(std::fabs(l), &e);        // |l| = f * 2^e, f in [0.5, 1)
+        const uint64_t mant = (uint64_t)std::ldexp(f, 53);     // exact, in [2^52, 2^53)
+        items.push_back({knodes[q], e, e - 53 + __builtin_ctzll(mant), mant, l < 0});
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.e > y.e; });
+    col_node.clear();
+    chunks.clear();
+    std::vector<std::array<int8_t, 4>> dg;
+    constexpr size_t kMaxChunk = (size_t(1) << 17) - GRAM_BK;  // keeps every int32 digit sum below 127 * 2^17
+    size_t b0 = 0;
+    while (b0 < items.size()) {
+        const int top = items[b0].e;  // every member is < 2^top
+        int lsb = items[b0].lsb;
+        size_t b1 = b0 + 1;
+        const int groups0 = (top - lsb + 27) / 28;
+        while (b1 < items.size()) {
+            const int nl = std::min(lsb, items[b1].lsb);
+            const int g = (top - nl + 27) / 28;
+            if (g > std::max(groups0, 1) || top - nl > 56) break;
+            lsb = nl;
+            ++b1;
+        }
+        const int bits = top - lsb;
+        const int n_groups = std::max(1, (bits + 27) / 28);
+        for (int g = 0; g < n_groups; ++g) {
+            const double scale = std::ldexp(1.0, lsb + 28 * g);
+            for (size_t c0 = b0; c0 < b1; c0 += kMaxChunk) {
+                const size_t c1 = std::min(b1, c0 + kMaxChunk);
+                GramChunk c;
+                c.k0 = (uint32_t)col_node.size();
+                c.scale = scale;
+                for (size_t q = c0; q < c1; ++q) {
+                    const Item& it = items[q];
+                    const int sh = it.e - 53 - lsb;  // M' = mant * 2^sh, exact
+                    const uint64_t v = sh >= 0 ? (it.mant << sh) : (it.mant >> (-sh));
+                    std::array<int8_t, 4> d{};
+                    for (int r = 0; r < 4; ++r) {
+                        const int bit = 7 * (4 * g + r);
+                        const int dig = bit < 64 ? (int)((v >> bit) & 127u) : 0;
+                        d[r] = (int8_t)(it.neg ? -dig : dig);
+                    }
+                    col_node.push_back(it.k);
+                    dg.push_back(d);
+                }
+                while (col_node.size() % GRAM_BK) { col_node.push_back(-1); dg.push_back({0, 0, 0, 0}); }
+                c.k1 = (uint32_t)col_node.size();
+                chunks.push_back(c);
+            }
+        }
+        b0 = b1;
+    }
+    Kp = col_node.size();
+    digits.assign(4 * std::max<size_t>(Kp, 1), 0);
+    for (size_t c = 0; c < Kp; ++c)
+        for (int r = 0; r < 4; ++r) digits[(size_t)r * Kp + c] = dg[c][r];
+    return true;
+}
+
+bool PlanImpl::embed(DevMem& dst, bool normalize, std::string* err) {
+    if (!dst.ensure(std::max<size_t>(m, 1) * ldn * sizeof(double), err)) return false;
+    double* E = dst.as<double>();
+    SSU_HIP(launch_embed_leaves(E, ldn, d_leaf_k.as<int32_t>(), d_leaf_row.as<int32_t>(), (uint32_t)leaf_k.size(),
+                                d_indptr.as<uint32_t>(), d_indices.as<uint32_t>(), d_data.as<double>(),
+                                d_totals.as<double>(), normalize, st));
+    for (size_t h = 0; h + 1 < level_off.size(); ++h) {
+        const uint32_t a = level_off[h], b = level_off[h + 1];
+        SSU_HIP(launch_embed_level(E, ldn, d_level_nodes.as<int32_t>() + a, b - a, d_child_ptr.as<int32_t>(),
+                                   d_child_idx.as<int32_t>(), st));
+    }
+    return true;
+}
+
+Plan::Plan() : p_(new PlanImpl) {}
+Plan::~Plan() = default;
+
+std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nodes, const ComputeSpec& spec,
+                                   std::string* err) {
+    if (!device_available(err)) return nullptr;
+    // the device kernels index with these values: reject malformed CSR up front
+    for (uint32_t i = 0; i < table.n_obs; ++i)
+        if (table.indptr[i + 1] < table.indptr[i]) { if (err) *err = "table indptr is not monotone"; return nullptr; }
+    for (uint64_t j = table.n_obs ? table.indptr[0] : 0; table.n_obs && j < table.indptr[table.n_obs]; ++j)
+        if (table.indices[j] >= table.n_samples) { if (err) *err = "table sample index out of range"; return nullptr; }
+    for (uint32_t k = 0; k < nodes.m; ++k)
+        if (nodes.obs_row[k] >= (int32_t)table.n_obs || nodes.parent[k] < 0 || (uint32_t)nodes.parent[k] > nodes.m) {
+            if (err) *err = "inconsistent postorder arrays";
+            return nullptr;
+        }
+    std::unique_ptr<Plan> plan(new Plan());
+    PlanImpl& P = *plan->p_;
+    P.spec = spec;
+    P.n = table.n_samples;
+    P.m = nodes.m;
+    P.ldn = round_up(std::max<uint32_t>(P.n, 1), PAIR_TILE);
+
+    // schedule: leaves, height levels, children lists (postorder, left to right)
+    int32_t max_h = 0;
+    for (uint32_t k = 0; k < P.m; ++k) {
+        if (nodes.is_leaf[k]) { P.leaf_k.push_back((int32_t)k); P.leaf_row.push_back(nodes.obs_row[k]); }
+        max_h = std::max(max_h, nodes.height[k]);
+    }
+    std::vector<std::vector<int32_t>> by_level(max_h + 1);
+    for (uint32_t k = 0; k < P.m; ++k)
+        if (!nodes.is_leaf[k]) by_level[nodes.height[k]].push_back((int32_t)k);
+    P.level_off.push_back(0);
+    for (int32_t h = 1; h <= max_h; ++h) {
+        P.level_nodes.insert(P.level_nodes.end(), by_level[h].begin(), by_level[h].end());
+        P.level_off.push_back((uint32_t)P.level_nodes.size());
+    }
+    std::vector<int32_t> n_child(P.m + 1, 0);
+    for (uint32_t k = 0; k < P.m; ++k)
+        if ((uint32_t)nodes.parent[k] < P.m) n_child[nodes.parent[k]]++;
+    P.child_ptr.assign(P.m + 1, 0);
+    for (uint32_t k = 0; k < P.m; ++k) P.child_ptr[k + 1] = P.child_ptr[k] + n_child[k];
+    P.child_idx.assign(std::max<int32_t>(P.child_ptr[P.m], 1), 0);
+    std::vector<int32_t> fill(P.child_ptr.begin(), P.child_ptr.end() - 1);
+    for (uint32_t k = 0; k < P.m; ++k)
+        if ((uint32_t)nodes.parent[k] < P.m) P.child_idx[fill[nodes.parent[k]]++] = (int32_t)k;
+
+    for (uint32_t k = 0; k < P.m; ++k) {
+        P.len_all.push_back(nodes.length[k]);
+        if (spec.bypass_tips && nodes.is_leaf[k]) continue;  // src/unifrac_cmp.cpp:123-124
+        P.knodes.push_back((int32_t)k);
+        P.klen_d.push_back(nodes.length[k]);
+        P.klen_f.push_back((float)nodes.length[k]);          // TFloat lengths, src/unifrac_cmp.cpp:127
+    }
+
+    const bool unw = spec.method == Method::unweighted || spec.method == Method::unweighted_unnormalized;
+    P.gram = unw && !spec.vaw && spec.unweighted_kernel == 0;
+    if (P.gram && !P.build_gram_columns()) P.gram = false;  // non-finite lengths: VALU path
+    P.n_pad = round_up(std::max<uint32_t>(P.n, 1), GRAM_BM);
+
+    std::string e2;
+    if (!err) err = &e2;
+    hipError_t he = hipStreamCreateWithFlags(&P.st, hipStreamNonBlocking);
+    if (he != hipSuccess) { *err = std::string("hipStreamCreate: ") + hipGetErrorString(he); return nullptr; }
+    for (auto& e : P.ev)
+        if (hipEventCreate(&e) != hipSuccess) { *err = "hipEventCreate failed"; return nullptr; }
+
+    P.totals = table.totals;
+    const uint64_t nnz = table.nnz();
+    const uint32_t base = table.n_obs ? table.indptr[0] : 0;
+    std::vector<uint32_t> ip(table.indptr, table.indptr + table.n_obs + 1);
+    for (auto& x : ip) x -= base;
+    bool ok = upload(P.d_indptr, ip, P.st, err) &&
+              upload_raw(P.d_indices, table.indices + base, nnz * sizeof(uint32_t), P.st, err) &&
+              upload_raw(P.d_data, table.data + base, nnz * sizeof(double), P.st, err) &&
+              upload(P.d_totals, P.totals, P.st, err) && upload(P.d_leaf_k, P.leaf_k, P.st, err) &&
+              upload(P.d_leaf_row, P.leaf_row, P.st, err) && upload(P.d_level_nodes, P.level_nodes, P.st, err) &&
+              upload(P.d_child_ptr, P.child_ptr, P.st, err) && upload(P.d_child_idx, P.child_idx, P.st, err) &&
+              upload(P.d_knodes, P.knodes, P.st, err) && upload(P.d_len_all, P.len_all, P.st, err);
+    if (ok) ok = spec.fp64 ? upload(P.d_klen, P.klen_d, P.st, err) : upload(P.d_klen, P.klen_f, P.st, err);
+    if (ok) {
+        // lengths as the accumulation sees them, in FP64, for the per-sample denominators
+        std::vector<double> ls(P.klen_f.size());
+        for (size_t q = 0; q < ls.size(); ++q) ls[q] = spec.fp64 ? P.klen_d[q] : (double)P.klen_f[q];
+        ok = upload(P.d_klen_sum, ls, P.st, err);
+    }
+    if (ok && P.gram && P.Kp)
+        ok = upload(P.d_col, P.col_node, P.st, err) && upload(P.d_digits, P.digits, P.st, err) &&
+             upload(P.d_chunks, P.chunks, P.st, err);
+    if (!ok) return nullptr;
+    if (hipStreamSynchronize(P.st) != hipSuccess) { *err = "upload failed"; return nullptr; }
+    return plan;
+}
+
+bool Plan::execute(std::string* err) {
+    PlanImpl& P = *p_;
+    const ComputeSpec& s = P.spec;
+    const size_t nn = (size_t)P.n * P.n;
+    const size_t osz = s.fp64 ? sizeof(double) : sizeof(float);
+    if (!P.d_out.ensure(std::max<size_t>(nn, 1) * osz, err)) return false;
+    const uint32_t K = (uint32_t)P.knodes.size();
+
+    SSU_HIP(hipEventRecord(P.ev[0], P.st));
+    if (!P.embed(P.d_E, s.normalize, err)) return false;
+    if (s.vaw && !P.embed(P.d_C, false, err)) return false;  // raw counts, src/unifrac_cmp.cpp:250-251
+    SSU_HIP(hipEventRecord(P.ev[1], P.st));
+
+    if (P.gram) {
+        const size_t Kp = P.Kp;
+        if (Kp) {
+            if (!P.d_Pt.ensure(P.n_pad * Kp, err)) return false;
+            SSU_HIP(launch_presence_T(P.d_Pt.as<int8_t>(), Kp, P.d_E.as<double>(), P.ldn, P.d_col.as<int32_t>(),
+                                      (uint32_t)Kp, (uint32_t)P.n_pad, P.n, P.st));
+        }
+        SSU_HIP(hipEventRecord(P.ev[2], P.st));
+        if (!P.d_G.ensure(P.n_pad * P.n_pad * sizeof(double), err)) return false;
+        if (Kp) {
+            SSU_HIP(launch_gram_i8(P.d_G.as<double>(), P.d_Pt.as<int8_t>(), Kp, P.d_digits.as<int8_t>(),
+                                   P.d_chunks.as<GramChunk>(), (uint32_t)P.chunks.size(), (uint32_t)P.n_pad, P.st));
+        } else {
+            SSU_HIP(hipMemsetAsync(P.d_G.p, 0, P.n_pad * P.n_pad * sizeof(double), P.st));
+        }
+        SSU_HIP(hipEventRecord(P.ev[3], P.st));
+        const bool normalized = s.method == Method::unweighted;
+        if (s.fp64)
+            SSU_HIP(launch_unweighted_combine<double>(P.d_out.as<double>(), P.d_G.as<double>(), P.n_pad, P.n,
+                                                      normalized, P.st));
+        else
+            SSU_HIP(launch_unweighted_combine<float>(P.d_out.as<float>(), P.d_G.as<double>(), P.n_pad, P.n,
+                                                     normalized, P.st));
+        SSU_HIP(hipEventRecord(P.ev[4], P.st));
+    } else {
+        PairArgs a{};
+        a.n = P.n;
+        a.K = K;
+        a.ldn = P.ldn;
+        a.alpha = s.alpha;
+        a.totals = P.d_totals.as<double>();
+        a.out = P.d_out.p;
+        a.out_fp64 = s.fp64;
+        a.len = P.d_klen.p;
+        PairOp op = PairOp::weighted;
+        switch (s.method) {
+        case Method::unweighted: op = s.vaw ? PairOp::vaw_unweighted : PairOp::unweighted_plain; a.normalized = true; break;
+        case Method::unweighted_unnormalized: op = s.vaw ? PairOp::vaw_unweighted : PairOp::unweighted_plain; a.normalized = false; break;
+        case Method::weighted_normalized: op = s.vaw ? PairOp::vaw_weighted : PairOp::weighted; a.normalized = true; break;
+        case Method::weighted_unnormalized: op = s.vaw ? PairOp::vaw_weighted : PairOp::weighted; a.normalized = false; break;
+        case Method::generalized: op = s.vaw ? PairOp::vaw_generalized : PairOp::generalized; a.normalized = true; break;
+        }
+        if (op == PairOp::weighted && a.normalized) {
+            // denominators: sum_k l_k (u_i + u_j) = S_i + S_j
+            if (!P.d_S.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err)) return false;
+            SSU_HIP(launch_sample_sums(P.d_S.as<double>(), P.d_E.as<double>(), P.ldn, P.d_knodes.as<int32_t>(),
+                                       P.d_klen_sum.as<double>(), K, P.n, P.st));
+            a.S = P.d_S.as<double>();
+        }
+        if (s.fp64) {
+            a.U = P.d_E.p;
+            a.Cn = s.vaw ? P.d_C.p : nullptr;
+            a.rows = P.d_knodes.as<int32_t>();
+        } else {
+            if (!P.d_U.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
+            SSU_HIP(launch_gather_f32(P.d_U.as<float>(), P.d_E.as<double>(), P.ldn, P.d_knodes.as<int32_t>(), K, P.st));
+            a.U = P.d_U.p;
+            if (s.vaw) {
+                if (!P.d_Cu.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
+                SSU_HIP(launch_gather_f32(P.d_Cu.as<float>(), P.d_C.as<double>(), P.ldn, P.d_knodes.as<int32_t>(), K,
+                                          P.st));
+                a.Cn = P.d_Cu.p;
+            }
+            a.rows = nullptr;
+        }
+        SSU_HIP(hipEventRecord(P.ev[2], P.st));
+        SSU_HIP(launch_pairs(op, s.fp64, a, P.st));
+        SSU_HIP(hipEventRecord(P.ev[3], P.st));
+        SSU_HIP(hipEventRecord(P.ev[4], P.st));
+    }
+    SSU_HIP(hipEventRecord(P.ev[5], P.st));
+    SSU_HIP(hipStreamSynchronize(P.st));
+    for (int i = 0; i < PH_TOTAL; ++i) SSU_HIP(hipEventElapsedTime(&P.times[i], P.ev[i], P.ev[i + 1]));
+    SSU_HIP(hipEventElapsedTime(&P.times[PH_TOTAL], P.ev[0], P.ev[5]));
+    return true;
+}
+
+bool Plan::copy_matrix(void* host, std::string* err) const {
+    const PlanImpl& P = *p_;
+    const size_t bytes = (size_t)P.n * P.n * (P.spec.fp64 ? sizeof(double) : sizeof(float));
+    if (bytes) SSU_HIP(hipMemcpy(host, P.d_out.p, bytes, hipMemcpyDeviceToHost));
+    return true;
+}
+
+bool Plan::copy_stripes(uint32_t start, uint32_t stop, double* const* host, std::string* err) const {
+    PlanImpl& P = *p_;
+    if (stop <= start) return true;
+    const size_t rows = stop - start;
+    if (!P.d_stripes.ensure(rows * P.n * sizeof(double), err)) return false;
+    if (P.spec.fp64)
+        SSU_HIP(launch_matrix_to_stripes<double>(P.d_stripes.as<double>(), P.d_out.as<double>(), P.n, start, stop, P.st));
+    else
+        SSU_HIP(launch_matrix_to_stripes<float>(P.d_stripes.as<double>(), P.d_out.as<float>(), P.n, start, stop, P.st));
+    SSU_HIP(hipStreamSynchronize(P.st));
+    for (size_t r = 0; r < rows; ++r)
+        SSU_HIP(hipMemcpy(host[r], P.d_stripes.as<double>() + r * P.n, P.n * sizeof(double), hipMemcpyDeviceToHost));
+    return true;
+}
+
+bool Plan::faith(double* host, std::string* err) {
+    PlanImpl& P = *p_;
+    if (!P.embed(P.d_E, true, err)) return false;
+    if (!P.d_faith.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err)) return false;
+    SSU_HIP(launch_faith(P.d_faith.as<double>(), P.d_E.as<double>(), P.ldn, P.d_len_all.as<double>(), P.m, P.n, P.st));
+    SSU_HIP(hipStreamSynchronize(P.st));
+    if (P.n) SSU_HIP(hipMemcpy(host, P.d_faith.p, P.n * sizeof(double), hipMemcpyDeviceToHost));
+    return true;
+}
+
+const float* Plan::timings() const { return p_->times; }
+uint32_t Plan::n_samples() const { return p_->n; }
+uint64_t Plan::processed_nodes() const { return p_->knodes.size(); }
+bool Plan::out_fp64() const { return p_->spec.fp64; }
+const char* Plan::main_kernel() const { return p_->gram ? "k_gram_i8" : "k_pairs"; }
+const void* Plan::device_output() const { return p_->d_out.p; }
+
+}  // namespace ssu

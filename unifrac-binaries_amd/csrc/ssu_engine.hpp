@@ -1,0 +1,65 @@
+// Device pipeline for one (table, tree, method) problem.  No HIP types here:
+// the C API layer includes this header from plain C++.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ssu_table.hpp"
+#include "ssu_tree.hpp"
+
+namespace ssu {
+
+enum class Method : int {
+    unweighted = 0,
+    unweighted_unnormalized = 1,
+    weighted_normalized = 2,
+    weighted_unnormalized = 3,
+    generalized = 4,
+};
+
+struct ComputeSpec {
+    Method method = Method::unweighted;
+    bool fp64 = false;             // "*_fp64" method names
+    bool vaw = false;              // variance adjusted
+    double alpha = 1.0;            // generalized
+    bool bypass_tips = false;
+    bool normalize = true;         // normalize_sample_counts
+    int unweighted_kernel = 0;     // 0: exact int8 MFMA Gram, 1: plain FP32/FP64 VALU tile kernel
+};
+
+// Phases timed with HIP events on the plan's stream during execute().
+enum Phase : int { PH_EMBED = 0, PH_PREP, PH_MAIN, PH_COMBINE, PH_TOTAL, PH_COUNT };
+
+struct PlanImpl;
+
+class Plan {
+public:
+    ~Plan();
+    // Uploads the table and postorder arrays; returns nullptr and sets err on failure.
+    static std::unique_ptr<Plan> create(const Table& table, const PostorderNodes& nodes, const ComputeSpec& spec,
+                                        std::string* err);
+    // Full n x n distance matrix into device memory (FP64 if spec.fp64, FP32 otherwise).
+    bool execute(std::string* err);
+    bool copy_matrix(void* host, std::string* err) const;                 // n*n elements
+    bool copy_stripes(uint32_t start, uint32_t stop, double* const* host, std::string* err) const;
+    // Faith PD over all non-root nodes (needs only the embedding).
+    bool faith(double* host, std::string* err);
+    const float* timings() const;  // ms per Phase, last execute()
+    uint32_t n_samples() const;
+    uint64_t processed_nodes() const;  // nodes entering the pair accumulation
+    bool out_fp64() const;
+    const char* main_kernel() const;
+    const void* device_output() const;
+
+private:
+    Plan();
+    std::unique_ptr<PlanImpl> p_;
+};
+
+// true iff a HIP device is usable; err explains why not
+bool device_available(std::string* err);
+
+}  // namespace ssu

@@ -1,0 +1,594 @@
+// gfx950 kernels for the Strided State UniFrac hot path.
+//
+//   embedding     level-synchronous postorder propagation of per-sample values
+//                 (src/unifrac_internal.cpp:230-258 semantics, FP64)
+//   Gram          unweighted UniFrac as an exact integer Gram on int8 MFMA:
+//                 G = A diag(l) A^T over 0/1 presence, l split into 7-bit digits
+//   pair tiles    register-tiled VALU kernels for weighted / generalized / VAW
+//                 (src/unifrac_task_impl.hpp formulas), FP32 with FP64 merges or FP64
+//   Faith PD      src/unifrac.cpp:362-387, same FP64 summation order
+#include "ssu_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+namespace ssu {
+
+namespace {
+
+constexpr int EMB_THREADS = 256;
+
+inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+// ------------------------------------------------------------------ embedding
+
+// One workgroup per leaf: clear the row, then scatter the leaf's CSR row
+// (src/biom_inmem.cpp:473-497: value / total, or the raw value).
+__global__ void __launch_bounds__(EMB_THREADS) k_embed_leaves(double* __restrict__ E, size_t ldn,
+                                                              const int32_t* __restrict__ leaf_k,
+                                                              const int32_t* __restrict__ leaf_row,
+                                                              const uint32_t* __restrict__ indptr,
+                                                              const uint32_t* __restrict__ indices,
+                                                              const double* __restrict__ data,
+                                                              const double* __restrict__ totals, int normalize) {
+    const uint32_t li = blockIdx.x;
+    double* row = E + (size_t)leaf_k[li] * ldn;
+    for (size_t s = threadIdx.x; s < ldn; s += blockDim.x) row[s] = 0.0;
+    __syncthreads();
+    const int32_t r = leaf_row[li];
+    if (r < 0) return;
+    for (uint32_t j = indptr[r] + threadIdx.x; j < indptr[r + 1]; j += blockDim.x) {
+        const uint32_t s = indices[j];
+        row[s] = normalize ? data[j] / totals[s] : data[j];
+    }
+}
+
+// Internal nodes of one height level: E[k] = ((0 + E[c0]) + E[c1]) + ..., children left to right.
+__global__ void __launch_bounds__(EMB_THREADS) k_embed_level(double* __restrict__ E, size_t ldn,
+                                                             const int32_t* __restrict__ nodes, uint32_t cols_blocks,
+                                                             const int32_t* __restrict__ child_ptr,
+                                                             const int32_t* __restrict__ child_idx) {
+    const uint32_t ni = blockIdx.x / cols_blocks;
+    const size_t s = (size_t)(blockIdx.x % cols_blocks) * EMB_THREADS + threadIdx.x;
+    if (s >= ldn) return;
+    const int32_t k = nodes[ni];
+    double acc = 0.0;
+    for (int32_t c = child_ptr[k]; c < child_ptr[k + 1]; ++c) acc += E[(size_t)child_idx[c] * ldn + s];
+    E[(size_t)k * ldn + s] = acc;
+}
+
+__global__ void __launch_bounds__(EMB_THREADS) k_gather_f32(float* __restrict__ U, const double* __restrict__ E,
+                                                            size_t ldn, const int32_t* __restrict__ knodes,
+                                                            uint32_t cols_blocks) {
+    const uint32_t r = blockIdx.x / cols_blocks;
+    const size_t s = (size_t)(blockIdx.x % cols_blocks) * EMB_THREADS + threadIdx.x;
+    if (s >= ldn) return;
+    U[(size_t)r * ldn + s] = (float)E[(size_t)knodes[r] * ldn + s];
+}
+
+__global__ void __launch_bounds__(EMB_THREADS) k_sample_sums(double* __restrict__ S, const double* __restrict__ E,
+                                                             size_t ldn, const int32_t* __restrict__ knodes,
+                                                             const double* __restrict__ klen, uint32_t K, uint32_t n) {
+    const uint32_t s = blockIdx.x * EMB_THREADS + threadIdx.x;
+    if (s >= n) return;
+    double acc = 0.0;
+    for (uint32_t k = 0; k < K; ++k) acc += klen[k] * E[(size_t)knodes[k] * ldn + s];
+    S[s] = acc;
+}
+
+// Faith PD: identical FP64 summation order to src/unifrac.cpp:373-386.
+__global__ void __launch_bounds__(EMB_THREADS) k_faith(double* __restrict__ out, const double* __restrict__ E,
+                                                       size_t ldn, const double* __restrict__ len, uint32_t m,
+                                                       uint32_t n) {
+    const uint32_t s = blockIdx.x * EMB_THREADS + threadIdx.x;
+    if (s >= n) return;
+    double acc = 0.0;
+    for (uint32_t k = 0; k < m; ++k) acc += (double)(E[(size_t)k * ldn + s] > 0.0) * len[k];
+    out[s] = acc;
+}
+
+// ------------------------------------------------------------------ Gram path
+
+// 64 K'-columns x 64 samples per workgroup: read node rows (coalesced over
+// samples), write sample rows (coalesced over columns).
+__global__ void __launch_bounds__(256) k_presence_T(int8_t* __restrict__ Pt, size_t ldp, const double* __restrict__ E,
+                                                    size_t ldn, const int32_t* __restrict__ col_node, uint32_t n) {
+    __shared__ uint8_t tile[64][64 + 4];
+    const uint32_t c0 = blockIdx.x * 64, s0 = blockIdx.y * 64;
+    const uint32_t t = threadIdx.x;
+    const uint32_t ls = t & 63;
+    for (uint32_t q = t >> 6; q < 64; q += 4) {
+        const int32_t node = col_node[c0 + q];
+        uint8_t v = 0;
+        const uint32_t s = s0 + ls;
+        if (node >= 0 && s < n) v = E[(size_t)node * ldn + s] > 0.0 ? 0xFF : 0x00;
+        tile[q][ls] = v;
+    }
+    __syncthreads();
+    // each thread writes 16 consecutive columns of one sample row
+    const uint32_t row = t >> 2, cb = (t & 3) * 16;
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) x |= (uint32_t)tile[cb + q * 4 + b][row] << (8 * b);
+        w[q] = x;
+    }
+    *reinterpret_cast<uint4*>(Pt + (size_t)(s0 + row) * ldp + c0 + cb) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// G is kept in MFMA fragment order: tile (bi, bj) occupies 128*128 consecutive
+// doubles; inside it, quadrant w (rows (w>>1)*64, columns (w&1)*64), block
+// (mi, nj), register r and lane l hold element ((((w*2 + mi)*2 + nj)*16 + r)*64 + l).
+// With the 32x32 C layout (lane l, register r -> row 8(r/4) + 4(l/32) + r%4,
+// column l%32) this maps back to (i, j) as below.
+__device__ __forceinline__ size_t gram_slot(uint32_t i, uint32_t j, uint32_t tiles) {
+    const uint32_t bi = i / GRAM_BM, bj = j / GRAM_BM;
+    const uint32_t il = i % GRAM_BM, jl = j % GRAM_BM;
+    const uint32_t w = (il / 64) * 2 + (jl / 64);
+    const uint32_t mi = (il % 64) / 32, nj = (jl % 64) / 32;
+    const uint32_t r32 = il % 32, c32 = jl % 32;
+    const uint32_t r = (r32 / 8) * 4 + (r32 % 4);
+    const uint32_t lane = c32 + 32 * ((r32 / 4) % 2);
+    return ((size_t)bi * tiles + bj) * (GRAM_BM * GRAM_BM) + ((((w * 2 + mi) * 2 + nj) * 16 + r) * 64 + lane);
+}
+
+constexpr int GLDA = GRAM_BK + 16;  // padded LDS row in bytes: ds_read_b128 stays conflict-free
+constexpr int GRAM_THREADS = 512;
+
+// Exact Gram on v_mfma_i32_32x32x32_i8.
+//   A operand: presence of the row samples, bytes 0x00 / 0xFF (0 / -1)
+//   B operand: presence of the column samples AND digit byte d_q of the column's length
+// so each MFMA accumulates -sum_c d_q(c) [i present][j present] exactly in int32.
+// Workgroup: 8 waves on a 128x128 output tile.  Wave w owns quadrant w&3
+// (rows ((w&3)>>1)*64, columns (w&1)*64, as 2x2 MFMA blocks of 32x32) and the
+// digit pair p = w>>2 (digits 2p, 2p+1): 128 accumulator registers per wave,
+// two waves per SIMD.  At the end of each K-chunk a wave merges its two
+// digits exactly, 2^(14p) (acc_2p + 2^7 acc_2p+1), and adds it times the
+// chunk scale into the fragment-ordered FP64 tile of G; the p = 1 waves add
+// first, then the p = 0 waves, separated by a barrier.
+__global__ void __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_gram_i8(double* __restrict__ G, uint32_t tiles, const int8_t* __restrict__ Pt, size_t ldp,
+          const int8_t* __restrict__ digits, const GramChunk* __restrict__ chunks, uint32_t n_chunks, uint32_t n_stages) {
+    const uint32_t bi = blockIdx.y, bj = blockIdx.x;
+    if (bi > bj) return;
+    __shared__ __attribute__((aligned(16))) int8_t sA[GRAM_BM * GLDA];
+    __shared__ __attribute__((aligned(16))) int8_t sB[GRAM_BM * GLDA];
+    __shared__ __attribute__((aligned(16))) int8_t sD[4 * GRAM_BK];
+
+    const uint32_t t = threadIdx.x;
+    const uint32_t lane = t & 63, wave = t >> 6;
+    const uint32_t quad = wave & 3, dp = wave >> 2;
+    const uint32_t wr = quad >> 1, wc = quad & 1;
+    const size_t Kp = ldp;
+
+    static_assert(GRAM_BK == 128 && GRAM_THREADS == 512, "staging below assumes 128-column stages, 512 threads");
+    // thread t stages rows t/8 and t/8 + 64 of both operands, 16-byte column group t%8
+    const uint32_t lrow = t >> 3, lc16 = (t & 7) * 16;
+    const int8_t* gA0 = Pt + ((size_t)bi * GRAM_BM + lrow) * ldp + lc16;
+    const int8_t* gB0 = Pt + ((size_t)bj * GRAM_BM + lrow) * ldp + lc16;
+    const size_t rstep = (size_t)64 * ldp;
+    int8_t* sA0 = sA + lrow * GLDA + lc16;
+    int8_t* sB0 = sB + lrow * GLDA + lc16;
+    const int8_t* gD0 = digits + (size_t)lrow * Kp + lc16;   // digit plane lrow, for t < 32
+    int8_t* sD0 = sD + lrow * GRAM_BK + lc16;
+    uint4 ra0, ra1, rb0, rb1, rd;
+#define SSU_GLOAD(stage)                                                              \
+    do {                                                                              \
+        const size_t k0_ = (size_t)(stage) * GRAM_BK;                                 \
+        ra0 = *reinterpret_cast<const uint4*>(gA0 + k0_);                             \
+        ra1 = *reinterpret_cast<const uint4*>(gA0 + rstep + k0_);                     \
+        rb0 = *reinterpret_cast<const uint4*>(gB0 + k0_);                             \
+        rb1 = *reinterpret_cast<const uint4*>(gB0 + rstep + k0_);                     \
+        if (t < 32) rd = *reinterpret_cast<const uint4*>(gD0 + k0_);                  \
+    } while (0)
+#define SSU_SSTORE()                                                                  \
+    do {                                                                              \
+        *reinterpret_cast<uint4*>(sA0) = ra0;                                         \
+        *reinterpret_cast<uint4*>(sA0 + 64 * GLDA) = ra1;                             \
+        *reinterpret_cast<uint4*>(sB0) = rb0;                                         \
+        *reinterpret_cast<uint4*>(sB0 + 64 * GLDA) = rb1;                             \
+        if (t < 32) *reinterpret_cast<uint4*>(sD0) = rd;                              \
+    } while (0)
+
+    v16i acc[2][2][2];  // [digit of the pair][mi][nj]
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int nj = 0; nj < 2; ++nj) acc[d][mi][nj] = (v16i){0};
+
+    // this lane's slot in the fragment-ordered G tile
+    double* gl = G + ((size_t)bi * tiles + bj) * (GRAM_BM * GRAM_BM) + (size_t)quad * 4096 + lane;
+    const uint32_t arow = wr * 64 + (lane & 31);
+    const uint32_t brow = wc * 64 + (lane & 31);
+    const uint32_t khalf = (lane >> 5) * 16;
+    const int8_t* sDp = sD + (2 * dp) * GRAM_BK;
+
+    uint32_t ch = 0;
+    uint32_t ch_end_stage = n_chunks ? chunks[0].k1 / GRAM_BK : 0;
+    if (n_stages) { SSU_GLOAD(0); SSU_SSTORE(); }
+    __syncthreads();
+    for (uint32_t s = 0; s < n_stages; ++s) {
+        if (s + 1 < n_stages) SSU_GLOAD(s + 1);
+#pragma unroll
+        for (int kk = 0; kk < GRAM_BK / 32; ++kk) {
+            const uint32_t ko = kk * 32 + khalf;
+            const v4i a0 = *reinterpret_cast<const v4i*>(sA + arow * GLDA + ko);
+            const v4i a1 = *reinterpret_cast<const v4i*>(sA + (arow + 32) * GLDA + ko);
+            const v4i p0 = *reinterpret_cast<const v4i*>(sB + brow * GLDA + ko);
+            const v4i p1 = *reinterpret_cast<const v4i*>(sB + (brow + 32) * GLDA + ko);
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                const v4i dw = *reinterpret_cast<const v4i*>(sDp + d * GRAM_BK + ko);
+                const v4i b0 = p0 & dw, b1 = p1 & dw;
+                acc[d][0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, acc[d][0][0], 0, 0, 0);
+                acc[d][0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, acc[d][0][1], 0, 0, 0);
+                acc[d][1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, acc[d][1][0], 0, 0, 0);
+                acc[d][1][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, acc[d][1][1], 0, 0, 0);
+            }
+        }
+        if (s + 1 == ch_end_stage) {
+            // |acc| <= 127 * 2^17, so acc_lo + 2^7 acc_hi and its 2^14 / power-of-two scalings are
+            // exact in FP64; only the add into G rounds.
+            const double sc = -chunks[ch].scale * (dp ? 16384.0 : 1.0);  // A carries -1 for "present"
+            const bool first = (ch == 0);
+            double* gb = gl;
+            asm volatile("" : "+v"(gb));  // keep the 64 slot addresses out of the main loop's registers
+            for (uint32_t turn = 0; turn < 2; ++turn) {
+                if (turn == 1 - dp) {
+#pragma unroll
+                    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                        for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+                            for (int r = 0; r < 16; ++r) {
+                                const double v = fma((double)acc[1][mi][nj][r], 128.0, (double)acc[0][mi][nj][r]);
+                                double* gp = gb + ((mi * 2 + nj) * 16 + r) * 64;
+                                *gp = (first && dp == 1) ? v * sc : fma(v, sc, *gp);
+                            }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int d = 0; d < 2; ++d)
+#pragma unroll
+                for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                    for (int nj = 0; nj < 2; ++nj) acc[d][mi][nj] = (v16i){0};
+            ++ch;
+            if (ch < n_chunks) ch_end_stage = chunks[ch].k1 / GRAM_BK;
+        }
+        __syncthreads();
+        if (s + 1 < n_stages) {
+            SSU_SSTORE();
+            __syncthreads();
+        }
+    }
+#undef SSU_GLOAD
+#undef SSU_SSTORE
+}
+
+template <class TOut>
+__global__ void __launch_bounds__(256) k_unweighted_combine(TOut* __restrict__ out, const double* __restrict__ G,
+                                                            uint32_t tiles, uint32_t n, int normalized) {
+    const uint32_t j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const uint32_t i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= n || j >= n) return;
+    double d = 0.0;
+    if (i != j) {
+        const uint32_t a = i < j ? i : j, b = i < j ? j : i;
+        const double g = G[gram_slot(a, b, tiles)];
+        const double ta = G[gram_slot(a, a, tiles)], tb = G[gram_slot(b, b, tiles)];
+        const double x = (ta + tb) - 2.0 * g;   // length-weighted XOR
+        const double o = (ta + tb) - g;         // length-weighted OR
+        d = normalized ? x / o : x;
+    }
+    out[(size_t)i * n + j] = (TOut)d;
+}
+
+// ------------------------------------------------------------------ VALU pair tiles
+
+template <class T> __device__ __forceinline__ T dabs(T x) { return x < T(0) ? -x : x; }
+
+template <class T> __device__ __forceinline__ T dsqrt(T x);
+template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
+template <> __device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
+template <class T> __device__ __forceinline__ T dpow(T x, T a);
+template <> __device__ __forceinline__ float dpow<float>(float x, float a) { return powf(x, a); }
+template <> __device__ __forceinline__ double dpow<double>(double x, double a) { return pow(x, a); }
+
+template <PairOp OP> struct PairTraits {
+    static constexpr bool two = OP != PairOp::weighted;       // numerator and denominator accumulators
+    static constexpr bool counts = OP == PairOp::vaw_weighted || OP == PairOp::vaw_generalized ||
+                                   OP == PairOp::vaw_unweighted;
+};
+
+// Per-node term for one pair.  u, v: embedded values; cu, cv: embedded counts
+// (VAW); m: c_k + c_l sample totals (VAW); l: branch length.
+template <PairOp OP, class T>
+__device__ __forceinline__ void pair_term(T u, T v, T cu, T cv, T m, T l, T alpha, T& num, T& den) {
+    if constexpr (OP == PairOp::weighted) {
+        // src/unifrac_task_impl.hpp:117-127
+        num = fma(dabs(u - v), l, num);
+    } else if constexpr (OP == PairOp::unweighted_plain) {
+        // presence is staged as 0/1 values: |pu - pv| = xor, max = or
+        num = fma(dabs(u - v), l, num);
+        den = fma(u > v ? u : v, l, den);
+    } else if constexpr (OP == PairOp::generalized) {
+        // src/unifrac_task_impl.hpp:1016-1024
+        const T s = u + v;
+        const bool nz = s != T(0);
+        const T ss = nz ? s : T(1);
+        const T w = nz ? dpow<T>(ss, alpha) * l : T(0);
+        num = fma(w, dabs(u - v) / ss, num);
+        den += w;
+    } else {
+        // src/unifrac_task_impl.hpp:881-893, 1098-1109, 1817-1829
+        const T mi = cu + cv;
+        const T vaw = dsqrt<T>(mi * (m - mi));
+        const bool ok = vaw > T(0);
+        const T rv = ok ? T(1) / vaw : T(0);
+        if constexpr (OP == PairOp::vaw_weighted) {
+            num = fma(dabs(u - v) * l, rv, num);
+            den = fma((u + v) * l, rv, den);
+        } else if constexpr (OP == PairOp::vaw_generalized) {
+            const T s1 = (u + v) * rv, d1 = dabs(u - v) * rv;
+            const T ss = ok ? s1 : T(1);
+            const T w = ok ? dpow<T>(ss, alpha) * l : T(0);
+            num = fma(w, d1 / ss, num);
+            den += w;
+        } else {  // vaw_unweighted: u, v are 0/1 presence values
+            const T lv = l * rv;
+            num = fma(dabs(u - v), lv, num);
+            den = fma(u > v ? u : v, lv, den);
+        }
+    }
+}
+
+// Stage width (nodes) and how many nodes an FP32 accumulator may collect
+// before it is merged into FP64.
+constexpr int PAIR_EB = 16;
+constexpr int PAIR_MERGE = 64;
+
+// One workgroup per 128x128 tile of the pair matrix (upper-triangle tiles);
+// thread (ti, tj) owns rows 8*ti.. (RI rows) and columns RJ*tj.. of the tile.
+template <PairOp OP, class T, int RI, int RJ>
+__global__ void __launch_bounds__((PAIR_TILE / RI) * (PAIR_TILE / RJ))
+k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __restrict__ rows, size_t ldn,
+        const T* __restrict__ len, uint32_t K,
+        uint32_t n, const double* __restrict__ S, const double* __restrict__ totals, double alpha_d, int normalized,
+        void* __restrict__ out_v, int out_fp64) {
+    constexpr int NTI = PAIR_TILE / RI, NTJ = PAIR_TILE / RJ, NT = NTI * NTJ;
+    constexpr bool TWO = PairTraits<OP>::two;
+    constexpr bool CNT = PairTraits<OP>::counts;
+    constexpr bool MIXED = sizeof(T) == 4;  // FP32 terms, FP64 merges
+    const uint32_t bi = blockIdx.y, bj = blockIdx.x;
+    if (bi > bj) return;
+    const uint32_t t = threadIdx.x;
+    const uint32_t ti = t % NTI, tj = t / NTI;
+
+    __shared__ __attribute__((aligned(16))) T su[PAIR_EB][PAIR_TILE];
+    __shared__ __attribute__((aligned(16))) T sv[PAIR_EB][PAIR_TILE];
+    __shared__ __attribute__((aligned(16))) T scu[CNT ? PAIR_EB : 1][CNT ? PAIR_TILE : 1];
+    __shared__ __attribute__((aligned(16))) T scv[CNT ? PAIR_EB : 1][CNT ? PAIR_TILE : 1];
+    __shared__ T sl[PAIR_EB];
+
+    const T alpha = (T)alpha_d;
+    const size_t i0 = (size_t)bi * PAIR_TILE, j0 = (size_t)bj * PAIR_TILE;
+    T mrow[RI][RJ];  // VAW: c_i + c_j
+    if constexpr (CNT) {
+#pragma unroll
+        for (int r = 0; r < RI; ++r)
+#pragma unroll
+            for (int c = 0; c < RJ; ++c) {
+                const size_t i = i0 + ti * RI + r, j = j0 + tj * RJ + c;
+                mrow[r][c] = (T)((i < n ? totals[i] : 0.0) + (j < n ? totals[j] : 0.0));
+            }
+    }
+
+    T num[RI][RJ], den[RI][RJ];
+    double num64[MIXED ? RI : 1][MIXED ? RJ : 1], den64[MIXED && TWO ? RI : 1][MIXED && TWO ? RJ : 1];
+#pragma unroll
+    for (int r = 0; r < RI; ++r)
+#pragma unroll
+        for (int c = 0; c < RJ; ++c) {
+            num[r][c] = T(0); den[r][c] = T(0);
+            if constexpr (MIXED) { num64[r][c] = 0.0; if constexpr (TWO) den64[r][c] = 0.0; }
+        }
+
+    for (uint32_t k0 = 0; k0 < K; k0 += PAIR_EB) {
+        // stage PAIR_EB node rows of both sample blocks
+        for (uint32_t q = t; q < PAIR_EB * PAIR_TILE; q += NT) {
+            const uint32_t e = q / PAIR_TILE, x = q % PAIR_TILE;
+            const uint32_t k = k0 + e;
+            T a = T(0), b = T(0), ca = T(0), cb = T(0);
+            if (k < K) {
+                const size_t rk = (size_t)(rows ? rows[k] : (int32_t)k) * ldn;
+                a = U[rk + i0 + x];
+                b = U[rk + j0 + x];
+                if constexpr (CNT) { ca = Cn[rk + i0 + x]; cb = Cn[rk + j0 + x]; }
+            }
+            if constexpr (OP == PairOp::unweighted_plain || OP == PairOp::vaw_unweighted) {
+                a = a > T(0) ? T(1) : T(0);
+                b = b > T(0) ? T(1) : T(0);
+            }
+            su[e][x] = a; sv[e][x] = b;
+            if constexpr (CNT) { scu[e][x] = ca; scv[e][x] = cb; }
+        }
+        if (t < PAIR_EB) sl[t] = (k0 + t < K) ? len[k0 + t] : T(0);
+        __syncthreads();
+#pragma unroll 2
+        for (int e = 0; e < PAIR_EB; ++e) {
+            T u[RI], v[RJ], cu[RI], cv[RJ];
+#pragma unroll
+            for (int r = 0; r < RI; ++r) { u[r] = su[e][ti * RI + r]; if constexpr (CNT) cu[r] = scu[e][ti * RI + r]; else cu[r] = T(0); }
+#pragma unroll
+            for (int c = 0; c < RJ; ++c) { v[c] = sv[e][tj * RJ + c]; if constexpr (CNT) cv[c] = scv[e][tj * RJ + c]; else cv[c] = T(0); }
+            const T l = sl[e];
+#pragma unroll
+            for (int r = 0; r < RI; ++r)
+#pragma unroll
+                for (int c = 0; c < RJ; ++c)
+                    pair_term<OP, T>(u[r], v[c], cu[r], cv[c], CNT ? mrow[r][c] : T(0), l, alpha, num[r][c], den[r][c]);
+        }
+        __syncthreads();
+        if constexpr (MIXED) {
+            if (((k0 / PAIR_EB + 1) % (PAIR_MERGE / PAIR_EB)) == 0 || k0 + PAIR_EB >= K) {
+#pragma unroll
+                for (int r = 0; r < RI; ++r)
+#pragma unroll
+                    for (int c = 0; c < RJ; ++c) {
+                        num64[r][c] += (double)num[r][c]; num[r][c] = T(0);
+                        if constexpr (TWO) { den64[r][c] += (double)den[r][c]; den[r][c] = T(0); }
+                    }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int r = 0; r < RI; ++r)
+#pragma unroll
+        for (int c = 0; c < RJ; ++c) {
+            const size_t i = i0 + ti * RI + r, j = j0 + tj * RJ + c;
+            if (i >= n || j >= n) continue;
+            double nm, dn = 0.0;
+            if constexpr (MIXED) { nm = num64[r][c]; if constexpr (TWO) dn = den64[r][c]; }
+            else { nm = (double)num[r][c]; dn = (double)den[r][c]; }
+            double d;
+            if (i == j) d = 0.0;
+            else if (!normalized) d = nm;
+            else if (TWO) d = nm / dn;
+            else d = nm / (S[i] + S[j]);
+            if (out_fp64) {
+                double* o = (double*)out_v;
+                o[i * n + j] = d; o[j * n + i] = d;
+            } else {
+                float* o = (float*)out_v;
+                o[i * n + j] = (float)d; o[j * n + i] = (float)d;
+            }
+        }
+}
+
+template <class TIn>
+__global__ void __launch_bounds__(256) k_matrix_to_stripes(double* __restrict__ st, const TIn* __restrict__ M,
+                                                           uint32_t n, uint32_t start) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t s = start + blockIdx.y;
+    if (k >= n) return;
+    const uint32_t l = (uint32_t)(((uint64_t)k + s + 1) % n);
+    st[(size_t)blockIdx.y * n + k] = (double)M[(size_t)k * n + l];
+}
+
+template <PairOp OP, class T, int RI, int RJ>
+hipError_t launch_pairs_t(const PairArgs& a, hipStream_t st) {
+    const uint32_t tiles = (a.n + PAIR_TILE - 1) / PAIR_TILE;
+    dim3 grid(tiles, tiles);
+    constexpr int NT = (PAIR_TILE / RI) * (PAIR_TILE / RJ);
+    hipLaunchKernelGGL((k_pairs<OP, T, RI, RJ>), grid, dim3(NT), 0, st, (const T*)a.U, (const T*)a.Cn, a.rows, a.ldn,
+                       (const T*)a.len, a.K, a.n, a.S, a.totals, a.alpha, (int)a.normalized, a.out, (int)a.out_fp64);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_pairs_prec(PairOp op, const PairArgs& a, hipStream_t st) {
+    switch (op) {
+    case PairOp::weighted: return launch_pairs_t<PairOp::weighted, T, 8, 8>(a, st);
+    case PairOp::unweighted_plain: return launch_pairs_t<PairOp::unweighted_plain, T, 4, 8>(a, st);
+    case PairOp::generalized: return launch_pairs_t<PairOp::generalized, T, 4, 8>(a, st);
+    case PairOp::vaw_weighted: return launch_pairs_t<PairOp::vaw_weighted, T, 4, 8>(a, st);
+    case PairOp::vaw_generalized: return launch_pairs_t<PairOp::vaw_generalized, T, 4, 8>(a, st);
+    case PairOp::vaw_unweighted: return launch_pairs_t<PairOp::vaw_unweighted, T, 4, 8>(a, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+
+hipError_t launch_embed_leaves(double* E, size_t ldn, const int32_t* leaf_k, const int32_t* leaf_row, uint32_t n_leaves,
+                               const uint32_t* indptr, const uint32_t* indices, const double* data,
+                               const double* totals, bool normalize, hipStream_t st) {
+    if (!n_leaves) return hipSuccess;
+    hipLaunchKernelGGL(k_embed_leaves, dim3(n_leaves), dim3(EMB_THREADS), 0, st, E, ldn, leaf_k, leaf_row, indptr,
+                       indices, data, totals, (int)normalize);
+    return hipGetLastError();
+}
+
+hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint32_t n_nodes, const int32_t* child_ptr,
+                              const int32_t* child_idx, hipStream_t st) {
+    if (!n_nodes) return hipSuccess;
+    const uint32_t cb = grid1(ldn, EMB_THREADS);
+    hipLaunchKernelGGL(k_embed_level, dim3(n_nodes * cb), dim3(EMB_THREADS), 0, st, E, ldn, nodes, cb, child_ptr,
+                       child_idx);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_f32(float* U, const double* E, size_t ldn, const int32_t* knodes, uint32_t K, hipStream_t st) {
+    if (!K) return hipSuccess;
+    const uint32_t cb = grid1(ldn, EMB_THREADS);
+    hipLaunchKernelGGL(k_gather_f32, dim3(K * cb), dim3(EMB_THREADS), 0, st, U, E, ldn, knodes, cb);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_sums(double* S, const double* E, size_t ldn, const int32_t* knodes, const double* klen,
+                              uint32_t K, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_sample_sums, dim3(grid1(n, EMB_THREADS)), dim3(EMB_THREADS), 0, st, S, E, ldn, knodes, klen,
+                       K, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_faith(double* out, const double* E, size_t ldn, const double* len, uint32_t m, uint32_t n,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_faith, dim3(grid1(n, EMB_THREADS)), dim3(EMB_THREADS), 0, st, out, E, ldn, len, m, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_presence_T(int8_t* Pt, size_t ldp, const double* E, size_t ldn, const int32_t* col_node,
+                             uint32_t Kp, uint32_t n_pad, uint32_t n, hipStream_t st) {
+    if (Kp % 64 || n_pad % 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_presence_T, dim3(Kp / 64, n_pad / 64), dim3(256), 0, st, Pt, ldp, E, ldn, col_node, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gram_i8(double* G, const int8_t* Pt, size_t ldp, const int8_t* digits, const GramChunk* chunks,
+                          uint32_t n_chunks, uint32_t n_pad, hipStream_t st) {
+    if (n_pad % GRAM_BM || ldp % GRAM_BK) return hipErrorInvalidValue;
+    const uint32_t tiles = n_pad / GRAM_BM;
+    hipLaunchKernelGGL(k_gram_i8, dim3(tiles, tiles), dim3(GRAM_THREADS), 0, st, G, tiles, Pt, ldp, digits, chunks, n_chunks,
+                       (uint32_t)(ldp / GRAM_BK));
+    return hipGetLastError();
+}
+
+template <class TOut>
+hipError_t launch_unweighted_combine(TOut* out, const double* G, uint32_t n_pad, uint32_t n, bool normalized,
+                                     hipStream_t st) {
+    hipLaunchKernelGGL((k_unweighted_combine<TOut>), dim3(grid1(n, 64), grid1(n, 4)), dim3(256), 0, st, out, G,
+                       n_pad / GRAM_BM, n, (int)normalized);
+    return hipGetLastError();
+}
+template hipError_t launch_unweighted_combine<float>(float*, const double*, uint32_t, uint32_t, bool, hipStream_t);
+template hipError_t launch_unweighted_combine<double>(double*, const double*, uint32_t, uint32_t, bool, hipStream_t);
+
+hipError_t launch_pairs(PairOp op, bool fp64, const PairArgs& a, hipStream_t st) {
+    return fp64 ? launch_pairs_prec<double>(op, a, st) : launch_pairs_prec<float>(op, a, st);
+}
+
+template <class TIn>
+hipError_t launch_matrix_to_stripes(double* stripes, const TIn* M, uint32_t n, uint32_t start, uint32_t stop,
+                                    hipStream_t st) {
+    if (stop <= start) return hipSuccess;
+    hipLaunchKernelGGL((k_matrix_to_stripes<TIn>), dim3(grid1(n, 256), stop - start), dim3(256), 0, st, stripes, M, n,
+                       start);
+    return hipGetLastError();
+}
+template hipError_t launch_matrix_to_stripes<float>(double*, const float*, uint32_t, uint32_t, uint32_t, hipStream_t);
+template hipError_t launch_matrix_to_stripes<double>(double*, const double*, uint32_t, uint32_t, uint32_t, hipStream_t);
+
+}  // namespace ssu

@@ -1,0 +1,87 @@
+// Launchers for the gfx950 kernels.  Plain pointers and sizes only; every
+// launcher enqueues on the given stream and returns the launch status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace ssu {
+
+// ---- K-chunk descriptor of the exact integer Gram (unweighted) -----------
+// A chunk is a run of K' columns of the presence matrix whose branch lengths
+// share one power-of-two scale.  Column c of the chunk carries 7-bit digits
+// d_0..d_3 of the integer mantissa for digit group g:
+//   length = scale * sum_q d_q * 2^(7q)    (this group's share)
+struct GramChunk {
+    uint32_t k0, k1;   // [k0, k1) in padded K' columns, multiples of GRAM_BK
+    double scale;      // 2^e of digit 0 of this group
+};
+
+constexpr int GRAM_BM = 128;   // output tile rows per workgroup
+constexpr int GRAM_BK = 128;   // K' columns per LDS stage
+constexpr int PAIR_TILE = 128; // samples per side of a VALU pair tile
+
+// ---------------------------------------------------------------- embedding
+hipError_t launch_embed_leaves(double* E, size_t ldn, const int32_t* leaf_k, const int32_t* leaf_row, uint32_t n_leaves,
+                               const uint32_t* indptr, const uint32_t* indices, const double* data,
+                               const double* totals, bool normalize, hipStream_t st);
+hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint32_t n_nodes, const int32_t* child_ptr,
+                              const int32_t* child_idx, hipStream_t st);
+
+// node-major FP64 -> FP32 copy of the processed rows (row r <- E[knodes[r]])
+hipError_t launch_gather_f32(float* U, const double* E, size_t ldn, const int32_t* knodes, uint32_t K, hipStream_t st);
+// per-sample S_s = sum_k len_k * E[knodes[k]][s] (FP64, sequential in k)
+hipError_t launch_sample_sums(double* S, const double* E, size_t ldn, const int32_t* knodes, const double* klen,
+                              uint32_t K, uint32_t n, hipStream_t st);
+// Faith PD: out[s] = sum over all non-root nodes k in postorder of (E[k][s] > 0) * len[k]
+hipError_t launch_faith(double* out, const double* E, size_t ldn, const double* len, uint32_t m, uint32_t n,
+                        hipStream_t st);
+
+// ---------------------------------------------------------------- Gram path
+// Pt[s][c] = 0xFF if E[col_node[c]][s] > 0 else 0 (col_node < 0 -> padding column)
+hipError_t launch_presence_T(int8_t* Pt, size_t ldp, const double* E, size_t ldn, const int32_t* col_node,
+                             uint32_t Kp, uint32_t n_pad, uint32_t n, hipStream_t st);
+// G = sum_c len_c * P[i][c] * P[j][c] for the upper-triangle 128x128 tiles, FP64,
+// stored tile by tile in MFMA fragment order (n_pad/128)^2 * 128^2 doubles
+hipError_t launch_gram_i8(double* G, const int8_t* Pt, size_t ldp, const int8_t* digits, const GramChunk* chunks,
+                          uint32_t n_chunks, uint32_t n_pad, hipStream_t st);
+// out[i][j] from G and T = diag(G): (T_i+T_j-2G)/(T_i+T_j-G) or T_i+T_j-2G
+template <class TOut>
+hipError_t launch_unweighted_combine(TOut* out, const double* G, uint32_t n_pad, uint32_t n, bool normalized,
+                                     hipStream_t st);
+
+// ---------------------------------------------------------------- VALU pair tiles
+enum class PairOp : int {
+    weighted = 0,        // sum l |u - v|
+    unweighted_plain,    // sum l [u>0 xor v>0], sum l [u>0 or v>0]  (FP32 stripe baseline)
+    generalized,         // sum l (u+v)^a |u-v|/(u+v), sum l (u+v)^a
+    vaw_weighted,        // sum l |u-v| / vaw, sum l (u+v) / vaw
+    vaw_generalized,
+    vaw_unweighted,      // sum l xor / vaw, sum l or / vaw
+};
+
+struct PairArgs {
+    const void* U;          // node rows, float or double; row of pair-node k is rows[k] (or k)
+    const void* Cn;         // raw-count rows, same indexing (VAW only)
+    const int32_t* rows;    // nullptr: identity
+    size_t ldn;
+    const void* len;        // [K] lengths, float or double
+    uint32_t K;
+    uint32_t n;
+    const double* S;        // per-sample denominators for weighted_normalized (nullptr otherwise)
+    const double* totals;   // per-sample raw totals (VAW)
+    double alpha;
+    bool normalized;        // divide by the accumulated / S-based total
+    void* out;              // n x n output
+    bool out_fp64;
+};
+hipError_t launch_pairs(PairOp op, bool fp64, const PairArgs& a, hipStream_t st);
+
+// --------------------------------------------------------- output layouts
+// stripes[s - start][k] = M[k][(k+s+1) % n]   (double output)
+template <class TIn>
+hipError_t launch_matrix_to_stripes(double* stripes, const TIn* M, uint32_t n, uint32_t start, uint32_t stop,
+                                    hipStream_t st);
+
+}  // namespace ssu

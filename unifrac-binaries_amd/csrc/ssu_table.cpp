@@ -1,0 +1,85 @@
+#include "ssu_table.hpp"
+
+namespace ssu {
+
+void Table::finish() {
+    // src/biom_inmem.cpp:513-526: totals accumulate in observation order
+    totals.assign(n_samples, 0.0);
+    for (uint32_t i = 0; i < n_obs; ++i)
+        for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j) totals[indices[j]] += data[j];
+    obs_index.clear();
+    obs_index.reserve(n_obs * 2 + 1);
+    for (uint32_t i = 0; i < n_obs; ++i) obs_index[obs_ids[i]] = i;
+}
+
+Table Table::borrow(const char* const* oids, const char* const* sids, const uint32_t* ip, const uint32_t* ix,
+                    const double* d, uint32_t n_obs, uint32_t n_samples) {
+    Table t;
+    t.n_obs = n_obs;
+    t.n_samples = n_samples;
+    t.obs_ids.reserve(n_obs);
+    for (uint32_t i = 0; i < n_obs; ++i) t.obs_ids.emplace_back(oids[i]);
+    t.sample_ids.reserve(n_samples);
+    for (uint32_t i = 0; i < n_samples; ++i) t.sample_ids.emplace_back(sids[i]);
+    t.indptr = ip;
+    t.indices = ix;
+    t.data = d;
+    t.finish();
+    return t;
+}
+
+Table Table::own(std::vector<std::string> oids, std::vector<std::string> sids, std::vector<uint32_t> ip,
+                 std::vector<uint32_t> ix, std::vector<double> d) {
+    Table t;
+    t.n_obs = (uint32_t)oids.size();
+    t.n_samples = (uint32_t)sids.size();
+    t.obs_ids = std::move(oids);
+    t.sample_ids = std::move(sids);
+    t.own_indptr_ = std::move(ip);
+    t.own_indices_ = std::move(ix);
+    t.own_data_ = std::move(d);
+    if (t.own_indptr_.empty()) t.own_indptr_.push_back(0);
+    t.indptr = t.own_indptr_.data();
+    t.indices = t.own_indices_.data();
+    t.data = t.own_data_.data();
+    t.finish();
+    return t;
+}
+
+Table Table::dense_pair(const char* const* oids, const double* s1, const double* s2, uint32_t n_obs) {
+    std::vector<std::string> o, s{"Sample1", "Sample2"};
+    std::vector<uint32_t> ip{0}, ix;
+    std::vector<double> d;
+    const double* cols[2] = {s1, s2};
+    for (uint32_t i = 0; i < n_obs; ++i) {
+        o.emplace_back(oids[i]);
+        for (uint32_t j = 0; j < 2; ++j)
+            if (cols[j][i] > 0.0) { ix.push_back(j); d.push_back(cols[j][i]); }
+        ip.push_back((uint32_t)ix.size());
+    }
+    return own(std::move(o), std::move(s), std::move(ip), std::move(ix), std::move(d));
+}
+
+Table Table::filtered(double min_total) const {
+    std::vector<int64_t> new_index(n_samples, -1);
+    std::vector<std::string> sids;
+    for (uint32_t s = 0; s < n_samples; ++s)
+        if (totals[s] >= min_total) { new_index[s] = (int64_t)sids.size(); sids.push_back(sample_ids[s]); }
+    std::vector<std::string> oids;
+    std::vector<uint32_t> ip{0}, ix;
+    std::vector<double> d;
+    if (!sids.empty()) {
+        for (uint32_t i = 0; i < n_obs; ++i) {
+            const size_t before = ix.size();
+            for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j) {
+                const int64_t ni = new_index[indices[j]];
+                if (ni >= 0) { ix.push_back((uint32_t)ni); d.push_back(data[j]); }
+            }
+            if (ix.size() > before) { oids.push_back(obs_ids[i]); ip.push_back((uint32_t)ix.size()); }
+        }
+    }
+    if (oids.empty()) sids.clear();  // degenerate: nothing left (src/biom_inmem.cpp:332-337)
+    return own(std::move(oids), std::move(sids), std::move(ip), std::move(ix), std::move(d));
+}
+
+}  // namespace ssu

@@ -1,0 +1,271 @@
+#include "ssu_tree.hpp"
+
+#include <cctype>
+#include <cstdlib>
+#include <cstring>
+
+namespace ssu {
+
+namespace {
+
+bool structure_char(char c) { return c == '(' || c == ')' || c == ',' || c == ';'; }
+
+// Parentheses from newick text, src/tree.cpp:316-363.  Text inside quotes is
+// skipped.  "(x)" and a last child after ',' each add a tip "10" ahead of the
+// closing ")"; a ")" that follows ")" only closes.
+std::vector<uint8_t> newick_parens(const char* nw) {
+    std::vector<uint8_t> s;
+    s.reserve(strlen(nw) / 2 + 16);
+    char last = '\0';
+    bool maybe_single = false, quoted = false;
+    for (const char* p = nw; *p; ++p) {
+        const char c = *p;
+        if (c == '\'') quoted = !quoted;
+        if (quoted) continue;
+        switch (c) {
+        case '(':
+            s.push_back(1);
+            last = c;
+            maybe_single = true;
+            break;
+        case ')':
+            if (maybe_single || last == ',') {
+                s.push_back(1); s.push_back(0); s.push_back(0);
+                maybe_single = false;
+            } else {
+                s.push_back(0);
+            }
+            last = c;
+            break;
+        case ',':
+            if (last != ')') { s.push_back(1); s.push_back(0); }
+            maybe_single = false;
+            last = c;
+            break;
+        default:
+            break;
+        }
+    }
+    return s;
+}
+
+// Next newick token, src/tree.cpp:455-490.  Newlines vanish, quote marks are
+// dropped, and a structure character that ends a non-empty token is eaten
+// together with it.
+std::string next_token(const char*& cur, const char* end) {
+    std::string tok;
+    bool quoted = false;
+    do {
+        const char c = *cur++;
+        if (c == '\n') continue;
+        const bool q = (c == '\'');
+        if (quoted && q) { quoted = false; continue; }
+        if (!quoted && q) { quoted = true; continue; }
+        if (structure_char(c) && !quoted) {
+            if (tok.empty()) tok.push_back(c);
+            break;
+        }
+        tok.push_back(c);
+    } while (cur != end);
+    return tok;
+}
+
+// Branch lengths go through a float parse, as std::stof does in src/tree.cpp:440-449.
+double parse_length(const std::string& s) { return (double)std::strtof(s.c_str(), nullptr); }
+
+}  // namespace
+
+void PhyloTree::index() {
+    const uint32_t n = nparens();
+    partner.assign(n, 0);
+    parent_open.assign(n, -1);
+    std::vector<uint32_t> stack;
+    stack.reserve(64);
+    valid_ = true;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (open[i]) {
+            parent_open[i] = stack.empty() ? -1 : (int32_t)stack.back();
+            stack.push_back(i);
+        } else {
+            if (stack.empty()) { valid_ = false; return; }
+            const uint32_t o = stack.back();
+            stack.pop_back();
+            partner[i] = o;
+            partner[o] = i;
+            parent_open[i] = parent_open[o];
+        }
+    }
+    if (!stack.empty()) valid_ = false;
+}
+
+PhyloTree PhyloTree::from_newick(const char* nw) {
+    PhyloTree t;
+    t.open = newick_parens(nw);
+    const uint32_t n = t.nparens();
+    t.length.assign(n, 0.0);
+    t.name.assign(n, std::string());
+    t.index();
+    if (!t.valid_) return t;
+
+    // Token -> node assignment, src/tree.cpp:384-425.  structure_idx walks the
+    // parentheses; "lag" re-synchronises after ")" runs whose characters the
+    // tokenizer swallowed.
+    const char* cur = nw;
+    const char* end = nw + strlen(nw);
+    while (end != cur && std::isspace((unsigned char)end[-1])) --end;
+    char last = '\0';
+    uint32_t idx = 0, lag = 0;
+    while (cur != end) {
+        std::string tok = next_token(cur, end);
+        if (tok.size() == 1 && structure_char(tok[0])) {
+            if (tok[0] == '(') {
+                ++idx;
+            } else if (tok[0] == ')' || tok[0] == ',') {
+                ++idx;
+                if (last == ')') ++lag;
+            }
+        } else {
+            idx += lag;
+            lag = 0;
+            if (idx < n) {
+                const uint32_t o = t.open_of(idx);
+                // src/tree.cpp:432-448: name before the last ':', float length after it
+                const size_t colon = tok.find_last_of(':');
+                if (colon == 0) {
+                    t.length[o] = parse_length(tok.substr(1));
+                    t.name[o].clear();
+                } else if (colon != std::string::npos) {
+                    t.name[o] = tok.substr(0, colon);
+                    t.length[o] = parse_length(tok.substr(colon + 1));
+                } else {
+                    t.name[o] = tok;
+                    t.length[o] = 0.0;
+                }
+                idx += t.is_leaf(o) ? 2 : 1;
+            }
+        }
+        last = tok.empty() ? '\0' : tok[0];
+    }
+    return t;
+}
+
+PhyloTree PhyloTree::from_arrays(const bool* structure, const double* lengths, const char* const* names, int n) {
+    PhyloTree t;
+    t.open.resize(n > 0 ? n : 0);
+    t.length.resize(t.open.size());
+    t.name.resize(t.open.size());
+    for (int i = 0; i < n; ++i) {
+        t.open[i] = structure[i] ? 1 : 0;
+        t.length[i] = lengths[i];
+        t.name[i] = names[i] ? names[i] : "";
+    }
+    t.index();
+    return t;
+}
+
+PhyloTree PhyloTree::from_vectors(std::vector<uint8_t> o, std::vector<double> l, std::vector<std::string> nm) {
+    PhyloTree t;
+    t.open = std::move(o);
+    t.length = std::move(l);
+    t.name = std::move(nm);
+    t.index();
+    return t;
+}
+
+std::unordered_set<std::string> PhyloTree::tip_names() const {
+    std::unordered_set<std::string> s;
+    for (uint32_t i = 0; i < nparens(); ++i)
+        if (is_leaf(i)) s.insert(name[i]);
+    return s;
+}
+
+size_t PhyloTree::n_tips() const { return tip_names().size(); }
+
+// src/tree.cpp:126-143 (mask)
+PhyloTree PhyloTree::masked(const std::vector<uint8_t>& keep, const std::vector<double>& lengths) const {
+    std::vector<uint8_t> o;
+    std::vector<double> l;
+    std::vector<std::string> nm;
+    size_t c = 0;
+    for (uint8_t k : keep) c += k;
+    o.reserve(c); l.reserve(c); nm.reserve(c);
+    for (uint32_t i = 0; i < nparens(); ++i)
+        if (keep[i]) { o.push_back(open[i]); l.push_back(lengths[i]); nm.push_back(name[i]); }
+    return from_vectors(std::move(o), std::move(l), std::move(nm));
+}
+
+PhyloTree PhyloTree::shear(const std::unordered_set<std::string>& keep) const {
+    std::vector<uint8_t> mask(nparens(), 0);
+    for (uint32_t i = 0; i < nparens(); ++i) {
+        if (is_leaf(i) && keep.count(name[i])) {
+            mask[i] = 1;
+            mask[i + 1] = 1;
+            int32_t p = parent_open[i];
+            while (p != -1 && !mask[p]) {
+                mask[p] = 1;
+                mask[partner[p]] = 1;
+                p = parent_open[p];
+            }
+        }
+    }
+    return masked(mask, length);
+}
+
+PhyloTree PhyloTree::collapse() const {
+    std::vector<uint8_t> mask(nparens(), 0);
+    std::vector<double> nl(length);
+    for (uint32_t cur = 0; cur < nparens(); ++cur) {
+        if (!open[cur]) continue;  // preorder visits open positions left to right
+        if (is_leaf(cur) || cur == 0) {
+            mask[cur] = 1;
+            mask[partner[cur]] = 1;
+        } else {
+            const uint32_t first = cur + 1;
+            const uint32_t last = open_of(partner[cur] - 1);
+            if (first == last) {
+                nl[first] = nl[first] + nl[cur];
+            } else {
+                mask[cur] = 1;
+                mask[partner[cur]] = 1;
+            }
+        }
+    }
+    return masked(mask, nl);
+}
+
+PostorderNodes postorder_nodes(const PhyloTree& t, const std::unordered_map<std::string, uint32_t>& obs_index) {
+    PostorderNodes p;
+    const uint32_t n = t.nparens();
+    const uint32_t total = n / 2;
+    p.m = total > 0 ? total - 1 : 0;
+    std::vector<int32_t> post_of_open(n, -1);
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (!t.open[i]) post_of_open[t.partner[i]] = (int32_t)k++;
+    p.parent.assign(p.m, (int32_t)p.m);
+    p.length.assign(p.m, 0.0);
+    p.is_leaf.assign(p.m, 0);
+    p.obs_row.assign(p.m, -1);
+    p.height.assign(p.m, 0);
+    std::vector<int32_t> h(total, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (t.open[i]) continue;
+        const uint32_t o = t.partner[i];
+        const uint32_t kk = (uint32_t)post_of_open[o];
+        if (kk >= p.m) continue;  // root
+        const int32_t po = t.parent_open[o];
+        const int32_t pk = po >= 0 ? post_of_open[po] : (int32_t)p.m;
+        p.parent[kk] = pk;
+        p.length[kk] = t.length[o];
+        p.is_leaf[kk] = t.is_leaf(o) ? 1 : 0;
+        if (p.is_leaf[kk]) {
+            auto it = obs_index.find(t.name[o]);
+            if (it != obs_index.end()) p.obs_row[kk] = (int32_t)it->second;
+        }
+        p.height[kk] = h[kk];
+        if (pk >= 0 && (uint32_t)pk < total && h[pk] < h[kk] + 1) h[pk] = h[kk] + 1;
+    }
+    return p;
+}
+
+}  // namespace ssu

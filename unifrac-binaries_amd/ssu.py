@@ -1,0 +1,276 @@
+"""ctypes binding of libssu.so (include/ssu.h).
+
+Mirrors the reference C API (biocore/unifrac-binaries src/api.hpp) closely
+enough that tests read like the reference's own C tests: the same entry point
+names, argument meanings and status codes.  Loading fails loudly when the
+library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libssu.so")
+
+# src/status_enum.hpp:4-6
+OKAY, TREE_MISSING, TABLE_MISSING, TABLE_EMPTY, UNKNOWN_METHOD, TABLE_AND_TREE_DO_NOT_OVERLAP, OUTPUT_ERROR, \
+    INVALID_METHOD, GROUPING_MISSING = range(9)
+READ_OKAY, WRITE_OKAY, OPEN_ERROR, READ_ERROR, MAGIC_INCOMPATIBLE, BAD_HEADER, UNEXPECTED_END, WRITE_ERROR = range(8)
+
+
+class SupportBiom(C.Structure):  # src/api.hpp:145-154
+    _fields_ = [("obs_ids", C.POINTER(C.c_char_p)), ("sample_ids", C.POINTER(C.c_char_p)),
+                ("indices", C.POINTER(C.c_uint32)), ("indptr", C.POINTER(C.c_uint32)),
+                ("data", C.POINTER(C.c_double)), ("n_obs", C.c_int), ("n_samples", C.c_int), ("nnz", C.c_int)]
+
+
+class SupportBPTree(C.Structure):  # src/api.hpp:163-168
+    _fields_ = [("structure", C.POINTER(C.c_bool)), ("lengths", C.POINTER(C.c_double)),
+                ("names", C.POINTER(C.c_char_p)), ("n_parens", C.c_int)]
+
+
+class MatFullFP64(C.Structure):  # src/api.hpp:54-59
+    _fields_ = [("n_samples", C.c_uint32), ("flags", C.c_uint32), ("matrix", C.POINTER(C.c_double)),
+                ("sample_ids", C.POINTER(C.c_char_p))]
+
+
+class MatFullFP32(C.Structure):  # src/api.hpp:67-72
+    _fields_ = [("n_samples", C.c_uint32), ("flags", C.c_uint32), ("matrix", C.POINTER(C.c_float)),
+                ("sample_ids", C.POINTER(C.c_char_p))]
+
+
+class Mat(C.Structure):  # src/api.hpp:40-46
+    _fields_ = [("n_samples", C.c_uint), ("cf_size", C.c_uint), ("is_upper_triangle", C.c_bool),
+                ("condensed_form", C.POINTER(C.c_double)), ("sample_ids", C.POINTER(C.c_char_p))]
+
+
+class RVec(C.Structure):  # src/api.hpp:82-86
+    _fields_ = [("n_samples", C.c_uint), ("values", C.POINTER(C.c_double)), ("sample_ids", C.POINTER(C.c_char_p))]
+
+
+class PartialMat(C.Structure):  # src/api.hpp:99-107
+    _fields_ = [("n_samples", C.c_uint32), ("sample_ids", C.POINTER(C.c_char_p)),
+                ("stripes", C.POINTER(C.POINTER(C.c_double))), ("stripe_start", C.c_uint32),
+                ("stripe_stop", C.c_uint32), ("stripe_total", C.c_uint32), ("is_upper_triangle", C.c_bool)]
+
+
+# every symbol include/ssu.h declares
+EXPORTS = [
+    "ssu_set_random_seed", "read_bptree_opaque", "load_bptree_opaque", "convert_bptree_opaque",
+    "destroy_bptree_opaque", "get_bptree_opaque_els", "destroy_mat", "destroy_mat_full_fp64",
+    "destroy_mat_full_fp32", "destroy_partial_mat", "destroy_partial_dyn_mat", "destroy_results_vec",
+    "one_off_v3", "one_off_wtree_v3", "one_off", "one_off_wtree", "one_off_matrix_inmem_v3",
+    "one_off_matrix_inmem_v2", "one_off_inmem", "one_off_matrix_inmem_fp32_v3", "one_off_matrix_inmem_fp32_v2",
+    "one_off_inmem_fp32", "one_off_matrix_v3", "one_off_matrix_v3t", "one_off_matrix_v2", "one_off_matrix_v2t",
+    "one_off_matrix", "one_off_matrix_fp32_v3", "one_off_matrix_fp32_v3t", "one_off_matrix_fp32_v2",
+    "one_off_matrix_fp32_v2t", "one_off_matrix_fp32", "one_dense_pair_v3t", "one_dense_pair_v3",
+    "one_dense_pair_v2t", "one_dense_pair_v2", "faith_pd_one_off", "partial_v3", "partial", "write_mat",
+    "write_mat_from_matrix", "write_mat_from_matrix_fp32", "write_vec", "ssu_faith_pd_inmem",
+    "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
+    "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
+    "ssu_newick_to_bp", "ssu_device_available", "ssu_last_error",
+]
+
+_lib = None
+
+
+def lib():
+    """The loaded libssu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name in EXPORTS:
+            getattr(L, name)  # fail loudly on a missing export
+        status_fns = [n for n in EXPORTS if n.startswith(("one_off", "one_dense", "faith", "partial", "ssu_faith",
+                                                          "ssu_plan_create", "ssu_plan_execute", "ssu_plan_copy"))]
+        for n in status_fns:
+            getattr(L, n).restype = C.c_int
+        L.ssu_plan_pair_nodes.restype = C.c_uint64
+        L.ssu_plan_main_kernel.restype = C.c_char_p
+        L.ssu_last_error.restype = C.c_char_p
+        L.ssu_postorder.restype = C.c_int
+        L.ssu_newick_to_bp.restype = C.c_int
+        L.get_bptree_opaque_els.restype = C.c_int
+        L.read_bptree_opaque.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def device_available() -> bool:
+    return bool(lib().ssu_device_available())
+
+
+def last_error() -> str:
+    return lib().ssu_last_error().decode()
+
+
+def _cstrs(strs):
+    keep = [s.encode() if isinstance(s, str) else s for s in strs]
+    arr = (C.c_char_p * max(1, len(keep)))(*keep)
+    return arr, keep
+
+
+class Inputs:
+    """support_biom_t + support_bptree_t built from numpy arrays (kept alive here)."""
+
+    def __init__(self, structure, lengths, names, obs_ids, sample_ids, indptr, indices, data):
+        self.structure = np.ascontiguousarray(structure, dtype=np.bool_)
+        self.lengths = np.ascontiguousarray(lengths, dtype=np.float64)
+        self.names_c, self._k1 = _cstrs(list(names))
+        self.obs_c, self._k2 = _cstrs(list(obs_ids))
+        self.samp_c, self._k3 = _cstrs(list(sample_ids))
+        self.sample_ids = list(sample_ids)
+        self.obs_ids = list(obs_ids)
+        self.indptr = np.ascontiguousarray(indptr, dtype=np.uint32)
+        self.indices = np.ascontiguousarray(indices, dtype=np.uint32)
+        self.data = np.ascontiguousarray(data, dtype=np.float64)
+        self.tree = SupportBPTree(self.structure.ctypes.data_as(C.POINTER(C.c_bool)),
+                                  self.lengths.ctypes.data_as(C.POINTER(C.c_double)), self.names_c,
+                                  int(self.structure.size))
+        self.table = SupportBiom(self.obs_c, self.samp_c, self.indices.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                 self.indptr.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                 self.data.ctypes.data_as(C.POINTER(C.c_double)), len(self.obs_ids),
+                                 len(self.sample_ids), int(self.data.size))
+
+    @property
+    def n_samples(self):
+        return len(self.sample_ids)
+
+
+def matrix_inmem(inp: Inputs, method: str, variance_adjust=False, alpha=1.0, bypass_tips=False,
+                 normalize=True, fp32=None, subsample_depth=0, with_replacement=True, mmap_dir=None):
+    """one_off_matrix_inmem_v3 / _fp32_v3 -> (status, n x n ndarray or None)."""
+    L = lib()
+    if fp32 is None:
+        fp32 = not method.endswith("_fp64")
+    md = mmap_dir.encode() if mmap_dir else None
+    if fp32:
+        res = C.POINTER(MatFullFP32)()
+        st = L.one_off_matrix_inmem_fp32_v3(C.byref(inp.table), C.byref(inp.tree), method.encode(),
+                                            C.c_bool(variance_adjust), C.c_double(alpha), C.c_bool(bypass_tips),
+                                            C.c_bool(normalize), C.c_uint(1), C.c_uint(subsample_depth),
+                                            C.c_bool(with_replacement), md, C.byref(res))
+        if st != OKAY:
+            return st, None
+        n = res.contents.n_samples
+        m = np.ctypeslib.as_array(res.contents.matrix, shape=(n * n,)).reshape(n, n).copy()
+        L.destroy_mat_full_fp32(C.byref(res))
+    else:
+        res = C.POINTER(MatFullFP64)()
+        st = L.one_off_matrix_inmem_v3(C.byref(inp.table), C.byref(inp.tree), method.encode(),
+                                       C.c_bool(variance_adjust), C.c_double(alpha), C.c_bool(bypass_tips),
+                                       C.c_bool(normalize), C.c_uint(1), C.c_uint(subsample_depth),
+                                       C.c_bool(with_replacement), md, C.byref(res))
+        if st != OKAY:
+            return st, None
+        n = res.contents.n_samples
+        m = np.ctypeslib.as_array(res.contents.matrix, shape=(n * n,)).reshape(n, n).copy()
+        L.destroy_mat_full_fp64(C.byref(res))
+    return st, m
+
+
+def faith_pd_inmem(inp: Inputs):
+    L = lib()
+    res = C.POINTER(RVec)()
+    st = L.ssu_faith_pd_inmem(C.byref(inp.table), C.byref(inp.tree), C.byref(res))
+    if st != OKAY:
+        return st, None, None
+    n = res.contents.n_samples
+    vals = np.ctypeslib.as_array(res.contents.values, shape=(n,)).copy()
+    ids = [res.contents.sample_ids[i].decode() for i in range(n)]
+    L.destroy_results_vec(C.byref(res))
+    return st, vals, ids
+
+
+class Plan:
+    """Device-resident plan (extension): upload once, execute, time, copy."""
+
+    def __init__(self, inp: Inputs, method: str, variance_adjust=False, alpha=1.0, bypass_tips=False,
+                 normalize=True, unweighted_kernel=0):
+        L = lib()
+        self._inp = inp
+        self.h = C.c_void_p()
+        st = L.ssu_plan_create_inmem(C.byref(inp.table), C.byref(inp.tree), method.encode(),
+                                     C.c_bool(variance_adjust), C.c_double(alpha), C.c_bool(bypass_tips),
+                                     C.c_bool(normalize), C.c_int(unweighted_kernel), C.byref(self.h))
+        if st != OKAY:
+            raise RuntimeError(f"ssu_plan_create_inmem failed: status {st}: {last_error()}")
+        self.n = inp.n_samples
+
+    def execute(self):
+        st = lib().ssu_plan_execute(self.h)
+        if st != OKAY:
+            raise RuntimeError(f"ssu_plan_execute failed: {last_error()}")
+
+    def timings(self):
+        ms = (C.c_double * 8)()
+        k = lib().ssu_plan_timings(self.h, ms, 8)
+        keys = ["embed", "prep", "main", "combine", "total"]
+        return {keys[i]: ms[i] for i in range(min(k, 5))}
+
+    def pair_nodes(self) -> int:
+        return int(lib().ssu_plan_pair_nodes(self.h))
+
+    def main_kernel(self) -> str:
+        return lib().ssu_plan_main_kernel(self.h).decode()
+
+    def matrix(self):
+        L = lib()
+        eb = L.ssu_plan_result_bytes(self.h)
+        out = np.empty((self.n, self.n), dtype=np.float64 if eb == 8 else np.float32)
+        st = L.ssu_plan_copy_matrix(self.h, out.ctypes.data_as(C.c_void_p))
+        if st != OKAY:
+            raise RuntimeError(last_error())
+        return out
+
+    def close(self):
+        if self.h:
+            lib().ssu_plan_destroy(C.byref(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def postorder(structure, lengths, names, obs_ids):
+    """Host-only sheared/collapsed postorder arrays (no device needed)."""
+    L = lib()
+    st = np.ascontiguousarray(structure, dtype=np.bool_)
+    le = np.ascontiguousarray(lengths, dtype=np.float64)
+    nm, k1 = _cstrs(list(names))
+    tr = SupportBPTree(st.ctypes.data_as(C.POINTER(C.c_bool)), le.ctypes.data_as(C.POINTER(C.c_double)), nm,
+                       int(st.size))
+    ob, k2 = _cstrs(list(obs_ids))
+    m = st.size // 2 + 1
+    parent = np.zeros(m, np.int32)
+    length = np.zeros(m)
+    leaf = np.zeros(m, np.int32)
+    row = np.zeros(m, np.int32)
+    k = L.ssu_postorder(C.byref(tr), C.c_int(len(obs_ids)), ob, parent.ctypes.data_as(C.c_void_p),
+                        length.ctypes.data_as(C.c_void_p), leaf.ctypes.data_as(C.c_void_p),
+                        row.ctypes.data_as(C.c_void_p))
+    if k < 0:
+        raise ValueError("invalid tree")
+    return parent[:k], length[:k], leaf[:k], row[:k]
+
+
+def newick_to_bp(newick: str):
+    L = lib()
+    cap = 4 * len(newick) + 16
+    s = np.zeros(cap, np.uint8)
+    le = np.zeros(cap)
+    bufs = [C.create_string_buffer(256) for _ in range(cap)]
+    arr = (C.c_char_p * cap)(*[C.cast(b, C.c_char_p) for b in bufs])
+    k = L.ssu_newick_to_bp(newick.encode(), C.c_int(cap), s.ctypes.data_as(C.c_void_p),
+                           le.ctypes.data_as(C.c_void_p), arr, C.c_int(256))
+    if k < 0:
+        raise ValueError("newick too large for buffer")
+    return s[:k].astype(bool), le[:k], [bufs[i].value.decode() for i in range(k)]
