@@ -191,6 +191,8 @@ def main():
         del m
     log(f"max rel err vs FP64 oracle: {errors}")
 
+    for p in plans.values():
+        p.close()
     plain = None
     if not args.no_plain:
         pp = ssu.Plan(inp, "unweighted", unweighted_kernel=1)
@@ -202,8 +204,6 @@ def main():
                  "gram_speedup_total": pt["total"] / kernels["unweighted"]["device_total_ms"]}
         pp.close()
         log(f"plain FP32 unweighted kernel: {pt['main']:.1f} ms main")
-    for p in plans.values():
-        p.close()
 
     cpu = None
     if not args.no_cpu_baseline:
