@@ -96,6 +96,23 @@ def cpu_baseline(tree, table, methods, budget_s):
     return kind, threads, total_time, per
 
 
+def copy_probe(nbytes):
+    """hipMemcpy device-to-device of nbytes: the runtime's copy kernel reads and writes nbytes each."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    for _ in range(2):
+        hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: {nbytes} bytes device-to-device, twice")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +124,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of reference CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 unweighted kernel")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -116,6 +135,8 @@ def main():
     import ssu
     import synth
 
+    if args.copy_probe:
+        copy_probe(1 << 30)
     if not ssu.device_available():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)

@@ -294,7 +294,9 @@ __global__ void __launch_bounds__(256) k_unweighted_combine(TOut* __restrict__ o
 
 // ------------------------------------------------------------------ VALU pair tiles
 
-template <class T> __device__ __forceinline__ T dabs(T x) { return x < T(0) ? -x : x; }
+// |x| through the IEEE fabs builtins, so the compiler folds it into a source modifier
+__device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
 
 template <class T> __device__ __forceinline__ T dsqrt(T x);
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
@@ -351,13 +353,15 @@ __device__ __forceinline__ void pair_term(T u, T v, T cu, T cv, T m, T l, T alph
     }
 }
 
-// Stage width (nodes) and how many nodes an FP32 accumulator may collect
-// before it is merged into FP64.
-constexpr int PAIR_EB = 16;
+// How many nodes an FP32 accumulator may collect before it is merged into FP64.
 constexpr int PAIR_MERGE = 64;
+template <class T> struct PairStage { static constexpr int EB = sizeof(T) == 4 ? 16 : 8; };
 
 // One workgroup per 128x128 tile of the pair matrix (upper-triangle tiles);
-// thread (ti, tj) owns rows 8*ti.. (RI rows) and columns RJ*tj.. of the tile.
+// thread (ti, tj) owns rows RI*ti.. and columns RJ*tj.. of the tile.  EB nodes
+// of both 128-sample panels (and of the count panels for VAW) are staged in
+// LDS per step; the next step's rows are prefetched into registers while the
+// current one is consumed, and LDS is double-buffered (one barrier per step).
 template <PairOp OP, class T, int RI, int RJ>
 __global__ void __launch_bounds__((PAIR_TILE / RI) * (PAIR_TILE / RJ))
 k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __restrict__ rows, size_t ldn,
@@ -368,20 +372,25 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
     constexpr bool TWO = PairTraits<OP>::two;
     constexpr bool CNT = PairTraits<OP>::counts;
     constexpr bool MIXED = sizeof(T) == 4;  // FP32 terms, FP64 merges
+    constexpr bool BIN = OP == PairOp::unweighted_plain || OP == PairOp::vaw_unweighted;
+    constexpr int EB = PairStage<T>::EB;
+    constexpr int VEC = 16 / sizeof(T);                      // elements per 16-byte load
+    constexpr int NPANEL = CNT ? 4 : 2;                      // u_i, u_j (, c_i, c_j)
+    constexpr int LOADS = NPANEL * EB * PAIR_TILE / VEC;     // 16-byte loads per step
+    static_assert(LOADS % NT == 0, "staging must divide evenly");
+    constexpr int LPT = LOADS / NT;                          // per thread
+    typedef __attribute__((ext_vector_type(VEC))) T vecT;
     const uint32_t bi = blockIdx.y, bj = blockIdx.x;
     if (bi > bj) return;
     const uint32_t t = threadIdx.x;
     const uint32_t ti = t % NTI, tj = t / NTI;
 
-    __shared__ __attribute__((aligned(16))) T su[PAIR_EB][PAIR_TILE];
-    __shared__ __attribute__((aligned(16))) T sv[PAIR_EB][PAIR_TILE];
-    __shared__ __attribute__((aligned(16))) T scu[CNT ? PAIR_EB : 1][CNT ? PAIR_TILE : 1];
-    __shared__ __attribute__((aligned(16))) T scv[CNT ? PAIR_EB : 1][CNT ? PAIR_TILE : 1];
-    __shared__ T sl[PAIR_EB];
+    __shared__ __attribute__((aligned(16))) T sp[2][NPANEL][EB][PAIR_TILE];
+    __shared__ T sl[2][EB];
 
     const T alpha = (T)alpha_d;
     const size_t i0 = (size_t)bi * PAIR_TILE, j0 = (size_t)bj * PAIR_TILE;
-    T mrow[RI][RJ];  // VAW: c_i + c_j
+    T mrow[CNT ? RI : 1][CNT ? RJ : 1];  // VAW: c_i + c_j
     if constexpr (CNT) {
 #pragma unroll
         for (int r = 0; r < RI; ++r)
@@ -392,54 +401,90 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
             }
     }
 
-    T num[RI][RJ], den[RI][RJ];
+    T num[RI][RJ], den[TWO ? RI : 1][TWO ? RJ : 1];
     double num64[MIXED ? RI : 1][MIXED ? RJ : 1], den64[MIXED && TWO ? RI : 1][MIXED && TWO ? RJ : 1];
 #pragma unroll
     for (int r = 0; r < RI; ++r)
 #pragma unroll
         for (int c = 0; c < RJ; ++c) {
-            num[r][c] = T(0); den[r][c] = T(0);
+            num[r][c] = T(0);
+            if constexpr (TWO) den[r][c] = T(0);
             if constexpr (MIXED) { num64[r][c] = 0.0; if constexpr (TWO) den64[r][c] = 0.0; }
         }
 
-    for (uint32_t k0 = 0; k0 < K; k0 += PAIR_EB) {
-        // stage PAIR_EB node rows of both sample blocks
-        for (uint32_t q = t; q < PAIR_EB * PAIR_TILE; q += NT) {
-            const uint32_t e = q / PAIR_TILE, x = q % PAIR_TILE;
+    vecT pre[LPT];
+    T pre_l = T(0);
+    // load step k0 into registers: load q -> panel, node e, 16-byte column group x
+    auto fetch = [&](uint32_t k0) __attribute__((always_inline)) {
+#pragma unroll
+        for (int q = 0; q < LPT; ++q) {
+            const uint32_t idx = t + (uint32_t)NT * q;
+            const uint32_t panel = idx / (EB * PAIR_TILE / VEC);
+            const uint32_t rem = idx % (EB * PAIR_TILE / VEC);
+            const uint32_t e = rem / (PAIR_TILE / VEC), x = (rem % (PAIR_TILE / VEC)) * VEC;
             const uint32_t k = k0 + e;
-            T a = T(0), b = T(0), ca = T(0), cb = T(0);
+            vecT v = (vecT)(T(0));
             if (k < K) {
                 const size_t rk = (size_t)(rows ? rows[k] : (int32_t)k) * ldn;
-                a = U[rk + i0 + x];
-                b = U[rk + j0 + x];
-                if constexpr (CNT) { ca = Cn[rk + i0 + x]; cb = Cn[rk + j0 + x]; }
+                const T* src = ((panel & 2) ? Cn : U) + rk + ((panel & 1) ? j0 : i0) + x;
+                v = *reinterpret_cast<const vecT*>(src);
             }
-            if constexpr (OP == PairOp::unweighted_plain || OP == PairOp::vaw_unweighted) {
-                a = a > T(0) ? T(1) : T(0);
-                b = b > T(0) ? T(1) : T(0);
-            }
-            su[e][x] = a; sv[e][x] = b;
-            if constexpr (CNT) { scu[e][x] = ca; scv[e][x] = cb; }
+            pre[q] = v;
         }
-        if (t < PAIR_EB) sl[t] = (k0 + t < K) ? len[k0 + t] : T(0);
-        __syncthreads();
+        if (t < EB) pre_l = (k0 + t < K) ? len[k0 + t] : T(0);
+    };
+    auto stash = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int q = 0; q < LPT; ++q) {
+            const uint32_t idx = t + (uint32_t)NT * q;
+            const uint32_t panel = idx / (EB * PAIR_TILE / VEC);
+            const uint32_t rem = idx % (EB * PAIR_TILE / VEC);
+            const uint32_t e = rem / (PAIR_TILE / VEC), x = (rem % (PAIR_TILE / VEC)) * VEC;
+            vecT v = pre[q];
+            if constexpr (BIN) {
+                if (panel < 2) {
+#pragma unroll
+                    for (int z = 0; z < VEC; ++z) v[z] = v[z] > T(0) ? T(1) : T(0);
+                }
+            }
+            *reinterpret_cast<vecT*>(&sp[buf][panel][e][x]) = v;
+        }
+        if (t < EB) sl[buf][t] = pre_l;
+    };
+
+    if (K) { fetch(0); stash(0); }
+    __syncthreads();
+    int buf = 0;
+    for (uint32_t k0 = 0; k0 < K; k0 += EB) {
+        const bool more = k0 + EB < K;
+        if (more) fetch(k0 + EB);
 #pragma unroll 2
-        for (int e = 0; e < PAIR_EB; ++e) {
+        for (int e = 0; e < EB; ++e) {
             T u[RI], v[RJ], cu[RI], cv[RJ];
 #pragma unroll
-            for (int r = 0; r < RI; ++r) { u[r] = su[e][ti * RI + r]; if constexpr (CNT) cu[r] = scu[e][ti * RI + r]; else cu[r] = T(0); }
+            for (int r = 0; r < RI; ++r) {
+                u[r] = sp[buf][0][e][ti * RI + r];
+                cu[r] = CNT ? sp[buf][CNT ? 2 : 0][e][ti * RI + r] : T(0);
+            }
 #pragma unroll
-            for (int c = 0; c < RJ; ++c) { v[c] = sv[e][tj * RJ + c]; if constexpr (CNT) cv[c] = scv[e][tj * RJ + c]; else cv[c] = T(0); }
-            const T l = sl[e];
+            for (int c = 0; c < RJ; ++c) {
+                v[c] = sp[buf][1][e][tj * RJ + c];
+                cv[c] = CNT ? sp[buf][CNT ? 3 : 1][e][tj * RJ + c] : T(0);
+            }
+            const T l = sl[buf][e];
+            T dummy = T(0);
 #pragma unroll
             for (int r = 0; r < RI; ++r)
 #pragma unroll
-                for (int c = 0; c < RJ; ++c)
-                    pair_term<OP, T>(u[r], v[c], cu[r], cv[c], CNT ? mrow[r][c] : T(0), l, alpha, num[r][c], den[r][c]);
+                for (int c = 0; c < RJ; ++c) {
+                    if constexpr (TWO)
+                        pair_term<OP, T>(u[r], v[c], cu[r], cv[c], CNT ? mrow[r][c] : T(0), l, alpha, num[r][c], den[r][c]);
+                    else
+                        pair_term<OP, T>(u[r], v[c], cu[r], cv[c], T(0), l, alpha, num[r][c], dummy);
+                }
         }
-        __syncthreads();
         if constexpr (MIXED) {
-            if (((k0 / PAIR_EB + 1) % (PAIR_MERGE / PAIR_EB)) == 0 || k0 + PAIR_EB >= K) {
+            if (((k0 / EB + 1) % (PAIR_MERGE / EB)) == 0 || !more) {
 #pragma unroll
                 for (int r = 0; r < RI; ++r)
 #pragma unroll
@@ -449,6 +494,9 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
                     }
             }
         }
+        if (more) stash(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
     }
 
 #pragma unroll
@@ -459,7 +507,7 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
             if (i >= n || j >= n) continue;
             double nm, dn = 0.0;
             if constexpr (MIXED) { nm = num64[r][c]; if constexpr (TWO) dn = den64[r][c]; }
-            else { nm = (double)num[r][c]; dn = (double)den[r][c]; }
+            else { nm = (double)num[r][c]; if constexpr (TWO) dn = (double)den[r][c]; }
             double d;
             if (i == j) d = 0.0;
             else if (!normalized) d = nm;
