@@ -5,16 +5,18 @@ Workload (default): random binary tree with 100,000 tips (Exp(0.1) lengths,
 FP32-rounded), 16,384 samples at 1% tip density with Zipf(1.1) tip popularity,
 lognormal counts -- the "16k samples" configuration.  One step computes the
 full 16,384 x 16,384 unweighted matrix (exact int8-MFMA Gram) and the full
-weighted_normalized matrix (FP32 VALU pair tiles with FP64 merges), both with
-the table and tree already resident in HBM and the matrices left in HBM.
+weighted_normalized matrix (shared-node min formulation: FP64 VALU over the
+FP32-rounded embedding, visiting per 64x64 block only the nodes both panels
+hold), both with the table and tree already resident in HBM and the matrices
+left in HBM.
 
 metric: pair-node updates/sec = sum over both methods of P * N / t, with
 P = n(n-1)/2 and N the nodes the method processes after shear/collapse.
 
 Prints one JSON line (rank 0, N=1).  Extra fields: per-kernel HIP-event
 timings, the max relative error of both matrices against the FP64 oracle on a
-48-sample subset, the plain-FP32 unweighted kernel for comparison, a roofline
-entry for the dominant kernel and the CPU baseline.
+48-sample subset, the plain dense pair-tile kernels (unweighted and weighted)
+for comparison, a roofline entry for the dominant kernel and the CPU baseline.
 """
 import argparse
 import json
@@ -28,7 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "unifrac-binaries_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 (vector/matrix) peak
+PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 vector peak (packed FP32)
+PEAK_FP64_TFLOPS = 78.6     # MI355X published FP64 vector peak
+FLOPS_PER_TERM_SPARSE = 3   # min, fused multiply-add (FP64)
 PEAK_INT8_TOPS = 5000.0     # MI355X published dense INT8 MFMA peak, TOP/s
 FLOPS_PER_PAIR_NODE_W = 4   # sub, abs, fused multiply-add
 OPS_PER_PAIR_NODE_GRAM = 8  # 4 int8 digit passes x one multiply-add
@@ -123,7 +127,7 @@ def main():
     ap.add_argument("--tips", type=int, default=None)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of reference CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 unweighted kernel")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain dense pair-tile kernels")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -176,7 +180,9 @@ def main():
         kernels[name] = {"kernel": p.main_kernel(), "main_ms": avg(name, "main"), "embed_ms": avg(name, "embed"),
                          "prep_ms": avg(name, "prep"), "combine_ms": avg(name, "combine"),
                          "device_total_ms": avg(name, "total"),
-                         "pair_node_per_s": pair_nodes[name] / (avg(name, "total") / 1000.0)}
+                         "pair_node_per_s": pair_nodes[name] / (avg(name, "total") / 1000.0),
+                         "kernel_terms": p.kernel_terms()}
+        kernels[name]["terms_per_pair_node"] = kernels[name]["kernel_terms"] / max(pair_nodes[name], 1)
     # dominant kernel: the one with the largest main-kernel time
     dom = max(kernels, key=lambda k: kernels[k]["main_ms"])
     dk = kernels[dom]
@@ -184,6 +190,10 @@ def main():
         achieved = OPS_PER_PAIR_NODE_GRAM * pair_nodes[dom] / (dk["main_ms"] / 1000.0) / 1e12
         roofline = {"kernel": dk["kernel"], "bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS,
                     "unit": "TOP/s", "frac": achieved / PEAK_INT8_TOPS, "traffic": None}
+    elif dk["kernel"] == "k_pairs_sparse":
+        achieved = FLOPS_PER_TERM_SPARSE * dk["kernel_terms"] / (dk["main_ms"] / 1000.0) / 1e12
+        roofline = {"kernel": dk["kernel"], "bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                    "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None}
     else:
         achieved = FLOPS_PER_PAIR_NODE_W * pair_nodes[dom] / (dk["main_ms"] / 1000.0) / 1e12
         roofline = {"kernel": dk["kernel"], "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
@@ -216,15 +226,18 @@ def main():
         p.close()
     plain = None
     if not args.no_plain:
-        pp = ssu.Plan(inp, "unweighted", unweighted_kernel=1)
-        pp.execute()
-        pp.execute()
-        pt = pp.timings()
-        plain = {"kernel": pp.main_kernel(), "main_ms": pt["main"], "device_total_ms": pt["total"],
-                 "gram_speedup_main": pt["main"] / kernels["unweighted"]["main_ms"],
-                 "gram_speedup_total": pt["total"] / kernels["unweighted"]["device_total_ms"]}
-        pp.close()
-        log(f"plain FP32 unweighted kernel: {pt['main']:.1f} ms main")
+        plain = {}
+        for name in plans:
+            pp = ssu.Plan(inp, name, plain_kernel=1)
+            pp.execute()
+            pp.execute()
+            pt = pp.timings()
+            plain[name] = {"kernel": pp.main_kernel(), "main_ms": pt["main"], "device_total_ms": pt["total"],
+                           "speedup_main": pt["main"] / kernels[name]["main_ms"],
+                           "speedup_total": pt["total"] / kernels[name]["device_total_ms"]}
+            pp.close()
+            log(f"plain dense kernel {name}: {pt['main']:.1f} ms main "
+                f"({plain[name]['speedup_main']:.1f}x slower than {kernels[name]['kernel']})")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -246,7 +259,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8 MFMA (unweighted, exact) + fp32/fp64 (weighted)",
+        "dtype": "int8 MFMA (unweighted, exact) + fp64 VALU on fp32-rounded embedding (weighted)",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: random binary tree {len(tree[0]) // 4 + 1} tips, "
                                f"{n} samples, unweighted + weighted_normalized, full n x n matrices",
@@ -254,7 +267,7 @@ def main():
                    "methods": list(plans)},
         "max_rel_err_vs_fp64": errors,
         "kernels": kernels,
-        "plain_fp32_unweighted": plain,
+        "plain_dense_kernels": plain,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -279,12 +279,15 @@ SSU_EXTERN ComputeStatus ssu_faith_pd_inmem(const support_biom_t* table_data, co
 
 /* Device-resident plan: inputs uploaded once to HBM, then executed any
  * number of times; the n x n result stays in device memory until copied.
- * For benchmarking and for callers that chain GPU work on the matrix. */
+ * For benchmarking and for callers that chain GPU work on the matrix.
+ * plain_kernel = 0 selects the specialised kernels (exact int8 MFMA Gram for
+ * unweighted, shared-node min-formulation for weighted), 1 the plain dense
+ * pair-tile kernel for every method (a cross-check and a baseline). */
 typedef struct ssu_plan ssu_plan_t;
 SSU_EXTERN ComputeStatus ssu_plan_create_inmem(const support_biom_t* table_data, const support_bptree_t* tree_data,
                                                const char* unifrac_method, bool variance_adjust, double alpha,
                                                bool bypass_tips, bool normalize_sample_counts,
-                                               int unweighted_kernel, ssu_plan_t** plan);
+                                               int plain_kernel, ssu_plan_t** plan);
 SSU_EXTERN ComputeStatus ssu_plan_execute(ssu_plan_t* plan);
 /* element size of the result: 8 (fp64 method names) or 4 */
 SSU_EXTERN int ssu_plan_result_bytes(const ssu_plan_t* plan);
@@ -292,6 +295,9 @@ SSU_EXTERN ComputeStatus ssu_plan_copy_matrix(const ssu_plan_t* plan, void* host
 /* last execute(): milliseconds per phase (embed, prep, main kernel, combine, total) from HIP events */
 SSU_EXTERN int ssu_plan_timings(const ssu_plan_t* plan, double* ms, int cap);
 SSU_EXTERN uint64_t ssu_plan_pair_nodes(const ssu_plan_t* plan);   /* n(n-1)/2 * processed nodes */
+/* (pair, node) terms the main kernel of the last execute() evaluated, padding
+ * included; the shared-node weighted kernel counts only the visited nodes */
+SSU_EXTERN uint64_t ssu_plan_kernel_terms(const ssu_plan_t* plan);
 SSU_EXTERN const char* ssu_plan_main_kernel(const ssu_plan_t* plan);
 SSU_EXTERN void ssu_plan_destroy(ssu_plan_t** plan);
 

@@ -212,12 +212,12 @@ def test_reference_vectors_on_gpu(ssu):
             assert rel_err(got, z[c + "__out"]) < 1e-4, c
 
 
-def test_plain_fp32_unweighted_kernel_matches_gram(ssu, O, synth):
+def test_plain_fp32_unweighted_matches_gram(ssu, O, synth):
     tree, table = _case(synth, 500, 300, 0.05, 61, zipf=1.1)
     inp = inputs_from(ssu, tree, table)
     say("plain vs gram")
-    p_gram = ssu.Plan(inp, "unweighted", unweighted_kernel=0)
-    p_plain = ssu.Plan(inp, "unweighted", unweighted_kernel=1)
+    p_gram = ssu.Plan(inp, "unweighted", plain_kernel=0)
+    p_plain = ssu.Plan(inp, "unweighted", plain_kernel=1)
     p_gram.execute()
     p_plain.execute()
     assert p_gram.main_kernel() == "k_gram_i8" and p_plain.main_kernel() == "k_pairs"
@@ -226,6 +226,74 @@ def test_plain_fp32_unweighted_kernel_matches_gram(ssu, O, synth):
     assert rel_err(p_plain.matrix(), ref) <= 1e-5
     p_gram.close()
     p_plain.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("method", ["weighted_normalized", "weighted_unnormalized"])
+def test_weighted_shared_node_kernel_matches_dense(ssu, O, synth, method, prec):
+    """Shared-node (min) formulation vs the dense pair-tile kernel vs the FP64 oracle;
+    n = 300 leaves a ragged last 64-sample panel."""
+    tree, table = _case(synth, 500, 300, 0.05, 71, zipf=1.1)
+    inp = inputs_from(ssu, tree, table)
+    name = method + ("_fp64" if prec == "fp64" else "")
+    say("shared-node vs dense", name)
+    ps = ssu.Plan(inp, name, plain_kernel=0)
+    pd = ssu.Plan(inp, name, plain_kernel=1)
+    ps.execute()
+    pd.execute()
+    assert ps.main_kernel() == "k_pairs_sparse" and pd.main_kernel() == "k_pairs"
+    ref = oracle_matrix(O, tree, table, method)
+    ms, md = ps.matrix(), pd.matrix()
+    assert rel_err(ms, ref) <= TOL[prec]
+    assert rel_err(md, ref) <= TOL[prec]
+    assert np.array_equal(ms, ms.T) and np.all(np.diag(ms) == 0)
+    ps.close()
+    pd.close()
+
+
+def test_weighted_duplicate_samples(ssu, O, synth):
+    """Identical samples: d = 0 exactly in FP64 arithmetic; the shared-node
+    kernel gets S_i + S_j - 2M, clamped at 0, within the absolute bound."""
+    st, le, nm = synth.random_tree(300, 91)
+    obs, sids, ip, ix, d = synth.random_table(300, 40, 0.1, 92)
+    dense = np.zeros((len(obs), 40))
+    for r in range(len(obs)):
+        dense[r, ix[ip[r]:ip[r + 1]]] = d[ip[r]:ip[r + 1]]
+    dense[:, 20:] = dense[:, :20] * 3.0      # same proportions, other totals
+    rows = [np.flatnonzero(dense[r]) for r in range(len(obs))]
+    keep = [r for r in range(len(obs)) if len(rows[r])]
+    obs2 = [obs[r] for r in keep]
+    ip2 = np.concatenate([[0], np.cumsum([len(rows[r]) for r in keep])]).astype(np.uint32)
+    ix2 = np.concatenate([rows[r] for r in keep]).astype(np.uint32)
+    d2 = np.concatenate([dense[r, rows[r]] for r in keep])
+    table = (obs2, sids, ip2, ix2, d2)
+    inp = ssu.Inputs(st, le, nm, *table)
+    for name, tol in (("weighted_normalized_fp64", 1e-11), ("weighted_normalized", 1e-6),
+                      ("weighted_unnormalized", 1e-6)):
+        say("duplicates", name)
+        s, m = ssu.matrix_inmem(inp, name)
+        assert s == ssu.OKAY
+        ref = oracle_matrix(O, (st, le, nm), table, name.replace("_fp64", ""))
+        assert rel_err(m, ref) <= tol
+        dup = np.array([m[i, i + 20] for i in range(20)])
+        assert np.all(dup >= 0) and np.max(dup) <= tol
+
+
+def test_weighted_negative_counts_use_dense_kernel(ssu, O, synth):
+    """min(u, v) only vanishes for u, v >= 0: negative counts select the dense kernel."""
+    tree, table = _case(synth, 200, 50, 0.1, 93)
+    obs, sids, ip, ix, d = table
+    d = d.copy()
+    d[::7] *= -1.0
+    table = (obs, sids, ip, ix, d)
+    inp = inputs_from(ssu, tree, table)
+    say("negative counts")
+    p = ssu.Plan(inp, "weighted_unnormalized_fp64")
+    p.execute()
+    assert p.main_kernel() == "k_pairs"
+    ref = oracle_matrix(O, tree, table, "weighted_unnormalized")
+    assert rel_err(p.matrix(), ref) <= 1e-11
+    p.close()
 
 
 def test_file_entry_points(ssu, O, ref_trees, ref_tables, goldens, tmp_path):

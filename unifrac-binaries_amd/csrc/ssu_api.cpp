@@ -844,7 +844,7 @@ struct ssu_plan {
 
 ComputeStatus ssu_plan_create_inmem(const support_biom_t* td, const support_bptree_t* tr, const char* method_name,
                                     bool variance_adjust, double alpha, bool bypass_tips, bool normalize_sample_counts,
-                                    int unweighted_kernel, ssu_plan_t** out) {
+                                    int plain_kernel, ssu_plan_t** out) {
     Method m;
     bool fp64;
     if (!parse_method(method_name, m, fp64)) return unknown_method;
@@ -856,7 +856,7 @@ ComputeStatus ssu_plan_create_inmem(const support_biom_t* td, const support_bptr
     ComputeStatus st = validate(p->table, tree);
     if (st != okay) return st;
     ComputeSpec spec = make_spec(m, fp64, variance_adjust, alpha, bypass_tips, normalize_sample_counts);
-    spec.unweighted_kernel = unweighted_kernel;
+    spec.plain_kernel = plain_kernel;
     std::string err;
     ssu::PostorderNodes nodes = synced_nodes(p->table, tree);
     p->plan = Plan::create(p->table, nodes, spec, &err);
@@ -893,6 +893,8 @@ uint64_t ssu_plan_pair_nodes(const ssu_plan_t* plan) {
     const uint64_t n = plan->plan->n_samples();
     return n * (n - 1) / 2 * plan->plan->processed_nodes();
 }
+
+uint64_t ssu_plan_kernel_terms(const ssu_plan_t* plan) { return plan ? plan->plan->kernel_terms() : 0; }
 
 const char* ssu_plan_main_kernel(const ssu_plan_t* plan) { return plan ? plan->plan->main_kernel() : ""; }
 

@@ -81,6 +81,7 @@ struct PlanImpl {
     uint32_t n = 0, m = 0;
     size_t ldn = 0;
     bool gram = false;
+    bool sparse = false;             // weighted through the shared-node kernel
     bool bad_lengths = false;
 
     // host schedule
@@ -104,7 +105,7 @@ struct PlanImpl {
     std::vector<double> totals;
 
     DevMem d_indptr, d_indices, d_data, d_totals, d_leaf_k, d_leaf_row, d_level_nodes, d_child_ptr, d_child_idx;
-    DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
+    DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
         d_faith, d_stripes;
 
     ~PlanImpl() {
@@ -260,9 +261,21 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     }
 
     const bool unw = spec.method == Method::unweighted || spec.method == Method::unweighted_unnormalized;
-    P.gram = unw && !spec.vaw && spec.unweighted_kernel == 0;
+    P.gram = unw && !spec.vaw && spec.plain_kernel == 0;
     if (P.gram && !P.build_gram_columns()) P.gram = false;  // non-finite lengths: VALU path
     P.n_pad = round_up(std::max<uint32_t>(P.n, 1), GRAM_BM);
+    // The shared-node weighted kernel needs u >= 0 (so min(u, v) vanishes when
+    // either sample lacks the node) and finite lengths (S_i + S_j - 2M).
+    const bool weighted = spec.method == Method::weighted_normalized || spec.method == Method::weighted_unnormalized;
+    P.sparse = weighted && !spec.vaw && spec.plain_kernel == 0;
+    if (P.sparse) {
+        for (size_t q = 0; q < P.klen_d.size() && P.sparse; ++q)
+            if (!std::isfinite(spec.fp64 ? P.klen_d[q] : (double)P.klen_f[q])) P.sparse = false;
+        const uint64_t nz = table.nnz();
+        const uint32_t b0 = table.n_obs ? table.indptr[0] : 0;
+        for (uint64_t q = 0; q < nz && P.sparse; ++q)
+            if (!(table.data[b0 + q] >= 0.0)) P.sparse = false;
+    }
 
     std::string e2;
     if (!err) err = &e2;
@@ -335,6 +348,43 @@ bool Plan::execute(std::string* err) {
             SSU_HIP(launch_unweighted_combine<float>(P.d_out.as<float>(), P.d_G.as<double>(), P.n_pad, P.n,
                                                      normalized, P.st));
         SSU_HIP(hipEventRecord(P.ev[4], P.st));
+    } else if (P.sparse) {
+        const uint32_t Kw = (K + 31) / 32;
+        if (!P.d_S.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err)) return false;
+        if (!P.d_Spart.ensure(std::max<uint32_t>(gather_segments(K, P.ldn), 1) * P.ldn * sizeof(double), err))
+            return false;
+        if (!P.d_masks.ensure(std::max<size_t>((P.ldn / 64) * Kw, 1) * sizeof(uint32_t), err)) return false;
+        SparseArgs a{};
+        if (s.fp64) {
+            SSU_HIP(launch_gather_sums(nullptr, P.d_S.as<double>(), P.d_Spart.as<double>(), P.d_masks.as<uint32_t>(),
+                                       P.d_E.as<double>(), P.ldn, P.d_knodes.as<int32_t>(), P.d_klen_sum.as<double>(), K,
+                                       P.n, P.st));
+            a.U = P.d_E.p;
+            a.rows = P.d_knodes.as<int32_t>();
+        } else {
+            if (!P.d_U.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
+            SSU_HIP(launch_gather_sums(P.d_U.as<float>(), P.d_S.as<double>(), P.d_Spart.as<double>(),
+                                       P.d_masks.as<uint32_t>(), P.d_E.as<double>(), P.ldn, P.d_knodes.as<int32_t>(),
+                                       P.d_klen_sum.as<double>(), K, P.n, P.st));
+            a.U = P.d_U.p;
+            a.rows = nullptr;
+        }
+        a.ldn = P.ldn;
+        a.len = P.d_klen_sum.as<double>();
+        a.masks = P.d_masks.as<uint32_t>();
+        a.Kw = Kw;
+        a.n = P.n;
+        a.S = P.d_S.as<double>();
+        a.normalized = s.method == Method::weighted_normalized;
+        a.out = P.d_out.p;
+        a.out_fp64 = s.fp64;
+        const size_t nb = (P.n + 63) / 64;
+        if (!P.d_visits.ensure(std::max<size_t>(nb * nb, 1) * sizeof(uint32_t), err)) return false;
+        a.visits = P.d_visits.as<uint32_t>();
+        SSU_HIP(hipEventRecord(P.ev[2], P.st));
+        SSU_HIP(launch_pairs_sparse(s.fp64, a, P.st));
+        SSU_HIP(hipEventRecord(P.ev[3], P.st));
+        SSU_HIP(hipEventRecord(P.ev[4], P.st));
     } else {
         PairArgs a{};
         a.n = P.n;
@@ -353,25 +403,34 @@ bool Plan::execute(std::string* err) {
         case Method::weighted_unnormalized: op = s.vaw ? PairOp::vaw_weighted : PairOp::weighted; a.normalized = false; break;
         case Method::generalized: op = s.vaw ? PairOp::vaw_generalized : PairOp::generalized; a.normalized = true; break;
         }
-        if (op == PairOp::weighted && a.normalized) {
-            // denominators: sum_k l_k (u_i + u_j) = S_i + S_j
+        // weighted_normalized denominators: sum_k l_k (u_i + u_j) = S_i + S_j
+        const bool sums = op == PairOp::weighted && a.normalized;
+        double* S = nullptr;
+        double* Spart = nullptr;
+        if (sums) {
             if (!P.d_S.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err)) return false;
-            SSU_HIP(launch_sample_sums(P.d_S.as<double>(), P.d_E.as<double>(), P.ldn, P.d_knodes.as<int32_t>(),
-                                       P.d_klen_sum.as<double>(), K, P.n, P.st));
-            a.S = P.d_S.as<double>();
+            const size_t segs = std::max<uint32_t>(gather_segments(K, P.ldn), 1);
+            if (!P.d_Spart.ensure(segs * P.ldn * sizeof(double), err)) return false;
+            S = P.d_S.as<double>();
+            Spart = P.d_Spart.as<double>();
+            a.S = S;
         }
         if (s.fp64) {
             a.U = P.d_E.p;
             a.Cn = s.vaw ? P.d_C.p : nullptr;
             a.rows = P.d_knodes.as<int32_t>();
+            if (sums)
+                SSU_HIP(launch_gather_sums(nullptr, S, Spart, nullptr, P.d_E.as<double>(), P.ldn, P.d_knodes.as<int32_t>(),
+                                           P.d_klen_sum.as<double>(), K, P.n, P.st));
         } else {
             if (!P.d_U.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
-            SSU_HIP(launch_gather_f32(P.d_U.as<float>(), P.d_E.as<double>(), P.ldn, P.d_knodes.as<int32_t>(), K, P.st));
+            SSU_HIP(launch_gather_sums(P.d_U.as<float>(), S, Spart, nullptr, P.d_E.as<double>(), P.ldn,
+                                       P.d_knodes.as<int32_t>(), P.d_klen_sum.as<double>(), K, P.n, P.st));
             a.U = P.d_U.p;
             if (s.vaw) {
                 if (!P.d_Cu.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
-                SSU_HIP(launch_gather_f32(P.d_Cu.as<float>(), P.d_C.as<double>(), P.ldn, P.d_knodes.as<int32_t>(), K,
-                                          P.st));
+                SSU_HIP(launch_gather_sums(P.d_Cu.as<float>(), nullptr, nullptr, nullptr, P.d_C.as<double>(), P.ldn,
+                                           P.d_knodes.as<int32_t>(), nullptr, K, P.n, P.st));
                 a.Cn = P.d_Cu.p;
             }
             a.rows = nullptr;
@@ -423,8 +482,30 @@ bool Plan::faith(double* host, std::string* err) {
 const float* Plan::timings() const { return p_->times; }
 uint32_t Plan::n_samples() const { return p_->n; }
 uint64_t Plan::processed_nodes() const { return p_->knodes.size(); }
+
+uint64_t Plan::kernel_terms() const {
+    const PlanImpl& P = *p_;
+    const uint64_t K = P.knodes.size();
+    if (P.gram) {
+        const uint64_t t = P.n_pad / GRAM_BM;
+        return t * (t + 1) / 2 * GRAM_BM * GRAM_BM * (uint64_t)P.Kp;
+    }
+    if (P.sparse) {
+        const size_t nb = (P.n + 63) / 64;
+        if (!nb || !P.d_visits.p) return 0;
+        std::vector<uint32_t> v(nb * nb);
+        if (hipMemcpy(v.data(), P.d_visits.p, v.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+            return 0;
+        uint64_t sum = 0;
+        for (size_t i = 0; i < nb; ++i)
+            for (size_t j = i; j < nb; ++j) sum += v[i * nb + j];
+        return sum * 64 * 64;
+    }
+    const uint64_t t = (P.n + PAIR_TILE - 1) / PAIR_TILE;
+    return t * (t + 1) / 2 * PAIR_TILE * PAIR_TILE * K;
+}
 bool Plan::out_fp64() const { return p_->spec.fp64; }
-const char* Plan::main_kernel() const { return p_->gram ? "k_gram_i8" : "k_pairs"; }
+const char* Plan::main_kernel() const { return p_->gram ? "k_gram_i8" : p_->sparse ? "k_pairs_sparse" : "k_pairs"; }
 const void* Plan::device_output() const { return p_->d_out.p; }
 
 }  // namespace ssu

@@ -27,7 +27,7 @@ struct ComputeSpec {
     double alpha = 1.0;            // generalized
     bool bypass_tips = false;
     bool normalize = true;         // normalize_sample_counts
-    int unweighted_kernel = 0;     // 0: exact int8 MFMA Gram, 1: plain FP32/FP64 VALU tile kernel
+    int plain_kernel = 0;     // 0: MFMA Gram (unweighted) / shared-node kernel (weighted); 1: dense pair tiles
 };
 
 // Phases timed with HIP events on the plan's stream during execute().
@@ -50,6 +50,9 @@ public:
     const float* timings() const;  // ms per Phase, last execute()
     uint32_t n_samples() const;
     uint64_t processed_nodes() const;  // nodes entering the pair accumulation
+    // (pair, node) terms the last execute's main kernel evaluated, padding and
+    // the full diagonal tiles included (shared-node kernel: visited nodes only)
+    uint64_t kernel_terms() const;
     bool out_fp64() const;
     const char* main_kernel() const;
     const void* device_output() const;

@@ -10,6 +10,7 @@
 #include "ssu_kernels.hpp"
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 
 namespace ssu {
@@ -44,46 +45,89 @@ __global__ void __launch_bounds__(EMB_THREADS) k_embed_leaves(double* __restrict
 }
 
 // Internal nodes of one height level: E[k] = ((0 + E[c0]) + E[c1]) + ..., children left to right.
+// blockIdx.x covers samples; blockIdx.y strides over the level's nodes.
 __global__ void __launch_bounds__(EMB_THREADS) k_embed_level(double* __restrict__ E, size_t ldn,
-                                                             const int32_t* __restrict__ nodes, uint32_t cols_blocks,
+                                                             const int32_t* __restrict__ nodes, uint32_t n_nodes,
                                                              const int32_t* __restrict__ child_ptr,
                                                              const int32_t* __restrict__ child_idx) {
-    const uint32_t ni = blockIdx.x / cols_blocks;
-    const size_t s = (size_t)(blockIdx.x % cols_blocks) * EMB_THREADS + threadIdx.x;
+    const size_t s = (size_t)blockIdx.x * EMB_THREADS + threadIdx.x;
     if (s >= ldn) return;
-    const int32_t k = nodes[ni];
+    for (uint32_t ni = blockIdx.y; ni < n_nodes; ni += gridDim.y) {
+        const int32_t k = nodes[ni];
+        double acc = 0.0;
+        for (int32_t c = child_ptr[k]; c < child_ptr[k + 1]; ++c) acc += E[(size_t)child_idx[c] * ldn + s];
+        E[(size_t)k * ldn + s] = acc;
+    }
+}
+
+// Segmented pass over the processed rows: blockIdx.y owns rows [k0, k1), k0 a
+// multiple of 32.  Per thread (sample s) and row r, e = E[knodes[r]][s]:
+//   GATHER: U[r][s] = (float)e
+//   SUMS:   Spart[seg][s] = sum_{r in seg} klen[r] * e', increasing r, where e'
+//           is e as the pair kernels see it ((float)e when GATHER, else e)
+//   MASKS:  bit (r % 32) of masks[panel][r / 32] = any e > 0 over the 64
+//           samples of the panel (one wavefront = one panel)
+template <bool GATHER, bool SUMS, bool MASKS>
+__global__ void __launch_bounds__(EMB_THREADS) k_gather_sums(float* __restrict__ U, double* __restrict__ Spart,
+                                                             uint32_t* __restrict__ masks, uint32_t Kw,
+                                                             const double* __restrict__ E, size_t ldn,
+                                                             const int32_t* __restrict__ knodes,
+                                                             const double* __restrict__ klen, uint32_t K,
+                                                             uint32_t rows_per_seg) {
+    const size_t s = (size_t)blockIdx.x * EMB_THREADS + threadIdx.x;
+    if (s >= ldn) return;
+    const uint32_t k0 = blockIdx.y * rows_per_seg;
+    const uint32_t k1 = min(K, k0 + rows_per_seg);
+    const size_t panel = s / 64;
     double acc = 0.0;
-    for (int32_t c = child_ptr[k]; c < child_ptr[k + 1]; ++c) acc += E[(size_t)child_idx[c] * ldn + s];
-    E[(size_t)k * ldn + s] = acc;
+    uint32_t bits = 0;
+#pragma unroll 8
+    for (uint32_t r = k0; r < k1; ++r) {
+        double e = E[(size_t)knodes[r] * ldn + s];
+        if constexpr (GATHER) {
+            const float f = (float)e;
+            U[(size_t)r * ldn + s] = f;
+            e = (double)f;
+        }
+        if constexpr (SUMS) acc += klen[r] * e;
+        if constexpr (MASKS) {
+            bits |= (__ballot(e > 0.0) != 0ull ? 1u : 0u) << (r & 31);
+            if ((r & 31) == 31 || r + 1 == k1) {
+                if ((threadIdx.x & 63) == 0) masks[panel * Kw + (r >> 5)] = bits;
+                bits = 0;
+            }
+        }
+    }
+    if constexpr (SUMS) Spart[(size_t)blockIdx.y * ldn + s] = acc;
 }
 
-__global__ void __launch_bounds__(EMB_THREADS) k_gather_f32(float* __restrict__ U, const double* __restrict__ E,
-                                                            size_t ldn, const int32_t* __restrict__ knodes,
-                                                            uint32_t cols_blocks) {
-    const uint32_t r = blockIdx.x / cols_blocks;
-    const size_t s = (size_t)(blockIdx.x % cols_blocks) * EMB_THREADS + threadIdx.x;
-    if (s >= ldn) return;
-    U[(size_t)r * ldn + s] = (float)E[(size_t)knodes[r] * ldn + s];
-}
-
-__global__ void __launch_bounds__(EMB_THREADS) k_sample_sums(double* __restrict__ S, const double* __restrict__ E,
-                                                             size_t ldn, const int32_t* __restrict__ knodes,
-                                                             const double* __restrict__ klen, uint32_t K, uint32_t n) {
+__global__ void __launch_bounds__(EMB_THREADS) k_sum_segments(double* __restrict__ S, const double* __restrict__ Spart,
+                                                              size_t ldn, uint32_t segs, uint32_t n) {
     const uint32_t s = blockIdx.x * EMB_THREADS + threadIdx.x;
     if (s >= n) return;
     double acc = 0.0;
-    for (uint32_t k = 0; k < K; ++k) acc += klen[k] * E[(size_t)knodes[k] * ldn + s];
+    for (uint32_t g = 0; g < segs; ++g) acc += Spart[(size_t)g * ldn + s];
     S[s] = acc;
 }
 
-// Faith PD: identical FP64 summation order to src/unifrac.cpp:373-386.
-__global__ void __launch_bounds__(EMB_THREADS) k_faith(double* __restrict__ out, const double* __restrict__ E,
-                                                       size_t ldn, const double* __restrict__ len, uint32_t m,
-                                                       uint32_t n) {
-    const uint32_t s = blockIdx.x * EMB_THREADS + threadIdx.x;
+// Faith PD: identical FP64 summation order to src/unifrac.cpp:373-386. The
+// chain of adds is inherently sequential per sample, so each thread issues
+// FAITH_BATCH independent loads ahead of the ordered adds.
+constexpr uint32_t FAITH_BATCH = 16;
+__global__ void __launch_bounds__(64) k_faith(double* __restrict__ out, const double* __restrict__ E, size_t ldn,
+                                              const double* __restrict__ len, uint32_t m, uint32_t n) {
+    const uint32_t s = blockIdx.x * 64 + threadIdx.x;
     if (s >= n) return;
     double acc = 0.0;
-    for (uint32_t k = 0; k < m; ++k) acc += (double)(E[(size_t)k * ldn + s] > 0.0) * len[k];
+    uint32_t k = 0;
+    for (; k + FAITH_BATCH <= m; k += FAITH_BATCH) {
+        double v[FAITH_BATCH];
+#pragma unroll
+        for (uint32_t i = 0; i < FAITH_BATCH; ++i) v[i] = E[(size_t)(k + i) * ldn + s];
+#pragma unroll
+        for (uint32_t i = 0; i < FAITH_BATCH; ++i) acc += (double)(v[i] > 0.0) * len[k + i];
+    }
+    for (; k < m; ++k) acc += (double)(E[(size_t)k * ldn + s] > 0.0) * len[k];
     out[s] = acc;
 }
 
@@ -523,6 +567,128 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
         }
 }
 
+// ------------------------------------------------------ sparse weighted path
+//
+// Weighted UniFrac through the minimum:  |u - v| = u + v - 2 min(u, v), so
+//   sum_k l_k |u_ik - u_jk| = S_i + S_j - 2 M_ij,   M_ij = sum_k l_k min(u_ik, u_jk),
+// with S the per-sample sums.  min(u, v) is zero unless both samples hold the
+// node, so a 64x64 block of pairs only visits the nodes present in both of its
+// sample panels: the AND of the two panels' node masks.  One wavefront owns one
+// block; each lane keeps an 8x8 register tile of FP64 accumulators.
+
+constexpr uint32_t SP_B = 64;        // block edge in samples
+constexpr uint32_t SP_EB = 8;        // nodes staged per step
+constexpr uint32_t SP_WORDS = 64;    // mask words (32 nodes each) per list chunk
+
+template <class TIn, class TOut>
+__global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, const int32_t* __restrict__ rows,
+                                                     size_t ldn, const double* __restrict__ len,
+                                                     const uint32_t* __restrict__ masks, uint32_t Kw, uint32_t n,
+                                                     const double* __restrict__ S, int normalized,
+                                                     TOut* __restrict__ out, uint32_t* __restrict__ visits) {
+    const uint32_t bi = blockIdx.y, bj = blockIdx.x;
+    if (bi > bj) return;
+    uint32_t visited = 0;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t li = lane >> 3, lj = lane & 7;
+
+    __shared__ uint16_t list[SP_WORDS * 32];
+    __shared__ __attribute__((aligned(16))) double sv[SP_EB][2 * SP_B];
+    __shared__ double sl[SP_EB];
+
+    double acc[8][8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[r][c] = 0.0;
+
+    const uint32_t* mI = masks + (size_t)bi * Kw;
+    const uint32_t* mJ = masks + (size_t)bj * Kw;
+    const size_t offI = (size_t)bi * SP_B + lane, offJ = (size_t)bj * SP_B + lane;
+
+    for (uint32_t w0 = 0; w0 < Kw; w0 += SP_WORDS) {
+        // active nodes of this chunk, in increasing order, as offsets from w0*32
+        const uint32_t w = w0 + lane;
+        uint32_t word = w < Kw ? (mI[w] & mJ[w]) : 0u;
+        const uint32_t cnt = __popc(word);
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d, 64);
+            if ((int)lane >= d) incl += y;
+        }
+        const uint32_t total = __shfl(incl, 63, 64);
+        visited += total;
+        uint32_t o = incl - cnt;
+        while (word) {
+            const uint32_t b = __ffs(word) - 1;
+            list[o++] = (uint16_t)(lane * 32 + b);
+            word &= word - 1;
+        }
+        __syncthreads();
+        const uint32_t base = w0 * 32;
+        // the next step's rows are loaded into registers while the current
+        // step is consumed from LDS
+        TIn pi[SP_EB], pj[SP_EB];
+        double pl = 0.0;
+        auto fetch = [&](uint32_t t0) __attribute__((always_inline)) {
+#pragma unroll
+            for (uint32_t e = 0; e < SP_EB; ++e) {
+                pi[e] = TIn(0);
+                pj[e] = TIn(0);
+                if (t0 + e < total) {
+                    const uint32_t r = base + list[t0 + e];
+                    const size_t row = (size_t)(rows ? rows[r] : (int32_t)r) * ldn;
+                    pi[e] = U[row + offI];
+                    pj[e] = U[row + offJ];
+                    if (lane == e) pl = len[r];
+                }
+            }
+        };
+        if (total) fetch(0);
+        for (uint32_t t0 = 0; t0 < total; t0 += SP_EB) {
+#pragma unroll
+            for (uint32_t e = 0; e < SP_EB; ++e) {
+                sv[e][lane] = (double)pi[e];
+                sv[e][SP_B + lane] = (double)pj[e];
+            }
+            if (lane < SP_EB) sl[lane] = pl;
+            __syncthreads();
+            if (t0 + SP_EB < total) fetch(t0 + SP_EB);
+#pragma unroll 2
+            for (uint32_t e = 0; e < SP_EB; ++e) {
+                double a[8], b[8];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) a[r] = sv[e][li * 8 + r];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) b[c] = sv[e][SP_B + lj * 8 + c];
+                const double l = sl[e];
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) acc[r][c] = __builtin_fma(__builtin_fmin(a[r], b[c]), l, acc[r][c]);
+            }
+            __syncthreads();
+        }
+    }
+
+    if (visits && lane == 0) visits[(size_t)bi * gridDim.x + bj] = visited;
+    const size_t i0 = (size_t)bi * SP_B + li * 8, j0 = (size_t)bj * SP_B + lj * 8;
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const size_t i = i0 + r, j = j0 + c;
+            if (i >= n || j >= n) continue;
+            const double tot = S[i] + S[j];
+            double x = tot - 2.0 * acc[r][c];
+            x = x < 0.0 ? 0.0 : x;   // rounding can undershoot 0; NaN (empty samples) passes
+            const double d = i == j ? 0.0 : (normalized ? x / tot : x);
+            out[i * n + j] = (TOut)d;
+            out[j * n + i] = (TOut)d;
+        }
+}
+
 template <class TIn>
 __global__ void __launch_bounds__(256) k_matrix_to_stripes(double* __restrict__ st, const TIn* __restrict__ M,
                                                            uint32_t n, uint32_t start) {
@@ -573,28 +739,59 @@ hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint3
                               const int32_t* child_idx, hipStream_t st) {
     if (!n_nodes) return hipSuccess;
     const uint32_t cb = grid1(ldn, EMB_THREADS);
-    hipLaunchKernelGGL(k_embed_level, dim3(n_nodes * cb), dim3(EMB_THREADS), 0, st, E, ldn, nodes, cb, child_ptr,
+    const uint32_t gy = std::min<uint32_t>(n_nodes, 8192);
+    hipLaunchKernelGGL(k_embed_level, dim3(cb, gy), dim3(EMB_THREADS), 0, st, E, ldn, nodes, n_nodes, child_ptr,
                        child_idx);
     return hipGetLastError();
 }
 
-hipError_t launch_gather_f32(float* U, const double* E, size_t ldn, const int32_t* knodes, uint32_t K, hipStream_t st) {
-    if (!K) return hipSuccess;
+uint32_t gather_segments(uint32_t K, size_t ldn) {
+    if (!K) return 0;
     const uint32_t cb = grid1(ldn, EMB_THREADS);
-    hipLaunchKernelGGL(k_gather_f32, dim3(K * cb), dim3(EMB_THREADS), 0, st, U, E, ldn, knodes, cb);
-    return hipGetLastError();
+    const uint32_t want = std::max<uint32_t>(1, 4096 / cb);       // >= 4096 workgroups
+    const uint32_t most = (K + 63) / 64;                           // >= 64 rows per segment
+    return std::min(want, most);
 }
 
-hipError_t launch_sample_sums(double* S, const double* E, size_t ldn, const int32_t* knodes, const double* klen,
-                              uint32_t K, uint32_t n, hipStream_t st) {
-    hipLaunchKernelGGL(k_sample_sums, dim3(grid1(n, EMB_THREADS)), dim3(EMB_THREADS), 0, st, S, E, ldn, knodes, klen,
-                       K, n);
+template <bool G, bool S, bool M>
+static void gather_sums_t(dim3 grid, hipStream_t st, float* U, double* Spart, uint32_t* masks, uint32_t Kw,
+                          const double* E, size_t ldn, const int32_t* knodes, const double* klen, uint32_t K,
+                          uint32_t rps) {
+    hipLaunchKernelGGL((k_gather_sums<G, S, M>), grid, dim3(EMB_THREADS), 0, st, U, Spart, masks, Kw, E, ldn, knodes,
+                       klen, K, rps);
+}
+
+hipError_t launch_gather_sums(float* U, double* S, double* Spart, uint32_t* masks, const double* E, size_t ldn,
+                              const int32_t* knodes, const double* klen, uint32_t K, uint32_t n, hipStream_t st) {
+    if (masks && ldn % 64) return hipErrorInvalidValue;
+    if (!K) {
+        if (S && n) hipLaunchKernelGGL(k_sum_segments, dim3(grid1(n, EMB_THREADS)), dim3(EMB_THREADS), 0, st, S, Spart, ldn, 0u, n);
+        return hipGetLastError();
+    }
+    const uint32_t cb = grid1(ldn, EMB_THREADS);
+    const uint32_t segs0 = gather_segments(K, ldn);
+    const uint32_t rps = (((K + segs0 - 1) / segs0) + 31) / 32 * 32;   // mask words never straddle segments
+    const uint32_t segs = (K + rps - 1) / rps;
+    const uint32_t Kw = (K + 31) / 32;
+    const dim3 grid(cb, segs);
+    const int key = (U ? 4 : 0) | (S ? 2 : 0) | (masks ? 1 : 0);
+    switch (key) {
+    case 7: gather_sums_t<true, true, true>(grid, st, U, Spart, masks, Kw, E, ldn, knodes, klen, K, rps); break;
+    case 6: gather_sums_t<true, true, false>(grid, st, U, Spart, masks, Kw, E, ldn, knodes, klen, K, rps); break;
+    case 4: gather_sums_t<true, false, false>(grid, st, U, Spart, masks, Kw, E, ldn, knodes, klen, K, rps); break;
+    case 3: gather_sums_t<false, true, true>(grid, st, U, Spart, masks, Kw, E, ldn, knodes, klen, K, rps); break;
+    case 2: gather_sums_t<false, true, false>(grid, st, U, Spart, masks, Kw, E, ldn, knodes, klen, K, rps); break;
+    case 0: return hipSuccess;
+    default: return hipErrorInvalidValue;
+    }
+    if (S) hipLaunchKernelGGL(k_sum_segments, dim3(grid1(n, EMB_THREADS)), dim3(EMB_THREADS), 0, st, S, Spart, ldn, segs, n);
     return hipGetLastError();
 }
 
 hipError_t launch_faith(double* out, const double* E, size_t ldn, const double* len, uint32_t m, uint32_t n,
                         hipStream_t st) {
-    hipLaunchKernelGGL(k_faith, dim3(grid1(n, EMB_THREADS)), dim3(EMB_THREADS), 0, st, out, E, ldn, len, m, n);
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_faith, dim3(grid1(n, 64)), dim3(64), 0, st, out, E, ldn, len, m, n);
     return hipGetLastError();
 }
 
@@ -626,6 +823,28 @@ template hipError_t launch_unweighted_combine<double>(double*, const double*, ui
 
 hipError_t launch_pairs(PairOp op, bool fp64, const PairArgs& a, hipStream_t st) {
     return fp64 ? launch_pairs_prec<double>(op, a, st) : launch_pairs_prec<float>(op, a, st);
+}
+
+template <class TIn, class TOut>
+static void pairs_sparse_t(dim3 grid, hipStream_t st, const SparseArgs& a) {
+    hipLaunchKernelGGL((k_pairs_sparse<TIn, TOut>), grid, dim3(64), 0, st, (const TIn*)a.U, a.rows, a.ldn, a.len,
+                       a.masks, a.Kw, a.n, a.S, (int)a.normalized, (TOut*)a.out, a.visits);
+}
+
+hipError_t launch_pairs_sparse(bool in_fp64, const SparseArgs& a, hipStream_t st) {
+    if (a.ldn % SP_B) return hipErrorInvalidValue;
+    const uint32_t nb = (a.n + SP_B - 1) / SP_B;
+    if (!nb) return hipSuccess;
+    if ((size_t)nb * SP_B > a.ldn) return hipErrorInvalidValue;
+    const dim3 grid(nb, nb);
+    if (in_fp64) {
+        if (a.out_fp64) pairs_sparse_t<double, double>(grid, st, a);
+        else pairs_sparse_t<double, float>(grid, st, a);
+    } else {
+        if (a.out_fp64) pairs_sparse_t<float, double>(grid, st, a);
+        else pairs_sparse_t<float, float>(grid, st, a);
+    }
+    return hipGetLastError();
 }
 
 template <class TIn>

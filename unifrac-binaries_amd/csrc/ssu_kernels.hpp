@@ -29,11 +29,16 @@ hipError_t launch_embed_leaves(double* E, size_t ldn, const int32_t* leaf_k, con
 hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint32_t n_nodes, const int32_t* child_ptr,
                               const int32_t* child_idx, hipStream_t st);
 
-// node-major FP64 -> FP32 copy of the processed rows (row r <- E[knodes[r]])
-hipError_t launch_gather_f32(float* U, const double* E, size_t ldn, const int32_t* knodes, uint32_t K, hipStream_t st);
-// per-sample S_s = sum_k len_k * E[knodes[k]][s] (FP64, sequential in k)
-hipError_t launch_sample_sums(double* S, const double* E, size_t ldn, const int32_t* knodes, const double* klen,
-                              uint32_t K, uint32_t n, hipStream_t st);
+// One segmented pass over the processed rows r = 0..K-1 (row r is E[knodes[r]]):
+//   U (if non-null): node-major FP32 copy, U[r][s] = (float)E[knodes[r]][s];
+//   S (if non-null): S[s] = sum_r klen[r] * u_rs in FP64 (u as the pair kernels
+//     see it: FP32-rounded when U is written), summed per segment then over
+//     segments in order; Spart holds gather_segments(K, ldn) * ldn doubles;
+//   masks (if non-null; ldn % 64 == 0): [ldn/64][ceil(K/32)] words, bit r%32 of
+//     word r/32 of panel p set iff some sample of p*64..p*64+63 has u_rs > 0.
+uint32_t gather_segments(uint32_t K, size_t ldn);
+hipError_t launch_gather_sums(float* U, double* S, double* Spart, uint32_t* masks, const double* E, size_t ldn,
+                              const int32_t* knodes, const double* klen, uint32_t K, uint32_t n, hipStream_t st);
 // Faith PD: out[s] = sum over all non-root nodes k in postorder of (E[k][s] > 0) * len[k]
 hipError_t launch_faith(double* out, const double* E, size_t ldn, const double* len, uint32_t m, uint32_t n,
                         hipStream_t st);
@@ -77,6 +82,24 @@ struct PairArgs {
     bool out_fp64;
 };
 hipError_t launch_pairs(PairOp op, bool fp64, const PairArgs& a, hipStream_t st);
+
+// Weighted (normalized or not) through d_ij = S_i + S_j - 2 sum_k l_k min(u_ik, u_jk),
+// visiting per 64x64 block only the nodes whose bit is set in both panels' masks.
+struct SparseArgs {
+    const void* U;          // node rows, float (row r) or double (row rows[r])
+    const int32_t* rows;    // nullptr: identity
+    size_t ldn;             // multiple of 64
+    const double* len;      // [K] lengths as the accumulation uses them
+    const uint32_t* masks;  // from launch_gather_sums
+    uint32_t Kw;            // ceil(K / 32)
+    uint32_t n;
+    const double* S;        // per-sample sums from launch_gather_sums
+    bool normalized;
+    void* out;              // n x n
+    bool out_fp64;
+    uint32_t* visits;       // optional [nb][nb], nb = ceil(n/64): nodes visited per block (bi <= bj)
+};
+hipError_t launch_pairs_sparse(bool in_fp64, const SparseArgs& a, hipStream_t st);
 
 // --------------------------------------------------------- output layouts
 // stripes[s - start][k] = M[k][(k+s+1) % n]   (double output)

@@ -70,7 +70,7 @@ EXPORTS = [
     "one_dense_pair_v2t", "one_dense_pair_v2", "faith_pd_one_off", "partial_v3", "partial", "write_mat",
     "write_mat_from_matrix", "write_mat_from_matrix_fp32", "write_vec", "ssu_faith_pd_inmem",
     "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
-    "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
+    "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
     "ssu_newick_to_bp", "ssu_device_available", "ssu_last_error",
 ]
 
@@ -91,6 +91,7 @@ def lib():
         for n in status_fns:
             getattr(L, n).restype = C.c_int
         L.ssu_plan_pair_nodes.restype = C.c_uint64
+        L.ssu_plan_kernel_terms.restype = C.c_uint64
         L.ssu_plan_main_kernel.restype = C.c_char_p
         L.ssu_last_error.restype = C.c_char_p
         L.ssu_postorder.restype = C.c_int
@@ -191,13 +192,13 @@ class Plan:
     """Device-resident plan (extension): upload once, execute, time, copy."""
 
     def __init__(self, inp: Inputs, method: str, variance_adjust=False, alpha=1.0, bypass_tips=False,
-                 normalize=True, unweighted_kernel=0):
+                 normalize=True, plain_kernel=0):
         L = lib()
         self._inp = inp
         self.h = C.c_void_p()
         st = L.ssu_plan_create_inmem(C.byref(inp.table), C.byref(inp.tree), method.encode(),
                                      C.c_bool(variance_adjust), C.c_double(alpha), C.c_bool(bypass_tips),
-                                     C.c_bool(normalize), C.c_int(unweighted_kernel), C.byref(self.h))
+                                     C.c_bool(normalize), C.c_int(plain_kernel), C.byref(self.h))
         if st != OKAY:
             raise RuntimeError(f"ssu_plan_create_inmem failed: status {st}: {last_error()}")
         self.n = inp.n_samples
@@ -215,6 +216,9 @@ class Plan:
 
     def pair_nodes(self) -> int:
         return int(lib().ssu_plan_pair_nodes(self.h))
+
+    def kernel_terms(self) -> int:
+        return int(lib().ssu_plan_kernel_terms(self.h))
 
     def main_kernel(self) -> str:
         return lib().ssu_plan_main_kernel(self.h).decode()
