@@ -268,6 +268,23 @@ SSU_EXTERN IOStatus write_mat_from_matrix(const char* filename, mat_full_fp64_t*
 SSU_EXTERN IOStatus write_mat_from_matrix_fp32(const char* filename, mat_full_fp32_t* result);   /* api.hpp:691 */
 SSU_EXTERN IOStatus write_vec(const char* filename, r_vec* result);                              /* api.hpp:763 */
 
+/* Partial-result files (stripes [start, stop) of the full matrix, LZ4 blocks,
+ * magic PARTIAL_MAGIC_V2) and their merge into a full matrix. */
+SSU_EXTERN IOStatus write_partial(const char* filename, const partial_mat_t* result);          /* api.hpp:857 */
+SSU_EXTERN IOStatus read_partial(const char* filename, partial_mat_t** result);                /* api.hpp:872 */
+SSU_EXTERN IOStatus read_partial_header(const char* input_filename, partial_dyn_mat_t** result_out); /* api.hpp:887 */
+SSU_EXTERN IOStatus read_partial_one_stripe(partial_dyn_mat_t* result, uint32_t stripe_idx);  /* api.hpp:903 */
+SSU_EXTERN MergeStatus validate_partial(const partial_dyn_mat_t* const* partial_mats, int n_partials); /* api.hpp:908 */
+SSU_EXTERN MergeStatus merge_partial_to_mmap_matrix(partial_dyn_mat_t** partial_mats, int n_partials,
+                                                    const char* mmap_dir, mat_full_fp64_t** result);   /* api.hpp:975 */
+SSU_EXTERN MergeStatus merge_partial_to_mmap_matrix_fp32(partial_dyn_mat_t** partial_mats, int n_partials,
+                                                         const char* mmap_dir, mat_full_fp32_t** result); /* api.hpp:991 */
+#ifdef __cplusplus
+/* api.hpp:942, :958 declare these two without extern "C" (C++ linkage) */
+MergeStatus merge_partial_to_matrix(partial_dyn_mat_t** partial_mats, int n_partials, mat_full_fp64_t** result);
+MergeStatus merge_partial_to_matrix_fp32(partial_dyn_mat_t** partial_mats, int n_partials, mat_full_fp32_t** result);
+#endif
+
 /* ==================================================================
  * Extensions (no counterpart in the reference)
  * ================================================================== */

@@ -335,3 +335,39 @@ def test_file_entry_points(ssu, O, ref_trees, ref_tables, goldens, tmp_path):
                                 C.byref(out)) == ssu.OKAY
     assert abs(out.value - d["exp13"]) < 1e-5
     L.destroy_bptree_opaque(C.byref(op))
+
+
+def test_partial_files_merge_to_full_matrix(ssu, ref_trees, tmp_path):
+    """partial_v3 on the GPU for two stripe ranges, written with write_partial,
+    merged with merge_partial_to_mmap_matrix: equal to one_off_matrix_v3."""
+    L = ssu.lib()
+    tre = tmp_path / "crawford.tre"
+    tre.write_text(ref_trees["crawford"])
+    biom = os.path.join(GOLDEN, "crawford.biom").encode()
+    say("partial merge crawford")
+    full = C.POINTER(ssu.MatFullFP64)()
+    assert L.one_off_matrix_v3(biom, str(tre).encode(), b"weighted_normalized_fp64", False, C.c_double(1.0), False,
+                               True, 1, 0, True, None, C.byref(full)) == ssu.OKAY, ssu.last_error()
+    n = full.contents.n_samples
+    ref = np.ctypeslib.as_array(full.contents.matrix, shape=(n * n,)).copy().reshape(n, n)
+    L.destroy_mat_full_fp64(C.byref(full))
+    total = (n + 1) // 2
+    mats = []
+    for a, b in ((0, 2), (2, total)):
+        pm = C.POINTER(ssu.PartialMat)()
+        assert L.partial_v3(biom, str(tre).encode(), b"weighted_normalized_fp64", False, C.c_double(1.0), False, True,
+                            1, a, b, C.byref(pm)) == ssu.OKAY
+        path = str(tmp_path / f"part{a}.partial").encode()
+        assert L.write_partial(path, pm) == ssu.WRITE_OKAY
+        L.destroy_partial_mat(C.byref(pm))
+        dyn = C.POINTER(ssu.PartialDynMat)()
+        assert L.read_partial_header(path, C.byref(dyn)) == ssu.READ_OKAY
+        mats.append(dyn)
+    arr = (C.POINTER(ssu.PartialDynMat) * 2)(*mats)
+    res = C.POINTER(ssu.MatFullFP64)()
+    assert L.merge_partial_to_mmap_matrix(arr, 2, str(tmp_path).encode(), C.byref(res)) == ssu.MERGE_OKAY
+    got = np.ctypeslib.as_array(res.contents.matrix, shape=(n * n,)).copy().reshape(n, n)
+    L.destroy_mat_full_fp64(C.byref(res))
+    for d in mats:
+        L.destroy_partial_dyn_mat(C.byref(d))
+    assert np.array_equal(got, ref)

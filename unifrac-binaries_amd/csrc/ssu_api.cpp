@@ -20,6 +20,7 @@
 
 #include "ssu_biom.hpp"
 #include "ssu_engine.hpp"
+#include "ssu_results.hpp"
 #include "ssu_table.hpp"
 #include "ssu_tree.hpp"
 
@@ -29,18 +30,24 @@ using ssu::PhyloTree;
 using ssu::Plan;
 using ssu::Table;
 
+namespace ssu {
 namespace {
-
 thread_local std::string g_last_error;
-std::mt19937_64 g_rng(5489u);
-
-constexpr uint32_t MMAP_FD_MASK = 0x0fff;  // src/api.cpp:28-29
-constexpr uint32_t MMAP_FLAG = 0x1000;
-
+}
 void set_error(const std::string& e) {
     g_last_error = e;
     if (!e.empty()) fprintf(stderr, "ssu: %s\n", e.c_str());
 }
+const std::string& last_error() { return g_last_error; }
+}  // namespace ssu
+
+namespace {
+
+std::mt19937_64 g_rng(5489u);
+using ssu::destroy_full;
+using ssu::dup_ids;
+using ssu::init_full;
+using ssu::set_error;
 
 // src/api.cpp:55-88 (SET_METHOD): bare names run in FP32, "_fp64" in FP64
 bool parse_method(const char* name, Method& m, bool& fp64) {
@@ -63,11 +70,6 @@ bool parse_method(const char* name, Method& m, bool& fp64) {
     return true;
 }
 
-char** dup_ids(const std::vector<std::string>& ids) {
-    char** out = (char**)malloc(sizeof(char*) * (ids.empty() ? 1 : ids.size()));
-    for (size_t i = 0; i < ids.size(); ++i) out[i] = strdup(ids[i].c_str());
-    return out;
-}
 
 // src/api.cpp:94-100 (VALIDATE_TREE_TABLE)
 ComputeStatus validate(const Table& t, const PhyloTree& tree) {
@@ -117,59 +119,6 @@ ComputeStatus run_full(const Table& t, const PhyloTree& tree, const ComputeSpec&
         for (size_t i = 0; i < nn; ++i) out[i] = (TOut)tmp[i];
     }
     return okay;
-}
-
-// src/api.cpp:270-327 (initialize_mat_full_no_biom_T): malloc, or an unlinked mmap'd file under mmap_dir
-template <class TReal, class TMat>
-bool init_full(TMat*& r, const std::vector<std::string>& ids, const char* mmap_dir) {
-    r = (TMat*)malloc(sizeof(TMat));
-    r->n_samples = (uint32_t)ids.size();
-    r->flags = 0;
-    r->sample_ids = dup_ids(ids);
-    const uint64_t n = r->n_samples;
-    const uint64_t msize = sizeof(TReal) * n * n;
-    if (mmap_dir && mmap_dir[0] == 0) mmap_dir = nullptr;
-    if (!mmap_dir) {
-        r->matrix = (TReal*)malloc(msize ? msize : sizeof(TReal));
-    } else {
-        std::string tmpl = std::string(mmap_dir) + "/su_mmap_XXXXXX";
-        int fd = mkostemp(&tmpl[0], O_NOATIME);
-        r->matrix = nullptr;
-        if (fd >= 0) {
-            unlink(tmpl.c_str());
-            if (ftruncate(fd, (off_t)msize) == 0) {
-                void* p = mmap(nullptr, msize ? msize : 1, PROT_READ | PROT_WRITE, MAP_SHARED | MAP_NORESERVE, fd, 0);
-                if (p != MAP_FAILED) {
-                    r->matrix = (TReal*)p;
-                    r->flags = ((uint32_t)fd & MMAP_FD_MASK) | MMAP_FLAG;
-                } else {
-                    close(fd);
-                }
-            } else {
-                close(fd);
-            }
-        }
-    }
-    return r->matrix != nullptr;
-}
-
-template <class TReal, class TMat>
-void destroy_full(TMat** result) {
-    if (!result || !*result) return;
-    TMat* r = *result;
-    for (uint32_t i = 0; i < r->n_samples; ++i) free(r->sample_ids[i]);
-    free(r->sample_ids);
-    if (r->matrix) {
-        if ((r->flags & MMAP_FLAG) == 0) {
-            free(r->matrix);
-        } else {
-            const uint64_t n = r->n_samples;
-            munmap(r->matrix, (sizeof(TReal) * n * n) > 0 ? sizeof(TReal) * n * n : 1);
-            close((int)(r->flags & MMAP_FD_MASK));
-        }
-    }
-    free(r);
-    *result = nullptr;
 }
 
 // Rarefaction: samples below the depth are dropped, the rest drawn down to
@@ -942,4 +891,4 @@ int ssu_device_available(void) {
     return ssu::device_available(&err) ? 1 : 0;
 }
 
-const char* ssu_last_error(void) { return g_last_error.c_str(); }
+const char* ssu_last_error(void) { return ssu::last_error().c_str(); }

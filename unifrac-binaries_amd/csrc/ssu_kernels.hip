@@ -576,6 +576,14 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
 // sample panels: the AND of the two panels' node masks.  One wavefront owns one
 // block; each lane keeps an 8x8 register tile of FP64 accumulators.
 
+// v_min_f64 without the NaN-quieting canonicalisation the compiler adds for
+// operands it cannot see produced (here: LDS loads of converted values)
+__device__ __forceinline__ double vmin_f64(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 constexpr uint32_t SP_B = 64;        // block edge in samples
 constexpr uint32_t SP_EB = 8;        // nodes staged per step
 constexpr uint32_t SP_WORDS = 64;    // mask words (32 nodes each) per list chunk
@@ -617,7 +625,7 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
             const uint32_t y = __shfl_up(incl, d, 64);
             if ((int)lane >= d) incl += y;
         }
-        const uint32_t total = __shfl(incl, 63, 64);
+        const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
         visited += total;
         uint32_t o = incl - cnt;
         while (word) {
@@ -628,20 +636,27 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
         __syncthreads();
         const uint32_t base = w0 * 32;
         // the next step's rows are loaded into registers while the current
-        // step is consumed from LDS
+        // step is consumed from LDS; lane e < SP_EB resolves node t0 + e
         TIn pi[SP_EB], pj[SP_EB];
         double pl = 0.0;
         auto fetch = [&](uint32_t t0) __attribute__((always_inline)) {
+            const uint32_t idx = t0 + lane;
+            int32_t row = -1;
+            pl = 0.0;
+            if (lane < SP_EB && idx < total) {
+                const uint32_t r = base + list[idx];
+                row = rows ? rows[r] : (int32_t)r;
+                pl = len[r];
+            }
 #pragma unroll
             for (uint32_t e = 0; e < SP_EB; ++e) {
+                const int32_t re = __builtin_amdgcn_readlane(row, e);
                 pi[e] = TIn(0);
                 pj[e] = TIn(0);
-                if (t0 + e < total) {
-                    const uint32_t r = base + list[t0 + e];
-                    const size_t row = (size_t)(rows ? rows[r] : (int32_t)r) * ldn;
-                    pi[e] = U[row + offI];
-                    pj[e] = U[row + offJ];
-                    if (lane == e) pl = len[r];
+                if (re >= 0) {
+                    const size_t off = (size_t)re * ldn;
+                    pi[e] = U[off + offI];
+                    pj[e] = U[off + offJ];
                 }
             }
         };
@@ -666,7 +681,7 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
 #pragma unroll
                 for (int r = 0; r < 8; ++r)
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) acc[r][c] = __builtin_fma(__builtin_fmin(a[r], b[c]), l, acc[r][c]);
+                    for (int c = 0; c < 8; ++c) acc[r][c] = __builtin_fma(vmin_f64(a[r], b[c]), l, acc[r][c]);
             }
             __syncthreads();
         }

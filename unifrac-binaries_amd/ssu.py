@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libssu.so")
 OKAY, TREE_MISSING, TABLE_MISSING, TABLE_EMPTY, UNKNOWN_METHOD, TABLE_AND_TREE_DO_NOT_OVERLAP, OUTPUT_ERROR, \
     INVALID_METHOD, GROUPING_MISSING = range(9)
 READ_OKAY, WRITE_OKAY, OPEN_ERROR, READ_ERROR, MAGIC_INCOMPATIBLE, BAD_HEADER, UNEXPECTED_END, WRITE_ERROR = range(8)
+MERGE_OKAY, INCOMPLETE_STRIPE_SET, SAMPLE_ID_CONSISTENCY, SQUARE_MISMATCH, PARTIALS_MISMATCH, STRIPES_OVERLAP = range(6)
 
 
 class SupportBiom(C.Structure):  # src/api.hpp:145-154
@@ -57,6 +58,13 @@ class PartialMat(C.Structure):  # src/api.hpp:99-107
                 ("stripe_stop", C.c_uint32), ("stripe_total", C.c_uint32), ("is_upper_triangle", C.c_bool)]
 
 
+class PartialDynMat(C.Structure):  # src/api.hpp:122-132
+    _fields_ = [("n_samples", C.c_uint32), ("sample_ids", C.POINTER(C.c_char_p)),
+                ("offsets", C.POINTER(C.c_uint64)), ("stripes", C.POINTER(C.POINTER(C.c_double))),
+                ("stripe_start", C.c_uint32), ("stripe_stop", C.c_uint32), ("stripe_total", C.c_uint32),
+                ("is_upper_triangle", C.c_bool), ("filename", C.c_char_p)]
+
+
 # every symbol include/ssu.h declares
 EXPORTS = [
     "ssu_set_random_seed", "read_bptree_opaque", "load_bptree_opaque", "convert_bptree_opaque",
@@ -68,7 +76,9 @@ EXPORTS = [
     "one_off_matrix", "one_off_matrix_fp32_v3", "one_off_matrix_fp32_v3t", "one_off_matrix_fp32_v2",
     "one_off_matrix_fp32_v2t", "one_off_matrix_fp32", "one_dense_pair_v3t", "one_dense_pair_v3",
     "one_dense_pair_v2t", "one_dense_pair_v2", "faith_pd_one_off", "partial_v3", "partial", "write_mat",
-    "write_mat_from_matrix", "write_mat_from_matrix_fp32", "write_vec", "ssu_faith_pd_inmem",
+    "write_mat_from_matrix", "write_mat_from_matrix_fp32", "write_vec", "write_partial", "read_partial",
+    "read_partial_header", "read_partial_one_stripe", "validate_partial", "merge_partial_to_mmap_matrix",
+    "merge_partial_to_mmap_matrix_fp32", "ssu_faith_pd_inmem",
     "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
     "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
     "ssu_newick_to_bp", "ssu_device_available", "ssu_last_error",
@@ -98,6 +108,10 @@ def lib():
         L.ssu_newick_to_bp.restype = C.c_int
         L.get_bptree_opaque_els.restype = C.c_int
         L.read_bptree_opaque.restype = C.c_int
+        for n in ("write_partial", "read_partial", "read_partial_header", "read_partial_one_stripe",
+                  "validate_partial", "merge_partial_to_mmap_matrix", "merge_partial_to_mmap_matrix_fp32"):
+            getattr(L, n).restype = C.c_int
+        L.read_partial_one_stripe.argtypes = [C.POINTER(PartialDynMat), C.c_uint32]
         _lib = L
     return _lib
 
