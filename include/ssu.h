@@ -268,6 +268,13 @@ SSU_EXTERN IOStatus write_mat_from_matrix(const char* filename, mat_full_fp64_t*
 SSU_EXTERN IOStatus write_mat_from_matrix_fp32(const char* filename, mat_full_fp32_t* result);   /* api.hpp:691 */
 SSU_EXTERN IOStatus write_vec(const char* filename, r_vec* result);                              /* api.hpp:763 */
 
+/* Distance-matrix HDF5 files (format "BDSM", version "2020.12"): "order" and,
+ * when save_dist, "matrix".  pcoa_dims must be 0 (no PCoA in this build). */
+SSU_EXTERN IOStatus write_mat_from_matrix_hdf5_fp64(const char* filename, mat_full_fp64_t* result,
+                                                    unsigned int pcoa_dims, int save_dist);  /* api.hpp:722 */
+SSU_EXTERN IOStatus write_mat_from_matrix_hdf5_fp32(const char* filename, mat_full_fp32_t* result,
+                                                    unsigned int pcoa_dims, int save_dist);  /* api.hpp:752 */
+
 /* Partial-result files (stripes [start, stop) of the full matrix, LZ4 blocks,
  * magic PARTIAL_MAGIC_V2) and their merge into a full matrix. */
 SSU_EXTERN IOStatus write_partial(const char* filename, const partial_mat_t* result);          /* api.hpp:857 */
@@ -328,6 +335,9 @@ SSU_EXTERN int ssu_postorder(const support_bptree_t* tree_data, int n_obs, const
 SSU_EXTERN int ssu_newick_to_bp(const char* newick, int cap, uint8_t* structure, double* lengths, char** names_buf,
                                 int name_cap);
 /* 1 if a HIP device is visible */
+/* number of samples of a BIOM file (its /sample/ids), or -1; for the ssu
+ * CLI's partial-report mode */
+SSU_EXTERN long ssu_biom_n_samples(const char* biom_filename);
 SSU_EXTERN int ssu_device_available(void);
 SSU_EXTERN const char* ssu_last_error(void);
 

@@ -774,6 +774,44 @@ IOStatus write_mat_from_matrix_fp32(const char* output_filename, mat_full_fp32_t
     return write_full_txt(output_filename, result);
 }
 
+// src/api_compat.hpp:180-189 -> src/api.cpp:1787-1935.  PCoA (pcoa_dims > 0)
+// needs the third-party eigensolver the reference links and is refused here.
+template <class TMat>
+IOStatus write_full_hdf5(const char* filename, TMat* result, unsigned int pcoa_dims, int save_dist, bool fp64) {
+    if (!filename || !result) return write_error;
+    if (pcoa_dims > 0) {
+        set_error("PCoA output is not available in this build (pcoa_dims must be 0)");
+        return write_error;
+    }
+    std::string err;
+    if (!ssu::write_matrix_hdf5(filename, result->n_samples, result->sample_ids, result->matrix, fp64, save_dist != 0,
+                                &err)) {
+        set_error(err);
+        return write_error;
+    }
+    return write_okay;
+}
+
+IOStatus write_mat_from_matrix_hdf5_fp64(const char* filename, mat_full_fp64_t* result, unsigned int pcoa_dims,
+                                         int save_dist) {
+    return write_full_hdf5(filename, result, pcoa_dims, save_dist, true);
+}
+
+IOStatus write_mat_from_matrix_hdf5_fp32(const char* filename, mat_full_fp32_t* result, unsigned int pcoa_dims,
+                                         int save_dist) {
+    return write_full_hdf5(filename, result, pcoa_dims, save_dist, false);
+}
+
+long ssu_biom_n_samples(const char* biom_filename) {
+    uint64_t n = 0;
+    std::string err;
+    if (!biom_filename || !ssu::biom_n_samples(biom_filename, n, &err)) {
+        set_error(err);
+        return -1;
+    }
+    return (long)n;
+}
+
 // src/api.cpp:1759-1775 (write_vec): "#SampleID\tfaith_pd" then one line per sample
 IOStatus write_vec(const char* output_filename, r_vec* result) {
     FILE* f = fopen(output_filename, "w");

@@ -19,6 +19,7 @@ typedef int htri_t;
 typedef unsigned long long hsize_t;
 constexpr hid_t P_DEFAULT = 0, S_ALL = 0;
 constexpr unsigned F_ACC_RDONLY = 0u;
+constexpr unsigned F_ACC_TRUNC = 2u;
 constexpr size_t T_VARIABLE = (size_t)-1;
 
 struct H5 {
@@ -42,9 +43,16 @@ struct H5 {
     herr_t (*Tset_size)(hid_t, size_t);
     herr_t (*Tclose)(hid_t);
     herr_t (*Eset_auto2)(hid_t, void*, void*);
+    hid_t (*Fcreate)(const char*, unsigned, hid_t, hid_t);
+    hid_t (*Screate_simple)(int, const hsize_t*, const hsize_t*);
+    hid_t (*Dcreate2)(hid_t, const char*, hid_t, hid_t, hid_t, hid_t, hid_t);
+    herr_t (*Dwrite)(hid_t, hid_t, hid_t, hid_t, hid_t, const void*);
     hid_t* NATIVE_DOUBLE_g;
     hid_t* NATIVE_UINT32_g;
     hid_t* C_S1_g;
+    hid_t* FORTRAN_S1_g;
+    hid_t* IEEE_F64LE_g;
+    hid_t* IEEE_F32LE_g;
 };
 
 H5& h5() {
@@ -70,8 +78,11 @@ H5& h5() {
         sym(h.Tis_variable_str, "H5Tis_variable_str"); sym(h.Tget_size, "H5Tget_size");
         sym(h.Tcopy, "H5Tcopy"); sym(h.Tset_size, "H5Tset_size"); sym(h.Tclose, "H5Tclose");
         sym(h.Eset_auto2, "H5Eset_auto2");
+        sym(h.Fcreate, "H5Fcreate"); sym(h.Screate_simple, "H5Screate_simple"); sym(h.Dcreate2, "H5Dcreate2");
+        sym(h.Dwrite, "H5Dwrite");
         sym(h.NATIVE_DOUBLE_g, "H5T_NATIVE_DOUBLE_g"); sym(h.NATIVE_UINT32_g, "H5T_NATIVE_UINT32_g");
-        sym(h.C_S1_g, "H5T_C_S1_g");
+        sym(h.C_S1_g, "H5T_C_S1_g"); sym(h.FORTRAN_S1_g, "H5T_FORTRAN_S1_g");
+        sym(h.IEEE_F64LE_g, "H5T_IEEE_F64LE_g"); sym(h.IEEE_F32LE_g, "H5T_IEEE_F32LE_g");
         if (!good || h.open() < 0) { h.why = "libhdf5 lacks the 1.10 C API"; return; }
         h.Eset_auto2(0 /* H5E_DEFAULT */, nullptr, nullptr);  // silence the error stack printer
         h.ok = true;
@@ -129,7 +140,60 @@ bool read_strings(H5& h, hid_t file, const char* path, std::vector<std::string>&
     return true;
 }
 
+// one fixed-length (Fortran, unterminated) string, as src/api.cpp:1167-1189 stores it
+bool write_string(H5& h, hid_t file, const char* name, const std::string& v) {
+    Handle ft{h.Tcopy(*h.FORTRAN_S1_g), h.Tclose};
+    Handle mt{h.Tcopy(*h.C_S1_g), h.Tclose};
+    if (ft.id < 0 || mt.id < 0 || h.Tset_size(ft.id, v.size()) < 0 || h.Tset_size(mt.id, v.size() + 1) < 0)
+        return false;
+    const hsize_t one = 1;
+    Handle sp{h.Screate_simple(1, &one, nullptr), h.Sclose};
+    Handle ds{h.Dcreate2(file, name, ft.id, sp.id, P_DEFAULT, P_DEFAULT, P_DEFAULT), h.Dclose};
+    return ds.id >= 0 && h.Dwrite(ds.id, mt.id, S_ALL, S_ALL, P_DEFAULT, v.c_str()) >= 0;
+}
+
+// variable-length C strings (src/api.cpp:1192-1216)
+bool write_strings(H5& h, hid_t file, const char* name, uint32_t n, const char* const* v) {
+    Handle t{h.Tcopy(*h.C_S1_g), h.Tclose};
+    if (t.id < 0 || h.Tset_size(t.id, T_VARIABLE) < 0) return false;
+    const hsize_t dims = n;
+    Handle sp{h.Screate_simple(1, &dims, nullptr), h.Sclose};
+    Handle ds{h.Dcreate2(file, name, t.id, sp.id, P_DEFAULT, P_DEFAULT, P_DEFAULT), h.Dclose};
+    return ds.id >= 0 && (n == 0 || h.Dwrite(ds.id, t.id, S_ALL, S_ALL, P_DEFAULT, v) >= 0);
+}
+
 }  // namespace
+
+bool biom_n_samples(const char* filename, uint64_t& n, std::string* err) {
+    H5& h = h5();
+    if (!h.ok) { if (err) *err = h.why; return false; }
+    Handle f{h.Fopen(filename, F_ACC_RDONLY, P_DEFAULT), h.Fclose};
+    if (f.id < 0) { if (err) *err = std::string("cannot open BIOM file ") + filename; return false; }
+    Handle ds{h.Dopen2(f.id, "/sample/ids", P_DEFAULT), h.Dclose};
+    hsize_t m = 0;
+    if (ds.id < 0 || !dims1(h, ds.id, m)) { if (err) *err = std::string("no /sample/ids in ") + filename; return false; }
+    n = m;
+    return true;
+}
+
+bool write_matrix_hdf5(const char* filename, uint32_t n, const char* const* ids, const void* matrix, bool fp64,
+                       bool save_dist, std::string* err) {
+    H5& h = h5();
+    if (!h.ok) { if (err) *err = h.why; return false; }
+    Handle f{h.Fcreate(filename, F_ACC_TRUNC, P_DEFAULT, P_DEFAULT), h.Fclose};
+    if (f.id < 0) { if (err) *err = std::string("cannot create ") + filename; return false; }
+    bool ok = write_string(h, f.id, "format", "BDSM") && write_string(h, f.id, "version", "2020.12") &&
+              write_strings(h, f.id, "order", n, ids);
+    if (ok && save_dist) {
+        const hsize_t dims[2] = {n, n};
+        const hid_t t = fp64 ? *h.IEEE_F64LE_g : *h.IEEE_F32LE_g;
+        Handle sp{h.Screate_simple(2, dims, nullptr), h.Sclose};
+        Handle ds{h.Dcreate2(f.id, "matrix", t, sp.id, P_DEFAULT, P_DEFAULT, P_DEFAULT), h.Dclose};
+        ok = ds.id >= 0 && ((uint64_t)n * n == 0 || h.Dwrite(ds.id, t, S_ALL, S_ALL, P_DEFAULT, matrix) >= 0);
+    }
+    if (!ok && err) *err = std::string("HDF5 write failed for ") + filename;
+    return ok;
+}
 
 bool load_biom(const char* filename, Table& out, std::string* err) {
     H5& h = h5();

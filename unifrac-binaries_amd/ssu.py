@@ -78,10 +78,11 @@ EXPORTS = [
     "one_dense_pair_v2t", "one_dense_pair_v2", "faith_pd_one_off", "partial_v3", "partial", "write_mat",
     "write_mat_from_matrix", "write_mat_from_matrix_fp32", "write_vec", "write_partial", "read_partial",
     "read_partial_header", "read_partial_one_stripe", "validate_partial", "merge_partial_to_mmap_matrix",
-    "merge_partial_to_mmap_matrix_fp32", "ssu_faith_pd_inmem",
+    "merge_partial_to_mmap_matrix_fp32", "write_mat_from_matrix_hdf5_fp64", "write_mat_from_matrix_hdf5_fp32",
+    "ssu_faith_pd_inmem",
     "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
     "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
-    "ssu_newick_to_bp", "ssu_device_available", "ssu_last_error",
+    "ssu_newick_to_bp", "ssu_biom_n_samples", "ssu_device_available", "ssu_last_error",
 ]
 
 _lib = None
@@ -111,6 +112,9 @@ def lib():
         for n in ("write_partial", "read_partial", "read_partial_header", "read_partial_one_stripe",
                   "validate_partial", "merge_partial_to_mmap_matrix", "merge_partial_to_mmap_matrix_fp32"):
             getattr(L, n).restype = C.c_int
+        L.write_mat_from_matrix_hdf5_fp64.restype = C.c_int
+        L.write_mat_from_matrix_hdf5_fp32.restype = C.c_int
+        L.ssu_biom_n_samples.restype = C.c_long
         L.read_partial_one_stripe.argtypes = [C.POINTER(PartialDynMat), C.c_uint32]
         _lib = L
     return _lib
