@@ -371,3 +371,31 @@ def test_partial_files_merge_to_full_matrix(ssu, ref_trees, tmp_path):
     for d in mats:
         L.destroy_partial_dyn_mat(C.byref(d))
     assert np.array_equal(got, ref)
+
+
+BLOCKED = [("unweighted", {}, "fp32"), ("unweighted", {}, "fp64"), ("unweighted_unnormalized", {}, "fp32"),
+           ("weighted_normalized", {}, "fp32"), ("weighted_unnormalized", {}, "fp32"),
+           ("generalized", {"alpha": 0.5}, "fp32"), ("weighted_normalized", {"variance_adjust": True}, "fp32"),
+           ("unweighted", {"variance_adjust": True}, "fp32"), ("weighted_normalized", {"normalize": False}, "fp32")]
+
+
+@pytest.mark.parametrize("method,kw,prec", BLOCKED)
+def test_sample_blocked_embedding(ssu, O, synth, monkeypatch, method, kw, prec):
+    """The embedding built in 128-sample column blocks (how samples beyond the
+    device-memory budget are handled) gives the same matrices."""
+    tree, table = _case(synth, 400, 300, 0.05, 131, zipf=1.1)
+    monkeypatch.setenv("SSU_EMBED_BLOCK_SAMPLES", "128")
+    say("blocked", method, kw, prec)
+    s, m = ssu.matrix_inmem(inputs_from(ssu, tree, table), method + ("_fp64" if prec == "fp64" else ""), **kw)
+    assert s == ssu.OKAY, ssu.last_error()
+    ref = oracle_matrix(O, tree, table, method, **kw)
+    assert rel_err(m, ref) <= TOL[prec]
+
+
+def test_sample_blocked_faith(ssu, O, synth, monkeypatch):
+    tree, table = _case(synth, 400, 300, 0.05, 133, zipf=1.1)
+    monkeypatch.setenv("SSU_EMBED_BLOCK_SAMPLES", "128")
+    say("blocked faith")
+    s, vals, _ = ssu.faith_pd_inmem(inputs_from(ssu, tree, table))
+    st, le, nm = tree
+    assert s == ssu.OKAY and np.array_equal(vals, O.faith_pd(O.Tree(st.astype(np.uint8), le, nm), O.Table(*table)))

@@ -80,6 +80,7 @@ struct PlanImpl {
     ComputeSpec spec;
     uint32_t n = 0, m = 0;
     size_t ldn = 0;
+    size_t blk = 0;                  // sample columns per embedding block (ldn: one block)
     bool gram = false;
     bool sparse = false;             // weighted through the shared-node kernel
     bool bad_lengths = false;
@@ -115,7 +116,8 @@ struct PlanImpl {
     }
 
     bool build_gram_columns();
-    bool embed(DevMem& dst, bool normalize, std::string* err);
+    // node-major FP64 embedding of sample columns [c0, c0 + w) into dst (row stride w)
+    bool embed(DevMem& dst, bool normalize, size_t c0, size_t w, std::string* err);
 };
 
 // Exact Gram column order.  Every non-zero length is an integer times a power
@@ -191,15 +193,15 @@ bool PlanImpl::build_gram_columns() {
     return true;
 }
 
-bool PlanImpl::embed(DevMem& dst, bool normalize, std::string* err) {
-    if (!dst.ensure(std::max<size_t>(m, 1) * ldn * sizeof(double), err)) return false;
+bool PlanImpl::embed(DevMem& dst, bool normalize, size_t c0, size_t w, std::string* err) {
+    if (!dst.ensure(std::max<size_t>(m, 1) * blk * sizeof(double), err)) return false;
     double* E = dst.as<double>();
-    SSU_HIP(launch_embed_leaves(E, ldn, d_leaf_k.as<int32_t>(), d_leaf_row.as<int32_t>(), (uint32_t)leaf_k.size(),
-                                d_indptr.as<uint32_t>(), d_indices.as<uint32_t>(), d_data.as<double>(),
-                                d_totals.as<double>(), normalize, st));
+    SSU_HIP(launch_embed_leaves(E, w, (uint32_t)c0, d_leaf_k.as<int32_t>(), d_leaf_row.as<int32_t>(),
+                                (uint32_t)leaf_k.size(), d_indptr.as<uint32_t>(), d_indices.as<uint32_t>(),
+                                d_data.as<double>(), d_totals.as<double>(), normalize, st));
     for (size_t h = 0; h + 1 < level_off.size(); ++h) {
         const uint32_t a = level_off[h], b = level_off[h + 1];
-        SSU_HIP(launch_embed_level(E, ldn, d_level_nodes.as<int32_t>() + a, b - a, d_child_ptr.as<int32_t>(),
+        SSU_HIP(launch_embed_level(E, w, d_level_nodes.as<int32_t>() + a, b - a, d_child_ptr.as<int32_t>(),
                                    d_child_idx.as<int32_t>(), st));
     }
     return true;
@@ -277,6 +279,24 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
             if (!(table.data[b0 + q] >= 0.0)) P.sparse = false;
     }
 
+    // Embedding blocks: the FP64 embedding of every node for every sample
+    // (m x ldn doubles) is built in sample-column blocks when it would exceed
+    // EMBED_BUDGET; each block feeds the FP32 operand, masks, sums or presence
+    // bytes of its columns.  FP64 pair kernels read the embedding itself, so
+    // they keep a single block.  SSU_EMBED_BLOCK_SAMPLES overrides the width.
+    constexpr size_t EMBED_BUDGET = (size_t)48 << 30;
+    const bool need_full = spec.fp64 && !P.gram;
+    const size_t row_bytes = std::max<size_t>(P.m, 1) * sizeof(double) * (spec.vaw ? 2 : 1);
+    P.blk = P.ldn;
+    if (!need_full) {
+        if (const char* env = getenv("SSU_EMBED_BLOCK_SAMPLES")) {
+            const long v = atol(env);
+            if (v > 0) P.blk = std::min(P.ldn, round_up((size_t)v, (size_t)PAIR_TILE));
+        } else if (row_bytes * P.ldn > EMBED_BUDGET) {
+            P.blk = std::max<size_t>(PAIR_TILE, EMBED_BUDGET / row_bytes / PAIR_TILE * PAIR_TILE);
+        }
+    }
+
     std::string e2;
     if (!err) err = &e2;
     hipError_t he = hipStreamCreateWithFlags(&P.st, hipStreamNonBlocking);
@@ -318,57 +338,81 @@ bool Plan::execute(std::string* err) {
     const size_t osz = s.fp64 ? sizeof(double) : sizeof(float);
     if (!P.d_out.ensure(std::max<size_t>(nn, 1) * osz, err)) return false;
     const uint32_t K = (uint32_t)P.knodes.size();
+    const bool single = P.blk >= P.ldn;
+
+    // what the pair stage needs from each embedding block
+    PairOp op = PairOp::weighted;
+    bool normalized = true;
+    switch (s.method) {
+    case Method::unweighted: op = s.vaw ? PairOp::vaw_unweighted : PairOp::unweighted_plain; break;
+    case Method::unweighted_unnormalized: op = s.vaw ? PairOp::vaw_unweighted : PairOp::unweighted_plain; normalized = false; break;
+    case Method::weighted_normalized: op = s.vaw ? PairOp::vaw_weighted : PairOp::weighted; break;
+    case Method::weighted_unnormalized: op = s.vaw ? PairOp::vaw_weighted : PairOp::weighted; normalized = false; break;
+    case Method::generalized: op = s.vaw ? PairOp::vaw_generalized : PairOp::generalized; break;
+    }
+    const bool dense = !P.gram && !P.sparse;
+    // per-sample sums S: the shared-node kernel always, the dense kernel for weighted_normalized
+    const bool sums = P.sparse || (dense && op == PairOp::weighted && normalized);
+    const bool gather = !P.gram && !s.fp64;            // FP32 operand U (and Cu for VAW)
+    const uint32_t Kw = (K + 31) / 32;
+    if (sums) {
+        if (!P.d_S.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err)) return false;
+        const size_t w_last = P.ldn - (P.ldn - 1) / P.blk * P.blk;
+        const size_t parts = std::max<size_t>({(size_t)gather_segments(K, P.blk) * P.blk,
+                                               (size_t)gather_segments(K, w_last) * w_last, 1});
+        if (!P.d_Spart.ensure(parts * sizeof(double), err)) return false;
+    }
+    if (P.sparse && !P.d_masks.ensure(std::max<size_t>((P.ldn / 64) * Kw, 1) * sizeof(uint32_t), err)) return false;
+    if (gather) {
+        if (!P.d_U.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
+        if (s.vaw && !P.d_Cu.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
+    }
+    if (P.gram && P.Kp && !P.d_Pt.ensure(P.n_pad * P.Kp, err)) return false;
 
     SSU_HIP(hipEventRecord(P.ev[0], P.st));
-    if (!P.embed(P.d_E, s.normalize, err)) return false;
-    if (s.vaw && !P.embed(P.d_C, false, err)) return false;  // raw counts, src/unifrac_cmp.cpp:250-251
-    SSU_HIP(hipEventRecord(P.ev[1], P.st));
+    for (size_t c0 = 0; c0 < P.ldn; c0 += P.blk) {
+        const size_t w = std::min(P.blk, P.ldn - c0);
+        if (!P.embed(P.d_E, s.normalize, c0, w, err)) return false;
+        if (s.vaw && !P.embed(P.d_C, false, c0, w, err)) return false;  // raw counts, src/unifrac_cmp.cpp:250-251
+        if (single) SSU_HIP(hipEventRecord(P.ev[1], P.st));
+        if (P.gram) {
+            if (P.Kp)
+                SSU_HIP(launch_presence_T(P.d_Pt.as<int8_t>(), P.Kp, P.d_E.as<double>(), w, (uint32_t)c0,
+                                          P.d_col.as<int32_t>(), (uint32_t)P.Kp, P.n, P.st));
+        } else if (gather || sums) {
+            SSU_HIP(launch_gather_sums(gather ? P.d_U.as<float>() : nullptr, P.ldn, sums ? P.d_S.as<double>() : nullptr,
+                                       P.d_Spart.as<double>(), P.sparse ? P.d_masks.as<uint32_t>() : nullptr,
+                                       P.d_E.as<double>(), w, (uint32_t)c0, P.d_knodes.as<int32_t>(),
+                                       P.d_klen_sum.as<double>(), K, P.n, P.st));
+            if (gather && s.vaw)
+                SSU_HIP(launch_gather_sums(P.d_Cu.as<float>(), P.ldn, nullptr, nullptr, nullptr, P.d_C.as<double>(), w,
+                                           (uint32_t)c0, P.d_knodes.as<int32_t>(), nullptr, K, P.n, P.st));
+        }
+    }
+    if (!single) SSU_HIP(hipEventRecord(P.ev[1], P.st));   // blocked: embed and prep share one phase
+    SSU_HIP(hipEventRecord(P.ev[2], P.st));
 
     if (P.gram) {
-        const size_t Kp = P.Kp;
-        if (Kp) {
-            if (!P.d_Pt.ensure(P.n_pad * Kp, err)) return false;
-            SSU_HIP(launch_presence_T(P.d_Pt.as<int8_t>(), Kp, P.d_E.as<double>(), P.ldn, P.d_col.as<int32_t>(),
-                                      (uint32_t)Kp, (uint32_t)P.n_pad, P.n, P.st));
-        }
-        SSU_HIP(hipEventRecord(P.ev[2], P.st));
         if (!P.d_G.ensure(P.n_pad * P.n_pad * sizeof(double), err)) return false;
-        if (Kp) {
-            SSU_HIP(launch_gram_i8(P.d_G.as<double>(), P.d_Pt.as<int8_t>(), Kp, P.d_digits.as<int8_t>(),
+        if (P.Kp) {
+            SSU_HIP(launch_gram_i8(P.d_G.as<double>(), P.d_Pt.as<int8_t>(), P.Kp, P.d_digits.as<int8_t>(),
                                    P.d_chunks.as<GramChunk>(), (uint32_t)P.chunks.size(), (uint32_t)P.n_pad, P.st));
         } else {
             SSU_HIP(hipMemsetAsync(P.d_G.p, 0, P.n_pad * P.n_pad * sizeof(double), P.st));
         }
         SSU_HIP(hipEventRecord(P.ev[3], P.st));
-        const bool normalized = s.method == Method::unweighted;
+        const bool norm = s.method == Method::unweighted;
         if (s.fp64)
-            SSU_HIP(launch_unweighted_combine<double>(P.d_out.as<double>(), P.d_G.as<double>(), P.n_pad, P.n,
-                                                      normalized, P.st));
+            SSU_HIP(launch_unweighted_combine<double>(P.d_out.as<double>(), P.d_G.as<double>(), P.n_pad, P.n, norm,
+                                                      P.st));
         else
-            SSU_HIP(launch_unweighted_combine<float>(P.d_out.as<float>(), P.d_G.as<double>(), P.n_pad, P.n,
-                                                     normalized, P.st));
+            SSU_HIP(launch_unweighted_combine<float>(P.d_out.as<float>(), P.d_G.as<double>(), P.n_pad, P.n, norm,
+                                                     P.st));
         SSU_HIP(hipEventRecord(P.ev[4], P.st));
     } else if (P.sparse) {
-        const uint32_t Kw = (K + 31) / 32;
-        if (!P.d_S.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err)) return false;
-        if (!P.d_Spart.ensure(std::max<uint32_t>(gather_segments(K, P.ldn), 1) * P.ldn * sizeof(double), err))
-            return false;
-        if (!P.d_masks.ensure(std::max<size_t>((P.ldn / 64) * Kw, 1) * sizeof(uint32_t), err)) return false;
         SparseArgs a{};
-        if (s.fp64) {
-            SSU_HIP(launch_gather_sums(nullptr, P.d_S.as<double>(), P.d_Spart.as<double>(), P.d_masks.as<uint32_t>(),
-                                       P.d_E.as<double>(), P.ldn, P.d_knodes.as<int32_t>(), P.d_klen_sum.as<double>(), K,
-                                       P.n, P.st));
-            a.U = P.d_E.p;
-            a.rows = P.d_knodes.as<int32_t>();
-        } else {
-            if (!P.d_U.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
-            SSU_HIP(launch_gather_sums(P.d_U.as<float>(), P.d_S.as<double>(), P.d_Spart.as<double>(),
-                                       P.d_masks.as<uint32_t>(), P.d_E.as<double>(), P.ldn, P.d_knodes.as<int32_t>(),
-                                       P.d_klen_sum.as<double>(), K, P.n, P.st));
-            a.U = P.d_U.p;
-            a.rows = nullptr;
-        }
+        a.U = s.fp64 ? P.d_E.p : P.d_U.p;
+        a.rows = s.fp64 ? P.d_knodes.as<int32_t>() : nullptr;
         a.ldn = P.ldn;
         a.len = P.d_klen_sum.as<double>();
         a.masks = P.d_masks.as<uint32_t>();
@@ -381,7 +425,6 @@ bool Plan::execute(std::string* err) {
         const size_t nb = (P.n + 63) / 64;
         if (!P.d_visits.ensure(std::max<size_t>(nb * nb, 1) * sizeof(uint32_t), err)) return false;
         a.visits = P.d_visits.as<uint32_t>();
-        SSU_HIP(hipEventRecord(P.ev[2], P.st));
         SSU_HIP(launch_pairs_sparse(s.fp64, a, P.st));
         SSU_HIP(hipEventRecord(P.ev[3], P.st));
         SSU_HIP(hipEventRecord(P.ev[4], P.st));
@@ -395,47 +438,17 @@ bool Plan::execute(std::string* err) {
         a.out = P.d_out.p;
         a.out_fp64 = s.fp64;
         a.len = P.d_klen.p;
-        PairOp op = PairOp::weighted;
-        switch (s.method) {
-        case Method::unweighted: op = s.vaw ? PairOp::vaw_unweighted : PairOp::unweighted_plain; a.normalized = true; break;
-        case Method::unweighted_unnormalized: op = s.vaw ? PairOp::vaw_unweighted : PairOp::unweighted_plain; a.normalized = false; break;
-        case Method::weighted_normalized: op = s.vaw ? PairOp::vaw_weighted : PairOp::weighted; a.normalized = true; break;
-        case Method::weighted_unnormalized: op = s.vaw ? PairOp::vaw_weighted : PairOp::weighted; a.normalized = false; break;
-        case Method::generalized: op = s.vaw ? PairOp::vaw_generalized : PairOp::generalized; a.normalized = true; break;
-        }
-        // weighted_normalized denominators: sum_k l_k (u_i + u_j) = S_i + S_j
-        const bool sums = op == PairOp::weighted && a.normalized;
-        double* S = nullptr;
-        double* Spart = nullptr;
-        if (sums) {
-            if (!P.d_S.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err)) return false;
-            const size_t segs = std::max<uint32_t>(gather_segments(K, P.ldn), 1);
-            if (!P.d_Spart.ensure(segs * P.ldn * sizeof(double), err)) return false;
-            S = P.d_S.as<double>();
-            Spart = P.d_Spart.as<double>();
-            a.S = S;
-        }
+        a.normalized = normalized;
+        a.S = sums ? P.d_S.as<double>() : nullptr;
         if (s.fp64) {
             a.U = P.d_E.p;
             a.Cn = s.vaw ? P.d_C.p : nullptr;
             a.rows = P.d_knodes.as<int32_t>();
-            if (sums)
-                SSU_HIP(launch_gather_sums(nullptr, S, Spart, nullptr, P.d_E.as<double>(), P.ldn, P.d_knodes.as<int32_t>(),
-                                           P.d_klen_sum.as<double>(), K, P.n, P.st));
         } else {
-            if (!P.d_U.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
-            SSU_HIP(launch_gather_sums(P.d_U.as<float>(), S, Spart, nullptr, P.d_E.as<double>(), P.ldn,
-                                       P.d_knodes.as<int32_t>(), P.d_klen_sum.as<double>(), K, P.n, P.st));
             a.U = P.d_U.p;
-            if (s.vaw) {
-                if (!P.d_Cu.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
-                SSU_HIP(launch_gather_sums(P.d_Cu.as<float>(), nullptr, nullptr, nullptr, P.d_C.as<double>(), P.ldn,
-                                           P.d_knodes.as<int32_t>(), nullptr, K, P.n, P.st));
-                a.Cn = P.d_Cu.p;
-            }
+            a.Cn = s.vaw ? P.d_Cu.p : nullptr;
             a.rows = nullptr;
         }
-        SSU_HIP(hipEventRecord(P.ev[2], P.st));
         SSU_HIP(launch_pairs(op, s.fp64, a, P.st));
         SSU_HIP(hipEventRecord(P.ev[3], P.st));
         SSU_HIP(hipEventRecord(P.ev[4], P.st));
@@ -471,9 +484,13 @@ bool Plan::copy_stripes(uint32_t start, uint32_t stop, double* const* host, std:
 
 bool Plan::faith(double* host, std::string* err) {
     PlanImpl& P = *p_;
-    if (!P.embed(P.d_E, true, err)) return false;
     if (!P.d_faith.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err)) return false;
-    SSU_HIP(launch_faith(P.d_faith.as<double>(), P.d_E.as<double>(), P.ldn, P.d_len_all.as<double>(), P.m, P.n, P.st));
+    for (size_t c0 = 0; c0 < P.ldn && c0 < P.n; c0 += P.blk) {
+        const size_t w = std::min(P.blk, P.ldn - c0);
+        if (!P.embed(P.d_E, true, c0, w, err)) return false;
+        SSU_HIP(launch_faith(P.d_faith.as<double>() + c0, P.d_E.as<double>(), w, P.d_len_all.as<double>(), P.m,
+                             (uint32_t)std::min<size_t>(w, P.n - c0), P.st));
+    }
     SSU_HIP(hipStreamSynchronize(P.st));
     if (P.n) SSU_HIP(hipMemcpy(host, P.d_faith.p, P.n * sizeof(double), hipMemcpyDeviceToHost));
     return true;

@@ -24,8 +24,9 @@ inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b)
 // ------------------------------------------------------------------ embedding
 
 // One workgroup per leaf: clear the row, then scatter the leaf's CSR row
-// (src/biom_inmem.cpp:473-497: value / total, or the raw value).
-__global__ void __launch_bounds__(EMB_THREADS) k_embed_leaves(double* __restrict__ E, size_t ldn,
+// (src/biom_inmem.cpp:473-497: value / total, or the raw value).  E holds
+// the sample columns [col0, col0 + ld) of the embedding.
+__global__ void __launch_bounds__(EMB_THREADS) k_embed_leaves(double* __restrict__ E, size_t ld, uint32_t col0,
                                                               const int32_t* __restrict__ leaf_k,
                                                               const int32_t* __restrict__ leaf_row,
                                                               const uint32_t* __restrict__ indptr,
@@ -33,14 +34,15 @@ __global__ void __launch_bounds__(EMB_THREADS) k_embed_leaves(double* __restrict
                                                               const double* __restrict__ data,
                                                               const double* __restrict__ totals, int normalize) {
     const uint32_t li = blockIdx.x;
-    double* row = E + (size_t)leaf_k[li] * ldn;
-    for (size_t s = threadIdx.x; s < ldn; s += blockDim.x) row[s] = 0.0;
+    double* row = E + (size_t)leaf_k[li] * ld;
+    for (size_t s = threadIdx.x; s < ld; s += blockDim.x) row[s] = 0.0;
     __syncthreads();
     const int32_t r = leaf_row[li];
     if (r < 0) return;
     for (uint32_t j = indptr[r] + threadIdx.x; j < indptr[r + 1]; j += blockDim.x) {
         const uint32_t s = indices[j];
-        row[s] = normalize ? data[j] / totals[s] : data[j];
+        if (s < col0 || s - col0 >= ld) continue;
+        row[s - col0] = normalize ? data[j] / totals[s] : data[j];
     }
 }
 
@@ -67,26 +69,29 @@ __global__ void __launch_bounds__(EMB_THREADS) k_embed_level(double* __restrict_
 //           is e as the pair kernels see it ((float)e when GATHER, else e)
 //   MASKS:  bit (r % 32) of masks[panel][r / 32] = any e > 0 over the 64
 //           samples of the panel (one wavefront = one panel)
+// E holds the sample columns [col0, col0 + lde); U (row stride ldu) and the
+// masks are indexed by absolute sample, Spart by column within the block.
 template <bool GATHER, bool SUMS, bool MASKS>
-__global__ void __launch_bounds__(EMB_THREADS) k_gather_sums(float* __restrict__ U, double* __restrict__ Spart,
+__global__ void __launch_bounds__(EMB_THREADS) k_gather_sums(float* __restrict__ U, size_t ldu, uint32_t col0,
+                                                             double* __restrict__ Spart,
                                                              uint32_t* __restrict__ masks, uint32_t Kw,
-                                                             const double* __restrict__ E, size_t ldn,
+                                                             const double* __restrict__ E, size_t lde,
                                                              const int32_t* __restrict__ knodes,
                                                              const double* __restrict__ klen, uint32_t K,
                                                              uint32_t rows_per_seg) {
     const size_t s = (size_t)blockIdx.x * EMB_THREADS + threadIdx.x;
-    if (s >= ldn) return;
+    if (s >= lde) return;
     const uint32_t k0 = blockIdx.y * rows_per_seg;
     const uint32_t k1 = min(K, k0 + rows_per_seg);
-    const size_t panel = s / 64;
+    const size_t panel = (col0 + s) / 64;
     double acc = 0.0;
     uint32_t bits = 0;
 #pragma unroll 8
     for (uint32_t r = k0; r < k1; ++r) {
-        double e = E[(size_t)knodes[r] * ldn + s];
+        double e = E[(size_t)knodes[r] * lde + s];
         if constexpr (GATHER) {
             const float f = (float)e;
-            U[(size_t)r * ldn + s] = f;
+            U[(size_t)r * ldu + col0 + s] = f;
             e = (double)f;
         }
         if constexpr (SUMS) acc += klen[r] * e;
@@ -98,7 +103,7 @@ __global__ void __launch_bounds__(EMB_THREADS) k_gather_sums(float* __restrict__
             }
         }
     }
-    if constexpr (SUMS) Spart[(size_t)blockIdx.y * ldn + s] = acc;
+    if constexpr (SUMS) Spart[(size_t)blockIdx.y * lde + s] = acc;
 }
 
 __global__ void __launch_bounds__(EMB_THREADS) k_sum_segments(double* __restrict__ S, const double* __restrict__ Spart,
@@ -112,7 +117,8 @@ __global__ void __launch_bounds__(EMB_THREADS) k_sum_segments(double* __restrict
 
 // Faith PD: identical FP64 summation order to src/unifrac.cpp:373-386. The
 // chain of adds is inherently sequential per sample, so each thread issues
-// FAITH_BATCH independent loads ahead of the ordered adds.
+// FAITH_BATCH independent loads ahead of the ordered adds.  E holds n sample
+// columns (row stride ldn); out points at the first of them.
 constexpr uint32_t FAITH_BATCH = 16;
 __global__ void __launch_bounds__(64) k_faith(double* __restrict__ out, const double* __restrict__ E, size_t ldn,
                                               const double* __restrict__ len, uint32_t m, uint32_t n) {
@@ -135,8 +141,11 @@ __global__ void __launch_bounds__(64) k_faith(double* __restrict__ out, const do
 
 // 64 K'-columns x 64 samples per workgroup: read node rows (coalesced over
 // samples), write sample rows (coalesced over columns).
+// E holds the sample columns [col0, col0 + gridDim.y * 64) (row stride lde);
+// Pt rows are absolute samples.
 __global__ void __launch_bounds__(256) k_presence_T(int8_t* __restrict__ Pt, size_t ldp, const double* __restrict__ E,
-                                                    size_t ldn, const int32_t* __restrict__ col_node, uint32_t n) {
+                                                    size_t lde, uint32_t col0, const int32_t* __restrict__ col_node,
+                                                    uint32_t n) {
     __shared__ uint8_t tile[64][64 + 4];
     const uint32_t c0 = blockIdx.x * 64, s0 = blockIdx.y * 64;
     const uint32_t t = threadIdx.x;
@@ -145,7 +154,7 @@ __global__ void __launch_bounds__(256) k_presence_T(int8_t* __restrict__ Pt, siz
         const int32_t node = col_node[c0 + q];
         uint8_t v = 0;
         const uint32_t s = s0 + ls;
-        if (node >= 0 && s < n) v = E[(size_t)node * ldn + s] > 0.0 ? 0xFF : 0x00;
+        if (node >= 0 && col0 + s < n) v = E[(size_t)node * lde + s] > 0.0 ? 0xFF : 0x00;
         tile[q][ls] = v;
     }
     __syncthreads();
@@ -159,7 +168,7 @@ __global__ void __launch_bounds__(256) k_presence_T(int8_t* __restrict__ Pt, siz
         for (int b = 0; b < 4; ++b) x |= (uint32_t)tile[cb + q * 4 + b][row] << (8 * b);
         w[q] = x;
     }
-    *reinterpret_cast<uint4*>(Pt + (size_t)(s0 + row) * ldp + c0 + cb) = make_uint4(w[0], w[1], w[2], w[3]);
+    *reinterpret_cast<uint4*>(Pt + (size_t)(col0 + s0 + row) * ldp + c0 + cb) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -741,11 +750,11 @@ hipError_t launch_pairs_prec(PairOp op, const PairArgs& a, hipStream_t st) {
 
 // ------------------------------------------------------------------ launchers
 
-hipError_t launch_embed_leaves(double* E, size_t ldn, const int32_t* leaf_k, const int32_t* leaf_row, uint32_t n_leaves,
-                               const uint32_t* indptr, const uint32_t* indices, const double* data,
+hipError_t launch_embed_leaves(double* E, size_t ld, uint32_t col0, const int32_t* leaf_k, const int32_t* leaf_row,
+                               uint32_t n_leaves, const uint32_t* indptr, const uint32_t* indices, const double* data,
                                const double* totals, bool normalize, hipStream_t st) {
     if (!n_leaves) return hipSuccess;
-    hipLaunchKernelGGL(k_embed_leaves, dim3(n_leaves), dim3(EMB_THREADS), 0, st, E, ldn, leaf_k, leaf_row, indptr,
+    hipLaunchKernelGGL(k_embed_leaves, dim3(n_leaves), dim3(EMB_THREADS), 0, st, E, ld, col0, leaf_k, leaf_row, indptr,
                        indices, data, totals, (int)normalize);
     return hipGetLastError();
 }
@@ -769,20 +778,25 @@ uint32_t gather_segments(uint32_t K, size_t ldn) {
 }
 
 template <bool G, bool S, bool M>
-static void gather_sums_t(dim3 grid, hipStream_t st, float* U, double* Spart, uint32_t* masks, uint32_t Kw,
-                          const double* E, size_t ldn, const int32_t* knodes, const double* klen, uint32_t K,
-                          uint32_t rps) {
-    hipLaunchKernelGGL((k_gather_sums<G, S, M>), grid, dim3(EMB_THREADS), 0, st, U, Spart, masks, Kw, E, ldn, knodes,
-                       klen, K, rps);
+static void gather_sums_t(dim3 grid, hipStream_t st, float* U, size_t ldu, uint32_t col0, double* Spart,
+                          uint32_t* masks, uint32_t Kw, const double* E, size_t lde, const int32_t* knodes,
+                          const double* klen, uint32_t K, uint32_t rps) {
+    hipLaunchKernelGGL((k_gather_sums<G, S, M>), grid, dim3(EMB_THREADS), 0, st, U, ldu, col0, Spart, masks, Kw, E,
+                       lde, knodes, klen, K, rps);
 }
 
-hipError_t launch_gather_sums(float* U, double* S, double* Spart, uint32_t* masks, const double* E, size_t ldn,
-                              const int32_t* knodes, const double* klen, uint32_t K, uint32_t n, hipStream_t st) {
-    if (masks && ldn % 64) return hipErrorInvalidValue;
+hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, uint32_t* masks, const double* E,
+                              size_t lde, uint32_t col0, const int32_t* knodes, const double* klen, uint32_t K,
+                              uint32_t n, hipStream_t st) {
+    if (masks && (lde % 64 || col0 % 64)) return hipErrorInvalidValue;
+    if (col0 >= n) return hipSuccess;
+    const uint32_t cnt = (uint32_t)std::min<size_t>(lde, n - col0);   // real samples in this block
     if (!K) {
-        if (S && n) hipLaunchKernelGGL(k_sum_segments, dim3(grid1(n, EMB_THREADS)), dim3(EMB_THREADS), 0, st, S, Spart, ldn, 0u, n);
+        if (S) hipLaunchKernelGGL(k_sum_segments, dim3(grid1(cnt, EMB_THREADS)), dim3(EMB_THREADS), 0, st, S + col0,
+                                  Spart, lde, 0u, cnt);
         return hipGetLastError();
     }
+    const size_t ldn = lde;
     const uint32_t cb = grid1(ldn, EMB_THREADS);
     const uint32_t segs0 = gather_segments(K, ldn);
     const uint32_t rps = (((K + segs0 - 1) / segs0) + 31) / 32 * 32;   // mask words never straddle segments
@@ -791,15 +805,16 @@ hipError_t launch_gather_sums(float* U, double* S, double* Spart, uint32_t* mask
     const dim3 grid(cb, segs);
     const int key = (U ? 4 : 0) | (S ? 2 : 0) | (masks ? 1 : 0);
     switch (key) {
-    case 7: gather_sums_t<true, true, true>(grid, st, U, Spart, masks, Kw, E, ldn, knodes, klen, K, rps); break;
-    case 6: gather_sums_t<true, true, false>(grid, st, U, Spart, masks, Kw, E, ldn, knodes, klen, K, rps); break;
-    case 4: gather_sums_t<true, false, false>(grid, st, U, Spart, masks, Kw, E, ldn, knodes, klen, K, rps); break;
-    case 3: gather_sums_t<false, true, true>(grid, st, U, Spart, masks, Kw, E, ldn, knodes, klen, K, rps); break;
-    case 2: gather_sums_t<false, true, false>(grid, st, U, Spart, masks, Kw, E, ldn, knodes, klen, K, rps); break;
+    case 7: gather_sums_t<true, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, E, lde, knodes, klen, K, rps); break;
+    case 6: gather_sums_t<true, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, E, lde, knodes, klen, K, rps); break;
+    case 4: gather_sums_t<true, false, false>(grid, st, U, ldu, col0, Spart, masks, Kw, E, lde, knodes, klen, K, rps); break;
+    case 3: gather_sums_t<false, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, E, lde, knodes, klen, K, rps); break;
+    case 2: gather_sums_t<false, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, E, lde, knodes, klen, K, rps); break;
     case 0: return hipSuccess;
     default: return hipErrorInvalidValue;
     }
-    if (S) hipLaunchKernelGGL(k_sum_segments, dim3(grid1(n, EMB_THREADS)), dim3(EMB_THREADS), 0, st, S, Spart, ldn, segs, n);
+    if (S) hipLaunchKernelGGL(k_sum_segments, dim3(grid1(cnt, EMB_THREADS)), dim3(EMB_THREADS), 0, st, S + col0, Spart,
+                              lde, segs, cnt);
     return hipGetLastError();
 }
 
@@ -810,10 +825,11 @@ hipError_t launch_faith(double* out, const double* E, size_t ldn, const double* 
     return hipGetLastError();
 }
 
-hipError_t launch_presence_T(int8_t* Pt, size_t ldp, const double* E, size_t ldn, const int32_t* col_node,
-                             uint32_t Kp, uint32_t n_pad, uint32_t n, hipStream_t st) {
-    if (Kp % 64 || n_pad % 64) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_presence_T, dim3(Kp / 64, n_pad / 64), dim3(256), 0, st, Pt, ldp, E, ldn, col_node, n);
+hipError_t launch_presence_T(int8_t* Pt, size_t ldp, const double* E, size_t lde, uint32_t col0, const int32_t* col_node,
+                             uint32_t Kp, uint32_t n, hipStream_t st) {
+    if (Kp % 64 || lde % 64 || col0 % 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_presence_T, dim3(Kp / 64, (uint32_t)(lde / 64)), dim3(256), 0, st, Pt, ldp, E, lde, col0,
+                       col_node, n);
     return hipGetLastError();
 }
 

@@ -23,30 +23,34 @@ constexpr int GRAM_BK = 128;   // K' columns per LDS stage
 constexpr int PAIR_TILE = 128; // samples per side of a VALU pair tile
 
 // ---------------------------------------------------------------- embedding
-hipError_t launch_embed_leaves(double* E, size_t ldn, const int32_t* leaf_k, const int32_t* leaf_row, uint32_t n_leaves,
-                               const uint32_t* indptr, const uint32_t* indices, const double* data,
+// E holds the sample columns [col0, col0 + ld) of the node-major embedding
+hipError_t launch_embed_leaves(double* E, size_t ld, uint32_t col0, const int32_t* leaf_k, const int32_t* leaf_row,
+                               uint32_t n_leaves, const uint32_t* indptr, const uint32_t* indices, const double* data,
                                const double* totals, bool normalize, hipStream_t st);
 hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint32_t n_nodes, const int32_t* child_ptr,
                               const int32_t* child_idx, hipStream_t st);
 
-// One segmented pass over the processed rows r = 0..K-1 (row r is E[knodes[r]]):
-//   U (if non-null): node-major FP32 copy, U[r][s] = (float)E[knodes[r]][s];
+// One segmented pass over the processed rows r = 0..K-1 (row r is E[knodes[r]])
+// of an embedding block holding sample columns [col0, col0 + lde):
+//   U (if non-null): node-major FP32 copy, U[r][col0 + s] = (float)E[knodes[r]][s] (row stride ldu);
 //   S (if non-null): S[s] = sum_r klen[r] * u_rs in FP64 (u as the pair kernels
 //     see it: FP32-rounded when U is written), summed per segment then over
-//     segments in order; Spart holds gather_segments(K, ldn) * ldn doubles;
-//   masks (if non-null; ldn % 64 == 0): [ldn/64][ceil(K/32)] words, bit r%32 of
+//     segments in order, into S[col0 ..]; Spart holds gather_segments(K, lde) * lde doubles;
+//   masks (if non-null; lde, col0 multiples of 64): [panels][ceil(K/32)] words, bit r%32 of
 //     word r/32 of panel p set iff some sample of p*64..p*64+63 has u_rs > 0.
 uint32_t gather_segments(uint32_t K, size_t ldn);
-hipError_t launch_gather_sums(float* U, double* S, double* Spart, uint32_t* masks, const double* E, size_t ldn,
-                              const int32_t* knodes, const double* klen, uint32_t K, uint32_t n, hipStream_t st);
+hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, uint32_t* masks, const double* E,
+                              size_t lde, uint32_t col0, const int32_t* knodes, const double* klen, uint32_t K,
+                              uint32_t n, hipStream_t st);
 // Faith PD: out[s] = sum over all non-root nodes k in postorder of (E[k][s] > 0) * len[k]
 hipError_t launch_faith(double* out, const double* E, size_t ldn, const double* len, uint32_t m, uint32_t n,
                         hipStream_t st);
 
 // ---------------------------------------------------------------- Gram path
 // Pt[s][c] = 0xFF if E[col_node[c]][s] > 0 else 0 (col_node < 0 -> padding column)
-hipError_t launch_presence_T(int8_t* Pt, size_t ldp, const double* E, size_t ldn, const int32_t* col_node,
-                             uint32_t Kp, uint32_t n_pad, uint32_t n, hipStream_t st);
+// for the embedding block of sample columns [col0, col0 + lde): Pt rows col0 ..
+hipError_t launch_presence_T(int8_t* Pt, size_t ldp, const double* E, size_t lde, uint32_t col0, const int32_t* col_node,
+                             uint32_t Kp, uint32_t n, hipStream_t st);
 // G = sum_c len_c * P[i][c] * P[j][c] for the upper-triangle 128x128 tiles, FP64,
 // stored tile by tile in MFMA fragment order (n_pad/128)^2 * 128^2 doubles
 hipError_t launch_gram_i8(double* G, const int8_t* Pt, size_t ldp, const int8_t* digits, const GramChunk* chunks,
