@@ -583,10 +583,11 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
 // with S the per-sample sums.  min(u, v) is zero unless both samples hold the
 // node, so a 64x64 block of pairs only visits the nodes present in both of its
 // sample panels: the AND of the two panels' node masks.  One wavefront owns one
-// block; each lane keeps an 8x8 register tile of FP64 accumulators.
+// block; each lane keeps an 8x8 register tile of FP64 accumulators (v_min_f64
+// and v_fma_f64 issue at the same rate as their FP32 forms on this GPU).
 
 // v_min_f64 without the NaN-quieting canonicalisation the compiler adds for
-// operands it cannot see produced (here: LDS loads of converted values)
+// operands it cannot see produced (here: LDS loads of staged values)
 __device__ __forceinline__ double vmin_f64(double a, double b) {
     double r;
     asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
@@ -596,6 +597,11 @@ __device__ __forceinline__ double vmin_f64(double a, double b) {
 constexpr uint32_t SP_B = 64;        // block edge in samples
 constexpr uint32_t SP_EB = 8;        // nodes staged per step
 constexpr uint32_t SP_WORDS = 64;    // mask words (32 nodes each) per list chunk
+// Staging: the 8 values a lane reads for its rows (or columns) are contiguous;
+// groups of 8 sit SP_GS doubles (80 bytes) apart so that the 8 distinct
+// 16-byte reads of one ds_read_b128 pass fall in different LDS banks.
+constexpr uint32_t SP_GS = 10;
+constexpr uint32_t SP_HALF = 8 * SP_GS;   // offset of the column panel in a staged node
 
 template <class TIn, class TOut>
 __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, const int32_t* __restrict__ rows,
@@ -610,7 +616,7 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
     const uint32_t li = lane >> 3, lj = lane & 7;
 
     __shared__ uint16_t list[SP_WORDS * 32];
-    __shared__ __attribute__((aligned(16))) double sv[SP_EB][2 * SP_B];
+    __shared__ __attribute__((aligned(16))) double sv[SP_EB][2 * SP_HALF];
     __shared__ double sl[SP_EB];
 
     double acc[8][8];
@@ -622,6 +628,7 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
     const uint32_t* mI = masks + (size_t)bi * Kw;
     const uint32_t* mJ = masks + (size_t)bj * Kw;
     const size_t offI = (size_t)bi * SP_B + lane, offJ = (size_t)bj * SP_B + lane;
+    const uint32_t slot = (lane >> 3) * SP_GS + (lane & 7);   // where this lane stages its sample
 
     for (uint32_t w0 = 0; w0 < Kw; w0 += SP_WORDS) {
         // active nodes of this chunk, in increasing order, as offsets from w0*32
@@ -673,8 +680,8 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
         for (uint32_t t0 = 0; t0 < total; t0 += SP_EB) {
 #pragma unroll
             for (uint32_t e = 0; e < SP_EB; ++e) {
-                sv[e][lane] = (double)pi[e];
-                sv[e][SP_B + lane] = (double)pj[e];
+                sv[e][slot] = (double)pi[e];
+                sv[e][SP_HALF + slot] = (double)pj[e];
             }
             if (lane < SP_EB) sl[lane] = pl;
             __syncthreads();
@@ -683,9 +690,9 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
             for (uint32_t e = 0; e < SP_EB; ++e) {
                 double a[8], b[8];
 #pragma unroll
-                for (int r = 0; r < 8; ++r) a[r] = sv[e][li * 8 + r];
+                for (int r = 0; r < 8; ++r) a[r] = sv[e][li * SP_GS + r];
 #pragma unroll
-                for (int c = 0; c < 8; ++c) b[c] = sv[e][SP_B + lj * 8 + c];
+                for (int c = 0; c < 8; ++c) b[c] = sv[e][SP_HALF + lj * SP_GS + c];
                 const double l = sl[e];
 #pragma unroll
                 for (int r = 0; r < 8; ++r)
