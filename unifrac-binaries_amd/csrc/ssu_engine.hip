@@ -83,6 +83,7 @@ struct PlanImpl {
     size_t blk = 0;                  // sample columns per embedding block (ldn: one block)
     bool gram = false;
     bool sparse = false;             // weighted through the shared-node kernel
+    std::vector<uint32_t> block_order;  // its 64x64 blocks in launch order
     bool bad_lengths = false;
 
     // host schedule
@@ -106,7 +107,7 @@ struct PlanImpl {
     std::vector<double> totals;
 
     DevMem d_indptr, d_indices, d_data, d_totals, d_leaf_k, d_leaf_row, d_level_nodes, d_child_ptr, d_child_idx;
-    DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
+    DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_order, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
         d_faith, d_stripes;
 
     ~PlanImpl() {
@@ -323,6 +324,10 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         for (size_t q = 0; q < ls.size(); ++q) ls[q] = spec.fp64 ? P.klen_d[q] : (double)P.klen_f[q];
         ok = upload(P.d_klen_sum, ls, P.st, err);
     }
+    if (ok && P.sparse) {
+        P.block_order = sparse_block_order(P.n);
+        ok = upload(P.d_order, P.block_order, P.st, err);
+    }
     if (ok && P.gram && P.Kp)
         ok = upload(P.d_col, P.col_node, P.st, err) && upload(P.d_digits, P.digits, P.st, err) &&
              upload(P.d_chunks, P.chunks, P.st, err);
@@ -425,6 +430,8 @@ bool Plan::execute(std::string* err) {
         const size_t nb = (P.n + 63) / 64;
         if (!P.d_visits.ensure(std::max<size_t>(nb * nb, 1) * sizeof(uint32_t), err)) return false;
         a.visits = P.d_visits.as<uint32_t>();
+        a.order = P.d_order.as<uint32_t>();
+        a.n_blocks = (uint32_t)P.block_order.size();
         SSU_HIP(launch_pairs_sparse(s.fp64, a, P.st));
         SSU_HIP(hipEventRecord(P.ev[3], P.st));
         SSU_HIP(hipEventRecord(P.ev[4], P.st));

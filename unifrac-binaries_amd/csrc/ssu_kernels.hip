@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <vector>
 #include <cmath>
 
 namespace ssu {
@@ -595,29 +596,41 @@ __device__ __forceinline__ double vmin_f64(double a, double b) {
 }
 
 constexpr uint32_t SP_B = 64;        // block edge in samples
-constexpr uint32_t SP_EB = 8;        // nodes staged per step
 constexpr uint32_t SP_WORDS = 64;    // mask words (32 nodes each) per list chunk
 // Staging: the 8 values a lane reads for its rows (or columns) are contiguous;
 // groups of 8 sit SP_GS doubles (80 bytes) apart so that the 8 distinct
 // 16-byte reads of one ds_read_b128 pass fall in different LDS banks.
 constexpr uint32_t SP_GS = 10;
 constexpr uint32_t SP_HALF = 8 * SP_GS;   // offset of the column panel in a staged node
+// nodes per step; two steps are in flight in registers while one is consumed
+template <class TIn> struct SpStep { static constexpr uint32_t EB = sizeof(TIn) == 4 ? 8 : 4; };
+
+// Blocks are numbered so that each XCD (workgroups go to the 8 XCDs round
+// robin) walks a contiguous run of `order`, which lists the upper-triangle
+// blocks super-tile by super-tile: the blocks resident on one XCD then share
+// their sample panels in that XCD's L2.
+constexpr uint32_t SP_XCDS = 8;
 
 template <class TIn, class TOut>
 __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, const int32_t* __restrict__ rows,
                                                      size_t ldn, const double* __restrict__ len,
                                                      const uint32_t* __restrict__ masks, uint32_t Kw, uint32_t n,
                                                      const double* __restrict__ S, int normalized,
-                                                     TOut* __restrict__ out, uint32_t* __restrict__ visits) {
-    const uint32_t bi = blockIdx.y, bj = blockIdx.x;
-    if (bi > bj) return;
+                                                     TOut* __restrict__ out, uint32_t* __restrict__ visits,
+                                                     const uint32_t* __restrict__ order, uint32_t n_blocks,
+                                                     uint32_t per_xcd) {
+    constexpr uint32_t EB = SpStep<TIn>::EB;
+    const uint32_t seq = (blockIdx.x % SP_XCDS) * per_xcd + blockIdx.x / SP_XCDS;
+    if (seq >= n_blocks) return;
+    const uint32_t bij = order[seq];
+    const uint32_t bi = bij >> 16, bj = bij & 0xffffu;
     uint32_t visited = 0;
     const uint32_t lane = threadIdx.x;
     const uint32_t li = lane >> 3, lj = lane & 7;
 
     __shared__ uint16_t list[SP_WORDS * 32];
-    __shared__ __attribute__((aligned(16))) double sv[SP_EB][2 * SP_HALF];
-    __shared__ double sl[SP_EB];
+    __shared__ __attribute__((aligned(16))) double sv[EB][2 * SP_HALF];
+    __shared__ double sl[EB];
 
     double acc[8][8];
 #pragma unroll
@@ -651,59 +664,64 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
         }
         __syncthreads();
         const uint32_t base = w0 * 32;
-        // the next step's rows are loaded into registers while the current
-        // step is consumed from LDS; lane e < SP_EB resolves node t0 + e
-        TIn pi[SP_EB], pj[SP_EB];
-        double pl = 0.0;
-        auto fetch = [&](uint32_t t0) __attribute__((always_inline)) {
+        // lane e < EB resolves node t0 + e; its panel rows land in registers
+        TIn pA[EB], qA[EB], pB[EB], qB[EB];
+        double lA = 0.0, lB = 0.0;
+        auto fetch = [&](TIn (&p)[EB], TIn (&q)[EB], double& l, uint32_t t0) __attribute__((always_inline)) {
             const uint32_t idx = t0 + lane;
             int32_t row = -1;
-            pl = 0.0;
-            if (lane < SP_EB && idx < total) {
+            l = 0.0;
+            if (lane < EB && idx < total) {
                 const uint32_t r = base + list[idx];
                 row = rows ? rows[r] : (int32_t)r;
-                pl = len[r];
+                l = len[r];
             }
 #pragma unroll
-            for (uint32_t e = 0; e < SP_EB; ++e) {
+            for (uint32_t e = 0; e < EB; ++e) {
                 const int32_t re = __builtin_amdgcn_readlane(row, e);
-                pi[e] = TIn(0);
-                pj[e] = TIn(0);
+                p[e] = TIn(0);
+                q[e] = TIn(0);
                 if (re >= 0) {
                     const size_t off = (size_t)re * ldn;
-                    pi[e] = U[off + offI];
-                    pj[e] = U[off + offJ];
+                    p[e] = U[off + offI];
+                    q[e] = U[off + offJ];
                 }
             }
         };
-        if (total) fetch(0);
-        for (uint32_t t0 = 0; t0 < total; t0 += SP_EB) {
+        auto step = [&](TIn (&p)[EB], TIn (&q)[EB], double& l, uint32_t t_next) __attribute__((always_inline)) {
 #pragma unroll
-            for (uint32_t e = 0; e < SP_EB; ++e) {
-                sv[e][slot] = (double)pi[e];
-                sv[e][SP_HALF + slot] = (double)pj[e];
+            for (uint32_t e = 0; e < EB; ++e) {
+                sv[e][slot] = (double)p[e];
+                sv[e][SP_HALF + slot] = (double)q[e];
             }
-            if (lane < SP_EB) sl[lane] = pl;
+            if (lane < EB) sl[lane] = l;
             __syncthreads();
-            if (t0 + SP_EB < total) fetch(t0 + SP_EB);
+            if (t_next < total) fetch(p, q, l, t_next);   // two steps ahead of its use
 #pragma unroll 2
-            for (uint32_t e = 0; e < SP_EB; ++e) {
+            for (uint32_t e = 0; e < EB; ++e) {
                 double a[8], b[8];
 #pragma unroll
                 for (int r = 0; r < 8; ++r) a[r] = sv[e][li * SP_GS + r];
 #pragma unroll
                 for (int c = 0; c < 8; ++c) b[c] = sv[e][SP_HALF + lj * SP_GS + c];
-                const double l = sl[e];
+                const double ln = sl[e];
 #pragma unroll
                 for (int r = 0; r < 8; ++r)
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) acc[r][c] = __builtin_fma(vmin_f64(a[r], b[c]), l, acc[r][c]);
+                    for (int c = 0; c < 8; ++c) acc[r][c] = __builtin_fma(vmin_f64(a[r], b[c]), ln, acc[r][c]);
             }
             __syncthreads();
+        };
+        if (total) fetch(pA, qA, lA, 0);
+        if (total > EB) fetch(pB, qB, lB, EB);
+        for (uint32_t t0 = 0; t0 < total; t0 += 2 * EB) {
+            step(pA, qA, lA, t0 + 2 * EB);
+            if (t0 + EB >= total) break;
+            step(pB, qB, lB, t0 + 3 * EB);
         }
     }
 
-    if (visits && lane == 0) visits[(size_t)bi * gridDim.x + bj] = visited;
+    if (visits && lane == 0) visits[(size_t)bi * ((n + SP_B - 1) / SP_B) + bj] = visited;
     const size_t i0 = (size_t)bi * SP_B + li * 8, j0 = (size_t)bj * SP_B + lj * 8;
 #pragma unroll
     for (int r = 0; r < 8; ++r)
@@ -864,23 +882,38 @@ hipError_t launch_pairs(PairOp op, bool fp64, const PairArgs& a, hipStream_t st)
 }
 
 template <class TIn, class TOut>
-static void pairs_sparse_t(dim3 grid, hipStream_t st, const SparseArgs& a) {
-    hipLaunchKernelGGL((k_pairs_sparse<TIn, TOut>), grid, dim3(64), 0, st, (const TIn*)a.U, a.rows, a.ldn, a.len,
-                       a.masks, a.Kw, a.n, a.S, (int)a.normalized, (TOut*)a.out, a.visits);
+static void pairs_sparse_t(uint32_t grid, hipStream_t st, const SparseArgs& a, uint32_t per_xcd) {
+    hipLaunchKernelGGL((k_pairs_sparse<TIn, TOut>), dim3(grid), dim3(64), 0, st, (const TIn*)a.U, a.rows, a.ldn, a.len,
+                       a.masks, a.Kw, a.n, a.S, (int)a.normalized, (TOut*)a.out, a.visits, a.order, a.n_blocks,
+                       per_xcd);
+}
+
+std::vector<uint32_t> sparse_block_order(uint32_t n) {
+    constexpr uint32_t ST = 16;   // super-tile edge in blocks: 256 blocks ~ the waves resident on one XCD
+    const uint32_t nb = (n + SP_B - 1) / SP_B;
+    std::vector<uint32_t> order;
+    order.reserve((size_t)nb * (nb + 1) / 2);
+    for (uint32_t si = 0; si < nb; si += ST)
+        for (uint32_t sj = si; sj < nb; sj += ST)
+            for (uint32_t bi = si; bi < std::min(nb, si + ST); ++bi)
+                for (uint32_t bj = std::max(bi, sj); bj < std::min(nb, sj + ST); ++bj) order.push_back(bi << 16 | bj);
+    return order;
 }
 
 hipError_t launch_pairs_sparse(bool in_fp64, const SparseArgs& a, hipStream_t st) {
     if (a.ldn % SP_B) return hipErrorInvalidValue;
     const uint32_t nb = (a.n + SP_B - 1) / SP_B;
     if (!nb) return hipSuccess;
-    if ((size_t)nb * SP_B > a.ldn) return hipErrorInvalidValue;
-    const dim3 grid(nb, nb);
+    if ((size_t)nb * SP_B > a.ldn || nb > 65535 || a.n_blocks != nb * (nb + 1) / 2 || !a.order)
+        return hipErrorInvalidValue;
+    const uint32_t per_xcd = (a.n_blocks + SP_XCDS - 1) / SP_XCDS;
+    const uint32_t grid = per_xcd * SP_XCDS;
     if (in_fp64) {
-        if (a.out_fp64) pairs_sparse_t<double, double>(grid, st, a);
-        else pairs_sparse_t<double, float>(grid, st, a);
+        if (a.out_fp64) pairs_sparse_t<double, double>(grid, st, a, per_xcd);
+        else pairs_sparse_t<double, float>(grid, st, a, per_xcd);
     } else {
-        if (a.out_fp64) pairs_sparse_t<float, double>(grid, st, a);
-        else pairs_sparse_t<float, float>(grid, st, a);
+        if (a.out_fp64) pairs_sparse_t<float, double>(grid, st, a, per_xcd);
+        else pairs_sparse_t<float, float>(grid, st, a, per_xcd);
     }
     return hipGetLastError();
 }

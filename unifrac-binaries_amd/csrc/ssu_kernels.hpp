@@ -3,6 +3,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <vector>
 #include <cstddef>
 #include <cstdint>
 
@@ -102,7 +104,11 @@ struct SparseArgs {
     void* out;              // n x n
     bool out_fp64;
     uint32_t* visits;       // optional [nb][nb], nb = ceil(n/64): nodes visited per block (bi <= bj)
+    const uint32_t* order;  // device copy of sparse_block_order(n)
+    uint32_t n_blocks;      // its length, nb (nb + 1) / 2
 };
+// upper-triangle 64x64 blocks (bi << 16 | bj), super-tile by super-tile
+std::vector<uint32_t> sparse_block_order(uint32_t n);
 hipError_t launch_pairs_sparse(bool in_fp64, const SparseArgs& a, hipStream_t st);
 
 // --------------------------------------------------------- output layouts
