@@ -83,7 +83,7 @@ struct PlanImpl {
     size_t blk = 0;                  // sample columns per embedding block (ldn: one block)
     bool gram = false;
     bool sparse = false;             // weighted through the shared-node kernel
-    std::vector<uint32_t> block_order;  // its 64x64 blocks in launch order
+    std::vector<uint32_t> block_order;  // pair blocks / Gram tiles in launch order
     bool bad_lengths = false;
 
     // host schedule
@@ -324,8 +324,14 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         for (size_t q = 0; q < ls.size(); ++q) ls[q] = spec.fp64 ? P.klen_d[q] : (double)P.klen_f[q];
         ok = upload(P.d_klen_sum, ls, P.st, err);
     }
+    // 16 x 16 super-tiles of 64-sample blocks, 6 x 6 of 128-sample Gram tiles:
+    // about the number of blocks (tiles) resident on one XCD
     if (ok && P.sparse) {
-        P.block_order = sparse_block_order(P.n);
+        P.block_order = upper_block_order((P.n + 63) / 64, 16);
+        ok = upload(P.d_order, P.block_order, P.st, err);
+    }
+    if (ok && P.gram) {
+        P.block_order = upper_block_order((uint32_t)(P.n_pad / GRAM_BM), 6);
         ok = upload(P.d_order, P.block_order, P.st, err);
     }
     if (ok && P.gram && P.Kp)
@@ -401,7 +407,8 @@ bool Plan::execute(std::string* err) {
         if (!P.d_G.ensure(P.n_pad * P.n_pad * sizeof(double), err)) return false;
         if (P.Kp) {
             SSU_HIP(launch_gram_i8(P.d_G.as<double>(), P.d_Pt.as<int8_t>(), P.Kp, P.d_digits.as<int8_t>(),
-                                   P.d_chunks.as<GramChunk>(), (uint32_t)P.chunks.size(), (uint32_t)P.n_pad, P.st));
+                                   P.d_chunks.as<GramChunk>(), (uint32_t)P.chunks.size(), (uint32_t)P.n_pad,
+                                   P.d_order.as<uint32_t>(), (uint32_t)P.block_order.size(), P.st));
         } else {
             SSU_HIP(hipMemsetAsync(P.d_G.p, 0, P.n_pad * P.n_pad * sizeof(double), P.st));
         }

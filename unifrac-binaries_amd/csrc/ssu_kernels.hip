@@ -19,6 +19,7 @@ namespace ssu {
 namespace {
 
 constexpr int EMB_THREADS = 256;
+constexpr uint32_t N_XCDS = 8;   // MI355X: 8 XCDs, each with its own L2
 
 inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
@@ -207,9 +208,12 @@ constexpr int GRAM_THREADS = 512;
 // first, then the p = 0 waves, separated by a barrier.
 __global__ void __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_gram_i8(double* __restrict__ G, uint32_t tiles, const int8_t* __restrict__ Pt, size_t ldp,
-          const int8_t* __restrict__ digits, const GramChunk* __restrict__ chunks, uint32_t n_chunks, uint32_t n_stages) {
-    const uint32_t bi = blockIdx.y, bj = blockIdx.x;
-    if (bi > bj) return;
+          const int8_t* __restrict__ digits, const GramChunk* __restrict__ chunks, uint32_t n_chunks, uint32_t n_stages,
+          const uint32_t* __restrict__ order, uint32_t n_tiles, uint32_t per_xcd) {
+    // tiles in XCD-contiguous runs of a super-tile order (see upper_block_order)
+    const uint32_t seq = (blockIdx.x % N_XCDS) * per_xcd + blockIdx.x / N_XCDS;
+    if (seq >= n_tiles) return;
+    const uint32_t bi = order[seq] >> 16, bj = order[seq] & 0xffffu;
     __shared__ __attribute__((aligned(16))) int8_t sA[GRAM_BM * GLDA];
     __shared__ __attribute__((aligned(16))) int8_t sB[GRAM_BM * GLDA];
     __shared__ __attribute__((aligned(16))) int8_t sD[4 * GRAM_BK];
@@ -605,11 +609,9 @@ constexpr uint32_t SP_HALF = 8 * SP_GS;   // offset of the column panel in a sta
 // nodes per step; two steps are in flight in registers while one is consumed
 template <class TIn> struct SpStep { static constexpr uint32_t EB = sizeof(TIn) == 4 ? 8 : 4; };
 
-// Blocks are numbered so that each XCD (workgroups go to the 8 XCDs round
-// robin) walks a contiguous run of `order`, which lists the upper-triangle
-// blocks super-tile by super-tile: the blocks resident on one XCD then share
-// their sample panels in that XCD's L2.
-constexpr uint32_t SP_XCDS = 8;
+// Blocks are numbered so that each XCD (workgroups go to the XCDs round robin)
+// walks a contiguous run of `order` (upper_block_order): the blocks resident
+// on one XCD then share their sample panels in that XCD's L2.
 
 template <class TIn, class TOut>
 __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, const int32_t* __restrict__ rows,
@@ -620,7 +622,7 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
                                                      const uint32_t* __restrict__ order, uint32_t n_blocks,
                                                      uint32_t per_xcd) {
     constexpr uint32_t EB = SpStep<TIn>::EB;
-    const uint32_t seq = (blockIdx.x % SP_XCDS) * per_xcd + blockIdx.x / SP_XCDS;
+    const uint32_t seq = (blockIdx.x % N_XCDS) * per_xcd + blockIdx.x / N_XCDS;
     if (seq >= n_blocks) return;
     const uint32_t bij = order[seq];
     const uint32_t bi = bij >> 16, bj = bij & 0xffffu;
@@ -859,11 +861,13 @@ hipError_t launch_presence_T(int8_t* Pt, size_t ldp, const double* E, size_t lde
 }
 
 hipError_t launch_gram_i8(double* G, const int8_t* Pt, size_t ldp, const int8_t* digits, const GramChunk* chunks,
-                          uint32_t n_chunks, uint32_t n_pad, hipStream_t st) {
+                          uint32_t n_chunks, uint32_t n_pad, const uint32_t* order, uint32_t n_tiles, hipStream_t st) {
     if (n_pad % GRAM_BM || ldp % GRAM_BK) return hipErrorInvalidValue;
     const uint32_t tiles = n_pad / GRAM_BM;
-    hipLaunchKernelGGL(k_gram_i8, dim3(tiles, tiles), dim3(GRAM_THREADS), 0, st, G, tiles, Pt, ldp, digits, chunks, n_chunks,
-                       (uint32_t)(ldp / GRAM_BK));
+    if (tiles > 65535 || n_tiles != tiles * (tiles + 1) / 2 || !order) return hipErrorInvalidValue;
+    const uint32_t per_xcd = (n_tiles + N_XCDS - 1) / N_XCDS;
+    hipLaunchKernelGGL(k_gram_i8, dim3(per_xcd * N_XCDS), dim3(GRAM_THREADS), 0, st, G, tiles, Pt, ldp, digits, chunks,
+                       n_chunks, (uint32_t)(ldp / GRAM_BK), order, n_tiles, per_xcd);
     return hipGetLastError();
 }
 
@@ -888,15 +892,13 @@ static void pairs_sparse_t(uint32_t grid, hipStream_t st, const SparseArgs& a, u
                        per_xcd);
 }
 
-std::vector<uint32_t> sparse_block_order(uint32_t n) {
-    constexpr uint32_t ST = 16;   // super-tile edge in blocks: 256 blocks ~ the waves resident on one XCD
-    const uint32_t nb = (n + SP_B - 1) / SP_B;
+std::vector<uint32_t> upper_block_order(uint32_t nb, uint32_t super) {
     std::vector<uint32_t> order;
     order.reserve((size_t)nb * (nb + 1) / 2);
-    for (uint32_t si = 0; si < nb; si += ST)
-        for (uint32_t sj = si; sj < nb; sj += ST)
-            for (uint32_t bi = si; bi < std::min(nb, si + ST); ++bi)
-                for (uint32_t bj = std::max(bi, sj); bj < std::min(nb, sj + ST); ++bj) order.push_back(bi << 16 | bj);
+    for (uint32_t si = 0; si < nb; si += super)
+        for (uint32_t sj = si; sj < nb; sj += super)
+            for (uint32_t bi = si; bi < std::min(nb, si + super); ++bi)
+                for (uint32_t bj = std::max(bi, sj); bj < std::min(nb, sj + super); ++bj) order.push_back(bi << 16 | bj);
     return order;
 }
 
@@ -906,8 +908,8 @@ hipError_t launch_pairs_sparse(bool in_fp64, const SparseArgs& a, hipStream_t st
     if (!nb) return hipSuccess;
     if ((size_t)nb * SP_B > a.ldn || nb > 65535 || a.n_blocks != nb * (nb + 1) / 2 || !a.order)
         return hipErrorInvalidValue;
-    const uint32_t per_xcd = (a.n_blocks + SP_XCDS - 1) / SP_XCDS;
-    const uint32_t grid = per_xcd * SP_XCDS;
+    const uint32_t per_xcd = (a.n_blocks + N_XCDS - 1) / N_XCDS;
+    const uint32_t grid = per_xcd * N_XCDS;
     if (in_fp64) {
         if (a.out_fp64) pairs_sparse_t<double, double>(grid, st, a, per_xcd);
         else pairs_sparse_t<double, float>(grid, st, a, per_xcd);

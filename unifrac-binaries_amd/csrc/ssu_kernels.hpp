@@ -56,7 +56,7 @@ hipError_t launch_presence_T(int8_t* Pt, size_t ldp, const double* E, size_t lde
 // G = sum_c len_c * P[i][c] * P[j][c] for the upper-triangle 128x128 tiles, FP64,
 // stored tile by tile in MFMA fragment order (n_pad/128)^2 * 128^2 doubles
 hipError_t launch_gram_i8(double* G, const int8_t* Pt, size_t ldp, const int8_t* digits, const GramChunk* chunks,
-                          uint32_t n_chunks, uint32_t n_pad, hipStream_t st);
+                          uint32_t n_chunks, uint32_t n_pad, const uint32_t* order, uint32_t n_tiles, hipStream_t st);
 // out[i][j] from G and T = diag(G): (T_i+T_j-2G)/(T_i+T_j-G) or T_i+T_j-2G
 template <class TOut>
 hipError_t launch_unweighted_combine(TOut* out, const double* G, uint32_t n_pad, uint32_t n, bool normalized,
@@ -104,11 +104,12 @@ struct SparseArgs {
     void* out;              // n x n
     bool out_fp64;
     uint32_t* visits;       // optional [nb][nb], nb = ceil(n/64): nodes visited per block (bi <= bj)
-    const uint32_t* order;  // device copy of sparse_block_order(n)
+    const uint32_t* order;  // device copy of upper_block_order(ceil(n/64), 16)
     uint32_t n_blocks;      // its length, nb (nb + 1) / 2
 };
-// upper-triangle 64x64 blocks (bi << 16 | bj), super-tile by super-tile
-std::vector<uint32_t> sparse_block_order(uint32_t n);
+// The upper-triangle blocks (bi <= bj < nb) as bi << 16 | bj, super x super
+// super-tile by super-tile; kernels walk it in XCD-contiguous runs.
+std::vector<uint32_t> upper_block_order(uint32_t nb, uint32_t super);
 hipError_t launch_pairs_sparse(bool in_fp64, const SparseArgs& a, hipStream_t st);
 
 // --------------------------------------------------------- output layouts
