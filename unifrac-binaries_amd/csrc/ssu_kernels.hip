@@ -666,39 +666,46 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
         }
         __syncthreads();
         const uint32_t base = w0 * 32;
-        // lane e < EB resolves node t0 + e; its panel rows land in registers
+        // lane e < EB resolves node t0 + e; its panel rows land in registers.
+        // The loads are unconditional (row 0 stands in past the list end and
+        // is masked out at staging), so the compiler can wait for one step's
+        // loads while the other step's stay in flight.
         TIn pA[EB], qA[EB], pB[EB], qB[EB];
         double lA = 0.0, lB = 0.0;
-        auto fetch = [&](TIn (&p)[EB], TIn (&q)[EB], double& l, uint32_t t0) __attribute__((always_inline)) {
+        uint64_t vA = 0, vB = 0;   // which of the step's EB nodes are real
+        auto fetch = [&](TIn (&p)[EB], TIn (&q)[EB], double& l, uint64_t& valid, uint32_t t0)
+                         __attribute__((always_inline)) {
             const uint32_t idx = t0 + lane;
-            int32_t row = -1;
+            const bool real = lane < EB && idx < total;
+            int32_t row = 0;
             l = 0.0;
-            if (lane < EB && idx < total) {
+            if (real) {
                 const uint32_t r = base + list[idx];
-                row = rows ? rows[r] : (int32_t)r;
+                // FP32 operand rows are the processed rows themselves; only FP64
+                // operands read the embedding through rows[]
+                if constexpr (sizeof(TIn) == 8) row = rows[r];
+                else row = (int32_t)r;
                 l = len[r];
             }
+            valid = __ballot(real);
 #pragma unroll
             for (uint32_t e = 0; e < EB; ++e) {
-                const int32_t re = __builtin_amdgcn_readlane(row, e);
-                p[e] = TIn(0);
-                q[e] = TIn(0);
-                if (re >= 0) {
-                    const size_t off = (size_t)re * ldn;
-                    p[e] = U[off + offI];
-                    q[e] = U[off + offJ];
-                }
+                const size_t off = (size_t)__builtin_amdgcn_readlane(row, e) * ldn;
+                p[e] = U[off + offI];
+                q[e] = U[off + offJ];
             }
         };
-        auto step = [&](TIn (&p)[EB], TIn (&q)[EB], double& l, uint32_t t_next) __attribute__((always_inline)) {
+        auto step = [&](TIn (&p)[EB], TIn (&q)[EB], double& l, uint64_t& valid, uint32_t t_next)
+                        __attribute__((always_inline)) {
 #pragma unroll
             for (uint32_t e = 0; e < EB; ++e) {
-                sv[e][slot] = (double)p[e];
-                sv[e][SP_HALF + slot] = (double)q[e];
+                const bool on = (valid >> e) & 1;
+                sv[e][slot] = on ? (double)p[e] : 0.0;
+                sv[e][SP_HALF + slot] = on ? (double)q[e] : 0.0;
             }
             if (lane < EB) sl[lane] = l;
             __syncthreads();
-            if (t_next < total) fetch(p, q, l, t_next);   // two steps ahead of its use
+            if (t_next < total) fetch(p, q, l, valid, t_next);   // two steps ahead of its use
 #pragma unroll 2
             for (uint32_t e = 0; e < EB; ++e) {
                 double a[8], b[8];
@@ -714,12 +721,12 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
             }
             __syncthreads();
         };
-        if (total) fetch(pA, qA, lA, 0);
-        if (total > EB) fetch(pB, qB, lB, EB);
+        if (total) fetch(pA, qA, lA, vA, 0);
+        if (total > EB) fetch(pB, qB, lB, vB, EB);
         for (uint32_t t0 = 0; t0 < total; t0 += 2 * EB) {
-            step(pA, qA, lA, t0 + 2 * EB);
+            step(pA, qA, lA, vA, t0 + 2 * EB);
             if (t0 + EB >= total) break;
-            step(pB, qB, lB, t0 + 3 * EB);
+            step(pB, qB, lB, vB, t0 + 3 * EB);
         }
     }
 
