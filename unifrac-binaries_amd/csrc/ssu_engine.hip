@@ -327,7 +327,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     // 16 x 16 super-tiles of 64-sample blocks, 6 x 6 of 128-sample Gram tiles:
     // about the number of blocks (tiles) resident on one XCD
     if (ok && P.sparse) {
-        P.block_order = upper_block_order((P.n + 63) / 64, 16);
+        P.block_order = spec.fp64 ? upper_block_order((P.n + 63) / 64, 16) : rect_block_order(P.n);
         ok = upload(P.d_order, P.block_order, P.st, err);
     }
     if (ok && P.gram) {
@@ -373,7 +373,7 @@ bool Plan::execute(std::string* err) {
                                                (size_t)gather_segments(K, w_last) * w_last, 1});
         if (!P.d_Spart.ensure(parts * sizeof(double), err)) return false;
     }
-    if (P.sparse && !P.d_masks.ensure(std::max<size_t>((P.ldn / 64) * Kw, 1) * sizeof(uint32_t), err)) return false;
+    if (P.sparse && !P.d_masks.ensure(std::max<size_t>((P.ldn / 32) * Kw, 1) * sizeof(uint32_t), err)) return false;
     if (gather) {
         if (!P.d_U.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
         if (s.vaw && !P.d_Cu.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
@@ -435,11 +435,12 @@ bool Plan::execute(std::string* err) {
         a.out = P.d_out.p;
         a.out_fp64 = s.fp64;
         const size_t nb = (P.n + 63) / 64;
-        if (!P.d_visits.ensure(std::max<size_t>(nb * nb, 1) * sizeof(uint32_t), err)) return false;
+        if (!P.d_visits.ensure(std::max<size_t>(2 * nb * nb, 1) * sizeof(uint32_t), err)) return false;
         a.visits = P.d_visits.as<uint32_t>();
         a.order = P.d_order.as<uint32_t>();
         a.n_blocks = (uint32_t)P.block_order.size();
-        SSU_HIP(launch_pairs_sparse(s.fp64, a, P.st));
+        if (s.fp64) SSU_HIP(launch_pairs_sparse(true, a, P.st));
+        else SSU_HIP(launch_pairs_sparse32(a, P.st));
         SSU_HIP(hipEventRecord(P.ev[3], P.st));
         SSU_HIP(hipEventRecord(P.ev[4], P.st));
     } else {
@@ -522,15 +523,15 @@ uint64_t Plan::kernel_terms() const {
         return t * (t + 1) / 2 * GRAM_BM * GRAM_BM * (uint64_t)P.Kp;
     }
     if (P.sparse) {
-        const size_t nb = (P.n + 63) / 64;
-        if (!nb || !P.d_visits.p) return 0;
-        std::vector<uint32_t> v(nb * nb);
+        // visited nodes per block, times the pairs a block evaluates
+        const size_t nbi = P.spec.fp64 ? (P.n + 63) / 64 : (P.n + 31) / 32, nbj = (P.n + 63) / 64;
+        if (!nbj || !P.d_visits.p) return 0;
+        std::vector<uint32_t> v(nbi * nbj);
         if (hipMemcpy(v.data(), P.d_visits.p, v.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
             return 0;
         uint64_t sum = 0;
-        for (size_t i = 0; i < nb; ++i)
-            for (size_t j = i; j < nb; ++j) sum += v[i * nb + j];
-        return sum * 64 * 64;
+        for (uint32_t bij : P.block_order) sum += v[(size_t)(bij >> 16) * nbj + (bij & 0xffffu)];
+        return sum * (P.spec.fp64 ? 64 * 64 : 32 * 64);
     }
     const uint64_t t = (P.n + PAIR_TILE - 1) / PAIR_TILE;
     return t * (t + 1) / 2 * PAIR_TILE * PAIR_TILE * K;

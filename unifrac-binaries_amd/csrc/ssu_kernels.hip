@@ -69,8 +69,8 @@ __global__ void __launch_bounds__(EMB_THREADS) k_embed_level(double* __restrict_
 //   GATHER: U[r][s] = (float)e
 //   SUMS:   Spart[seg][s] = sum_{r in seg} klen[r] * e', increasing r, where e'
 //           is e as the pair kernels see it ((float)e when GATHER, else e)
-//   MASKS:  bit (r % 32) of masks[panel][r / 32] = any e > 0 over the 64
-//           samples of the panel (one wavefront = one panel)
+//   MASKS:  bit (r % 32) of masks[panel][r / 32] = any e > 0 over the 32
+//           samples of the panel (one wavefront = two panels)
 // E holds the sample columns [col0, col0 + lde); U (row stride ldu) and the
 // masks are indexed by absolute sample, Spart by column within the block.
 template <bool GATHER, bool SUMS, bool MASKS>
@@ -85,9 +85,9 @@ __global__ void __launch_bounds__(EMB_THREADS) k_gather_sums(float* __restrict__
     if (s >= lde) return;
     const uint32_t k0 = blockIdx.y * rows_per_seg;
     const uint32_t k1 = min(K, k0 + rows_per_seg);
-    const size_t panel = (col0 + s) / 64;
+    const size_t panel = (col0 + s) / 64 * 2;   // the wave's lower 32-sample panel
     double acc = 0.0;
-    uint32_t bits = 0;
+    uint32_t lo = 0, hi = 0;
 #pragma unroll 8
     for (uint32_t r = k0; r < k1; ++r) {
         double e = E[(size_t)knodes[r] * lde + s];
@@ -98,10 +98,15 @@ __global__ void __launch_bounds__(EMB_THREADS) k_gather_sums(float* __restrict__
         }
         if constexpr (SUMS) acc += klen[r] * e;
         if constexpr (MASKS) {
-            bits |= (__ballot(e > 0.0) != 0ull ? 1u : 0u) << (r & 31);
+            const uint64_t b = __ballot(e > 0.0);
+            lo |= ((uint32_t)b != 0u ? 1u : 0u) << (r & 31);
+            hi |= ((uint32_t)(b >> 32) != 0u ? 1u : 0u) << (r & 31);
             if ((r & 31) == 31 || r + 1 == k1) {
-                if ((threadIdx.x & 63) == 0) masks[panel * Kw + (r >> 5)] = bits;
-                bits = 0;
+                if ((threadIdx.x & 63) == 0) {
+                    masks[panel * Kw + (r >> 5)] = lo;
+                    masks[(panel + 1) * Kw + (r >> 5)] = hi;
+                }
+                lo = hi = 0;
             }
         }
     }
@@ -599,6 +604,27 @@ __device__ __forceinline__ double vmin_f64(double a, double b) {
     return r;
 }
 
+// Compacts the set bits of the wave's 64 mask words (lane l holds word l) into
+// list[], in increasing order of l * 32 + bit; returns their number (uniform).
+__device__ __forceinline__ uint32_t build_list(uint16_t* list, uint32_t word, uint32_t lane) {
+    const uint32_t cnt = __popc(word);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, 64);
+        if ((int)lane >= d) incl += y;
+    }
+    const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+    uint32_t o = incl - cnt;
+    while (word) {
+        const uint32_t b = __ffs(word) - 1;
+        list[o++] = (uint16_t)(lane * 32 + b);
+        word &= word - 1;
+    }
+    __syncthreads();
+    return total;
+}
+
 constexpr uint32_t SP_B = 64;        // block edge in samples
 constexpr uint32_t SP_WORDS = 64;    // mask words (32 nodes each) per list chunk
 // Staging: the 8 values a lane reads for its rows (or columns) are contiguous;
@@ -640,31 +666,17 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
 #pragma unroll
         for (int c = 0; c < 8; ++c) acc[r][c] = 0.0;
 
-    const uint32_t* mI = masks + (size_t)bi * Kw;
-    const uint32_t* mJ = masks + (size_t)bj * Kw;
+    const uint32_t* mI = masks + (size_t)(2 * bi) * Kw;   // two 32-sample panels per side
+    const uint32_t* mJ = masks + (size_t)(2 * bj) * Kw;
     const size_t offI = (size_t)bi * SP_B + lane, offJ = (size_t)bj * SP_B + lane;
     const uint32_t slot = (lane >> 3) * SP_GS + (lane & 7);   // where this lane stages its sample
 
     for (uint32_t w0 = 0; w0 < Kw; w0 += SP_WORDS) {
         // active nodes of this chunk, in increasing order, as offsets from w0*32
         const uint32_t w = w0 + lane;
-        uint32_t word = w < Kw ? (mI[w] & mJ[w]) : 0u;
-        const uint32_t cnt = __popc(word);
-        uint32_t incl = cnt;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t y = __shfl_up(incl, d, 64);
-            if ((int)lane >= d) incl += y;
-        }
-        const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+        uint32_t word = w < Kw ? ((mI[w] | mI[Kw + w]) & (mJ[w] | mJ[Kw + w])) : 0u;
+        const uint32_t total = build_list(list, word, lane);
         visited += total;
-        uint32_t o = incl - cnt;
-        while (word) {
-            const uint32_t b = __ffs(word) - 1;
-            list[o++] = (uint16_t)(lane * 32 + b);
-            word &= word - 1;
-        }
-        __syncthreads();
         const uint32_t base = w0 * 32;
         // lane e < EB resolves node t0 + e; its panel rows land in registers.
         // The loads are unconditional (row 0 stands in past the list end and
@@ -742,6 +754,149 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
             double x = tot - 2.0 * acc[r][c];
             x = x < 0.0 ? 0.0 : x;   // rounding can undershoot 0; NaN (empty samples) passes
             const double d = i == j ? 0.0 : (normalized ? x / tot : x);
+            out[i * n + j] = (TOut)d;
+            out[j * n + i] = (TOut)d;
+        }
+}
+
+// FP32 operands: 32 x 64 blocks of pairs (rows: a 32-sample panel, columns:
+// a 64-sample panel), one wavefront each, a 4 x 8 tile per lane.  Terms are
+// v_min_f32 + v_fma_f32 (this pair issues about 1.3x faster than its FP64
+// form), summed in FP32 over S3_MERGE listed nodes and then merged into FP64.
+// The narrower row panel also skips more nodes than a 64 x 64 block.  Only
+// pairs i < j are written (both (i, j) and (j, i)), so every pair has one
+// writer even where blocks straddle the diagonal.
+constexpr uint32_t S3_BI = 32, S3_BJ = 64;
+constexpr uint32_t S3_EB = 8;          // nodes per step; two steps in flight
+constexpr uint32_t S3_MERGE = 64;      // FP32 accumulation span, in listed nodes
+constexpr uint32_t S3_GSJ = 12;        // column groups of 8 floats sit 48 bytes apart (bank-conflict free)
+constexpr uint32_t S3_JOFF = 32;       // the 32 row values come first, contiguous
+constexpr uint32_t S3_NODE = S3_JOFF + 8 * S3_GSJ;
+
+__device__ __forceinline__ float vmin_f32(float a, float b) {
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+template <class TOut>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_pairs_sparse32(const float* __restrict__ U, size_t ldn,
+                                                       const double* __restrict__ len,
+                                                       const uint32_t* __restrict__ masks, uint32_t Kw, uint32_t n,
+                                                       const double* __restrict__ S, int normalized,
+                                                       TOut* __restrict__ out, uint32_t* __restrict__ visits,
+                                                       const uint32_t* __restrict__ order, uint32_t n_blocks,
+                                                       uint32_t per_xcd) {
+    constexpr uint32_t EB = S3_EB;
+    const uint32_t seq = (blockIdx.x % N_XCDS) * per_xcd + blockIdx.x / N_XCDS;
+    if (seq >= n_blocks) return;
+    const uint32_t pi = order[seq] >> 16, pj = order[seq] & 0xffffu;   // 32-panel, 64-panel
+    uint32_t visited = 0;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t li = lane >> 3, lj = lane & 7;
+
+    __shared__ uint16_t list[SP_WORDS * 32];
+    __shared__ __attribute__((aligned(16))) float sv[EB][S3_NODE];
+    __shared__ float sl[EB];
+
+    float acc[4][8];
+    double acc64[4][8];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) { acc[r][c] = 0.0f; acc64[r][c] = 0.0; }
+    uint32_t pending = 0;
+
+    const uint32_t* mI = masks + (size_t)pi * Kw;
+    const uint32_t* mJ = masks + (size_t)(2 * pj) * Kw;
+    const size_t offI = (size_t)pi * S3_BI + (lane & 31), offJ = (size_t)pj * S3_BJ + lane;
+    const uint32_t slotJ = S3_JOFF + (lane >> 3) * S3_GSJ + (lane & 7);
+
+    for (uint32_t w0 = 0; w0 < Kw; w0 += SP_WORDS) {
+        const uint32_t w = w0 + lane;
+        const uint32_t word = w < Kw ? (mI[w] & (mJ[w] | mJ[Kw + w])) : 0u;
+        const uint32_t total = build_list(list, word, lane);
+        visited += total;
+        const uint32_t base = w0 * 32;
+        float pA[EB], qA[EB], pB[EB], qB[EB];
+        float lA = 0.0f, lB = 0.0f;
+        uint64_t vA = 0, vB = 0;
+        auto fetch = [&](float (&p)[EB], float (&q)[EB], float& l, uint64_t& valid, uint32_t t0)
+                         __attribute__((always_inline)) {
+            const uint32_t idx = t0 + lane;
+            const bool real = lane < EB && idx < total;
+            int32_t row = 0;
+            l = 0.0f;
+            if (real) {
+                const uint32_t r = base + list[idx];
+                row = (int32_t)r;
+                l = (float)len[r];
+            }
+            valid = __ballot(real);
+#pragma unroll
+            for (uint32_t e = 0; e < EB; ++e) {
+                const size_t off = (size_t)__builtin_amdgcn_readlane(row, e) * ldn;
+                p[e] = U[off + offI];      // lanes 32..63 repeat lanes 0..31
+                q[e] = U[off + offJ];
+            }
+        };
+        auto step = [&](float (&p)[EB], float (&q)[EB], float& l, uint64_t& valid, uint32_t t_next)
+                        __attribute__((always_inline)) {
+#pragma unroll
+            for (uint32_t e = 0; e < EB; ++e) {
+                const bool on = (valid >> e) & 1;
+                if (lane < S3_BI) sv[e][lane] = on ? p[e] : 0.0f;
+                sv[e][slotJ] = on ? q[e] : 0.0f;
+            }
+            if (lane < EB) sl[lane] = l;
+            __syncthreads();
+            if (t_next < total) fetch(p, q, l, valid, t_next);
+#pragma unroll 2
+            for (uint32_t e = 0; e < EB; ++e) {
+                float a[4], b[8];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) a[r] = sv[e][li * 4 + r];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) b[c] = sv[e][S3_JOFF + lj * S3_GSJ + c];
+                const float ln = sl[e];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) acc[r][c] = __builtin_fmaf(vmin_f32(a[r], b[c]), ln, acc[r][c]);
+            }
+            pending += EB;
+            if (pending >= S3_MERGE) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) { acc64[r][c] += (double)acc[r][c]; acc[r][c] = 0.0f; }
+                pending = 0;
+            }
+            __syncthreads();
+        };
+        if (total) fetch(pA, qA, lA, vA, 0);
+        if (total > EB) fetch(pB, qB, lB, vB, EB);
+        for (uint32_t t0 = 0; t0 < total; t0 += 2 * EB) {
+            step(pA, qA, lA, vA, t0 + 2 * EB);
+            if (t0 + EB >= total) break;
+            step(pB, qB, lB, vB, t0 + 3 * EB);
+        }
+    }
+
+    if (visits && lane == 0) visits[(size_t)pi * ((n + S3_BJ - 1) / S3_BJ) + pj] = visited;
+    const size_t i0 = (size_t)pi * S3_BI + li * 4, j0 = (size_t)pj * S3_BJ + lj * 8;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const size_t i = i0 + r, j = j0 + c;
+            if (j >= n || i > j) continue;
+            if (i == j) { out[i * n + i] = (TOut)0; continue; }
+            const double m = acc64[r][c] + (double)acc[r][c];
+            const double tot = S[i] + S[j];
+            double x = tot - 2.0 * m;
+            x = x < 0.0 ? 0.0 : x;   // rounding can undershoot 0; NaN (empty samples) passes
+            const double d = normalized ? x / tot : x;
             out[i * n + j] = (TOut)d;
             out[j * n + i] = (TOut)d;
         }
@@ -907,6 +1062,35 @@ std::vector<uint32_t> upper_block_order(uint32_t nb, uint32_t super) {
             for (uint32_t bi = si; bi < std::min(nb, si + super); ++bi)
                 for (uint32_t bj = std::max(bi, sj); bj < std::min(nb, sj + super); ++bj) order.push_back(bi << 16 | bj);
     return order;
+}
+
+std::vector<uint32_t> rect_block_order(uint32_t n) {
+    constexpr uint32_t SI = 32, SJ = 16;   // 32 x 16 blocks of 32 x 64 pairs: ~ the waves resident on one XCD
+    const uint32_t nbi = (n + S3_BI - 1) / S3_BI, nbj = (n + S3_BJ - 1) / S3_BJ;
+    std::vector<uint32_t> order;
+    for (uint32_t si = 0; si < nbi; si += SI)
+        for (uint32_t sj = si / 2; sj < nbj; sj += SJ)
+            for (uint32_t bi = si; bi < std::min(nbi, si + SI); ++bi)
+                for (uint32_t bj = sj; bj < std::min(nbj, sj + SJ); ++bj)
+                    if (bi <= 2 * bj + 1) order.push_back(bi << 16 | bj);
+    return order;
+}
+
+template <class TOut>
+static void pairs_sparse32_t(uint32_t grid, hipStream_t st, const SparseArgs& a, uint32_t per_xcd) {
+    hipLaunchKernelGGL((k_pairs_sparse32<TOut>), dim3(grid), dim3(64), 0, st, (const float*)a.U, a.ldn, a.len, a.masks,
+                       a.Kw, a.n, a.S, (int)a.normalized, (TOut*)a.out, a.visits, a.order, a.n_blocks, per_xcd);
+}
+
+hipError_t launch_pairs_sparse32(const SparseArgs& a, hipStream_t st) {
+    if (a.ldn % S3_BJ) return hipErrorInvalidValue;
+    const uint32_t nbi = (a.n + S3_BI - 1) / S3_BI, nbj = (a.n + S3_BJ - 1) / S3_BJ;
+    if (!nbi) return hipSuccess;
+    if ((size_t)nbj * S3_BJ > a.ldn || nbi > 65535 || !a.order || !a.n_blocks) return hipErrorInvalidValue;
+    const uint32_t per_xcd = (a.n_blocks + N_XCDS - 1) / N_XCDS;
+    if (a.out_fp64) pairs_sparse32_t<double>(per_xcd * N_XCDS, st, a, per_xcd);
+    else pairs_sparse32_t<float>(per_xcd * N_XCDS, st, a, per_xcd);
+    return hipGetLastError();
 }
 
 hipError_t launch_pairs_sparse(bool in_fp64, const SparseArgs& a, hipStream_t st) {

@@ -38,8 +38,8 @@ hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint3
 //   S (if non-null): S[s] = sum_r klen[r] * u_rs in FP64 (u as the pair kernels
 //     see it: FP32-rounded when U is written), summed per segment then over
 //     segments in order, into S[col0 ..]; Spart holds gather_segments(K, lde) * lde doubles;
-//   masks (if non-null; lde, col0 multiples of 64): [panels][ceil(K/32)] words, bit r%32 of
-//     word r/32 of panel p set iff some sample of p*64..p*64+63 has u_rs > 0.
+//   masks (if non-null; lde, col0 multiples of 64): [ldn/32][ceil(K/32)] words, bit r%32 of
+//     word r/32 of panel p set iff some sample of p*32..p*32+31 has u_rs > 0.
 uint32_t gather_segments(uint32_t K, size_t ldn);
 hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, uint32_t* masks, const double* E,
                               size_t lde, uint32_t col0, const int32_t* knodes, const double* klen, uint32_t K,
@@ -111,6 +111,11 @@ struct SparseArgs {
 // super-tile by super-tile; kernels walk it in XCD-contiguous runs.
 std::vector<uint32_t> upper_block_order(uint32_t nb, uint32_t super);
 hipError_t launch_pairs_sparse(bool in_fp64, const SparseArgs& a, hipStream_t st);
+// FP32 operands: 32 x 64 blocks (32-sample row panel pi <= 2 pj + 1, 64-sample
+// column panel pj), FP32 terms merged into FP64; `order` = rect_block_order(n)
+// and visits is [ceil(n/32)][ceil(n/64)].
+std::vector<uint32_t> rect_block_order(uint32_t n);
+hipError_t launch_pairs_sparse32(const SparseArgs& a, hipStream_t st);
 
 // --------------------------------------------------------- output layouts
 // stripes[s - start][k] = M[k][(k+s+1) % n]   (double output)
