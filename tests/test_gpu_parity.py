@@ -241,7 +241,7 @@ def test_weighted_shared_node_kernel_matches_dense(ssu, O, synth, method, prec):
     pd = ssu.Plan(inp, name, plain_kernel=1)
     ps.execute()
     pd.execute()
-    assert ps.main_kernel() == "k_pairs_sparse" and pd.main_kernel() == "k_pairs"
+    assert ps.main_kernel() == ("k_pairs_sparse" if prec == "fp64" else "k_pairs_sparse32") and pd.main_kernel() == "k_pairs"
     ref = oracle_matrix(O, tree, table, method)
     ms, md = ps.matrix(), pd.matrix()
     assert rel_err(ms, ref) <= TOL[prec]

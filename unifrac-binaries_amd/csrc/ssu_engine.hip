@@ -537,7 +537,11 @@ uint64_t Plan::kernel_terms() const {
     return t * (t + 1) / 2 * PAIR_TILE * PAIR_TILE * K;
 }
 bool Plan::out_fp64() const { return p_->spec.fp64; }
-const char* Plan::main_kernel() const { return p_->gram ? "k_gram_i8" : p_->sparse ? "k_pairs_sparse" : "k_pairs"; }
+const char* Plan::main_kernel() const {
+    if (p_->gram) return "k_gram_i8";
+    if (p_->sparse) return p_->spec.fp64 ? "k_pairs_sparse" : "k_pairs_sparse32";
+    return "k_pairs";
+}
 const void* Plan::device_output() const { return p_->d_out.p; }
 
 }  // namespace ssu
