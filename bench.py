@@ -67,6 +67,23 @@ def max_rel_err(a, b):
     return float(np.max(np.where(s > 0, d / np.where(s > 0, s, 1), d)))
 
 
+def parse_methods(text):
+    """'weighted_normalized,generalized:alpha=0.5,weighted_normalized:vaw' -> {label: (method, kwargs)}"""
+    out = {}
+    for item in text.split(","):
+        parts = item.strip().split(":")
+        kw = {}
+        for opt in parts[1:]:
+            if opt == "vaw":
+                kw["variance_adjust"] = True
+            elif opt.startswith("alpha="):
+                kw["alpha"] = float(opt[6:])
+            else:
+                raise SystemExit(f"bench.py: unknown method option {opt!r}")
+        out[item.strip()] = (parts[0], kw)
+    return out
+
+
 def cpu_baseline(tree, table, methods, budget_s):
     """Reference CPU code (oracle/_ref) on stripe sub-ranges; extrapolated to the full matrix."""
     import oracle as O
@@ -77,13 +94,13 @@ def cpu_baseline(tree, table, methods, budget_s):
     S = (n + 1) // 2
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     total_time, per = 0.0, {}
-    for method in methods:
+    for label, (method, kw) in methods.items():
         def run(k):
             t0 = time.perf_counter()
             if kind == "reference":
-                O.ref_stripes(otree, otab, method, fp32=True, start=0, stop=k)
+                O.ref_stripes(otree, otab, method, fp32=True, start=0, stop=k, **kw)
             else:
-                O.stripes(otree, otab, method, start=0, stop=k, threads=threads)
+                O.stripes(otree, otab, method, start=0, stop=k, threads=threads, **kw)
             return time.perf_counter() - t0
         t1 = run(1)
         if t1 * 4 < budget_s / len(methods):
@@ -94,9 +111,9 @@ def cpu_baseline(tree, table, methods, budget_s):
         else:
             slope, intercept = t1, 0.0
         full = intercept + S * slope
-        per[method] = {"stripe_s": slope, "setup_s": intercept, "full_matrix_s": full}
+        per[label] = {"stripe_s": slope, "setup_s": intercept, "full_matrix_s": full}
         total_time += full
-        log(f"cpu baseline {method}: {slope:.3f} s/stripe, setup {intercept:.2f} s, full {full:.0f} s")
+        log(f"cpu baseline {label}: {slope:.3f} s/stripe, setup {intercept:.2f} s, full {full:.0f} s")
     return kind, threads, total_time, per
 
 
@@ -123,6 +140,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="unw_100k_16384")
+    ap.add_argument("--methods", default="unweighted,weighted_normalized",
+                    help="comma list of method[:alpha=A][:vaw]; one step runs every listed matrix")
     ap.add_argument("--samples", type=int, default=None)
     ap.add_argument("--tips", type=int, default=None)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of reference CPU work")
@@ -150,10 +169,8 @@ def main():
     log(f"workload {args.workload}: {len(tree[0]) // 2} nodes, {n} samples, nnz {table[4].size} "
         f"({time.perf_counter() - t0:.1f} s to generate)")
     inp = ssu.Inputs(*tree, *table)
-    plans = {
-        "unweighted": ssu.Plan(inp, "unweighted"),
-        "weighted_normalized": ssu.Plan(inp, "weighted_normalized"),
-    }
+    specs = parse_methods(args.methods)
+    plans = {label: ssu.Plan(inp, m, **kw) for label, (m, kw) in specs.items()}
     for name, p in plans.items():
         log(f"plan {name}: main kernel {p.main_kernel()}, pair-nodes {p.pair_nodes():.4g}")
     for _ in range(args.warmup):
@@ -201,10 +218,11 @@ def main():
         achieved = FLOPS_PER_PAIR_NODE_W * pair_nodes[dom] / (dk["main_ms"] / 1000.0) / 1e12
         roofline = {"kernel": dk["kernel"], "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None}
-    g = kernels["unweighted"]
-    g_ops = OPS_PER_PAIR_NODE_GRAM * pair_nodes["unweighted"] / (g["main_ms"] / 1000.0) / 1e12
-    kernels["unweighted"]["mfma_int8_tops"] = g_ops
-    kernels["unweighted"]["mfma_frac"] = g_ops / PEAK_INT8_TOPS
+    for name, k in kernels.items():
+        if k["kernel"] == "k_gram_i8":
+            g_ops = OPS_PER_PAIR_NODE_GRAM * pair_nodes[name] / (k["main_ms"] / 1000.0) / 1e12
+            k["mfma_int8_tops"] = g_ops
+            k["mfma_frac"] = g_ops / PEAK_INT8_TOPS
     prof = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(prof):
         tr = json.load(open(prof)).get(f"{args.workload}:{dk['kernel']}")
@@ -220,7 +238,7 @@ def main():
     errors = {}
     for name, p in plans.items():
         m = p.matrix()
-        ref = O.matrix(otree, O.Table(*sub), name)
+        ref = O.matrix(otree, O.Table(*sub), specs[name][0], **specs[name][1])
         errors[name] = max_rel_err(m[np.ix_(idx, idx)], ref)
         del m
     log(f"max rel err vs FP64 oracle: {errors}")
@@ -230,7 +248,7 @@ def main():
     plain = None
     if not args.no_plain:
         plain = {}
-        for name in plans:
+        for name in [x for x in plans if x in ("unweighted", "weighted_normalized")]:
             pp = ssu.Plan(inp, name, plain_kernel=1)
             pp.execute()
             pp.execute()
@@ -244,7 +262,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        kind, threads, t_full, per = cpu_baseline(tree, table, list(plans), args.cpu_budget)
+        kind, threads, t_full, per = cpu_baseline(tree, table, specs, args.cpu_budget)
         cpu = {"value": sum(pair_nodes.values()) / t_full, "unit": "pair-node updates/s", "cores": threads,
                "kind": kind,
                "sample": f"stripes [0,1) and [0,3) of both matrices (FP32 reference kernels, OpenMP), "
@@ -265,7 +283,7 @@ def main():
         "dtype": "int8 MFMA (unweighted, exact) + fp32 VALU terms with fp64 merges (weighted)",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: random binary tree {len(tree[0]) // 4 + 1} tips, "
-                               f"{n} samples, unweighted + weighted_normalized, full n x n matrices",
+                               f"{n} samples, {' + '.join(plans)}, full n x n matrices",
                    "samples": n, "tree_nodes": len(tree[0]) // 2, "nnz": int(table[4].size),
                    "methods": list(plans)},
         "max_rel_err_vs_fp64": errors,

@@ -533,8 +533,11 @@ uint64_t Plan::kernel_terms() const {
         for (uint32_t bij : P.block_order) sum += v[(size_t)(bij >> 16) * nbj + (bij & 0xffffu)];
         return sum * (P.spec.fp64 ? 64 * 64 : 32 * 64);
     }
-    const uint64_t t = (P.n + PAIR_TILE - 1) / PAIR_TILE;
-    return t * (t + 1) / 2 * PAIR_TILE * PAIR_TILE * K;
+    // dense tiles: 128 x 128 for weighted / plain unweighted, 64 x 64 for generalized and VAW
+    const bool small = P.spec.vaw || P.spec.method == Method::generalized;
+    const uint64_t edge = small ? 64 : PAIR_TILE;
+    const uint64_t t = (P.n + edge - 1) / edge;
+    return t * (t + 1) / 2 * edge * edge * K;
 }
 bool Plan::out_fp64() const { return p_->spec.fp64; }
 const char* Plan::main_kernel() const {

@@ -386,28 +386,50 @@ __device__ __forceinline__ void pair_term(T u, T v, T cu, T cv, T m, T l, T alph
         num = fma(dabs(u - v), l, num);
         den = fma(u > v ? u : v, l, den);
     } else if constexpr (OP == PairOp::generalized) {
-        // src/unifrac_task_impl.hpp:1016-1024
+        // src/unifrac_task_impl.hpp:1016-1024: w = s^a l, num += w |u-v| / s, den += w (s = u+v > 0)
         const T s = u + v;
         const bool nz = s != T(0);
         const T ss = nz ? s : T(1);
-        const T w = nz ? dpow<T>(ss, alpha) * l : T(0);
-        num = fma(w, dabs(u - v) / ss, num);
-        den += w;
+        if constexpr (sizeof(T) == 4) {
+            // FP32: s^a and s^(a-1) from the hardware log2 / exp2 (about 1 ulp each);
+            // the FP64 merges and the FP64 oracle comparison bound the result.
+            const float lg = __builtin_amdgcn_logf(ss);
+            const float w = nz ? __builtin_amdgcn_exp2f(alpha * lg) * l : 0.0f;
+            const float q = nz ? __builtin_amdgcn_exp2f((alpha - 1.0f) * lg) * l : 0.0f;
+            num = fma(q, dabs(u - v), num);
+            den += w;
+        } else {
+            const T w = nz ? dpow<T>(ss, alpha) * l : T(0);
+            num = fma(w, dabs(u - v) / ss, num);
+            den += w;
+        }
     } else {
-        // src/unifrac_task_impl.hpp:881-893, 1098-1109, 1817-1829
+        // src/unifrac_task_impl.hpp:881-893, 1098-1109, 1817-1829: terms divided by
+        // sqrt(mi (m - mi)) where that is > 0 (FP32: one hardware reciprocal square root)
         const T mi = cu + cv;
-        const T vaw = dsqrt<T>(mi * (m - mi));
-        const bool ok = vaw > T(0);
-        const T rv = ok ? T(1) / vaw : T(0);
+        const T vv = mi * (m - mi);
+        const bool ok = vv > T(0);
+        T rv;
+        if constexpr (sizeof(T) == 4) rv = ok ? __builtin_amdgcn_rsqf(vv) : 0.0f;
+        else rv = ok ? T(1) / dsqrt<T>(vv) : T(0);
         if constexpr (OP == PairOp::vaw_weighted) {
             num = fma(dabs(u - v) * l, rv, num);
             den = fma((u + v) * l, rv, den);
         } else if constexpr (OP == PairOp::vaw_generalized) {
             const T s1 = (u + v) * rv, d1 = dabs(u - v) * rv;
-            const T ss = ok ? s1 : T(1);
-            const T w = ok ? dpow<T>(ss, alpha) * l : T(0);
-            num = fma(w, d1 / ss, num);
-            den += w;
+            const bool on = ok && s1 != T(0);
+            const T ss = on ? s1 : T(1);
+            if constexpr (sizeof(T) == 4) {
+                const float lg = __builtin_amdgcn_logf(ss);
+                const float w = on ? __builtin_amdgcn_exp2f(alpha * lg) * l : 0.0f;
+                const float q = on ? __builtin_amdgcn_exp2f((alpha - 1.0f) * lg) * l : 0.0f;
+                num = fma(q, d1, num);
+                den += w;
+            } else {
+                const T w = on ? dpow<T>(ss, alpha) * l : T(0);
+                num = fma(w, d1 / ss, num);
+                den += w;
+            }
         } else {  // vaw_unweighted: u, v are 0/1 presence values
             const T lv = l * rv;
             num = fma(dabs(u - v), lv, num);
@@ -425,13 +447,13 @@ template <class T> struct PairStage { static constexpr int EB = sizeof(T) == 4 ?
 // of both 128-sample panels (and of the count panels for VAW) are staged in
 // LDS per step; the next step's rows are prefetched into registers while the
 // current one is consumed, and LDS is double-buffered (one barrier per step).
-template <PairOp OP, class T, int RI, int RJ>
-__global__ void __launch_bounds__((PAIR_TILE / RI) * (PAIR_TILE / RJ))
+template <PairOp OP, class T, int RI, int RJ, int TT>
+__global__ void __launch_bounds__((TT / RI) * (TT / RJ))
 k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __restrict__ rows, size_t ldn,
         const T* __restrict__ len, uint32_t K,
         uint32_t n, const double* __restrict__ S, const double* __restrict__ totals, double alpha_d, int normalized,
         void* __restrict__ out_v, int out_fp64) {
-    constexpr int NTI = PAIR_TILE / RI, NTJ = PAIR_TILE / RJ, NT = NTI * NTJ;
+    constexpr int NTI = TT / RI, NTJ = TT / RJ, NT = NTI * NTJ;
     constexpr bool TWO = PairTraits<OP>::two;
     constexpr bool CNT = PairTraits<OP>::counts;
     constexpr bool MIXED = sizeof(T) == 4;  // FP32 terms, FP64 merges
@@ -439,7 +461,7 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
     constexpr int EB = PairStage<T>::EB;
     constexpr int VEC = 16 / sizeof(T);                      // elements per 16-byte load
     constexpr int NPANEL = CNT ? 4 : 2;                      // u_i, u_j (, c_i, c_j)
-    constexpr int LOADS = NPANEL * EB * PAIR_TILE / VEC;     // 16-byte loads per step
+    constexpr int LOADS = NPANEL * EB * TT / VEC;     // 16-byte loads per step
     static_assert(LOADS % NT == 0, "staging must divide evenly");
     constexpr int LPT = LOADS / NT;                          // per thread
     typedef __attribute__((ext_vector_type(VEC))) T vecT;
@@ -448,11 +470,11 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
     const uint32_t t = threadIdx.x;
     const uint32_t ti = t % NTI, tj = t / NTI;
 
-    __shared__ __attribute__((aligned(16))) T sp[2][NPANEL][EB][PAIR_TILE];
+    __shared__ __attribute__((aligned(16))) T sp[2][NPANEL][EB][TT];
     __shared__ T sl[2][EB];
 
     const T alpha = (T)alpha_d;
-    const size_t i0 = (size_t)bi * PAIR_TILE, j0 = (size_t)bj * PAIR_TILE;
+    const size_t i0 = (size_t)bi * TT, j0 = (size_t)bj * TT;
     T mrow[CNT ? RI : 1][CNT ? RJ : 1];  // VAW: c_i + c_j
     if constexpr (CNT) {
 #pragma unroll
@@ -482,9 +504,9 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
 #pragma unroll
         for (int q = 0; q < LPT; ++q) {
             const uint32_t idx = t + (uint32_t)NT * q;
-            const uint32_t panel = idx / (EB * PAIR_TILE / VEC);
-            const uint32_t rem = idx % (EB * PAIR_TILE / VEC);
-            const uint32_t e = rem / (PAIR_TILE / VEC), x = (rem % (PAIR_TILE / VEC)) * VEC;
+            const uint32_t panel = idx / (EB * TT / VEC);
+            const uint32_t rem = idx % (EB * TT / VEC);
+            const uint32_t e = rem / (TT / VEC), x = (rem % (TT / VEC)) * VEC;
             const uint32_t k = k0 + e;
             vecT v = (vecT)(T(0));
             if (k < K) {
@@ -500,9 +522,9 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
 #pragma unroll
         for (int q = 0; q < LPT; ++q) {
             const uint32_t idx = t + (uint32_t)NT * q;
-            const uint32_t panel = idx / (EB * PAIR_TILE / VEC);
-            const uint32_t rem = idx % (EB * PAIR_TILE / VEC);
-            const uint32_t e = rem / (PAIR_TILE / VEC), x = (rem % (PAIR_TILE / VEC)) * VEC;
+            const uint32_t panel = idx / (EB * TT / VEC);
+            const uint32_t rem = idx % (EB * TT / VEC);
+            const uint32_t e = rem / (TT / VEC), x = (rem % (TT / VEC)) * VEC;
             vecT v = pre[q];
             if constexpr (BIN) {
                 if (panel < 2) {
@@ -912,25 +934,28 @@ __global__ void __launch_bounds__(256) k_matrix_to_stripes(double* __restrict__ 
     st[(size_t)blockIdx.y * n + k] = (double)M[(size_t)k * n + l];
 }
 
-template <PairOp OP, class T, int RI, int RJ>
+template <PairOp OP, class T, int RI, int RJ, int TT = PAIR_TILE>
 hipError_t launch_pairs_t(const PairArgs& a, hipStream_t st) {
-    const uint32_t tiles = (a.n + PAIR_TILE - 1) / PAIR_TILE;
+    static_assert(PAIR_TILE % TT == 0, "the row stride is padded to PAIR_TILE");
+    const uint32_t tiles = (a.n + TT - 1) / TT;
     dim3 grid(tiles, tiles);
-    constexpr int NT = (PAIR_TILE / RI) * (PAIR_TILE / RJ);
-    hipLaunchKernelGGL((k_pairs<OP, T, RI, RJ>), grid, dim3(NT), 0, st, (const T*)a.U, (const T*)a.Cn, a.rows, a.ldn,
+    constexpr int NT = (TT / RI) * (TT / RJ);
+    hipLaunchKernelGGL((k_pairs<OP, T, RI, RJ, TT>), grid, dim3(NT), 0, st, (const T*)a.U, (const T*)a.Cn, a.rows, a.ldn,
                        (const T*)a.len, a.K, a.n, a.S, a.totals, a.alpha, (int)a.normalized, a.out, (int)a.out_fp64);
     return hipGetLastError();
 }
 
+// Two-accumulator ops (generalized, VAW) keep 4 x 4 pairs per thread on 64 x 64
+// tiles: their FP32 + FP64 accumulators for 4 x 8 pairs do not fit in registers.
 template <class T>
 hipError_t launch_pairs_prec(PairOp op, const PairArgs& a, hipStream_t st) {
     switch (op) {
     case PairOp::weighted: return launch_pairs_t<PairOp::weighted, T, 8, 8>(a, st);
     case PairOp::unweighted_plain: return launch_pairs_t<PairOp::unweighted_plain, T, 4, 8>(a, st);
-    case PairOp::generalized: return launch_pairs_t<PairOp::generalized, T, 4, 8>(a, st);
-    case PairOp::vaw_weighted: return launch_pairs_t<PairOp::vaw_weighted, T, 4, 8>(a, st);
-    case PairOp::vaw_generalized: return launch_pairs_t<PairOp::vaw_generalized, T, 4, 8>(a, st);
-    case PairOp::vaw_unweighted: return launch_pairs_t<PairOp::vaw_unweighted, T, 4, 8>(a, st);
+    case PairOp::generalized: return launch_pairs_t<PairOp::generalized, T, 4, 4, 64>(a, st);
+    case PairOp::vaw_weighted: return launch_pairs_t<PairOp::vaw_weighted, T, 4, 4, 64>(a, st);
+    case PairOp::vaw_generalized: return launch_pairs_t<PairOp::vaw_generalized, T, 4, 4, 64>(a, st);
+    case PairOp::vaw_unweighted: return launch_pairs_t<PairOp::vaw_unweighted, T, 4, 4, 64>(a, st);
     }
     return hipErrorInvalidValue;
 }
