@@ -147,6 +147,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of reference CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain dense pair-tile kernels")
+    ap.add_argument("--sequential", action="store_true",
+                    help="keep one plan in device memory at a time (for workloads whose plans do not fit together)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -170,35 +172,75 @@ def main():
         f"({time.perf_counter() - t0:.1f} s to generate)")
     inp = ssu.Inputs(*tree, *table)
     specs = parse_methods(args.methods)
-    plans = {label: ssu.Plan(inp, m, **kw) for label, (m, kw) in specs.items()}
-    for name, p in plans.items():
-        log(f"plan {name}: main kernel {p.main_kernel()}, pair-nodes {p.pair_nodes():.4g}")
-    for _ in range(args.warmup):
-        for p in plans.values():
-            p.execute()
-    log("warmup done")
-    times = {k: [] for k in plans}
-    t_start = time.perf_counter()
-    for step in range(args.steps):
+    import oracle as O
+    rng = np.random.default_rng(7)
+    idx = np.sort(rng.choice(n, size=min(48, n), replace=False))
+    sub = sub_table(table, idx)
+    otree = O.Tree(tree[0].astype(np.uint8), tree[1], tree[2])
+    errors = {}
+
+    def check(name, p):
+        """max relative error of the plan's matrix on a 48-sample subset, against the FP64 oracle"""
+        m = p.matrix()
+        ref = O.matrix(otree, O.Table(*sub), specs[name][0], **specs[name][1])
+        errors[name] = max_rel_err(m[np.ix_(idx, idx)], ref)
+
+    times = {k: [] for k in specs}
+    plans_info = {}
+    if not args.sequential:
+        # every matrix of a step computed back to back, all plans resident in HBM
+        plans = {label: ssu.Plan(inp, m, **kw) for label, (m, kw) in specs.items()}
         for name, p in plans.items():
-            p.execute()
-            times[name].append(p.timings())
-        log(f"step {step + 1}/{args.steps} done")
-    elapsed = time.perf_counter() - t_start
+            log(f"plan {name}: main kernel {p.main_kernel()}, pair-nodes {p.pair_nodes():.4g}")
+        for _ in range(args.warmup):
+            for p in plans.values():
+                p.execute()
+        log("warmup done")
+        t_start = time.perf_counter()
+        for step in range(args.steps):
+            for name, p in plans.items():
+                p.execute()
+                times[name].append(p.timings())
+            log(f"step {step + 1}/{args.steps} done")
+        elapsed = time.perf_counter() - t_start
+        for name, p in plans.items():
+            plans_info[name] = (p.main_kernel(), p.pair_nodes(), p.kernel_terms())
+            check(name, p)
+        for p in plans.values():
+            p.close()
+    else:
+        # one plan resident at a time (workloads whose plans do not fit in HBM together);
+        # a step is the sum of the per-plan step times
+        elapsed = 0.0
+        for name, (m, kw) in specs.items():
+            p = ssu.Plan(inp, m, **kw)
+            log(f"plan {name}: main kernel {p.main_kernel()}, pair-nodes {p.pair_nodes():.4g}")
+            for _ in range(args.warmup):
+                p.execute()
+            t_start = time.perf_counter()
+            for step in range(args.steps):
+                p.execute()
+                times[name].append(p.timings())
+            elapsed += time.perf_counter() - t_start
+            log(f"plan {name}: {args.steps} steps done")
+            plans_info[name] = (p.main_kernel(), p.pair_nodes(), p.kernel_terms())
+            check(name, p)
+            p.close()
+    log(f"max rel err vs FP64 oracle: {errors}")
     ms_per_step = 1000.0 * elapsed / args.steps
-    pair_nodes = {k: p.pair_nodes() for k, p in plans.items()}
+    pair_nodes = {k: v[1] for k, v in plans_info.items()}
     value = sum(pair_nodes.values()) / (elapsed / args.steps)
 
     def avg(name, key):
         return float(np.mean([t[key] for t in times[name]]))
 
     kernels = {}
-    for name, p in plans.items():
-        kernels[name] = {"kernel": p.main_kernel(), "main_ms": avg(name, "main"), "embed_ms": avg(name, "embed"),
+    for name, (kern, pn, terms) in plans_info.items():
+        kernels[name] = {"kernel": kern, "main_ms": avg(name, "main"), "embed_ms": avg(name, "embed"),
                          "prep_ms": avg(name, "prep"), "combine_ms": avg(name, "combine"),
                          "device_total_ms": avg(name, "total"),
-                         "pair_node_per_s": pair_nodes[name] / (avg(name, "total") / 1000.0),
-                         "kernel_terms": p.kernel_terms()}
+                         "pair_node_per_s": pn / (avg(name, "total") / 1000.0),
+                         "kernel_terms": terms}
         kernels[name]["terms_per_pair_node"] = kernels[name]["kernel_terms"] / max(pair_nodes[name], 1)
     # dominant kernel: the one with the largest main-kernel time
     dom = max(kernels, key=lambda k: kernels[k]["main_ms"])
@@ -229,26 +271,10 @@ def main():
         if tr and not args.samples and not args.tips:
             roofline["traffic"] = tr
 
-    # accuracy against the FP64 oracle on a sample subset
-    import oracle as O
-    rng = np.random.default_rng(7)
-    idx = np.sort(rng.choice(n, size=min(48, n), replace=False))
-    sub = sub_table(table, idx)
-    otree = O.Tree(tree[0].astype(np.uint8), tree[1], tree[2])
-    errors = {}
-    for name, p in plans.items():
-        m = p.matrix()
-        ref = O.matrix(otree, O.Table(*sub), specs[name][0], **specs[name][1])
-        errors[name] = max_rel_err(m[np.ix_(idx, idx)], ref)
-        del m
-    log(f"max rel err vs FP64 oracle: {errors}")
-
-    for p in plans.values():
-        p.close()
     plain = None
     if not args.no_plain:
         plain = {}
-        for name in [x for x in plans if x in ("unweighted", "weighted_normalized")]:
+        for name in [x for x in specs if x in ("unweighted", "weighted_normalized")]:
             pp = ssu.Plan(inp, name, plain_kernel=1)
             pp.execute()
             pp.execute()
@@ -283,9 +309,9 @@ def main():
         "dtype": "int8 MFMA (unweighted, exact) + fp32 VALU terms with fp64 merges (weighted)",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: random binary tree {len(tree[0]) // 4 + 1} tips, "
-                               f"{n} samples, {' + '.join(plans)}, full n x n matrices",
+                               f"{n} samples, {' + '.join(specs)}, full n x n matrices",
                    "samples": n, "tree_nodes": len(tree[0]) // 2, "nnz": int(table[4].size),
-                   "methods": list(plans)},
+                   "methods": list(specs)},
         "max_rel_err_vs_fp64": errors,
         "kernels": kernels,
         "plain_dense_kernels": plain,
