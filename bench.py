@@ -134,12 +134,104 @@ def copy_probe(nbytes):
     log(f"copy probe: {nbytes} bytes device-to-device, twice")
 
 
+def bench_rarefy(args):
+    """Batched rarefaction: every table's replicates x methods replayed from one HIP graph per table."""
+    import ssu
+    import synth
+    import oracle as O
+    c = synth.RAREFY
+    t0 = time.perf_counter()
+    tree, tables = synth.rarefy_workload(tips=args.tips)
+    log(f"workload rarefy: {len(tree[0]) // 2} nodes, tables {[len(t[1]) for t in tables]} "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+    specs = parse_methods(args.methods)
+    methods = list(specs)
+    reps = args.reps or c["reps"]
+    batches = []
+    for k, t in enumerate(tables):
+        inp = ssu.Inputs(*tree, *t)
+        b = ssu.Rarefaction(inp, [specs[m][0] for m in methods], c["depth"], reps, with_replacement=True,
+                            seed=1000 + k, **(specs[methods[0]][1]))
+        batches.append((inp, b, t))
+        log(f"table {k}: {len(t[1])} samples -> {b.n} kept, pair-nodes {b.pair_nodes():.4g}")
+    for _ in range(max(args.warmup, 1)):        # the first execute runs eagerly and captures the graph
+        for _, b, _ in batches:
+            b.execute()
+    log("warmup done (graphs captured)")
+    dev_ms = [[] for _ in batches]
+    t_start = time.perf_counter()
+    for step in range(args.steps):
+        for k, (_, b, _) in enumerate(batches):
+            b.execute()
+            dev_ms[k].append(b.last_ms())
+        log(f"step {step + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t_start
+    assert all(b.graph_used() for _, b, _ in batches)
+    pair_nodes = sum(b.pair_nodes() for _, b, _ in batches)
+    value = pair_nodes / (elapsed / args.steps)
+    # accuracy: replicate 0 of the smallest table, 48 samples, against the FP64 oracle on its counts
+    _, b0, t0_ = batches[0]
+    si, oi, ip, ix = b0.kept()
+    rt = ([t0_[0][i] for i in oi], [t0_[1][s] for s in si], ip, ix, b0.counts(0))
+    rng = np.random.default_rng(7)
+    idx = np.sort(rng.choice(b0.n, size=min(48, b0.n), replace=False))
+    sub = sub_table(rt, idx)
+    otree = O.Tree(tree[0].astype(np.uint8), tree[1], tree[2])
+    errors = {}
+    for k, m in enumerate(methods):
+        ref = O.matrix(otree, O.Table(*sub), specs[m][0], **specs[m][1])
+        errors[m] = max_rel_err(b0.matrix(k, 0)[np.ix_(idx, idx)], ref)
+    log(f"max rel err vs FP64 oracle (table 0, replicate 0): {errors}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the reference's per-replicate work: one replicate of the 1,536-sample table
+        _, b1, t1_ = batches[1]
+        si, oi, ip, ix = b1.kept()
+        rt1 = ([t1_[0][i] for i in oi], [t1_[1][s] for s in si], ip, ix, b1.counts(0))
+        kind, threads, t_full, per = cpu_baseline(tree, rt1, specs, args.cpu_budget)
+        cpu = {"value": (b1.pair_nodes() / reps) / t_full, "unit": "pair-node updates/s", "cores": threads,
+               "kind": kind,
+               "sample": f"one replicate of the {b1.n}-sample table: stripes [0,1) and [0,3) of each matrix "
+                         f"(FP32 reference kernels, OpenMP), extrapolated to all {(b1.n + 1) // 2} stripes; "
+                         f"host subsampling not timed",
+               "per_method": per}
+    out = {
+        "metric": "pair-node updates/sec, batched rarefaction (4 tables x 32 replicates, depth 1000)",
+        "value": value,
+        "unit": "pair-node updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8 MFMA (unweighted, exact) + fp32 VALU terms with fp64 merges (weighted)",
+        "data": "synthetic",
+        "config": {"workload": f"rarefy_20k_4tables: random binary tree {len(tree[0]) // 4 + 1} tips, tables of "
+                               f"{', '.join(str(len(t[1])) for t in tables)} samples at 2% density, "
+                               f"{reps} replicates at depth {c['depth']} with replacement, {' + '.join(methods)}",
+                   "kept_samples": [b.n for _, b, _ in batches], "replicates": reps, "depth": c["depth"],
+                   "methods": methods},
+        "max_rel_err_vs_fp64": errors,
+        "device_ms_per_table": [float(np.mean(v)) for v in dev_ms],
+        "pair_nodes_per_step": pair_nodes,
+        "roofline": None,
+        "roofline_note": "one HIP graph per table (sampler + every replicate's matrices); per-kernel "
+                         "durations come from the rocprofv3 kernel trace",
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="unw_100k_16384")
+    ap.add_argument("--workload", default="unw_100k_16384",
+                    help="unw_100k_16384, cpu_1k_256, w_50k_8192, large_200k_65536 or rarefy_20k_4tables")
+    ap.add_argument("--reps", type=int, default=None, help="rarefaction replicates (rarefy_20k_4tables)")
     ap.add_argument("--methods", default="unweighted,weighted_normalized",
                     help="comma list of method[:alpha=A][:vaw]; one step runs every listed matrix")
     ap.add_argument("--samples", type=int, default=None)
@@ -160,6 +252,12 @@ def main():
     import ssu
     import synth
 
+    if args.workload == "rarefy_20k_4tables":
+        if not ssu.device_available():
+            print("bench.py: no HIP device visible", file=sys.stderr)
+            sys.exit(1)
+        bench_rarefy(args)
+        return
     if args.copy_probe:
         copy_probe(1 << 30)
     if not ssu.device_available():

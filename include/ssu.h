@@ -325,6 +325,40 @@ SSU_EXTERN uint64_t ssu_plan_kernel_terms(const ssu_plan_t* plan);
 SSU_EXTERN const char* ssu_plan_main_kernel(const ssu_plan_t* plan);
 SSU_EXTERN void ssu_plan_destroy(ssu_plan_t** plan);
 
+/* Rarefaction replicates on the device (the reference subsamples on the host,
+ * src/biom_subsampled.cpp:155-321, one table per call).  Samples with a total
+ * below depth are dropped; each of n_reps replicates draws every kept sample
+ * down to depth (with replacement: multinomial in proportion to the counts;
+ * without: depth of the sample's whole-count items), then computes every listed
+ * method's full matrix.  The first execute runs eagerly and captures the whole
+ * sequence into a HIP graph, later executes replay it.  Random numbers come from
+ * a counter-based generator seeded with `seed`: statistical, not bitwise,
+ * parity with the reference's sampler.  bypass_tips is not supported here. */
+typedef struct ssu_rarefy ssu_rarefy_t;
+SSU_EXTERN ComputeStatus ssu_rarefy_create_inmem(const support_biom_t* table_data, const support_bptree_t* tree_data,
+                                                 const char* const* unifrac_methods, int n_methods,
+                                                 bool variance_adjust, double alpha, bool normalize_sample_counts,
+                                                 unsigned int depth, bool with_replacement, uint64_t seed,
+                                                 unsigned int n_reps, ssu_rarefy_t** out);
+SSU_EXTERN ComputeStatus ssu_rarefy_execute(ssu_rarefy_t* batch);
+/* kept samples / observations / CSR entries of the rarefied tables */
+SSU_EXTERN unsigned int ssu_rarefy_n_samples(const ssu_rarefy_t* batch);
+SSU_EXTERN unsigned int ssu_rarefy_n_obs(const ssu_rarefy_t* batch);
+SSU_EXTERN uint64_t ssu_rarefy_nnz(const ssu_rarefy_t* batch);
+/* original sample index of each kept sample, original observation row of each
+ * kept row, and the kept CSR structure (indptr[n_obs+1], indices[nnz]) */
+SSU_EXTERN void ssu_rarefy_kept(const ssu_rarefy_t* batch, uint32_t* sample_index, uint32_t* obs_index,
+                                uint32_t* indptr, uint32_t* indices);
+/* replicate rep's counts, aligned with the kept CSR (nnz values) */
+SSU_EXTERN ComputeStatus ssu_rarefy_copy_counts(const ssu_rarefy_t* batch, unsigned int rep, double* data);
+/* replicate rep's matrix of method `method` (index into unifrac_methods), n x n
+ * doubles for "_fp64" names, floats otherwise */
+SSU_EXTERN ComputeStatus ssu_rarefy_copy_matrix(const ssu_rarefy_t* batch, int method, unsigned int rep, void* host_out);
+SSU_EXTERN double ssu_rarefy_last_ms(const ssu_rarefy_t* batch);     /* device time of the last execute */
+SSU_EXTERN int ssu_rarefy_graph_used(const ssu_rarefy_t* batch);     /* 1: the last execute replayed the graph */
+SSU_EXTERN uint64_t ssu_rarefy_pair_nodes(const ssu_rarefy_t* batch);
+SSU_EXTERN void ssu_rarefy_destroy(ssu_rarefy_t** batch);
+
 /* Host-side tree handling only (no device): sheared + collapsed postorder
  * arrays of tree_data restricted to obs_ids.  Arrays hold n_parens/2 entries.
  * Returns the number of non-root nodes, or -1. */

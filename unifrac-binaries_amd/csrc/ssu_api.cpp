@@ -20,6 +20,7 @@
 
 #include "ssu_biom.hpp"
 #include "ssu_engine.hpp"
+#include "ssu_rarefy.hpp"
 #include "ssu_results.hpp"
 #include "ssu_table.hpp"
 #include "ssu_tree.hpp"
@@ -800,6 +801,90 @@ IOStatus write_mat_from_matrix_hdf5_fp64(const char* filename, mat_full_fp64_t* 
 IOStatus write_mat_from_matrix_hdf5_fp32(const char* filename, mat_full_fp32_t* result, unsigned int pcoa_dims,
                                          int save_dist) {
     return write_full_hdf5(filename, result, pcoa_dims, save_dist, false);
+}
+
+struct ssu_rarefy {
+    Table table;
+    std::unique_ptr<ssu::RarefyBatch> batch;
+};
+
+ComputeStatus ssu_rarefy_create_inmem(const support_biom_t* td, const support_bptree_t* tr,
+                                      const char* const* methods, int n_methods, bool variance_adjust, double alpha,
+                                      bool normalize_sample_counts, unsigned int depth, bool with_replacement,
+                                      uint64_t seed, unsigned int n_reps, ssu_rarefy_t** out) {
+    if (!out) return output_error;
+    *out = nullptr;
+    if (!methods || n_methods <= 0) return unknown_method;
+    std::vector<ComputeSpec> specs;
+    for (int k = 0; k < n_methods; ++k) {
+        Method m;
+        bool fp64;
+        if (!parse_method(methods[k], m, fp64)) return unknown_method;
+        specs.push_back(make_spec(m, fp64, variance_adjust, alpha, false, normalize_sample_counts));
+    }
+    if (!td || !tr || td->n_samples <= 0 || td->n_obs <= 0) return table_empty;
+    auto r = std::make_unique<ssu_rarefy>();
+    r->table = Table::borrow(td->obs_ids, td->sample_ids, td->indptr, td->indices, td->data, (uint32_t)td->n_obs,
+                             (uint32_t)td->n_samples);
+    PhyloTree tree = PhyloTree::from_arrays(tr->structure, tr->lengths, tr->names, tr->n_parens);
+    const ComputeStatus v = validate(r->table, tree);
+    if (v != okay) return v;
+    std::string err;
+    r->batch = ssu::RarefyBatch::create(r->table, tree, specs, depth, with_replacement, seed, n_reps, &err);
+    if (!r->batch) {
+        set_error(err);
+        return err.find("reaches the rarefaction depth") != std::string::npos ? table_empty : output_error;
+    }
+    *out = r.release();
+    return okay;
+}
+
+ComputeStatus ssu_rarefy_execute(ssu_rarefy_t* r) {
+    if (!r || !r->batch) return output_error;
+    std::string err;
+    if (!r->batch->execute(&err)) { set_error(err); return output_error; }
+    return okay;
+}
+
+unsigned int ssu_rarefy_n_samples(const ssu_rarefy_t* r) { return r && r->batch ? r->batch->n_samples() : 0; }
+unsigned int ssu_rarefy_n_obs(const ssu_rarefy_t* r) { return r && r->batch ? r->batch->n_obs() : 0; }
+uint64_t ssu_rarefy_nnz(const ssu_rarefy_t* r) { return r && r->batch ? r->batch->nnz() : 0; }
+
+void ssu_rarefy_kept(const ssu_rarefy_t* r, uint32_t* sample_index, uint32_t* obs_index, uint32_t* indptr,
+                     uint32_t* indices) {
+    if (!r || !r->batch) return;
+    const auto& b = *r->batch;
+    const Table& t = b.kept();
+    if (sample_index) std::copy(b.sample_index().begin(), b.sample_index().end(), sample_index);
+    if (obs_index) std::copy(b.obs_index().begin(), b.obs_index().end(), obs_index);
+    const uint32_t base = t.n_obs ? t.indptr[0] : 0;
+    if (indptr)
+        for (uint32_t i = 0; i <= t.n_obs; ++i) indptr[i] = t.indptr[i] - base;
+    if (indices) std::copy(t.indices + base, t.indices + base + t.nnz(), indices);
+}
+
+ComputeStatus ssu_rarefy_copy_counts(const ssu_rarefy_t* r, unsigned int rep, double* data) {
+    if (!r || !r->batch || !data) return output_error;
+    std::string err;
+    if (!r->batch->copy_counts(rep, data, &err)) { set_error(err); return output_error; }
+    return okay;
+}
+
+ComputeStatus ssu_rarefy_copy_matrix(const ssu_rarefy_t* r, int method, unsigned int rep, void* host_out) {
+    if (!r || !r->batch || !host_out || method < 0) return output_error;
+    std::string err;
+    if (!r->batch->copy_matrix((size_t)method, rep, host_out, &err)) { set_error(err); return output_error; }
+    return okay;
+}
+
+double ssu_rarefy_last_ms(const ssu_rarefy_t* r) { return r && r->batch ? r->batch->last_ms() : 0.0; }
+int ssu_rarefy_graph_used(const ssu_rarefy_t* r) { return r && r->batch && r->batch->graph_used() ? 1 : 0; }
+uint64_t ssu_rarefy_pair_nodes(const ssu_rarefy_t* r) { return r && r->batch ? r->batch->pair_nodes() : 0; }
+
+void ssu_rarefy_destroy(ssu_rarefy_t** r) {
+    if (!r || !*r) return;
+    delete *r;
+    *r = nullptr;
 }
 
 long ssu_biom_n_samples(const char* biom_filename) {

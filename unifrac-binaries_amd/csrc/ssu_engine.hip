@@ -110,10 +110,22 @@ struct PlanImpl {
     DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_order, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
         d_faith, d_stripes;
 
+    hipStream_t own_st = nullptr;    // created with the plan; st may point at a caller's stream
+    // per-execution inputs and output (nullptr: the plan's own table data / totals / result)
+    const double* x_data = nullptr;
+    const double* x_totals = nullptr;
+    void* x_out = nullptr;
+    bool quiet = false;              // no events, no host synchronisation (stream capture)
+
+    const double* data_ptr() const { return x_data ? x_data : d_data.as<double>(); }
+    const double* totals_ptr() const { return x_totals ? x_totals : d_totals.as<double>(); }
+    void* out_ptr() const { return x_out ? x_out : d_out.p; }
+    hipError_t mark(int i) { return quiet ? hipSuccess : hipEventRecord(ev[i], st); }
+
     ~PlanImpl() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
-        if (st) (void)hipStreamDestroy(st);
+        if (own_st) (void)hipStreamDestroy(own_st);
     }
 
     bool build_gram_columns();
@@ -199,7 +211,7 @@ bool PlanImpl::embed(DevMem& dst, bool normalize, size_t c0, size_t w, std::stri
     double* E = dst.as<double>();
     SSU_HIP(launch_embed_leaves(E, w, (uint32_t)c0, d_leaf_k.as<int32_t>(), d_leaf_row.as<int32_t>(),
                                 (uint32_t)leaf_k.size(), d_indptr.as<uint32_t>(), d_indices.as<uint32_t>(),
-                                d_data.as<double>(), d_totals.as<double>(), normalize, st));
+                                data_ptr(), totals_ptr(), normalize, st));
     for (size_t h = 0; h + 1 < level_off.size(); ++h) {
         const uint32_t a = level_off[h], b = level_off[h + 1];
         SSU_HIP(launch_embed_level(E, w, d_level_nodes.as<int32_t>() + a, b - a, d_child_ptr.as<int32_t>(),
@@ -300,8 +312,9 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
 
     std::string e2;
     if (!err) err = &e2;
-    hipError_t he = hipStreamCreateWithFlags(&P.st, hipStreamNonBlocking);
+    hipError_t he = hipStreamCreateWithFlags(&P.own_st, hipStreamNonBlocking);
     if (he != hipSuccess) { *err = std::string("hipStreamCreate: ") + hipGetErrorString(he); return nullptr; }
+    P.st = P.own_st;
     for (auto& e : P.ev)
         if (hipEventCreate(&e) != hipSuccess) { *err = "hipEventCreate failed"; return nullptr; }
 
@@ -342,12 +355,18 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     return plan;
 }
 
-bool Plan::execute(std::string* err) {
+bool Plan::execute(std::string* err) { return execute_into(nullptr, nullptr, nullptr, false, err); }
+
+bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_out, bool quiet, std::string* err) {
     PlanImpl& P = *p_;
+    P.x_data = d_data;
+    P.x_totals = d_totals;
+    P.x_out = d_out;
+    P.quiet = quiet;
     const ComputeSpec& s = P.spec;
     const size_t nn = (size_t)P.n * P.n;
     const size_t osz = s.fp64 ? sizeof(double) : sizeof(float);
-    if (!P.d_out.ensure(std::max<size_t>(nn, 1) * osz, err)) return false;
+    if (!d_out && !P.d_out.ensure(std::max<size_t>(nn, 1) * osz, err)) return false;
     const uint32_t K = (uint32_t)P.knodes.size();
     const bool single = P.blk >= P.ldn;
 
@@ -380,12 +399,12 @@ bool Plan::execute(std::string* err) {
     }
     if (P.gram && P.Kp && !P.d_Pt.ensure(P.n_pad * P.Kp, err)) return false;
 
-    SSU_HIP(hipEventRecord(P.ev[0], P.st));
+    SSU_HIP(P.mark(0));
     for (size_t c0 = 0; c0 < P.ldn; c0 += P.blk) {
         const size_t w = std::min(P.blk, P.ldn - c0);
         if (!P.embed(P.d_E, s.normalize, c0, w, err)) return false;
         if (s.vaw && !P.embed(P.d_C, false, c0, w, err)) return false;  // raw counts, src/unifrac_cmp.cpp:250-251
-        if (single) SSU_HIP(hipEventRecord(P.ev[1], P.st));
+        if (single) SSU_HIP(P.mark(1));
         if (P.gram) {
             if (P.Kp)
                 SSU_HIP(launch_presence_T(P.d_Pt.as<int8_t>(), P.Kp, P.d_E.as<double>(), w, (uint32_t)c0,
@@ -400,8 +419,8 @@ bool Plan::execute(std::string* err) {
                                            (uint32_t)c0, P.d_knodes.as<int32_t>(), nullptr, K, P.n, P.st));
         }
     }
-    if (!single) SSU_HIP(hipEventRecord(P.ev[1], P.st));   // blocked: embed and prep share one phase
-    SSU_HIP(hipEventRecord(P.ev[2], P.st));
+    if (!single) SSU_HIP(P.mark(1));   // blocked: embed and prep share one phase
+    SSU_HIP(P.mark(2));
 
     if (P.gram) {
         if (!P.d_G.ensure(P.n_pad * P.n_pad * sizeof(double), err)) return false;
@@ -410,17 +429,17 @@ bool Plan::execute(std::string* err) {
                                    P.d_chunks.as<GramChunk>(), (uint32_t)P.chunks.size(), (uint32_t)P.n_pad,
                                    P.d_order.as<uint32_t>(), (uint32_t)P.block_order.size(), P.st));
         } else {
-            SSU_HIP(hipMemsetAsync(P.d_G.p, 0, P.n_pad * P.n_pad * sizeof(double), P.st));
+            SSU_HIP(launch_fill(P.d_G.as<double>(), P.n_pad * P.n_pad, 0.0, P.st));   // no memset node in captures
         }
-        SSU_HIP(hipEventRecord(P.ev[3], P.st));
+        SSU_HIP(P.mark(3));
         const bool norm = s.method == Method::unweighted;
         if (s.fp64)
-            SSU_HIP(launch_unweighted_combine<double>(P.d_out.as<double>(), P.d_G.as<double>(), P.n_pad, P.n, norm,
+            SSU_HIP(launch_unweighted_combine<double>((double*)P.out_ptr(), P.d_G.as<double>(), P.n_pad, P.n, norm,
                                                       P.st));
         else
-            SSU_HIP(launch_unweighted_combine<float>(P.d_out.as<float>(), P.d_G.as<double>(), P.n_pad, P.n, norm,
+            SSU_HIP(launch_unweighted_combine<float>((float*)P.out_ptr(), P.d_G.as<double>(), P.n_pad, P.n, norm,
                                                      P.st));
-        SSU_HIP(hipEventRecord(P.ev[4], P.st));
+        SSU_HIP(P.mark(4));
     } else if (P.sparse) {
         SparseArgs a{};
         a.U = s.fp64 ? P.d_E.p : P.d_U.p;
@@ -432,7 +451,7 @@ bool Plan::execute(std::string* err) {
         a.n = P.n;
         a.S = P.d_S.as<double>();
         a.normalized = s.method == Method::weighted_normalized;
-        a.out = P.d_out.p;
+        a.out = P.out_ptr();
         a.out_fp64 = s.fp64;
         const size_t nb = (P.n + 63) / 64;
         if (!P.d_visits.ensure(std::max<size_t>(2 * nb * nb, 1) * sizeof(uint32_t), err)) return false;
@@ -441,16 +460,16 @@ bool Plan::execute(std::string* err) {
         a.n_blocks = (uint32_t)P.block_order.size();
         if (s.fp64) SSU_HIP(launch_pairs_sparse(true, a, P.st));
         else SSU_HIP(launch_pairs_sparse32(a, P.st));
-        SSU_HIP(hipEventRecord(P.ev[3], P.st));
-        SSU_HIP(hipEventRecord(P.ev[4], P.st));
+        SSU_HIP(P.mark(3));
+        SSU_HIP(P.mark(4));
     } else {
         PairArgs a{};
         a.n = P.n;
         a.K = K;
         a.ldn = P.ldn;
         a.alpha = s.alpha;
-        a.totals = P.d_totals.as<double>();
-        a.out = P.d_out.p;
+        a.totals = P.totals_ptr();
+        a.out = P.out_ptr();
         a.out_fp64 = s.fp64;
         a.len = P.d_klen.p;
         a.normalized = normalized;
@@ -465,10 +484,11 @@ bool Plan::execute(std::string* err) {
             a.rows = nullptr;
         }
         SSU_HIP(launch_pairs(op, s.fp64, a, P.st));
-        SSU_HIP(hipEventRecord(P.ev[3], P.st));
-        SSU_HIP(hipEventRecord(P.ev[4], P.st));
+        SSU_HIP(P.mark(3));
+        SSU_HIP(P.mark(4));
     }
-    SSU_HIP(hipEventRecord(P.ev[5], P.st));
+    SSU_HIP(P.mark(5));
+    if (P.quiet) return true;
     SSU_HIP(hipStreamSynchronize(P.st));
     for (int i = 0; i < PH_TOTAL; ++i) SSU_HIP(hipEventElapsedTime(&P.times[i], P.ev[i], P.ev[i + 1]));
     SSU_HIP(hipEventElapsedTime(&P.times[PH_TOTAL], P.ev[0], P.ev[5]));
@@ -499,6 +519,8 @@ bool Plan::copy_stripes(uint32_t start, uint32_t stop, double* const* host, std:
 
 bool Plan::faith(double* host, std::string* err) {
     PlanImpl& P = *p_;
+    P.x_data = nullptr;    // the plan's own table
+    P.x_totals = nullptr;
     if (!P.d_faith.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err)) return false;
     for (size_t c0 = 0; c0 < P.ldn && c0 < P.n; c0 += P.blk) {
         const size_t w = std::min(P.blk, P.ldn - c0);
@@ -512,6 +534,11 @@ bool Plan::faith(double* host, std::string* err) {
 }
 
 const float* Plan::timings() const { return p_->times; }
+void Plan::set_stream(void* stream) {
+    (void)hipStreamSynchronize(p_->st);   // work already queued finishes before later work goes elsewhere
+    p_->st = stream ? (hipStream_t)stream : p_->own_st;
+}
+void* Plan::stream() const { return p_->st; }
 uint32_t Plan::n_samples() const { return p_->n; }
 uint64_t Plan::processed_nodes() const { return p_->knodes.size(); }
 

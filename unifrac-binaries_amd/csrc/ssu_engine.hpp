@@ -43,6 +43,14 @@ public:
                                         std::string* err);
     // Full n x n distance matrix into device memory (FP64 if spec.fp64, FP32 otherwise).
     bool execute(std::string* err);
+    // The same with the table's CSR values, the per-sample totals and the result
+    // in caller-owned device buffers (nullptr: the plan's own).  quiet: records no
+    // events and does not synchronise, so the work can be captured into a graph;
+    // every buffer the plan needs must already exist (run it once non-quiet first).
+    bool execute_into(const double* d_data, const double* d_totals, void* d_out, bool quiet, std::string* err);
+    // Launch on a caller's HIP stream (nullptr: the plan's own stream).
+    void set_stream(void* stream);
+    void* stream() const;
     bool copy_matrix(void* host, std::string* err) const;                 // n*n elements
     bool copy_stripes(uint32_t start, uint32_t stop, double* const* host, std::string* err) const;
     // Faith PD over all non-root nodes (needs only the embedding).

@@ -962,7 +962,17 @@ hipError_t launch_pairs_prec(PairOp op, const PairArgs& a, hipStream_t st) {
 
 }  // namespace
 
+__global__ void __launch_bounds__(256) k_fill(double* __restrict__ p, size_t n, double v) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = v;
+}
+
 // ------------------------------------------------------------------ launchers
+
+hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)std::min<size_t>(grid1(n, 256), 65536)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
 
 hipError_t launch_embed_leaves(double* E, size_t ld, uint32_t col0, const int32_t* leaf_k, const int32_t* leaf_row,
                                uint32_t n_leaves, const uint32_t* indptr, const uint32_t* indices, const double* data,

@@ -24,6 +24,9 @@ constexpr int GRAM_BM = 128;   // output tile rows per workgroup
 constexpr int GRAM_BK = 128;   // K' columns per LDS stage
 constexpr int PAIR_TILE = 128; // samples per side of a VALU pair tile
 
+// p[0..n) = v (a kernel, so that it can sit in a captured graph)
+hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st);
+
 // ---------------------------------------------------------------- embedding
 // E holds the sample columns [col0, col0 + ld) of the node-major embedding
 hipError_t launch_embed_leaves(double* E, size_t ld, uint32_t col0, const int32_t* leaf_k, const int32_t* leaf_row,

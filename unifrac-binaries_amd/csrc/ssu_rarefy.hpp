@@ -1,0 +1,48 @@
+// Rarefaction replicates on the device: each replicate subsamples every kept
+// sample of one table to the same depth (src/biom_subsampled.cpp:155-262:
+// with replacement = multinomial draws in proportion to the counts; without =
+// a uniform sample of `depth` of the sample's items), then computes one or more
+// distance matrices from it.  All replicates and methods run on one stream; the
+// sequence is captured once into a HIP graph and replayed by later executions.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ssu_engine.hpp"
+
+namespace ssu {
+
+struct RarefyImpl;
+
+class RarefyBatch {
+public:
+    ~RarefyBatch();
+    // Samples whose total is below depth are dropped first (src/biom_subsampled.cpp:290).
+    static std::unique_ptr<RarefyBatch> create(const Table& table, const PhyloTree& tree,
+                                               const std::vector<ComputeSpec>& specs, uint32_t depth,
+                                               bool with_replacement, uint64_t seed, uint32_t reps,
+                                               std::string* err);
+    bool execute(std::string* err);       // all replicates x methods; graph replay after the first call
+    uint32_t n_samples() const;           // kept samples
+    uint32_t n_obs() const;
+    uint64_t nnz() const;
+    uint32_t reps() const;
+    const Table& kept() const;            // the kept-sample table (its CSR order is the counts' order)
+    const std::vector<uint32_t>& sample_index() const;   // original index of each kept sample
+    const std::vector<uint32_t>& obs_index() const;      // original observation row of each kept row
+    bool copy_matrix(size_t spec, uint32_t rep, void* host, std::string* err) const;
+    bool copy_counts(uint32_t rep, double* host, std::string* err) const;
+    bool out_fp64(size_t spec) const;
+    float last_ms() const;                // device time of the last execute()
+    bool graph_used() const;              // the last execute() replayed the captured graph
+    uint64_t pair_nodes() const;          // sum over replicates and methods of n(n-1)/2 * processed nodes
+
+private:
+    RarefyBatch();
+    std::unique_ptr<RarefyImpl> p_;
+};
+
+}  // namespace ssu

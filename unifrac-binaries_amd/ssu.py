@@ -83,6 +83,9 @@ EXPORTS = [
     "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
     "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
     "ssu_newick_to_bp", "ssu_biom_n_samples", "ssu_device_available", "ssu_last_error",
+    "ssu_rarefy_create_inmem", "ssu_rarefy_execute", "ssu_rarefy_n_samples", "ssu_rarefy_n_obs", "ssu_rarefy_nnz",
+    "ssu_rarefy_kept", "ssu_rarefy_copy_counts", "ssu_rarefy_copy_matrix", "ssu_rarefy_last_ms",
+    "ssu_rarefy_graph_used", "ssu_rarefy_pair_nodes", "ssu_rarefy_destroy",
 ]
 
 _lib = None
@@ -116,6 +119,14 @@ def lib():
         L.write_mat_from_matrix_hdf5_fp32.restype = C.c_int
         L.ssu_biom_n_samples.restype = C.c_long
         L.read_partial_one_stripe.argtypes = [C.POINTER(PartialDynMat), C.c_uint32]
+        for n in ("ssu_rarefy_create_inmem", "ssu_rarefy_execute", "ssu_rarefy_copy_counts", "ssu_rarefy_copy_matrix",
+                  "ssu_rarefy_graph_used"):
+            getattr(L, n).restype = C.c_int
+        L.ssu_rarefy_n_samples.restype = C.c_uint
+        L.ssu_rarefy_n_obs.restype = C.c_uint
+        L.ssu_rarefy_nnz.restype = C.c_uint64
+        L.ssu_rarefy_pair_nodes.restype = C.c_uint64
+        L.ssu_rarefy_last_ms.restype = C.c_double
         _lib = L
     return _lib
 
@@ -253,6 +264,76 @@ class Plan:
     def close(self):
         if self.h:
             lib().ssu_plan_destroy(C.byref(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Rarefaction:
+    """Device rarefaction replicates (extension): n_reps subsampled tables of
+    one input at one depth, each through every listed method; the first
+    execute runs eagerly and captures a HIP graph that later executes replay."""
+
+    def __init__(self, inp: Inputs, methods, depth, n_reps, with_replacement=True, seed=0, variance_adjust=False,
+                 alpha=1.0, normalize=True):
+        L = lib()
+        self._inp = inp
+        self.methods = list(methods)
+        arr, self._keep = _cstrs(self.methods)
+        self.h = C.c_void_p()
+        st = L.ssu_rarefy_create_inmem(C.byref(inp.table), C.byref(inp.tree), arr, C.c_int(len(self.methods)),
+                                       C.c_bool(variance_adjust), C.c_double(alpha), C.c_bool(normalize),
+                                       C.c_uint(depth), C.c_bool(with_replacement), C.c_uint64(seed),
+                                       C.c_uint(n_reps), C.byref(self.h))
+        if st != OKAY:
+            raise RuntimeError(f"ssu_rarefy_create_inmem failed: status {st}: {last_error()}")
+        self.n = int(L.ssu_rarefy_n_samples(self.h))
+        self.n_obs = int(L.ssu_rarefy_n_obs(self.h))
+        self.nnz = int(L.ssu_rarefy_nnz(self.h))
+        self.n_reps = n_reps
+
+    def execute(self):
+        if lib().ssu_rarefy_execute(self.h) != OKAY:
+            raise RuntimeError(f"ssu_rarefy_execute failed: {last_error()}")
+
+    def kept(self):
+        """(sample_index, obs_index, indptr, indices) of the kept table."""
+        si = np.zeros(self.n, np.uint32)
+        oi = np.zeros(self.n_obs, np.uint32)
+        ip = np.zeros(self.n_obs + 1, np.uint32)
+        ix = np.zeros(max(self.nnz, 1), np.uint32)
+        lib().ssu_rarefy_kept(self.h, *(a.ctypes.data_as(C.c_void_p) for a in (si, oi, ip, ix)))
+        return si, oi, ip, ix[:self.nnz]
+
+    def counts(self, rep):
+        out = np.zeros(max(self.nnz, 1))
+        if lib().ssu_rarefy_copy_counts(self.h, C.c_uint(rep), out.ctypes.data_as(C.c_void_p)) != OKAY:
+            raise RuntimeError(last_error())
+        return out[:self.nnz]
+
+    def matrix(self, method, rep):
+        k = self.methods.index(method) if isinstance(method, str) else int(method)
+        out = np.empty((self.n, self.n), np.float64 if self.methods[k].endswith("_fp64") else np.float32)
+        if lib().ssu_rarefy_copy_matrix(self.h, C.c_int(k), C.c_uint(rep), out.ctypes.data_as(C.c_void_p)) != OKAY:
+            raise RuntimeError(last_error())
+        return out
+
+    def last_ms(self) -> float:
+        return float(lib().ssu_rarefy_last_ms(self.h))
+
+    def graph_used(self) -> bool:
+        return bool(lib().ssu_rarefy_graph_used(self.h))
+
+    def pair_nodes(self) -> int:
+        return int(lib().ssu_rarefy_pair_nodes(self.h))
+
+    def close(self):
+        if self.h:
+            lib().ssu_rarefy_destroy(C.byref(self.h))
             self.h = C.c_void_p()
 
     def __del__(self):

@@ -143,6 +143,22 @@ CONFIGS = {
 }
 
 
+# Batched rarefaction: four tables of uneven sample counts on one 20k-tip tree,
+# 2% density, lognormal counts; 32 replicates each at depth 1000 with replacement.
+RAREFY = dict(tips=20_000, tables=(512, 1536, 3000, 6000), density=0.02, counts="lognormal", depth=1000, reps=32,
+              seed=20261204)
+
+
+def rarefy_workload(tables=None, tips=None):
+    """(tree arrays, [table arrays, ...]) of the batched rarefaction workload."""
+    c = dict(RAREFY)
+    tips = tips or c["tips"]
+    tree = random_tree(tips, c["seed"])
+    sizes = tables or c["tables"]
+    out = [random_table(tips, n, c["density"], c["seed"] + 1 + k, None, c["counts"]) for k, n in enumerate(sizes)]
+    return tree, out
+
+
 def workload(name: str, samples: int | None = None, tips: int | None = None):
     """(tree arrays, table arrays) for a named workload; sizes may be overridden for smaller runs."""
     c = dict(CONFIGS[name])
