@@ -274,6 +274,74 @@ SSU_EXTERN IOStatus write_mat_from_matrix_hdf5_fp64(const char* filename, mat_fu
                                                     unsigned int pcoa_dims, int save_dist);  /* api.hpp:722 */
 SSU_EXTERN IOStatus write_mat_from_matrix_hdf5_fp32(const char* filename, mat_full_fp32_t* result,
                                                     unsigned int pcoa_dims, int save_dist);  /* api.hpp:752 */
+/* As above plus the statistics datasets (stat_methods, stat_test_names,
+ * stat_grouping_names, stat_n_groups, stat_values, stat_pvalues,
+ * stat_n_permutations) when stat_n_vals > 0. */
+SSU_EXTERN IOStatus write_mat_from_matrix_hdf5_fp64_v2(const char* output_filename, mat_full_fp64_t* result,
+                                                       unsigned int pcoa_dims, int save_dist, unsigned int stat_n_vals,
+                                                       const char** stat_method_arr, const char** stat_name_arr,
+                                                       const double* stat_val_arr, const double* stat_pval_arr,
+                                                       const unsigned int* stat_perm_count_arr,
+                                                       const char** stat_group_name_arr,
+                                                       const unsigned int* stat_group_count_arr);   /* api.hpp:714 */
+SSU_EXTERN IOStatus write_mat_from_matrix_hdf5_fp32_v2(const char* output_filename, mat_full_fp32_t* result,
+                                                       unsigned int pcoa_dims, int save_dist, unsigned int stat_n_vals,
+                                                       const char** stat_method_arr, const char** stat_name_arr,
+                                                       const float* stat_val_arr, const float* stat_pval_arr,
+                                                       const unsigned int* stat_perm_count_arr,
+                                                       const char** stat_group_name_arr,
+                                                       const unsigned int* stat_group_count_arr);   /* api.hpp:744 */
+
+/* One matrix from files straight to an output file.  format: hdf5 (precision
+ * of the method), hdf5_fp32, hdf5_fp64 or hdf5_nodist.  pcoa_dims and
+ * permanova_perms must be 0: PCoA and PERMANOVA are not part of this library
+ * (output_error, with the reason in ssu_last_error()). */
+SSU_EXTERN ComputeStatus unifrac_to_file_v3(const char* biom_filename, const char* tree_filename,
+                                            const char* out_filename, const char* unifrac_method,
+                                            bool variance_adjust, double alpha, bool bypass_tips,
+                                            bool normalize_sample_counts, unsigned int n_substeps, const char* format,
+                                            unsigned int subsample_depth, bool subsample_with_replacement,
+                                            unsigned int pcoa_dims, unsigned int permanova_perms,
+                                            const char* grouping_filename, const char* grouping_columns,
+                                            const char* mmap_dir);                                  /* api.hpp:524 */
+SSU_EXTERN ComputeStatus unifrac_to_txt_file_v3(const char* biom_filename, const char* tree_filename,
+                                                const char* out_filename, const char* unifrac_method,
+                                                bool variance_adjust, double alpha, bool bypass_tips,
+                                                bool normalize_sample_counts, unsigned int n_substeps,
+                                                const char* mmap_dir);                              /* api.hpp:533 */
+SSU_EXTERN ComputeStatus unifrac_to_file_v2(const char* biom_filename, const char* tree_filename,
+                                            const char* out_filename, const char* unifrac_method,
+                                            bool variance_adjust, double alpha, bool bypass_tips,
+                                            unsigned int n_substeps, const char* format, unsigned int subsample_depth,
+                                            bool subsample_with_replacement, unsigned int pcoa_dims,
+                                            unsigned int permanova_perms, const char* grouping_filename,
+                                            const char* grouping_columns, const char* mmap_dir);    /* api.hpp:539 */
+SSU_EXTERN ComputeStatus unifrac_to_file(const char* biom_filename, const char* tree_filename,
+                                         const char* out_filename, const char* unifrac_method, bool variance_adjust,
+                                         double alpha, bool bypass_tips, unsigned int n_substeps, const char* format,
+                                         unsigned int pcoa_dims, const char* mmap_dir);             /* api.hpp:548 */
+
+/* Rarefaction replicates to one HDF5 file (format "SUJK", version "2023.03"):
+ * "order" and, unless format is hdf5_nodist, "matrix:<i>" for i < n_subsamples.
+ * The replicates are drawn and computed on the device, from one HIP graph per
+ * batch of replicates.  pcoa_dims and permanova_perms must be 0. */
+SSU_EXTERN ComputeStatus unifrac_multi_to_file_v3(const char* biom_filename, const char* tree_filename,
+                                                  const char* out_filename, const char* unifrac_method,
+                                                  bool variance_adjust, double alpha, bool bypass_tips,
+                                                  bool normalize_sample_counts, unsigned int n_substeps,
+                                                  const char* format, unsigned int n_subsamples,
+                                                  unsigned int subsample_depth, bool subsample_with_replacement,
+                                                  unsigned int pcoa_dims, unsigned int permanova_perms,
+                                                  const char* grouping_filename, const char* grouping_columns,
+                                                  const char* mmap_dir);                            /* api.hpp:583 */
+SSU_EXTERN ComputeStatus unifrac_multi_to_file_v2(const char* biom_filename, const char* tree_filename,
+                                                  const char* out_filename, const char* unifrac_method,
+                                                  bool variance_adjust, double alpha, bool bypass_tips,
+                                                  unsigned int n_substeps, const char* format,
+                                                  unsigned int n_subsamples, unsigned int subsample_depth,
+                                                  bool subsample_with_replacement, unsigned int pcoa_dims,
+                                                  unsigned int permanova_perms, const char* grouping_filename,
+                                                  const char* grouping_columns, const char* mmap_dir);  /* api.hpp:592 */
 
 /* Partial-result files (stripes [start, stop) of the full matrix, LZ4 blocks,
  * magic PARTIAL_MAGIC_V2) and their merge into a full matrix. */

@@ -1,0 +1,288 @@
+"""File-to-file entry points and rarefaction multi mode.
+
+unifrac_to_file_v3 / _v2 / unifrac_to_file / unifrac_to_txt_file_v3
+(src/api.cpp:989-1165), unifrac_multi_to_file_v3 / _v2 (src/api.cpp:1444-1598),
+write_mat_from_matrix_hdf5_*_v2 (src/api.cpp:1787-1935) and the CLI's multi
+mode (src/su.cpp:472-545).
+
+CPU tests: argument and status handling that returns before any device work,
+and the HDF5 statistics datasets.  GPU tests: the matrices in the output files
+against the reference's goldens and against the one-off matrices; rarefaction
+at a depth equal to every sample's total, without replacement, keeps every
+count, so each replicate must equal the unrarefied matrix (a closed-form check
+of the multi path that does not depend on the random draws).
+"""
+import ctypes as C
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SSU = os.path.join(HERE, "..", "unifrac-binaries_amd", "bin", "ssu")
+H5PY = "/opt/conda/bin/python3.9"
+
+
+def need_h5py():
+    if not os.path.exists(H5PY):
+        pytest.skip("no h5py interpreter to read or write HDF5 files")
+
+
+def h5_dump(path):
+    """Every dataset of an HDF5 file as lists (strings decoded)."""
+    need_h5py()
+    code = ("import h5py, json, sys\n"
+            "f = h5py.File(sys.argv[1], 'r')\n"
+            "out = {}\n"
+            "for k in f:\n"
+            "    v = f[k][()]\n"
+            "    if hasattr(v, 'tolist'): v = v.tolist()\n"
+            "    if isinstance(v, list): v = [x.decode() if isinstance(x, bytes) else x for x in v]\n"
+            "    elif isinstance(v, bytes): v = v.decode()\n"
+            "    out[k] = {'value': v, 'dtype': str(f[k].dtype)}\n"
+            "print(json.dumps(out))\n")
+    r = subprocess.run([H5PY, "-c", code, path], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        pytest.skip(f"h5py unavailable: {r.stderr[-300:]}")
+    return json.loads(r.stdout)
+
+
+def write_biom(path, obs_ids, sample_ids, indptr, indices, data):
+    """A minimal BIOM 2.1 file: the datasets the loader reads (src/biom.hpp:64-72)."""
+    need_h5py()
+    payload = json.dumps({"o": list(obs_ids), "s": list(sample_ids), "p": [int(x) for x in indptr],
+                          "i": [int(x) for x in indices], "d": [float(x) for x in data]})
+    code = ("import h5py, json, sys, numpy as np\n"
+            "d = json.loads(sys.stdin.read())\n"
+            "f = h5py.File(sys.argv[1], 'w')\n"
+            "st = h5py.special_dtype(vlen=str)\n"
+            "f.create_dataset('observation/ids', data=np.array(d['o'], dtype=object), dtype=st)\n"
+            "f.create_dataset('sample/ids', data=np.array(d['s'], dtype=object), dtype=st)\n"
+            "f.create_dataset('observation/matrix/indptr', data=np.array(d['p'], dtype=np.int32))\n"
+            "f.create_dataset('observation/matrix/indices', data=np.array(d['i'], dtype=np.int32))\n"
+            "f.create_dataset('observation/matrix/data', data=np.array(d['d'], dtype=np.float64))\n"
+            "f.close()\n")
+    r = subprocess.run([H5PY, "-c", code, path], input=payload, capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        pytest.skip(f"h5py unavailable: {r.stderr[-300:]}")
+
+
+def cstr(x):
+    return None if x is None else str(x).encode()
+
+
+def to_file_v3(ssu, biom, tree, out, method, fmt, pcoa=0, perms=0, depth=0, with_rep=True, bypass=False):
+    return ssu.lib().unifrac_to_file_v3(cstr(biom), cstr(tree), cstr(out), method.encode(), False, C.c_double(1.0),
+                                        bypass, True, 1, cstr(fmt), depth, with_rep, pcoa, perms, None, None, None)
+
+
+def multi_v3(ssu, biom, tree, out, method, fmt, n_sub, depth, with_rep=True, pcoa=0, perms=0, bypass=False):
+    return ssu.lib().unifrac_multi_to_file_v3(cstr(biom), cstr(tree), cstr(out), method.encode(), False,
+                                              C.c_double(1.0), bypass, True, 1, cstr(fmt), n_sub, depth, with_rep,
+                                              pcoa, perms, None, None, None)
+
+
+@pytest.fixture(scope="module")
+def test_tre(ref_trees, tmp_path_factory):
+    p = tmp_path_factory.mktemp("trees") / "test.tre"
+    p.write_text(ref_trees["test"])
+    return str(p)
+
+
+TEST_BIOM = os.path.join(GOLDEN, "test.biom")
+
+
+# ----------------------------------------------------------------- CPU tests
+
+def test_entry_point_status_codes(ssu, test_tre, tmp_path):
+    out = str(tmp_path / "o.h5")
+    assert to_file_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "bogus") == ssu.UNKNOWN_METHOD
+    assert to_file_v3(ssu, TEST_BIOM, test_tre, out, "nope", "hdf5") == ssu.UNKNOWN_METHOD
+    assert to_file_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "hdf5", pcoa=3) == ssu.OUTPUT_ERROR
+    assert "PCoA" in ssu.last_error()
+    assert to_file_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "hdf5", perms=99) == ssu.OUTPUT_ERROR
+    assert "PERMANOVA" in ssu.last_error()
+    assert multi_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "hdf5", 3, 0) == ssu.TABLE_EMPTY
+    assert multi_v3(ssu, TEST_BIOM + ".missing", test_tre, out, "unweighted", "hdf5", 3, 5) == ssu.TABLE_MISSING
+    assert multi_v3(ssu, TEST_BIOM, test_tre + ".missing", out, "unweighted", "hdf5", 3, 5) == ssu.TREE_MISSING
+    assert multi_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "hdf5", 3, 5, pcoa=10) == ssu.OUTPUT_ERROR
+    assert multi_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "ascii", 3, 5) == ssu.UNKNOWN_METHOD
+
+
+def test_hdf5_v2_statistics_datasets(ssu, tmp_path):
+    n = 4
+    m = np.arange(n * n, dtype=np.float64).reshape(n, n)
+    m = m + m.T
+    np.fill_diagonal(m, 0)
+    ids = [f"s{i}".encode() for i in range(n)]
+    idp = (C.c_char_p * n)(*ids)
+    res = ssu.MatFullFP64(n, 0, m.ctypes.data_as(C.POINTER(C.c_double)), idp)
+    strs = lambda xs: (C.c_char_p * len(xs))(*[x.encode() for x in xs])
+    vals = np.array([1.5, 2.5])
+    pvals = np.array([0.01, 0.2])
+    perms = (C.c_uint * 2)(999, 99)
+    ngroups = (C.c_uint * 2)(3, 2)
+    out = str(tmp_path / "stats.h5")
+    st = ssu.lib().write_mat_from_matrix_hdf5_fp64_v2(
+        out.encode(), C.byref(res), 0, 1, 2, strs(["PERMANOVA", "PERMANOVA"]), strs(["pseudo-F", "pseudo-F"]),
+        vals.ctypes.data_as(C.c_void_p), pvals.ctypes.data_as(C.c_void_p), perms, strs(["g1:0", "g2:0"]), ngroups)
+    assert st == ssu.WRITE_OKAY
+    d = h5_dump(out)
+    assert d["format"]["value"] == ["BDSM"] or d["format"]["value"] == "BDSM"
+    assert d["order"]["value"] == ["s0", "s1", "s2", "s3"]
+    assert np.array_equal(np.array(d["matrix"]["value"]), m)
+    assert d["stat_methods"]["value"] == ["PERMANOVA", "PERMANOVA"]
+    assert d["stat_test_names"]["value"] == ["pseudo-F", "pseudo-F"]
+    assert d["stat_grouping_names"]["value"] == ["g1:0", "g2:0"]
+    assert d["stat_n_groups"]["value"] == [3, 2] and d["stat_n_groups"]["dtype"] == "uint32"
+    assert d["stat_values"]["value"] == [1.5, 2.5] and d["stat_values"]["dtype"] == "float64"
+    assert d["stat_pvalues"]["value"] == [0.01, 0.2]
+    assert d["stat_n_permutations"]["value"] == [999, 99]
+    st = ssu.lib().write_mat_from_matrix_hdf5_fp64_v2(out.encode(), C.byref(res), 2, 1, 0, None, None, None, None,
+                                                      None, None, None)
+    assert st == ssu.WRITE_ERROR                                 # PCoA refused
+
+
+def test_cli_multi_option_errors():
+    def run(*a):
+        return subprocess.run([SSU, *a], capture_output=True, text=True, timeout=60)
+    r = run("--mode", "multi", "-i", "x", "-t", "y", "-o", "z", "-m", "unweighted")
+    assert r.returncode != 0 and "--subsample-depth required in multi mode." in r.stderr
+    r = run("-i", "x", "-t", "y", "-o", "z", "-m", "unweighted", "--n-subsamples", "3")
+    assert r.returncode != 0 and "--n-subsamples only allowed in multi mode." in r.stderr
+    r = run("--mode", "multi", "-i", "x", "-t", "y", "-o", "z", "-m", "unweighted", "--subsample-depth", "5",
+            "--format", "ascii", "--pcoa", "0")
+    assert r.returncode != 0 and "ASCII format not supported in multi mode" in r.stderr
+    r = run("--mode", "multi", "-i", "x", "-t", "y", "-o", "z", "-m", "unweighted", "--subsample-depth", "5",
+            "--n-subsamples", "0", "--pcoa", "0")
+    assert r.returncode != 0 and "n_subsamples cannot be 0." in r.stderr
+
+
+# ----------------------------------------------------------------- GPU tests
+
+def equal_total_table(synth, tips=200, n=24, seed=5, total=60):
+    """Random tree and a table whose every sample sums to `total`."""
+    tree = synth.random_tree(tips, seed)
+    rng = np.random.default_rng(seed)
+    obs = [f"t{i}" for i in range(tips)]
+    dense = np.zeros((tips, n))
+    for s in range(n):
+        rows = rng.choice(tips, size=12, replace=False)
+        dense[rows, s] = rng.multinomial(total - 12, np.ones(12) / 12) + 1
+    ip, ix, d = [0], [], []
+    keep_obs = []
+    for i in range(tips):
+        nz = np.nonzero(dense[i])[0]
+        if nz.size:
+            keep_obs.append(obs[i])
+            ix += nz.tolist()
+            d += dense[i, nz].tolist()
+            ip.append(len(ix))
+    table = (keep_obs, [f"S{s}" for s in range(n)], np.array(ip), np.array(ix), np.array(d))
+    return tree, table
+
+
+@pytest.mark.gpu
+def test_unifrac_to_file_matches_goldens(ssu, goldens, test_tre, tmp_path):
+    out = str(tmp_path / "o.h5")
+    assert to_file_v3(ssu, TEST_BIOM, test_tre, out, "unweighted_fp64", "hdf5") == ssu.OKAY
+    d = h5_dump(out)
+    m = np.array(d["matrix"]["value"])
+    assert d["matrix"]["dtype"] == "float64" and d["order"]["value"] == [f"Sample{i}" for i in range(1, 7)]
+    exp = np.array(goldens["condensed_unweighted"]["values"])
+    assert np.max(np.abs(m[np.triu_indices(6, 1)] - exp)) < 1e-8
+    assert to_file_v3(ssu, TEST_BIOM, test_tre, out, "unweighted_fp64", "hdf5_fp32") == ssu.OKAY
+    assert h5_dump(out)["matrix"]["dtype"] == "float32"
+    assert to_file_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "hdf5_nodist") == ssu.OKAY
+    assert "matrix" not in h5_dump(out)
+    txt = str(tmp_path / "o.txt")
+    assert ssu.lib().unifrac_to_txt_file_v3(TEST_BIOM.encode(), test_tre.encode(), txt.encode(),
+                                            b"unweighted_fp64", False, C.c_double(1.0), False, True, 1, None) == 0
+    lines = open(txt).read().splitlines()
+    vals = np.array([[float(x) for x in ln.split("\t")[1:]] for ln in lines[1:]])
+    assert np.max(np.abs(vals[np.triu_indices(6, 1)] - exp)) < 1e-8
+    assert ssu.lib().unifrac_to_file(TEST_BIOM.encode(), test_tre.encode(), out.encode(), b"unweighted_fp64", False,
+                                     C.c_double(1.0), False, 1, b"hdf5_fp64", 0, None) == ssu.OKAY
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bypass", [False, True])
+@pytest.mark.parametrize("method", ["unweighted_fp64", "weighted_normalized_fp64"])
+def test_multi_full_depth_equals_one_off(ssu, synth, tmp_path, method, bypass):
+    tree, table = equal_total_table(synth)
+    biom = str(tmp_path / "t.biom")
+    write_biom(biom, *table)
+    tre = tmp_path / "t.tre"
+    tre.write_text(synth.bp_to_newick(*tree))
+    out = str(tmp_path / "multi.h5")
+    assert multi_v3(ssu, biom, str(tre), out, method, "hdf5_fp64", 3, 60, with_rep=False, bypass=bypass) == ssu.OKAY
+    d = h5_dump(out)
+    assert d["format"]["value"] in ("SUJK", ["SUJK"]) and d["version"]["value"] in ("2023.03", ["2023.03"])
+    assert d["order"]["value"] == table[1]
+    inp = ssu.Inputs(*ssu.newick_to_bp(synth.bp_to_newick(*tree)), *table)
+    st, ref = ssu.matrix_inmem(inp, method, bypass_tips=bypass)
+    assert st == ssu.OKAY
+    for i in range(3):
+        m = np.array(d[f"matrix:{i}"]["value"])
+        assert np.max(np.abs(m - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))), i
+    assert "matrix:3" not in d
+
+
+@pytest.mark.gpu
+def test_multi_with_replacement_and_dropped_samples(ssu, synth, tmp_path):
+    tree, table = equal_total_table(synth, seed=9)
+    data = table[4].copy()
+    data[table[3] == 0] = 0.5            # sample S0 now sums below 60 and must be dropped
+    table = (*table[:4], data)
+    biom = str(tmp_path / "t.biom")
+    write_biom(biom, *table)
+    tre = tmp_path / "t.tre"
+    tre.write_text(synth.bp_to_newick(*tree))
+    out = str(tmp_path / "multi.h5")
+    assert multi_v3(ssu, biom, str(tre), out, "unweighted", "hdf5", 5, 40) == ssu.OKAY
+    d = h5_dump(out)
+    assert d["order"]["value"] == table[1][1:]
+    ms = [np.array(d[f"matrix:{i}"]["value"]) for i in range(5)]
+    assert d["matrix:0"]["dtype"] == "float32"
+    for m in ms:
+        assert m.shape == (23, 23) and np.array_equal(m, m.T) and np.all(np.diag(m) == 0)
+        assert np.all((m >= 0) & (m <= 1))
+    assert any(not np.array_equal(ms[0], x) for x in ms[1:])
+    assert multi_v3(ssu, biom, str(tre), out, "unweighted", "hdf5", 2, 10_000) == ssu.TABLE_EMPTY
+
+
+@pytest.mark.gpu
+def test_one_off_subsample_on_device(ssu, synth):
+    tree, table = equal_total_table(synth, seed=13)
+    inp = ssu.Inputs(*tree, *table)
+    st, ref = ssu.matrix_inmem(inp, "weighted_normalized_fp64")
+    st2, m = ssu.matrix_inmem(inp, "weighted_normalized_fp64", subsample_depth=60, with_replacement=False)
+    assert st == st2 == ssu.OKAY
+    assert np.max(np.abs(m - ref)) <= 1e-12
+    st, m = ssu.matrix_inmem(inp, "unweighted_fp64", subsample_depth=30)
+    assert st == ssu.OKAY and m.shape == ref.shape
+    st, m = ssu.matrix_inmem(inp, "unweighted_fp64", subsample_depth=61)
+    assert st == ssu.TABLE_EMPTY
+
+
+@pytest.mark.gpu
+def test_cli_multi_mode(ssu, synth, tmp_path):
+    tree, table = equal_total_table(synth, seed=17)
+    biom = str(tmp_path / "t.biom")
+    write_biom(biom, *table)
+    tre = tmp_path / "t.tre"
+    tre.write_text(synth.bp_to_newick(*tree))
+    out = str(tmp_path / "cli.h5")
+    r = subprocess.run([SSU, "--mode", "multi", "-i", biom, "-t", str(tre), "-m", "unweighted", "-o", out,
+                        "--format", "hdf5_fp32", "--pcoa", "0", "--n-subsamples", "2", "--subsample-depth", "60",
+                        "--subsample-replacement", "without"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = h5_dump(out)
+    inp = ssu.Inputs(*ssu.newick_to_bp(synth.bp_to_newick(*tree)), *table)
+    st, ref = ssu.matrix_inmem(inp, "unweighted")
+    for i in range(2):
+        assert np.max(np.abs(np.array(d[f"matrix:{i}"]["value"]) - ref)) <= 1e-6

@@ -1,6 +1,6 @@
 // ssu: command-line front end over libssu.so, with the options and modes of
 // the reference's src/su.cpp (one-off, partial, merge-partial, check-partial,
-// partial-report).  Output formats: ascii (the default) and, when the HDF5 C
+// partial-report, multi).  Output formats: ascii (the default) and, when the HDF5 C
 // library can be loaded at run time, hdf5 / hdf5_fp32 / hdf5_fp64 /
 // hdf5_nodist without PCoA or PERMANOVA (those need --pcoa 0 and no -g/-c).
 #include <glob.h>
@@ -31,7 +31,7 @@ void usage() {
            "    -a\t\t[OPTIONAL] Generalized UniFrac alpha, default is 1.\n"
            "    -f\t\t[OPTIONAL] Bypass tips.\n"
            "    --vaw\t[OPTIONAL] Variance adjusted.\n"
-           "    --mode\t[OPTIONAL] one-off (default) | partial | partial-report | merge-partial | check-partial\n"
+           "    --mode\t[OPTIONAL] one-off (default) | partial | partial-report | merge-partial | check-partial | multi\n"
            "    --start\t[OPTIONAL] If mode==partial, the starting stripe.\n"
            "    --stop\t[OPTIONAL] If mode==partial, the stopping stripe.\n"
            "    --partial-pattern\t[OPTIONAL] If mode==merge-partial or check-partial, a glob pattern for partial outputs.\n"
@@ -42,6 +42,7 @@ void usage() {
            "    --format|-r\t[OPTIONAL] ascii (default) | hdf5 | hdf5_fp32 | hdf5_fp64 | hdf5_nodist\n"
            "    --subsample-depth\t[OPTIONAL] Depth of subsampling of the input BIOM before computing UniFrac.\n"
            "    --subsample-replacement\t[OPTIONAL] with (default) or without.\n"
+           "    --n-subsamples\t[OPTIONAL] If mode==multi, the number of subsampled matrices (default: 100).\n"
            "    --pcoa\t[OPTIONAL] Number of PCoA dimensions; only 0 is supported by this build.\n"
            "    --seed\t[OPTIONAL] Seed for the subsampling generator.\n"
            "    --diskbuf\t[OPTIONAL] Directory for a disk-backed result matrix.\n"
@@ -91,7 +92,7 @@ bool fp64_method(const std::string& m) { return m.size() > 5 && m.compare(m.size
 
 // src/su.cpp:612-640
 Format get_format(const std::string& f, const std::string& method, const std::string& mode) {
-    if (f.empty()) return mode != "multi" ? format_ascii : format_hdf5_nodist;
+    if (f.empty()) return (mode != "multi" && mode != "multiple") ? format_ascii : format_hdf5_nodist;
     if (f == "ascii") return format_ascii;
     if (f == "hdf5_fp32") return format_hdf5_fp32;
     if (f == "hdf5_fp64") return format_hdf5_fp64;
@@ -154,6 +155,35 @@ int mode_one_off(const Args& a, Format fmt, const std::string& method, unsigned 
     if (m64) destroy_mat_full_fp64(&m64);
     if (m32) destroy_mat_full_fp32(&m32);
     return rc;
+}
+
+const char* format_name(Format f) {
+    switch (f) {
+    case format_hdf5_fp32: return "hdf5_fp32";
+    case format_hdf5_fp64: return "hdf5_fp64";
+    case format_hdf5_nodist: return "hdf5_nodist";
+    default: return "ascii";
+    }
+}
+
+// src/su.cpp:472-545
+int mode_multi(const Args& a, Format fmt, const std::string& method, unsigned n_subsamples, unsigned depth,
+               bool with_replacement, bool vaw, double alpha, bool bypass, bool normalize, unsigned nsub,
+               const std::string& diskbuf) {
+    const std::string table = a.get("-i"), tree = a.get("-t"), out = a.get("-o");
+    if (out.empty()) { err("output filename missing"); return EXIT_FAILURE; }
+    if (table.empty()) { err("table filename missing"); return EXIT_FAILURE; }
+    if (tree.empty()) { err("tree filename missing"); return EXIT_FAILURE; }
+    if (method.empty()) { err("method missing"); return EXIT_FAILURE; }
+    if (depth < 1) { err("subsample_depth cannot be 0."); return EXIT_FAILURE; }
+    if (n_subsamples < 1) { err("n_subsamples cannot be 0."); return EXIT_FAILURE; }
+    if (fmt == format_ascii) { err("ASCII format not supported in multi mode"); return EXIT_FAILURE; }
+    const ComputeStatus st = unifrac_multi_to_file_v3(
+        table.c_str(), tree.c_str(), out.c_str(), method.c_str(), vaw, alpha, bypass, normalize, nsub,
+        format_name(fmt), n_subsamples, depth, with_replacement, 0, 0, nullptr, nullptr,
+        diskbuf.empty() ? nullptr : diskbuf.c_str());
+    if (st != okay) fprintf(stderr, "Compute failed in multi: %s\n", compute_message(st));
+    return st == okay ? EXIT_SUCCESS : EXIT_FAILURE;
 }
 
 // src/su.cpp:354-397
@@ -317,18 +347,15 @@ int main(int argc, char** argv) {
     }
     const std::string pcoa_s = a.get("--pcoa");
     const unsigned pcoa = pcoa_s.empty() ? 10 : (unsigned)atoi(pcoa_s.c_str());
-    if (fmt != format_ascii && pcoa > 0) {
-        err("PCoA is not available in this build: pass --pcoa 0 with HDF5 output");
-        return EXIT_FAILURE;
-    }
-    if (!a.get("-g").empty() || !a.get("-c").empty() || !a.get("--permanova").empty()) {
-        err("PERMANOVA is not available in this build");
-        return EXIT_FAILURE;
-    }
     const std::string seed_s = a.get("--seed");
     if (!seed_s.empty()) ssu_set_random_seed((unsigned)atoi(seed_s.c_str()));
+    const bool multi = mode == "multi" || mode == "multiple";
     const std::string depth_s = a.get("--subsample-depth");
+    if (depth_s.empty() && multi) { err("--subsample-depth required in multi mode."); return EXIT_FAILURE; }
     const unsigned depth = depth_s.empty() ? 0 : (unsigned)atoi(depth_s.c_str());
+    const std::string nsubs_s = a.get("--n-subsamples");
+    if (!nsubs_s.empty() && !multi) { err("--n-subsamples only allowed in multi mode."); return EXIT_FAILURE; }
+    const unsigned n_subsamples = nsubs_s.empty() ? 100 : (unsigned)atoi(nsubs_s.c_str());
     const std::string repl = a.get("--subsample-replacement");
     bool with_replacement = true;
     if (repl == "without") with_replacement = false;
@@ -337,6 +364,14 @@ int main(int argc, char** argv) {
         return EXIT_FAILURE;
     }
     const std::string diskbuf = a.get("--diskbuf");
+    if (fmt != format_ascii && pcoa > 0) {
+        err("PCoA is not available in this build: pass --pcoa 0 with HDF5 output");
+        return EXIT_FAILURE;
+    }
+    if (!a.get("-g").empty() || !a.get("-c").empty() || !a.get("--permanova").empty()) {
+        err("PERMANOVA is not available in this build");
+        return EXIT_FAILURE;
+    }
 
     if (mode.empty() || mode == "one-off")
         return mode_one_off(a, fmt, method, depth, with_replacement, vaw, alpha, bypass, normalize, nsub, diskbuf);
@@ -347,10 +382,9 @@ int main(int argc, char** argv) {
     if (mode == "merge-partial") return mode_merge_partial(a, fmt, diskbuf);
     if (mode == "check-partial") return mode_check_partial(a);
     if (mode == "partial-report") return mode_partial_report(a, (unsigned)npart, bare);
-    if (mode == "multi" || mode == "multiple") {
-        err("multi mode is not available in this build");
-        return EXIT_FAILURE;
-    }
+    if (multi)
+        return mode_multi(a, fmt, method, n_subsamples, depth, with_replacement, vaw, alpha, bypass, normalize, nsub,
+                          diskbuf);
     err("Unknown mode. Valid options are: one-off, partial, merge-partial, check-partial, partial-report, multi");
     return EXIT_SUCCESS;
 }

@@ -122,56 +122,45 @@ ComputeStatus run_full(const Table& t, const PhyloTree& tree, const ComputeSpec&
     return okay;
 }
 
-// Rarefaction: samples below the depth are dropped, the rest drawn down to
-// exactly `depth` counts; observations left empty are dropped
-// (src/biom_subsampled.cpp:288-321).  The random stream is this library's
-// own (std::mt19937_64), so only the statistics match the reference.
-Table subsample(const Table& t, uint32_t depth, bool with_replacement) {
-    std::vector<int64_t> keep(t.n_samples, -1);
-    std::vector<std::string> sids;
-    for (uint32_t s = 0; s < t.n_samples; ++s)
-        if (t.totals[s] >= depth) { keep[s] = (int64_t)sids.size(); sids.push_back(t.sample_ids[s]); }
-    // per kept sample: list of (obs, count)
-    std::vector<std::vector<std::pair<uint32_t, double>>> cols(sids.size());
-    for (uint32_t i = 0; i < t.n_obs; ++i)
-        for (uint32_t j = t.indptr[i]; j < t.indptr[i + 1]; ++j)
-            if (keep[t.indices[j]] >= 0 && t.data[j] > 0) cols[keep[t.indices[j]]].push_back({i, t.data[j]});
-    std::vector<std::vector<std::pair<uint32_t, double>>> drawn(sids.size());
-    for (size_t s = 0; s < cols.size(); ++s) {
-        auto& c = cols[s];
-        std::vector<double> out(c.size(), 0.0);
-        if (with_replacement) {
-            std::vector<double> w(c.size());
-            for (size_t q = 0; q < c.size(); ++q) w[q] = c[q].second;
-            std::discrete_distribution<size_t> dist(w.begin(), w.end());
-            for (uint32_t d = 0; d < depth; ++d) out[dist(g_rng)] += 1.0;
-        } else {
-            std::vector<uint64_t> remaining(c.size());
-            uint64_t total = 0;
-            for (size_t q = 0; q < c.size(); ++q) { remaining[q] = (uint64_t)std::floor(c[q].second); total += remaining[q]; }
-            for (uint32_t d = 0; d < depth && total > 0; ++d) {
-                uint64_t r = std::uniform_int_distribution<uint64_t>(0, total - 1)(g_rng);
-                size_t q = 0;
-                while (r >= remaining[q]) { r -= remaining[q]; ++q; }
-                remaining[q]--; total--; out[q] += 1.0;
-            }
-        }
-        for (size_t q = 0; q < c.size(); ++q)
-            if (out[q] > 0) drawn[s].push_back({c[q].first, out[q]});
-    }
-    std::vector<std::vector<std::pair<uint32_t, double>>> rows(t.n_obs);
-    for (size_t s = 0; s < drawn.size(); ++s)
-        for (auto& e : drawn[s]) rows[e.first].push_back({(uint32_t)s, e.second});
+// A rarefied table: the kept samples and rows of the batch with one
+// replicate's counts, zero entries and rows left empty dropped
+// (src/biom_subsampled.cpp:298-321, steal_nonzero).
+Table rarefied_table(const Table& k, const std::vector<double>& counts) {
+    const uint32_t base = k.n_obs ? k.indptr[0] : 0;
     std::vector<std::string> oids;
     std::vector<uint32_t> ip{0}, ix;
     std::vector<double> d;
-    for (uint32_t i = 0; i < t.n_obs; ++i) {
-        if (rows[i].empty()) continue;
-        oids.push_back(t.obs_ids[i]);
-        for (auto& e : rows[i]) { ix.push_back(e.first); d.push_back(e.second); }
-        ip.push_back((uint32_t)ix.size());
+    for (uint32_t i = 0; i < k.n_obs; ++i) {
+        const size_t before = ix.size();
+        for (uint32_t j = k.indptr[i]; j < k.indptr[i + 1]; ++j)
+            if (counts[j - base] > 0) { ix.push_back(k.indices[j]); d.push_back(counts[j - base]); }
+        if (ix.size() > before) { oids.push_back(k.obs_ids[i]); ip.push_back((uint32_t)ix.size()); }
     }
-    return Table::own(std::move(oids), std::move(sids), std::move(ip), std::move(ix), std::move(d));
+    return Table::own(std::move(oids), std::vector<std::string>(k.sample_ids), std::move(ip), std::move(ix),
+                      std::move(d));
+}
+
+// Rarefaction (src/biom_subsampled.cpp:277-321) drawn on the device: samples
+// below the depth are dropped, the rest drawn down to `depth` counts.  The
+// generator is this library's counter-based one, keyed from the
+// ssu_set_random_seed stream, so only the statistics match the reference's.
+ComputeStatus rarefied_tables(const Table& t, const PhyloTree& tree, uint32_t depth, bool with_replacement,
+                              uint32_t reps, std::vector<Table>& out) {
+    std::string err;
+    auto b = ssu::RarefyBatch::create(t, tree, {}, depth, with_replacement, g_rng(), reps, &err);
+    if (!b) {
+        if (err.find("reaches the rarefaction depth") != std::string::npos) return table_empty;
+        set_error(err);
+        return output_error;
+    }
+    if (!b->execute(&err)) { set_error(err); return output_error; }
+    std::vector<double> counts(b->nnz());
+    for (uint32_t r = 0; r < reps; ++r) {
+        if (!b->copy_counts(r, counts.data(), &err)) { set_error(err); return output_error; }
+        out.push_back(rarefied_table(b->kept(), counts));
+        if (out.back().n_obs == 0) return table_empty;
+    }
+    return okay;
 }
 
 // src/api.cpp:638-698 (one_off_matrix_T) + :700-714 (optional subsampling)
@@ -185,7 +174,10 @@ ComputeStatus matrix_common(const Table& t0, const PhyloTree& tree, const char* 
     const Table* tp = &t0;
     Table sub;
     if (subsample_depth > 0) {
-        sub = subsample(t0, subsample_depth, with_replacement);
+        std::vector<Table> subs;
+        const ComputeStatus st = rarefied_tables(t0, tree, subsample_depth, with_replacement, 1, subs);
+        if (st != okay) return st;
+        sub = std::move(subs[0]);
         if (sub.n_samples == 0 || sub.n_obs == 0) return table_empty;
         tp = &sub;
     }
@@ -801,6 +793,291 @@ IOStatus write_mat_from_matrix_hdf5_fp64(const char* filename, mat_full_fp64_t* 
 IOStatus write_mat_from_matrix_hdf5_fp32(const char* filename, mat_full_fp32_t* result, unsigned int pcoa_dims,
                                          int save_dist) {
     return write_full_hdf5(filename, result, pcoa_dims, save_dist, false);
+}
+
+namespace {
+
+// src/api.cpp:243-271 (is_fp64_method, is_fp64)
+ComputeStatus method_precision(const char* method, bool& fp64) {
+    Method m;
+    return parse_method(method, m, fp64) ? okay : unknown_method;
+}
+
+ComputeStatus format_precision(const char* method, const char* format, bool& fp64, bool& save_dist) {
+    const std::string f = format ? format : "";
+    if (f == "hdf5_fp32") { fp64 = false; save_dist = true; return okay; }
+    if (f == "hdf5_fp64") { fp64 = true; save_dist = true; return okay; }
+    if (f == "hdf5") { save_dist = true; return method_precision(method, fp64); }
+    if (f == "hdf5_nodist") { save_dist = false; return method_precision(method, fp64); }
+    return unknown_method;
+}
+
+// PCoA and PERMANOVA need an eigensolver and a grouping reader this library
+// does not carry; callers asking for them get output_error and a message.
+bool refuse_stats(unsigned int pcoa_dims, unsigned int permanova_perms) {
+    if (pcoa_dims > 0) { set_error("PCoA is not available in this build (pcoa_dims must be 0)"); return true; }
+    if (permanova_perms > 0) { set_error("PERMANOVA is not available in this build (permanova_perms must be 0)"); return true; }
+    return false;
+}
+
+// src/api.cpp:1787-1935 with the statistics datasets of :1880-1916
+template <class TMat, class TReal>
+IOStatus write_full_hdf5_stats(const char* filename, TMat* result, unsigned int pcoa_dims, int save_dist, bool fp64,
+                               unsigned int n_vals, const char* const* methods, const char* const* names,
+                               const TReal* vals, const TReal* pvals, const unsigned int* perms,
+                               const char* const* group_names, const unsigned int* group_counts) {
+    if (!filename || !result) return write_error;
+    if (pcoa_dims > 0) {
+        set_error("PCoA output is not available in this build (pcoa_dims must be 0)");
+        return write_error;
+    }
+    std::string err;
+    auto w = ssu::H5Writer::create(filename, &err);
+    if (!w) { set_error(err); return write_error; }
+    const uint32_t n = result->n_samples;
+    bool ok = w->string("format", "BDSM") && w->string("version", "2020.12") &&
+              w->strings("order", n, result->sample_ids);
+    if (ok && save_dist) ok = w->matrix("matrix", n, n, result->matrix, fp64);
+    if (ok && n_vals > 0) {
+        ok = methods && names && vals && pvals && perms && group_names && group_counts &&
+             w->strings("stat_methods", n_vals, methods) && w->strings("stat_test_names", n_vals, names) &&
+             w->strings("stat_grouping_names", n_vals, group_names) &&
+             w->u32s("stat_n_groups", n_vals, group_counts) && w->reals("stat_values", n_vals, vals, fp64) &&
+             w->reals("stat_pvalues", n_vals, pvals, fp64) && w->u32s("stat_n_permutations", n_vals, perms);
+    }
+    if (!ok) { set_error(std::string("HDF5 write failed for ") + filename); return write_error; }
+    return write_okay;
+}
+
+// Copies matrix r of method k of a batch into host memory of element type TReal.
+template <class TReal>
+bool batch_matrix(const ssu::RarefyBatch& b, uint32_t r, TReal* out, std::string* err) {
+    const size_t nn = (size_t)b.n_samples() * b.n_samples();
+    if (b.out_fp64(0) == (sizeof(TReal) == sizeof(double))) return b.copy_matrix(0, r, out, err);
+    if (b.out_fp64(0)) {
+        std::vector<double> tmp(nn);
+        if (!b.copy_matrix(0, r, tmp.data(), err)) return false;
+        for (size_t i = 0; i < nn; ++i) out[i] = (TReal)tmp[i];
+    } else {
+        std::vector<float> tmp(nn);
+        if (!b.copy_matrix(0, r, tmp.data(), err)) return false;
+        for (size_t i = 0; i < nn; ++i) out[i] = (TReal)tmp[i];
+    }
+    return true;
+}
+
+constexpr size_t MULTI_DEVICE_BUDGET = size_t(32) << 30;   // bytes of replicate matrices held on the device at once
+
+// src/api.cpp:1444-1563 (unifrac_multi_to_file_T): n_subsamples rarefied
+// tables, one matrix each, into an HDF5 file of format "SUJK" version
+// "2023.03" with datasets "order" and "matrix:<i>".  Without bypass_tips all
+// replicates of a chunk run from one device batch (one HIP graph); with it,
+// each replicate's table is sheared to its own observations first, as the
+// reference does, and computed on its own.
+template <class TReal>
+ComputeStatus multi_to_file(bool save_dist, const char* biom_filename, const char* tree_filename,
+                            const char* out_filename, const char* method_name, bool vaw, double alpha, bool bypass,
+                            bool normalize, unsigned int n_subsamples, unsigned int depth, bool with_replacement,
+                            unsigned int pcoa_dims, unsigned int permanova_perms) {
+    if (depth < 1) {
+        fprintf(stderr, "ERROR: subsampling depth cannot be 0.\n");
+        return table_empty;
+    }
+    if (!biom_filename || !file_exists(biom_filename)) return table_missing;
+    if (!tree_filename || !file_exists(tree_filename)) return tree_missing;
+    if (refuse_stats(pcoa_dims, permanova_perms)) return output_error;
+    Method m;
+    bool m64;
+    if (!parse_method(method_name, m, m64)) return unknown_method;
+    const PhyloTree tree = PhyloTree::from_newick(read_file(tree_filename).c_str());
+    Table t;
+    ComputeStatus st = load_table(biom_filename, t);
+    if (st != okay) return st;
+    st = validate(t, tree);
+    if (st != okay) return st;
+    const ComputeSpec spec = make_spec(m, m64, vaw, alpha, bypass, normalize);
+    const bool fp64 = sizeof(TReal) == sizeof(double);
+    std::string err;
+    auto w = ssu::H5Writer::create(out_filename, &err);
+    if (!w || !w->string("format", "SUJK") || !w->string("version", "2023.03")) {
+        set_error(err.empty() ? std::string("HDF5 write failed for ") + out_filename : err);
+        return output_error;
+    }
+    uint32_t n_kept = 0;
+    for (uint32_t n = 0; n < t.n_samples; ++n) n_kept += t.totals[n] >= depth;
+    const size_t per_rep = std::max<size_t>((size_t)n_kept * n_kept * (m64 ? 8 : 4), 1);
+    const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(n_subsamples, MULTI_DEVICE_BUDGET / per_rep));
+    std::vector<TReal> mat;
+    uint32_t done = 0;
+    bool order_written = false;
+    auto emit = [&](uint32_t n, const std::vector<std::string>& ids) -> bool {
+        if (!order_written) {
+            std::vector<const char*> p;
+            for (const auto& x : ids) p.push_back(x.c_str());
+            if (!w->strings("order", n, p.data())) return false;
+            order_written = true;
+        }
+        if (!save_dist) return true;
+        const std::string name = "matrix:" + std::to_string(done);
+        return w->matrix(name.c_str(), n, n, mat.data(), fp64);
+    };
+    while (done < n_subsamples) {
+        const uint32_t reps = std::min(chunk, n_subsamples - done);
+        if (!bypass) {
+            auto b = ssu::RarefyBatch::create(t, tree, {spec}, depth, with_replacement, g_rng(), reps, &err);
+            if (!b) {
+                if (err.find("reaches the rarefaction depth") != std::string::npos) return table_empty;
+                set_error(err);
+                return output_error;
+            }
+            if (!b->execute(&err)) { set_error(err); return output_error; }
+            const uint32_t n = b->n_samples();
+            mat.resize((size_t)n * n);
+            for (uint32_t r = 0; r < reps; ++r, ++done) {
+                if (!batch_matrix<TReal>(*b, r, mat.data(), &err)) { set_error(err); return output_error; }
+                if (!emit(n, b->kept().sample_ids)) { set_error("HDF5 write failed"); return output_error; }
+            }
+        } else {
+            std::vector<Table> subs;
+            st = rarefied_tables(t, tree, depth, with_replacement, reps, subs);
+            if (st != okay) return st;
+            for (const Table& sub : subs) {
+                mat.resize((size_t)sub.n_samples * sub.n_samples);
+                st = run_full<TReal>(sub, tree, spec, mat.data());
+                if (st != okay) return st;
+                if (!emit(sub.n_samples, sub.sample_ids)) { set_error("HDF5 write failed"); return output_error; }
+                ++done;
+            }
+        }
+    }
+    return okay;
+}
+
+}  // namespace
+
+IOStatus write_mat_from_matrix_hdf5_fp64_v2(const char* output_filename, mat_full_fp64_t* result,
+                                            unsigned int pcoa_dims, int save_dist, unsigned int stat_n_vals,
+                                            const char** stat_method_arr, const char** stat_name_arr,
+                                            const double* stat_val_arr, const double* stat_pval_arr,
+                                            const unsigned int* stat_perm_count_arr, const char** stat_group_name_arr,
+                                            const unsigned int* stat_group_count_arr) {
+    return write_full_hdf5_stats(output_filename, result, pcoa_dims, save_dist, true, stat_n_vals, stat_method_arr,
+                                 stat_name_arr, stat_val_arr, stat_pval_arr, stat_perm_count_arr, stat_group_name_arr,
+                                 stat_group_count_arr);
+}
+
+IOStatus write_mat_from_matrix_hdf5_fp32_v2(const char* output_filename, mat_full_fp32_t* result,
+                                            unsigned int pcoa_dims, int save_dist, unsigned int stat_n_vals,
+                                            const char** stat_method_arr, const char** stat_name_arr,
+                                            const float* stat_val_arr, const float* stat_pval_arr,
+                                            const unsigned int* stat_perm_count_arr, const char** stat_group_name_arr,
+                                            const unsigned int* stat_group_count_arr) {
+    return write_full_hdf5_stats(output_filename, result, pcoa_dims, save_dist, false, stat_n_vals, stat_method_arr,
+                                 stat_name_arr, stat_val_arr, stat_pval_arr, stat_perm_count_arr, stat_group_name_arr,
+                                 stat_group_count_arr);
+}
+
+// src/api.cpp:989-1117
+ComputeStatus unifrac_to_file_v3(const char* biom_filename, const char* tree_filename, const char* out_filename,
+                                 const char* unifrac_method, bool variance_adjust, double alpha, bool bypass_tips,
+                                 bool normalize_sample_counts, unsigned int n_substeps, const char* format,
+                                 unsigned int subsample_depth, bool subsample_with_replacement, unsigned int pcoa_dims,
+                                 unsigned int permanova_perms, const char*, const char*, const char* mmap_dir) {
+    bool fp64 = false, save_dist = true;
+    ComputeStatus rc = format_precision(unifrac_method, format, fp64, save_dist);
+    if (rc != okay) return rc;
+    if (refuse_stats(pcoa_dims, permanova_perms)) return output_error;
+    if (fp64) {
+        mat_full_fp64_t* result = nullptr;
+        rc = one_off_matrix_v3(biom_filename, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips,
+                               normalize_sample_counts, n_substeps, subsample_depth, subsample_with_replacement,
+                               mmap_dir, &result);
+        if (rc == okay && write_mat_from_matrix_hdf5_fp64(out_filename, result, 0, save_dist) != write_okay)
+            rc = output_error;
+        if (result) destroy_mat_full_fp64(&result);
+    } else {
+        mat_full_fp32_t* result = nullptr;
+        rc = one_off_matrix_fp32_v3(biom_filename, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips,
+                                    normalize_sample_counts, n_substeps, subsample_depth, subsample_with_replacement,
+                                    mmap_dir, &result);
+        if (rc == okay && write_mat_from_matrix_hdf5_fp32(out_filename, result, 0, save_dist) != write_okay)
+            rc = output_error;
+        if (result) destroy_mat_full_fp32(&result);
+    }
+    return rc;
+}
+
+ComputeStatus unifrac_to_file_v2(const char* biom_filename, const char* tree_filename, const char* out_filename,
+                                 const char* unifrac_method, bool variance_adjust, double alpha, bool bypass_tips,
+                                 unsigned int n_substeps, const char* format, unsigned int subsample_depth,
+                                 bool subsample_with_replacement, unsigned int pcoa_dims, unsigned int permanova_perms,
+                                 const char* grouping_filename, const char* grouping_columns, const char* mmap_dir) {
+    return unifrac_to_file_v3(biom_filename, tree_filename, out_filename, unifrac_method, variance_adjust, alpha,
+                              bypass_tips, true, n_substeps, format, subsample_depth, subsample_with_replacement,
+                              pcoa_dims, permanova_perms, grouping_filename, grouping_columns, mmap_dir);
+}
+
+ComputeStatus unifrac_to_file(const char* biom_filename, const char* tree_filename, const char* out_filename,
+                              const char* unifrac_method, bool variance_adjust, double alpha, bool bypass_tips,
+                              unsigned int n_substeps, const char* format, unsigned int pcoa_dims,
+                              const char* mmap_dir) {
+    return unifrac_to_file_v2(biom_filename, tree_filename, out_filename, unifrac_method, variance_adjust, alpha,
+                              bypass_tips, n_substeps, format, 0, true, pcoa_dims, 0, nullptr, nullptr, mmap_dir);
+}
+
+// src/api.cpp:1119-1165
+ComputeStatus unifrac_to_txt_file_v3(const char* biom_filename, const char* tree_filename, const char* out_filename,
+                                     const char* unifrac_method, bool variance_adjust, double alpha, bool bypass_tips,
+                                     bool normalize_sample_counts, unsigned int n_substeps, const char* mmap_dir) {
+    bool fp64 = false;
+    ComputeStatus rc = method_precision(unifrac_method, fp64);
+    if (rc != okay) return rc;
+    if (fp64) {
+        mat_full_fp64_t* result = nullptr;
+        rc = one_off_matrix_v3(biom_filename, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips,
+                               normalize_sample_counts, n_substeps, 0, true, mmap_dir, &result);
+        if (rc == okay && write_mat_from_matrix(out_filename, result) != write_okay) rc = output_error;
+        if (result) destroy_mat_full_fp64(&result);
+    } else {
+        mat_full_fp32_t* result = nullptr;
+        rc = one_off_matrix_fp32_v3(biom_filename, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips,
+                                    normalize_sample_counts, n_substeps, 0, true, mmap_dir, &result);
+        if (rc == okay && write_mat_from_matrix_fp32(out_filename, result) != write_okay) rc = output_error;
+        if (result) destroy_mat_full_fp32(&result);
+    }
+    return rc;
+}
+
+// src/api.cpp:1565-1598
+ComputeStatus unifrac_multi_to_file_v3(const char* biom_filename, const char* tree_filename, const char* out_filename,
+                                       const char* unifrac_method, bool variance_adjust, double alpha,
+                                       bool bypass_tips, bool normalize_sample_counts, unsigned int,
+                                       const char* format, unsigned int n_subsamples, unsigned int subsample_depth,
+                                       bool subsample_with_replacement, unsigned int pcoa_dims,
+                                       unsigned int permanova_perms, const char*, const char*, const char*) {
+    bool fp64 = false, save_dist = true;
+    const ComputeStatus rc = format_precision(unifrac_method, format, fp64, save_dist);
+    if (rc != okay) return rc;
+    if (fp64)
+        return multi_to_file<double>(save_dist, biom_filename, tree_filename, out_filename, unifrac_method,
+                                     variance_adjust, alpha, bypass_tips, normalize_sample_counts, n_subsamples,
+                                     subsample_depth, subsample_with_replacement, pcoa_dims, permanova_perms);
+    return multi_to_file<float>(save_dist, biom_filename, tree_filename, out_filename, unifrac_method, variance_adjust,
+                                alpha, bypass_tips, normalize_sample_counts, n_subsamples, subsample_depth,
+                                subsample_with_replacement, pcoa_dims, permanova_perms);
+}
+
+ComputeStatus unifrac_multi_to_file_v2(const char* biom_filename, const char* tree_filename, const char* out_filename,
+                                       const char* unifrac_method, bool variance_adjust, double alpha,
+                                       bool bypass_tips, unsigned int n_substeps, const char* format,
+                                       unsigned int n_subsamples, unsigned int subsample_depth,
+                                       bool subsample_with_replacement, unsigned int pcoa_dims,
+                                       unsigned int permanova_perms, const char* grouping_filename,
+                                       const char* grouping_columns, const char* mmap_dir) {
+    return unifrac_multi_to_file_v3(biom_filename, tree_filename, out_filename, unifrac_method, variance_adjust,
+                                    alpha, bypass_tips, true, n_substeps, format, n_subsamples, subsample_depth,
+                                    subsample_with_replacement, pcoa_dims, permanova_perms, grouping_filename,
+                                    grouping_columns, mmap_dir);
 }
 
 struct ssu_rarefy {

@@ -42,6 +42,8 @@ struct H5 {
     hid_t (*Tcopy)(hid_t);
     herr_t (*Tset_size)(hid_t, size_t);
     herr_t (*Tclose)(hid_t);
+    int (*Tget_cset)(hid_t);
+    herr_t (*Tset_cset)(hid_t, int);
     herr_t (*Eset_auto2)(hid_t, void*, void*);
     hid_t (*Fcreate)(const char*, unsigned, hid_t, hid_t);
     hid_t (*Screate_simple)(int, const hsize_t*, const hsize_t*);
@@ -53,6 +55,7 @@ struct H5 {
     hid_t* FORTRAN_S1_g;
     hid_t* IEEE_F64LE_g;
     hid_t* IEEE_F32LE_g;
+    hid_t* STD_U32LE_g;
 };
 
 H5& h5() {
@@ -77,12 +80,14 @@ H5& h5() {
         sym(h.Sget_simple_extent_dims, "H5Sget_simple_extent_dims"); sym(h.Sclose, "H5Sclose");
         sym(h.Tis_variable_str, "H5Tis_variable_str"); sym(h.Tget_size, "H5Tget_size");
         sym(h.Tcopy, "H5Tcopy"); sym(h.Tset_size, "H5Tset_size"); sym(h.Tclose, "H5Tclose");
+        sym(h.Tget_cset, "H5Tget_cset"); sym(h.Tset_cset, "H5Tset_cset");
         sym(h.Eset_auto2, "H5Eset_auto2");
         sym(h.Fcreate, "H5Fcreate"); sym(h.Screate_simple, "H5Screate_simple"); sym(h.Dcreate2, "H5Dcreate2");
         sym(h.Dwrite, "H5Dwrite");
         sym(h.NATIVE_DOUBLE_g, "H5T_NATIVE_DOUBLE_g"); sym(h.NATIVE_UINT32_g, "H5T_NATIVE_UINT32_g");
         sym(h.C_S1_g, "H5T_C_S1_g"); sym(h.FORTRAN_S1_g, "H5T_FORTRAN_S1_g");
         sym(h.IEEE_F64LE_g, "H5T_IEEE_F64LE_g"); sym(h.IEEE_F32LE_g, "H5T_IEEE_F32LE_g");
+        sym(h.STD_U32LE_g, "H5T_STD_U32LE_g");
         if (!good || h.open() < 0) { h.why = "libhdf5 lacks the 1.10 C API"; return; }
         h.Eset_auto2(0 /* H5E_DEFAULT */, nullptr, nullptr);  // silence the error stack printer
         h.ok = true;
@@ -121,9 +126,13 @@ bool read_strings(H5& h, hid_t file, const char* path, std::vector<std::string>&
     out.reserve(n);
     if (n == 0) return true;
     Handle ft{h.Dget_type(ds.id), h.Tclose};
+    // the memory type takes the file's character set (ASCII or UTF-8): HDF5
+    // converts between strings of one set only
+    const int cset = h.Tget_cset(ft.id);
     if (h.Tis_variable_str(ft.id) > 0) {
         Handle mt{h.Tcopy(*h.C_S1_g), h.Tclose};
         h.Tset_size(mt.id, T_VARIABLE);
+        if (cset >= 0) h.Tset_cset(mt.id, cset);
         std::vector<char*> buf(n, nullptr);
         if (h.Dread(ds.id, mt.id, S_ALL, S_ALL, P_DEFAULT, buf.data()) < 0) return false;
         for (auto* s : buf) out.emplace_back(s ? s : "");
@@ -133,6 +142,7 @@ bool read_strings(H5& h, hid_t file, const char* path, std::vector<std::string>&
         const size_t w = h.Tget_size(ft.id);
         Handle mt{h.Tcopy(*h.C_S1_g), h.Tclose};
         h.Tset_size(mt.id, w);
+        if (cset >= 0) h.Tset_cset(mt.id, cset);
         std::vector<char> buf(n * w);
         if (h.Dread(ds.id, mt.id, S_ALL, S_ALL, P_DEFAULT, buf.data()) < 0) return false;
         for (hsize_t i = 0; i < n; ++i) out.emplace_back(std::string(&buf[i * w], strnlen(&buf[i * w], w)));
@@ -162,7 +172,53 @@ bool write_strings(H5& h, hid_t file, const char* name, uint32_t n, const char* 
     return ds.id >= 0 && (n == 0 || h.Dwrite(ds.id, t.id, S_ALL, S_ALL, P_DEFAULT, v) >= 0);
 }
 
+// n-dimensional numeric dataset (src/api.cpp:1218-1262)
+bool write_array(H5& h, hid_t file, const char* name, int rank, const hsize_t* dims, hid_t type, const void* v) {
+    Handle sp{h.Screate_simple(rank, dims, nullptr), h.Sclose};
+    Handle ds{h.Dcreate2(file, name, type, sp.id, P_DEFAULT, P_DEFAULT, P_DEFAULT), h.Dclose};
+    uint64_t count = 1;
+    for (int r = 0; r < rank; ++r) count *= dims[r];
+    return sp.id >= 0 && ds.id >= 0 && (count == 0 || h.Dwrite(ds.id, type, S_ALL, S_ALL, P_DEFAULT, v) >= 0);
+}
+
 }  // namespace
+
+std::unique_ptr<H5Writer> H5Writer::create(const char* filename, std::string* err) {
+    H5& h = h5();
+    if (!h.ok) { if (err) *err = h.why; return nullptr; }
+    std::unique_ptr<H5Writer> w(new H5Writer());
+    w->file_ = h.Fcreate(filename, F_ACC_TRUNC, P_DEFAULT, P_DEFAULT);
+    if (w->file_ < 0) { if (err) *err = std::string("cannot create ") + filename; return nullptr; }
+    return w;
+}
+
+H5Writer::~H5Writer() {
+    if (file_ >= 0) h5().Fclose(file_);
+}
+
+bool H5Writer::string(const char* name, const std::string& v) { return write_string(h5(), file_, name, v); }
+
+bool H5Writer::strings(const char* name, uint64_t n, const char* const* v) {
+    return write_strings(h5(), file_, name, (uint32_t)n, v);
+}
+
+bool H5Writer::reals(const char* name, uint64_t n, const void* v, bool fp64) {
+    H5& h = h5();
+    const hsize_t dims[1] = {n};
+    return write_array(h, file_, name, 1, dims, fp64 ? *h.IEEE_F64LE_g : *h.IEEE_F32LE_g, v);
+}
+
+bool H5Writer::matrix(const char* name, uint64_t rows, uint64_t cols, const void* v, bool fp64) {
+    H5& h = h5();
+    const hsize_t dims[2] = {rows, cols};
+    return write_array(h, file_, name, 2, dims, fp64 ? *h.IEEE_F64LE_g : *h.IEEE_F32LE_g, v);
+}
+
+bool H5Writer::u32s(const char* name, uint64_t n, const uint32_t* v) {
+    H5& h = h5();
+    const hsize_t dims[1] = {n};
+    return write_array(h, file_, name, 1, dims, *h.STD_U32LE_g, v);
+}
 
 bool biom_n_samples(const char* filename, uint64_t& n, std::string* err) {
     H5& h = h5();
@@ -178,19 +234,10 @@ bool biom_n_samples(const char* filename, uint64_t& n, std::string* err) {
 
 bool write_matrix_hdf5(const char* filename, uint32_t n, const char* const* ids, const void* matrix, bool fp64,
                        bool save_dist, std::string* err) {
-    H5& h = h5();
-    if (!h.ok) { if (err) *err = h.why; return false; }
-    Handle f{h.Fcreate(filename, F_ACC_TRUNC, P_DEFAULT, P_DEFAULT), h.Fclose};
-    if (f.id < 0) { if (err) *err = std::string("cannot create ") + filename; return false; }
-    bool ok = write_string(h, f.id, "format", "BDSM") && write_string(h, f.id, "version", "2020.12") &&
-              write_strings(h, f.id, "order", n, ids);
-    if (ok && save_dist) {
-        const hsize_t dims[2] = {n, n};
-        const hid_t t = fp64 ? *h.IEEE_F64LE_g : *h.IEEE_F32LE_g;
-        Handle sp{h.Screate_simple(2, dims, nullptr), h.Sclose};
-        Handle ds{h.Dcreate2(f.id, "matrix", t, sp.id, P_DEFAULT, P_DEFAULT, P_DEFAULT), h.Dclose};
-        ok = ds.id >= 0 && ((uint64_t)n * n == 0 || h.Dwrite(ds.id, t, S_ALL, S_ALL, P_DEFAULT, matrix) >= 0);
-    }
+    auto w = H5Writer::create(filename, err);
+    if (!w) return false;
+    bool ok = w->string("format", "BDSM") && w->string("version", "2020.12") && w->strings("order", n, ids);
+    if (ok && save_dist) ok = w->matrix("matrix", n, n, matrix, fp64);
     if (!ok && err) *err = std::string("HDF5 write failed for ") + filename;
     return ok;
 }

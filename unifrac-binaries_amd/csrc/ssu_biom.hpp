@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 
 #include "ssu_table.hpp"
@@ -20,5 +21,24 @@ bool biom_n_samples(const char* filename, uint64_t& n, std::string* err);
 // when save_dist, the n x n "matrix" in IEEE little-endian FP64 or FP32.
 bool write_matrix_hdf5(const char* filename, uint32_t n, const char* const* ids, const void* matrix, bool fp64,
                        bool save_dist, std::string* err);
+
+// An HDF5 output file written dataset by dataset, with the dataset encodings of
+// src/api.cpp:1167-1262: one fixed-length Fortran string, arrays of
+// variable-length C strings, 1-D and 2-D IEEE little-endian reals, 1-D
+// little-endian uint32.  The file closes when the writer is destroyed.
+class H5Writer {
+public:
+    static std::unique_ptr<H5Writer> create(const char* filename, std::string* err);
+    ~H5Writer();
+    bool string(const char* name, const std::string& v);
+    bool strings(const char* name, uint64_t n, const char* const* v);
+    bool reals(const char* name, uint64_t n, const void* v, bool fp64);
+    bool matrix(const char* name, uint64_t rows, uint64_t cols, const void* v, bool fp64);
+    bool u32s(const char* name, uint64_t n, const uint32_t* v);
+
+private:
+    H5Writer() = default;
+    int64_t file_ = -1;
+};
 
 }  // namespace ssu

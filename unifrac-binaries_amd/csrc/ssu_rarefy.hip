@@ -208,7 +208,7 @@ std::unique_ptr<RarefyBatch> RarefyBatch::create(const Table& table, const Phylo
     std::string e2;
     if (!err) err = &e2;
     if (!device_available(err)) return nullptr;
-    if (depth == 0 || reps == 0 || specs.empty()) { *err = "depth, replicates and methods must be non-zero"; return nullptr; }
+    if (depth == 0 || reps == 0) { *err = "depth and replicates must be non-zero"; return nullptr; }
     std::unique_ptr<RarefyBatch> b(new RarefyBatch());
     RarefyImpl& R = *b->p_;
     R.specs = specs;
@@ -255,10 +255,14 @@ std::unique_ptr<RarefyBatch> RarefyBatch::create(const Table& table, const Phylo
         R.hsize = 1;
         while (R.hsize < 2 * depth) R.hsize <<= 1;
     }
-    // one plan per method over the kept table, all on the batch's stream
-    std::unordered_set<std::string> keep(t.obs_ids.begin(), t.obs_ids.end());
-    PhyloTree sheared = tree.shear(keep).collapse();
-    PostorderNodes nodes = postorder_nodes(sheared, t.obs_index);
+    // one plan per method over the kept table, all on the batch's stream (no
+    // methods: the batch only draws the tables)
+    PostorderNodes nodes;
+    if (!specs.empty()) {
+        std::unordered_set<std::string> keep(t.obs_ids.begin(), t.obs_ids.end());
+        PhyloTree sheared = tree.shear(keep).collapse();
+        nodes = postorder_nodes(sheared, t.obs_index);
+    }
     if (hipStreamCreateWithFlags(&R.st, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&R.e0) != hipSuccess || hipEventCreate(&R.e1) != hipSuccess) {
         *err = "stream/event creation failed";

@@ -21,6 +21,7 @@ class RarefyBatch {
 public:
     ~RarefyBatch();
     // Samples whose total is below depth are dropped first (src/biom_subsampled.cpp:290).
+    // With no specs the batch only draws the tables (copy_counts).
     static std::unique_ptr<RarefyBatch> create(const Table& table, const PhyloTree& tree,
                                                const std::vector<ComputeSpec>& specs, uint32_t depth,
                                                bool with_replacement, uint64_t seed, uint32_t reps,

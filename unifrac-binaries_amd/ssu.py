@@ -79,7 +79,9 @@ EXPORTS = [
     "write_mat_from_matrix", "write_mat_from_matrix_fp32", "write_vec", "write_partial", "read_partial",
     "read_partial_header", "read_partial_one_stripe", "validate_partial", "merge_partial_to_mmap_matrix",
     "merge_partial_to_mmap_matrix_fp32", "write_mat_from_matrix_hdf5_fp64", "write_mat_from_matrix_hdf5_fp32",
-    "ssu_faith_pd_inmem",
+    "ssu_faith_pd_inmem", "write_mat_from_matrix_hdf5_fp64_v2", "write_mat_from_matrix_hdf5_fp32_v2",
+    "unifrac_to_file_v3", "unifrac_to_txt_file_v3", "unifrac_to_file_v2", "unifrac_to_file",
+    "unifrac_multi_to_file_v3", "unifrac_multi_to_file_v2",
     "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
     "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
     "ssu_newick_to_bp", "ssu_biom_n_samples", "ssu_device_available", "ssu_last_error",
@@ -114,6 +116,10 @@ def lib():
         L.read_bptree_opaque.restype = C.c_int
         for n in ("write_partial", "read_partial", "read_partial_header", "read_partial_one_stripe",
                   "validate_partial", "merge_partial_to_mmap_matrix", "merge_partial_to_mmap_matrix_fp32"):
+            getattr(L, n).restype = C.c_int
+        for n in ("unifrac_to_file_v3", "unifrac_to_txt_file_v3", "unifrac_to_file_v2", "unifrac_to_file",
+                  "unifrac_multi_to_file_v3", "unifrac_multi_to_file_v2", "write_mat_from_matrix_hdf5_fp64_v2",
+                  "write_mat_from_matrix_hdf5_fp32_v2"):
             getattr(L, n).restype = C.c_int
         L.write_mat_from_matrix_hdf5_fp64.restype = C.c_int
         L.write_mat_from_matrix_hdf5_fp32.restype = C.c_int
