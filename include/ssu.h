@@ -384,12 +384,16 @@ SSU_EXTERN ComputeStatus ssu_plan_execute(ssu_plan_t* plan);
 /* element size of the result: 8 (fp64 method names) or 4 */
 SSU_EXTERN int ssu_plan_result_bytes(const ssu_plan_t* plan);
 SSU_EXTERN ComputeStatus ssu_plan_copy_matrix(const ssu_plan_t* plan, void* host_out);
-/* last execute(): milliseconds per phase (embed, prep, main kernel, combine, total) from HIP events */
+/* last execute(): milliseconds per phase (embed, prep, main kernel, combine, total, posting-list share of
+ * the FP32 weighted path) from HIP events */
 SSU_EXTERN int ssu_plan_timings(const ssu_plan_t* plan, double* ms, int cap);
 SSU_EXTERN uint64_t ssu_plan_pair_nodes(const ssu_plan_t* plan);   /* n(n-1)/2 * processed nodes */
 /* (pair, node) terms the main kernel of the last execute() evaluated, padding
  * included; the shared-node weighted kernel counts only the visited nodes */
 SSU_EXTERN uint64_t ssu_plan_kernel_terms(const ssu_plan_t* plan);
+/* (pair, node) terms of the last execute() summed from posting lists (FP32
+ * weighted path: nodes held by few samples), not counted in kernel_terms */
+SSU_EXTERN uint64_t ssu_plan_sparse_terms(const ssu_plan_t* plan);
 SSU_EXTERN const char* ssu_plan_main_kernel(const ssu_plan_t* plan);
 SSU_EXTERN void ssu_plan_destroy(ssu_plan_t** plan);
 

@@ -251,6 +251,63 @@ def test_weighted_shared_node_kernel_matches_dense(ssu, O, synth, method, prec):
     pd.close()
 
 
+@pytest.mark.parametrize("theta", ["0", "0.05", "0.3", "1.0"])
+def test_weighted_occupancy_split(ssu, O, synth, monkeypatch, theta):
+    """FP32 shared-node path with nodes held by fewer than theta * n samples
+    summed from posting lists (fixed-point row sums) and added in the tiled
+    kernel's epilogue; theta 0 turns the split off, 1.0 sends almost every node
+    to the posting lists.  Repeated executions are bitwise identical."""
+    tree, table = _case(synth, 500, 300, 0.05, 73, zipf=1.1)
+    monkeypatch.setenv("SSU_SPARSE_SPLIT", theta)
+    inp = inputs_from(ssu, tree, table)
+    for method in ("weighted_normalized", "weighted_unnormalized"):
+        say("occupancy split", theta, method)
+        p = ssu.Plan(inp, method)
+        p.execute()
+        m = p.matrix()
+        assert rel_err(m, oracle_matrix(O, tree, table, method)) <= TOL["fp32"]
+        assert np.array_equal(m, m.T) and np.all(np.diag(m) == 0)
+        p.execute()
+        assert np.array_equal(p.matrix(), m)
+        p.close()
+
+
+def _sub_table(table, idx):
+    obs_ids, sample_ids, indptr, indices, data = table
+    remap = -np.ones(len(sample_ids), np.int64)
+    remap[idx] = np.arange(len(idx))
+    rows = np.repeat(np.arange(len(obs_ids)), np.diff(np.asarray(indptr, np.int64)))
+    keep = remap[np.asarray(indices)] >= 0
+    r, c, v = rows[keep], remap[np.asarray(indices)[keep]], np.asarray(data)[keep]
+    order = np.lexsort((c, r))
+    r, c, v = r[order], c[order], v[order]
+    used, counts = np.unique(r, return_counts=True)
+    ip = np.zeros(used.size + 1, np.uint32)
+    np.cumsum(counts, out=ip[1:])
+    return [obs_ids[i] for i in used], [sample_ids[i] for i in idx], ip, c.astype(np.uint32), v
+
+
+def test_weighted_split_across_row_chunks(ssu, O, synth, monkeypatch):
+    """4,500 samples: rows span two 4,096-column chunks of the posting-list
+    kernel.  Split against no split on the device, and a 48-sample subset
+    against the FP64 oracle."""
+    tree, table = _case(synth, 300, 4500, 0.04, 79)
+    inp = inputs_from(ssu, tree, table)
+    out = {}
+    for theta in ("0", "0.2"):
+        monkeypatch.setenv("SSU_SPARSE_SPLIT", theta)
+        say("row chunks", theta)
+        p = ssu.Plan(inp, "weighted_normalized")
+        p.execute()
+        out[theta] = p.matrix()
+        p.close()
+    assert rel_err(out["0.2"], out["0"]) <= 2e-6
+    idx = np.sort(np.random.default_rng(5).choice(4500, 48, replace=False))
+    idx[-1] = 4499
+    ref = oracle_matrix(O, tree, _sub_table(table, idx), "weighted_normalized")
+    assert rel_err(out["0.2"][np.ix_(idx, idx)], ref) <= TOL["fp32"]
+
+
 def test_weighted_duplicate_samples(ssu, O, synth):
     """Identical samples: d = 0 exactly in FP64 arithmetic; the shared-node
     kernel gets S_i + S_j - 2M, clamped at 0, within the absolute bound."""

@@ -1237,6 +1237,8 @@ int ssu_plan_timings(const ssu_plan_t* plan, double* ms, int cap) {
     return k;
 }
 
+uint64_t ssu_plan_sparse_terms(const ssu_plan_t* plan) { return plan ? plan->plan->sparse_terms() : 0; }
+
 uint64_t ssu_plan_pair_nodes(const ssu_plan_t* plan) {
     if (!plan) return 0;
     const uint64_t n = plan->plan->n_samples();

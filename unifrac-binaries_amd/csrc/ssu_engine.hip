@@ -83,6 +83,7 @@ struct PlanImpl {
     size_t blk = 0;                  // sample columns per embedding block (ldn: one block)
     bool gram = false;
     bool sparse = false;             // weighted through the shared-node kernel
+    uint32_t split_cap = 0;          // FP32 shared-node path: nodes held by fewer samples use posting lists (0: off)
     std::vector<uint32_t> block_order;  // pair blocks / Gram tiles in launch order
     bool bad_lengths = false;
 
@@ -108,7 +109,7 @@ struct PlanImpl {
 
     DevMem d_indptr, d_indices, d_data, d_totals, d_leaf_k, d_leaf_row, d_level_nodes, d_child_ptr, d_child_idx;
     DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_order, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
-        d_faith, d_stripes;
+        d_faith, d_stripes, d_smask, d_dmask, d_pc, d_pidx, d_pval, d_walked, d_rows, d_rcount, d_roff, d_rbsum;
 
     hipStream_t own_st = nullptr;    // created with the plan; st may point at a caller's stream
     // per-execution inputs and output (nullptr: the plan's own table data / totals / result)
@@ -291,6 +292,20 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         for (uint64_t q = 0; q < nz && P.sparse; ++q)
             if (!(table.data[b0 + q] >= 0.0)) P.sparse = false;
     }
+    // Occupancy split (FP32 shared-node path, non-negative lengths): nodes held
+    // by fewer than theta * n samples are summed from posting lists instead of
+    // the 32 x 64 tiles.  Postings and row lists take K x cap entries each
+    // (16 bytes per slot together), kept within SPLIT_BUDGET by lowering theta.  SSU_SPARSE_SPLIT sets theta (0: off).
+    if (P.sparse && !spec.fp64 && P.n >= 2) {
+        double theta = 0.2;
+        if (const char* env = getenv("SSU_SPARSE_SPLIT")) theta = atof(env);
+        bool nonneg = true;
+        for (float l : P.klen_f) nonneg = nonneg && l >= 0.0f;
+        constexpr double SPLIT_BUDGET = 16.0 * (1u << 30);
+        const double K = (double)std::max<size_t>(P.knodes.size(), 1);
+        theta = std::min(theta, SPLIT_BUDGET / (K * P.n * 16.0));
+        if (nonneg && theta > 0.0) P.split_cap = (uint32_t)std::max(1.0, std::ceil(theta * P.n));
+    }
 
     // Embedding blocks: the FP64 embedding of every node for every sample
     // (m x ldn doubles) is built in sample-column blocks when it would exceed
@@ -393,6 +408,22 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         if (!P.d_Spart.ensure(parts * sizeof(double), err)) return false;
     }
     if (P.sparse && !P.d_masks.ensure(std::max<size_t>((P.ldn / 32) * Kw, 1) * sizeof(uint32_t), err)) return false;
+    const bool split = P.sparse && P.split_cap > 0 && K > 0;
+    const uint32_t npan = (P.n + 31) / 32;
+    if (split) {
+        const size_t kk = std::max<size_t>(K, 1);
+        if (!P.d_smask.ensure((P.ldn / 32) * kk * sizeof(uint32_t), err) ||
+            !P.d_dmask.ensure(std::max<size_t>(Kw, 1) * sizeof(uint32_t), err) ||
+            !P.d_pc.ensure((size_t)(npan + 1) * kk * sizeof(uint32_t), err) ||
+            !P.d_pidx.ensure(kk * P.split_cap * sizeof(uint32_t), err) ||
+            !P.d_pval.ensure(kk * P.split_cap * sizeof(float), err) ||
+            !P.d_walked.ensure((size_t)sparse_rows_blocks(P.n) * sizeof(uint32_t), err) ||
+            !P.d_rows.ensure(kk * P.split_cap * sizeof(unsigned long long), err) ||
+            !P.d_rcount.ensure((size_t)P.n * row_list_chunks(K) * sizeof(uint32_t), err) ||
+            !P.d_roff.ensure(((size_t)P.n * row_list_chunks(K) + 1) * sizeof(uint32_t), err) ||
+            !P.d_rbsum.ensure((row_list_scan_blocks(P.n, K) + 1) * sizeof(uint32_t), err))
+            return false;
+    }
     if (gather) {
         if (!P.d_U.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
         if (s.vaw && !P.d_Cu.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
@@ -412,14 +443,39 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         } else if (gather || sums) {
             SSU_HIP(launch_gather_sums(gather ? P.d_U.as<float>() : nullptr, P.ldn, sums ? P.d_S.as<double>() : nullptr,
                                        P.d_Spart.as<double>(), P.sparse ? P.d_masks.as<uint32_t>() : nullptr,
-                                       P.d_E.as<double>(), w, (uint32_t)c0, P.d_knodes.as<int32_t>(),
+                                       split ? P.d_smask.as<uint32_t>() : nullptr, P.d_E.as<double>(), w, (uint32_t)c0, P.d_knodes.as<int32_t>(),
                                        P.d_klen_sum.as<double>(), K, P.n, P.st));
             if (gather && s.vaw)
-                SSU_HIP(launch_gather_sums(P.d_Cu.as<float>(), P.ldn, nullptr, nullptr, nullptr, P.d_C.as<double>(), w,
+                SSU_HIP(launch_gather_sums(P.d_Cu.as<float>(), P.ldn, nullptr, nullptr, nullptr, nullptr, P.d_C.as<double>(), w,
                                            (uint32_t)c0, P.d_knodes.as<int32_t>(), nullptr, K, P.n, P.st));
         }
     }
     if (!single) SSU_HIP(P.mark(1));   // blocked: embed and prep share one phase
+    if (split) {   // posting lists, row lists and each row's sparse-node share, into the output
+        SSU_HIP(P.mark(PH_COUNT));
+        SparseNodesArgs b{};
+        b.dmask = P.d_dmask.as<uint32_t>();
+        b.smask = P.d_smask.as<uint32_t>();
+        b.pc = P.d_pc.as<uint32_t>();
+        b.pidx = P.d_pidx.as<uint32_t>();
+        b.pval = P.d_pval.as<float>();
+        b.U = P.d_U.as<float>();
+        b.ldu = P.ldn;
+        b.len = P.d_klen_sum.as<double>();
+        b.S = P.d_S.as<double>();
+        b.out = P.out_ptr();
+        b.out_fp64 = false;
+        b.walked = P.d_walked.as<uint32_t>();
+        b.rows = P.d_rows.as<unsigned long long>();
+        b.rcount = P.d_rcount.as<uint32_t>();
+        b.roff = P.d_roff.as<uint32_t>();
+        b.rbsum = P.d_rbsum.as<uint32_t>();
+        b.K = K;
+        b.n = P.n;
+        b.npan = npan;
+        b.cap = P.split_cap;
+        SSU_HIP(launch_sparse_nodes(b, P.st));
+    }
     SSU_HIP(P.mark(2));
 
     if (P.gram) {
@@ -458,6 +514,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         a.visits = P.d_visits.as<uint32_t>();
         a.order = P.d_order.as<uint32_t>();
         a.n_blocks = (uint32_t)P.block_order.size();
+        if (split) a.dmask = P.d_dmask.as<uint32_t>();   // dense nodes only; adds the posting-list share
         if (s.fp64) SSU_HIP(launch_pairs_sparse(true, a, P.st));
         else SSU_HIP(launch_pairs_sparse32(a, P.st));
         SSU_HIP(P.mark(3));
@@ -492,6 +549,11 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
     SSU_HIP(hipStreamSynchronize(P.st));
     for (int i = 0; i < PH_TOTAL; ++i) SSU_HIP(hipEventElapsedTime(&P.times[i], P.ev[i], P.ev[i + 1]));
     SSU_HIP(hipEventElapsedTime(&P.times[PH_TOTAL], P.ev[0], P.ev[5]));
+    P.times[PH_SPARSE] = 0.0f;
+    if (split) {   // prep ends where the posting-list phase starts
+        SSU_HIP(hipEventElapsedTime(&P.times[PH_PREP], P.ev[1], P.ev[PH_COUNT]));
+        SSU_HIP(hipEventElapsedTime(&P.times[PH_SPARSE], P.ev[PH_COUNT], P.ev[2]));
+    }
     return true;
 }
 
@@ -565,6 +627,15 @@ uint64_t Plan::kernel_terms() const {
     const uint64_t edge = small ? 64 : PAIR_TILE;
     const uint64_t t = (P.n + edge - 1) / edge;
     return t * (t + 1) / 2 * edge * edge * K;
+}
+uint64_t Plan::sparse_terms() const {
+    const PlanImpl& P = *p_;
+    if (!P.sparse || !P.split_cap || !P.d_walked.p) return 0;
+    std::vector<uint32_t> wk(sparse_rows_blocks(P.n));
+    if (hipMemcpy(wk.data(), P.d_walked.p, wk.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    uint64_t sum = 0;
+    for (uint32_t x : wk) sum += x;
+    return sum;
 }
 bool Plan::out_fp64() const { return p_->spec.fp64; }
 const char* Plan::main_kernel() const {

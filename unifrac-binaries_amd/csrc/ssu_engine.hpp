@@ -31,7 +31,9 @@ struct ComputeSpec {
 };
 
 // Phases timed with HIP events on the plan's stream during execute().
-enum Phase : int { PH_EMBED = 0, PH_PREP, PH_MAIN, PH_COMBINE, PH_TOTAL, PH_COUNT };
+// Phases of one execution; PH_SPARSE is the posting-list share of the FP32
+// weighted path (zero elsewhere), PH_MAIN the main kernel alone.
+enum Phase : int { PH_EMBED = 0, PH_PREP, PH_MAIN, PH_COMBINE, PH_TOTAL, PH_SPARSE, PH_COUNT };
 
 struct PlanImpl;
 
@@ -56,6 +58,7 @@ public:
     // Faith PD over all non-root nodes (needs only the embedding).
     bool faith(double* host, std::string* err);
     const float* timings() const;  // ms per Phase, last execute()
+    uint64_t sparse_terms() const;   // (pair, node) terms walked from posting lists in the last execute()
     uint32_t n_samples() const;
     uint64_t processed_nodes() const;  // nodes entering the pair accumulation
     // (pair, node) terms the last execute's main kernel evaluated, padding and

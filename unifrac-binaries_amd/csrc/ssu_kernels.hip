@@ -77,12 +77,14 @@ template <bool GATHER, bool SUMS, bool MASKS>
 __global__ void __launch_bounds__(EMB_THREADS) k_gather_sums(float* __restrict__ U, size_t ldu, uint32_t col0,
                                                              double* __restrict__ Spart,
                                                              uint32_t* __restrict__ masks, uint32_t Kw,
+                                                             uint32_t* __restrict__ smask,
                                                              const double* __restrict__ E, size_t lde,
                                                              const int32_t* __restrict__ knodes,
                                                              const double* __restrict__ klen, uint32_t K,
-                                                             uint32_t rows_per_seg) {
+                                                             uint32_t rows_per_seg, uint32_t n_valid) {
     const size_t s = (size_t)blockIdx.x * EMB_THREADS + threadIdx.x;
     if (s >= lde) return;
+    const bool real = s < n_valid;   // columns past the last sample never enter a mask
     const uint32_t k0 = blockIdx.y * rows_per_seg;
     const uint32_t k1 = min(K, k0 + rows_per_seg);
     const size_t panel = (col0 + s) / 64 * 2;   // the wave's lower 32-sample panel
@@ -98,9 +100,13 @@ __global__ void __launch_bounds__(EMB_THREADS) k_gather_sums(float* __restrict__
         }
         if constexpr (SUMS) acc += klen[r] * e;
         if constexpr (MASKS) {
-            const uint64_t b = __ballot(e > 0.0);
+            const uint64_t b = __ballot(real && e > 0.0);
             lo |= ((uint32_t)b != 0u ? 1u : 0u) << (r & 31);
             hi |= ((uint32_t)(b >> 32) != 0u ? 1u : 0u) << (r & 31);
+            if (smask && (threadIdx.x & 63) == 0) {   // the samples themselves, [panel][K]
+                smask[panel * K + r] = (uint32_t)b;
+                smask[(panel + 1) * K + r] = (uint32_t)(b >> 32);
+            }
             if ((r & 31) == 31 || r + 1 == k1) {
                 if ((threadIdx.x & 63) == 0) {
                     masks[panel * Kw + (r >> 5)] = lo;
@@ -808,7 +814,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                                                        const double* __restrict__ S, int normalized,
                                                        TOut* __restrict__ out, uint32_t* __restrict__ visits,
                                                        const uint32_t* __restrict__ order, uint32_t n_blocks,
-                                                       uint32_t per_xcd) {
+                                                       uint32_t per_xcd, const uint32_t* __restrict__ dmask,
+                                                       int add_partial) {
     constexpr uint32_t EB = S3_EB;
     const uint32_t seq = (blockIdx.x % N_XCDS) * per_xcd + blockIdx.x / N_XCDS;
     if (seq >= n_blocks) return;
@@ -836,7 +843,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 
     for (uint32_t w0 = 0; w0 < Kw; w0 += SP_WORDS) {
         const uint32_t w = w0 + lane;
-        const uint32_t word = w < Kw ? (mI[w] & (mJ[w] | mJ[Kw + w])) : 0u;
+        uint32_t word = w < Kw ? (mI[w] & (mJ[w] | mJ[Kw + w])) : 0u;
+        if (dmask && w < Kw) word &= dmask[w];   // the other nodes come from the posting-list kernel
         const uint32_t total = build_list(list, word, lane);
         visited += total;
         const uint32_t base = w0 * 32;
@@ -914,7 +922,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             const size_t i = i0 + r, j = j0 + c;
             if (j >= n || i > j) continue;
             if (i == j) { out[i * n + i] = (TOut)0; continue; }
-            const double m = acc64[r][c] + (double)acc[r][c];
+            double m = acc64[r][c] + (double)acc[r][c];
+            if (add_partial) m += (double)out[i * n + j];   // the sparse nodes' share, from k_pairs_rows
             const double tot = S[i] + S[j];
             double x = tot - 2.0 * m;
             x = x < 0.0 ? 0.0 : x;   // rounding can undershoot 0; NaN (empty samples) passes
@@ -922,6 +931,286 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             out[i * n + j] = (TOut)d;
             out[j * n + i] = (TOut)d;
         }
+}
+
+// --------------------------------------------- sparse nodes: posting lists
+
+// Node occupancy split: a node held by at least `thr` samples is dense (its
+// bit set in dmask, it stays with the tiled shared-node kernel); the rest go
+// to the posting-list kernels.  One thread per node; smask is [npan][K].
+__global__ void __launch_bounds__(64) k_node_split(uint32_t* __restrict__ dmask, const uint32_t* __restrict__ smask,
+                                                   uint32_t K, uint32_t npan, uint32_t thr) {
+    const uint32_t r = blockIdx.x * 64 + threadIdx.x;
+    uint32_t cnt = 0;
+    if (r < K)
+        for (uint32_t p = 0; p < npan; ++p) cnt += __popc(smask[(size_t)p * K + r]);
+    const uint64_t b = __ballot(r < K && cnt >= thr);
+    const uint32_t w = blockIdx.x * 2 + (threadIdx.x >> 5);
+    if ((threadIdx.x & 31) == 0 && w * 32 < K) dmask[w] = (threadIdx.x < 32) ? (uint32_t)b : (uint32_t)(b >> 32);
+}
+
+// Posting lists of the sparse nodes: node r's samples in increasing order
+// (pidx) with their FP32 values (pval), at r * cap; pc[p * K + r] = number of
+// node r's samples below 32 p, for p = 0..npan.  One thread per node.
+__global__ void __launch_bounds__(64) k_postings(uint32_t* __restrict__ pidx, float* __restrict__ pval,
+                                                 uint32_t* __restrict__ pc, const uint32_t* __restrict__ smask,
+                                                 const uint32_t* __restrict__ dmask, const float* __restrict__ U,
+                                                 size_t ldu, uint32_t K, uint32_t npan, uint32_t cap) {
+    const uint32_t r = blockIdx.x * 64 + threadIdx.x;
+    if (r >= K || ((dmask[r >> 5] >> (r & 31)) & 1u)) return;
+    const float* ur = U + (size_t)r * ldu;
+    uint32_t* ix = pidx + (size_t)r * cap;
+    float* vx = pval + (size_t)r * cap;
+    uint32_t cnt = 0;
+    for (uint32_t p = 0; p < npan; ++p) {
+        pc[(size_t)p * K + r] = cnt;
+        uint32_t w = smask[(size_t)p * K + r];
+        while (w && cnt < cap) {   // cnt < cap holds for every sparse node (fewer than thr = cap samples)
+            const uint32_t s = p * 32 + (uint32_t)(__ffs(w) - 1);
+            ix[cnt] = s;
+            vx[cnt] = ur[s];
+            ++cnt;
+            w &= w - 1;
+        }
+    }
+    pc[(size_t)npan * K + r] = cnt;
+}
+
+// Fixed point for the LDS accumulators: 0 <= x < 2^62, truncated.  Integer
+// adds are associative, so the row sums do not depend on the order in which
+// lanes add their terms.
+__device__ __forceinline__ unsigned long long to_fixed(double x) {
+    const uint32_t hi = (uint32_t)(x * 0x1.0p-32);
+    const uint32_t lo = (uint32_t)(x - (double)hi * 0x1.0p32);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Row lists of the sparse nodes: for sample a, the sparse nodes r that a holds,
+// in increasing r, each as r << 32 | (a's index in r's posting list).  One
+// wave per (32-sample panel P, chunk of RL_KC nodes) transposes the panel's
+// bits with ballots, so the order is deterministic.  With rows == nullptr the
+// wave writes its per-sample counts to counts[a * nkc + chunk]; otherwise it
+// writes its entries from offsets[a * nkc + chunk] (the exclusive scan of the
+// counts in that order).
+constexpr uint32_t RL_KC = 1024;
+__global__ void __launch_bounds__(64) k_row_lists(unsigned long long* __restrict__ rows, uint32_t* __restrict__ counts,
+                                                  const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ smask,
+                                                  const uint32_t* __restrict__ dmask, const uint32_t* __restrict__ pc,
+                                                  uint32_t K, uint32_t n, uint32_t nkc) {
+    const uint32_t P = blockIdx.x, kc = blockIdx.y, lane = threadIdx.x;
+    const uint32_t a_lane = P * 32 + (lane & 31);
+    uint32_t len = 0;   // lane s < 32: entries so far of sample 32 P + s in this chunk
+    unsigned long long* dst =
+        (rows && lane < 32 && a_lane < n) ? rows + offsets[(size_t)a_lane * nkc + kc] : nullptr;
+    const uint32_t k_end = min(K, (kc + 1) * RL_KC);
+    for (uint32_t k0 = kc * RL_KC; k0 < k_end; k0 += 64) {
+        const uint32_t r = k0 + lane;
+        uint32_t w = 0, base = 0;
+        if (r < k_end && !((dmask[r >> 5] >> (r & 31)) & 1u)) {
+            w = smask[(size_t)P * K + r];
+            if (rows && w) base = pc[(size_t)P * K + r];
+        }
+        if (__ballot(w != 0u) == 0ull) continue;   // uniform
+#pragma unroll 4
+        for (uint32_t s = 0; s < 32; ++s) {
+            const bool has = (w >> s) & 1u;
+            const uint64_t m = __ballot(has);
+            if (rows) {
+                // sample 32 P + s's list and its length so far live in lane s (read with every lane active)
+                const uint32_t len_s = __shfl(len, (int)s, 64);
+                unsigned long long* d = (unsigned long long*)__shfl((long long)dst, (int)s, 64);
+                if (has) {
+                    const uint32_t at = len_s + __popcll(m & ((1ull << lane) - 1ull));
+                    d[at] = ((unsigned long long)r << 32) | (base + __popc(w & ((1u << s) - 1u)));
+                }
+            }
+            if (lane == s) len += (uint32_t)__popcll(m);
+        }
+    }
+    if (!rows && lane < 32 && a_lane < n) counts[(size_t)a_lane * nkc + kc] = len;
+}
+
+// Exclusive prefix sum over m elements in three launches: per-block sums,
+// a one-block scan of those, then each block's elements.  offsets[m] = total.
+constexpr uint32_t SCAN_T = 256, SCAN_PER = 16, SCAN_BLOCK = SCAN_T * SCAN_PER;
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* part, uint32_t& total) {
+    const uint32_t t = threadIdx.x;
+    part[t] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < SCAN_T; d <<= 1) {
+        const uint32_t y = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += y;
+        __syncthreads();
+    }
+    total = part[SCAN_T - 1];
+    const uint32_t r = part[t] - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_sums(uint32_t* __restrict__ bsum, const uint32_t* __restrict__ in,
+                                                      size_t m) {
+    __shared__ uint32_t part[SCAN_T];
+    const size_t base = (size_t)blockIdx.x * SCAN_BLOCK;
+    uint32_t v = 0;
+    for (uint32_t i = 0; i < SCAN_PER; ++i) {
+        const size_t q = base + (size_t)i * SCAN_T + threadIdx.x;
+        if (q < m) v += in[q];
+    }
+    uint32_t total;
+    (void)block_exclusive(v, part, total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_top(uint32_t* __restrict__ bsum, uint32_t nb) {
+    __shared__ uint32_t part[SCAN_T];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += SCAN_T) {
+        const uint32_t q = b0 + threadIdx.x;
+        const uint32_t v = q < nb ? bsum[q] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive(v, part, total);
+        if (q < nb) bsum[q] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_apply(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
+                                                       const uint32_t* __restrict__ bsum, size_t m, uint32_t nb) {
+    __shared__ uint32_t part[SCAN_T];
+    const size_t base = (size_t)blockIdx.x * SCAN_BLOCK;
+    // each thread owns SCAN_PER consecutive elements
+    const size_t q0 = base + (size_t)threadIdx.x * SCAN_PER;
+    uint32_t v[SCAN_PER], sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < SCAN_PER; ++i) {
+        v[i] = q0 + i < m ? in[q0 + i] : 0u;
+        sum += v[i];
+    }
+    uint32_t total;
+    uint32_t run = bsum[blockIdx.x] + block_exclusive(sum, part, total);
+#pragma unroll
+    for (uint32_t i = 0; i < SCAN_PER; ++i)
+        if (q0 + i < m) { out[q0 + i] = run; run += v[i]; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[m] = bsum[nb];
+}
+
+constexpr uint32_t SR_CW = 4096;   // output columns per workgroup (64-bit LDS accumulators, 32 KB)
+constexpr uint32_t SR_WAVES = 8;   // waves sharing one accumulator, each taking every 8th round of entries
+
+// Sparse-node share of M for row a: sum over sparse nodes r held by both a and
+// b > a of l_r * min(u_ra, u_rb), written as FP32 into out[a * n + b] for the
+// tiled kernel's epilogue.  One workgroup of 8 waves per (row a, 4096-column
+// chunk c).  Row a's list is taken 64 entries at a time (round q goes to wave
+// q % 8); a round's posting ranges inside the chunk (past a) are walked with
+// the wave's 64 lanes spread evenly over the concatenated postings.  Terms are
+// FP32 (as in the tiled kernel), accumulated in 2^-(61-e) fixed point where
+// S[a] < 2^e bounds the row's sum; integer adds make the result independent of
+// the order in which waves and lanes add.
+template <class TOut>
+__global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__ out, uint32_t* __restrict__ walked,
+                                                   const unsigned long long* __restrict__ rows,
+                                                   const uint32_t* __restrict__ offsets,
+                                                   const uint32_t* __restrict__ pc,
+                                                   const uint32_t* __restrict__ pidx, const float* __restrict__ pval,
+                                                   const double* __restrict__ len, const double* __restrict__ S,
+                                                   uint32_t K, uint32_t n, uint32_t npan, uint32_t cap,
+                                                   uint32_t nchunks, uint32_t nkc) {
+    const uint32_t a = blockIdx.x / nchunks, c = blockIdx.x % nchunks;
+    const uint32_t b0 = c * SR_CW, b1 = min(n, b0 + SR_CW);
+    if (a >= n || b1 <= a + 1) {   // uniform: no column above the diagonal here
+        if (walked && threadIdx.x == 0) walked[blockIdx.x] = 0;
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ unsigned long long acc[SR_CW];
+    __shared__ uint32_t pre_s[SR_WAVES][65], lst_s[SR_WAVES][64], lnode_s[SR_WAVES][64], wk[SR_WAVES];
+    __shared__ float lu_s[SR_WAVES][64], ll_s[SR_WAVES][64];
+    uint32_t* pre = pre_s[wv];
+    uint32_t* lst = lst_s[wv];
+    uint32_t* lnode = lnode_s[wv];
+    float* lu = lu_s[wv];
+    float* ll = ll_s[wv];
+    for (uint32_t q = threadIdx.x; q < SR_CW; q += 64 * SR_WAVES) acc[q] = 0ull;
+    __syncthreads();
+    int e2 = 0;
+    (void)frexp(S[a], &e2);   // S[a] < 2^e2 bounds every partial sum of the row
+    const double scale = ldexp(1.0, 61 - e2), inv = ldexp(1.0, e2 - 61);
+    const uint32_t pb0 = b0 >> 5, pb1 = b1 == n ? npan : (b1 >> 5);
+    const uint32_t e0 = offsets[(size_t)a * nkc], e1 = offsets[(size_t)(a + 1) * nkc];
+    uint32_t total_walked = 0;
+    for (uint32_t r0 = e0 + 64 * wv; r0 < e1; r0 += 64 * SR_WAVES) {   // uniform per wave
+        uint32_t start = 0, end = 0, r = 0;
+        float ua = 0.0f, l = 0.0f;
+        if (r0 + lane < e1) {
+            const unsigned long long ent = rows[r0 + lane];
+            r = (uint32_t)(ent >> 32);
+            const uint32_t pa = (uint32_t)ent;
+            const uint32_t cs = pc[(size_t)pb0 * K + r], ce = pc[(size_t)pb1 * K + r];
+            start = max(pa + 1, cs);
+            end = ce;
+            ua = pval[(size_t)r * cap + pa];
+            l = (float)len[r];
+        }
+        const uint32_t cnt = end > start ? end - start : 0u;
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d, 64);
+            if ((int)lane >= d) incl += y;
+        }
+        const uint32_t T = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+        if (T == 0) continue;   // uniform per wave
+        pre[lane] = incl - cnt;
+        if (lane == 63) pre[64] = incl;
+        lst[lane] = start;
+        lnode[lane] = r;
+        lu[lane] = ua;
+        ll[lane] = l;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // four postings per lane in flight: positions first, then every load, then the adds
+        uint32_t e = 0;
+        for (uint32_t t0 = lane; t0 < T; t0 += 256) {
+            size_t qq[4];
+            uint32_t eq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t t = min(t0 + 64 * u, T - 1);
+                while (pre[e + 1] <= t) ++e;
+                eq[u] = e;
+                qq[u] = (size_t)lnode[e] * cap + lst[e] + (t - pre[e]);
+            }
+            uint32_t bq[4];
+            float vq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                bq[u] = pidx[qq[u]];
+                vq[u] = pval[qq[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (t0 + 64 * u < T)
+                    atomicAdd(&acc[bq[u] - b0], to_fixed((double)(ll[eq[u]] * fminf(lu[eq[u]], vq[u])) * scale));
+        }
+        total_walked += T;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (lane == 0) wk[wv] = total_walked;
+    __syncthreads();
+    if (walked && threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (uint32_t w = 0; w < SR_WAVES; ++w) sum += wk[w];
+        walked[blockIdx.x] = sum;
+    }
+    for (uint32_t b = max(a + 1, b0) + threadIdx.x; b < b1; b += 64 * SR_WAVES)
+        out[(size_t)a * n + b] = (TOut)((double)acc[b - b0] * inv);
 }
 
 template <class TIn>
@@ -1003,16 +1292,17 @@ uint32_t gather_segments(uint32_t K, size_t ldn) {
 
 template <bool G, bool S, bool M>
 static void gather_sums_t(dim3 grid, hipStream_t st, float* U, size_t ldu, uint32_t col0, double* Spart,
-                          uint32_t* masks, uint32_t Kw, const double* E, size_t lde, const int32_t* knodes,
-                          const double* klen, uint32_t K, uint32_t rps) {
-    hipLaunchKernelGGL((k_gather_sums<G, S, M>), grid, dim3(EMB_THREADS), 0, st, U, ldu, col0, Spart, masks, Kw, E,
-                       lde, knodes, klen, K, rps);
+                          uint32_t* masks, uint32_t Kw, uint32_t* smask, const double* E, size_t lde,
+                          const int32_t* knodes, const double* klen, uint32_t K, uint32_t rps, uint32_t n_valid) {
+    hipLaunchKernelGGL((k_gather_sums<G, S, M>), grid, dim3(EMB_THREADS), 0, st, U, ldu, col0, Spart, masks, Kw,
+                       smask, E, lde, knodes, klen, K, rps, n_valid);
 }
 
-hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, uint32_t* masks, const double* E,
-                              size_t lde, uint32_t col0, const int32_t* knodes, const double* klen, uint32_t K,
-                              uint32_t n, hipStream_t st) {
+hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, uint32_t* masks, uint32_t* smask,
+                              const double* E, size_t lde, uint32_t col0, const int32_t* knodes, const double* klen,
+                              uint32_t K, uint32_t n, hipStream_t st) {
     if (masks && (lde % 64 || col0 % 64)) return hipErrorInvalidValue;
+    if (smask && !masks) return hipErrorInvalidValue;
     if (col0 >= n) return hipSuccess;
     const uint32_t cnt = (uint32_t)std::min<size_t>(lde, n - col0);   // real samples in this block
     if (!K) {
@@ -1029,11 +1319,11 @@ hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, ui
     const dim3 grid(cb, segs);
     const int key = (U ? 4 : 0) | (S ? 2 : 0) | (masks ? 1 : 0);
     switch (key) {
-    case 7: gather_sums_t<true, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, E, lde, knodes, klen, K, rps); break;
-    case 6: gather_sums_t<true, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, E, lde, knodes, klen, K, rps); break;
-    case 4: gather_sums_t<true, false, false>(grid, st, U, ldu, col0, Spart, masks, Kw, E, lde, knodes, klen, K, rps); break;
-    case 3: gather_sums_t<false, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, E, lde, knodes, klen, K, rps); break;
-    case 2: gather_sums_t<false, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, E, lde, knodes, klen, K, rps); break;
+    case 7: gather_sums_t<true, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
+    case 6: gather_sums_t<true, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
+    case 4: gather_sums_t<true, false, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
+    case 3: gather_sums_t<false, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
+    case 2: gather_sums_t<false, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
     case 0: return hipSuccess;
     default: return hipErrorInvalidValue;
     }
@@ -1114,7 +1404,47 @@ std::vector<uint32_t> rect_block_order(uint32_t n) {
 template <class TOut>
 static void pairs_sparse32_t(uint32_t grid, hipStream_t st, const SparseArgs& a, uint32_t per_xcd) {
     hipLaunchKernelGGL((k_pairs_sparse32<TOut>), dim3(grid), dim3(64), 0, st, (const float*)a.U, a.ldn, a.len, a.masks,
-                       a.Kw, a.n, a.S, (int)a.normalized, (TOut*)a.out, a.visits, a.order, a.n_blocks, per_xcd);
+                       a.Kw, a.n, a.S, (int)a.normalized, (TOut*)a.out, a.visits, a.order, a.n_blocks, per_xcd,
+                       a.dmask, (int)(a.dmask != nullptr));
+}
+
+hipError_t launch_sparse_nodes(const SparseNodesArgs& a, hipStream_t st) {
+    if (!a.K || !a.n) return hipSuccess;
+    if (!a.dmask || !a.smask || !a.pc || !a.pidx || !a.pval || !a.U || !a.len || !a.S || !a.out || !a.cap ||
+        !a.rows || !a.rcount || !a.roff || !a.rbsum ||
+        a.npan != (a.n + 31) / 32 || a.ldu < (size_t)a.npan * 32)
+        return hipErrorInvalidValue;
+    const uint32_t gk = (a.K + 63) / 64;
+    hipLaunchKernelGGL(k_node_split, dim3(gk), dim3(64), 0, st, a.dmask, a.smask, a.K, a.npan, a.cap);
+    hipLaunchKernelGGL(k_postings, dim3(gk), dim3(64), 0, st, a.pidx, a.pval, a.pc, a.smask, a.dmask, a.U, a.ldu, a.K,
+                       a.npan, a.cap);
+    // row lists: counts per (sample, node chunk), their exclusive scan, then the entries
+    const uint32_t nkc = row_list_chunks(a.K);
+    const size_t m = (size_t)a.n * nkc;
+    const uint32_t nsb = (uint32_t)((m + SCAN_BLOCK - 1) / SCAN_BLOCK);
+    hipLaunchKernelGGL(k_row_lists, dim3(a.npan, nkc), dim3(64), 0, st, nullptr, a.rcount, nullptr, a.smask, a.dmask,
+                       a.pc, a.K, a.n, nkc);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_T), 0, st, a.rbsum, a.rcount, m);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, a.rbsum, nsb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_T), 0, st, a.roff, a.rcount, a.rbsum, m, nsb);
+    hipLaunchKernelGGL(k_row_lists, dim3(a.npan, nkc), dim3(64), 0, st, a.rows, nullptr, a.roff, a.smask, a.dmask,
+                       a.pc, a.K, a.n, nkc);
+    const uint32_t nchunks = (a.n + SR_CW - 1) / SR_CW;
+    const uint64_t blocks = (uint64_t)a.n * nchunks;
+    if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+    if (a.out_fp64)
+        hipLaunchKernelGGL((k_pairs_rows<double>), dim3((uint32_t)blocks), dim3(64 * SR_WAVES), 0, st, (double*)a.out, a.walked,
+                           a.rows, a.roff, a.pc, a.pidx, a.pval, a.len, a.S, a.K, a.n, a.npan, a.cap, nchunks, nkc);
+    else
+        hipLaunchKernelGGL((k_pairs_rows<float>), dim3((uint32_t)blocks), dim3(64 * SR_WAVES), 0, st, (float*)a.out, a.walked,
+                           a.rows, a.roff, a.pc, a.pidx, a.pval, a.len, a.S, a.K, a.n, a.npan, a.cap, nchunks, nkc);
+    return hipGetLastError();
+}
+
+uint32_t sparse_rows_blocks(uint32_t n) { return n * ((n + SR_CW - 1) / SR_CW); }
+uint32_t row_list_chunks(uint32_t K) { return std::max<uint32_t>(1, (K + RL_KC - 1) / RL_KC); }
+size_t row_list_scan_blocks(uint32_t n, uint32_t K) {
+    return ((size_t)n * row_list_chunks(K) + SCAN_BLOCK - 1) / SCAN_BLOCK;
 }
 
 hipError_t launch_pairs_sparse32(const SparseArgs& a, hipStream_t st) {

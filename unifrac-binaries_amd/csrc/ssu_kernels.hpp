@@ -44,7 +44,8 @@ hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint3
 //   masks (if non-null; lde, col0 multiples of 64): [ldn/32][ceil(K/32)] words, bit r%32 of
 //     word r/32 of panel p set iff some sample of p*32..p*32+31 has u_rs > 0.
 uint32_t gather_segments(uint32_t K, size_t ldn);
-hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, uint32_t* masks, const double* E,
+hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, uint32_t* masks, uint32_t* smask,
+                              const double* E,
                               size_t lde, uint32_t col0, const int32_t* knodes, const double* klen, uint32_t K,
                               uint32_t n, hipStream_t st);
 // Faith PD: out[s] = sum over all non-root nodes k in postorder of (E[k][s] > 0) * len[k]
@@ -109,7 +110,37 @@ struct SparseArgs {
     uint32_t* visits;       // optional [nb][nb], nb = ceil(n/64): nodes visited per block (bi <= bj)
     const uint32_t* order;  // device copy of upper_block_order(ceil(n/64), 16)
     uint32_t n_blocks;      // its length, nb (nb + 1) / 2
+    const uint32_t* dmask = nullptr;   // FP32 kernel: visit only these nodes and add the partial sums in `out`
 };
+
+// Occupancy split of the FP32 shared-node path.  Nodes held by at least `cap`
+// samples are dense (bit set in dmask, [ceil(K/32)] words); for the others
+// the posting lists are built from U and the sample masks, and each row's
+// share of M over them is written to out[a * n + b] (b > a) for the tiled
+// kernel to add (SparseArgs::dmask).
+struct SparseNodesArgs {
+    uint32_t* dmask;
+    const uint32_t* smask;  // [npan][K] sample bits, from launch_gather_sums
+    uint32_t* pc;           // [npan + 1][K]
+    uint32_t* pidx;         // [K][cap]
+    float* pval;            // [K][cap]
+    const float* U;         // [K][ldu]
+    size_t ldu;
+    const double* len;      // [K]
+    const double* S;        // [n]
+    void* out;
+    bool out_fp64;
+    uint32_t* walked;       // optional [sparse_rows_blocks(n)]: postings walked per wave
+    unsigned long long* rows;   // [K * cap] row-list entries (node << 32 | position in its postings)
+    uint32_t* rcount;       // [n * row_list_chunks(K)]
+    uint32_t* roff;         // [n * row_list_chunks(K) + 1]
+    uint32_t* rbsum;        // [row_list_scan_blocks(n, K) + 1]
+    uint32_t K, n, npan, cap;
+};
+hipError_t launch_sparse_nodes(const SparseNodesArgs& a, hipStream_t st);
+uint32_t sparse_rows_blocks(uint32_t n);
+uint32_t row_list_chunks(uint32_t K);
+size_t row_list_scan_blocks(uint32_t n, uint32_t K);
 // The upper-triangle blocks (bi <= bj < nb) as bi << 16 | bj, super x super
 // super-tile by super-tile; kernels walk it in XCD-contiguous runs.
 std::vector<uint32_t> upper_block_order(uint32_t nb, uint32_t super);

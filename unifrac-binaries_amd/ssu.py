@@ -83,7 +83,7 @@ EXPORTS = [
     "unifrac_to_file_v3", "unifrac_to_txt_file_v3", "unifrac_to_file_v2", "unifrac_to_file",
     "unifrac_multi_to_file_v3", "unifrac_multi_to_file_v2",
     "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
-    "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
+    "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_sparse_terms", "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
     "ssu_newick_to_bp", "ssu_biom_n_samples", "ssu_device_available", "ssu_last_error",
     "ssu_rarefy_create_inmem", "ssu_rarefy_execute", "ssu_rarefy_n_samples", "ssu_rarefy_n_obs", "ssu_rarefy_nnz",
     "ssu_rarefy_kept", "ssu_rarefy_copy_counts", "ssu_rarefy_copy_matrix", "ssu_rarefy_last_ms",
@@ -108,6 +108,7 @@ def lib():
             getattr(L, n).restype = C.c_int
         L.ssu_plan_pair_nodes.restype = C.c_uint64
         L.ssu_plan_kernel_terms.restype = C.c_uint64
+        L.ssu_plan_sparse_terms.restype = C.c_uint64
         L.ssu_plan_main_kernel.restype = C.c_char_p
         L.ssu_last_error.restype = C.c_char_p
         L.ssu_postorder.restype = C.c_int
@@ -246,14 +247,17 @@ class Plan:
     def timings(self):
         ms = (C.c_double * 8)()
         k = lib().ssu_plan_timings(self.h, ms, 8)
-        keys = ["embed", "prep", "main", "combine", "total"]
-        return {keys[i]: ms[i] for i in range(min(k, 5))}
+        keys = ["embed", "prep", "main", "combine", "total", "sparse"]
+        return {keys[i]: ms[i] for i in range(min(k, 6))}
 
     def pair_nodes(self) -> int:
         return int(lib().ssu_plan_pair_nodes(self.h))
 
     def kernel_terms(self) -> int:
         return int(lib().ssu_plan_kernel_terms(self.h))
+
+    def sparse_terms(self) -> int:
+        return int(lib().ssu_plan_sparse_terms(self.h))
 
     def main_kernel(self) -> str:
         return lib().ssu_plan_main_kernel(self.h).decode()
