@@ -268,8 +268,9 @@ SSU_EXTERN IOStatus write_mat_from_matrix(const char* filename, mat_full_fp64_t*
 SSU_EXTERN IOStatus write_mat_from_matrix_fp32(const char* filename, mat_full_fp32_t* result);   /* api.hpp:691 */
 SSU_EXTERN IOStatus write_vec(const char* filename, r_vec* result);                              /* api.hpp:763 */
 
-/* Distance-matrix HDF5 files (format "BDSM", version "2020.12"): "order" and,
- * when save_dist, "matrix".  pcoa_dims must be 0 (no PCoA in this build). */
+/* Distance-matrix HDF5 files (format "BDSM", version "2020.12"): "order", and
+ * "matrix" when save_dist; pcoa_dims > 0 adds "pcoa_method" ("FSVD"),
+ * "pcoa_eigvals", "pcoa_proportion_explained" and "pcoa_samples". */
 SSU_EXTERN IOStatus write_mat_from_matrix_hdf5_fp64(const char* filename, mat_full_fp64_t* result,
                                                     unsigned int pcoa_dims, int save_dist);  /* api.hpp:722 */
 SSU_EXTERN IOStatus write_mat_from_matrix_hdf5_fp32(const char* filename, mat_full_fp32_t* result,
@@ -292,10 +293,22 @@ SSU_EXTERN IOStatus write_mat_from_matrix_hdf5_fp32_v2(const char* output_filena
                                                        const char** stat_group_name_arr,
                                                        const unsigned int* stat_group_count_arr);   /* api.hpp:744 */
 
+/* PERMANOVA of each grouping column (TSV file: header line, sample id in the
+ * first field) on the matrix: pseudo-F, permutation p-value and number of
+ * groups per column; grouping_missing when a column or a sample is absent. */
+SSU_EXTERN ComputeStatus compute_permanova_fp64(const char* grouping_filename, unsigned int n_columns,
+                                                const char** columns, mat_full_fp64_t* result,
+                                                unsigned int permanova_perms, double* fstats, double* pvalues,
+                                                unsigned int* n_groups);                            /* api.hpp:616 */
+SSU_EXTERN ComputeStatus compute_permanova_fp32(const char* grouping_filename, unsigned int n_columns,
+                                                const char** columns, mat_full_fp32_t* result,
+                                                unsigned int permanova_perms, float* fstats, float* pvalues,
+                                                unsigned int* n_groups);                            /* api.hpp:636 */
+
 /* One matrix from files straight to an output file.  format: hdf5 (precision
- * of the method), hdf5_fp32, hdf5_fp64 or hdf5_nodist.  pcoa_dims and
- * permanova_perms must be 0: PCoA and PERMANOVA are not part of this library
- * (output_error, with the reason in ssu_last_error()). */
+ * of the method), hdf5_fp32, hdf5_fp64 or hdf5_nodist.  pcoa_dims > 0 adds
+ * the PCoA datasets, permanova_perms > 0 the PERMANOVA statistics of each
+ * comma-separated grouping column. */
 SSU_EXTERN ComputeStatus unifrac_to_file_v3(const char* biom_filename, const char* tree_filename,
                                             const char* out_filename, const char* unifrac_method,
                                             bool variance_adjust, double alpha, bool bypass_tips,
@@ -322,9 +335,10 @@ SSU_EXTERN ComputeStatus unifrac_to_file(const char* biom_filename, const char* 
                                          unsigned int pcoa_dims, const char* mmap_dir);             /* api.hpp:548 */
 
 /* Rarefaction replicates to one HDF5 file (format "SUJK", version "2023.03"):
- * "order" and, unless format is hdf5_nodist, "matrix:<i>" for i < n_subsamples.
- * The replicates are drawn and computed on the device, from one HIP graph per
- * batch of replicates.  pcoa_dims and permanova_perms must be 0. */
+ * "order" and, unless format is hdf5_nodist, "matrix:<i>" for i < n_subsamples,
+ * "pcoa_*:<i>" when pcoa_dims > 0 and the stat_* datasets when
+ * permanova_perms > 0.  The replicates are drawn and computed on the device,
+ * from one HIP graph per batch of replicates. */
 SSU_EXTERN ComputeStatus unifrac_multi_to_file_v3(const char* biom_filename, const char* tree_filename,
                                                   const char* out_filename, const char* unifrac_method,
                                                   bool variance_adjust, double alpha, bool bypass_tips,
@@ -396,6 +410,25 @@ SSU_EXTERN uint64_t ssu_plan_kernel_terms(const ssu_plan_t* plan);
 SSU_EXTERN uint64_t ssu_plan_sparse_terms(const ssu_plan_t* plan);
 SSU_EXTERN const char* ssu_plan_main_kernel(const ssu_plan_t* plan);
 SSU_EXTERN void ssu_plan_destroy(ssu_plan_t** plan);
+
+/* Gower-centred matrix, PCoA (randomized FSVD, k = dims + 2 probes, two power
+ * levels) and PERMANOVA on an n x n matrix, on the device.  The reference runs
+ * these in its third-party skbb library (src/skbio_alt.cpp:144-233,
+ * src/skbio_alt.hpp:22-52: su::mat_to_centered, su::pcoa, su::permanova),
+ * which exports no C entry point of its own.  samples is n x dims, sample
+ * major; dimensions past n come back zero.  grouping[i] in 0..groups-1; F and
+ * p are NaN when there is one group or every sample is its own group.  Random
+ * numbers follow the ssu_set_random_seed stream. */
+SSU_EXTERN ComputeStatus ssu_center_matrix_fp64(const double* mat, unsigned int n, double* centered);
+SSU_EXTERN ComputeStatus ssu_center_matrix_fp32(const float* mat, unsigned int n, float* centered);
+SSU_EXTERN ComputeStatus ssu_pcoa_fp64(const double* mat, unsigned int n, unsigned int dims, double* eigenvalues,
+                                       double* samples, double* proportion_explained);
+SSU_EXTERN ComputeStatus ssu_pcoa_fp32(const float* mat, unsigned int n, unsigned int dims, float* eigenvalues,
+                                       float* samples, float* proportion_explained);
+SSU_EXTERN ComputeStatus ssu_permanova_fp64(const double* mat, unsigned int n, const uint32_t* grouping,
+                                            unsigned int n_perm, double* fstat, double* pvalue);
+SSU_EXTERN ComputeStatus ssu_permanova_fp32(const float* mat, unsigned int n, const uint32_t* grouping,
+                                            unsigned int n_perm, float* fstat, float* pvalue);
 
 /* Rarefaction replicates on the device (the reference subsamples on the host,
  * src/biom_subsampled.cpp:155-321, one table per call).  Samples with a total

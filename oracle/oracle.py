@@ -3,6 +3,10 @@
 ctypes front-end for
 
 * ``liboracle.so``: the clean-room FP64 CPU restatement in ``ssu_oracle.c``;
+* numpy restatements of the distance-matrix statistics the reference takes
+  from its third-party skbb library (``src/skbio_alt.cpp:144-233``): Gower
+  centring, exact PCoA (a full symmetric eigendecomposition, the limit the
+  randomized FSVD approximates) and the PERMANOVA pseudo-F;
 * ``_ref/libssu_ref.so``: the reference implementation's own CPU code
   (``src/tree.cpp``, ``src/biom_inmem.cpp``, ``src/unifrac_internal.cpp`` and
   the ``run_*_T`` kernels of ``src/unifrac_task_impl.hpp``) behind
@@ -259,3 +263,45 @@ def ref_sample_counts(table: Table) -> np.ndarray:
     out = np.zeros(table.n_samples)
     ref().ref_sample_counts(*_ref_table_args(table), out.ctypes.data_as(C.c_void_p))
     return out
+
+
+# ------------------------------------------------ distance-matrix statistics
+# The published algorithms (scikit-bio's ``center_distance_matrix``, ``pcoa``
+# and ``permanova``) that skbb implements for src/skbio_alt.cpp:144-233; the
+# reference's vectors for them are in src/test_ska.cpp.
+
+
+def center_matrix(m: np.ndarray) -> np.ndarray:
+    """B = -1/2 (I - 11'/n) D∘D (I - 11'/n) (src/skbio_alt.hpp:22-27)."""
+    a = -0.5 * np.asarray(m, np.float64) ** 2
+    r = a.mean(axis=1)
+    c = a.mean(axis=0)
+    return a - r[:, None] - c[None, :] + a.mean()
+
+
+def pcoa_exact(m: np.ndarray, dims: int):
+    """Eigenvalues (largest first), coordinates v * sqrt(lambda) and proportion
+    lambda / trace(B) from a full eigendecomposition of the centred matrix."""
+    b = center_matrix(m)
+    w, v = np.linalg.eigh(b)
+    order = np.argsort(-np.abs(w))[:dims]
+    w, v = w[order], v[:, order]
+    return w, v * np.sqrt(np.maximum(w, 0.0)), w / np.trace(b)
+
+
+def permanova_f(m: np.ndarray, grouping) -> float:
+    """Pseudo-F over the upper triangle (scikit-bio ``_compute_f_stat``):
+    s_T = sum d^2 / n, s_W = sum_g sum_{i<j in g} d^2 / n_g,
+    F = ((s_T - s_W) / (G - 1)) / (s_W / (n - G))."""
+    d = np.asarray(m, np.float64)
+    g = np.asarray(grouping)
+    n = d.shape[0]
+    iu = np.triu_indices(n, 1)
+    d2 = d[iu] ** 2
+    s_t = d2.sum() / n
+    same = g[iu[0]] == g[iu[1]]
+    groups, sizes = np.unique(g, return_counts=True)
+    size_of = dict(zip(groups.tolist(), sizes.tolist()))
+    s_w = sum(d2[same & (g[iu[0]] == k)].sum() / size_of[k] for k in groups.tolist())
+    ng = len(groups)
+    return ((s_t - s_w) / (ng - 1)) / (s_w / (n - ng))

@@ -50,6 +50,8 @@ def h5_read(path):
             "out['format'] = f['format'][0].decode(); out['version'] = f['version'][0].decode()\n"
             "out['order'] = [x.decode() if isinstance(x, bytes) else x for x in f['order'][:]]\n"
             "out['matrix'] = f['matrix'][:].tolist() if 'matrix' in f else None\n"
+            "for k in ('pcoa_eigvals', 'pcoa_samples', 'pcoa_proportion_explained'):\n"
+            "    if k in f: out[k] = f[k][:].tolist()\n"
             "out['dtype'] = str(f['matrix'].dtype) if 'matrix' in f else None\n"
             "print(json.dumps(out))\n")
     r = subprocess.run([H5PY, "-c", code, path], capture_output=True, text=True, timeout=120)
@@ -143,9 +145,7 @@ def test_merge_partial_hdf5(ssu, tmp_path, fmt, dtype):
     h = h5_read(out)
     assert h["format"] == "BDSM" and h["version"] == "2020.12" and h["order"] == ids and h["dtype"] == dtype
     assert np.array_equal(np.asarray(h["matrix"]), full.astype(dtype).astype(np.float64))
-    r = run(SSU, "--mode", "merge-partial", "--partial-pattern", str(tmp_path / "m.start*.partial"), "-o", out,
-            "--format", fmt)
-    assert r.returncode != 0 and "PCoA" in r.stderr     # default --pcoa 10 is refused, not ignored
+    assert "pcoa_eigvals" not in h
 
 
 # ----------------------------------------------------------------- GPU tests
@@ -205,3 +205,22 @@ def test_faithpd_on_reference_inputs(ref_trees, goldens, tmp_path):
     lines = out.read_text().splitlines()
     assert lines[0] == "#SampleID\tfaith_pd"
     assert [float(x.split("\t")[1]) for x in lines[1:]] == [float(v) for v in goldens["faith_test_tre"]["values"]]
+
+
+@pytest.mark.gpu
+def test_merge_partial_hdf5_with_pcoa(ssu, O, tmp_path):
+    """--pcoa 3 (the default is 10) adds the PCoA datasets to the merged matrix
+    (src/su.cpp:172-237); the eigenvalues match the exact decomposition."""
+    ids = [f"s{i}" for i in range(12)]
+    pts = np.random.default_rng(3).normal(size=(12, 3)) * np.array([3.0, 1.5, 0.5])
+    full = np.sqrt(((pts[:, None, :] - pts[None, :, :]) ** 2).sum(-1))
+    write_partials(ssu, tmp_path, ids, full, [0, 3, 6])
+    out = str(tmp_path / "merged.h5")
+    r = run(SSU, "--mode", "merge-partial", "--partial-pattern", str(tmp_path / "m.start*.partial"), "-o", out,
+            "--format", "hdf5_fp64", "--pcoa", "3")
+    assert r.returncode == 0, r.stderr
+    h = h5_read(out)
+    w, coords, prop = O.pcoa_exact(full, 3)
+    assert np.max(np.abs(np.array(h["pcoa_eigvals"]) - w)) < 1e-9
+    assert np.max(np.abs(np.abs(np.array(h["pcoa_samples"])) - np.abs(coords))) < 1e-9
+    assert np.max(np.abs(np.array(h["pcoa_proportion_explained"]) - prop)) < 1e-9

@@ -102,14 +102,12 @@ def test_entry_point_status_codes(ssu, test_tre, tmp_path):
     out = str(tmp_path / "o.h5")
     assert to_file_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "bogus") == ssu.UNKNOWN_METHOD
     assert to_file_v3(ssu, TEST_BIOM, test_tre, out, "nope", "hdf5") == ssu.UNKNOWN_METHOD
-    assert to_file_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "hdf5", pcoa=3) == ssu.OUTPUT_ERROR
-    assert "PCoA" in ssu.last_error()
-    assert to_file_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "hdf5", perms=99) == ssu.OUTPUT_ERROR
-    assert "PERMANOVA" in ssu.last_error()
+    # PERMANOVA without a grouping file or columns
+    assert to_file_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "hdf5", perms=99) == ssu.GROUPING_MISSING
     assert multi_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "hdf5", 3, 0) == ssu.TABLE_EMPTY
     assert multi_v3(ssu, TEST_BIOM + ".missing", test_tre, out, "unweighted", "hdf5", 3, 5) == ssu.TABLE_MISSING
     assert multi_v3(ssu, TEST_BIOM, test_tre + ".missing", out, "unweighted", "hdf5", 3, 5) == ssu.TREE_MISSING
-    assert multi_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "hdf5", 3, 5, pcoa=10) == ssu.OUTPUT_ERROR
+    assert multi_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "hdf5", 3, 5, perms=9) == ssu.GROUPING_MISSING
     assert multi_v3(ssu, TEST_BIOM, test_tre, out, "unweighted", "ascii", 3, 5) == ssu.UNKNOWN_METHOD
 
 
@@ -142,9 +140,6 @@ def test_hdf5_v2_statistics_datasets(ssu, tmp_path):
     assert d["stat_values"]["value"] == [1.5, 2.5] and d["stat_values"]["dtype"] == "float64"
     assert d["stat_pvalues"]["value"] == [0.01, 0.2]
     assert d["stat_n_permutations"]["value"] == [999, 99]
-    st = ssu.lib().write_mat_from_matrix_hdf5_fp64_v2(out.encode(), C.byref(res), 2, 1, 0, None, None, None, None,
-                                                      None, None, None)
-    assert st == ssu.WRITE_ERROR                                 # PCoA refused
 
 
 def test_cli_multi_option_errors():
@@ -160,6 +155,10 @@ def test_cli_multi_option_errors():
     r = run("--mode", "multi", "-i", "x", "-t", "y", "-o", "z", "-m", "unweighted", "--subsample-depth", "5",
             "--n-subsamples", "0", "--pcoa", "0")
     assert r.returncode != 0 and "n_subsamples cannot be 0." in r.stderr
+    r = run("-i", "x", "-t", "y", "-o", "z", "-m", "unweighted", "--permanova", "99", "--format", "hdf5")
+    assert r.returncode != 0 and "grouping filename missing" in r.stderr
+    r = run("-i", "x", "-t", "y", "-o", "z", "-m", "unweighted", "--permanova", "99", "-g", "g.tsv")
+    assert r.returncode != 0 and "grouping columns missing" in r.stderr
 
 
 # ----------------------------------------------------------------- GPU tests

@@ -200,3 +200,59 @@ def test_oracle_vs_reference_live(O, synth, method, kw):
     assert rel_err(a, b) < 1e-11
     if method == "unweighted":
         np.testing.assert_array_equal(O.faith_pd(tree, tab), O.ref_faith_pd(tree, tab))
+
+
+# --------------------------------------- distance-matrix statistics (oracle)
+
+def _ska():
+    import json
+    from conftest import GOLDEN
+    return json.load(open(os.path.join(GOLDEN, "ska_vectors.json")))
+
+
+def test_oracle_centering_matches_reference_vectors(O):
+    """src/test_ska.cpp:7-60 (6 x 6) and the centred crawford matrix of :62-140."""
+    v = _ska()
+    c = v["center_mat"]
+    m = np.array(c["matrix"]).reshape(6, 6)
+    assert np.max(np.abs(O.center_matrix(m) - np.array(c["expected"]).reshape(6, 6))) < 1e-6
+    p = v["pcoa"]
+    m = np.array(p["matrix"]).reshape(9, 9)
+    assert np.max(np.abs(O.center_matrix(m) - np.array(p["centered"]).reshape(9, 9))) < 1e-5
+
+
+def test_oracle_pcoa_matches_reference_vectors(O):
+    """src/test_ska.cpp:142-245: eigenvalues, coordinates (up to sign) and
+    proportion explained of the 9-sample crawford matrix, 5 dimensions."""
+    p = _ska()["pcoa"]
+    m = np.array(p["matrix"]).reshape(9, 9)
+    w, coords, prop = O.pcoa_exact(m, 5)
+    assert np.max(np.abs(w - np.array(p["eigenvalues"]))) < 1e-5
+    assert np.max(np.abs(np.abs(coords) - np.abs(np.array(p["samples"]).reshape(9, 5)))) < 1e-5
+    assert np.max(np.abs(prop - np.array(p["proportion"]))) < 1e-5
+    # the coordinates are the unit eigenvectors scaled by sqrt(eigenvalue)
+    ev = np.array(p["eigenvectors"]).reshape(9, 5)
+    assert np.max(np.abs(np.abs(ev * np.sqrt(w)) - np.abs(coords))) < 1e-5
+
+
+@pytest.mark.parametrize("key", ["ties", "noties", "unequal"])
+def test_oracle_permanova_f_matches_reference_vectors(O, key):
+    """src/test_ska.cpp test_permanova_*: pseudo-F of skbio's examples, both
+    groupings and both the FP64 (asymmetric in the noties case: only the upper
+    triangle counts) and the FP32 matrices."""
+    v = _ska()[f"permanova_{key}"]
+    n = len(v["grouping"])
+    for mk in ("matrix_fp64", "matrix_fp32"):
+        m = np.array(v[mk]).reshape(n, n)
+        for gk in ("grouping", "grouping_swapped"):
+            assert abs(O.permanova_f(m, v[gk]) - v["stat"]) < 1e-5
+
+
+def test_oracle_permanova_f_closed_form(O):
+    """Two groups of two with within-group distance a and between-group b:
+    s_T = (2a^2 + 4b^2)/4, s_W = a^2, F = (s_T - s_W) / (s_W / 2)."""
+    a, b = 1.5, 4.0
+    m = np.array([[0, a, b, b], [a, 0, b, b], [b, b, 0, a], [b, b, a, 0]], float)
+    s_t = (2 * a * a + 4 * b * b) / 4
+    s_w = a * a
+    assert abs(O.permanova_f(m, [0, 0, 1, 1]) - (s_t - s_w) / (s_w / 2)) < 1e-12

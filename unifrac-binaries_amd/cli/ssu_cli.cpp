@@ -2,7 +2,7 @@
 // the reference's src/su.cpp (one-off, partial, merge-partial, check-partial,
 // partial-report, multi).  Output formats: ascii (the default) and, when the HDF5 C
 // library can be loaded at run time, hdf5 / hdf5_fp32 / hdf5_fp64 /
-// hdf5_nodist without PCoA or PERMANOVA (those need --pcoa 0 and no -g/-c).
+// hdf5_nodist, with PCoA (--pcoa, default 10) and PERMANOVA (-g/-c/--permanova).
 #include <glob.h>
 
 #include <cstdio>
@@ -43,7 +43,10 @@ void usage() {
            "    --subsample-depth\t[OPTIONAL] Depth of subsampling of the input BIOM before computing UniFrac.\n"
            "    --subsample-replacement\t[OPTIONAL] with (default) or without.\n"
            "    --n-subsamples\t[OPTIONAL] If mode==multi, the number of subsampled matrices (default: 100).\n"
-           "    --pcoa\t[OPTIONAL] Number of PCoA dimensions; only 0 is supported by this build.\n"
+           "    -g\t\t[OPTIONAL] Grouping (TSV) file for PERMANOVA.\n"
+           "    -c\t\t[OPTIONAL] The column(s) to use for grouping, multiple values comma separated.\n"
+           "    --permanova\t[OPTIONAL] Number of PERMANOVA permutations (default: 999 with -g and -c, none if 0).\n"
+           "    --pcoa\t[OPTIONAL] Number of PCoA dimensions (default: 10, none if 0).\n"
            "    --seed\t[OPTIONAL] Seed for the subsampling generator.\n"
            "    --diskbuf\t[OPTIONAL] Directory for a disk-backed result matrix.\n"
            "\n"
@@ -110,51 +113,20 @@ std::vector<std::string> glob_files(const std::string& pat) {
     return out;
 }
 
-int write_full(const std::string& out, Format fmt, mat_full_fp64_t* m64, mat_full_fp32_t* m32) {
+int write_full(const std::string& out, Format fmt, unsigned pcoa, mat_full_fp64_t* m64, mat_full_fp32_t* m32) {
     IOStatus st;
     if (fmt == format_ascii) {
         st = m64 ? write_mat_from_matrix(out.c_str(), m64) : write_mat_from_matrix_fp32(out.c_str(), m32);
     } else {
         const int save_dist = fmt != format_hdf5_nodist;
-        st = m64 ? write_mat_from_matrix_hdf5_fp64(out.c_str(), m64, 0, save_dist)
-                 : write_mat_from_matrix_hdf5_fp32(out.c_str(), m32, 0, save_dist);
+        st = m64 ? write_mat_from_matrix_hdf5_fp64(out.c_str(), m64, pcoa, save_dist)
+                 : write_mat_from_matrix_hdf5_fp32(out.c_str(), m32, pcoa, save_dist);
     }
     if (st != write_okay) {
         err("Unable to write; err " + std::to_string((int)st) + (ssu_last_error()[0] ? std::string(": ") + ssu_last_error() : ""));
         return EXIT_FAILURE;
     }
     return EXIT_SUCCESS;
-}
-
-// src/su.cpp:399-470
-int mode_one_off(const Args& a, Format fmt, const std::string& method, unsigned depth, bool with_replacement,
-                 bool vaw, double alpha, bool bypass, bool normalize, unsigned nsub, const std::string& diskbuf) {
-    const std::string table = a.get("-i"), tree = a.get("-t"), out = a.get("-o");
-    if (out.empty()) { err("output filename missing"); return EXIT_FAILURE; }
-    if (table.empty()) { err("table filename missing"); return EXIT_FAILURE; }
-    if (tree.empty()) { err("tree filename missing"); return EXIT_FAILURE; }
-    if (method.empty()) { err("method missing"); return EXIT_FAILURE; }
-    if (fmt == format_ascii && depth > 0) { err("Subsampling not supported with ASCII output."); return EXIT_FAILURE; }
-    const char* mmap_dir = diskbuf.empty() ? nullptr : diskbuf.c_str();
-    // the matrix precision follows the output format for hdf5_fp32/_fp64, else the method
-    const bool want64 = fmt == format_hdf5_fp64 || ((fmt == format_ascii || fmt == format_hdf5_nodist) && fp64_method(method));
-    ComputeStatus st;
-    mat_full_fp64_t* m64 = nullptr;
-    mat_full_fp32_t* m32 = nullptr;
-    if (want64)
-        st = one_off_matrix_v3(table.c_str(), tree.c_str(), method.c_str(), vaw, alpha, bypass, normalize, nsub, depth,
-                               with_replacement, mmap_dir, &m64);
-    else
-        st = one_off_matrix_fp32_v3(table.c_str(), tree.c_str(), method.c_str(), vaw, alpha, bypass, normalize, nsub,
-                                    depth, with_replacement, mmap_dir, &m32);
-    if (st != okay) {
-        fprintf(stderr, "Compute failed in one_off: %s\n", compute_message(st));
-        return EXIT_FAILURE;
-    }
-    const int rc = write_full(out, fmt, m64, m32);
-    if (m64) destroy_mat_full_fp64(&m64);
-    if (m32) destroy_mat_full_fp32(&m32);
-    return rc;
 }
 
 const char* format_name(Format f) {
@@ -166,22 +138,52 @@ const char* format_name(Format f) {
     }
 }
 
+// src/su.cpp:399-470
+int mode_one_off(const Args& a, Format fmt, const std::string& method, unsigned depth, bool with_replacement,
+                 unsigned pcoa, unsigned perms, const std::string& grouping, const std::string& columns, bool vaw,
+                 double alpha, bool bypass, bool normalize, unsigned nsub, const std::string& diskbuf) {
+    const std::string table = a.get("-i"), tree = a.get("-t"), out = a.get("-o");
+    if (out.empty()) { err("output filename missing"); return EXIT_FAILURE; }
+    if (table.empty()) { err("table filename missing"); return EXIT_FAILURE; }
+    if (tree.empty()) { err("tree filename missing"); return EXIT_FAILURE; }
+    if (perms > 0 && grouping.empty()) { err("grouping filename missing"); return EXIT_FAILURE; }
+    if (perms > 0 && columns.empty()) { err("grouping columns missing"); return EXIT_FAILURE; }
+    if (method.empty()) { err("method missing"); return EXIT_FAILURE; }
+    const char* mmap_dir = diskbuf.empty() ? nullptr : diskbuf.c_str();
+    ComputeStatus st;
+    if (fmt == format_ascii) {
+        if (depth > 0) { err("Subsampling not supported with ASCII output."); return EXIT_FAILURE; }
+        st = unifrac_to_txt_file_v3(table.c_str(), tree.c_str(), out.c_str(), method.c_str(), vaw, alpha, bypass,
+                                    normalize, nsub, mmap_dir);
+    } else {
+        st = unifrac_to_file_v3(table.c_str(), tree.c_str(), out.c_str(), method.c_str(), vaw, alpha, bypass,
+                                normalize, nsub, format_name(fmt), depth, with_replacement, pcoa, perms,
+                                perms > 0 ? grouping.c_str() : nullptr, perms > 0 ? columns.c_str() : nullptr,
+                                mmap_dir);
+    }
+    if (st != okay) fprintf(stderr, "Compute failed in one_off: %s\n", compute_message(st));
+    return st == okay ? EXIT_SUCCESS : EXIT_FAILURE;
+}
+
 // src/su.cpp:472-545
 int mode_multi(const Args& a, Format fmt, const std::string& method, unsigned n_subsamples, unsigned depth,
-               bool with_replacement, bool vaw, double alpha, bool bypass, bool normalize, unsigned nsub,
+               bool with_replacement, unsigned pcoa, unsigned perms, const std::string& grouping,
+               const std::string& columns, bool vaw, double alpha, bool bypass, bool normalize, unsigned nsub,
                const std::string& diskbuf) {
     const std::string table = a.get("-i"), tree = a.get("-t"), out = a.get("-o");
     if (out.empty()) { err("output filename missing"); return EXIT_FAILURE; }
     if (table.empty()) { err("table filename missing"); return EXIT_FAILURE; }
     if (tree.empty()) { err("tree filename missing"); return EXIT_FAILURE; }
+    if (perms > 0 && grouping.empty()) { err("grouping filename missing"); return EXIT_FAILURE; }
+    if (perms > 0 && columns.empty()) { err("grouping columns missing"); return EXIT_FAILURE; }
     if (method.empty()) { err("method missing"); return EXIT_FAILURE; }
     if (depth < 1) { err("subsample_depth cannot be 0."); return EXIT_FAILURE; }
     if (n_subsamples < 1) { err("n_subsamples cannot be 0."); return EXIT_FAILURE; }
     if (fmt == format_ascii) { err("ASCII format not supported in multi mode"); return EXIT_FAILURE; }
     const ComputeStatus st = unifrac_multi_to_file_v3(
         table.c_str(), tree.c_str(), out.c_str(), method.c_str(), vaw, alpha, bypass, normalize, nsub,
-        format_name(fmt), n_subsamples, depth, with_replacement, 0, 0, nullptr, nullptr,
-        diskbuf.empty() ? nullptr : diskbuf.c_str());
+        format_name(fmt), n_subsamples, depth, with_replacement, pcoa, perms, perms > 0 ? grouping.c_str() : nullptr,
+        perms > 0 ? columns.c_str() : nullptr, diskbuf.empty() ? nullptr : diskbuf.c_str());
     if (st != okay) fprintf(stderr, "Compute failed in multi: %s\n", compute_message(st));
     return st == okay ? EXIT_SUCCESS : EXIT_FAILURE;
 }
@@ -236,7 +238,7 @@ const char* PATTERN_HELP =
     "etc, then a pattern of 'ssu.unweighted.start*.partial' would make sense";
 
 // src/su.cpp:239-300
-int mode_merge_partial(const Args& a, Format fmt, const std::string& diskbuf) {
+int mode_merge_partial(const Args& a, Format fmt, unsigned pcoa, const std::string& diskbuf) {
     const std::string out = a.get("-o"), pattern = a.get("--partial-pattern");
     if (out.empty()) { err("output filename missing"); return EXIT_FAILURE; }
     if (pattern.empty()) { err(PATTERN_HELP); return EXIT_FAILURE; }
@@ -248,12 +250,12 @@ int mode_merge_partial(const Args& a, Format fmt, const std::string& diskbuf) {
     if (fmt == format_hdf5_fp32) {
         mat_full_fp32_t* m = nullptr;
         ms = merge_partial_to_mmap_matrix_fp32(mats.data(), (int)mats.size(), mmap_dir, &m);
-        if (ms == merge_okay) rc = write_full(out, fmt, nullptr, m);
+        if (ms == merge_okay) rc = write_full(out, fmt, pcoa, nullptr, m);
         if (m) destroy_mat_full_fp32(&m);
     } else {
         mat_full_fp64_t* m = nullptr;
         ms = merge_partial_to_mmap_matrix(mats.data(), (int)mats.size(), mmap_dir, &m);
-        if (ms == merge_okay) rc = write_full(out, fmt, m, nullptr);
+        if (ms == merge_okay) rc = write_full(out, fmt, pcoa, m, nullptr);
         if (m) destroy_mat_full_fp64(&m);
     }
     if (ms != merge_okay) err("Unable to complete merge; err " + std::to_string((int)ms));
@@ -347,6 +349,10 @@ int main(int argc, char** argv) {
     }
     const std::string pcoa_s = a.get("--pcoa");
     const unsigned pcoa = pcoa_s.empty() ? 10 : (unsigned)atoi(pcoa_s.c_str());
+    // src/su.cpp:700-711: 999 permutations by default when a grouping file and columns are given
+    const std::string grouping = a.get("-g"), columns = a.get("-c"), perm_s = a.get("--permanova");
+    const unsigned perms = !perm_s.empty() ? (unsigned)atoi(perm_s.c_str())
+                                           : ((grouping.empty() || columns.empty()) ? 0u : 999u);
     const std::string seed_s = a.get("--seed");
     if (!seed_s.empty()) ssu_set_random_seed((unsigned)atoi(seed_s.c_str()));
     const bool multi = mode == "multi" || mode == "multiple";
@@ -364,27 +370,20 @@ int main(int argc, char** argv) {
         return EXIT_FAILURE;
     }
     const std::string diskbuf = a.get("--diskbuf");
-    if (fmt != format_ascii && pcoa > 0) {
-        err("PCoA is not available in this build: pass --pcoa 0 with HDF5 output");
-        return EXIT_FAILURE;
-    }
-    if (!a.get("-g").empty() || !a.get("-c").empty() || !a.get("--permanova").empty()) {
-        err("PERMANOVA is not available in this build");
-        return EXIT_FAILURE;
-    }
 
     if (mode.empty() || mode == "one-off")
-        return mode_one_off(a, fmt, method, depth, with_replacement, vaw, alpha, bypass, normalize, nsub, diskbuf);
+        return mode_one_off(a, fmt, method, depth, with_replacement, pcoa, perms, grouping, columns, vaw, alpha, bypass,
+                            normalize, nsub, diskbuf);
     if (mode == "partial") {
         if (depth > 0) { err("Cannot subsample in partial mode."); return EXIT_FAILURE; }
         return mode_partial(a, method, vaw, alpha, bypass, normalize, nsub, start, stop);
     }
-    if (mode == "merge-partial") return mode_merge_partial(a, fmt, diskbuf);
+    if (mode == "merge-partial") return mode_merge_partial(a, fmt, pcoa, diskbuf);
     if (mode == "check-partial") return mode_check_partial(a);
     if (mode == "partial-report") return mode_partial_report(a, (unsigned)npart, bare);
     if (multi)
-        return mode_multi(a, fmt, method, n_subsamples, depth, with_replacement, vaw, alpha, bypass, normalize, nsub,
-                          diskbuf);
+        return mode_multi(a, fmt, method, n_subsamples, depth, with_replacement, pcoa, perms, grouping, columns, vaw,
+                          alpha, bypass, normalize, nsub, diskbuf);
     err("Unknown mode. Valid options are: one-off, partial, merge-partial, check-partial, partial-report, multi");
     return EXIT_SUCCESS;
 }

@@ -21,6 +21,8 @@
 #include "ssu_biom.hpp"
 #include "ssu_engine.hpp"
 #include "ssu_rarefy.hpp"
+#include "ssu_stats.hpp"
+#include "ssu_tsv.hpp"
 #include "ssu_results.hpp"
 #include "ssu_table.hpp"
 #include "ssu_tree.hpp"
@@ -767,33 +769,6 @@ IOStatus write_mat_from_matrix_fp32(const char* output_filename, mat_full_fp32_t
     return write_full_txt(output_filename, result);
 }
 
-// src/api_compat.hpp:180-189 -> src/api.cpp:1787-1935.  PCoA (pcoa_dims > 0)
-// needs the third-party eigensolver the reference links and is refused here.
-template <class TMat>
-IOStatus write_full_hdf5(const char* filename, TMat* result, unsigned int pcoa_dims, int save_dist, bool fp64) {
-    if (!filename || !result) return write_error;
-    if (pcoa_dims > 0) {
-        set_error("PCoA output is not available in this build (pcoa_dims must be 0)");
-        return write_error;
-    }
-    std::string err;
-    if (!ssu::write_matrix_hdf5(filename, result->n_samples, result->sample_ids, result->matrix, fp64, save_dist != 0,
-                                &err)) {
-        set_error(err);
-        return write_error;
-    }
-    return write_okay;
-}
-
-IOStatus write_mat_from_matrix_hdf5_fp64(const char* filename, mat_full_fp64_t* result, unsigned int pcoa_dims,
-                                         int save_dist) {
-    return write_full_hdf5(filename, result, pcoa_dims, save_dist, true);
-}
-
-IOStatus write_mat_from_matrix_hdf5_fp32(const char* filename, mat_full_fp32_t* result, unsigned int pcoa_dims,
-                                         int save_dist) {
-    return write_full_hdf5(filename, result, pcoa_dims, save_dist, false);
-}
 
 namespace {
 
@@ -812,25 +787,49 @@ ComputeStatus format_precision(const char* method, const char* format, bool& fp6
     return unknown_method;
 }
 
-// PCoA and PERMANOVA need an eigensolver and a grouping reader this library
-// does not carry; callers asking for them get output_error and a message.
-bool refuse_stats(unsigned int pcoa_dims, unsigned int permanova_perms) {
-    if (pcoa_dims > 0) { set_error("PCoA is not available in this build (pcoa_dims must be 0)"); return true; }
-    if (permanova_perms > 0) { set_error("PERMANOVA is not available in this build (permanova_perms must be 0)"); return true; }
+// PERMANOVA needs a grouping file and its columns (src/su.cpp:495-503)
+bool missing_grouping(unsigned int permanova_perms, const char* grouping_filename, const char* grouping_columns) {
+    if (permanova_perms > 0 && (!grouping_filename || !grouping_columns || !grouping_columns[0])) {
+        set_error("PERMANOVA needs a grouping file and grouping columns");
+        return true;
+    }
     return false;
 }
 
-// src/api.cpp:1787-1935 with the statistics datasets of :1880-1916
+// PCoA of a full matrix (src/skbio_alt.cpp:170-214) into arrays of the
+// result's element type; the seed comes from the ssu_set_random_seed stream.
+template <class TReal>
+bool pcoa_of(const TReal* matrix, uint32_t n, unsigned int dims, std::vector<TReal>& eig, std::vector<TReal>& samples,
+             std::vector<TReal>& prop, std::string* err) {
+    std::vector<double> e(dims), s((size_t)n * dims), p(dims);
+    if (!ssu::pcoa(matrix, sizeof(TReal) == sizeof(double), n, dims, g_rng(), e.data(), s.data(), p.data(), err))
+        return false;
+    eig.assign(e.begin(), e.end());
+    samples.assign(s.begin(), s.end());
+    prop.assign(p.begin(), p.end());
+    return true;
+}
+
+// the three PCoA datasets, in the order of src/api.cpp:1264-1290 (append_hdf5_pcoa)
+template <class TReal>
+bool write_pcoa(ssu::H5Writer& w, const TReal* matrix, uint32_t n, unsigned int dims, const std::string& suffix,
+                std::string* err) {
+    std::vector<TReal> eig, samples, prop;
+    if (!pcoa_of(matrix, n, dims, eig, samples, prop, err)) return false;
+    const bool fp64 = sizeof(TReal) == sizeof(double);
+    return w.reals(("pcoa_eigvals" + suffix).c_str(), dims, eig.data(), fp64) &&
+           w.reals(("pcoa_proportion_explained" + suffix).c_str(), dims, prop.data(), fp64) &&
+           w.matrix(("pcoa_samples" + suffix).c_str(), n, dims, samples.data(), fp64);
+}
+
+// src/api.cpp:1787-1935, including PCoA (:1834-1860) and the statistics
+// datasets (:1880-1916)
 template <class TMat, class TReal>
 IOStatus write_full_hdf5_stats(const char* filename, TMat* result, unsigned int pcoa_dims, int save_dist, bool fp64,
                                unsigned int n_vals, const char* const* methods, const char* const* names,
                                const TReal* vals, const TReal* pvals, const unsigned int* perms,
                                const char* const* group_names, const unsigned int* group_counts) {
     if (!filename || !result) return write_error;
-    if (pcoa_dims > 0) {
-        set_error("PCoA output is not available in this build (pcoa_dims must be 0)");
-        return write_error;
-    }
     std::string err;
     auto w = ssu::H5Writer::create(filename, &err);
     if (!w) { set_error(err); return write_error; }
@@ -838,6 +837,10 @@ IOStatus write_full_hdf5_stats(const char* filename, TMat* result, unsigned int 
     bool ok = w->string("format", "BDSM") && w->string("version", "2020.12") &&
               w->strings("order", n, result->sample_ids);
     if (ok && save_dist) ok = w->matrix("matrix", n, n, result->matrix, fp64);
+    if (ok && pcoa_dims > 0) {
+        ok = w->string("pcoa_method", "FSVD") && write_pcoa(*w, result->matrix, n, pcoa_dims, "", &err);
+        if (!ok && !err.empty()) { set_error(err); return write_error; }
+    }
     if (ok && n_vals > 0) {
         ok = methods && names && vals && pvals && perms && group_names && group_counts &&
              w->strings("stat_methods", n_vals, methods) && w->strings("stat_test_names", n_vals, names) &&
@@ -847,6 +850,35 @@ IOStatus write_full_hdf5_stats(const char* filename, TMat* result, unsigned int 
     }
     if (!ok) { set_error(std::string("HDF5 write failed for ") + filename); return write_error; }
     return write_okay;
+}
+
+// src/api.cpp:951-976 (compute_permanova_T): each grouping column separately
+template <class TReal, class TMat>
+ComputeStatus permanova_columns(const char* grouping_filename, unsigned int n_columns, const char* const* columns,
+                                const TMat* result, unsigned int permanova_perms, TReal* fstats, TReal* pvalues,
+                                unsigned int* n_groups) {
+    if (!grouping_filename || !result || (n_columns && (!columns || !fstats || !pvalues || !n_groups)))
+        return grouping_missing;
+    std::vector<std::string> ids(result->sample_ids, result->sample_ids + result->n_samples);
+    for (unsigned int c = 0; c < n_columns; ++c) {
+        std::vector<uint32_t> grouping;
+        uint32_t groups = 0;
+        std::string err;
+        if (!ssu::read_grouping(grouping_filename, ids, columns[c], grouping, groups, &err)) {
+            set_error(err);
+            return grouping_missing;
+        }
+        double f = 0.0, p = 0.0;
+        if (!ssu::permanova(result->matrix, sizeof(TReal) == sizeof(double), result->n_samples, grouping.data(),
+                            permanova_perms, g_rng(), &f, &p, &err)) {
+            set_error(err);
+            return output_error;
+        }
+        fstats[c] = (TReal)f;
+        pvalues[c] = (TReal)p;
+        n_groups[c] = groups;
+    }
+    return okay;
 }
 
 // Copies matrix r of method k of a batch into host memory of element type TReal.
@@ -878,14 +910,17 @@ template <class TReal>
 ComputeStatus multi_to_file(bool save_dist, const char* biom_filename, const char* tree_filename,
                             const char* out_filename, const char* method_name, bool vaw, double alpha, bool bypass,
                             bool normalize, unsigned int n_subsamples, unsigned int depth, bool with_replacement,
-                            unsigned int pcoa_dims, unsigned int permanova_perms) {
+                            unsigned int pcoa_dims, unsigned int permanova_perms, const char* grouping_filename,
+                            const char* grouping_columns) {
     if (depth < 1) {
         fprintf(stderr, "ERROR: subsampling depth cannot be 0.\n");
         return table_empty;
     }
     if (!biom_filename || !file_exists(biom_filename)) return table_missing;
     if (!tree_filename || !file_exists(tree_filename)) return tree_missing;
-    if (refuse_stats(pcoa_dims, permanova_perms)) return output_error;
+    if (missing_grouping(permanova_perms, grouping_filename, grouping_columns)) return grouping_missing;
+    const std::vector<std::string> columns =
+        permanova_perms > 0 ? ssu::split_commas(grouping_columns) : std::vector<std::string>();
     Method m;
     bool m64;
     if (!parse_method(method_name, m, m64)) return unknown_method;
@@ -899,10 +934,15 @@ ComputeStatus multi_to_file(bool save_dist, const char* biom_filename, const cha
     const bool fp64 = sizeof(TReal) == sizeof(double);
     std::string err;
     auto w = ssu::H5Writer::create(out_filename, &err);
-    if (!w || !w->string("format", "SUJK") || !w->string("version", "2023.03")) {
+    if (!w || !w->string("format", "SUJK") || !w->string("version", "2023.03") ||
+        (pcoa_dims > 0 && !w->string("pcoa_method", "FSVD"))) {
         set_error(err.empty() ? std::string("HDF5 write failed for ") + out_filename : err);
         return output_error;
     }
+    // statistics of every replicate x column (src/api.cpp:1509-1559)
+    std::vector<TReal> st_f, st_p;
+    std::vector<unsigned int> st_groups;
+    std::vector<std::string> st_names;
     uint32_t n_kept = 0;
     for (uint32_t n = 0; n < t.n_samples; ++n) n_kept += t.totals[n] >= depth;
     const size_t per_rep = std::max<size_t>((size_t)n_kept * n_kept * (m64 ? 8 : 4), 1);
@@ -910,16 +950,35 @@ ComputeStatus multi_to_file(bool save_dist, const char* biom_filename, const cha
     std::vector<TReal> mat;
     uint32_t done = 0;
     bool order_written = false;
+    ComputeStatus emit_status = okay;
     auto emit = [&](uint32_t n, const std::vector<std::string>& ids) -> bool {
+        std::vector<const char*> p;
+        for (const auto& x : ids) p.push_back(x.c_str());
         if (!order_written) {
-            std::vector<const char*> p;
-            for (const auto& x : ids) p.push_back(x.c_str());
             if (!w->strings("order", n, p.data())) return false;
             order_written = true;
         }
-        if (!save_dist) return true;
-        const std::string name = "matrix:" + std::to_string(done);
-        return w->matrix(name.c_str(), n, n, mat.data(), fp64);
+        const std::string tag = ":" + std::to_string(done);
+        if (permanova_perms > 0) {   // before PCoA, which does not touch the matrix here either
+            struct { uint32_t n_samples; uint32_t flags; TReal* matrix; char** sample_ids; } view{
+                n, 0, mat.data(), const_cast<char**>(p.data())};
+            std::vector<const char*> cols;
+            for (const auto& c : columns) cols.push_back(c.c_str());
+            std::vector<TReal> f(columns.size()), pv(columns.size());
+            std::vector<unsigned int> g(columns.size());
+            emit_status = permanova_columns<TReal>(grouping_filename, (unsigned int)cols.size(), cols.data(), &view,
+                                                   permanova_perms, f.data(), pv.data(), g.data());
+            if (emit_status != okay) return false;
+            for (size_t c = 0; c < columns.size(); ++c) {
+                st_f.push_back(f[c]);
+                st_p.push_back(pv[c]);
+                st_groups.push_back(g[c]);
+                st_names.push_back(columns[c] + tag);
+            }
+        }
+        if (save_dist && !w->matrix(("matrix" + tag).c_str(), n, n, mat.data(), fp64)) return false;
+        if (pcoa_dims > 0 && !write_pcoa(*w, mat.data(), n, pcoa_dims, tag, &err)) return false;
+        return true;
     };
     while (done < n_subsamples) {
         const uint32_t reps = std::min(chunk, n_subsamples - done);
@@ -935,7 +994,11 @@ ComputeStatus multi_to_file(bool save_dist, const char* biom_filename, const cha
             mat.resize((size_t)n * n);
             for (uint32_t r = 0; r < reps; ++r, ++done) {
                 if (!batch_matrix<TReal>(*b, r, mat.data(), &err)) { set_error(err); return output_error; }
-                if (!emit(n, b->kept().sample_ids)) { set_error("HDF5 write failed"); return output_error; }
+                if (!emit(n, b->kept().sample_ids)) {
+                    if (emit_status != okay) return emit_status;
+                    set_error(err.empty() ? "HDF5 write failed" : err);
+                    return output_error;
+                }
             }
         } else {
             std::vector<Table> subs;
@@ -945,15 +1008,120 @@ ComputeStatus multi_to_file(bool save_dist, const char* biom_filename, const cha
                 mat.resize((size_t)sub.n_samples * sub.n_samples);
                 st = run_full<TReal>(sub, tree, spec, mat.data());
                 if (st != okay) return st;
-                if (!emit(sub.n_samples, sub.sample_ids)) { set_error("HDF5 write failed"); return output_error; }
+                if (!emit(sub.n_samples, sub.sample_ids)) {
+                    if (emit_status != okay) return emit_status;
+                    set_error(err.empty() ? "HDF5 write failed" : err);
+                    return output_error;
+                }
                 ++done;
             }
         }
+    }
+    if (permanova_perms > 0 && !st_f.empty()) {   // src/api.cpp:1390-1424 (write_stats)
+        const size_t k = st_f.size();
+        std::vector<const char*> meth(k, "PERMANOVA"), test(k, "pseudo-F"), names;
+        for (const auto& x : st_names) names.push_back(x.c_str());
+        std::vector<unsigned int> perms(k, permanova_perms);
+        const bool ok = w->strings("stat_methods", k, meth.data()) && w->strings("stat_test_names", k, test.data()) &&
+                        w->strings("stat_grouping_names", k, names.data()) &&
+                        w->u32s("stat_n_groups", k, st_groups.data()) && w->reals("stat_values", k, st_f.data(), fp64) &&
+                        w->reals("stat_pvalues", k, st_p.data(), fp64) &&
+                        w->u32s("stat_n_permutations", k, perms.data());
+        if (!ok) { set_error("HDF5 write failed"); return output_error; }
     }
     return okay;
 }
 
 }  // namespace
+
+IOStatus write_mat_from_matrix_hdf5_fp64(const char* filename, mat_full_fp64_t* result, unsigned int pcoa_dims,
+                                         int save_dist) {
+    return write_full_hdf5_stats<mat_full_fp64_t, double>(filename, result, pcoa_dims, save_dist, true, 0, nullptr,
+                                                          nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+IOStatus write_mat_from_matrix_hdf5_fp32(const char* filename, mat_full_fp32_t* result, unsigned int pcoa_dims,
+                                         int save_dist) {
+    return write_full_hdf5_stats<mat_full_fp32_t, float>(filename, result, pcoa_dims, save_dist, false, 0, nullptr,
+                                                         nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+ComputeStatus compute_permanova_fp64(const char* grouping_filename, unsigned int n_columns, const char** columns,
+                                     mat_full_fp64_t* result, unsigned int permanova_perms, double* fstats,
+                                     double* pvalues, unsigned int* n_groups) {
+    return permanova_columns<double>(grouping_filename, n_columns, columns, result, permanova_perms, fstats, pvalues,
+                                     n_groups);
+}
+
+ComputeStatus compute_permanova_fp32(const char* grouping_filename, unsigned int n_columns, const char** columns,
+                                     mat_full_fp32_t* result, unsigned int permanova_perms, float* fstats,
+                                     float* pvalues, unsigned int* n_groups) {
+    return permanova_columns<float>(grouping_filename, n_columns, columns, result, permanova_perms, fstats, pvalues,
+                                    n_groups);
+}
+
+ComputeStatus ssu_center_matrix_fp64(const double* mat, unsigned int n, double* centered) {
+    if (!mat || !centered) return output_error;
+    std::string err;
+    if (!ssu::center_matrix(mat, true, n, centered, &err)) { set_error(err); return output_error; }
+    return okay;
+}
+
+ComputeStatus ssu_center_matrix_fp32(const float* mat, unsigned int n, float* centered) {
+    if (!mat || !centered) return output_error;
+    std::string err;
+    std::vector<double> tmp((size_t)n * n);
+    if (!ssu::center_matrix(mat, false, n, tmp.data(), &err)) { set_error(err); return output_error; }
+    for (size_t q = 0; q < tmp.size(); ++q) centered[q] = (float)tmp[q];
+    return okay;
+}
+
+template <class T>
+ComputeStatus pcoa_api(const T* mat, unsigned int n, unsigned int dims, T* eigenvalues, T* samples, T* proportion) {
+    if (!mat || !eigenvalues || !samples || !proportion) return output_error;
+    std::vector<T> e, s, p;
+    std::string err;
+    if (!pcoa_of(mat, n, dims, e, s, p, &err)) { set_error(err); return output_error; }
+    std::copy(e.begin(), e.end(), eigenvalues);
+    std::copy(s.begin(), s.end(), samples);
+    std::copy(p.begin(), p.end(), proportion);
+    return okay;
+}
+
+ComputeStatus ssu_pcoa_fp64(const double* mat, unsigned int n, unsigned int dims, double* eigenvalues,
+                            double* samples, double* proportion_explained) {
+    return pcoa_api(mat, n, dims, eigenvalues, samples, proportion_explained);
+}
+
+ComputeStatus ssu_pcoa_fp32(const float* mat, unsigned int n, unsigned int dims, float* eigenvalues, float* samples,
+                            float* proportion_explained) {
+    return pcoa_api(mat, n, dims, eigenvalues, samples, proportion_explained);
+}
+
+template <class T>
+ComputeStatus permanova_api(const T* mat, unsigned int n, const uint32_t* grouping, unsigned int n_perm, T* fstat,
+                            T* pvalue) {
+    if (!mat || !grouping || !fstat || !pvalue) return output_error;
+    double f = 0.0, p = 0.0;
+    std::string err;
+    if (!ssu::permanova(mat, sizeof(T) == sizeof(double), n, grouping, n_perm, g_rng(), &f, &p, &err)) {
+        set_error(err);
+        return output_error;
+    }
+    *fstat = (T)f;
+    *pvalue = (T)p;
+    return okay;
+}
+
+ComputeStatus ssu_permanova_fp64(const double* mat, unsigned int n, const uint32_t* grouping, unsigned int n_perm,
+                                 double* fstat, double* pvalue) {
+    return permanova_api(mat, n, grouping, n_perm, fstat, pvalue);
+}
+
+ComputeStatus ssu_permanova_fp32(const float* mat, unsigned int n, const uint32_t* grouping, unsigned int n_perm,
+                                 float* fstat, float* pvalue) {
+    return permanova_api(mat, n, grouping, n_perm, fstat, pvalue);
+}
 
 IOStatus write_mat_from_matrix_hdf5_fp64_v2(const char* output_filename, mat_full_fp64_t* result,
                                             unsigned int pcoa_dims, int save_dist, unsigned int stat_n_vals,
@@ -978,30 +1146,56 @@ IOStatus write_mat_from_matrix_hdf5_fp32_v2(const char* output_filename, mat_ful
 }
 
 // src/api.cpp:989-1117
+template <class TReal, class TMat>
+ComputeStatus to_file_T(TMat* result, const char* out_filename, bool save_dist, unsigned int pcoa_dims,
+                        unsigned int permanova_perms, const char* grouping_filename, const char* grouping_columns) {
+    const bool fp64 = sizeof(TReal) == sizeof(double);
+    const std::vector<std::string> columns = ssu::split_commas(grouping_columns);
+    const unsigned int k = permanova_perms > 0 ? (unsigned int)columns.size() : 0;
+    std::vector<const char*> cols;
+    for (const auto& c : columns) cols.push_back(c.c_str());
+    std::vector<TReal> f(k), p(k);
+    std::vector<unsigned int> g(k);
+    if (k) {
+        const ComputeStatus st = permanova_columns<TReal>(grouping_filename, k, cols.data(), result, permanova_perms,
+                                                          f.data(), p.data(), g.data());
+        if (st != okay) return st;
+    }
+    std::vector<const char*> meth(k, "PERMANOVA"), test(k, "pseudo-F");
+    std::vector<unsigned int> perms(k, permanova_perms);
+    const IOStatus io = write_full_hdf5_stats<TMat, TReal>(out_filename, result, pcoa_dims, save_dist, fp64, k,
+                                                           meth.data(), test.data(), f.data(), p.data(), perms.data(),
+                                                           cols.data(), g.data());
+    return io == write_okay ? okay : output_error;
+}
+
 ComputeStatus unifrac_to_file_v3(const char* biom_filename, const char* tree_filename, const char* out_filename,
                                  const char* unifrac_method, bool variance_adjust, double alpha, bool bypass_tips,
                                  bool normalize_sample_counts, unsigned int n_substeps, const char* format,
                                  unsigned int subsample_depth, bool subsample_with_replacement, unsigned int pcoa_dims,
-                                 unsigned int permanova_perms, const char*, const char*, const char* mmap_dir) {
+                                 unsigned int permanova_perms, const char* grouping_filename,
+                                 const char* grouping_columns, const char* mmap_dir) {
     bool fp64 = false, save_dist = true;
     ComputeStatus rc = format_precision(unifrac_method, format, fp64, save_dist);
     if (rc != okay) return rc;
-    if (refuse_stats(pcoa_dims, permanova_perms)) return output_error;
+    if (missing_grouping(permanova_perms, grouping_filename, grouping_columns)) return grouping_missing;
     if (fp64) {
         mat_full_fp64_t* result = nullptr;
         rc = one_off_matrix_v3(biom_filename, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips,
                                normalize_sample_counts, n_substeps, subsample_depth, subsample_with_replacement,
                                mmap_dir, &result);
-        if (rc == okay && write_mat_from_matrix_hdf5_fp64(out_filename, result, 0, save_dist) != write_okay)
-            rc = output_error;
+        if (rc == okay)
+            rc = to_file_T<double>(result, out_filename, save_dist, pcoa_dims, permanova_perms, grouping_filename,
+                                   grouping_columns);
         if (result) destroy_mat_full_fp64(&result);
     } else {
         mat_full_fp32_t* result = nullptr;
         rc = one_off_matrix_fp32_v3(biom_filename, tree_filename, unifrac_method, variance_adjust, alpha, bypass_tips,
                                     normalize_sample_counts, n_substeps, subsample_depth, subsample_with_replacement,
                                     mmap_dir, &result);
-        if (rc == okay && write_mat_from_matrix_hdf5_fp32(out_filename, result, 0, save_dist) != write_okay)
-            rc = output_error;
+        if (rc == okay)
+            rc = to_file_T<float>(result, out_filename, save_dist, pcoa_dims, permanova_perms, grouping_filename,
+                                  grouping_columns);
         if (result) destroy_mat_full_fp32(&result);
     }
     return rc;
@@ -1054,17 +1248,20 @@ ComputeStatus unifrac_multi_to_file_v3(const char* biom_filename, const char* tr
                                        bool bypass_tips, bool normalize_sample_counts, unsigned int,
                                        const char* format, unsigned int n_subsamples, unsigned int subsample_depth,
                                        bool subsample_with_replacement, unsigned int pcoa_dims,
-                                       unsigned int permanova_perms, const char*, const char*, const char*) {
+                                       unsigned int permanova_perms, const char* grouping_filename,
+                                       const char* grouping_columns, const char*) {
     bool fp64 = false, save_dist = true;
     const ComputeStatus rc = format_precision(unifrac_method, format, fp64, save_dist);
     if (rc != okay) return rc;
     if (fp64)
         return multi_to_file<double>(save_dist, biom_filename, tree_filename, out_filename, unifrac_method,
                                      variance_adjust, alpha, bypass_tips, normalize_sample_counts, n_subsamples,
-                                     subsample_depth, subsample_with_replacement, pcoa_dims, permanova_perms);
+                                     subsample_depth, subsample_with_replacement, pcoa_dims, permanova_perms,
+                                     grouping_filename, grouping_columns);
     return multi_to_file<float>(save_dist, biom_filename, tree_filename, out_filename, unifrac_method, variance_adjust,
                                 alpha, bypass_tips, normalize_sample_counts, n_subsamples, subsample_depth,
-                                subsample_with_replacement, pcoa_dims, permanova_perms);
+                                subsample_with_replacement, pcoa_dims, permanova_perms, grouping_filename,
+                                grouping_columns);
 }
 
 ComputeStatus unifrac_multi_to_file_v2(const char* biom_filename, const char* tree_filename, const char* out_filename,

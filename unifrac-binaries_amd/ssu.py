@@ -81,7 +81,9 @@ EXPORTS = [
     "merge_partial_to_mmap_matrix_fp32", "write_mat_from_matrix_hdf5_fp64", "write_mat_from_matrix_hdf5_fp32",
     "ssu_faith_pd_inmem", "write_mat_from_matrix_hdf5_fp64_v2", "write_mat_from_matrix_hdf5_fp32_v2",
     "unifrac_to_file_v3", "unifrac_to_txt_file_v3", "unifrac_to_file_v2", "unifrac_to_file",
-    "unifrac_multi_to_file_v3", "unifrac_multi_to_file_v2",
+    "unifrac_multi_to_file_v3", "unifrac_multi_to_file_v2", "compute_permanova_fp64", "compute_permanova_fp32",
+    "ssu_center_matrix_fp64", "ssu_center_matrix_fp32", "ssu_pcoa_fp64", "ssu_pcoa_fp32", "ssu_permanova_fp64",
+    "ssu_permanova_fp32",
     "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
     "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_sparse_terms", "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
     "ssu_newick_to_bp", "ssu_biom_n_samples", "ssu_device_available", "ssu_last_error",
@@ -117,6 +119,10 @@ def lib():
         L.read_bptree_opaque.restype = C.c_int
         for n in ("write_partial", "read_partial", "read_partial_header", "read_partial_one_stripe",
                   "validate_partial", "merge_partial_to_mmap_matrix", "merge_partial_to_mmap_matrix_fp32"):
+            getattr(L, n).restype = C.c_int
+        for n in ("compute_permanova_fp64", "compute_permanova_fp32", "ssu_center_matrix_fp64",
+                  "ssu_center_matrix_fp32", "ssu_pcoa_fp64", "ssu_pcoa_fp32", "ssu_permanova_fp64",
+                  "ssu_permanova_fp32"):
             getattr(L, n).restype = C.c_int
         for n in ("unifrac_to_file_v3", "unifrac_to_txt_file_v3", "unifrac_to_file_v2", "unifrac_to_file",
                   "unifrac_multi_to_file_v3", "unifrac_multi_to_file_v2", "write_mat_from_matrix_hdf5_fp64_v2",
@@ -281,6 +287,50 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def _mat_ptr(m, dtype):
+    a = np.ascontiguousarray(m, dtype=dtype)
+    return a, a.ctypes.data_as(C.c_void_p)
+
+
+def center_matrix(m, fp32=False):
+    """Gower-centred matrix on the device (ssu_center_matrix_fp64 / _fp32)."""
+    dt = np.float32 if fp32 else np.float64
+    a, pa = _mat_ptr(m, dt)
+    out = np.empty_like(a)
+    fn = lib().ssu_center_matrix_fp32 if fp32 else lib().ssu_center_matrix_fp64
+    if fn(pa, C.c_uint(a.shape[0]), out.ctypes.data_as(C.c_void_p)) != OKAY:
+        raise RuntimeError(last_error())
+    return out
+
+
+def pcoa(m, dims, fp32=False):
+    """(eigenvalues, samples n x dims, proportion_explained) by randomized FSVD on the device."""
+    dt = np.float32 if fp32 else np.float64
+    a, pa = _mat_ptr(m, dt)
+    n = a.shape[0]
+    e = np.zeros(dims, dt)
+    s = np.zeros((n, dims), dt)
+    p = np.zeros(dims, dt)
+    fn = lib().ssu_pcoa_fp32 if fp32 else lib().ssu_pcoa_fp64
+    if fn(pa, C.c_uint(n), C.c_uint(dims), e.ctypes.data_as(C.c_void_p), s.ctypes.data_as(C.c_void_p),
+          p.ctypes.data_as(C.c_void_p)) != OKAY:
+        raise RuntimeError(last_error())
+    return e, s, p
+
+
+def permanova(m, grouping, n_perm=999, fp32=False):
+    """(pseudo-F, p-value) on the device."""
+    dt = np.float32 if fp32 else np.float64
+    a, pa = _mat_ptr(m, dt)
+    g = np.ascontiguousarray(grouping, dtype=np.uint32)
+    f = (C.c_float if fp32 else C.c_double)()
+    p = (C.c_float if fp32 else C.c_double)()
+    fn = lib().ssu_permanova_fp32 if fp32 else lib().ssu_permanova_fp64
+    if fn(pa, C.c_uint(a.shape[0]), g.ctypes.data_as(C.c_void_p), C.c_uint(n_perm), C.byref(f), C.byref(p)) != OKAY:
+        raise RuntimeError(last_error())
+    return f.value, p.value
 
 
 class Rarefaction:
