@@ -457,7 +457,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         b.dmask = P.d_dmask.as<uint32_t>();
         b.smask = P.d_smask.as<uint32_t>();
         b.pc = P.d_pc.as<uint32_t>();
-        b.pidx = P.d_pidx.as<uint32_t>();
+        b.pidx = P.d_pidx.p;
         b.pval = P.d_pval.as<float>();
         b.U = P.d_U.as<float>();
         b.ldu = P.ldn;

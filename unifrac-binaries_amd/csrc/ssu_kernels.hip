@@ -952,14 +952,15 @@ __global__ void __launch_bounds__(64) k_node_split(uint32_t* __restrict__ dmask,
 // Posting lists of the sparse nodes: node r's samples in increasing order
 // (pidx) with their FP32 values (pval), at r * cap; pc[p * K + r] = number of
 // node r's samples below 32 p, for p = 0..npan.  One thread per node.
-__global__ void __launch_bounds__(64) k_postings(uint32_t* __restrict__ pidx, float* __restrict__ pval,
+template <class IX>
+__global__ void __launch_bounds__(64) k_postings(IX* __restrict__ pidx, float* __restrict__ pval,
                                                  uint32_t* __restrict__ pc, const uint32_t* __restrict__ smask,
                                                  const uint32_t* __restrict__ dmask, const float* __restrict__ U,
                                                  size_t ldu, uint32_t K, uint32_t npan, uint32_t cap) {
     const uint32_t r = blockIdx.x * 64 + threadIdx.x;
     if (r >= K || ((dmask[r >> 5] >> (r & 31)) & 1u)) return;
     const float* ur = U + (size_t)r * ldu;
-    uint32_t* ix = pidx + (size_t)r * cap;
+    IX* ix = pidx + (size_t)r * cap;
     float* vx = pval + (size_t)r * cap;
     uint32_t cnt = 0;
     for (uint32_t p = 0; p < npan; ++p) {
@@ -967,7 +968,7 @@ __global__ void __launch_bounds__(64) k_postings(uint32_t* __restrict__ pidx, fl
         uint32_t w = smask[(size_t)p * K + r];
         while (w && cnt < cap) {   // cnt < cap holds for every sparse node (fewer than thr = cap samples)
             const uint32_t s = p * 32 + (uint32_t)(__ffs(w) - 1);
-            ix[cnt] = s;
+            ix[cnt] = (IX)s;
             vx[cnt] = ur[s];
             ++cnt;
             w &= w - 1;
@@ -1110,12 +1111,12 @@ constexpr uint32_t SR_WAVES = 8;   // waves sharing one accumulator, each taking
 // FP32 (as in the tiled kernel), accumulated in 2^-(61-e) fixed point where
 // S[a] < 2^e bounds the row's sum; integer adds make the result independent of
 // the order in which waves and lanes add.
-template <class TOut>
+template <class TOut, class IX>
 __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__ out, uint32_t* __restrict__ walked,
                                                    const unsigned long long* __restrict__ rows,
                                                    const uint32_t* __restrict__ offsets,
                                                    const uint32_t* __restrict__ pc,
-                                                   const uint32_t* __restrict__ pidx, const float* __restrict__ pval,
+                                                   const IX* __restrict__ pidx, const float* __restrict__ pval,
                                                    const double* __restrict__ len, const double* __restrict__ S,
                                                    uint32_t K, uint32_t n, uint32_t npan, uint32_t cap,
                                                    uint32_t nchunks, uint32_t nkc) {
@@ -1189,7 +1190,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             float vq[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                bq[u] = pidx[qq[u]];
+                bq[u] = (uint32_t)pidx[qq[u]];
                 vq[u] = pval[qq[u]];
             }
 #pragma unroll
@@ -1416,8 +1417,13 @@ hipError_t launch_sparse_nodes(const SparseNodesArgs& a, hipStream_t st) {
         return hipErrorInvalidValue;
     const uint32_t gk = (a.K + 63) / 64;
     hipLaunchKernelGGL(k_node_split, dim3(gk), dim3(64), 0, st, a.dmask, a.smask, a.K, a.npan, a.cap);
-    hipLaunchKernelGGL(k_postings, dim3(gk), dim3(64), 0, st, a.pidx, a.pval, a.pc, a.smask, a.dmask, a.U, a.ldu, a.K,
-                       a.npan, a.cap);
+    const bool narrow = a.n <= 65536;   // sample indices fit 16 bits: 6 bytes per posting instead of 8
+    if (narrow)
+        hipLaunchKernelGGL(k_postings<uint16_t>, dim3(gk), dim3(64), 0, st, (uint16_t*)a.pidx, a.pval, a.pc, a.smask,
+                           a.dmask, a.U, a.ldu, a.K, a.npan, a.cap);
+    else
+        hipLaunchKernelGGL(k_postings<uint32_t>, dim3(gk), dim3(64), 0, st, (uint32_t*)a.pidx, a.pval, a.pc, a.smask,
+                           a.dmask, a.U, a.ldu, a.K, a.npan, a.cap);
     // row lists: counts per (sample, node chunk), their exclusive scan, then the entries
     const uint32_t nkc = row_list_chunks(a.K);
     const size_t m = (size_t)a.n * nkc;
@@ -1432,12 +1438,15 @@ hipError_t launch_sparse_nodes(const SparseNodesArgs& a, hipStream_t st) {
     const uint32_t nchunks = (a.n + SR_CW - 1) / SR_CW;
     const uint64_t blocks = (uint64_t)a.n * nchunks;
     if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-    if (a.out_fp64)
-        hipLaunchKernelGGL((k_pairs_rows<double>), dim3((uint32_t)blocks), dim3(64 * SR_WAVES), 0, st, (double*)a.out, a.walked,
-                           a.rows, a.roff, a.pc, a.pidx, a.pval, a.len, a.S, a.K, a.n, a.npan, a.cap, nchunks, nkc);
+    if (a.out_fp64) return hipErrorInvalidValue;   // the FP32 shared-node path only
+    if (narrow)
+        hipLaunchKernelGGL((k_pairs_rows<float, uint16_t>), dim3((uint32_t)blocks), dim3(64 * SR_WAVES), 0, st,
+                           (float*)a.out, a.walked, a.rows, a.roff, a.pc, (const uint16_t*)a.pidx, a.pval, a.len, a.S,
+                           a.K, a.n, a.npan, a.cap, nchunks, nkc);
     else
-        hipLaunchKernelGGL((k_pairs_rows<float>), dim3((uint32_t)blocks), dim3(64 * SR_WAVES), 0, st, (float*)a.out, a.walked,
-                           a.rows, a.roff, a.pc, a.pidx, a.pval, a.len, a.S, a.K, a.n, a.npan, a.cap, nchunks, nkc);
+        hipLaunchKernelGGL((k_pairs_rows<float, uint32_t>), dim3((uint32_t)blocks), dim3(64 * SR_WAVES), 0, st,
+                           (float*)a.out, a.walked, a.rows, a.roff, a.pc, (const uint32_t*)a.pidx, a.pval, a.len, a.S,
+                           a.K, a.n, a.npan, a.cap, nchunks, nkc);
     return hipGetLastError();
 }
 

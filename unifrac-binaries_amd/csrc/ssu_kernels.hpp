@@ -122,7 +122,7 @@ struct SparseNodesArgs {
     uint32_t* dmask;
     const uint32_t* smask;  // [npan][K] sample bits, from launch_gather_sums
     uint32_t* pc;           // [npan + 1][K]
-    uint32_t* pidx;         // [K][cap]
+    void* pidx;             // [K][cap] sample indices: u16 when n <= 65,536, else u32
     float* pval;            // [K][cap]
     const float* U;         // [K][ldu]
     size_t ldu;
