@@ -37,7 +37,21 @@ PEAK_FP64_TFLOPS = 78.6     # MI355X published FP64 vector peak
 FLOPS_PER_TERM_SPARSE = 3   # min, fused multiply-add
 PEAK_INT8_TOPS = 5000.0     # MI355X published dense INT8 MFMA peak, TOP/s
 PEAK_HBM_GBS = 8000.0       # MI355X published HBM3E bandwidth, GB/s
-BYTES_PER_POSTING = 8       # sample index + FP32 value read per walked posting
+GRAM_TILE = 128             # Gram output tile edge
+
+
+def bytes_per_posting(name, n):
+    """Bytes a walked posting reads: its sample index (16-bit up to 65,536
+    samples, else 32-bit) and, for the weighted methods, its FP32 value."""
+    return (2 if n <= 65536 else 4) + (0 if name.startswith("unweighted") else 4)
+
+
+def gram_units(n, kernel_terms):
+    """(sample pair i <= j, Gram column) units one k_gram_i8 launch processes:
+    kernel_terms counts the padded upper-triangle tiles times the columns."""
+    t = (n + GRAM_TILE - 1) // GRAM_TILE
+    cols = kernel_terms / (t * (t + 1) // 2 * GRAM_TILE * GRAM_TILE)
+    return n * (n + 1) / 2 * cols
 FLOPS_PER_PAIR_NODE_W = 4   # sub, abs, fused multiply-add
 OPS_PER_PAIR_NODE_GRAM = 8  # 4 int8 digit passes x one multiply-add
 
@@ -350,12 +364,12 @@ def main():
     dom_ms, dom, dom_kernel = max(cands)
     dk = kernels[dom]
     if dom_kernel == "k_gram_i8":
-        achieved = OPS_PER_PAIR_NODE_GRAM * pair_nodes[dom] / (dom_ms / 1000.0) / 1e12
+        achieved = OPS_PER_PAIR_NODE_GRAM * gram_units(n, dk["kernel_terms"]) / (dom_ms / 1000.0) / 1e12
         roofline = {"kernel": dom_kernel, "bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS,
                     "unit": "TOP/s", "frac": achieved / PEAK_INT8_TOPS, "traffic": None}
     elif dom_kernel == "k_pairs_rows":
-        # each walked posting reads its sample index and value (8 bytes)
-        achieved = BYTES_PER_POSTING * dk["sparse_terms"] / (dom_ms / 1000.0) / 1e9
+        # each walked posting reads its sample index (and value)
+        achieved = bytes_per_posting(dom, n) * dk["sparse_terms"] / (dom_ms / 1000.0) / 1e9
         roofline = {"kernel": dom_kernel, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None}
     elif dom_kernel in ("k_pairs_sparse", "k_pairs_sparse32"):
@@ -370,7 +384,7 @@ def main():
                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None}
     for name, k in kernels.items():
         if k["kernel"] == "k_gram_i8":
-            g_ops = OPS_PER_PAIR_NODE_GRAM * pair_nodes[name] / (k["main_ms"] / 1000.0) / 1e12
+            g_ops = OPS_PER_PAIR_NODE_GRAM * gram_units(n, k["kernel_terms"]) / (k["main_ms"] / 1000.0) / 1e12
             k["mfma_int8_tops"] = g_ops
             k["mfma_frac"] = g_ops / PEAK_INT8_TOPS
     prof = os.path.join(ROOT, "profiles", "traffic.json")

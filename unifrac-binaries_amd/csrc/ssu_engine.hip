@@ -26,6 +26,11 @@ namespace {
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
+// Gram path occupancy split: smallest table it is used for, and the device
+// memory its posting lists, row lists, prefix counts and sample bits may take
+constexpr uint32_t GSPLIT_MIN_N = 1024;
+constexpr double GSPLIT_BUDGET = 8.0 * (1u << 30);
+
 struct DevMem {
     void* p = nullptr;
     size_t bytes = 0;
@@ -84,6 +89,13 @@ struct PlanImpl {
     bool gram = false;
     bool sparse = false;             // weighted through the shared-node kernel
     uint32_t split_cap = 0;          // FP32 shared-node path: nodes held by fewer samples use posting lists (0: off)
+    // Gram path occupancy split: sparse nodes (held by few samples of the plan's
+    // table) are summed from posting lists, the Gram keeps the others
+    uint32_t gsplit_cap = 0;         // posting slots per sparse node (0: off)
+    std::vector<int32_t> sp_node;    // the sparse nodes
+    std::vector<double> sp_len;      // their lengths as the accumulation sees them
+    std::vector<uint8_t> gram_keep;  // per processed node: a Gram column (empty: every node)
+    uint64_t nnz = 0;
     std::vector<uint32_t> block_order;  // pair blocks / Gram tiles in launch order
     bool bad_lengths = false;
 
@@ -103,13 +115,14 @@ struct PlanImpl {
     size_t Kp = 0, n_pad = 0;
 
     hipStream_t st = nullptr;
-    hipEvent_t ev[PH_COUNT + 1] = {};
+    hipEvent_t ev[PH_COUNT + 2] = {};  // ev[EV_SPLIT]: posting lists start, ev[EV_ROWS]: Gram path rows end
     float times[PH_COUNT] = {};
     std::vector<double> totals;
 
     DevMem d_indptr, d_indices, d_data, d_totals, d_leaf_k, d_leaf_row, d_level_nodes, d_child_ptr, d_child_idx;
     DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_order, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
-        d_faith, d_stripes, d_smask, d_dmask, d_pc, d_pidx, d_pval, d_walked, d_rows, d_rcount, d_roff, d_rbsum;
+        d_faith, d_stripes, d_smask, d_dmask, d_pc, d_pidx, d_pval, d_walked, d_rows, d_rcount, d_roff, d_rbsum,
+        d_sp_node, d_sp_len, d_umasks;
 
     hipStream_t own_st = nullptr;    // created with the plan; st may point at a caller's stream
     // per-execution inputs and output (nullptr: the plan's own table data / totals / result)
@@ -130,6 +143,7 @@ struct PlanImpl {
     }
 
     bool build_gram_columns();
+    bool split_gram(std::string* err);
     // node-major FP64 embedding of sample columns [c0, c0 + w) into dst (row stride w)
     bool embed(DevMem& dst, bool normalize, size_t c0, size_t w, std::string* err);
 };
@@ -146,7 +160,7 @@ bool PlanImpl::build_gram_columns() {
     std::vector<Item> items;
     for (size_t q = 0; q < knodes.size(); ++q) {
         const double l = spec.fp64 ? klen_d[q] : (double)klen_f[q];
-        if (l == 0.0) continue;
+        if (l == 0.0 || (!gram_keep.empty() && !gram_keep[q])) continue;
         if (!std::isfinite(l)) return false;
         int e = 0;
         const double f = std::frexp(std::fabs(l), &e);        // |l| = f * 2^e, f in [0.5, 1)
@@ -204,6 +218,76 @@ bool PlanImpl::build_gram_columns() {
     digits.assign(4 * std::max<size_t>(Kp, 1), 0);
     for (size_t c = 0; c < Kp; ++c)
         for (int r = 0; r < 4; ++r) digits[(size_t)r * Kp + c] = dg[c][r];
+    return true;
+}
+
+// Gram path occupancy split.  A node's structural occupancy is the number of
+// samples with a stored table entry below it: the embedding of an all-ones
+// copy of the table, counted per node.  Any values on the same structure (the
+// plan's own, or rarefied replicates drawn from it) make the node present in a
+// subset of those samples, so the classification and the posting capacity
+// hold for every execution.  Nodes held by fewer than theta * n samples (and
+// of positive length) leave the Gram for the posting lists; nodes held by no
+// sample leave it altogether.  SSU_GRAM_SPLIT sets theta (0: off),
+// SSU_GRAM_SPLIT_MIN_N the smallest table split (GSPLIT_MIN_N).
+bool PlanImpl::split_gram(std::string* err) {
+    double theta = 0.04;
+    if (const char* env = getenv("SSU_GRAM_SPLIT")) theta = atof(env);
+    uint32_t min_n = GSPLIT_MIN_N;
+    if (const char* env = getenv("SSU_GRAM_SPLIT_MIN_N")) min_n = (uint32_t)atol(env);
+    const uint32_t K = (uint32_t)knodes.size();
+    if (!(theta > 0.0) || K == 0 || n < std::max<uint32_t>(min_n, 2)) return true;
+    DevMem ones, cnt_d;
+    if (!ones.ensure(std::max<uint64_t>(nnz, 1) * sizeof(double), err) || !cnt_d.ensure(K * sizeof(uint32_t), err))
+        return false;
+    SSU_HIP(launch_fill(ones.as<double>(), nnz, 1.0, st));
+    SSU_HIP(hipMemsetAsync(cnt_d.p, 0, K * sizeof(uint32_t), st));
+    x_data = ones.as<double>();
+    for (size_t c0 = 0; c0 < ldn && c0 < n; c0 += blk) {
+        const size_t w = std::min(blk, ldn - c0);
+        if (!embed(d_E, false, c0, w, err)) { x_data = nullptr; return false; }
+        SSU_HIP(launch_count_present(cnt_d.as<uint32_t>(), d_E.as<double>(), w, (uint32_t)c0,
+                                     d_knodes.as<int32_t>(), K, n, st));
+    }
+    x_data = nullptr;
+    std::vector<uint32_t> cnt(K);
+    SSU_HIP(hipMemcpyAsync(cnt.data(), cnt_d.p, K * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    SSU_HIP(hipStreamSynchronize(st));
+
+    const size_t npan = (n + 31) / 32, panels = ldn / 32;
+    auto classify = [&](uint32_t thr) {
+        gram_keep.assign(K, 1);
+        sp_node.clear();
+        sp_len.clear();
+        uint32_t cap = 0;
+        for (uint32_t q = 0; q < K; ++q) {
+            const double l = spec.fp64 ? klen_d[q] : (double)klen_f[q];
+            if (cnt[q] == 0) { gram_keep[q] = 0; continue; }
+            if (cnt[q] < thr && l > 0.0 && std::isfinite(l)) {
+                gram_keep[q] = 0;
+                sp_node.push_back(knodes[q]);
+                sp_len.push_back(l);
+                cap = std::max(cap, cnt[q]);
+            }
+        }
+        return cap;
+    };
+    // postings (4-byte slots) and row lists (8-byte slots), prefix counts and sample bits
+    auto bytes = [&](uint32_t cap) {
+        const double ks = (double)sp_node.size();
+        return ks * cap * 12.0 + ks * (double)(npan + 1 + panels) * 4.0;
+    };
+    uint32_t thr = (uint32_t)std::max(1.0, std::ceil(theta * n));
+    uint32_t cap = classify(thr);
+    while (bytes(cap) > GSPLIT_BUDGET && thr > 1) cap = classify(thr /= 2);
+    if (!build_gram_columns()) return false;
+    gsplit_cap = sp_node.empty() ? 0 : cap;
+    if (gsplit_cap) {
+        const std::vector<uint32_t> zero((sp_node.size() + 31) / 32, 0u);
+        if (!upload(d_sp_node, sp_node, st, err) || !upload(d_sp_len, sp_len, st, err) || !upload(d_dmask, zero, st, err))
+            return false;
+        SSU_HIP(hipStreamSynchronize(st));   // `zero` is read before it goes out of scope
+    }
     return true;
 }
 
@@ -335,6 +419,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
 
     P.totals = table.totals;
     const uint64_t nnz = table.nnz();
+    P.nnz = nnz;
     const uint32_t base = table.n_obs ? table.indptr[0] : 0;
     std::vector<uint32_t> ip(table.indptr, table.indptr + table.n_obs + 1);
     for (auto& x : ip) x -= base;
@@ -362,6 +447,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         P.block_order = upper_block_order((uint32_t)(P.n_pad / GRAM_BM), 6);
         ok = upload(P.d_order, P.block_order, P.st, err);
     }
+    if (ok && P.gram) ok = P.split_gram(err);   // may narrow the Gram columns
     if (ok && P.gram && P.Kp)
         ok = upload(P.d_col, P.col_node, P.st, err) && upload(P.d_digits, P.digits, P.st, err) &&
              upload(P.d_chunks, P.chunks, P.st, err);
@@ -424,6 +510,24 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             !P.d_rbsum.ensure((row_list_scan_blocks(P.n, K) + 1) * sizeof(uint32_t), err))
             return false;
     }
+    // Gram path split: sample bits of the sparse nodes from each embedding block,
+    // posting and row lists before the Gram, each row's share added to G after it
+    const uint32_t Ks = (uint32_t)P.sp_node.size();
+    const bool gsplit = P.gram && P.gsplit_cap > 0 && Ks > 0;
+    if (gsplit) {
+        const size_t ks = Ks, kws = (ks + 31) / 32, panels = P.ldn / 32;
+        if (!P.d_umasks.ensure(panels * kws * sizeof(uint32_t), err) ||
+            !P.d_smask.ensure(panels * ks * sizeof(uint32_t), err) ||
+            !P.d_pc.ensure((size_t)(npan + 1) * ks * sizeof(uint32_t), err) ||
+            !P.d_pidx.ensure(ks * P.gsplit_cap * sizeof(uint32_t), err) ||
+            !P.d_walked.ensure((size_t)sparse_rows_blocks(P.n) * sizeof(uint32_t), err) ||
+            !P.d_rows.ensure(ks * P.gsplit_cap * sizeof(unsigned long long), err) ||
+            !P.d_rcount.ensure((size_t)P.n * row_list_chunks(Ks) * sizeof(uint32_t), err) ||
+            !P.d_roff.ensure(((size_t)P.n * row_list_chunks(Ks) + 1) * sizeof(uint32_t), err) ||
+            !P.d_rbsum.ensure((row_list_scan_blocks(P.n, Ks) + 1) * sizeof(uint32_t), err))
+            return false;
+    }
+    if (P.gram && !P.d_G.ensure(P.n_pad * P.n_pad * sizeof(double), err)) return false;
     if (gather) {
         if (!P.d_U.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
         if (s.vaw && !P.d_Cu.ensure(std::max<size_t>(K, 1) * P.ldn * sizeof(float), err)) return false;
@@ -440,6 +544,10 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             if (P.Kp)
                 SSU_HIP(launch_presence_T(P.d_Pt.as<int8_t>(), P.Kp, P.d_E.as<double>(), w, (uint32_t)c0,
                                           P.d_col.as<int32_t>(), (uint32_t)P.Kp, P.n, P.st));
+            if (gsplit)
+                SSU_HIP(launch_gather_sums(nullptr, 0, nullptr, nullptr, P.d_umasks.as<uint32_t>(),
+                                           P.d_smask.as<uint32_t>(), P.d_E.as<double>(), w, (uint32_t)c0,
+                                           P.d_sp_node.as<int32_t>(), nullptr, Ks, P.n, P.st));
         } else if (gather || sums) {
             SSU_HIP(launch_gather_sums(gather ? P.d_U.as<float>() : nullptr, P.ldn, sums ? P.d_S.as<double>() : nullptr,
                                        P.d_Spart.as<double>(), P.sparse ? P.d_masks.as<uint32_t>() : nullptr,
@@ -452,7 +560,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
     }
     if (!single) SSU_HIP(P.mark(1));   // blocked: embed and prep share one phase
     if (split) {   // posting lists, row lists and each row's sparse-node share, into the output
-        SSU_HIP(P.mark(PH_COUNT));
+        SSU_HIP(P.mark(EV_SPLIT));
         SparseNodesArgs b{};
         b.dmask = P.d_dmask.as<uint32_t>();
         b.smask = P.d_smask.as<uint32_t>();
@@ -476,10 +584,30 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         b.cap = P.split_cap;
         SSU_HIP(launch_sparse_nodes(b, P.st));
     }
+    SparseNodesArgs gb{};
+    if (gsplit) {   // posting lists and row lists of the Gram path's sparse nodes
+        SSU_HIP(P.mark(EV_SPLIT));
+        gb.dmask = P.d_dmask.as<uint32_t>();
+        gb.smask = P.d_smask.as<uint32_t>();
+        gb.pc = P.d_pc.as<uint32_t>();
+        gb.pidx = P.d_pidx.p;
+        gb.len = P.d_sp_len.as<double>();
+        gb.walked = P.d_walked.as<uint32_t>();
+        gb.rows = P.d_rows.as<unsigned long long>();
+        gb.rcount = P.d_rcount.as<uint32_t>();
+        gb.roff = P.d_roff.as<uint32_t>();
+        gb.rbsum = P.d_rbsum.as<uint32_t>();
+        gb.K = Ks;
+        gb.n = P.n;
+        gb.npan = npan;
+        gb.cap = P.gsplit_cap;
+        gb.gram = P.d_G.as<double>();
+        gb.n_pad = (uint32_t)P.n_pad;
+        SSU_HIP(launch_sparse_lists(gb, P.st));
+    }
     SSU_HIP(P.mark(2));
 
     if (P.gram) {
-        if (!P.d_G.ensure(P.n_pad * P.n_pad * sizeof(double), err)) return false;
         if (P.Kp) {
             SSU_HIP(launch_gram_i8(P.d_G.as<double>(), P.d_Pt.as<int8_t>(), P.Kp, P.d_digits.as<int8_t>(),
                                    P.d_chunks.as<GramChunk>(), (uint32_t)P.chunks.size(), (uint32_t)P.n_pad,
@@ -488,6 +616,10 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             SSU_HIP(launch_fill(P.d_G.as<double>(), P.n_pad * P.n_pad, 0.0, P.st));   // no memset node in captures
         }
         SSU_HIP(P.mark(3));
+        if (gsplit) {
+            SSU_HIP(launch_sparse_rows(gb, P.st));   // adds the sparse nodes' share to G
+            SSU_HIP(P.mark(EV_ROWS));
+        }
         const bool norm = s.method == Method::unweighted;
         if (s.fp64)
             SSU_HIP(launch_unweighted_combine<double>((double*)P.out_ptr(), P.d_G.as<double>(), P.n_pad, P.n, norm,
@@ -550,9 +682,15 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
     for (int i = 0; i < PH_TOTAL; ++i) SSU_HIP(hipEventElapsedTime(&P.times[i], P.ev[i], P.ev[i + 1]));
     SSU_HIP(hipEventElapsedTime(&P.times[PH_TOTAL], P.ev[0], P.ev[5]));
     P.times[PH_SPARSE] = 0.0f;
-    if (split) {   // prep ends where the posting-list phase starts
-        SSU_HIP(hipEventElapsedTime(&P.times[PH_PREP], P.ev[1], P.ev[PH_COUNT]));
-        SSU_HIP(hipEventElapsedTime(&P.times[PH_SPARSE], P.ev[PH_COUNT], P.ev[2]));
+    if (split || gsplit) {   // prep ends where the posting-list phase starts
+        SSU_HIP(hipEventElapsedTime(&P.times[PH_PREP], P.ev[1], P.ev[EV_SPLIT]));
+        SSU_HIP(hipEventElapsedTime(&P.times[PH_SPARSE], P.ev[EV_SPLIT], P.ev[2]));
+    }
+    if (gsplit) {   // the rows kernel runs between the Gram and the combine
+        float rows_ms = 0.0f;
+        SSU_HIP(hipEventElapsedTime(&rows_ms, P.ev[3], P.ev[EV_ROWS]));
+        SSU_HIP(hipEventElapsedTime(&P.times[PH_COMBINE], P.ev[EV_ROWS], P.ev[4]));
+        P.times[PH_SPARSE] += rows_ms;
     }
     return true;
 }
@@ -630,7 +768,7 @@ uint64_t Plan::kernel_terms() const {
 }
 uint64_t Plan::sparse_terms() const {
     const PlanImpl& P = *p_;
-    if (!P.sparse || !P.split_cap || !P.d_walked.p) return 0;
+    if (!((P.sparse && P.split_cap) || (P.gram && P.gsplit_cap)) || !P.d_walked.p) return 0;
     std::vector<uint32_t> wk(sparse_rows_blocks(P.n));
     if (hipMemcpy(wk.data(), P.d_walked.p, wk.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) return 0;
     uint64_t sum = 0;

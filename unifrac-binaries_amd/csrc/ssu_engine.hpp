@@ -30,10 +30,11 @@ struct ComputeSpec {
     int plain_kernel = 0;     // 0: MFMA Gram (unweighted) / shared-node kernel (weighted); 1: dense pair tiles
 };
 
-// Phases timed with HIP events on the plan's stream during execute().
-// Phases of one execution; PH_SPARSE is the posting-list share of the FP32
-// weighted path (zero elsewhere), PH_MAIN the main kernel alone.
+// Phases of one execution, timed with HIP events on the plan's stream during
+// execute(); PH_SPARSE is the posting-list share of an occupancy split (zero
+// without one), PH_MAIN the main kernel alone.
 enum Phase : int { PH_EMBED = 0, PH_PREP, PH_MAIN, PH_COMBINE, PH_TOTAL, PH_SPARSE, PH_COUNT };
+constexpr int EV_SPLIT = PH_COUNT, EV_ROWS = PH_COUNT + 1;   // extra events: posting lists start, rows end
 
 struct PlanImpl;
 

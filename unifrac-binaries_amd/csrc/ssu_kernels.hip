@@ -950,8 +950,9 @@ __global__ void __launch_bounds__(64) k_node_split(uint32_t* __restrict__ dmask,
 }
 
 // Posting lists of the sparse nodes: node r's samples in increasing order
-// (pidx) with their FP32 values (pval), at r * cap; pc[p * K + r] = number of
-// node r's samples below 32 p, for p = 0..npan.  One thread per node.
+// (pidx) with their FP32 values (pval, when non-null), at r * cap;
+// pc[p * K + r] = number of node r's samples below 32 p, for p = 0..npan.
+// One thread per node.
 template <class IX>
 __global__ void __launch_bounds__(64) k_postings(IX* __restrict__ pidx, float* __restrict__ pval,
                                                  uint32_t* __restrict__ pc, const uint32_t* __restrict__ smask,
@@ -959,9 +960,9 @@ __global__ void __launch_bounds__(64) k_postings(IX* __restrict__ pidx, float* _
                                                  size_t ldu, uint32_t K, uint32_t npan, uint32_t cap) {
     const uint32_t r = blockIdx.x * 64 + threadIdx.x;
     if (r >= K || ((dmask[r >> 5] >> (r & 31)) & 1u)) return;
-    const float* ur = U + (size_t)r * ldu;
+    const float* ur = U ? U + (size_t)r * ldu : nullptr;
     IX* ix = pidx + (size_t)r * cap;
-    float* vx = pval + (size_t)r * cap;
+    float* vx = pval ? pval + (size_t)r * cap : nullptr;
     uint32_t cnt = 0;
     for (uint32_t p = 0; p < npan; ++p) {
         pc[(size_t)p * K + r] = cnt;
@@ -969,7 +970,7 @@ __global__ void __launch_bounds__(64) k_postings(IX* __restrict__ pidx, float* _
         while (w && cnt < cap) {   // cnt < cap holds for every sparse node (fewer than thr = cap samples)
             const uint32_t s = p * 32 + (uint32_t)(__ffs(w) - 1);
             ix[cnt] = (IX)s;
-            vx[cnt] = ur[s];
+            if (vx) vx[cnt] = ur[s];
             ++cnt;
             w &= w - 1;
         }
@@ -1102,16 +1103,21 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(uint32_t* __restrict__ ou
 constexpr uint32_t SR_CW = 4096;   // output columns per workgroup (64-bit LDS accumulators, 32 KB)
 constexpr uint32_t SR_WAVES = 8;   // waves sharing one accumulator, each taking every 8th round of entries
 
-// Sparse-node share of M for row a: sum over sparse nodes r held by both a and
-// b > a of l_r * min(u_ra, u_rb), written as FP32 into out[a * n + b] for the
-// tiled kernel's epilogue.  One workgroup of 8 waves per (row a, 4096-column
-// chunk c).  Row a's list is taken 64 entries at a time (round q goes to wave
-// q % 8); a round's posting ranges inside the chunk (past a) are walked with
-// the wave's 64 lanes spread evenly over the concatenated postings.  Terms are
-// FP32 (as in the tiled kernel), accumulated in 2^-(61-e) fixed point where
-// S[a] < 2^e bounds the row's sum; integer adds make the result independent of
-// the order in which waves and lanes add.
-template <class TOut, class IX>
+// Sparse-node share for row a.  Weighted: sum over sparse nodes r held by
+// both a and b > a of l_r * min(u_ra, u_rb), written as FP32 into
+// out[a * n + b] for the tiled kernel's epilogue.  GRAM (unweighted): sum of
+// l_r over the sparse nodes a and b >= a share, added to the FP64 Gram entry
+// G(a, b) (the diagonal by the chunk that holds column a).  One workgroup of 8
+// waves per (row a, 4096-column chunk c).  Row a's list is taken 64 entries at
+// a time (round q goes to wave q % 8); a round's posting ranges inside the
+// chunk (past a) are walked with the wave's 64 lanes spread evenly over the
+// concatenated postings.  Weighted terms are FP32 (as in the tiled kernel);
+// terms are accumulated in 2^-(61-e) fixed point, where 2^e bounds the row's
+// sum (S[a] weighted; the FP64 sum of the row list's lengths unweighted), so
+// integer adds make the result independent of the order in which waves and
+// lanes add, and two samples holding the same nodes get bit-equal G(a, b)
+// and G(a, a).
+template <class TOut, class IX, bool GRAM>
 __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__ out, uint32_t* __restrict__ walked,
                                                    const unsigned long long* __restrict__ rows,
                                                    const uint32_t* __restrict__ offsets,
@@ -1119,10 +1125,12 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                                                    const IX* __restrict__ pidx, const float* __restrict__ pval,
                                                    const double* __restrict__ len, const double* __restrict__ S,
                                                    uint32_t K, uint32_t n, uint32_t npan, uint32_t cap,
-                                                   uint32_t nchunks, uint32_t nkc) {
+                                                   uint32_t nchunks, uint32_t nkc, double* __restrict__ G,
+                                                   uint32_t tiles) {
     const uint32_t a = blockIdx.x / nchunks, c = blockIdx.x % nchunks;
     const uint32_t b0 = c * SR_CW, b1 = min(n, b0 + SR_CW);
-    if (a >= n || b1 <= a + 1) {   // uniform: no column above the diagonal here
+    // uniform: nothing at or above the diagonal here (weighted: strictly above)
+    if (a >= n || (GRAM ? b1 <= a : b1 <= a + 1)) {
         if (walked && threadIdx.x == 0) walked[blockIdx.x] = 0;
         return;
     }
@@ -1130,22 +1138,49 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     __shared__ unsigned long long acc[SR_CW];
     __shared__ uint32_t pre_s[SR_WAVES][65], lst_s[SR_WAVES][64], lnode_s[SR_WAVES][64], wk[SR_WAVES];
     __shared__ float lu_s[SR_WAVES][64], ll_s[SR_WAVES][64];
+    __shared__ unsigned long long lfx_s[SR_WAVES][GRAM ? 64 : 1], diag_s;
+    __shared__ double bsum_s[SR_WAVES];
     uint32_t* pre = pre_s[wv];
     uint32_t* lst = lst_s[wv];
     uint32_t* lnode = lnode_s[wv];
     float* lu = lu_s[wv];
     float* ll = ll_s[wv];
+    unsigned long long* lfx = lfx_s[wv];
     for (uint32_t q = threadIdx.x; q < SR_CW; q += 64 * SR_WAVES) acc[q] = 0ull;
+    const uint32_t e0 = offsets[(size_t)a * nkc], e1 = offsets[(size_t)(a + 1) * nkc];
+    double bound;
+    if constexpr (GRAM) {
+        // FP64 sum of the row list's lengths, in a fixed order: per thread, then a
+        // butterfly per wave, then the waves in order
+        double x = 0.0;
+        for (uint32_t e = e0 + threadIdx.x; e < e1; e += 64 * SR_WAVES) x += len[(uint32_t)(rows[e] >> 32)];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+        if (lane == 0) bsum_s[wv] = x;
+        if (threadIdx.x == 0) diag_s = 0ull;
+        __syncthreads();
+        bound = 0.0;
+        for (uint32_t w = 0; w < SR_WAVES; ++w) bound += bsum_s[w];
+    } else {
+        bound = S[a];
+    }
     __syncthreads();
     int e2 = 0;
-    (void)frexp(S[a], &e2);   // S[a] < 2^e2 bounds every partial sum of the row
+    (void)frexp(bound, &e2);   // bound < 2^e2 bounds every partial sum of the row
     const double scale = ldexp(1.0, 61 - e2), inv = ldexp(1.0, e2 - 61);
+    const bool has_diag = GRAM && b0 <= a && a < b1;
+    if (GRAM && has_diag) {   // G(a, a): the same fixed-point terms, every one of them
+        unsigned long long x = 0ull;
+        for (uint32_t e = e0 + threadIdx.x; e < e1; e += 64 * SR_WAVES)
+            x += to_fixed(len[(uint32_t)(rows[e] >> 32)] * scale);
+        if (x) atomicAdd(&diag_s, x);
+    }
     const uint32_t pb0 = b0 >> 5, pb1 = b1 == n ? npan : (b1 >> 5);
-    const uint32_t e0 = offsets[(size_t)a * nkc], e1 = offsets[(size_t)(a + 1) * nkc];
     uint32_t total_walked = 0;
     for (uint32_t r0 = e0 + 64 * wv; r0 < e1; r0 += 64 * SR_WAVES) {   // uniform per wave
         uint32_t start = 0, end = 0, r = 0;
         float ua = 0.0f, l = 0.0f;
+        unsigned long long lf = 0ull;
         if (r0 + lane < e1) {
             const unsigned long long ent = rows[r0 + lane];
             r = (uint32_t)(ent >> 32);
@@ -1153,8 +1188,12 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             const uint32_t cs = pc[(size_t)pb0 * K + r], ce = pc[(size_t)pb1 * K + r];
             start = max(pa + 1, cs);
             end = ce;
-            ua = pval[(size_t)r * cap + pa];
-            l = (float)len[r];
+            if constexpr (GRAM) {
+                lf = to_fixed(len[r] * scale);
+            } else {
+                ua = pval[(size_t)r * cap + pa];
+                l = (float)len[r];
+            }
         }
         const uint32_t cnt = end > start ? end - start : 0u;
         uint32_t incl = cnt;
@@ -1169,8 +1208,12 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         if (lane == 63) pre[64] = incl;
         lst[lane] = start;
         lnode[lane] = r;
-        lu[lane] = ua;
-        ll[lane] = l;
+        if constexpr (GRAM) {
+            lfx[lane] = lf;
+        } else {
+            lu[lane] = ua;
+            ll[lane] = l;
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1191,12 +1234,16 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 bq[u] = (uint32_t)pidx[qq[u]];
-                vq[u] = pval[qq[u]];
+                if constexpr (!GRAM) vq[u] = pval[qq[u]];
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (t0 + 64 * u < T)
-                    atomicAdd(&acc[bq[u] - b0], to_fixed((double)(ll[eq[u]] * fminf(lu[eq[u]], vq[u])) * scale));
+                if (t0 + 64 * u < T) {
+                    if constexpr (GRAM)
+                        atomicAdd(&acc[bq[u] - b0], lfx[eq[u]]);
+                    else
+                        atomicAdd(&acc[bq[u] - b0], to_fixed((double)(ll[eq[u]] * fminf(lu[eq[u]], vq[u])) * scale));
+                }
         }
         total_walked += T;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1210,8 +1257,32 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         for (uint32_t w = 0; w < SR_WAVES; ++w) sum += wk[w];
         walked[blockIdx.x] = sum;
     }
-    for (uint32_t b = max(a + 1, b0) + threadIdx.x; b < b1; b += 64 * SR_WAVES)
-        out[(size_t)a * n + b] = (TOut)((double)acc[b - b0] * inv);
+    if constexpr (GRAM) {
+        if (has_diag && threadIdx.x == 0 && diag_s) G[gram_slot(a, a, tiles)] += (double)diag_s * inv;
+        for (uint32_t b = max(a + 1, b0) + threadIdx.x; b < b1; b += 64 * SR_WAVES) {
+            const unsigned long long v = acc[b - b0];
+            if (v) G[gram_slot(a, b, tiles)] += (double)v * inv;
+        }
+    } else {
+        for (uint32_t b = max(a + 1, b0) + threadIdx.x; b < b1; b += 64 * SR_WAVES)
+            out[(size_t)a * n + b] = (TOut)((double)acc[b - b0] * inv);
+    }
+}
+
+// cnt[r] += samples of the block with E[knodes[r]][s] > 0: one workgroup per node
+__global__ void __launch_bounds__(256) k_count_present(uint32_t* __restrict__ cnt, const double* __restrict__ E,
+                                                       size_t lde, const int32_t* __restrict__ knodes,
+                                                       uint32_t n_valid) {
+    __shared__ uint32_t part[4];
+    const uint32_t r = blockIdx.x;
+    const double* row = E + (size_t)knodes[r] * lde;
+    uint32_t c = 0;
+    for (uint32_t s = threadIdx.x; s < n_valid; s += 256) c += row[s] > 0.0 ? 1u : 0u;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[r] += part[0] + part[1] + part[2] + part[3];
 }
 
 template <class TIn>
@@ -1325,6 +1396,7 @@ hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, ui
     case 4: gather_sums_t<true, false, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
     case 3: gather_sums_t<false, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
     case 2: gather_sums_t<false, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
+    case 1: gather_sums_t<false, false, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
     case 0: return hipSuccess;
     default: return hipErrorInvalidValue;
     }
@@ -1409,21 +1481,29 @@ static void pairs_sparse32_t(uint32_t grid, hipStream_t st, const SparseArgs& a,
                        a.dmask, (int)(a.dmask != nullptr));
 }
 
-hipError_t launch_sparse_nodes(const SparseNodesArgs& a, hipStream_t st) {
+static bool sparse_args_ok(const SparseNodesArgs& a) {
+    const bool common = a.dmask && a.smask && a.pc && a.pidx && a.len && a.cap && a.rows && a.rcount && a.roff &&
+                        a.rbsum && a.npan == (a.n + 31) / 32;
+    if (!common) return false;
+    if (a.gram) return a.n_pad >= a.n && a.n_pad % GRAM_BM == 0;   // unweighted: values, sums and out unused
+    return a.pval && a.U && a.S && a.out && !a.out_fp64 && a.ldu >= (size_t)a.npan * 32;
+}
+
+hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
     if (!a.K || !a.n) return hipSuccess;
-    if (!a.dmask || !a.smask || !a.pc || !a.pidx || !a.pval || !a.U || !a.len || !a.S || !a.out || !a.cap ||
-        !a.rows || !a.rcount || !a.roff || !a.rbsum ||
-        a.npan != (a.n + 31) / 32 || a.ldu < (size_t)a.npan * 32)
-        return hipErrorInvalidValue;
+    if (!sparse_args_ok(a)) return hipErrorInvalidValue;
     const uint32_t gk = (a.K + 63) / 64;
-    hipLaunchKernelGGL(k_node_split, dim3(gk), dim3(64), 0, st, a.dmask, a.smask, a.K, a.npan, a.cap);
+    // weighted: dense nodes by occupancy, now; unweighted: all K nodes are sparse (dmask zero since plan creation)
+    if (!a.gram) hipLaunchKernelGGL(k_node_split, dim3(gk), dim3(64), 0, st, a.dmask, a.smask, a.K, a.npan, a.cap);
     const bool narrow = a.n <= 65536;   // sample indices fit 16 bits: 6 bytes per posting instead of 8
+    const float* U = a.gram ? nullptr : a.U;
+    float* pval = a.gram ? nullptr : a.pval;
     if (narrow)
-        hipLaunchKernelGGL(k_postings<uint16_t>, dim3(gk), dim3(64), 0, st, (uint16_t*)a.pidx, a.pval, a.pc, a.smask,
-                           a.dmask, a.U, a.ldu, a.K, a.npan, a.cap);
+        hipLaunchKernelGGL(k_postings<uint16_t>, dim3(gk), dim3(64), 0, st, (uint16_t*)a.pidx, pval, a.pc, a.smask,
+                           a.dmask, U, a.ldu, a.K, a.npan, a.cap);
     else
-        hipLaunchKernelGGL(k_postings<uint32_t>, dim3(gk), dim3(64), 0, st, (uint32_t*)a.pidx, a.pval, a.pc, a.smask,
-                           a.dmask, a.U, a.ldu, a.K, a.npan, a.cap);
+        hipLaunchKernelGGL(k_postings<uint32_t>, dim3(gk), dim3(64), 0, st, (uint32_t*)a.pidx, pval, a.pc, a.smask,
+                           a.dmask, U, a.ldu, a.K, a.npan, a.cap);
     // row lists: counts per (sample, node chunk), their exclusive scan, then the entries
     const uint32_t nkc = row_list_chunks(a.K);
     const size_t m = (size_t)a.n * nkc;
@@ -1435,18 +1515,46 @@ hipError_t launch_sparse_nodes(const SparseNodesArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_T), 0, st, a.roff, a.rcount, a.rbsum, m, nsb);
     hipLaunchKernelGGL(k_row_lists, dim3(a.npan, nkc), dim3(64), 0, st, a.rows, nullptr, a.roff, a.smask, a.dmask,
                        a.pc, a.K, a.n, nkc);
+    return hipGetLastError();
+}
+
+template <class IX, bool GRAM>
+static void pairs_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs& a, uint32_t nchunks, uint32_t nkc) {
+    hipLaunchKernelGGL((k_pairs_rows<float, IX, GRAM>), dim3(blocks), dim3(64 * SR_WAVES), 0, st,
+                       GRAM ? nullptr : (float*)a.out, a.walked, a.rows, a.roff, a.pc, (const IX*)a.pidx,
+                       GRAM ? nullptr : a.pval, a.len, GRAM ? nullptr : a.S, a.K, a.n, a.npan, a.cap, nchunks, nkc,
+                       a.gram, a.n_pad / GRAM_BM);
+}
+
+hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st) {
+    if (!a.K || !a.n) return hipSuccess;
+    if (!sparse_args_ok(a)) return hipErrorInvalidValue;
+    const uint32_t nkc = row_list_chunks(a.K);
     const uint32_t nchunks = (a.n + SR_CW - 1) / SR_CW;
     const uint64_t blocks = (uint64_t)a.n * nchunks;
     if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-    if (a.out_fp64) return hipErrorInvalidValue;   // the FP32 shared-node path only
-    if (narrow)
-        hipLaunchKernelGGL((k_pairs_rows<float, uint16_t>), dim3((uint32_t)blocks), dim3(64 * SR_WAVES), 0, st,
-                           (float*)a.out, a.walked, a.rows, a.roff, a.pc, (const uint16_t*)a.pidx, a.pval, a.len, a.S,
-                           a.K, a.n, a.npan, a.cap, nchunks, nkc);
-    else
-        hipLaunchKernelGGL((k_pairs_rows<float, uint32_t>), dim3((uint32_t)blocks), dim3(64 * SR_WAVES), 0, st,
-                           (float*)a.out, a.walked, a.rows, a.roff, a.pc, (const uint32_t*)a.pidx, a.pval, a.len, a.S,
-                           a.K, a.n, a.npan, a.cap, nchunks, nkc);
+    const bool narrow = a.n <= 65536;
+    if (a.gram) {
+        if (narrow) pairs_rows_t<uint16_t, true>((uint32_t)blocks, st, a, nchunks, nkc);
+        else pairs_rows_t<uint32_t, true>((uint32_t)blocks, st, a, nchunks, nkc);
+    } else {
+        if (narrow) pairs_rows_t<uint16_t, false>((uint32_t)blocks, st, a, nchunks, nkc);
+        else pairs_rows_t<uint32_t, false>((uint32_t)blocks, st, a, nchunks, nkc);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sparse_nodes(const SparseNodesArgs& a, hipStream_t st) {
+    if (a.gram) return hipErrorInvalidValue;   // the Gram tiles must be written between the two halves
+    const hipError_t e = launch_sparse_lists(a, st);
+    return e != hipSuccess ? e : launch_sparse_rows(a, st);
+}
+
+hipError_t launch_count_present(uint32_t* cnt, const double* E, size_t lde, uint32_t col0, const int32_t* knodes,
+                                uint32_t K, uint32_t n, hipStream_t st) {
+    if (!K || col0 >= n) return hipSuccess;
+    const uint32_t n_valid = (uint32_t)std::min<size_t>(lde, n - col0);
+    hipLaunchKernelGGL(k_count_present, dim3(K), dim3(256), 0, st, cnt, E, lde, knodes, n_valid);
     return hipGetLastError();
 }
 

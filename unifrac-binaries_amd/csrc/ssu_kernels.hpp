@@ -42,7 +42,8 @@ hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint3
 //     see it: FP32-rounded when U is written), summed per segment then over
 //     segments in order, into S[col0 ..]; Spart holds gather_segments(K, lde) * lde doubles;
 //   masks (if non-null; lde, col0 multiples of 64): [ldn/32][ceil(K/32)] words, bit r%32 of
-//     word r/32 of panel p set iff some sample of p*32..p*32+31 has u_rs > 0.
+//     word r/32 of panel p set iff some sample of p*32..p*32+31 has u_rs > 0;
+//   smask (if non-null, with masks): [ldn/32][K] words, bit s%32 of smask[s/32][r] = u_rs > 0.
 uint32_t gather_segments(uint32_t K, size_t ldn);
 hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, uint32_t* masks, uint32_t* smask,
                               const double* E,
@@ -113,22 +114,26 @@ struct SparseArgs {
     const uint32_t* dmask = nullptr;   // FP32 kernel: visit only these nodes and add the partial sums in `out`
 };
 
-// Occupancy split of the FP32 shared-node path.  Nodes held by at least `cap`
-// samples are dense (bit set in dmask, [ceil(K/32)] words); for the others
-// the posting lists are built from U and the sample masks, and each row's
-// share of M over them is written to out[a * n + b] (b > a) for the tiled
-// kernel to add (SparseArgs::dmask).
+// Occupancy split.  Weighted (FP32 shared-node path): nodes held by at least
+// `cap` samples are dense (bit set in dmask, [ceil(K/32)] words, set here by
+// launch_sparse_lists); for the others the posting lists are built from U and
+// the sample masks, and each row's share of M over them is written to
+// out[a * n + b] (b > a) for the tiled kernel to add (SparseArgs::dmask).
+// Unweighted (gram != nullptr): the K nodes are the plan's sparse nodes, fixed
+// when the plan is built (dmask all zero, U and pval unused); each row's share
+// of the shared length sum_r len[r] [a holds r][b holds r] is added to the
+// FP64 Gram tile entries G(a, b), b >= a, after launch_gram_i8 has written them.
 struct SparseNodesArgs {
     uint32_t* dmask;
     const uint32_t* smask;  // [npan][K] sample bits, from launch_gather_sums
     uint32_t* pc;           // [npan + 1][K]
     void* pidx;             // [K][cap] sample indices: u16 when n <= 65,536, else u32
-    float* pval;            // [K][cap]
-    const float* U;         // [K][ldu]
+    float* pval;            // [K][cap] (weighted)
+    const float* U;         // [K][ldu] (weighted)
     size_t ldu;
     const double* len;      // [K]
-    const double* S;        // [n]
-    void* out;
+    const double* S;        // [n] (weighted)
+    void* out;              // (weighted)
     bool out_fp64;
     uint32_t* walked;       // optional [sparse_rows_blocks(n)]: postings walked per wave
     unsigned long long* rows;   // [K * cap] row-list entries (node << 32 | position in its postings)
@@ -136,8 +141,19 @@ struct SparseNodesArgs {
     uint32_t* roff;         // [n * row_list_chunks(K) + 1]
     uint32_t* rbsum;        // [row_list_scan_blocks(n, K) + 1]
     uint32_t K, n, npan, cap;
+    double* gram = nullptr; // unweighted: the Gram tiles of launch_gram_i8
+    uint32_t n_pad = 0;     // their padded edge
 };
+// posting lists, per-panel prefix counts and row lists
+hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st);
+// each row's share over the posting lists (after launch_sparse_lists; in the
+// unweighted case after launch_gram_i8 as well)
+hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st);
+// both, in order (weighted)
 hipError_t launch_sparse_nodes(const SparseNodesArgs& a, hipStream_t st);
+// cnt[r] += number of samples s in [col0, min(n, col0 + lde)) with E[knodes[r]][s - col0] > 0
+hipError_t launch_count_present(uint32_t* cnt, const double* E, size_t lde, uint32_t col0, const int32_t* knodes,
+                                uint32_t K, uint32_t n, hipStream_t st);
 uint32_t sparse_rows_blocks(uint32_t n);
 uint32_t row_list_chunks(uint32_t K);
 size_t row_list_scan_blocks(uint32_t n, uint32_t K);
