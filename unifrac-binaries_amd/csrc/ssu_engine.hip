@@ -272,14 +272,17 @@ bool PlanImpl::split_gram(std::string* err) {
         }
         return cap;
     };
-    // postings (4-byte slots) and row lists (8-byte slots), prefix counts and sample bits
-    auto bytes = [&](uint32_t cap) {
+    // postings (4-byte slots) and row lists (8-byte slots), prefix counts and
+    // sample bits; fewer than 2^30 slots (32-bit slot offsets in the kernels)
+    auto fits = [&](uint32_t cap) {
         const double ks = (double)sp_node.size();
-        return ks * cap * 12.0 + ks * (double)(npan + 1 + panels) * 4.0;
+        return ks * cap * 12.0 + ks * (double)(npan + 1 + panels) * 4.0 <= GSPLIT_BUDGET &&
+               ks * cap < (double)(1u << 30);
     };
     uint32_t thr = (uint32_t)std::max(1.0, std::ceil(theta * n));
     uint32_t cap = classify(thr);
-    while (bytes(cap) > GSPLIT_BUDGET && thr > 1) cap = classify(thr /= 2);
+    while (!fits(cap) && thr > 1) cap = classify(thr /= 2);
+    if (!fits(cap)) cap = classify(0);   // no sparse nodes
     if (!build_gram_columns()) return false;
     gsplit_cap = sp_node.empty() ? 0 : cap;
     if (gsplit_cap) {
@@ -378,17 +381,18 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     }
     // Occupancy split (FP32 shared-node path, non-negative lengths): nodes held
     // by fewer than theta * n samples are summed from posting lists instead of
-    // the 32 x 64 tiles.  Postings and row lists take K x cap entries each
-    // (16 bytes per slot together), kept within SPLIT_BUDGET by lowering theta.  SSU_SPARSE_SPLIT sets theta (0: off).
+    // the 32 x 64 tiles.  Postings and row lists take K x cap slots each (16
+    // bytes per slot together), fewer than 2^30 (16 GiB, and 32-bit slot
+    // offsets in the kernels): theta is lowered to fit.  SSU_SPARSE_SPLIT sets theta (0: off).
     if (P.sparse && !spec.fp64 && P.n >= 2) {
         double theta = 0.2;
         if (const char* env = getenv("SSU_SPARSE_SPLIT")) theta = atof(env);
         bool nonneg = true;
         for (float l : P.klen_f) nonneg = nonneg && l >= 0.0f;
-        constexpr double SPLIT_BUDGET = 16.0 * (1u << 30);
-        const double K = (double)std::max<size_t>(P.knodes.size(), 1);
-        theta = std::min(theta, SPLIT_BUDGET / (K * P.n * 16.0));
-        if (nonneg && theta > 0.0) P.split_cap = (uint32_t)std::max(1.0, std::ceil(theta * P.n));
+        const uint64_t K = std::max<uint64_t>(P.knodes.size(), 1);
+        const uint64_t max_cap = ((1ull << 30) - 1) / K;
+        if (nonneg && theta > 0.0 && max_cap > 0)
+            P.split_cap = (uint32_t)std::min<uint64_t>(max_cap, (uint64_t)std::max(1.0, std::ceil(theta * P.n)));
     }
 
     // Embedding blocks: the FP64 embedding of every node for every sample

@@ -987,6 +987,14 @@ __device__ __forceinline__ unsigned long long to_fixed(double x) {
     return ((unsigned long long)hi << 32) | lo;
 }
 
+// round(m * ls) as an integer, for 0 <= m * ls < 2^51: the FMA adds 2^52 with
+// one rounding (to nearest, ties to even), which leaves the integer in the
+// low 52 bits of the mantissa.
+__device__ __forceinline__ unsigned long long fixed_term(float m, double ls) {
+    const double y = fma((double)m, ls, 0x1.0p52);
+    return (unsigned long long)__double_as_longlong(y) & 0x000FFFFFFFFFFFFFull;
+}
+
 // Row lists of the sparse nodes: for sample a, the sparse nodes r that a holds,
 // in increasing r, each as r << 32 | (a's index in r's posting list).  One
 // wave per (32-sample panel P, chunk of RL_KC nodes) transposes the panel's
@@ -1111,12 +1119,13 @@ constexpr uint32_t SR_WAVES = 8;   // waves sharing one accumulator, each taking
 // waves per (row a, 4096-column chunk c).  Row a's list is taken 64 entries at
 // a time (round q goes to wave q % 8); a round's posting ranges inside the
 // chunk (past a) are walked with the wave's 64 lanes spread evenly over the
-// concatenated postings.  Weighted terms are FP32 (as in the tiled kernel);
-// terms are accumulated in 2^-(61-e) fixed point, where 2^e bounds the row's
-// sum (S[a] weighted; the FP64 sum of the row list's lengths unweighted), so
-// integer adds make the result independent of the order in which waves and
-// lanes add, and two samples holding the same nodes get bit-equal G(a, b)
-// and G(a, a).
+// concatenated postings.  Terms are accumulated in fixed point, integers in
+// units of 2^(e - 51) (weighted: the FP64 product l_r * min, rounded to
+// nearest) or 2^(e - 61) (GRAM: l_r, truncated) where 2^e bounds the row's sum
+// (S[a] weighted; the FP64 sum of the row list's lengths unweighted).  Integer
+// adds make the result independent of the order in which waves and lanes add,
+// and two samples holding the same nodes get bit-equal G(a, b) and G(a, a).
+// Posting slots are addressed with 32-bit offsets (K * cap < 2^30).
 template <class TOut, class IX, bool GRAM>
 __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__ out, uint32_t* __restrict__ walked,
                                                    const unsigned long long* __restrict__ rows,
@@ -1136,16 +1145,15 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     }
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     __shared__ unsigned long long acc[SR_CW];
-    __shared__ uint32_t pre_s[SR_WAVES][65], lst_s[SR_WAVES][64], lnode_s[SR_WAVES][64], wk[SR_WAVES];
-    __shared__ float lu_s[SR_WAVES][64], ll_s[SR_WAVES][64];
-    __shared__ unsigned long long lfx_s[SR_WAVES][GRAM ? 64 : 1], diag_s;
+    __shared__ uint32_t pre_s[SR_WAVES][65], base_s[SR_WAVES][64], wk[SR_WAVES];
+    __shared__ float lu_s[SR_WAVES][GRAM ? 1 : 64];
+    // per entry: GRAM the fixed-point term, weighted l_r * scale (FP64 bits)
+    __shared__ unsigned long long q8_s[SR_WAVES][64], diag_s;
     __shared__ double bsum_s[SR_WAVES];
     uint32_t* pre = pre_s[wv];
-    uint32_t* lst = lst_s[wv];
-    uint32_t* lnode = lnode_s[wv];
+    uint32_t* base = base_s[wv];
     float* lu = lu_s[wv];
-    float* ll = ll_s[wv];
-    unsigned long long* lfx = lfx_s[wv];
+    unsigned long long* q8 = q8_s[wv];
     for (uint32_t q = threadIdx.x; q < SR_CW; q += 64 * SR_WAVES) acc[q] = 0ull;
     const uint32_t e0 = offsets[(size_t)a * nkc], e1 = offsets[(size_t)(a + 1) * nkc];
     double bound;
@@ -1167,7 +1175,8 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     __syncthreads();
     int e2 = 0;
     (void)frexp(bound, &e2);   // bound < 2^e2 bounds every partial sum of the row
-    const double scale = ldexp(1.0, 61 - e2), inv = ldexp(1.0, e2 - 61);
+    const int fb = GRAM ? 61 : 51;
+    const double scale = ldexp(1.0, fb - e2), inv = ldexp(1.0, e2 - fb);
     const bool has_diag = GRAM && b0 <= a && a < b1;
     if (GRAM && has_diag) {   // G(a, a): the same fixed-point terms, every one of them
         unsigned long long x = 0ull;
@@ -1179,8 +1188,8 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     uint32_t total_walked = 0;
     for (uint32_t r0 = e0 + 64 * wv; r0 < e1; r0 += 64 * SR_WAVES) {   // uniform per wave
         uint32_t start = 0, end = 0, r = 0;
-        float ua = 0.0f, l = 0.0f;
-        unsigned long long lf = 0ull;
+        float ua = 0.0f;
+        unsigned long long q = 0ull;
         if (r0 + lane < e1) {
             const unsigned long long ent = rows[r0 + lane];
             r = (uint32_t)(ent >> 32);
@@ -1189,10 +1198,10 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             start = max(pa + 1, cs);
             end = ce;
             if constexpr (GRAM) {
-                lf = to_fixed(len[r] * scale);
+                q = to_fixed(len[r] * scale);
             } else {
-                ua = pval[(size_t)r * cap + pa];
-                l = (float)len[r];
+                ua = pval[r * cap + pa];
+                q = (unsigned long long)__double_as_longlong(len[r] * scale);
             }
         }
         const uint32_t cnt = end > start ? end - start : 0u;
@@ -1206,28 +1215,22 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         if (T == 0) continue;   // uniform per wave
         pre[lane] = incl - cnt;
         if (lane == 63) pre[64] = incl;
-        lst[lane] = start;
-        lnode[lane] = r;
-        if constexpr (GRAM) {
-            lfx[lane] = lf;
-        } else {
-            lu[lane] = ua;
-            ll[lane] = l;
-        }
+        base[lane] = r * cap + start - (incl - cnt);   // slot of walk position t: base + t (mod 2^32)
+        if constexpr (!GRAM) lu[lane] = ua;
+        q8[lane] = q;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // four postings per lane in flight: positions first, then every load, then the adds
+        // four postings per lane in flight: slots first, then every load, then the adds
         uint32_t e = 0;
         for (uint32_t t0 = lane; t0 < T; t0 += 256) {
-            size_t qq[4];
-            uint32_t eq[4];
+            uint32_t qq[4], eq[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const uint32_t t = min(t0 + 64 * u, T - 1);
                 while (pre[e + 1] <= t) ++e;
                 eq[u] = e;
-                qq[u] = (size_t)lnode[e] * cap + lst[e] + (t - pre[e]);
+                qq[u] = base[e] + t;
             }
             uint32_t bq[4];
             float vq[4];
@@ -1239,10 +1242,13 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 if (t0 + 64 * u < T) {
-                    if constexpr (GRAM)
-                        atomicAdd(&acc[bq[u] - b0], lfx[eq[u]]);
-                    else
-                        atomicAdd(&acc[bq[u] - b0], to_fixed((double)(ll[eq[u]] * fminf(lu[eq[u]], vq[u])) * scale));
+                    if constexpr (GRAM) {
+                        atomicAdd(&acc[bq[u] - b0], q8[eq[u]]);
+                    } else {
+                        // u >= 0: the smaller float has the smaller bit pattern
+                        const float m = __uint_as_float(min(__float_as_uint(lu[eq[u]]), __float_as_uint(vq[u])));
+                        atomicAdd(&acc[bq[u] - b0], fixed_term(m, __longlong_as_double((long long)q8[eq[u]])));
+                    }
                 }
         }
         total_walked += T;
@@ -1482,8 +1488,9 @@ static void pairs_sparse32_t(uint32_t grid, hipStream_t st, const SparseArgs& a,
 }
 
 static bool sparse_args_ok(const SparseNodesArgs& a) {
+    // posting slots are addressed with 32-bit offsets (bytes included)
     const bool common = a.dmask && a.smask && a.pc && a.pidx && a.len && a.cap && a.rows && a.rcount && a.roff &&
-                        a.rbsum && a.npan == (a.n + 31) / 32;
+                        a.rbsum && a.npan == (a.n + 31) / 32 && (uint64_t)a.K * a.cap < (1ull << 30);
     if (!common) return false;
     if (a.gram) return a.n_pad >= a.n && a.n_pad % GRAM_BM == 0;   // unweighted: values, sums and out unused
     return a.pval && a.U && a.S && a.out && !a.out_fp64 && a.ldu >= (size_t)a.npan * 32;
