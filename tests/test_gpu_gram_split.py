@@ -208,3 +208,40 @@ def test_split_rarefaction_replicates_and_replays(ssu, O, synth, split):
         for rep in range(4):
             for k in range(2):
                 assert np.array_equal(r.matrix(k, rep), eager[k][rep]), (it, rep, k)
+
+
+@pytest.mark.parametrize("theta", ["0", "0.3"])
+def test_presence_bits_match_fp64_embedding(ssu, O, synth, split, monkeypatch, theta):
+    """The Gram path on presence bits (non-negative tables) and on the FP64
+    embedding (SSU_GRAM_BITS=0) build the same operands: equal bit for bit."""
+    tree, table = _case(synth, 500, 300, 0.05, 83, zipf=1.1)
+    split(theta)
+    inp = ssu.Inputs(*tree, *table)
+    out = {}
+    for bits in ("1", "0"):
+        monkeypatch.setenv("SSU_GRAM_BITS", bits)
+        for name in ("unweighted_fp64", "unweighted", "unweighted_unnormalized"):
+            say("presence bits", bits, theta, name)
+            s, m = ssu.matrix_inmem(inp, name)
+            assert s == ssu.OKAY, ssu.last_error()
+            out[bits, name] = m
+    for name in ("unweighted_fp64", "unweighted", "unweighted_unnormalized"):
+        assert np.array_equal(out["1", name], out["0", name]), name
+    assert rel_err(out["1", "unweighted_fp64"], oracle_matrix(O, tree, table, "unweighted")) <= TOL["fp64"]
+
+
+def test_negative_values_take_fp64_embedding(ssu, O, synth, split):
+    """A table with negative values: presence is E > 0 of the summed embedding
+    (a node can cancel to zero), so the Gram path keeps the FP64 embedding."""
+    st, le, nm = synth.random_tree(200, 17)
+    obs, sids, ip, ix, d = synth.random_table(200, 90, 0.1, 18)
+    d = d.copy()
+    d[::5] *= -1.0
+    table = (obs, sids, ip, ix, d)
+    split("0.3")
+    for name in ("unweighted_fp64", "unweighted"):
+        say("negative values", name)
+        s, m = ssu.matrix_inmem(ssu.Inputs(st, le, nm, *table), name)
+        assert s == ssu.OKAY, ssu.last_error()
+        ref = oracle_matrix(O, (st, le, nm), table, "unweighted")
+        assert rel_err(m, ref) <= TOL["fp64" if "fp64" in name else "fp32"]

@@ -96,6 +96,10 @@ struct PlanImpl {
     std::vector<double> sp_len;      // their lengths as the accumulation sees them
     std::vector<uint8_t> gram_keep;  // per processed node: a Gram column (empty: every node)
     uint64_t nnz = 0;
+    // Gram path on presence bits (every value finite and >= 0, n <= 262,144):
+    // no FP64 embedding; ldb words of 32 samples per node row
+    bool bits = false;
+    uint32_t ldb = 0;
     std::vector<uint32_t> block_order;  // pair blocks / Gram tiles in launch order
     bool bad_lengths = false;
 
@@ -122,7 +126,7 @@ struct PlanImpl {
     DevMem d_indptr, d_indices, d_data, d_totals, d_leaf_k, d_leaf_row, d_level_nodes, d_child_ptr, d_child_idx;
     DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_order, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
         d_faith, d_stripes, d_smask, d_dmask, d_pc, d_pidx, d_pval, d_walked, d_rows, d_rcount, d_roff, d_rbsum,
-        d_sp_node, d_sp_len, d_umasks;
+        d_sp_node, d_sp_len, d_umasks, d_B;
 
     hipStream_t own_st = nullptr;    // created with the plan; st may point at a caller's stream
     // per-execution inputs and output (nullptr: the plan's own table data / totals / result)
@@ -144,6 +148,8 @@ struct PlanImpl {
 
     bool build_gram_columns();
     bool split_gram(std::string* err);
+    // presence bits of every node (structural: every stored entry counts as present)
+    bool embed_bits(bool structural, std::string* err);
     // node-major FP64 embedding of sample columns [c0, c0 + w) into dst (row stride w)
     bool embed(DevMem& dst, bool normalize, size_t c0, size_t w, std::string* err);
 };
@@ -238,18 +244,23 @@ bool PlanImpl::split_gram(std::string* err) {
     const uint32_t K = (uint32_t)knodes.size();
     if (!(theta > 0.0) || K == 0 || n < std::max<uint32_t>(min_n, 2)) return true;
     DevMem ones, cnt_d;
-    if (!ones.ensure(std::max<uint64_t>(nnz, 1) * sizeof(double), err) || !cnt_d.ensure(K * sizeof(uint32_t), err))
-        return false;
-    SSU_HIP(launch_fill(ones.as<double>(), nnz, 1.0, st));
-    SSU_HIP(hipMemsetAsync(cnt_d.p, 0, K * sizeof(uint32_t), st));
-    x_data = ones.as<double>();
-    for (size_t c0 = 0; c0 < ldn && c0 < n; c0 += blk) {
-        const size_t w = std::min(blk, ldn - c0);
-        if (!embed(d_E, false, c0, w, err)) { x_data = nullptr; return false; }
-        SSU_HIP(launch_count_present(cnt_d.as<uint32_t>(), d_E.as<double>(), w, (uint32_t)c0,
-                                     d_knodes.as<int32_t>(), K, n, st));
+    if (!cnt_d.ensure(K * sizeof(uint32_t), err)) return false;
+    if (bits) {
+        if (!embed_bits(true, err)) return false;
+        SSU_HIP(launch_count_bits(cnt_d.as<uint32_t>(), d_B.as<uint32_t>(), ldb, d_knodes.as<int32_t>(), K, st));
+    } else {
+        if (!ones.ensure(std::max<uint64_t>(nnz, 1) * sizeof(double), err)) return false;
+        SSU_HIP(launch_fill(ones.as<double>(), nnz, 1.0, st));
+        SSU_HIP(hipMemsetAsync(cnt_d.p, 0, K * sizeof(uint32_t), st));
+        x_data = ones.as<double>();
+        for (size_t c0 = 0; c0 < ldn && c0 < n; c0 += blk) {
+            const size_t w = std::min(blk, ldn - c0);
+            if (!embed(d_E, false, c0, w, err)) { x_data = nullptr; return false; }
+            SSU_HIP(launch_count_present(cnt_d.as<uint32_t>(), d_E.as<double>(), w, (uint32_t)c0,
+                                         d_knodes.as<int32_t>(), K, n, st));
+        }
+        x_data = nullptr;
     }
-    x_data = nullptr;
     std::vector<uint32_t> cnt(K);
     SSU_HIP(hipMemcpyAsync(cnt.data(), cnt_d.p, K * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     SSU_HIP(hipStreamSynchronize(st));
@@ -290,6 +301,19 @@ bool PlanImpl::split_gram(std::string* err) {
         if (!upload(d_sp_node, sp_node, st, err) || !upload(d_sp_len, sp_len, st, err) || !upload(d_dmask, zero, st, err))
             return false;
         SSU_HIP(hipStreamSynchronize(st));   // `zero` is read before it goes out of scope
+    }
+    return true;
+}
+
+bool PlanImpl::embed_bits(bool structural, std::string* err) {
+    if (!d_B.ensure(std::max<size_t>(m, 1) * ldb * sizeof(uint32_t), err)) return false;
+    uint32_t* B = d_B.as<uint32_t>();
+    SSU_HIP(launch_bits_leaves(B, ldb, d_leaf_k.as<int32_t>(), d_leaf_row.as<int32_t>(), (uint32_t)leaf_k.size(),
+                               d_indptr.as<uint32_t>(), d_indices.as<uint32_t>(), data_ptr(), structural, st));
+    for (size_t h = 0; h + 1 < level_off.size(); ++h) {
+        const uint32_t a = level_off[h], b = level_off[h + 1];
+        SSU_HIP(launch_bits_level(B, ldb, d_level_nodes.as<int32_t>() + a, b - a, d_child_ptr.as<int32_t>(),
+                                  d_child_idx.as<int32_t>(), st));
     }
     return true;
 }
@@ -367,6 +391,21 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     P.gram = unw && !spec.vaw && spec.plain_kernel == 0;
     if (P.gram && !P.build_gram_columns()) P.gram = false;  // non-finite lengths: VALU path
     P.n_pad = round_up(std::max<uint32_t>(P.n, 1), GRAM_BM);
+    // Presence bits for the Gram path: with every value finite and >= 0 (and
+    // positive totals), E[k][s] > 0 exactly when a stored value > 0 lies below
+    // k, an OR up the tree.  SSU_GRAM_BITS=0 keeps the FP64 embedding.
+    if (P.gram && P.ldn / 32 <= BITS_MAX_WORDS) {
+        bool ok_vals = true;
+        if (const char* env = getenv("SSU_GRAM_BITS")) ok_vals = atoi(env) != 0;
+        const uint64_t nz = table.nnz();
+        const uint32_t b0 = table.n_obs ? table.indptr[0] : 0;
+        for (uint64_t q = 0; q < nz && ok_vals; ++q)
+            ok_vals = std::isfinite(table.data[b0 + q]) && table.data[b0 + q] >= 0.0;
+        if (ok_vals) {
+            P.bits = true;
+            P.ldb = (uint32_t)(P.ldn / 32);
+        }
+    }
     // The shared-node weighted kernel needs u >= 0 (so min(u, v) vanishes when
     // either sample lacks the node) and finite lengths (S_i + S_j - 2M).
     const bool weighted = spec.method == Method::weighted_normalized || spec.method == Method::weighted_unnormalized;
@@ -539,7 +578,17 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
     if (P.gram && P.Kp && !P.d_Pt.ensure(P.n_pad * P.Kp, err)) return false;
 
     SSU_HIP(P.mark(0));
-    for (size_t c0 = 0; c0 < P.ldn; c0 += P.blk) {
+    if (P.bits) {   // Gram path on presence bits: one block of every sample
+        if (!P.embed_bits(false, err)) return false;
+        SSU_HIP(P.mark(1));
+        if (P.Kp)
+            SSU_HIP(launch_presence_T_bits(P.d_Pt.as<int8_t>(), P.Kp, P.d_B.as<uint32_t>(), P.ldb,
+                                           P.d_col.as<int32_t>(), (uint32_t)P.Kp, (uint32_t)P.n_pad, P.n, P.st));
+        if (gsplit)
+            SSU_HIP(launch_bits_smask(P.d_smask.as<uint32_t>(), P.d_B.as<uint32_t>(), P.ldb,
+                                      P.d_sp_node.as<int32_t>(), Ks, npan, P.st));
+    }
+    for (size_t c0 = 0; c0 < P.ldn && !P.bits; c0 += P.blk) {
         const size_t w = std::min(P.blk, P.ldn - c0);
         if (!P.embed(P.d_E, s.normalize, c0, w, err)) return false;
         if (s.vaw && !P.embed(P.d_C, false, c0, w, err)) return false;  // raw counts, src/unifrac_cmp.cpp:250-251
@@ -562,7 +611,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
                                            (uint32_t)c0, P.d_knodes.as<int32_t>(), nullptr, K, P.n, P.st));
         }
     }
-    if (!single) SSU_HIP(P.mark(1));   // blocked: embed and prep share one phase
+    if (!single && !P.bits) SSU_HIP(P.mark(1));   // blocked: embed and prep share one phase
     if (split) {   // posting lists, row lists and each row's sparse-node share, into the output
         SSU_HIP(P.mark(EV_SPLIT));
         SparseNodesArgs b{};

@@ -150,6 +150,79 @@ __global__ void __launch_bounds__(64) k_faith(double* __restrict__ out, const do
     out[s] = acc;
 }
 
+// ------------------------------------------------------------------ presence bits
+// B is node-major with ldb 32-bit words per row: bit j of B[k][w] says that
+// sample 32 w + j holds node k.  With non-negative values a node is present
+// exactly when one of its children is, so the rows are ORs up the tree.
+
+// One workgroup per leaf: the row's bits from its CSR entries (value > 0, or
+// every stored entry when `structural`), gathered in LDS, then stored whole.
+__global__ void __launch_bounds__(256) k_bits_leaves(uint32_t* __restrict__ B, uint32_t ldb,
+                                                     const int32_t* __restrict__ leaf_k,
+                                                     const int32_t* __restrict__ leaf_row,
+                                                     const uint32_t* __restrict__ indptr,
+                                                     const uint32_t* __restrict__ indices,
+                                                     const double* __restrict__ data, int structural) {
+    extern __shared__ uint32_t bits_row[];
+    for (uint32_t w = threadIdx.x; w < ldb; w += 256) bits_row[w] = 0u;
+    __syncthreads();
+    const int32_t r = leaf_row[blockIdx.x];
+    if (r >= 0)
+        for (uint32_t j = indptr[r] + threadIdx.x; j < indptr[r + 1]; j += 256) {
+            const uint32_t s = indices[j];
+            if (structural || data[j] > 0.0) atomicOr(&bits_row[s >> 5], 1u << (s & 31));
+        }
+    __syncthreads();
+    uint32_t* dst = B + (size_t)leaf_k[blockIdx.x] * ldb;
+    for (uint32_t w = threadIdx.x; w < ldb; w += 256) dst[w] = bits_row[w];
+}
+
+// Internal nodes of one height level: one thread per (node, word).
+__global__ void __launch_bounds__(256) k_bits_level(uint32_t* __restrict__ B, uint32_t ldb,
+                                                    const int32_t* __restrict__ nodes, uint32_t n_nodes,
+                                                    const int32_t* __restrict__ child_ptr,
+                                                    const int32_t* __restrict__ child_idx) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (uint64_t)n_nodes * ldb) return;
+    const uint32_t ni = (uint32_t)(i / ldb), w = (uint32_t)(i % ldb);
+    const int32_t k = nodes[ni];
+    uint32_t acc = 0u;
+    for (int32_t c = child_ptr[k]; c < child_ptr[k + 1]; ++c) acc |= B[(size_t)child_idx[c] * ldb + w];
+    B[(size_t)k * ldb + w] = acc;
+}
+
+// cnt[r] = samples holding knodes[r]: one workgroup per node
+__global__ void __launch_bounds__(256) k_count_bits(uint32_t* __restrict__ cnt, const uint32_t* __restrict__ B,
+                                                    uint32_t ldb, const int32_t* __restrict__ knodes) {
+    __shared__ uint32_t part[4];
+    const uint32_t* row = B + (size_t)knodes[blockIdx.x] * ldb;
+    uint32_t c = 0;
+    for (uint32_t w = threadIdx.x; w < ldb; w += 256) c += __popc(row[w]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// smask[p][r] = B[nodes[r]][p] for the K listed nodes and panels p < npan:
+// 64 nodes x 64 words per workgroup, transposed through LDS.
+__global__ void __launch_bounds__(256) k_bits_smask(uint32_t* __restrict__ smask, const uint32_t* __restrict__ B,
+                                                    uint32_t ldb, const int32_t* __restrict__ nodes, uint32_t K,
+                                                    uint32_t npan) {
+    __shared__ uint32_t tile[64][65];
+    const uint32_t r0 = blockIdx.x * 64, p0 = blockIdx.y * 64, l = threadIdx.x & 63;
+    for (uint32_t q = threadIdx.x >> 6; q < 64; q += 4) {
+        const uint32_t r = r0 + q, p = p0 + l;
+        tile[q][l] = (r < K && p < npan) ? B[(size_t)nodes[r] * ldb + p] : 0u;
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x >> 6; q < 64; q += 4) {
+        const uint32_t p = p0 + q, r = r0 + l;
+        if (p < npan && r < K) smask[(size_t)p * K + r] = tile[l][q];
+    }
+}
+
 // ------------------------------------------------------------------ Gram path
 
 // 64 K'-columns x 64 samples per workgroup: read node rows (coalesced over
@@ -182,6 +255,34 @@ __global__ void __launch_bounds__(256) k_presence_T(int8_t* __restrict__ Pt, siz
         w[q] = x;
     }
     *reinterpret_cast<uint4*>(Pt + (size_t)(col0 + s0 + row) * ldp + c0 + cb) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Pt from presence bits (every sample in B, n_pad rows): as k_presence_T.
+__global__ void __launch_bounds__(256) k_presence_T_bits(int8_t* __restrict__ Pt, size_t ldp,
+                                                         const uint32_t* __restrict__ B, uint32_t ldb,
+                                                         const int32_t* __restrict__ col_node, uint32_t n) {
+    __shared__ uint8_t tile[64][64 + 4];
+    const uint32_t c0 = blockIdx.x * 64, s0 = blockIdx.y * 64;
+    const uint32_t t = threadIdx.x;
+    const uint32_t ls = t & 63;
+    for (uint32_t q = t >> 6; q < 64; q += 4) {
+        const int32_t node = col_node[c0 + q];
+        const uint32_t s = s0 + ls;
+        uint8_t v = 0;
+        if (node >= 0 && s < n) v = ((B[(size_t)node * ldb + (s >> 5)] >> (s & 31)) & 1u) ? 0xFF : 0x00;
+        tile[q][ls] = v;
+    }
+    __syncthreads();
+    const uint32_t row = t >> 2, cb = (t & 3) * 16;
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) x |= (uint32_t)tile[cb + q * 4 + b][row] << (8 * b);
+        w[q] = x;
+    }
+    *reinterpret_cast<uint4*>(Pt + (size_t)(s0 + row) * ldp + c0 + cb) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -1357,6 +1458,50 @@ hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint3
     const uint32_t gy = std::min<uint32_t>(n_nodes, 8192);
     hipLaunchKernelGGL(k_embed_level, dim3(cb, gy), dim3(EMB_THREADS), 0, st, E, ldn, nodes, n_nodes, child_ptr,
                        child_idx);
+    return hipGetLastError();
+}
+
+hipError_t launch_bits_leaves(uint32_t* B, uint32_t ldb, const int32_t* leaf_k, const int32_t* leaf_row,
+                              uint32_t n_leaves, const uint32_t* indptr, const uint32_t* indices, const double* data,
+                              bool structural, hipStream_t st) {
+    if (!n_leaves) return hipSuccess;
+    if (!ldb || ldb > BITS_MAX_WORDS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_bits_leaves, dim3(n_leaves), dim3(256), ldb * sizeof(uint32_t), st, B, ldb, leaf_k, leaf_row,
+                       indptr, indices, data, (int)structural);
+    return hipGetLastError();
+}
+
+hipError_t launch_bits_level(uint32_t* B, uint32_t ldb, const int32_t* nodes, uint32_t n_nodes,
+                             const int32_t* child_ptr, const int32_t* child_idx, hipStream_t st) {
+    if (!n_nodes) return hipSuccess;
+    const uint64_t blocks = ((uint64_t)n_nodes * ldb + 255) / 256;
+    if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_bits_level, dim3((uint32_t)blocks), dim3(256), 0, st, B, ldb, nodes, n_nodes, child_ptr,
+                       child_idx);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_bits(uint32_t* cnt, const uint32_t* B, uint32_t ldb, const int32_t* knodes, uint32_t K,
+                             hipStream_t st) {
+    if (!K) return hipSuccess;
+    hipLaunchKernelGGL(k_count_bits, dim3(K), dim3(256), 0, st, cnt, B, ldb, knodes);
+    return hipGetLastError();
+}
+
+hipError_t launch_bits_smask(uint32_t* smask, const uint32_t* B, uint32_t ldb, const int32_t* nodes, uint32_t K,
+                             uint32_t npan, hipStream_t st) {
+    if (!K || !npan) return hipSuccess;
+    if (npan > ldb) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_bits_smask, dim3((K + 63) / 64, (npan + 63) / 64), dim3(256), 0, st, smask, B, ldb, nodes, K,
+                       npan);
+    return hipGetLastError();
+}
+
+hipError_t launch_presence_T_bits(int8_t* Pt, size_t ldp, const uint32_t* B, uint32_t ldb, const int32_t* col_node,
+                                  uint32_t Kp, uint32_t n_pad, uint32_t n, hipStream_t st) {
+    if (Kp % 64 || n_pad % 64 || (size_t)ldb * 32 < n_pad) return hipErrorInvalidValue;
+    if (!Kp || !n_pad) return hipSuccess;
+    hipLaunchKernelGGL(k_presence_T_bits, dim3(Kp / 64, n_pad / 64), dim3(256), 0, st, Pt, ldp, B, ldb, col_node, n);
     return hipGetLastError();
 }
 

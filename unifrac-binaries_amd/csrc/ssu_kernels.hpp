@@ -35,6 +35,23 @@ hipError_t launch_embed_leaves(double* E, size_t ld, uint32_t col0, const int32_
 hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint32_t n_nodes, const int32_t* child_ptr,
                               const int32_t* child_idx, hipStream_t st);
 
+// Presence bits (node-major, ldb words of 32 samples per row; every sample of
+// the table in one block): bit s%32 of B[k][s/32] = sample s holds node k.
+// Leaves from the CSR (value > 0, or every stored entry when `structural`),
+// internal nodes level by level as the OR of their children.
+constexpr uint32_t BITS_MAX_WORDS = 8192;   // 262,144 samples: a leaf row is gathered in 32 KB of LDS
+hipError_t launch_bits_leaves(uint32_t* B, uint32_t ldb, const int32_t* leaf_k, const int32_t* leaf_row,
+                              uint32_t n_leaves, const uint32_t* indptr, const uint32_t* indices, const double* data,
+                              bool structural, hipStream_t st);
+hipError_t launch_bits_level(uint32_t* B, uint32_t ldb, const int32_t* nodes, uint32_t n_nodes,
+                             const int32_t* child_ptr, const int32_t* child_idx, hipStream_t st);
+// cnt[r] = popcount of B[knodes[r]]
+hipError_t launch_count_bits(uint32_t* cnt, const uint32_t* B, uint32_t ldb, const int32_t* knodes, uint32_t K,
+                             hipStream_t st);
+// smask[p][r] = B[nodes[r]][p], p < npan (the sample bits launch_gather_sums writes)
+hipError_t launch_bits_smask(uint32_t* smask, const uint32_t* B, uint32_t ldb, const int32_t* nodes, uint32_t K,
+                             uint32_t npan, hipStream_t st);
+
 // One segmented pass over the processed rows r = 0..K-1 (row r is E[knodes[r]])
 // of an embedding block holding sample columns [col0, col0 + lde):
 //   U (if non-null): node-major FP32 copy, U[r][col0 + s] = (float)E[knodes[r]][s] (row stride ldu);
@@ -58,6 +75,9 @@ hipError_t launch_faith(double* out, const double* E, size_t ldn, const double* 
 // for the embedding block of sample columns [col0, col0 + lde): Pt rows col0 ..
 hipError_t launch_presence_T(int8_t* Pt, size_t ldp, const double* E, size_t lde, uint32_t col0, const int32_t* col_node,
                              uint32_t Kp, uint32_t n, hipStream_t st);
+// Pt[s][c] for s < n_pad from presence bits (rows past n zero)
+hipError_t launch_presence_T_bits(int8_t* Pt, size_t ldp, const uint32_t* B, uint32_t ldb, const int32_t* col_node,
+                                  uint32_t Kp, uint32_t n_pad, uint32_t n, hipStream_t st);
 // G = sum_c len_c * P[i][c] * P[j][c] for the upper-triangle 128x128 tiles, FP64,
 // stored tile by tile in MFMA fragment order (n_pad/128)^2 * 128^2 doubles
 hipError_t launch_gram_i8(double* G, const int8_t* Pt, size_t ldp, const int8_t* digits, const GramChunk* chunks,
