@@ -424,7 +424,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     // bytes per slot together), fewer than 2^30 (16 GiB, and 32-bit slot
     // offsets in the kernels): theta is lowered to fit.  SSU_SPARSE_SPLIT sets theta (0: off).
     if (P.sparse && !spec.fp64 && P.n >= 2) {
-        double theta = 0.2;
+        double theta = 0.25;
         if (const char* env = getenv("SSU_SPARSE_SPLIT")) theta = atof(env);
         bool nonneg = true;
         for (float l : P.klen_f) nonneg = nonneg && l >= 0.0f;

@@ -1217,10 +1217,10 @@ constexpr uint32_t SR_WAVES = 8;   // waves sharing one accumulator, each taking
 // out[a * n + b] for the tiled kernel's epilogue.  GRAM (unweighted): sum of
 // l_r over the sparse nodes a and b >= a share, added to the FP64 Gram entry
 // G(a, b) (the diagonal by the chunk that holds column a).  One workgroup of 8
-// waves per (row a, 4096-column chunk c).  Row a's list is taken 64 entries at
-// a time (round q goes to wave q % 8); a round's posting ranges inside the
-// chunk (past a) are walked with the wave's 64 lanes spread evenly over the
-// concatenated postings.  Terms are accumulated in fixed point, integers in
+// waves per (row a, 4096-column chunk c), blockIdx = c * n + a.  Row a's list
+// is taken 64 entries at a time (round q goes to wave q % 8); a round's
+// posting ranges inside the chunk (past a) are walked with the wave's 64 lanes
+// spread evenly over the concatenated postings.  Terms are accumulated in fixed point, integers in
 // units of 2^(e - 51) (weighted: the FP64 product l_r * min, rounded to
 // nearest) or 2^(e - 61) (GRAM: l_r, truncated) where 2^e bounds the row's sum
 // (S[a] weighted; the FP64 sum of the row list's lengths unweighted).  Integer
@@ -1237,7 +1237,9 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                                                    uint32_t K, uint32_t n, uint32_t npan, uint32_t cap,
                                                    uint32_t nchunks, uint32_t nkc, double* __restrict__ G,
                                                    uint32_t tiles) {
-    const uint32_t a = blockIdx.x / nchunks, c = blockIdx.x % nchunks;
+    // chunk-major: the rows in flight share one column chunk of every posting
+    // list, a quarter of the posting data at 16k samples (cache-resident)
+    const uint32_t a = blockIdx.x % n, c = blockIdx.x / n;
     const uint32_t b0 = c * SR_CW, b1 = min(n, b0 + SR_CW);
     // uniform: nothing at or above the diagonal here (weighted: strictly above)
     if (a >= n || (GRAM ? b1 <= a : b1 <= a + 1)) {
