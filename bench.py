@@ -360,14 +360,15 @@ def main():
         kernels[name]["terms_per_pair_node"] = (terms + sterms) / max(pair_nodes[name], 1)
     # dominant kernel: the largest of the main kernels and the posting-list phases (k_pairs_rows)
     cands = [(k["main_ms"], name, k["kernel"]) for name, k in kernels.items()]
-    cands += [(k["sparse_ms"], name, "k_pairs_rows") for name, k in kernels.items() if k["sparse_terms"]]
+    cands += [(k["sparse_ms"], name, "k_pairs_rows" + ("[gram]" if name.startswith("unweighted") else ""))
+              for name, k in kernels.items() if k["sparse_terms"]]
     dom_ms, dom, dom_kernel = max(cands)
     dk = kernels[dom]
     if dom_kernel == "k_gram_i8":
         achieved = OPS_PER_PAIR_NODE_GRAM * gram_units(n, dk["kernel_terms"]) / (dom_ms / 1000.0) / 1e12
         roofline = {"kernel": dom_kernel, "bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS,
                     "unit": "TOP/s", "frac": achieved / PEAK_INT8_TOPS, "traffic": None}
-    elif dom_kernel == "k_pairs_rows":
+    elif dom_kernel.startswith("k_pairs_rows"):
         # each walked posting reads its sample index (and value)
         achieved = bytes_per_posting(dom, n) * dk["sparse_terms"] / (dom_ms / 1000.0) / 1e9
         roofline = {"kernel": dom_kernel, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
