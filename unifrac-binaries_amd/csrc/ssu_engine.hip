@@ -88,10 +88,11 @@ struct PlanImpl {
     size_t blk = 0;                  // sample columns per embedding block (ldn: one block)
     bool gram = false;
     bool sparse = false;             // weighted through the shared-node kernel
-    uint32_t split_cap = 0;          // FP32 shared-node path: nodes held by fewer samples use posting lists (0: off)
+    uint32_t split_thr = 0;          // FP32 shared-node path: nodes held by fewer samples use posting lists (0: off)
+    uint32_t split_slots = 0;        // bound on their postings in any execution
     // Gram path occupancy split: sparse nodes (held by few samples of the plan's
     // table) are summed from posting lists, the Gram keeps the others
-    uint32_t gsplit_cap = 0;         // posting slots per sparse node (0: off)
+    uint32_t gsplit_slots = 0;       // posting slots of all sparse nodes (0: off); d_poff holds their offsets
     std::vector<int32_t> sp_node;    // the sparse nodes
     std::vector<double> sp_len;      // their lengths as the accumulation sees them
     std::vector<uint8_t> gram_keep;  // per processed node: a Gram column (empty: every node)
@@ -126,7 +127,7 @@ struct PlanImpl {
     DevMem d_indptr, d_indices, d_data, d_totals, d_leaf_k, d_leaf_row, d_level_nodes, d_child_ptr, d_child_idx;
     DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_order, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
         d_faith, d_stripes, d_smask, d_dmask, d_pc, d_pidx, d_pval, d_walked, d_rows, d_rcount, d_roff, d_rbsum,
-        d_sp_node, d_sp_len, d_umasks, d_B;
+        d_sp_node, d_sp_len, d_umasks, d_B, d_poff, d_pcnt, d_obsum;
 
     hipStream_t own_st = nullptr;    // created with the plan; st may point at a caller's stream
     // per-execution inputs and output (nullptr: the plan's own table data / totals / result)
@@ -148,8 +149,11 @@ struct PlanImpl {
 
     bool build_gram_columns();
     bool split_gram(std::string* err);
-    // presence bits of every node (structural: every stored entry counts as present)
-    bool embed_bits(bool structural, std::string* err);
+    bool split_weighted(std::string* err);
+    bool structural_counts(std::vector<uint32_t>& cnt, std::string* err);
+    // presence bits of every node into dst, lb words per row (structural: every
+    // stored entry counts as present)
+    bool embed_bits(DevMem& dst, uint32_t lb, bool structural, std::string* err);
     // node-major FP64 embedding of sample columns [c0, c0 + w) into dst (row stride w)
     bool embed(DevMem& dst, bool normalize, size_t c0, size_t w, std::string* err);
 };
@@ -227,27 +231,23 @@ bool PlanImpl::build_gram_columns() {
     return true;
 }
 
-// Gram path occupancy split.  A node's structural occupancy is the number of
-// samples with a stored table entry below it: the embedding of an all-ones
-// copy of the table, counted per node.  Any values on the same structure (the
-// plan's own, or rarefied replicates drawn from it) make the node present in a
-// subset of those samples, so the classification and the posting capacity
-// hold for every execution.  Nodes held by fewer than theta * n samples (and
-// of positive length) leave the Gram for the posting lists; nodes held by no
-// sample leave it altogether.  SSU_GRAM_SPLIT sets theta (0: off),
-// SSU_GRAM_SPLIT_MIN_N the smallest table split (GSPLIT_MIN_N).
-bool PlanImpl::split_gram(std::string* err) {
-    double theta = 0.04;
-    if (const char* env = getenv("SSU_GRAM_SPLIT")) theta = atof(env);
-    uint32_t min_n = GSPLIT_MIN_N;
-    if (const char* env = getenv("SSU_GRAM_SPLIT_MIN_N")) min_n = (uint32_t)atol(env);
+// A node's structural occupancy: the number of samples with a stored table
+// entry below it (presence bits of an all-present copy of the table, or the
+// FP64 embedding of an all-ones copy, counted per node).  Any values on the
+// same structure (the plan's own, or rarefied replicates drawn from it) make
+// the node present in a subset of those samples, so bounds taken from it hold
+// for every execution.
+bool PlanImpl::structural_counts(std::vector<uint32_t>& cnt, std::string* err) {
     const uint32_t K = (uint32_t)knodes.size();
-    if (!(theta > 0.0) || K == 0 || n < std::max<uint32_t>(min_n, 2)) return true;
-    DevMem ones, cnt_d;
+    cnt.assign(K, 0);
+    if (!K) return true;
+    DevMem ones, cnt_d, bits_tmp;
     if (!cnt_d.ensure(K * sizeof(uint32_t), err)) return false;
-    if (bits) {
-        if (!embed_bits(true, err)) return false;
-        SSU_HIP(launch_count_bits(cnt_d.as<uint32_t>(), d_B.as<uint32_t>(), ldb, d_knodes.as<int32_t>(), K, st));
+    const uint32_t lb = (uint32_t)(ldn / 32);
+    if (lb <= BITS_MAX_WORDS) {
+        DevMem& B = bits ? d_B : bits_tmp;   // the Gram path keeps its bit rows for the executions
+        if (!embed_bits(B, lb, true, err)) return false;
+        SSU_HIP(launch_count_bits(cnt_d.as<uint32_t>(), B.as<uint32_t>(), lb, d_knodes.as<int32_t>(), K, st));
     } else {
         if (!ones.ensure(std::max<uint64_t>(nnz, 1) * sizeof(double), err)) return false;
         SSU_HIP(launch_fill(ones.as<double>(), nnz, 1.0, st));
@@ -261,16 +261,34 @@ bool PlanImpl::split_gram(std::string* err) {
         }
         x_data = nullptr;
     }
-    std::vector<uint32_t> cnt(K);
     SSU_HIP(hipMemcpyAsync(cnt.data(), cnt_d.p, K * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     SSU_HIP(hipStreamSynchronize(st));
+    return true;
+}
+
+// Gram path occupancy split.  Nodes held by fewer than theta * n samples (by
+// structural occupancy, and of positive length) leave the Gram for the posting
+// lists, each with exactly its structural occupancy as posting slots; nodes
+// held by no sample leave it altogether.  SSU_GRAM_SPLIT sets theta (0: off),
+// SSU_GRAM_SPLIT_MIN_N the smallest table split (GSPLIT_MIN_N).
+bool PlanImpl::split_gram(std::string* err) {
+    double theta = 0.04;
+    if (const char* env = getenv("SSU_GRAM_SPLIT")) theta = atof(env);
+    uint32_t min_n = GSPLIT_MIN_N;
+    if (const char* env = getenv("SSU_GRAM_SPLIT_MIN_N")) min_n = (uint32_t)atol(env);
+    const uint32_t K = (uint32_t)knodes.size();
+    if (!(theta > 0.0) || K == 0 || n < std::max<uint32_t>(min_n, 2)) return true;
+    std::vector<uint32_t> cnt;
+    if (!structural_counts(cnt, err)) return false;
 
     const size_t npan = (n + 31) / 32, panels = ldn / 32;
+    std::vector<uint32_t> off;
     auto classify = [&](uint32_t thr) {
         gram_keep.assign(K, 1);
         sp_node.clear();
         sp_len.clear();
-        uint32_t cap = 0;
+        off.assign(1, 0u);
+        uint64_t total = 0;
         for (uint32_t q = 0; q < K; ++q) {
             const double l = spec.fp64 ? klen_d[q] : (double)klen_f[q];
             if (cnt[q] == 0) { gram_keep[q] = 0; continue; }
@@ -278,41 +296,71 @@ bool PlanImpl::split_gram(std::string* err) {
                 gram_keep[q] = 0;
                 sp_node.push_back(knodes[q]);
                 sp_len.push_back(l);
-                cap = std::max(cap, cnt[q]);
+                total += cnt[q];
+                off.push_back((uint32_t)std::min<uint64_t>(total, 0xffffffffu));
             }
         }
-        return cap;
+        return total;
     };
     // postings (4-byte slots) and row lists (8-byte slots), prefix counts and
     // sample bits; fewer than 2^30 slots (32-bit slot offsets in the kernels)
-    auto fits = [&](uint32_t cap) {
+    auto fits = [&](uint64_t slots) {
         const double ks = (double)sp_node.size();
-        return ks * cap * 12.0 + ks * (double)(npan + 1 + panels) * 4.0 <= GSPLIT_BUDGET &&
-               ks * cap < (double)(1u << 30);
+        return slots * 12.0 + ks * (double)(npan + 1 + panels) * 4.0 <= GSPLIT_BUDGET && slots < (1u << 30);
     };
     uint32_t thr = (uint32_t)std::max(1.0, std::ceil(theta * n));
-    uint32_t cap = classify(thr);
-    while (!fits(cap) && thr > 1) cap = classify(thr /= 2);
-    if (!fits(cap)) cap = classify(0);   // no sparse nodes
+    uint64_t slots = classify(thr);
+    while (!fits(slots) && thr > 1) slots = classify(thr /= 2);
+    if (!fits(slots)) slots = classify(0);   // no sparse nodes
     if (!build_gram_columns()) return false;
-    gsplit_cap = sp_node.empty() ? 0 : cap;
-    if (gsplit_cap) {
+    gsplit_slots = sp_node.empty() ? 0 : (uint32_t)slots;
+    if (!sp_node.empty()) {
         const std::vector<uint32_t> zero((sp_node.size() + 31) / 32, 0u);
-        if (!upload(d_sp_node, sp_node, st, err) || !upload(d_sp_len, sp_len, st, err) || !upload(d_dmask, zero, st, err))
+        if (!upload(d_sp_node, sp_node, st, err) || !upload(d_sp_len, sp_len, st, err) ||
+            !upload(d_dmask, zero, st, err) || !upload(d_poff, off, st, err))
             return false;
-        SSU_HIP(hipStreamSynchronize(st));   // `zero` is read before it goes out of scope
+        SSU_HIP(hipStreamSynchronize(st));   // `zero` and `off` are read before they go out of scope
     }
     return true;
 }
 
-bool PlanImpl::embed_bits(bool structural, std::string* err) {
-    if (!d_B.ensure(std::max<size_t>(m, 1) * ldb * sizeof(uint32_t), err)) return false;
-    uint32_t* B = d_B.as<uint32_t>();
-    SSU_HIP(launch_bits_leaves(B, ldb, d_leaf_k.as<int32_t>(), d_leaf_row.as<int32_t>(), (uint32_t)leaf_k.size(),
+// FP32 shared-node path: nodes held by fewer than theta * n samples are summed
+// from posting lists instead of the 32 x 64 tiles (decided per execution from
+// the actual occupancy).  Their postings at any execution number at most the
+// sum over nodes of min(structural occupancy, thr - 1); that bound sizes the
+// posting and row-list arrays and must stay below 2^30 (32-bit slot offsets in
+// the kernels): theta is lowered to fit.  SSU_SPARSE_SPLIT sets theta (0: off).
+bool PlanImpl::split_weighted(std::string* err) {
+    double theta = 0.25;
+    if (const char* env = getenv("SSU_SPARSE_SPLIT")) theta = atof(env);
+    bool nonneg = true;
+    for (float l : klen_f) nonneg = nonneg && l >= 0.0f;
+    const uint32_t K = (uint32_t)knodes.size();
+    if (!nonneg || !(theta > 0.0) || K == 0 || n < 2) return true;
+    std::vector<uint32_t> cnt;
+    if (!structural_counts(cnt, err)) return false;
+    uint32_t thr = (uint32_t)std::max(1.0, std::ceil(theta * n));
+    auto bound = [&](uint32_t t) {
+        uint64_t b = 0;
+        for (uint32_t c : cnt) b += std::min<uint64_t>(c, t - 1);
+        return b;
+    };
+    uint64_t slots = bound(thr);
+    while (slots >= (1u << 30) && thr > 1) slots = bound(thr /= 2);
+    if (slots >= (1u << 30)) return true;
+    split_thr = thr;
+    split_slots = (uint32_t)std::max<uint64_t>(slots, 1);
+    return true;
+}
+
+bool PlanImpl::embed_bits(DevMem& dst, uint32_t lb, bool structural, std::string* err) {
+    if (!dst.ensure(std::max<size_t>(m, 1) * lb * sizeof(uint32_t), err)) return false;
+    uint32_t* B = dst.as<uint32_t>();
+    SSU_HIP(launch_bits_leaves(B, lb, d_leaf_k.as<int32_t>(), d_leaf_row.as<int32_t>(), (uint32_t)leaf_k.size(),
                                d_indptr.as<uint32_t>(), d_indices.as<uint32_t>(), data_ptr(), structural, st));
     for (size_t h = 0; h + 1 < level_off.size(); ++h) {
         const uint32_t a = level_off[h], b = level_off[h + 1];
-        SSU_HIP(launch_bits_level(B, ldb, d_level_nodes.as<int32_t>() + a, b - a, d_child_ptr.as<int32_t>(),
+        SSU_HIP(launch_bits_level(B, lb, d_level_nodes.as<int32_t>() + a, b - a, d_child_ptr.as<int32_t>(),
                                   d_child_idx.as<int32_t>(), st));
     }
     return true;
@@ -418,22 +466,6 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         for (uint64_t q = 0; q < nz && P.sparse; ++q)
             if (!(table.data[b0 + q] >= 0.0)) P.sparse = false;
     }
-    // Occupancy split (FP32 shared-node path, non-negative lengths): nodes held
-    // by fewer than theta * n samples are summed from posting lists instead of
-    // the 32 x 64 tiles.  Postings and row lists take K x cap slots each (16
-    // bytes per slot together), fewer than 2^30 (16 GiB, and 32-bit slot
-    // offsets in the kernels): theta is lowered to fit.  SSU_SPARSE_SPLIT sets theta (0: off).
-    if (P.sparse && !spec.fp64 && P.n >= 2) {
-        double theta = 0.25;
-        if (const char* env = getenv("SSU_SPARSE_SPLIT")) theta = atof(env);
-        bool nonneg = true;
-        for (float l : P.klen_f) nonneg = nonneg && l >= 0.0f;
-        const uint64_t K = std::max<uint64_t>(P.knodes.size(), 1);
-        const uint64_t max_cap = ((1ull << 30) - 1) / K;
-        if (nonneg && theta > 0.0 && max_cap > 0)
-            P.split_cap = (uint32_t)std::min<uint64_t>(max_cap, (uint64_t)std::max(1.0, std::ceil(theta * P.n)));
-    }
-
     // Embedding blocks: the FP64 embedding of every node for every sample
     // (m x ldn doubles) is built in sample-column blocks when it would exceed
     // EMBED_BUDGET; each block feeds the FP32 operand, masks, sums or presence
@@ -491,6 +523,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         ok = upload(P.d_order, P.block_order, P.st, err);
     }
     if (ok && P.gram) ok = P.split_gram(err);   // may narrow the Gram columns
+    if (ok && P.sparse && !spec.fp64) ok = P.split_weighted(err);
     if (ok && P.gram && P.Kp)
         ok = upload(P.d_col, P.col_node, P.st, err) && upload(P.d_digits, P.digits, P.st, err) &&
              upload(P.d_chunks, P.chunks, P.st, err);
@@ -537,17 +570,20 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         if (!P.d_Spart.ensure(parts * sizeof(double), err)) return false;
     }
     if (P.sparse && !P.d_masks.ensure(std::max<size_t>((P.ldn / 32) * Kw, 1) * sizeof(uint32_t), err)) return false;
-    const bool split = P.sparse && P.split_cap > 0 && K > 0;
+    const bool split = P.sparse && P.split_thr > 0 && K > 0;
     const uint32_t npan = (P.n + 31) / 32;
     if (split) {
         const size_t kk = std::max<size_t>(K, 1);
         if (!P.d_smask.ensure((P.ldn / 32) * kk * sizeof(uint32_t), err) ||
             !P.d_dmask.ensure(std::max<size_t>(Kw, 1) * sizeof(uint32_t), err) ||
             !P.d_pc.ensure((size_t)(npan + 1) * kk * sizeof(uint32_t), err) ||
-            !P.d_pidx.ensure(kk * P.split_cap * sizeof(uint32_t), err) ||
-            !P.d_pval.ensure(kk * P.split_cap * sizeof(float), err) ||
+            !P.d_pidx.ensure((size_t)P.split_slots * sizeof(uint32_t), err) ||
+            !P.d_pval.ensure((size_t)P.split_slots * sizeof(float), err) ||
+            !P.d_poff.ensure((kk + 1) * sizeof(uint32_t), err) ||
+            !P.d_pcnt.ensure(kk * sizeof(uint32_t), err) ||
+            !P.d_obsum.ensure((offset_scan_blocks(K) + 1) * sizeof(uint32_t), err) ||
             !P.d_walked.ensure((size_t)sparse_rows_blocks(P.n) * sizeof(uint32_t), err) ||
-            !P.d_rows.ensure(kk * P.split_cap * sizeof(unsigned long long), err) ||
+            !P.d_rows.ensure((size_t)P.split_slots * sizeof(unsigned long long), err) ||
             !P.d_rcount.ensure((size_t)P.n * row_list_chunks(K) * sizeof(uint32_t), err) ||
             !P.d_roff.ensure(((size_t)P.n * row_list_chunks(K) + 1) * sizeof(uint32_t), err) ||
             !P.d_rbsum.ensure((row_list_scan_blocks(P.n, K) + 1) * sizeof(uint32_t), err))
@@ -556,15 +592,15 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
     // Gram path split: sample bits of the sparse nodes from each embedding block,
     // posting and row lists before the Gram, each row's share added to G after it
     const uint32_t Ks = (uint32_t)P.sp_node.size();
-    const bool gsplit = P.gram && P.gsplit_cap > 0 && Ks > 0;
+    const bool gsplit = P.gram && P.gsplit_slots > 0 && Ks > 0;
     if (gsplit) {
         const size_t ks = Ks, kws = (ks + 31) / 32, panels = P.ldn / 32;
         if (!P.d_umasks.ensure(panels * kws * sizeof(uint32_t), err) ||
             !P.d_smask.ensure(panels * ks * sizeof(uint32_t), err) ||
             !P.d_pc.ensure((size_t)(npan + 1) * ks * sizeof(uint32_t), err) ||
-            !P.d_pidx.ensure(ks * P.gsplit_cap * sizeof(uint32_t), err) ||
+            !P.d_pidx.ensure((size_t)P.gsplit_slots * sizeof(uint32_t), err) ||
             !P.d_walked.ensure((size_t)sparse_rows_blocks(P.n) * sizeof(uint32_t), err) ||
-            !P.d_rows.ensure(ks * P.gsplit_cap * sizeof(unsigned long long), err) ||
+            !P.d_rows.ensure((size_t)P.gsplit_slots * sizeof(unsigned long long), err) ||
             !P.d_rcount.ensure((size_t)P.n * row_list_chunks(Ks) * sizeof(uint32_t), err) ||
             !P.d_roff.ensure(((size_t)P.n * row_list_chunks(Ks) + 1) * sizeof(uint32_t), err) ||
             !P.d_rbsum.ensure((row_list_scan_blocks(P.n, Ks) + 1) * sizeof(uint32_t), err))
@@ -579,7 +615,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
 
     SSU_HIP(P.mark(0));
     if (P.bits) {   // Gram path on presence bits: one block of every sample
-        if (!P.embed_bits(false, err)) return false;
+        if (!P.embed_bits(P.d_B, P.ldb, false, err)) return false;
         SSU_HIP(P.mark(1));
         if (P.Kp)
             SSU_HIP(launch_presence_T_bits(P.d_Pt.as<int8_t>(), P.Kp, P.d_B.as<uint32_t>(), P.ldb,
@@ -634,7 +670,11 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         b.K = K;
         b.n = P.n;
         b.npan = npan;
-        b.cap = P.split_cap;
+        b.poff = P.d_poff.as<uint32_t>();
+        b.pcnt = P.d_pcnt.as<uint32_t>();
+        b.obsum = P.d_obsum.as<uint32_t>();
+        b.slots = P.split_slots;
+        b.thr = P.split_thr;
         SSU_HIP(launch_sparse_nodes(b, P.st));
     }
     SparseNodesArgs gb{};
@@ -653,7 +693,8 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         gb.K = Ks;
         gb.n = P.n;
         gb.npan = npan;
-        gb.cap = P.gsplit_cap;
+        gb.poff = P.d_poff.as<uint32_t>();
+        gb.slots = P.gsplit_slots;
         gb.gram = P.d_G.as<double>();
         gb.n_pad = (uint32_t)P.n_pad;
         SSU_HIP(launch_sparse_lists(gb, P.st));
@@ -735,6 +776,14 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
     for (int i = 0; i < PH_TOTAL; ++i) SSU_HIP(hipEventElapsedTime(&P.times[i], P.ev[i], P.ev[i + 1]));
     SSU_HIP(hipEventElapsedTime(&P.times[PH_TOTAL], P.ev[0], P.ev[5]));
     P.times[PH_SPARSE] = 0.0f;
+    if (split) {   // the structural bound holds by construction; a breach would have skipped the lists
+        uint32_t total = 0;
+        SSU_HIP(hipMemcpy(&total, P.d_poff.as<uint32_t>() + K, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (total > P.split_slots) {
+            if (err) *err = "posting lists exceed the plan's structural bound";
+            return false;
+        }
+    }
     if (split || gsplit) {   // prep ends where the posting-list phase starts
         SSU_HIP(hipEventElapsedTime(&P.times[PH_PREP], P.ev[1], P.ev[EV_SPLIT]));
         SSU_HIP(hipEventElapsedTime(&P.times[PH_SPARSE], P.ev[EV_SPLIT], P.ev[2]));
@@ -821,7 +870,7 @@ uint64_t Plan::kernel_terms() const {
 }
 uint64_t Plan::sparse_terms() const {
     const PlanImpl& P = *p_;
-    if (!((P.sparse && P.split_cap) || (P.gram && P.gsplit_cap)) || !P.d_walked.p) return 0;
+    if (!((P.sparse && P.split_thr) || (P.gram && P.gsplit_slots)) || !P.d_walked.p) return 0;
     std::vector<uint32_t> wk(sparse_rows_blocks(P.n));
     if (hipMemcpy(wk.data(), P.d_walked.p, wk.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) return 0;
     uint64_t sum = 0;

@@ -1038,37 +1038,42 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 
 // Node occupancy split: a node held by at least `thr` samples is dense (its
 // bit set in dmask, it stays with the tiled shared-node kernel); the rest go
-// to the posting-list kernels.  One thread per node; smask is [npan][K].
-__global__ void __launch_bounds__(64) k_node_split(uint32_t* __restrict__ dmask, const uint32_t* __restrict__ smask,
-                                                   uint32_t K, uint32_t npan, uint32_t thr) {
+// to the posting-list kernels, pcnt[r] = their sample counts (0 for dense
+// nodes).  One thread per node; smask is [npan][K].
+__global__ void __launch_bounds__(64) k_node_split(uint32_t* __restrict__ dmask, uint32_t* __restrict__ pcnt,
+                                                   const uint32_t* __restrict__ smask, uint32_t K, uint32_t npan,
+                                                   uint32_t thr) {
     const uint32_t r = blockIdx.x * 64 + threadIdx.x;
     uint32_t cnt = 0;
     if (r < K)
         for (uint32_t p = 0; p < npan; ++p) cnt += __popc(smask[(size_t)p * K + r]);
+    if (r < K) pcnt[r] = cnt < thr ? cnt : 0u;
     const uint64_t b = __ballot(r < K && cnt >= thr);
     const uint32_t w = blockIdx.x * 2 + (threadIdx.x >> 5);
     if ((threadIdx.x & 31) == 0 && w * 32 < K) dmask[w] = (threadIdx.x < 32) ? (uint32_t)b : (uint32_t)(b >> 32);
 }
 
 // Posting lists of the sparse nodes: node r's samples in increasing order
-// (pidx) with their FP32 values (pval, when non-null), at r * cap;
-// pc[p * K + r] = number of node r's samples below 32 p, for p = 0..npan.
-// One thread per node.
+// (pidx) with their FP32 values (pval, when non-null), in slots
+// [poff[r], poff[r + 1]); pc[p * K + r] = number of node r's samples below
+// 32 p, for p = 0..npan.  One thread per node.
 template <class IX>
 __global__ void __launch_bounds__(64) k_postings(IX* __restrict__ pidx, float* __restrict__ pval,
                                                  uint32_t* __restrict__ pc, const uint32_t* __restrict__ smask,
                                                  const uint32_t* __restrict__ dmask, const float* __restrict__ U,
-                                                 size_t ldu, uint32_t K, uint32_t npan, uint32_t cap) {
+                                                 size_t ldu, uint32_t K, uint32_t npan,
+                                                 const uint32_t* __restrict__ poff, uint32_t slots) {
     const uint32_t r = blockIdx.x * 64 + threadIdx.x;
-    if (r >= K || ((dmask[r >> 5] >> (r & 31)) & 1u)) return;
+    if (r >= K || ((dmask[r >> 5] >> (r & 31)) & 1u) || poff[K] > slots) return;   // (the last: never, by the bound)
     const float* ur = U ? U + (size_t)r * ldu : nullptr;
-    IX* ix = pidx + (size_t)r * cap;
-    float* vx = pval ? pval + (size_t)r * cap : nullptr;
+    const uint32_t o = poff[r], cap = poff[r + 1] - o;
+    IX* ix = pidx + o;
+    float* vx = pval ? pval + o : nullptr;
     uint32_t cnt = 0;
     for (uint32_t p = 0; p < npan; ++p) {
         pc[(size_t)p * K + r] = cnt;
         uint32_t w = smask[(size_t)p * K + r];
-        while (w && cnt < cap) {   // cnt < cap holds for every sparse node (fewer than thr = cap samples)
+        while (w && cnt < cap) {   // the slots hold every sample of the node; the bound guards memory
             const uint32_t s = p * 32 + (uint32_t)(__ffs(w) - 1);
             ix[cnt] = (IX)s;
             if (vx) vx[cnt] = ur[s];
@@ -1108,8 +1113,10 @@ __global__ void __launch_bounds__(64) k_row_lists(unsigned long long* __restrict
                                                   const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ smask,
                                                   const uint32_t* __restrict__ dmask, const uint32_t* __restrict__ pc,
-                                                  uint32_t K, uint32_t n, uint32_t nkc) {
+                                                  uint32_t K, uint32_t n, uint32_t nkc,
+                                                  const uint32_t* __restrict__ poff, uint32_t slots) {
     const uint32_t P = blockIdx.x, kc = blockIdx.y, lane = threadIdx.x;
+    if (rows && poff[K] > slots) return;   // more postings than slots: never, by the plan's bound
     const uint32_t a_lane = P * 32 + (lane & 31);
     uint32_t len = 0;   // lane s < 32: entries so far of sample 32 P + s in this chunk
     unsigned long long* dst =
@@ -1226,7 +1233,7 @@ constexpr uint32_t SR_WAVES = 8;   // waves sharing one accumulator, each taking
 // (S[a] weighted; the FP64 sum of the row list's lengths unweighted).  Integer
 // adds make the result independent of the order in which waves and lanes add,
 // and two samples holding the same nodes get bit-equal G(a, b) and G(a, a).
-// Posting slots are addressed with 32-bit offsets (K * cap < 2^30).
+// Posting slots are addressed with 32-bit offsets (fewer than 2^30 slots).
 template <class TOut, class IX, bool GRAM>
 __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__ out, uint32_t* __restrict__ walked,
                                                    const unsigned long long* __restrict__ rows,
@@ -1234,15 +1241,17 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                                                    const uint32_t* __restrict__ pc,
                                                    const IX* __restrict__ pidx, const float* __restrict__ pval,
                                                    const double* __restrict__ len, const double* __restrict__ S,
-                                                   uint32_t K, uint32_t n, uint32_t npan, uint32_t cap,
+                                                   uint32_t K, uint32_t n, uint32_t npan,
+                                                   const uint32_t* __restrict__ poff, uint32_t slots,
                                                    uint32_t nchunks, uint32_t nkc, double* __restrict__ G,
                                                    uint32_t tiles) {
     // chunk-major: the rows in flight share one column chunk of every posting
     // list, a quarter of the posting data at 16k samples (cache-resident)
     const uint32_t a = blockIdx.x % n, c = blockIdx.x / n;
     const uint32_t b0 = c * SR_CW, b1 = min(n, b0 + SR_CW);
-    // uniform: nothing at or above the diagonal here (weighted: strictly above)
-    if (a >= n || (GRAM ? b1 <= a : b1 <= a + 1)) {
+    // uniform: nothing at or above the diagonal here (weighted: strictly above); more
+    // postings than slots never happens (the plan's bound) and is skipped, not read
+    if (a >= n || (GRAM ? b1 <= a : b1 <= a + 1) || poff[K] > slots) {
         if (walked && threadIdx.x == 0) walked[blockIdx.x] = 0;
         return;
     }
@@ -1290,7 +1299,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     const uint32_t pb0 = b0 >> 5, pb1 = b1 == n ? npan : (b1 >> 5);
     uint32_t total_walked = 0;
     for (uint32_t r0 = e0 + 64 * wv; r0 < e1; r0 += 64 * SR_WAVES) {   // uniform per wave
-        uint32_t start = 0, end = 0, r = 0;
+        uint32_t start = 0, end = 0, r = 0, slot0 = 0;
         float ua = 0.0f;
         unsigned long long q = 0ull;
         if (r0 + lane < e1) {
@@ -1300,10 +1309,11 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             const uint32_t cs = pc[(size_t)pb0 * K + r], ce = pc[(size_t)pb1 * K + r];
             start = max(pa + 1, cs);
             end = ce;
+            slot0 = poff[r];
             if constexpr (GRAM) {
                 q = to_fixed(len[r] * scale);
             } else {
-                ua = pval[r * cap + pa];
+                ua = pval[slot0 + pa];
                 q = (unsigned long long)__double_as_longlong(len[r] * scale);
             }
         }
@@ -1318,7 +1328,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         if (T == 0) continue;   // uniform per wave
         pre[lane] = incl - cnt;
         if (lane == 63) pre[64] = incl;
-        base[lane] = r * cap + start - (incl - cnt);   // slot of walk position t: base + t (mod 2^32)
+        base[lane] = slot0 + start - (incl - cnt);   // slot of walk position t: base + t (mod 2^32)
         if constexpr (!GRAM) lu[lane] = ua;
         q8[lane] = q;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1636,39 +1646,48 @@ static void pairs_sparse32_t(uint32_t grid, hipStream_t st, const SparseArgs& a,
 
 static bool sparse_args_ok(const SparseNodesArgs& a) {
     // posting slots are addressed with 32-bit offsets (bytes included)
-    const bool common = a.dmask && a.smask && a.pc && a.pidx && a.len && a.cap && a.rows && a.rcount && a.roff &&
-                        a.rbsum && a.npan == (a.n + 31) / 32 && (uint64_t)a.K * a.cap < (1ull << 30);
+    const bool common = a.dmask && a.smask && a.pc && a.pidx && a.poff && a.len && a.rows && a.rcount && a.roff &&
+                        a.rbsum && a.npan == (a.n + 31) / 32 && a.slots < (1u << 30);
     if (!common) return false;
     if (a.gram) return a.n_pad >= a.n && a.n_pad % GRAM_BM == 0;   // unweighted: values, sums and out unused
-    return a.pval && a.U && a.S && a.out && !a.out_fp64 && a.ldu >= (size_t)a.npan * 32;
+    return a.pcnt && a.obsum && a.thr && a.pval && a.U && a.S && a.out && !a.out_fp64 &&
+           a.ldu >= (size_t)a.npan * 32;
 }
 
 hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
     if (!a.K || !a.n) return hipSuccess;
     if (!sparse_args_ok(a)) return hipErrorInvalidValue;
     const uint32_t gk = (a.K + 63) / 64;
-    // weighted: dense nodes by occupancy, now; unweighted: all K nodes are sparse (dmask zero since plan creation)
-    if (!a.gram) hipLaunchKernelGGL(k_node_split, dim3(gk), dim3(64), 0, st, a.dmask, a.smask, a.K, a.npan, a.cap);
+    // weighted: dense nodes by occupancy, now, and the posting offsets as the scan of the sparse
+    // nodes' counts (at most `slots` by the plan's structural bound); unweighted: all K nodes are
+    // sparse (dmask zero) with offsets fixed since plan creation
+    if (!a.gram) {
+        const uint32_t nb = (uint32_t)((a.K + SCAN_BLOCK - 1) / SCAN_BLOCK);
+        hipLaunchKernelGGL(k_node_split, dim3(gk), dim3(64), 0, st, a.dmask, a.pcnt, a.smask, a.K, a.npan, a.thr);
+        hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(SCAN_T), 0, st, a.obsum, a.pcnt, (size_t)a.K);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, a.obsum, nb);
+        hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_T), 0, st, a.poff, a.pcnt, a.obsum, (size_t)a.K, nb);
+    }
     const bool narrow = a.n <= 65536;   // sample indices fit 16 bits: 6 bytes per posting instead of 8
     const float* U = a.gram ? nullptr : a.U;
     float* pval = a.gram ? nullptr : a.pval;
     if (narrow)
         hipLaunchKernelGGL(k_postings<uint16_t>, dim3(gk), dim3(64), 0, st, (uint16_t*)a.pidx, pval, a.pc, a.smask,
-                           a.dmask, U, a.ldu, a.K, a.npan, a.cap);
+                           a.dmask, U, a.ldu, a.K, a.npan, (const uint32_t*)a.poff, a.slots);
     else
         hipLaunchKernelGGL(k_postings<uint32_t>, dim3(gk), dim3(64), 0, st, (uint32_t*)a.pidx, pval, a.pc, a.smask,
-                           a.dmask, U, a.ldu, a.K, a.npan, a.cap);
+                           a.dmask, U, a.ldu, a.K, a.npan, (const uint32_t*)a.poff, a.slots);
     // row lists: counts per (sample, node chunk), their exclusive scan, then the entries
     const uint32_t nkc = row_list_chunks(a.K);
     const size_t m = (size_t)a.n * nkc;
     const uint32_t nsb = (uint32_t)((m + SCAN_BLOCK - 1) / SCAN_BLOCK);
     hipLaunchKernelGGL(k_row_lists, dim3(a.npan, nkc), dim3(64), 0, st, nullptr, a.rcount, nullptr, a.smask, a.dmask,
-                       a.pc, a.K, a.n, nkc);
+                       a.pc, a.K, a.n, nkc, (const uint32_t*)a.poff, a.slots);
     hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_T), 0, st, a.rbsum, a.rcount, m);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, a.rbsum, nsb);
     hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_T), 0, st, a.roff, a.rcount, a.rbsum, m, nsb);
     hipLaunchKernelGGL(k_row_lists, dim3(a.npan, nkc), dim3(64), 0, st, a.rows, nullptr, a.roff, a.smask, a.dmask,
-                       a.pc, a.K, a.n, nkc);
+                       a.pc, a.K, a.n, nkc, (const uint32_t*)a.poff, a.slots);
     return hipGetLastError();
 }
 
@@ -1676,7 +1695,8 @@ template <class IX, bool GRAM>
 static void pairs_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs& a, uint32_t nchunks, uint32_t nkc) {
     hipLaunchKernelGGL((k_pairs_rows<float, IX, GRAM>), dim3(blocks), dim3(64 * SR_WAVES), 0, st,
                        GRAM ? nullptr : (float*)a.out, a.walked, a.rows, a.roff, a.pc, (const IX*)a.pidx,
-                       GRAM ? nullptr : a.pval, a.len, GRAM ? nullptr : a.S, a.K, a.n, a.npan, a.cap, nchunks, nkc,
+                       GRAM ? nullptr : a.pval, a.len, GRAM ? nullptr : a.S, a.K, a.n, a.npan, (const uint32_t*)a.poff,
+                       a.slots, nchunks, nkc,
                        a.gram, a.n_pad / GRAM_BM);
 }
 
@@ -1714,6 +1734,7 @@ hipError_t launch_count_present(uint32_t* cnt, const double* E, size_t lde, uint
 
 uint32_t sparse_rows_blocks(uint32_t n) { return n * ((n + SR_CW - 1) / SR_CW); }
 uint32_t row_list_chunks(uint32_t K) { return std::max<uint32_t>(1, (K + RL_KC - 1) / RL_KC); }
+size_t offset_scan_blocks(uint32_t K) { return ((size_t)K + SCAN_BLOCK - 1) / SCAN_BLOCK; }
 size_t row_list_scan_blocks(uint32_t n, uint32_t K) {
     return ((size_t)n * row_list_chunks(K) + SCAN_BLOCK - 1) / SCAN_BLOCK;
 }

@@ -135,20 +135,28 @@ struct SparseArgs {
 };
 
 // Occupancy split.  Weighted (FP32 shared-node path): nodes held by at least
-// `cap` samples are dense (bit set in dmask, [ceil(K/32)] words, set here by
+// `thr` samples are dense (bit set in dmask, [ceil(K/32)] words, set here by
 // launch_sparse_lists); for the others the posting lists are built from U and
-// the sample masks, and each row's share of M over them is written to
-// out[a * n + b] (b > a) for the tiled kernel to add (SparseArgs::dmask).
-// Unweighted (gram != nullptr): the K nodes are the plan's sparse nodes, fixed
-// when the plan is built (dmask all zero, U and pval unused); each row's share
-// of the shared length sum_r len[r] [a holds r][b holds r] is added to the
-// FP64 Gram tile entries G(a, b), b >= a, after launch_gram_i8 has written them.
+// the sample masks at offsets scanned from their counts, and each row's share
+// of M over them is written to out[a * n + b] (b > a) for the tiled kernel to
+// add (SparseArgs::dmask).  Unweighted (gram != nullptr): the K nodes are the
+// plan's sparse nodes with offsets fixed when the plan is built (dmask all
+// zero, U and pval unused); each row's share of the shared length
+// sum_r len[r] [a holds r][b holds r] is added to the FP64 Gram tile entries
+// G(a, b), b >= a, after launch_gram_i8 has written them.  Posting and
+// row-list arrays hold `slots` entries, which must bound the postings of any
+// execution (fewer than 2^30: 32-bit slot offsets).
 struct SparseNodesArgs {
     uint32_t* dmask;
-    const uint32_t* smask;  // [npan][K] sample bits, from launch_gather_sums
+    const uint32_t* smask;  // [npan][K] sample bits, from launch_gather_sums / launch_bits_smask
     uint32_t* pc;           // [npan + 1][K]
-    void* pidx;             // [K][cap] sample indices: u16 when n <= 65,536, else u32
-    float* pval;            // [K][cap] (weighted)
+    void* pidx;             // [slots] sample indices: u16 when n <= 65,536, else u32
+    float* pval;            // [slots] (weighted)
+    uint32_t* poff;         // [K + 1] posting offsets (weighted: written here; unweighted: given)
+    uint32_t* pcnt;         // [K] (weighted) sparse nodes' sample counts
+    uint32_t* obsum;        // [ceil(K / 4096) + 1] (weighted) scan scratch
+    uint32_t slots;
+    uint32_t thr;           // (weighted) dense from this many samples
     const float* U;         // [K][ldu] (weighted)
     size_t ldu;
     const double* len;      // [K]
@@ -156,11 +164,11 @@ struct SparseNodesArgs {
     void* out;              // (weighted)
     bool out_fp64;
     uint32_t* walked;       // optional [sparse_rows_blocks(n)]: postings walked per wave
-    unsigned long long* rows;   // [K * cap] row-list entries (node << 32 | position in its postings)
+    unsigned long long* rows;   // [slots] row-list entries (node << 32 | position in its postings)
     uint32_t* rcount;       // [n * row_list_chunks(K)]
     uint32_t* roff;         // [n * row_list_chunks(K) + 1]
     uint32_t* rbsum;        // [row_list_scan_blocks(n, K) + 1]
-    uint32_t K, n, npan, cap;
+    uint32_t K, n, npan;
     double* gram = nullptr; // unweighted: the Gram tiles of launch_gram_i8
     uint32_t n_pad = 0;     // their padded edge
 };
@@ -175,6 +183,7 @@ hipError_t launch_sparse_nodes(const SparseNodesArgs& a, hipStream_t st);
 hipError_t launch_count_present(uint32_t* cnt, const double* E, size_t lde, uint32_t col0, const int32_t* knodes,
                                 uint32_t K, uint32_t n, hipStream_t st);
 uint32_t sparse_rows_blocks(uint32_t n);
+size_t offset_scan_blocks(uint32_t K);
 uint32_t row_list_chunks(uint32_t K);
 size_t row_list_scan_blocks(uint32_t n, uint32_t K);
 // The upper-triangle blocks (bi <= bj < nb) as bi << 16 | bj, super x super
