@@ -1218,6 +1218,8 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(uint32_t* __restrict__ ou
 
 constexpr uint32_t SR_CW = 4096;   // output columns per workgroup (64-bit LDS accumulators, 32 KB)
 constexpr uint32_t SR_WAVES = 8;   // waves sharing one accumulator, each taking every 8th round of entries
+constexpr uint32_t SR_UNROLL = 4;  // postings per lane with their loads in flight together
+constexpr uint32_t SR_WIN = 1024;  // walk positions resolved to entries at a time (64 lanes x 16 bytes)
 
 // Sparse-node share for row a.  Weighted: sum over sparse nodes r held by
 // both a and b > a of l_r * min(u_ra, u_rb), written as FP32 into
@@ -1227,7 +1229,8 @@ constexpr uint32_t SR_WAVES = 8;   // waves sharing one accumulator, each taking
 // waves per (row a, 4096-column chunk c), blockIdx = c * n + a.  Row a's list
 // is taken 64 entries at a time (round q goes to wave q % 8); a round's
 // posting ranges inside the chunk (past a) are walked with the wave's 64 lanes
-// spread evenly over the concatenated postings.  Terms are accumulated in fixed point, integers in
+// spread evenly over the concatenated postings, each position's entry read
+// from a per-wave byte map (entry starts marked, then a prefix max).  Terms are accumulated in fixed point, integers in
 // units of 2^(e - 51) (weighted: the FP64 product l_r * min, rounded to
 // nearest) or 2^(e - 61) (GRAM: l_r, truncated) where 2^e bounds the row's sum
 // (S[a] weighted; the FP64 sum of the row list's lengths unweighted).  Integer
@@ -1257,12 +1260,14 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     }
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     __shared__ unsigned long long acc[SR_CW];
-    __shared__ uint32_t pre_s[SR_WAVES][65], base_s[SR_WAVES][64], wk[SR_WAVES];
+    __shared__ uint32_t base_s[SR_WAVES][64], wk[SR_WAVES];
     __shared__ float lu_s[SR_WAVES][GRAM ? 1 : 64];
     // per entry: GRAM the fixed-point term, weighted l_r * scale (FP64 bits)
     __shared__ unsigned long long q8_s[SR_WAVES][64], diag_s;
     __shared__ double bsum_s[SR_WAVES];
-    uint32_t* pre = pre_s[wv];
+    __shared__ uint4 own_s[SR_WAVES][SR_WIN / 16];   // per wave: entry + 1 of every window position
+    uint4* own4 = own_s[wv];
+    uint8_t* own = reinterpret_cast<uint8_t*>(own4);
     uint32_t* base = base_s[wv];
     float* lu = lu_s[wv];
     unsigned long long* q8 = q8_s[wv];
@@ -1326,43 +1331,83 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         }
         const uint32_t T = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
         if (T == 0) continue;   // uniform per wave
-        pre[lane] = incl - cnt;
-        if (lane == 63) pre[64] = incl;
         base[lane] = slot0 + start - (incl - cnt);   // slot of walk position t: base + t (mod 2^32)
         if constexpr (!GRAM) lu[lane] = ua;
         q8[lane] = q;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // four postings per lane in flight: slots first, then every load, then the adds
-        uint32_t e = 0;
-        for (uint32_t t0 = lane; t0 < T; t0 += 256) {
-            uint32_t qq[4], eq[4];
+        // The walk positions [0, T) go SR_WIN at a time.  Each non-empty entry marks
+        // its first position (entry + 1) in the wave's window bytes; a prefix max
+        // (starts increase with the entry index) then names every position's
+        // entry, so a lane finds its postings without searching.
+        const uint32_t my_pre = incl - cnt;
+        uint32_t carry = 0;   // entry + 1 owning the position before the window
+        for (uint32_t w0 = 0; w0 < T; w0 += SR_WIN) {   // uniform per wave
+            const uint32_t wn = min(SR_WIN, T - w0);
+            own4[lane] = make_uint4(0u, 0u, 0u, 0u);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (cnt > 0 && my_pre >= w0 && my_pre < w0 + SR_WIN) own[my_pre - w0] = (uint8_t)(lane + 1);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // lane l resolves bytes [16 l, 16 l + 16): running max, then a wave prefix max
+            uint4 v4 = own4[lane];
+            uint32_t wd[4] = {v4.x, v4.y, v4.z, v4.w};
+            uint32_t run = 0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint32_t t = min(t0 + 64 * u, T - 1);
-                while (pre[e + 1] <= t) ++e;
-                eq[u] = e;
-                qq[u] = base[e] + t;
+            for (int i = 0; i < 16; ++i) run = max(run, (wd[i >> 2] >> (8 * (i & 3))) & 0xffu);
+            uint32_t incl_max = run;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(incl_max, d, 64);
+                if ((int)lane >= d) incl_max = max(incl_max, y);
             }
-            uint32_t bq[4];
-            float vq[4];
+            uint32_t cur = max(carry, __shfl_up(incl_max, 1, 64));
+            if (lane == 0) cur = carry;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                bq[u] = (uint32_t)pidx[qq[u]];
-                if constexpr (!GRAM) vq[u] = pval[qq[u]];
+            for (int i = 0; i < 16; ++i) {
+                cur = max(cur, (wd[i >> 2] >> (8 * (i & 3))) & 0xffu);
+                wd[i >> 2] = (wd[i >> 2] & ~(0xffu << (8 * (i & 3)))) | (cur << (8 * (i & 3)));
             }
+            own4[lane] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+            carry = max(carry, (uint32_t)__builtin_amdgcn_readlane((int)incl_max, 63));
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // SR_UNROLL postings per lane in flight: slots first, then every load, then the adds
+            for (uint32_t j0 = lane; j0 < wn; j0 += 64 * SR_UNROLL) {
+                uint32_t qq[SR_UNROLL], eq[SR_UNROLL];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (t0 + 64 * u < T) {
-                    if constexpr (GRAM) {
-                        atomicAdd(&acc[bq[u] - b0], q8[eq[u]]);
-                    } else {
-                        // u >= 0: the smaller float has the smaller bit pattern
-                        const float m = __uint_as_float(min(__float_as_uint(lu[eq[u]]), __float_as_uint(vq[u])));
-                        atomicAdd(&acc[bq[u] - b0], fixed_term(m, __longlong_as_double((long long)q8[eq[u]])));
-                    }
+                for (uint32_t u = 0; u < SR_UNROLL; ++u) {
+                    const uint32_t j = min(j0 + 64 * u, wn - 1);
+                    eq[u] = (uint32_t)own[j] - 1u;
+                    qq[u] = base[eq[u]] + w0 + j;
                 }
+                uint32_t bq[SR_UNROLL];
+                float vq[SR_UNROLL];
+#pragma unroll
+                for (uint32_t u = 0; u < SR_UNROLL; ++u) {
+                    bq[u] = (uint32_t)pidx[qq[u]];
+                    if constexpr (!GRAM) vq[u] = pval[qq[u]];
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < SR_UNROLL; ++u)
+                    if (j0 + 64 * u < wn) {
+                        if constexpr (GRAM) {
+                            atomicAdd(&acc[bq[u] - b0], q8[eq[u]]);
+                        } else {
+                            // u >= 0: the smaller float has the smaller bit pattern
+                            const float m = __uint_as_float(min(__float_as_uint(lu[eq[u]]), __float_as_uint(vq[u])));
+                            atomicAdd(&acc[bq[u] - b0], fixed_term(m, __longlong_as_double((long long)q8[eq[u]])));
+                        }
+                    }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         total_walked += T;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
