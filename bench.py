@@ -298,7 +298,7 @@ def main():
     def check(name, p):
         """max relative error of the plan's matrix on a 48-sample subset, against the FP64 oracle"""
         m = p.matrix()
-        ref = O.matrix(otree, O.Table(*sub), specs[name][0], **specs[name][1])
+        ref = O.matrix(otree, O.Table(*sub), specs[name][0].replace("_fp64", ""), **specs[name][1])
         errors[name] = max_rel_err(m[np.ix_(idx, idx)], ref)
 
     times = {k: [] for k in specs}

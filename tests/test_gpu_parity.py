@@ -251,21 +251,24 @@ def test_weighted_shared_node_kernel_matches_dense(ssu, O, synth, method, prec):
     pd.close()
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("theta", ["0", "0.05", "0.3", "1.0"])
-def test_weighted_occupancy_split(ssu, O, synth, monkeypatch, theta):
-    """FP32 shared-node path with nodes held by fewer than theta * n samples
-    summed from posting lists (fixed-point row sums) and added in the tiled
-    kernel's epilogue; theta 0 turns the split off, 1.0 sends almost every node
-    to the posting lists.  Repeated executions are bitwise identical."""
+def test_weighted_occupancy_split(ssu, O, synth, monkeypatch, theta, prec):
+    """Shared-node path with nodes held by fewer than theta * n samples summed
+    from posting lists (fixed-point row sums; FP32 or FP64 values) and added in
+    the tiled kernel's epilogue; theta 0 turns the split off, 1.0 sends almost
+    every node to the posting lists.  Repeated executions are bitwise identical."""
     tree, table = _case(synth, 500, 300, 0.05, 73, zipf=1.1)
     monkeypatch.setenv("SSU_SPARSE_SPLIT", theta)
     inp = inputs_from(ssu, tree, table)
     for method in ("weighted_normalized", "weighted_unnormalized"):
-        say("occupancy split", theta, method)
-        p = ssu.Plan(inp, method)
+        say("occupancy split", theta, method, prec)
+        p = ssu.Plan(inp, method + ("_fp64" if prec == "fp64" else ""))
         p.execute()
         m = p.matrix()
-        assert rel_err(m, oracle_matrix(O, tree, table, method)) <= TOL["fp32"]
+        assert rel_err(m, oracle_matrix(O, tree, table, method)) <= TOL[prec]
+        if theta != "0":
+            assert p.sparse_terms() > 0
         assert np.array_equal(m, m.T) and np.all(np.diag(m) == 0)
         p.execute()
         assert np.array_equal(p.matrix(), m)
@@ -287,25 +290,27 @@ def _sub_table(table, idx):
     return [obs_ids[i] for i in used], [sample_ids[i] for i in idx], ip, c.astype(np.uint32), v
 
 
-def test_weighted_split_across_row_chunks(ssu, O, synth, monkeypatch):
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_weighted_split_across_row_chunks(ssu, O, synth, monkeypatch, prec):
     """4,500 samples: rows span two 4,096-column chunks of the posting-list
     kernel.  Split against no split on the device, and a 48-sample subset
     against the FP64 oracle."""
     tree, table = _case(synth, 300, 4500, 0.04, 79)
     inp = inputs_from(ssu, tree, table)
+    name = "weighted_normalized" + ("_fp64" if prec == "fp64" else "")
     out = {}
     for theta in ("0", "0.2"):
         monkeypatch.setenv("SSU_SPARSE_SPLIT", theta)
-        say("row chunks", theta)
-        p = ssu.Plan(inp, "weighted_normalized")
+        say("row chunks", theta, prec)
+        p = ssu.Plan(inp, name)
         p.execute()
         out[theta] = p.matrix()
         p.close()
-    assert rel_err(out["0.2"], out["0"]) <= 2e-6
+    assert rel_err(out["0.2"], out["0"]) <= (2e-6 if prec == "fp32" else 1e-12)
     idx = np.sort(np.random.default_rng(5).choice(4500, 48, replace=False))
     idx[-1] = 4499
     ref = oracle_matrix(O, tree, _sub_table(table, idx), "weighted_normalized")
-    assert rel_err(out["0.2"][np.ix_(idx, idx)], ref) <= TOL["fp32"]
+    assert rel_err(out["0.2"][np.ix_(idx, idx)], ref) <= TOL[prec]
 
 
 def test_weighted_duplicate_samples(ssu, O, synth):

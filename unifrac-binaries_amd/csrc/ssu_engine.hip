@@ -324,7 +324,7 @@ bool PlanImpl::split_gram(std::string* err) {
     return true;
 }
 
-// FP32 shared-node path: nodes held by fewer than theta * n samples are summed
+// Shared-node weighted path: nodes held by fewer than theta * n samples are summed
 // from posting lists instead of the 32 x 64 tiles (decided per execution from
 // the actual occupancy).  Their postings at any execution number at most the
 // sum over nodes of min(structural occupancy, thr - 1); that bound sizes the
@@ -334,7 +334,7 @@ bool PlanImpl::split_weighted(std::string* err) {
     double theta = 0.25;
     if (const char* env = getenv("SSU_SPARSE_SPLIT")) theta = atof(env);
     bool nonneg = true;
-    for (float l : klen_f) nonneg = nonneg && l >= 0.0f;
+    for (size_t q = 0; q < knodes.size(); ++q) nonneg = nonneg && (spec.fp64 ? klen_d[q] >= 0.0 : klen_f[q] >= 0.0f);
     const uint32_t K = (uint32_t)knodes.size();
     if (!nonneg || !(theta > 0.0) || K == 0 || n < 2) return true;
     std::vector<uint32_t> cnt;
@@ -523,7 +523,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         ok = upload(P.d_order, P.block_order, P.st, err);
     }
     if (ok && P.gram) ok = P.split_gram(err);   // may narrow the Gram columns
-    if (ok && P.sparse && !spec.fp64) ok = P.split_weighted(err);
+    if (ok && P.sparse) ok = P.split_weighted(err);
     if (ok && P.gram && P.Kp)
         ok = upload(P.d_col, P.col_node, P.st, err) && upload(P.d_digits, P.digits, P.st, err) &&
              upload(P.d_chunks, P.chunks, P.st, err);
@@ -578,7 +578,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             !P.d_dmask.ensure(std::max<size_t>(Kw, 1) * sizeof(uint32_t), err) ||
             !P.d_pc.ensure((size_t)(npan + 1) * kk * sizeof(uint32_t), err) ||
             !P.d_pidx.ensure((size_t)P.split_slots * sizeof(uint32_t), err) ||
-            !P.d_pval.ensure((size_t)P.split_slots * sizeof(float), err) ||
+            !P.d_pval.ensure((size_t)P.split_slots * (s.fp64 ? sizeof(double) : sizeof(float)), err) ||
             !P.d_poff.ensure((kk + 1) * sizeof(uint32_t), err) ||
             !P.d_pcnt.ensure(kk * sizeof(uint32_t), err) ||
             !P.d_obsum.ensure((offset_scan_blocks(K) + 1) * sizeof(uint32_t), err) ||
@@ -655,13 +655,15 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         b.smask = P.d_smask.as<uint32_t>();
         b.pc = P.d_pc.as<uint32_t>();
         b.pidx = P.d_pidx.p;
-        b.pval = P.d_pval.as<float>();
-        b.U = P.d_U.as<float>();
+        b.pval = P.d_pval.p;
+        b.U = s.fp64 ? P.d_E.p : P.d_U.p;                       // FP64: the embedding rows themselves
+        b.urows = s.fp64 ? P.d_knodes.as<int32_t>() : nullptr;
+        b.values_fp64 = s.fp64;
         b.ldu = P.ldn;
         b.len = P.d_klen_sum.as<double>();
         b.S = P.d_S.as<double>();
         b.out = P.out_ptr();
-        b.out_fp64 = false;
+        b.out_fp64 = s.fp64;
         b.walked = P.d_walked.as<uint32_t>();
         b.rows = P.d_rows.as<unsigned long long>();
         b.rcount = P.d_rcount.as<uint32_t>();

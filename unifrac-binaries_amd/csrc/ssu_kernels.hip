@@ -775,7 +775,8 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
                                                      const double* __restrict__ S, int normalized,
                                                      TOut* __restrict__ out, uint32_t* __restrict__ visits,
                                                      const uint32_t* __restrict__ order, uint32_t n_blocks,
-                                                     uint32_t per_xcd) {
+                                                     uint32_t per_xcd, const uint32_t* __restrict__ dmask,
+                                                     int add_partial) {
     constexpr uint32_t EB = SpStep<TIn>::EB;
     const uint32_t seq = (blockIdx.x % N_XCDS) * per_xcd + blockIdx.x / N_XCDS;
     if (seq >= n_blocks) return;
@@ -804,6 +805,7 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
         // active nodes of this chunk, in increasing order, as offsets from w0*32
         const uint32_t w = w0 + lane;
         uint32_t word = w < Kw ? ((mI[w] | mI[Kw + w]) & (mJ[w] | mJ[Kw + w])) : 0u;
+        if (dmask && w < Kw) word &= dmask[w];   // the other nodes come from the posting-list kernel
         const uint32_t total = build_list(list, word, lane);
         visited += total;
         const uint32_t base = w0 * 32;
@@ -878,11 +880,15 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const size_t i = i0 + r, j = j0 + c;
-            if (i >= n || j >= n) continue;
+            // the i < j lane writes both entries (so the partial it reads is its own)
+            if (i >= n || j >= n || i > j) continue;
+            if (i == j) { out[i * n + i] = (TOut)0; continue; }
+            double m = acc[r][c];
+            if (add_partial) m += (double)out[i * n + j];   // the sparse nodes' share, from k_pairs_rows
             const double tot = S[i] + S[j];
-            double x = tot - 2.0 * acc[r][c];
+            double x = tot - 2.0 * m;
             x = x < 0.0 ? 0.0 : x;   // rounding can undershoot 0; NaN (empty samples) passes
-            const double d = i == j ? 0.0 : (normalized ? x / tot : x);
+            const double d = normalized ? x / tot : x;
             out[i * n + j] = (TOut)d;
             out[j * n + i] = (TOut)d;
         }
@@ -1054,21 +1060,21 @@ __global__ void __launch_bounds__(64) k_node_split(uint32_t* __restrict__ dmask,
 }
 
 // Posting lists of the sparse nodes: node r's samples in increasing order
-// (pidx) with their FP32 values (pval, when non-null), in slots
+// (pidx) with their values (pval, when non-null: row urows[r] of U, or row r), in slots
 // [poff[r], poff[r + 1]); pc[p * K + r] = number of node r's samples below
 // 32 p, for p = 0..npan.  One thread per node.
-template <class IX>
-__global__ void __launch_bounds__(64) k_postings(IX* __restrict__ pidx, float* __restrict__ pval,
+template <class IX, class TV>
+__global__ void __launch_bounds__(64) k_postings(IX* __restrict__ pidx, TV* __restrict__ pval,
                                                  uint32_t* __restrict__ pc, const uint32_t* __restrict__ smask,
-                                                 const uint32_t* __restrict__ dmask, const float* __restrict__ U,
-                                                 size_t ldu, uint32_t K, uint32_t npan,
-                                                 const uint32_t* __restrict__ poff, uint32_t slots) {
+                                                 const uint32_t* __restrict__ dmask, const TV* __restrict__ U,
+                                                 const int32_t* __restrict__ urows, size_t ldu, uint32_t K,
+                                                 uint32_t npan, const uint32_t* __restrict__ poff, uint32_t slots) {
     const uint32_t r = blockIdx.x * 64 + threadIdx.x;
     if (r >= K || ((dmask[r >> 5] >> (r & 31)) & 1u) || poff[K] > slots) return;   // (the last: never, by the bound)
-    const float* ur = U ? U + (size_t)r * ldu : nullptr;
+    const TV* ur = U ? U + (size_t)(urows ? urows[r] : (int32_t)r) * ldu : nullptr;
     const uint32_t o = poff[r], cap = poff[r + 1] - o;
     IX* ix = pidx + o;
-    float* vx = pval ? pval + o : nullptr;
+    TV* vx = pval ? pval + o : nullptr;
     uint32_t cnt = 0;
     for (uint32_t p = 0; p < npan; ++p) {
         pc[(size_t)p * K + r] = cnt;
@@ -1096,10 +1102,16 @@ __device__ __forceinline__ unsigned long long to_fixed(double x) {
 // round(m * ls) as an integer, for 0 <= m * ls < 2^51: the FMA adds 2^52 with
 // one rounding (to nearest, ties to even), which leaves the integer in the
 // low 52 bits of the mantissa.
-__device__ __forceinline__ unsigned long long fixed_term(float m, double ls) {
-    const double y = fma((double)m, ls, 0x1.0p52);
+__device__ __forceinline__ unsigned long long fixed_term(double m, double ls) {
+    const double y = fma(m, ls, 0x1.0p52);
     return (unsigned long long)__double_as_longlong(y) & 0x000FFFFFFFFFFFFFull;
 }
+
+// min of two values >= 0: for FP32 the smaller float has the smaller bit pattern
+__device__ __forceinline__ double vmin_nonneg(float a, float b) {
+    return (double)__uint_as_float(min(__float_as_uint(a), __float_as_uint(b)));
+}
+__device__ __forceinline__ double vmin_nonneg(double a, double b) { return a < b ? a : b; }
 
 // Row lists of the sparse nodes: for sample a, the sparse nodes r that a holds,
 // in increasing r, each as r << 32 | (a's index in r's posting list).  One
@@ -1230,19 +1242,21 @@ constexpr uint32_t SR_WIN = 1024;  // walk positions resolved to entries at a ti
 // is taken 64 entries at a time (round q goes to wave q % 8); a round's
 // posting ranges inside the chunk (past a) are walked with the wave's 64 lanes
 // spread evenly over the concatenated postings, each position's entry read
-// from a per-wave byte map (entry starts marked, then a prefix max).  Terms are accumulated in fixed point, integers in
-// units of 2^(e - 51) (weighted: the FP64 product l_r * min, rounded to
-// nearest) or 2^(e - 61) (GRAM: l_r, truncated) where 2^e bounds the row's sum
-// (S[a] weighted; the FP64 sum of the row list's lengths unweighted).  Integer
+// from a per-wave byte map (entry starts marked, then a prefix max).  Terms
+// are accumulated in fixed point, integers in units of 2^(e - 51) (FP32
+// weighted values: the FP64 product l_r * min, rounded to nearest) or
+// 2^(e - 61) (FP64 weighted values: l_r * min, and GRAM: l_r, truncated) where
+// 2^e bounds the row's sum (S[a] weighted; the FP64 sum of the row list's
+// lengths unweighted).  Weighted partial sums go out in the value type.  Integer
 // adds make the result independent of the order in which waves and lanes add,
 // and two samples holding the same nodes get bit-equal G(a, b) and G(a, a).
 // Posting slots are addressed with 32-bit offsets (fewer than 2^30 slots).
-template <class TOut, class IX, bool GRAM>
+template <class TOut, class IX, bool GRAM, class TV>
 __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__ out, uint32_t* __restrict__ walked,
                                                    const unsigned long long* __restrict__ rows,
                                                    const uint32_t* __restrict__ offsets,
                                                    const uint32_t* __restrict__ pc,
-                                                   const IX* __restrict__ pidx, const float* __restrict__ pval,
+                                                   const IX* __restrict__ pidx, const TV* __restrict__ pval,
                                                    const double* __restrict__ len, const double* __restrict__ S,
                                                    uint32_t K, uint32_t n, uint32_t npan,
                                                    const uint32_t* __restrict__ poff, uint32_t slots,
@@ -1261,7 +1275,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     __shared__ unsigned long long acc[SR_CW];
     __shared__ uint32_t base_s[SR_WAVES][64], wk[SR_WAVES];
-    __shared__ float lu_s[SR_WAVES][GRAM ? 1 : 64];
+    __shared__ TV lu_s[SR_WAVES][GRAM ? 1 : 64];
     // per entry: GRAM the fixed-point term, weighted l_r * scale (FP64 bits)
     __shared__ unsigned long long q8_s[SR_WAVES][64], diag_s;
     __shared__ double bsum_s[SR_WAVES];
@@ -1269,7 +1283,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     uint4* own4 = own_s[wv];
     uint8_t* own = reinterpret_cast<uint8_t*>(own4);
     uint32_t* base = base_s[wv];
-    float* lu = lu_s[wv];
+    TV* lu = lu_s[wv];
     unsigned long long* q8 = q8_s[wv];
     for (uint32_t q = threadIdx.x; q < SR_CW; q += 64 * SR_WAVES) acc[q] = 0ull;
     const uint32_t e0 = offsets[(size_t)a * nkc], e1 = offsets[(size_t)(a + 1) * nkc];
@@ -1292,7 +1306,9 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     __syncthreads();
     int e2 = 0;
     (void)frexp(bound, &e2);   // bound < 2^e2 bounds every partial sum of the row
-    const int fb = GRAM ? 61 : 51;
+    // FP32 weighted terms: 51-bit rounding (one FMA); FP64 values and GRAM: 61-bit truncation
+    constexpr bool FINE = GRAM || sizeof(TV) == 8;
+    const int fb = FINE ? 61 : 51;
     const double scale = ldexp(1.0, fb - e2), inv = ldexp(1.0, e2 - fb);
     const bool has_diag = GRAM && b0 <= a && a < b1;
     if (GRAM && has_diag) {   // G(a, a): the same fixed-point terms, every one of them
@@ -1305,7 +1321,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     uint32_t total_walked = 0;
     for (uint32_t r0 = e0 + 64 * wv; r0 < e1; r0 += 64 * SR_WAVES) {   // uniform per wave
         uint32_t start = 0, end = 0, r = 0, slot0 = 0;
-        float ua = 0.0f;
+        TV ua = (TV)0;
         unsigned long long q = 0ull;
         if (r0 + lane < e1) {
             const unsigned long long ent = rows[r0 + lane];
@@ -1387,7 +1403,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                     qq[u] = base[eq[u]] + w0 + j;
                 }
                 uint32_t bq[SR_UNROLL];
-                float vq[SR_UNROLL];
+                TV vq[SR_UNROLL];
 #pragma unroll
                 for (uint32_t u = 0; u < SR_UNROLL; ++u) {
                     bq[u] = (uint32_t)pidx[qq[u]];
@@ -1399,9 +1415,12 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                         if constexpr (GRAM) {
                             atomicAdd(&acc[bq[u] - b0], q8[eq[u]]);
                         } else {
-                            // u >= 0: the smaller float has the smaller bit pattern
-                            const float m = __uint_as_float(min(__float_as_uint(lu[eq[u]]), __float_as_uint(vq[u])));
-                            atomicAdd(&acc[bq[u] - b0], fixed_term(m, __longlong_as_double((long long)q8[eq[u]])));
+                            const double m = vmin_nonneg(lu[eq[u]], vq[u]);
+                            const double ls = __longlong_as_double((long long)q8[eq[u]]);
+                            if constexpr (FINE)
+                                atomicAdd(&acc[bq[u] - b0], to_fixed(m * ls));
+                            else
+                                atomicAdd(&acc[bq[u] - b0], fixed_term(m, ls));
                         }
                     }
             }
@@ -1657,7 +1676,7 @@ template <class TIn, class TOut>
 static void pairs_sparse_t(uint32_t grid, hipStream_t st, const SparseArgs& a, uint32_t per_xcd) {
     hipLaunchKernelGGL((k_pairs_sparse<TIn, TOut>), dim3(grid), dim3(64), 0, st, (const TIn*)a.U, a.rows, a.ldn, a.len,
                        a.masks, a.Kw, a.n, a.S, (int)a.normalized, (TOut*)a.out, a.visits, a.order, a.n_blocks,
-                       per_xcd);
+                       per_xcd, a.dmask, (int)(a.dmask != nullptr));
 }
 
 std::vector<uint32_t> upper_block_order(uint32_t nb, uint32_t super) {
@@ -1695,7 +1714,7 @@ static bool sparse_args_ok(const SparseNodesArgs& a) {
                         a.rbsum && a.npan == (a.n + 31) / 32 && a.slots < (1u << 30);
     if (!common) return false;
     if (a.gram) return a.n_pad >= a.n && a.n_pad % GRAM_BM == 0;   // unweighted: values, sums and out unused
-    return a.pcnt && a.obsum && a.thr && a.pval && a.U && a.S && a.out && !a.out_fp64 &&
+    return a.pcnt && a.obsum && a.thr && a.pval && a.U && a.S && a.out && a.out_fp64 == a.values_fp64 &&
            a.ldu >= (size_t)a.npan * 32;
 }
 
@@ -1714,14 +1733,19 @@ hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
         hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_T), 0, st, a.poff, a.pcnt, a.obsum, (size_t)a.K, nb);
     }
     const bool narrow = a.n <= 65536;   // sample indices fit 16 bits: 6 bytes per posting instead of 8
-    const float* U = a.gram ? nullptr : a.U;
-    float* pval = a.gram ? nullptr : a.pval;
-    if (narrow)
-        hipLaunchKernelGGL(k_postings<uint16_t>, dim3(gk), dim3(64), 0, st, (uint16_t*)a.pidx, pval, a.pc, a.smask,
-                           a.dmask, U, a.ldu, a.K, a.npan, (const uint32_t*)a.poff, a.slots);
-    else
-        hipLaunchKernelGGL(k_postings<uint32_t>, dim3(gk), dim3(64), 0, st, (uint32_t*)a.pidx, pval, a.pc, a.smask,
-                           a.dmask, U, a.ldu, a.K, a.npan, (const uint32_t*)a.poff, a.slots);
+    const int key = (narrow ? 1 : 0) | (a.values_fp64 ? 2 : 0);
+    const void* U = a.gram ? nullptr : a.U;
+    void* pval = a.gram ? nullptr : a.pval;
+#define SSU_POSTINGS(IX, TV)                                                                                   \
+    hipLaunchKernelGGL((k_postings<IX, TV>), dim3(gk), dim3(64), 0, st, (IX*)a.pidx, (TV*)pval, a.pc, a.smask, \
+                       a.dmask, (const TV*)U, a.urows, a.ldu, a.K, a.npan, (const uint32_t*)a.poff, a.slots)
+    switch (key) {
+    case 0: SSU_POSTINGS(uint32_t, float); break;
+    case 1: SSU_POSTINGS(uint16_t, float); break;
+    case 2: SSU_POSTINGS(uint32_t, double); break;
+    default: SSU_POSTINGS(uint16_t, double); break;
+    }
+#undef SSU_POSTINGS
     // row lists: counts per (sample, node chunk), their exclusive scan, then the entries
     const uint32_t nkc = row_list_chunks(a.K);
     const size_t m = (size_t)a.n * nkc;
@@ -1736,13 +1760,13 @@ hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <class IX, bool GRAM>
+// weighted: the partial sums go out in the value type (FP32 or FP64 output)
+template <class IX, bool GRAM, class TV>
 static void pairs_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs& a, uint32_t nchunks, uint32_t nkc) {
-    hipLaunchKernelGGL((k_pairs_rows<float, IX, GRAM>), dim3(blocks), dim3(64 * SR_WAVES), 0, st,
-                       GRAM ? nullptr : (float*)a.out, a.walked, a.rows, a.roff, a.pc, (const IX*)a.pidx,
-                       GRAM ? nullptr : a.pval, a.len, GRAM ? nullptr : a.S, a.K, a.n, a.npan, (const uint32_t*)a.poff,
-                       a.slots, nchunks, nkc,
-                       a.gram, a.n_pad / GRAM_BM);
+    hipLaunchKernelGGL((k_pairs_rows<TV, IX, GRAM, TV>), dim3(blocks), dim3(64 * SR_WAVES), 0, st,
+                       GRAM ? nullptr : (TV*)a.out, a.walked, a.rows, a.roff, a.pc, (const IX*)a.pidx,
+                       GRAM ? nullptr : (const TV*)a.pval, a.len, GRAM ? nullptr : a.S, a.K, a.n, a.npan,
+                       (const uint32_t*)a.poff, a.slots, nchunks, nkc, a.gram, a.n_pad / GRAM_BM);
 }
 
 hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st) {
@@ -1754,11 +1778,14 @@ hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st) {
     if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
     const bool narrow = a.n <= 65536;
     if (a.gram) {
-        if (narrow) pairs_rows_t<uint16_t, true>((uint32_t)blocks, st, a, nchunks, nkc);
-        else pairs_rows_t<uint32_t, true>((uint32_t)blocks, st, a, nchunks, nkc);
+        if (narrow) pairs_rows_t<uint16_t, true, float>((uint32_t)blocks, st, a, nchunks, nkc);
+        else pairs_rows_t<uint32_t, true, float>((uint32_t)blocks, st, a, nchunks, nkc);
+    } else if (a.values_fp64) {
+        if (narrow) pairs_rows_t<uint16_t, false, double>((uint32_t)blocks, st, a, nchunks, nkc);
+        else pairs_rows_t<uint32_t, false, double>((uint32_t)blocks, st, a, nchunks, nkc);
     } else {
-        if (narrow) pairs_rows_t<uint16_t, false>((uint32_t)blocks, st, a, nchunks, nkc);
-        else pairs_rows_t<uint32_t, false>((uint32_t)blocks, st, a, nchunks, nkc);
+        if (narrow) pairs_rows_t<uint16_t, false, float>((uint32_t)blocks, st, a, nchunks, nkc);
+        else pairs_rows_t<uint32_t, false, float>((uint32_t)blocks, st, a, nchunks, nkc);
     }
     return hipGetLastError();
 }

@@ -134,12 +134,13 @@ struct SparseArgs {
     const uint32_t* dmask = nullptr;   // FP32 kernel: visit only these nodes and add the partial sums in `out`
 };
 
-// Occupancy split.  Weighted (FP32 shared-node path): nodes held by at least
-// `thr` samples are dense (bit set in dmask, [ceil(K/32)] words, set here by
-// launch_sparse_lists); for the others the posting lists are built from U and
-// the sample masks at offsets scanned from their counts, and each row's share
-// of M over them is written to out[a * n + b] (b > a) for the tiled kernel to
-// add (SparseArgs::dmask).  Unweighted (gram != nullptr): the K nodes are the
+// Occupancy split.  Weighted (shared-node path, FP32 or FP64 values): nodes
+// held by at least `thr` samples are dense (bit set in dmask, [ceil(K/32)]
+// words, set here by launch_sparse_lists); for the others the posting lists
+// are built from the value rows and the sample masks at offsets scanned from
+// their counts, and each row's share of M over them is written, in the value
+// type, to out[a * n + b] (b > a) for the tiled kernel to add
+// (SparseArgs::dmask).  Unweighted (gram != nullptr): the K nodes are the
 // plan's sparse nodes with offsets fixed when the plan is built (dmask all
 // zero, U and pval unused); each row's share of the shared length
 // sum_r len[r] [a holds r][b holds r] is added to the FP64 Gram tile entries
@@ -151,13 +152,15 @@ struct SparseNodesArgs {
     const uint32_t* smask;  // [npan][K] sample bits, from launch_gather_sums / launch_bits_smask
     uint32_t* pc;           // [npan + 1][K]
     void* pidx;             // [slots] sample indices: u16 when n <= 65,536, else u32
-    float* pval;            // [slots] (weighted)
+    void* pval;             // [slots] (weighted) values: float, or double with values_fp64
     uint32_t* poff;         // [K + 1] posting offsets (weighted: written here; unweighted: given)
     uint32_t* pcnt;         // [K] (weighted) sparse nodes' sample counts
     uint32_t* obsum;        // [ceil(K / 4096) + 1] (weighted) scan scratch
     uint32_t slots;
     uint32_t thr;           // (weighted) dense from this many samples
-    const float* U;         // [K][ldu] (weighted)
+    const void* U;          // (weighted) value rows: row urows[r] (or r) of U, stride ldu
+    const int32_t* urows;   // nullptr: identity
+    bool values_fp64;       // (weighted) FP64 values, FP64 output
     size_t ldu;
     const double* len;      // [K]
     const double* S;        // [n] (weighted)
