@@ -461,3 +461,50 @@ def test_sample_blocked_faith(ssu, O, synth, monkeypatch):
     s, vals, _ = ssu.faith_pd_inmem(inputs_from(ssu, tree, table))
     st, le, nm = tree
     assert s == ssu.OKAY and np.array_equal(vals, O.faith_pd(O.Tree(st.astype(np.uint8), le, nm), O.Table(*table)))
+
+
+@pytest.mark.parametrize("name", ["weighted_normalized", "weighted_unnormalized", "weighted_normalized_fp64"])
+def test_weighted_near_duplicate_samples(ssu, O, synth, name):
+    """Samples 40..79 repeat 0..39 with one count raised by 1: distances of
+    1e-4..1e-2.  The distance is summed as l |u - v| (no difference of large
+    sums), so FP64 names stay within 1e-11 of the FP64 oracle.  FP32 names
+    carry the FP32 rounding of the proportions themselves (|u - v| of two
+    nearly equal FP32 values): the entries with d > 0.05 stay within 1e-6, and
+    every entry within the error of the reference's own FP32 kernels on the
+    same input (oracle/_ref, run here)."""
+    st, le, nm = synth.random_tree(400, 95)
+    obs, sids, ip, ix, d = synth.random_table(400, 80, 0.08, 96)
+    dense = np.zeros((len(obs), 80))
+    for r in range(len(obs)):
+        dense[r, ix[ip[r]:ip[r + 1]]] = d[ip[r]:ip[r + 1]]
+    dense[:, 40:] = dense[:, :40]
+    rng = np.random.default_rng(97)
+    for s in range(40, 80):
+        held = np.flatnonzero(dense[:, s])
+        dense[rng.choice(held), s] += 1.0
+    rows = [np.flatnonzero(dense[r]) for r in range(len(obs))]
+    keep = [r for r in range(len(obs)) if len(rows[r])]
+    table = ([obs[r] for r in keep], [f"s{i}" for i in range(80)],
+             np.concatenate([[0], np.cumsum([len(rows[r]) for r in keep])]).astype(np.uint32),
+             np.concatenate([rows[r] for r in keep]).astype(np.uint32),
+             np.concatenate([dense[r, rows[r]] for r in keep]))
+    say("near duplicates", name)
+    s, m = ssu.matrix_inmem(ssu.Inputs(st, le, nm, *table), name)
+    assert s == ssu.OKAY, ssu.last_error()
+    base = name.replace("_fp64", "")
+    ref = oracle_matrix(O, (st, le, nm), table, base)
+    near = np.array([ref[i, i + 40] for i in range(40)])
+    assert near.max() < 0.05
+    err = rel_err(m, ref)
+    say("max rel err", err, "near-duplicate distances", near.min(), near.max())
+    if name.endswith("fp64"):
+        assert err <= TOL["fp64"]
+        return
+    far = ref > 0.05
+    assert rel_err(m[far], ref[far]) <= TOL["fp32"]
+    assert O.ref_available(), "the reference build (oracle/_ref) is part of the tree"
+    T, Tb = O.Tree(np.asarray(st, np.uint8), le, nm), O.Table(*table)
+    ref32 = O.stripes_to_matrix(O.ref_stripes(T, Tb, base, fp32=True), 80)
+    err_ref32 = rel_err(ref32, ref)
+    say("reference FP32 kernels", err_ref32)
+    assert err <= max(TOL["fp32"], err_ref32)

@@ -127,7 +127,7 @@ struct PlanImpl {
     DevMem d_indptr, d_indices, d_data, d_totals, d_leaf_k, d_leaf_row, d_level_nodes, d_child_ptr, d_child_idx;
     DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_order, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
         d_faith, d_stripes, d_smask, d_dmask, d_pc, d_pidx, d_pval, d_walked, d_rows, d_rcount, d_roff, d_rbsum,
-        d_sp_node, d_sp_len, d_umasks, d_B, d_poff, d_pcnt, d_obsum;
+        d_sp_node, d_sp_len, d_umasks, d_B, d_poff, d_pcnt, d_obsum, d_Ssp;
 
     hipStream_t own_st = nullptr;    // created with the plan; st may point at a caller's stream
     // per-execution inputs and output (nullptr: the plan's own table data / totals / result)
@@ -581,6 +581,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             !P.d_pval.ensure((size_t)P.split_slots * (s.fp64 ? sizeof(double) : sizeof(float)), err) ||
             !P.d_poff.ensure((kk + 1) * sizeof(uint32_t), err) ||
             !P.d_pcnt.ensure(kk * sizeof(uint32_t), err) ||
+            !P.d_Ssp.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err) ||
             !P.d_obsum.ensure((offset_scan_blocks(K) + 1) * sizeof(uint32_t), err) ||
             !P.d_walked.ensure((size_t)sparse_rows_blocks(P.n) * sizeof(uint32_t), err) ||
             !P.d_rows.ensure((size_t)P.split_slots * sizeof(unsigned long long), err) ||
@@ -677,6 +678,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         b.obsum = P.d_obsum.as<uint32_t>();
         b.slots = P.split_slots;
         b.thr = P.split_thr;
+        b.ssp = P.d_Ssp.as<double>();
         SSU_HIP(launch_sparse_nodes(b, P.st));
     }
     SparseNodesArgs gb{};

@@ -895,12 +895,14 @@ __global__ void __launch_bounds__(64) k_pairs_sparse(const TIn* __restrict__ U, 
 }
 
 // FP32 operands: 32 x 64 blocks of pairs (rows: a 32-sample panel, columns:
-// a 64-sample panel), one wavefront each, a 4 x 8 tile per lane.  Terms are
-// v_min_f32 + v_fma_f32 (this pair issues about 1.3x faster than its FP64
-// form), summed in FP32 over S3_MERGE listed nodes and then merged into FP64.
-// The narrower row panel also skips more nodes than a 64 x 64 block.  Only
-// pairs i < j are written (both (i, j) and (j, i)), so every pair has one
-// writer even where blocks straddle the diagonal.
+// a 64-sample panel), one wavefront each, a 4 x 8 tile per lane.  The kernel
+// sums l * |u - v| directly over the nodes either panel holds (the dense
+// nodes, with the occupancy split): v_sub_f32 + v_fma_f32 with an |x| source
+// modifier, summed in FP32 over S3_MERGE listed nodes and then merged into
+// FP64, so the distance never comes from a difference of large sums and stays
+// accurate for near-identical samples.  S only normalises.  Only pairs i < j
+// are written (both (i, j) and (j, i)), so every pair has one writer even
+// where blocks straddle the diagonal.
 constexpr uint32_t S3_BI = 32, S3_BJ = 64;
 constexpr uint32_t S3_EB = 8;          // nodes per step; two steps in flight
 constexpr uint32_t S3_MERGE = 64;      // FP32 accumulation span, in listed nodes
@@ -908,11 +910,6 @@ constexpr uint32_t S3_GSJ = 12;        // column groups of 8 floats sit 48 bytes
 constexpr uint32_t S3_JOFF = 32;       // the 32 row values come first, contiguous
 constexpr uint32_t S3_NODE = S3_JOFF + 8 * S3_GSJ;
 
-__device__ __forceinline__ float vmin_f32(float a, float b) {
-    float r;
-    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
 
 template <class TOut>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_pairs_sparse32(const float* __restrict__ U, size_t ldn,
@@ -950,7 +947,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 
     for (uint32_t w0 = 0; w0 < Kw; w0 += SP_WORDS) {
         const uint32_t w = w0 + lane;
-        uint32_t word = w < Kw ? (mI[w] & (mJ[w] | mJ[Kw + w])) : 0u;
+        uint32_t word = w < Kw ? (mI[w] | mJ[w] | mJ[Kw + w]) : 0u;
         if (dmask && w < Kw) word &= dmask[w];   // the other nodes come from the posting-list kernel
         const uint32_t total = build_list(list, word, lane);
         visited += total;
@@ -999,7 +996,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) acc[r][c] = __builtin_fmaf(vmin_f32(a[r], b[c]), ln, acc[r][c]);
+                    for (int c = 0; c < 8; ++c) acc[r][c] = __builtin_fmaf(__builtin_fabsf(a[r] - b[c]), ln, acc[r][c]);
             }
             pending += EB;
             if (pending >= S3_MERGE) {
@@ -1029,12 +1026,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             const size_t i = i0 + r, j = j0 + c;
             if (j >= n || i > j) continue;
             if (i == j) { out[i * n + i] = (TOut)0; continue; }
-            double m = acc64[r][c] + (double)acc[r][c];
-            if (add_partial) m += (double)out[i * n + j];   // the sparse nodes' share, from k_pairs_rows
-            const double tot = S[i] + S[j];
-            double x = tot - 2.0 * m;
-            x = x < 0.0 ? 0.0 : x;   // rounding can undershoot 0; NaN (empty samples) passes
-            const double d = normalized ? x / tot : x;
+            double x = acc64[r][c] + (double)acc[r][c];
+            if (add_partial) x += (double)out[i * n + j];   // the sparse nodes' share, from k_pairs_rows
+            x = x < 0.0 ? 0.0 : x;   // the share's rounding can undershoot 0; NaN (empty samples) passes
+            const double d = normalized ? x / (S[i] + S[j]) : x;
             out[i * n + j] = (TOut)d;
             out[j * n + i] = (TOut)d;
         }
@@ -1261,7 +1256,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                                                    uint32_t K, uint32_t n, uint32_t npan,
                                                    const uint32_t* __restrict__ poff, uint32_t slots,
                                                    uint32_t nchunks, uint32_t nkc, double* __restrict__ G,
-                                                   uint32_t tiles) {
+                                                   uint32_t tiles, const double* __restrict__ ssp) {
     // chunk-major: the rows in flight share one column chunk of every posting
     // list, a quarter of the posting data at 16k samples (cache-resident)
     const uint32_t a = blockIdx.x % n, c = blockIdx.x / n;
@@ -1446,10 +1441,42 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             const unsigned long long v = acc[b - b0];
             if (v) G[gram_slot(a, b, tiles)] += (double)v * inv;
         }
+    } else if constexpr (sizeof(TV) == 4) {
+        // FP32 values: the sparse nodes' share of sum l |u_a - u_b| itself, formed in
+        // FP64 before the rounding to FP32
+        const double sa = ssp[a];
+        for (uint32_t b = max(a + 1, b0) + threadIdx.x; b < b1; b += 64 * SR_WAVES)
+            out[(size_t)a * n + b] = (TOut)((sa + ssp[b]) - 2.0 * ((double)acc[b - b0] * inv));
     } else {
         for (uint32_t b = max(a + 1, b0) + threadIdx.x; b < b1; b += 64 * SR_WAVES)
             out[(size_t)a * n + b] = (TOut)((double)acc[b - b0] * inv);
     }
+}
+
+// ssp[a] = sum over sample a's row list (its sparse nodes r, increasing) of
+// len[r] * value of a in r's postings, in FP64 in a fixed order: one wavefront
+// per sample, lane-strided partial sums then a butterfly.
+template <class TV>
+__global__ void __launch_bounds__(64) k_sparse_sums(double* __restrict__ ssp,
+                                                    const unsigned long long* __restrict__ rows,
+                                                    const uint32_t* __restrict__ offsets, uint32_t nkc,
+                                                    const uint32_t* __restrict__ poff, const TV* __restrict__ pval,
+                                                    const double* __restrict__ len, uint32_t K, uint32_t n,
+                                                    uint32_t slots) {
+    const uint32_t a = blockIdx.x, lane = threadIdx.x;
+    if (a >= n) return;
+    double x = 0.0;
+    if (poff[K] <= slots) {   // (never otherwise, by the plan's bound)
+        const uint32_t e0 = offsets[(size_t)a * nkc], e1 = offsets[(size_t)(a + 1) * nkc];
+        for (uint32_t e = e0 + lane; e < e1; e += 64) {
+            const unsigned long long ent = rows[e];
+            const uint32_t r = (uint32_t)(ent >> 32);
+            x += len[r] * (double)pval[poff[r] + (uint32_t)ent];
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+    if (lane == 0) ssp[a] = x;
 }
 
 // cnt[r] += samples of the block with E[knodes[r]][s] > 0: one workgroup per node
@@ -1766,7 +1793,7 @@ static void pairs_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs&
     hipLaunchKernelGGL((k_pairs_rows<TV, IX, GRAM, TV>), dim3(blocks), dim3(64 * SR_WAVES), 0, st,
                        GRAM ? nullptr : (TV*)a.out, a.walked, a.rows, a.roff, a.pc, (const IX*)a.pidx,
                        GRAM ? nullptr : (const TV*)a.pval, a.len, GRAM ? nullptr : a.S, a.K, a.n, a.npan,
-                       (const uint32_t*)a.poff, a.slots, nchunks, nkc, a.gram, a.n_pad / GRAM_BM);
+                       (const uint32_t*)a.poff, a.slots, nchunks, nkc, a.gram, a.n_pad / GRAM_BM, a.ssp);
 }
 
 hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st) {
@@ -1784,6 +1811,9 @@ hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st) {
         if (narrow) pairs_rows_t<uint16_t, false, double>((uint32_t)blocks, st, a, nchunks, nkc);
         else pairs_rows_t<uint32_t, false, double>((uint32_t)blocks, st, a, nchunks, nkc);
     } else {
+        if (!a.ssp) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_sparse_sums<float>, dim3(a.n), dim3(64), 0, st, a.ssp, a.rows, a.roff, nkc,
+                           (const uint32_t*)a.poff, (const float*)a.pval, a.len, a.K, a.n, a.slots);
         if (narrow) pairs_rows_t<uint16_t, false, float>((uint32_t)blocks, st, a, nchunks, nkc);
         else pairs_rows_t<uint32_t, false, float>((uint32_t)blocks, st, a, nchunks, nkc);
     }

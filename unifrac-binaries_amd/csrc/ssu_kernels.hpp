@@ -114,8 +114,10 @@ struct PairArgs {
 };
 hipError_t launch_pairs(PairOp op, bool fp64, const PairArgs& a, hipStream_t st);
 
-// Weighted (normalized or not) through d_ij = S_i + S_j - 2 sum_k l_k min(u_ik, u_jk),
-// visiting per 64x64 block only the nodes whose bit is set in both panels' masks.
+// Weighted (normalized or not).  FP64 (launch_pairs_sparse): d_ij = S_i + S_j -
+// 2 sum_k l_k min(u_ik, u_jk), visiting per 64x64 block only the nodes whose
+// bit is set in both panels' masks.  FP32 (launch_pairs_sparse32): sum_k l_k
+// |u_ik - u_jk| itself over the nodes either panel holds; S normalises.
 struct SparseArgs {
     const void* U;          // node rows, float (row r) or double (row rows[r])
     const int32_t* rows;    // nullptr: identity
@@ -131,16 +133,17 @@ struct SparseArgs {
     uint32_t* visits;       // optional [nb][nb], nb = ceil(n/64): nodes visited per block (bi <= bj)
     const uint32_t* order;  // device copy of upper_block_order(ceil(n/64), 16)
     uint32_t n_blocks;      // its length, nb (nb + 1) / 2
-    const uint32_t* dmask = nullptr;   // FP32 kernel: visit only these nodes and add the partial sums in `out`
+    const uint32_t* dmask = nullptr;   // visit only these nodes and add the posting-list shares in `out`
 };
 
 // Occupancy split.  Weighted (shared-node path, FP32 or FP64 values): nodes
 // held by at least `thr` samples are dense (bit set in dmask, [ceil(K/32)]
 // words, set here by launch_sparse_lists); for the others the posting lists
 // are built from the value rows and the sample masks at offsets scanned from
-// their counts, and each row's share of M over them is written, in the value
-// type, to out[a * n + b] (b > a) for the tiled kernel to add
-// (SparseArgs::dmask).  Unweighted (gram != nullptr): the K nodes are the
+// their counts, and each row's share over them is written, in the value type,
+// to out[a * n + b] (b > a) for the tiled kernel to add (SparseArgs::dmask):
+// with FP32 values the share of sum l |u_a - u_b| (ssp[a] + ssp[b] - 2 M,
+// formed in FP64), with FP64 values the share of M.  Unweighted (gram != nullptr): the K nodes are the
 // plan's sparse nodes with offsets fixed when the plan is built (dmask all
 // zero, U and pval unused); each row's share of the shared length
 // sum_r len[r] [a holds r][b holds r] is added to the FP64 Gram tile entries
@@ -172,6 +175,7 @@ struct SparseNodesArgs {
     uint32_t* roff;         // [n * row_list_chunks(K) + 1]
     uint32_t* rbsum;        // [row_list_scan_blocks(n, K) + 1]
     uint32_t K, n, npan;
+    double* ssp = nullptr;  // (weighted, FP32 values) [n]: per-sample sum of l * u over its sparse nodes
     double* gram = nullptr; // unweighted: the Gram tiles of launch_gram_i8
     uint32_t n_pad = 0;     // their padded edge
 };
