@@ -272,7 +272,7 @@ bool PlanImpl::structural_counts(std::vector<uint32_t>& cnt, std::string* err) {
 // held by no sample leave it altogether.  SSU_GRAM_SPLIT sets theta (0: off),
 // SSU_GRAM_SPLIT_MIN_N the smallest table split (GSPLIT_MIN_N).
 bool PlanImpl::split_gram(std::string* err) {
-    double theta = 0.04;
+    double theta = 0.06;
     if (const char* env = getenv("SSU_GRAM_SPLIT")) theta = atof(env);
     uint32_t min_n = GSPLIT_MIN_N;
     if (const char* env = getenv("SSU_GRAM_SPLIT_MIN_N")) min_n = (uint32_t)atol(env);
