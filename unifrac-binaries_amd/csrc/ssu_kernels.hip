@@ -1246,6 +1246,13 @@ constexpr uint32_t SR_WIN = 1024;  // walk positions resolved to entries at a ti
 // adds make the result independent of the order in which waves and lanes add,
 // and two samples holding the same nodes get bit-equal G(a, b) and G(a, a).
 // Posting slots are addressed with 32-bit offsets (fewer than 2^30 slots).
+// A round entry as the walk reads it: one 16-byte LDS read for FP32 values
+template <class TV> struct alignas(16) RowEntry {
+    uint32_t base;            // slot of walk position t: base + t (mod 2^32)
+    TV lu;                    // the row's value in this node (weighted)
+    unsigned long long q8;    // GRAM: the fixed-point term; weighted: l * scale (FP64 bits)
+};
+
 template <class TOut, class IX, bool GRAM, class TV>
 __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__ out, uint32_t* __restrict__ walked,
                                                    const unsigned long long* __restrict__ rows,
@@ -1269,17 +1276,15 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     }
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     __shared__ unsigned long long acc[SR_CW];
-    __shared__ uint32_t base_s[SR_WAVES][64], wk[SR_WAVES];
-    __shared__ TV lu_s[SR_WAVES][GRAM ? 1 : 64];
+    __shared__ uint32_t wk[SR_WAVES];
+    __shared__ RowEntry<TV> ent_s[SR_WAVES][64];   // per wave: the round's 64 entries
     // per entry: GRAM the fixed-point term, weighted l_r * scale (FP64 bits)
-    __shared__ unsigned long long q8_s[SR_WAVES][64], diag_s;
+    __shared__ unsigned long long diag_s;
     __shared__ double bsum_s[SR_WAVES];
     __shared__ uint4 own_s[SR_WAVES][SR_WIN / 16];   // per wave: entry + 1 of every window position
     uint4* own4 = own_s[wv];
     uint8_t* own = reinterpret_cast<uint8_t*>(own4);
-    uint32_t* base = base_s[wv];
-    TV* lu = lu_s[wv];
-    unsigned long long* q8 = q8_s[wv];
+    RowEntry<TV>* ents = ent_s[wv];
     for (uint32_t q = threadIdx.x; q < SR_CW; q += 64 * SR_WAVES) acc[q] = 0ull;
     const uint32_t e0 = offsets[(size_t)a * nkc], e1 = offsets[(size_t)(a + 1) * nkc];
     double bound;
@@ -1342,9 +1347,11 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         }
         const uint32_t T = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
         if (T == 0) continue;   // uniform per wave
-        base[lane] = slot0 + start - (incl - cnt);   // slot of walk position t: base + t (mod 2^32)
-        if constexpr (!GRAM) lu[lane] = ua;
-        q8[lane] = q;
+        RowEntry<TV> me;
+        me.base = slot0 + start - (incl - cnt);   // slot of walk position t: base + t (mod 2^32)
+        me.lu = ua;
+        me.q8 = q;
+        ents[lane] = me;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1390,12 +1397,13 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             // SR_UNROLL postings per lane in flight: slots first, then every load, then the adds
             for (uint32_t j0 = lane; j0 < wn; j0 += 64 * SR_UNROLL) {
-                uint32_t qq[SR_UNROLL], eq[SR_UNROLL];
+                uint32_t qq[SR_UNROLL];
+                RowEntry<TV> en[SR_UNROLL];
 #pragma unroll
                 for (uint32_t u = 0; u < SR_UNROLL; ++u) {
                     const uint32_t j = min(j0 + 64 * u, wn - 1);
-                    eq[u] = (uint32_t)own[j] - 1u;
-                    qq[u] = base[eq[u]] + w0 + j;
+                    en[u] = ents[(uint32_t)own[j] - 1u];
+                    qq[u] = en[u].base + w0 + j;
                 }
                 uint32_t bq[SR_UNROLL];
                 TV vq[SR_UNROLL];
@@ -1408,10 +1416,10 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                 for (uint32_t u = 0; u < SR_UNROLL; ++u)
                     if (j0 + 64 * u < wn) {
                         if constexpr (GRAM) {
-                            atomicAdd(&acc[bq[u] - b0], q8[eq[u]]);
+                            atomicAdd(&acc[bq[u] - b0], en[u].q8);
                         } else {
-                            const double m = vmin_nonneg(lu[eq[u]], vq[u]);
-                            const double ls = __longlong_as_double((long long)q8[eq[u]]);
+                            const double m = vmin_nonneg(en[u].lu, vq[u]);
+                            const double ls = __longlong_as_double((long long)en[u].q8);
                             if constexpr (FINE)
                                 atomicAdd(&acc[bq[u] - b0], to_fixed(m * ls));
                             else
