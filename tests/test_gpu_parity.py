@@ -508,3 +508,61 @@ def test_weighted_near_duplicate_samples(ssu, O, synth, name):
     err_ref32 = rel_err(ref32, ref)
     say("reference FP32 kernels", err_ref32)
     assert err <= max(TOL["fp32"], err_ref32)
+
+
+@pytest.mark.parametrize("alpha", [0.0, 0.5, 1.0])
+@pytest.mark.parametrize("theta", ["0", "0.1", "0.3"])
+def test_generalized_shared_node_path(ssu, O, synth, monkeypatch, theta, alpha):
+    """Generalized (FP32, 0 <= alpha <= 1) on the shared-node path: numerator
+    and denominator summed directly over the nodes either sample panel holds,
+    the rare nodes' shares from the posting lists (theta 0: no split).  Against
+    the FP64 oracle and the dense kernel (SSU_GEN_SPARSE=0)."""
+    tree, table = _case(synth, 500, 300, 0.05, 75, zipf=1.1)
+    monkeypatch.setenv("SSU_SPARSE_SPLIT", theta)
+    inp = inputs_from(ssu, tree, table)
+    say("generalized shared-node", theta, alpha)
+    p = ssu.Plan(inp, "generalized", alpha=alpha)
+    assert p.main_kernel() == "k_pairs_gen32"
+    p.execute()
+    m = p.matrix()
+    p.close()
+    ref = oracle_matrix(O, tree, table, "generalized", alpha=alpha)
+    assert rel_err(m, ref) <= TOL["fp32"]
+    assert np.array_equal(m, m.T) and np.all(np.diag(m) == 0)
+    monkeypatch.setenv("SSU_GEN_SPARSE", "0")
+    s, dense = ssu.matrix_inmem(inp, "generalized", alpha=alpha)
+    assert s == ssu.OKAY
+    assert rel_err(m, dense) <= 2e-6
+
+
+def test_generalized_near_duplicates(ssu, O, synth):
+    """Samples 40..79 repeat 0..39 with one count raised by 1: generalized
+    (alpha 0.5) within the error of the reference's own FP32 kernels on the
+    same input, and within 1e-6 where d > 0.05."""
+    st, le, nm = synth.random_tree(400, 95)
+    obs, sids, ip, ix, d = synth.random_table(400, 80, 0.08, 96)
+    dense = np.zeros((len(obs), 80))
+    for r in range(len(obs)):
+        dense[r, ix[ip[r]:ip[r + 1]]] = d[ip[r]:ip[r + 1]]
+    dense[:, 40:] = dense[:, :40]
+    rng = np.random.default_rng(97)
+    for s in range(40, 80):
+        held = np.flatnonzero(dense[:, s])
+        dense[rng.choice(held), s] += 1.0
+    rows = [np.flatnonzero(dense[r]) for r in range(len(obs))]
+    keep = [r for r in range(len(obs)) if len(rows[r])]
+    table = ([obs[r] for r in keep], [f"s{i}" for i in range(80)],
+             np.concatenate([[0], np.cumsum([len(rows[r]) for r in keep])]).astype(np.uint32),
+             np.concatenate([rows[r] for r in keep]).astype(np.uint32),
+             np.concatenate([dense[r, rows[r]] for r in keep]))
+    say("generalized near duplicates")
+    s, m = ssu.matrix_inmem(ssu.Inputs(st, le, nm, *table), "generalized", alpha=0.5)
+    assert s == ssu.OKAY, ssu.last_error()
+    ref = oracle_matrix(O, (st, le, nm), table, "generalized", alpha=0.5)
+    far = ref > 0.05
+    assert rel_err(m[far], ref[far]) <= TOL["fp32"]
+    T, Tb = O.Tree(np.asarray(st, np.uint8), le, nm), O.Table(*table)
+    ref32 = O.stripes_to_matrix(O.ref_stripes(T, Tb, "generalized", fp32=True, alpha=0.5), 80)
+    err, err_ref32 = rel_err(m, ref), rel_err(ref32, ref)
+    say("max rel err", err, "reference FP32 kernels", err_ref32)
+    assert err <= max(TOL["fp32"], err_ref32)

@@ -87,7 +87,8 @@ struct PlanImpl {
     size_t ldn = 0;
     size_t blk = 0;                  // sample columns per embedding block (ldn: one block)
     bool gram = false;
-    bool sparse = false;             // weighted through the shared-node kernel
+    bool sparse = false;             // weighted (or generalized) through the shared-node kernels
+    bool gen = false;                // generalized on the shared-node path
     uint32_t split_thr = 0;          // FP32 shared-node path: nodes held by fewer samples use posting lists (0: off)
     uint32_t split_slots = 0;        // bound on their postings in any execution
     // Gram path occupancy split: sparse nodes (held by few samples of the plan's
@@ -127,7 +128,7 @@ struct PlanImpl {
     DevMem d_indptr, d_indices, d_data, d_totals, d_leaf_k, d_leaf_row, d_level_nodes, d_child_ptr, d_child_idx;
     DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_order, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
         d_faith, d_stripes, d_smask, d_dmask, d_pc, d_pidx, d_pval, d_walked, d_rows, d_rcount, d_roff, d_rbsum,
-        d_sp_node, d_sp_len, d_umasks, d_B, d_poff, d_pcnt, d_obsum, d_Ssp;
+        d_sp_node, d_sp_len, d_umasks, d_B, d_poff, d_pcnt, d_obsum, d_Ssp, d_out2, d_pspmax;
 
     hipStream_t own_st = nullptr;    // created with the plan; st may point at a caller's stream
     // per-execution inputs and output (nullptr: the plan's own table data / totals / result)
@@ -457,7 +458,11 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     // The shared-node weighted kernel needs u >= 0 (so min(u, v) vanishes when
     // either sample lacks the node) and finite lengths (S_i + S_j - 2M).
     const bool weighted = spec.method == Method::weighted_normalized || spec.method == Method::weighted_unnormalized;
-    P.sparse = weighted && !spec.vaw && spec.plain_kernel == 0;
+    // generalized (FP32, 0 <= alpha <= 1) shares the shared-node path: numerator and
+    // denominator summed directly on the tiles, posting-list shares for the rare nodes
+    P.gen = spec.method == Method::generalized && !spec.vaw && !spec.fp64 && spec.plain_kernel == 0 &&
+            spec.alpha >= 0.0 && spec.alpha <= 1.0 && !(getenv("SSU_GEN_SPARSE") && atoi(getenv("SSU_GEN_SPARSE")) == 0);
+    P.sparse = (weighted && !spec.vaw && spec.plain_kernel == 0) || P.gen;
     if (P.sparse) {
         for (size_t q = 0; q < P.klen_d.size() && P.sparse; ++q)
             if (!std::isfinite(spec.fp64 ? P.klen_d[q] : (double)P.klen_f[q])) P.sparse = false;
@@ -466,6 +471,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         for (uint64_t q = 0; q < nz && P.sparse; ++q)
             if (!(table.data[b0 + q] >= 0.0)) P.sparse = false;
     }
+    P.gen = P.gen && P.sparse;
     // Embedding blocks: the FP64 embedding of every node for every sample
     // (m x ldn doubles) is built in sample-column blocks when it would exceed
     // EMBED_BUDGET; each block feeds the FP32 operand, masks, sums or presence
@@ -582,6 +588,8 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             !P.d_poff.ensure((kk + 1) * sizeof(uint32_t), err) ||
             !P.d_pcnt.ensure(kk * sizeof(uint32_t), err) ||
             !P.d_Ssp.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err) ||
+            (P.gen && !P.d_out2.ensure(std::max<size_t>(nn, 1) * sizeof(float), err)) ||
+            (P.gen && !P.d_pspmax.ensure(sizeof(double), err)) ||
             !P.d_obsum.ensure((offset_scan_blocks(K) + 1) * sizeof(uint32_t), err) ||
             !P.d_walked.ensure((size_t)sparse_rows_blocks(P.n) * sizeof(uint32_t), err) ||
             !P.d_rows.ensure((size_t)P.split_slots * sizeof(unsigned long long), err) ||
@@ -679,6 +687,10 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         b.slots = P.split_slots;
         b.thr = P.split_thr;
         b.ssp = P.d_Ssp.as<double>();
+        b.gen = P.gen;
+        b.alpha = (float)s.alpha;
+        b.out2 = P.gen ? P.d_out2.p : nullptr;
+        b.pspmax = P.gen ? P.d_pspmax.as<double>() : nullptr;
         SSU_HIP(launch_sparse_nodes(b, P.st));
     }
     SparseNodesArgs gb{};
@@ -746,6 +758,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         a.n_blocks = (uint32_t)P.block_order.size();
         if (split) a.dmask = P.d_dmask.as<uint32_t>();   // dense nodes only; adds the posting-list share
         if (s.fp64) SSU_HIP(launch_pairs_sparse(true, a, P.st));
+        else if (P.gen) SSU_HIP(launch_pairs_gen32(a, (float)s.alpha, split ? P.d_out2.p : nullptr, P.st));
         else SSU_HIP(launch_pairs_sparse32(a, P.st));
         SSU_HIP(P.mark(3));
         SSU_HIP(P.mark(4));
@@ -884,7 +897,7 @@ uint64_t Plan::sparse_terms() const {
 bool Plan::out_fp64() const { return p_->spec.fp64; }
 const char* Plan::main_kernel() const {
     if (p_->gram) return "k_gram_i8";
-    if (p_->sparse) return p_->spec.fp64 ? "k_pairs_sparse" : "k_pairs_sparse32";
+    if (p_->sparse) return p_->spec.fp64 ? "k_pairs_sparse" : (p_->gen ? "k_pairs_gen32" : "k_pairs_sparse32");
     return "k_pairs";
 }
 const void* Plan::device_output() const { return p_->d_out.p; }

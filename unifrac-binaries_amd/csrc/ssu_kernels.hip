@@ -911,15 +911,27 @@ constexpr uint32_t S3_JOFF = 32;       // the 32 row values come first, contiguo
 constexpr uint32_t S3_NODE = S3_JOFF + 8 * S3_GSJ;
 
 
-template <class TOut>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_pairs_sparse32(const float* __restrict__ U, size_t ldn,
+// generalized (GEN) terms, FP32 as in the dense kernel: w = s^a l, q = s^(a-1) l
+// from the hardware log2 / exp2 (s = u + v > 0), num += q |u - v|, den += w
+__device__ __forceinline__ void gen_term(float u, float v, float l, float alpha, float& num, float& den) {
+    const float s = u + v;
+    const bool nz = s != 0.0f;
+    const float lg = __builtin_amdgcn_logf(nz ? s : 1.0f);
+    const float w = nz ? __builtin_amdgcn_exp2f(alpha * lg) * l : 0.0f;
+    const float q = nz ? __builtin_amdgcn_exp2f((alpha - 1.0f) * lg) * l : 0.0f;
+    num = __builtin_fmaf(q, __builtin_fabsf(u - v), num);
+    den += w;
+}
+
+template <class TOut, bool GEN>
+__device__ __forceinline__ void pairs_sparse32_body(const float* __restrict__ U, size_t ldn,
                                                        const double* __restrict__ len,
                                                        const uint32_t* __restrict__ masks, uint32_t Kw, uint32_t n,
                                                        const double* __restrict__ S, int normalized,
                                                        TOut* __restrict__ out, uint32_t* __restrict__ visits,
                                                        const uint32_t* __restrict__ order, uint32_t n_blocks,
                                                        uint32_t per_xcd, const uint32_t* __restrict__ dmask,
-                                                       int add_partial) {
+                                                       int add_partial, float alpha, TOut* __restrict__ out2) {
     constexpr uint32_t EB = S3_EB;
     const uint32_t seq = (blockIdx.x % N_XCDS) * per_xcd + blockIdx.x / N_XCDS;
     if (seq >= n_blocks) return;
@@ -934,10 +946,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 
     float acc[4][8];
     double acc64[4][8];
+    float accd[GEN ? 4 : 1][GEN ? 8 : 1];        // GEN: the denominator
+    double accd64[GEN ? 4 : 1][GEN ? 8 : 1];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) { acc[r][c] = 0.0f; acc64[r][c] = 0.0; }
+        for (int c = 0; c < 8; ++c) {
+            acc[r][c] = 0.0f;
+            acc64[r][c] = 0.0;
+            if constexpr (GEN) { accd[r][c] = 0.0f; accd64[r][c] = 0.0; }
+        }
     uint32_t pending = 0;
 
     const uint32_t* mI = masks + (size_t)pi * Kw;
@@ -996,14 +1014,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) acc[r][c] = __builtin_fmaf(__builtin_fabsf(a[r] - b[c]), ln, acc[r][c]);
+                    for (int c = 0; c < 8; ++c) {
+                        if constexpr (GEN)
+                            gen_term(a[r], b[c], ln, alpha, acc[r][c], accd[r][c]);
+                        else
+                            acc[r][c] = __builtin_fmaf(__builtin_fabsf(a[r] - b[c]), ln, acc[r][c]);
+                    }
             }
             pending += EB;
             if (pending >= S3_MERGE) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) { acc64[r][c] += (double)acc[r][c]; acc[r][c] = 0.0f; }
+                    for (int c = 0; c < 8; ++c) {
+                        acc64[r][c] += (double)acc[r][c];
+                        acc[r][c] = 0.0f;
+                        if constexpr (GEN) { accd64[r][c] += (double)accd[r][c]; accd[r][c] = 0.0f; }
+                    }
                 pending = 0;
             }
             __syncthreads();
@@ -1029,10 +1056,38 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             double x = acc64[r][c] + (double)acc[r][c];
             if (add_partial) x += (double)out[i * n + j];   // the sparse nodes' share, from k_pairs_rows
             x = x < 0.0 ? 0.0 : x;   // the share's rounding can undershoot 0; NaN (empty samples) passes
-            const double d = normalized ? x / (S[i] + S[j]) : x;
+            double d;
+            if constexpr (GEN) {
+                double y = accd64[r][c] + (double)accd[r][c];
+                if (add_partial) y += (double)out2[i * n + j];
+                d = x / y;   // 0 / 0 (no node held by either sample) gives NaN, as the dense kernel
+            } else {
+                d = normalized ? x / (S[i] + S[j]) : x;
+            }
             out[i * n + j] = (TOut)d;
             out[j * n + i] = (TOut)d;
         }
+}
+
+template <class TOut>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_pairs_sparse32(
+        const float* __restrict__ U, size_t ldn, const double* __restrict__ len, const uint32_t* __restrict__ masks,
+        uint32_t Kw, uint32_t n, const double* __restrict__ S, int normalized, TOut* __restrict__ out,
+        uint32_t* __restrict__ visits, const uint32_t* __restrict__ order, uint32_t n_blocks, uint32_t per_xcd,
+        const uint32_t* __restrict__ dmask, int add_partial) {
+    pairs_sparse32_body<TOut, false>(U, ldn, len, masks, Kw, n, S, normalized, out, visits, order, n_blocks, per_xcd,
+                                     dmask, add_partial, 0.0f, nullptr);
+}
+
+// generalized: numerator and denominator summed directly (two accumulator sets, 2 waves per SIMD)
+template <class TOut>
+__global__ void __launch_bounds__(64) k_pairs_gen32(
+        const float* __restrict__ U, size_t ldn, const double* __restrict__ len, const uint32_t* __restrict__ masks,
+        uint32_t Kw, uint32_t n, TOut* __restrict__ out, TOut* __restrict__ out2, uint32_t* __restrict__ visits,
+        const uint32_t* __restrict__ order, uint32_t n_blocks, uint32_t per_xcd, const uint32_t* __restrict__ dmask,
+        int add_partial, float alpha) {
+    pairs_sparse32_body<TOut, true>(U, ldn, len, masks, Kw, n, nullptr, 1, out, visits, order, n_blocks, per_xcd,
+                                    dmask, add_partial, alpha, out2);
 }
 
 // --------------------------------------------- sparse nodes: posting lists
@@ -1247,13 +1302,29 @@ constexpr uint32_t SR_WIN = 1024;  // walk positions resolved to entries at a ti
 // and two samples holding the same nodes get bit-equal G(a, b) and G(a, a).
 // Posting slots are addressed with 32-bit offsets (fewer than 2^30 slots).
 // A round entry as the walk reads it: one 16-byte LDS read for FP32 values
-template <class TV> struct alignas(16) RowEntry {
+template <class TV, bool GEN> struct alignas(16) RowEntry {
     uint32_t base;            // slot of walk position t: base + t (mod 2^32)
     TV lu;                    // the row's value in this node (weighted)
     unsigned long long q8;    // GRAM: the fixed-point term; weighted: l * scale (FP64 bits)
 };
+template <class TV> struct alignas(16) RowEntry<TV, true> {   // generalized
+    uint32_t base;
+    float lu;                 // u_a
+    float uap;                // u_a^alpha
+    unsigned long long q8;    // l * scale (FP64 bits)
+};
 
-template <class TOut, class IX, bool GRAM, class TV>
+// x^alpha for x > 0 from the hardware log2 / exp2, as the generalized terms use them
+__device__ __forceinline__ float pow_alpha(float x, float alpha) {
+    return __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(x));
+}
+
+// signed fixed point (two's complement in the 64-bit accumulators), |x| < 2^62, rounded
+__device__ __forceinline__ unsigned long long to_fixed_signed(double x) {
+    return (unsigned long long)__double2ll_rn(x);
+}
+
+template <class TOut, class IX, bool GRAM, class TV, bool GEN>
 __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__ out, uint32_t* __restrict__ walked,
                                                    const unsigned long long* __restrict__ rows,
                                                    const uint32_t* __restrict__ offsets,
@@ -1263,11 +1334,15 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                                                    uint32_t K, uint32_t n, uint32_t npan,
                                                    const uint32_t* __restrict__ poff, uint32_t slots,
                                                    uint32_t nchunks, uint32_t nkc, double* __restrict__ G,
-                                                   uint32_t tiles, const double* __restrict__ ssp) {
+                                                   uint32_t tiles, const double* __restrict__ ssp,
+                                                   TOut* __restrict__ out2, const double* __restrict__ pspmax,
+                                                   float alpha) {
+    // GEN keeps two accumulators (numerator, denominator) on half-width chunks
+    constexpr uint32_t CW = GEN ? SR_CW / 2 : SR_CW;
     // chunk-major: the rows in flight share one column chunk of every posting
     // list, a quarter of the posting data at 16k samples (cache-resident)
     const uint32_t a = blockIdx.x % n, c = blockIdx.x / n;
-    const uint32_t b0 = c * SR_CW, b1 = min(n, b0 + SR_CW);
+    const uint32_t b0 = c * CW, b1 = min(n, b0 + CW);
     // uniform: nothing at or above the diagonal here (weighted: strictly above); more
     // postings than slots never happens (the plan's bound) and is skipped, not read
     if (a >= n || (GRAM ? b1 <= a : b1 <= a + 1) || poff[K] > slots) {
@@ -1277,14 +1352,15 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     __shared__ unsigned long long acc[SR_CW];
     __shared__ uint32_t wk[SR_WAVES];
-    __shared__ RowEntry<TV> ent_s[SR_WAVES][64];   // per wave: the round's 64 entries
+    using Ent = RowEntry<TV, GEN>;
+    __shared__ Ent ent_s[SR_WAVES][64];   // per wave: the round's 64 entries
     // per entry: GRAM the fixed-point term, weighted l_r * scale (FP64 bits)
     __shared__ unsigned long long diag_s;
     __shared__ double bsum_s[SR_WAVES];
     __shared__ uint4 own_s[SR_WAVES][SR_WIN / 16];   // per wave: entry + 1 of every window position
     uint4* own4 = own_s[wv];
     uint8_t* own = reinterpret_cast<uint8_t*>(own4);
-    RowEntry<TV>* ents = ent_s[wv];
+    Ent* ents = ent_s[wv];
     for (uint32_t q = threadIdx.x; q < SR_CW; q += 64 * SR_WAVES) acc[q] = 0ull;
     const uint32_t e0 = offsets[(size_t)a * nkc], e1 = offsets[(size_t)(a + 1) * nkc];
     double bound;
@@ -1300,6 +1376,9 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         __syncthreads();
         bound = 0.0;
         for (uint32_t w = 0; w < SR_WAVES; ++w) bound += bsum_s[w];
+    } else if constexpr (GEN) {
+        // |l (s^(a-1) |u - v| - u^a - v^a)| and |l (s^a - u^a - v^a)| are at most 3 l (u^a + v^a)
+        bound = 3.0 * (ssp[a] + pspmax[0]);
     } else {
         bound = S[a];
     }
@@ -1322,6 +1401,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     for (uint32_t r0 = e0 + 64 * wv; r0 < e1; r0 += 64 * SR_WAVES) {   // uniform per wave
         uint32_t start = 0, end = 0, r = 0, slot0 = 0;
         TV ua = (TV)0;
+        float uap = 0.0f;
         unsigned long long q = 0ull;
         if (r0 + lane < e1) {
             const unsigned long long ent = rows[r0 + lane];
@@ -1336,6 +1416,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             } else {
                 ua = pval[slot0 + pa];
                 q = (unsigned long long)__double_as_longlong(len[r] * scale);
+                if constexpr (GEN) uap = pow_alpha((float)ua, alpha);
             }
         }
         const uint32_t cnt = end > start ? end - start : 0u;
@@ -1347,10 +1428,11 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         }
         const uint32_t T = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
         if (T == 0) continue;   // uniform per wave
-        RowEntry<TV> me;
+        Ent me;
         me.base = slot0 + start - (incl - cnt);   // slot of walk position t: base + t (mod 2^32)
         me.lu = ua;
         me.q8 = q;
+        if constexpr (GEN) me.uap = uap;
         ents[lane] = me;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1398,7 +1480,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             // SR_UNROLL postings per lane in flight: slots first, then every load, then the adds
             for (uint32_t j0 = lane; j0 < wn; j0 += 64 * SR_UNROLL) {
                 uint32_t qq[SR_UNROLL];
-                RowEntry<TV> en[SR_UNROLL];
+                Ent en[SR_UNROLL];
 #pragma unroll
                 for (uint32_t u = 0; u < SR_UNROLL; ++u) {
                     const uint32_t j = min(j0 + 64 * u, wn - 1);
@@ -1417,6 +1499,16 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                     if (j0 + 64 * u < wn) {
                         if constexpr (GRAM) {
                             atomicAdd(&acc[bq[u] - b0], en[u].q8);
+                        } else if constexpr (GEN) {
+                            // a pair's correction to u^a + v^a (held by both): numerator and denominator
+                            const float uu = (float)en[u].lu, vv = (float)vq[u], sm = uu + vv;
+                            const float p1 = __builtin_amdgcn_exp2f((alpha - 1.0f) * __builtin_amdgcn_logf(sm));
+                            const double one = (double)en[u].uap + (double)pow_alpha(vv, alpha);
+                            const double ls = __longlong_as_double((long long)en[u].q8);
+                            const double cn = (double)p1 * (double)__builtin_fabsf(uu - vv) - one;
+                            const double cd = (double)p1 * (double)sm - one;
+                            atomicAdd(&acc[bq[u] - b0], to_fixed_signed(cn * ls));
+                            atomicAdd(&acc[CW + bq[u] - b0], to_fixed_signed(cd * ls));
                         } else {
                             const double m = vmin_nonneg(en[u].lu, vq[u]);
                             const double ls = __longlong_as_double((long long)en[u].q8);
@@ -1449,6 +1541,15 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             const unsigned long long v = acc[b - b0];
             if (v) G[gram_slot(a, b, tiles)] += (double)v * inv;
         }
+    } else if constexpr (GEN) {
+        // numerator and denominator shares: P_a + P_b plus the corrections of the
+        // shared nodes (ssp = per-sample sum of l u^a over the sparse nodes)
+        const double sa = ssp[a];
+        for (uint32_t b = max(a + 1, b0) + threadIdx.x; b < b1; b += 64 * SR_WAVES) {
+            const double base2 = sa + ssp[b];
+            out[(size_t)a * n + b] = (TOut)(base2 + (double)(long long)acc[b - b0] * inv);
+            out2[(size_t)a * n + b] = (TOut)(base2 + (double)(long long)acc[CW + b - b0] * inv);
+        }
     } else if constexpr (sizeof(TV) == 4) {
         // FP32 values: the sparse nodes' share of sum l |u_a - u_b| itself, formed in
         // FP64 before the rounding to FP32
@@ -1464,13 +1565,13 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
 // ssp[a] = sum over sample a's row list (its sparse nodes r, increasing) of
 // len[r] * value of a in r's postings, in FP64 in a fixed order: one wavefront
 // per sample, lane-strided partial sums then a butterfly.
-template <class TV>
+template <class TV, bool GEN>
 __global__ void __launch_bounds__(64) k_sparse_sums(double* __restrict__ ssp,
                                                     const unsigned long long* __restrict__ rows,
                                                     const uint32_t* __restrict__ offsets, uint32_t nkc,
                                                     const uint32_t* __restrict__ poff, const TV* __restrict__ pval,
                                                     const double* __restrict__ len, uint32_t K, uint32_t n,
-                                                    uint32_t slots) {
+                                                    uint32_t slots, float alpha) {
     const uint32_t a = blockIdx.x, lane = threadIdx.x;
     if (a >= n) return;
     double x = 0.0;
@@ -1479,12 +1580,30 @@ __global__ void __launch_bounds__(64) k_sparse_sums(double* __restrict__ ssp,
         for (uint32_t e = e0 + lane; e < e1; e += 64) {
             const unsigned long long ent = rows[e];
             const uint32_t r = (uint32_t)(ent >> 32);
-            x += len[r] * (double)pval[poff[r] + (uint32_t)ent];
+            const TV v = pval[poff[r] + (uint32_t)ent];
+            if constexpr (GEN)
+                x += len[r] * (double)pow_alpha((float)v, alpha);   // (GEN: l u^a)
+            else
+                x += len[r] * (double)v;
         }
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
     if (lane == 0) ssp[a] = x;
+}
+
+// mx[0] = max of v[0..n): one workgroup
+__global__ void __launch_bounds__(256) k_max(double* __restrict__ mx, const double* __restrict__ v, uint32_t n) {
+    __shared__ double part[256];
+    double m = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) m = fmax(m, v[i]);
+    part[threadIdx.x] = m;
+    __syncthreads();
+    for (uint32_t d = 128; d > 0; d >>= 1) {
+        if (threadIdx.x < d) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + d]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mx[0] = part[0];
 }
 
 // cnt[r] += samples of the block with E[knodes[r]][s] > 0: one workgroup per node
@@ -1796,23 +1915,32 @@ hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
 }
 
 // weighted: the partial sums go out in the value type (FP32 or FP64 output)
-template <class IX, bool GRAM, class TV>
+template <class IX, bool GRAM, class TV, bool GEN = false>
 static void pairs_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs& a, uint32_t nchunks, uint32_t nkc) {
-    hipLaunchKernelGGL((k_pairs_rows<TV, IX, GRAM, TV>), dim3(blocks), dim3(64 * SR_WAVES), 0, st,
+    hipLaunchKernelGGL((k_pairs_rows<TV, IX, GRAM, TV, GEN>), dim3(blocks), dim3(64 * SR_WAVES), 0, st,
                        GRAM ? nullptr : (TV*)a.out, a.walked, a.rows, a.roff, a.pc, (const IX*)a.pidx,
                        GRAM ? nullptr : (const TV*)a.pval, a.len, GRAM ? nullptr : a.S, a.K, a.n, a.npan,
-                       (const uint32_t*)a.poff, a.slots, nchunks, nkc, a.gram, a.n_pad / GRAM_BM, a.ssp);
+                       (const uint32_t*)a.poff, a.slots, nchunks, nkc, a.gram, a.n_pad / GRAM_BM, a.ssp,
+                       (TV*)a.out2, a.pspmax, a.alpha);
 }
 
 hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st) {
     if (!a.K || !a.n) return hipSuccess;
     if (!sparse_args_ok(a)) return hipErrorInvalidValue;
     const uint32_t nkc = row_list_chunks(a.K);
-    const uint32_t nchunks = (a.n + SR_CW - 1) / SR_CW;
+    const uint32_t cw = a.gen ? SR_CW / 2 : SR_CW;
+    const uint32_t nchunks = (a.n + cw - 1) / cw;
     const uint64_t blocks = (uint64_t)a.n * nchunks;
     if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
     const bool narrow = a.n <= 65536;
-    if (a.gram) {
+    if (a.gen) {   // generalized, FP32 values
+        if (a.gram || a.values_fp64 || !a.ssp || !a.out2 || !a.pspmax) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_sparse_sums<float, true>), dim3(a.n), dim3(64), 0, st, a.ssp, a.rows, a.roff, nkc,
+                           (const uint32_t*)a.poff, (const float*)a.pval, a.len, a.K, a.n, a.slots, a.alpha);
+        hipLaunchKernelGGL(k_max, dim3(1), dim3(256), 0, st, a.pspmax, a.ssp, a.n);
+        if (narrow) pairs_rows_t<uint16_t, false, float, true>((uint32_t)blocks, st, a, nchunks, nkc);
+        else pairs_rows_t<uint32_t, false, float, true>((uint32_t)blocks, st, a, nchunks, nkc);
+    } else if (a.gram) {
         if (narrow) pairs_rows_t<uint16_t, true, float>((uint32_t)blocks, st, a, nchunks, nkc);
         else pairs_rows_t<uint32_t, true, float>((uint32_t)blocks, st, a, nchunks, nkc);
     } else if (a.values_fp64) {
@@ -1820,8 +1948,8 @@ hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st) {
         else pairs_rows_t<uint32_t, false, double>((uint32_t)blocks, st, a, nchunks, nkc);
     } else {
         if (!a.ssp) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_sparse_sums<float>, dim3(a.n), dim3(64), 0, st, a.ssp, a.rows, a.roff, nkc,
-                           (const uint32_t*)a.poff, (const float*)a.pval, a.len, a.K, a.n, a.slots);
+        hipLaunchKernelGGL((k_sparse_sums<float, false>), dim3(a.n), dim3(64), 0, st, a.ssp, a.rows, a.roff, nkc,
+                           (const uint32_t*)a.poff, (const float*)a.pval, a.len, a.K, a.n, a.slots, 0.0f);
         if (narrow) pairs_rows_t<uint16_t, false, float>((uint32_t)blocks, st, a, nchunks, nkc);
         else pairs_rows_t<uint32_t, false, float>((uint32_t)blocks, st, a, nchunks, nkc);
     }
@@ -1842,11 +1970,23 @@ hipError_t launch_count_present(uint32_t* cnt, const double* E, size_t lde, uint
     return hipGetLastError();
 }
 
-uint32_t sparse_rows_blocks(uint32_t n) { return n * ((n + SR_CW - 1) / SR_CW); }
+uint32_t sparse_rows_blocks(uint32_t n) { return n * ((n + SR_CW / 2 - 1) / (SR_CW / 2)); }   // (half-width chunks)
 uint32_t row_list_chunks(uint32_t K) { return std::max<uint32_t>(1, (K + RL_KC - 1) / RL_KC); }
 size_t offset_scan_blocks(uint32_t K) { return ((size_t)K + SCAN_BLOCK - 1) / SCAN_BLOCK; }
 size_t row_list_scan_blocks(uint32_t n, uint32_t K) {
     return ((size_t)n * row_list_chunks(K) + SCAN_BLOCK - 1) / SCAN_BLOCK;
+}
+
+hipError_t launch_pairs_gen32(const SparseArgs& a, float alpha, void* out2, hipStream_t st) {
+    if (a.ldn % S3_BJ || a.out_fp64) return hipErrorInvalidValue;
+    const uint32_t nbj = (a.n + S3_BJ - 1) / S3_BJ;
+    if (!nbj) return hipSuccess;
+    if (a.dmask && !out2) return hipErrorInvalidValue;
+    const uint32_t per_xcd = (a.n_blocks + N_XCDS - 1) / N_XCDS;
+    hipLaunchKernelGGL((k_pairs_gen32<float>), dim3(per_xcd * N_XCDS), dim3(64), 0, st, (const float*)a.U, a.ldn, a.len,
+                       a.masks, a.Kw, a.n, (float*)a.out, (float*)out2, a.visits, a.order, a.n_blocks, per_xcd,
+                       a.dmask, (int)(a.dmask != nullptr), alpha);
+    return hipGetLastError();
 }
 
 hipError_t launch_pairs_sparse32(const SparseArgs& a, hipStream_t st) {

@@ -175,7 +175,11 @@ struct SparseNodesArgs {
     uint32_t* roff;         // [n * row_list_chunks(K) + 1]
     uint32_t* rbsum;        // [row_list_scan_blocks(n, K) + 1]
     uint32_t K, n, npan;
-    double* ssp = nullptr;  // (weighted, FP32 values) [n]: per-sample sum of l * u over its sparse nodes
+    double* ssp = nullptr;  // (weighted, FP32 values) [n]: per-sample sum of l * u (GEN: l * u^alpha) over its sparse nodes
+    bool gen = false;       // generalized (FP32 values): numerator shares to out, denominator shares to out2
+    float alpha = 1.0f;
+    void* out2 = nullptr;
+    double* pspmax = nullptr;   // (GEN) [1] scratch: max of ssp
     double* gram = nullptr; // unweighted: the Gram tiles of launch_gram_i8
     uint32_t n_pad = 0;     // their padded edge
 };
@@ -202,6 +206,10 @@ hipError_t launch_pairs_sparse(bool in_fp64, const SparseArgs& a, hipStream_t st
 // and visits is [ceil(n/32)][ceil(n/64)].
 std::vector<uint32_t> rect_block_order(uint32_t n);
 hipError_t launch_pairs_sparse32(const SparseArgs& a, hipStream_t st);
+// generalized (FP32, 0 <= alpha <= 1) on the same blocks: numerator and denominator summed
+// directly over the nodes either panel holds; with dmask, adds the posting-list shares in
+// out (numerator) and out2 (denominator)
+hipError_t launch_pairs_gen32(const SparseArgs& a, float alpha, void* out2, hipStream_t st);
 
 // --------------------------------------------------------- output layouts
 // stripes[s - start][k] = M[k][(k+s+1) % n]   (double output)
