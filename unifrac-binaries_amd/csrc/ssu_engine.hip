@@ -332,7 +332,8 @@ bool PlanImpl::split_gram(std::string* err) {
 // posting and row-list arrays and must stay below 2^30 (32-bit slot offsets in
 // the kernels): theta is lowered to fit.  SSU_SPARSE_SPLIT sets theta (0: off).
 bool PlanImpl::split_weighted(std::string* err) {
-    double theta = 0.25;
+    // generalized terms cost several times the weighted ones on the tiles, which moves the balance
+    double theta = gen ? 0.6 : 0.25;
     if (const char* env = getenv("SSU_SPARSE_SPLIT")) theta = atof(env);
     bool nonneg = true;
     for (size_t q = 0; q < knodes.size(); ++q) nonneg = nonneg && (spec.fp64 ? klen_d[q] >= 0.0 : klen_f[q] >= 0.0f);
