@@ -294,3 +294,24 @@ def test_rejects_bad_arguments(ssu, synth):
         ssu.Rarefaction(inp, ["unweighted"], 10, 0)
     with pytest.raises(RuntimeError, match=f"status {ssu.UNKNOWN_METHOD}"):
         ssu.Rarefaction(inp, ["bogus"], 10, 2)
+
+
+def test_generalized_fp32_replicates_and_replays(ssu, O, synth):
+    """FP32 generalized (shared-node path, occupancy split) on rarefied
+    replicates: every replicate against the oracle, graph replays equal eager."""
+    tree, table = small_case(synth, seed=53)
+    depth = depth_dropping_some(table, 10)
+    say("rarefy generalized fp32", depth)
+    r = ssu.Rarefaction(inputs(ssu, tree, table), ["generalized"], depth, 3, seed=8, alpha=0.5)
+    r.execute()
+    si, oi, ip, ix = r.kept()
+    st, le, nm = tree
+    otree = O.Tree(np.asarray(st, np.uint8), le, nm)
+    eager = [r.matrix(0, rep) for rep in range(3)]
+    for rep in range(3):
+        t = O.Table([table[0][i] for i in oi], [table[1][s] for s in si], ip, ix, r.counts(rep))
+        assert rel_err(eager[rep], O.matrix(otree, t, "generalized", alpha=0.5)) <= TOL["fp32"], rep
+    r.execute()
+    assert r.graph_used()
+    for rep in range(3):
+        assert np.array_equal(r.matrix(0, rep), eager[rep]), rep
