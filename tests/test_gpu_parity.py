@@ -313,6 +313,25 @@ def test_weighted_split_across_row_chunks(ssu, O, synth, monkeypatch, prec):
     assert rel_err(out["0.2"][np.ix_(idx, idx)], ref) <= TOL[prec]
 
 
+def test_sparse_terms_count_launched_rows_only(ssu, synth, monkeypatch):
+    """sparse_terms() sums the walked postings of the workgroups the row walk
+    launched: a generalized plan (half-width column chunks, twice the
+    workgroups) leaves its counts in memory a later weighted plan may get, and
+    the weighted count must not include them."""
+    tree, table = _case(synth, 300, 4500, 0.04, 79)
+    inp = inputs_from(ssu, tree, table)
+    monkeypatch.setenv("SSU_SPARSE_SPLIT", "0.2")
+    counts = []
+    for method, kw in (("weighted_normalized", {}), ("generalized", {"alpha": 0.5}), ("weighted_normalized", {})):
+        say("sparse terms", method)
+        p = ssu.Plan(inp, method, **kw)
+        p.execute()
+        counts.append(p.sparse_terms())
+        p.close()
+    assert counts[0] > 0 and counts[2] == counts[0]
+    assert counts[1] == counts[0]   # the same sparse nodes at the same theta: the same postings
+
+
 def test_weighted_duplicate_samples(ssu, O, synth):
     """Identical samples: d = 0 exactly in FP64 arithmetic; the shared-node
     kernel gets S_i + S_j - 2M, clamped at 0, within the absolute bound."""

@@ -889,7 +889,7 @@ uint64_t Plan::kernel_terms() const {
 uint64_t Plan::sparse_terms() const {
     const PlanImpl& P = *p_;
     if (!((P.sparse && P.split_thr) || (P.gram && P.gsplit_slots)) || !P.d_walked.p) return 0;
-    std::vector<uint32_t> wk(sparse_rows_blocks(P.n));
+    std::vector<uint32_t> wk(sparse_rows_launched(P.n, P.gen && !P.gram));
     if (hipMemcpy(wk.data(), P.d_walked.p, wk.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) return 0;
     uint64_t sum = 0;
     for (uint32_t x : wk) sum += x;

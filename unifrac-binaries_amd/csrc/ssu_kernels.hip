@@ -1324,6 +1324,42 @@ __device__ __forceinline__ unsigned long long to_fixed_signed(double x) {
     return (unsigned long long)__double2ll_rn(x);
 }
 
+// p[i] through a 32-bit byte offset from the (wave-uniform) base, so the load
+// takes the SGPR-base form: i * sizeof(T) < 2^32 (slots < 2^30, T of 4 bytes at most)
+template <class T> __device__ __forceinline__ T ld_slot(const T* __restrict__ p, uint32_t i) {
+    if constexpr (sizeof(T) <= 4)
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (uint32_t)(i * (uint32_t)sizeof(T)));
+    else
+        return p[i];
+}
+
+// One posting's term added to the row's accumulators; accb is the accumulator
+// array shifted so that accb[b] is column b (GEN: the denominator CW further on).
+template <class TV, bool GRAM, bool GEN, bool FINE, uint32_t CW>
+__device__ __forceinline__ void row_term(unsigned long long* accb, const RowEntry<TV, GEN>& en, uint32_t b, TV v,
+                                         float alpha) {
+    if constexpr (GRAM) {
+        atomicAdd(&accb[b], en.q8);
+    } else if constexpr (GEN) {
+        // a pair's correction to u^a + v^a (held by both): numerator and denominator
+        const float uu = (float)en.lu, vv = (float)v, sm = uu + vv;
+        const float p1 = __builtin_amdgcn_exp2f((alpha - 1.0f) * __builtin_amdgcn_logf(sm));
+        const double one = (double)en.uap + (double)pow_alpha(vv, alpha);
+        const double ls = __longlong_as_double((long long)en.q8);
+        const double cn = (double)p1 * (double)__builtin_fabsf(uu - vv) - one;
+        const double cd = (double)p1 * (double)sm - one;
+        atomicAdd(&accb[b], to_fixed_signed(cn * ls));
+        atomicAdd(&accb[CW + b], to_fixed_signed(cd * ls));
+    } else {
+        const double m = vmin_nonneg(en.lu, v);
+        const double ls = __longlong_as_double((long long)en.q8);
+        if constexpr (FINE)
+            atomicAdd(&accb[b], to_fixed(m * ls));
+        else
+            atomicAdd(&accb[b], fixed_term(m, ls));
+    }
+}
+
 template <class TOut, class IX, bool GRAM, class TV, bool GEN>
 __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__ out, uint32_t* __restrict__ walked,
                                                    const unsigned long long* __restrict__ rows,
@@ -1353,7 +1389,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     __shared__ unsigned long long acc[SR_CW];
     __shared__ uint32_t wk[SR_WAVES];
     using Ent = RowEntry<TV, GEN>;
-    __shared__ Ent ent_s[SR_WAVES][64];   // per wave: the round's 64 entries
+    __shared__ Ent ent_s[SR_WAVES][65];   // per wave: the round's 64 entries at 1..64
     // per entry: GRAM the fixed-point term, weighted l_r * scale (FP64 bits)
     __shared__ unsigned long long diag_s;
     __shared__ double bsum_s[SR_WAVES];
@@ -1389,6 +1425,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     constexpr bool FINE = GRAM || sizeof(TV) == 8;
     const int fb = FINE ? 61 : 51;
     const double scale = ldexp(1.0, fb - e2), inv = ldexp(1.0, e2 - fb);
+    unsigned long long* const accb = acc - b0;   // accb[b]: column b (b0 <= b < b1)
     const bool has_diag = GRAM && b0 <= a && a < b1;
     if (GRAM && has_diag) {   // G(a, a): the same fixed-point terms, every one of them
         unsigned long long x = 0ull;
@@ -1433,7 +1470,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         me.lu = ua;
         me.q8 = q;
         if constexpr (GEN) me.uap = uap;
-        ents[lane] = me;
+        ents[lane + 1] = me;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1477,47 +1514,45 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // SR_UNROLL postings per lane in flight: slots first, then every load, then the adds
-            for (uint32_t j0 = lane; j0 < wn; j0 += 64 * SR_UNROLL) {
-                uint32_t qq[SR_UNROLL];
-                Ent en[SR_UNROLL];
+            // SR_UNROLL postings per lane in flight: entries and slots first, then
+            // every load, then the adds.  A full window (every round but the last
+            // window of each) has each lane take positions lane + 64 i, no guards.
+            const uint32_t wb = w0 + lane;
+            if (wn == SR_WIN) {   // uniform
 #pragma unroll
-                for (uint32_t u = 0; u < SR_UNROLL; ++u) {
-                    const uint32_t j = min(j0 + 64 * u, wn - 1);
-                    en[u] = ents[(uint32_t)own[j] - 1u];
-                    qq[u] = en[u].base + w0 + j;
-                }
-                uint32_t bq[SR_UNROLL];
-                TV vq[SR_UNROLL];
+                for (uint32_t g = 0; g < SR_WIN / 64; g += SR_UNROLL) {
+                    Ent en[SR_UNROLL];
+                    uint32_t bq[SR_UNROLL];
+                    TV vq[SR_UNROLL];
 #pragma unroll
-                for (uint32_t u = 0; u < SR_UNROLL; ++u) {
-                    bq[u] = (uint32_t)pidx[qq[u]];
-                    if constexpr (!GRAM) vq[u] = pval[qq[u]];
-                }
+                    for (uint32_t u = 0; u < SR_UNROLL; ++u) en[u] = ents[own[lane + 64 * (g + u)]];
 #pragma unroll
-                for (uint32_t u = 0; u < SR_UNROLL; ++u)
-                    if (j0 + 64 * u < wn) {
-                        if constexpr (GRAM) {
-                            atomicAdd(&acc[bq[u] - b0], en[u].q8);
-                        } else if constexpr (GEN) {
-                            // a pair's correction to u^a + v^a (held by both): numerator and denominator
-                            const float uu = (float)en[u].lu, vv = (float)vq[u], sm = uu + vv;
-                            const float p1 = __builtin_amdgcn_exp2f((alpha - 1.0f) * __builtin_amdgcn_logf(sm));
-                            const double one = (double)en[u].uap + (double)pow_alpha(vv, alpha);
-                            const double ls = __longlong_as_double((long long)en[u].q8);
-                            const double cn = (double)p1 * (double)__builtin_fabsf(uu - vv) - one;
-                            const double cd = (double)p1 * (double)sm - one;
-                            atomicAdd(&acc[bq[u] - b0], to_fixed_signed(cn * ls));
-                            atomicAdd(&acc[CW + bq[u] - b0], to_fixed_signed(cd * ls));
-                        } else {
-                            const double m = vmin_nonneg(en[u].lu, vq[u]);
-                            const double ls = __longlong_as_double((long long)en[u].q8);
-                            if constexpr (FINE)
-                                atomicAdd(&acc[bq[u] - b0], to_fixed(m * ls));
-                            else
-                                atomicAdd(&acc[bq[u] - b0], fixed_term(m, ls));
-                        }
+                    for (uint32_t u = 0; u < SR_UNROLL; ++u) {
+                        const uint32_t qs = en[u].base + wb + 64 * (g + u);
+                        bq[u] = (uint32_t)ld_slot(pidx, qs);
+                        if constexpr (!GRAM) vq[u] = ld_slot(pval, qs);
                     }
+#pragma unroll
+                    for (uint32_t u = 0; u < SR_UNROLL; ++u)
+                        row_term<TV, GRAM, GEN, FINE, CW>(accb, en[u], bq[u], vq[u], alpha);
+                }
+            } else {
+                for (uint32_t j0 = lane; j0 < wn; j0 += 64 * SR_UNROLL) {
+                    Ent en[SR_UNROLL];
+                    uint32_t bq[SR_UNROLL];
+                    TV vq[SR_UNROLL];
+#pragma unroll
+                    for (uint32_t u = 0; u < SR_UNROLL; ++u) {
+                        const uint32_t j = min(j0 + 64 * u, wn - 1);
+                        en[u] = ents[own[j]];
+                        const uint32_t qs = en[u].base + w0 + j;
+                        bq[u] = (uint32_t)ld_slot(pidx, qs);
+                        if constexpr (!GRAM) vq[u] = ld_slot(pval, qs);
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < SR_UNROLL; ++u)
+                        if (j0 + 64 * u < wn) row_term<TV, GRAM, GEN, FINE, CW>(accb, en[u], bq[u], vq[u], alpha);
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -1971,6 +2006,10 @@ hipError_t launch_count_present(uint32_t* cnt, const double* E, size_t lde, uint
 }
 
 uint32_t sparse_rows_blocks(uint32_t n) { return n * ((n + SR_CW / 2 - 1) / (SR_CW / 2)); }   // (half-width chunks)
+uint32_t sparse_rows_launched(uint32_t n, bool gen) {
+    const uint32_t cw = gen ? SR_CW / 2 : SR_CW;
+    return n * ((n + cw - 1) / cw);
+}
 uint32_t row_list_chunks(uint32_t K) { return std::max<uint32_t>(1, (K + RL_KC - 1) / RL_KC); }
 size_t offset_scan_blocks(uint32_t K) { return ((size_t)K + SCAN_BLOCK - 1) / SCAN_BLOCK; }
 size_t row_list_scan_blocks(uint32_t n, uint32_t K) {

@@ -194,6 +194,8 @@ hipError_t launch_sparse_nodes(const SparseNodesArgs& a, hipStream_t st);
 hipError_t launch_count_present(uint32_t* cnt, const double* E, size_t lde, uint32_t col0, const int32_t* knodes,
                                 uint32_t K, uint32_t n, hipStream_t st);
 uint32_t sparse_rows_blocks(uint32_t n);
+// workgroups launch_sparse_rows launches (the entries of walked it writes)
+uint32_t sparse_rows_launched(uint32_t n, bool gen);
 size_t offset_scan_blocks(uint32_t K);
 uint32_t row_list_chunks(uint32_t K);
 size_t row_list_scan_blocks(uint32_t n, uint32_t K);
