@@ -398,6 +398,11 @@ SSU_EXTERN ComputeStatus ssu_plan_execute(ssu_plan_t* plan);
 /* element size of the result: 8 (fp64 method names) or 4 */
 SSU_EXTERN int ssu_plan_result_bytes(const ssu_plan_t* plan);
 SSU_EXTERN ComputeStatus ssu_plan_copy_matrix(const ssu_plan_t* plan, void* host_out);
+/* rows [row0, row0 + nrows) of the n x n result (nrows * n elements), for
+ * callers that stream a large matrix out in bands; output_error when the band
+ * does not lie inside the matrix */
+SSU_EXTERN ComputeStatus ssu_plan_copy_rows(const ssu_plan_t* plan, unsigned int row0, unsigned int nrows,
+                                            void* host_out);
 /* last execute(): milliseconds per phase (embed, prep, main kernel, combine, total, posting-list share of
  * the FP32 weighted path) from HIP events */
 SSU_EXTERN int ssu_plan_timings(const ssu_plan_t* plan, double* ms, int cap);

@@ -313,6 +313,31 @@ def test_weighted_split_across_row_chunks(ssu, O, synth, monkeypatch, prec):
     assert rel_err(out["0.2"][np.ix_(idx, idx)], ref) <= TOL[prec]
 
 
+@pytest.mark.parametrize("method,kw", [("unweighted", {}), ("weighted_normalized", {}),
+                                       ("weighted_normalized_fp64", {}), ("generalized", {"alpha": 0.5})])
+def test_wide_sample_index(ssu, O, synth, method, kw):
+    """65,600 samples: posting lists carry 32-bit sample indices (16-bit up to
+    65,536) and the row walk spans 17 column chunks.  The occupancy split must
+    be in use; rows of 48 samples on both sides of index 65,536, copied as row
+    bands, against the FP64 oracle on those samples."""
+    n = 65600
+    tree, table = _case(synth, 300, n, 0.02, 83)
+    inp = inputs_from(ssu, tree, table)
+    idx = np.unique(np.concatenate([np.random.default_rng(8).choice(n, 44, replace=False), [0, 65535, 65536, n - 1]]))
+    say("wide sample index", method, n)
+    p = ssu.Plan(inp, method, **kw)
+    p.execute()
+    assert p.sparse_terms() > 0
+    sub = np.stack([p.rows(int(i), 1)[0, idx] for i in idx])
+    band = p.rows(65530, 12)
+    p.close()
+    assert band.shape == (12, n)
+    assert np.array_equal(band[6, idx], sub[list(idx).index(65536)])   # the band's row 65,536
+    ref = oracle_matrix(O, tree, _sub_table(table, idx), method.replace("_fp64", ""), **kw)
+    assert rel_err(sub, ref) <= TOL["fp64" if method.endswith("fp64") else "fp32"]
+    assert np.array_equal(sub, sub.T) and np.all(np.diag(sub) == 0)
+
+
 def test_sparse_terms_count_launched_rows_only(ssu, synth, monkeypatch):
     """sparse_terms() sums the walked postings of the workgroups the row walk
     launched: a generalized plan (half-width column chunks, twice the

@@ -1426,6 +1426,12 @@ ComputeStatus ssu_plan_copy_matrix(const ssu_plan_t* plan, void* host_out) {
     return okay;
 }
 
+ComputeStatus ssu_plan_copy_rows(const ssu_plan_t* plan, unsigned int row0, unsigned int nrows, void* host_out) {
+    std::string err;
+    if (!plan || !plan->plan->copy_rows(row0, nrows, host_out, &err)) { set_error(err); return output_error; }
+    return okay;
+}
+
 int ssu_plan_timings(const ssu_plan_t* plan, double* ms, int cap) {
     if (!plan) return 0;
     const float* t = plan->plan->timings();

@@ -822,6 +822,19 @@ bool Plan::copy_matrix(void* host, std::string* err) const {
     return true;
 }
 
+bool Plan::copy_rows(uint32_t row0, uint32_t nrows, void* host, std::string* err) const {
+    const PlanImpl& P = *p_;
+    if ((uint64_t)row0 + nrows > P.n) {
+        if (err) *err = "ssu_plan_copy_rows: rows " + std::to_string(row0) + ".." + std::to_string((uint64_t)row0 + nrows) +
+                        " outside the " + std::to_string(P.n) + "-sample matrix";
+        return false;
+    }
+    const size_t es = P.spec.fp64 ? sizeof(double) : sizeof(float);
+    if (nrows) SSU_HIP(hipMemcpy(host, P.d_out.as<char>() + (size_t)row0 * P.n * es, (size_t)nrows * P.n * es,
+                                 hipMemcpyDeviceToHost));
+    return true;
+}
+
 bool Plan::copy_stripes(uint32_t start, uint32_t stop, double* const* host, std::string* err) const {
     PlanImpl& P = *p_;
     if (stop <= start) return true;

@@ -85,6 +85,7 @@ EXPORTS = [
     "ssu_center_matrix_fp64", "ssu_center_matrix_fp32", "ssu_pcoa_fp64", "ssu_pcoa_fp32", "ssu_permanova_fp64",
     "ssu_permanova_fp32",
     "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
+    "ssu_plan_copy_rows",
     "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_sparse_terms", "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
     "ssu_newick_to_bp", "ssu_biom_n_samples", "ssu_device_available", "ssu_last_error",
     "ssu_rarefy_create_inmem", "ssu_rarefy_execute", "ssu_rarefy_n_samples", "ssu_rarefy_n_obs", "ssu_rarefy_nnz",
@@ -273,6 +274,16 @@ class Plan:
         eb = L.ssu_plan_result_bytes(self.h)
         out = np.empty((self.n, self.n), dtype=np.float64 if eb == 8 else np.float32)
         st = L.ssu_plan_copy_matrix(self.h, out.ctypes.data_as(C.c_void_p))
+        if st != OKAY:
+            raise RuntimeError(last_error())
+        return out
+
+    def rows(self, row0, nrows):
+        """rows [row0, row0 + nrows) of the result"""
+        L = lib()
+        eb = L.ssu_plan_result_bytes(self.h)
+        out = np.empty((nrows, self.n), dtype=np.float64 if eb == 8 else np.float32)
+        st = L.ssu_plan_copy_rows(self.h, C.c_uint(row0), C.c_uint(nrows), out.ctypes.data_as(C.c_void_p))
         if st != OKAY:
             raise RuntimeError(last_error())
         return out
