@@ -25,9 +25,9 @@ inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b)
 
 // ------------------------------------------------------------------ embedding
 
-// One workgroup per leaf: clear the row, then scatter the leaf's CSR row
-// (src/biom_inmem.cpp:473-497: value / total, or the raw value).  E holds
-// the sample columns [col0, col0 + ld) of the embedding.
+// One workgroup per leaf: clear the row (16-byte stores; ld is even), then
+// scatter the leaf's CSR row (src/biom_inmem.cpp:473-497: value / total, or
+// the raw value).  E holds the sample columns [col0, col0 + ld) of the embedding.
 __global__ void __launch_bounds__(EMB_THREADS) k_embed_leaves(double* __restrict__ E, size_t ld, uint32_t col0,
                                                               const int32_t* __restrict__ leaf_k,
                                                               const int32_t* __restrict__ leaf_row,
@@ -37,7 +37,8 @@ __global__ void __launch_bounds__(EMB_THREADS) k_embed_leaves(double* __restrict
                                                               const double* __restrict__ totals, int normalize) {
     const uint32_t li = blockIdx.x;
     double* row = E + (size_t)leaf_k[li] * ld;
-    for (size_t s = threadIdx.x; s < ld; s += blockDim.x) row[s] = 0.0;
+    double2* row2 = reinterpret_cast<double2*>(row);
+    for (size_t s = threadIdx.x; s < ld / 2; s += blockDim.x) row2[s] = make_double2(0.0, 0.0);
     __syncthreads();
     const int32_t r = leaf_row[li];
     if (r < 0) return;
@@ -49,18 +50,25 @@ __global__ void __launch_bounds__(EMB_THREADS) k_embed_leaves(double* __restrict
 }
 
 // Internal nodes of one height level: E[k] = ((0 + E[c0]) + E[c1]) + ..., children left to right.
-// blockIdx.x covers samples; blockIdx.y strides over the level's nodes.
+// blockIdx.x covers samples, two per thread (16-byte accesses; ldn is even);
+// blockIdx.y strides over the level's nodes.
 __global__ void __launch_bounds__(EMB_THREADS) k_embed_level(double* __restrict__ E, size_t ldn,
                                                              const int32_t* __restrict__ nodes, uint32_t n_nodes,
                                                              const int32_t* __restrict__ child_ptr,
                                                              const int32_t* __restrict__ child_idx) {
-    const size_t s = (size_t)blockIdx.x * EMB_THREADS + threadIdx.x;
-    if (s >= ldn) return;
+    const size_t s2 = (size_t)blockIdx.x * EMB_THREADS + threadIdx.x;   // samples 2 s2, 2 s2 + 1
+    const size_t ld2 = ldn / 2;
+    if (s2 >= ld2) return;
+    double2* E2 = reinterpret_cast<double2*>(E);
     for (uint32_t ni = blockIdx.y; ni < n_nodes; ni += gridDim.y) {
         const int32_t k = nodes[ni];
-        double acc = 0.0;
-        for (int32_t c = child_ptr[k]; c < child_ptr[k + 1]; ++c) acc += E[(size_t)child_idx[c] * ldn + s];
-        E[(size_t)k * ldn + s] = acc;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int32_t c = child_ptr[k]; c < child_ptr[k + 1]; ++c) {
+            const double2 v = E2[(size_t)child_idx[c] * ld2 + s2];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        E2[(size_t)k * ld2 + s2] = acc;
     }
 }
 
@@ -1711,6 +1719,7 @@ hipError_t launch_embed_leaves(double* E, size_t ld, uint32_t col0, const int32_
                                uint32_t n_leaves, const uint32_t* indptr, const uint32_t* indices, const double* data,
                                const double* totals, bool normalize, hipStream_t st) {
     if (!n_leaves) return hipSuccess;
+    if (ld % 2) return hipErrorInvalidValue;   // rows of an even number of columns (16-byte stores)
     hipLaunchKernelGGL(k_embed_leaves, dim3(n_leaves), dim3(EMB_THREADS), 0, st, E, ld, col0, leaf_k, leaf_row, indptr,
                        indices, data, totals, (int)normalize);
     return hipGetLastError();
@@ -1719,7 +1728,8 @@ hipError_t launch_embed_leaves(double* E, size_t ld, uint32_t col0, const int32_
 hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint32_t n_nodes, const int32_t* child_ptr,
                               const int32_t* child_idx, hipStream_t st) {
     if (!n_nodes) return hipSuccess;
-    const uint32_t cb = grid1(ldn, EMB_THREADS);
+    if (ldn % 2) return hipErrorInvalidValue;   // two samples per thread
+    const uint32_t cb = grid1(ldn / 2, EMB_THREADS);
     const uint32_t gy = std::min<uint32_t>(n_nodes, 8192);
     hipLaunchKernelGGL(k_embed_level, dim3(cb, gy), dim3(EMB_THREADS), 0, st, E, ldn, nodes, n_nodes, child_ptr,
                        child_idx);
