@@ -4,13 +4,14 @@
 Workload (default): random binary tree with 100,000 tips (Exp(0.1) lengths,
 FP32-rounded), 16,384 samples at 1% tip density with Zipf(1.1) tip popularity,
 lognormal counts -- the "16k samples" configuration.  One step computes the
-full 16,384 x 16,384 unweighted matrix (exact int8-MFMA Gram) and the full
-weighted_normalized matrix (shared-node min formulation: nodes held by at
-least 20% of the samples through 32x64 pair tiles with FP32 terms merged into
-FP64, visiting per tile only the nodes both sample panels hold; rarer nodes
-through per-node posting lists walked row by row into exact fixed-point
-sums), both with the table and tree already resident in HBM and the matrices
-left in HBM.
+full 16,384 x 16,384 unweighted matrix (exact int8-MFMA Gram over the nodes
+held by at least 6% of the samples, the rarer ones from posting lists) and the full
+weighted_normalized matrix (nodes held by at least 25% of the samples through
+32x64 pair tiles summing l |u - v| in FP32, merged into FP64, visiting per
+tile only the nodes either sample panel holds; rarer nodes through per-node
+posting lists walked row by row into exact fixed-point sums of l min(u, v)),
+both with the table and tree already resident in HBM and the matrices left in
+HBM.
 
 metric: pair-node updates/sec = sum over both methods of P * N / t, with
 P = n(n-1)/2 and N the nodes the method processes after shear/collapse.
