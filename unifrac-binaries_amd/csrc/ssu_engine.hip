@@ -333,7 +333,7 @@ bool PlanImpl::split_gram(std::string* err) {
 // the kernels): theta is lowered to fit.  SSU_SPARSE_SPLIT sets theta (0: off).
 bool PlanImpl::split_weighted(std::string* err) {
     // generalized terms cost several times the weighted ones on the tiles, which moves the balance
-    double theta = gen ? 0.6 : 0.25;
+    double theta = gen ? 0.8 : 0.25;
     if (const char* env = getenv("SSU_SPARSE_SPLIT")) theta = atof(env);
     bool nonneg = true;
     for (size_t q = 0; q < knodes.size(); ++q) nonneg = nonneg && (spec.fp64 ? klen_d[q] >= 0.0 : klen_f[q] >= 0.0f);

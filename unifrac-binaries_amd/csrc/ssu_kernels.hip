@@ -1327,9 +1327,13 @@ __device__ __forceinline__ float pow_alpha(float x, float alpha) {
     return __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(x));
 }
 
-// signed fixed point (two's complement in the 64-bit accumulators), |x| < 2^62, rounded
-__device__ __forceinline__ unsigned long long to_fixed_signed(double x) {
-    return (unsigned long long)__double2ll_rn(x);
+// round(x * ls) as a signed integer (two's complement in the 64-bit
+// accumulators) for |x * ls| < 2^51: the FMA adds 1.5 * 2^52, so the sum lies
+// in [2^52, 2^53) where the FP64 spacing is 1 and its bits exceed those of
+// 1.5 * 2^52 by exactly the rounded product
+__device__ __forceinline__ unsigned long long fixed_term_signed(double x, double ls) {
+    const double y = fma(x, ls, 0x1.8p52);
+    return (unsigned long long)__double_as_longlong(y) - 0x4338000000000000ull;
 }
 
 // p[i] through a 32-bit byte offset from the (wave-uniform) base, so the load
@@ -1356,8 +1360,8 @@ __device__ __forceinline__ void row_term(unsigned long long* accb, const RowEntr
         const double ls = __longlong_as_double((long long)en.q8);
         const double cn = (double)p1 * (double)__builtin_fabsf(uu - vv) - one;
         const double cd = (double)p1 * (double)sm - one;
-        atomicAdd(&accb[b], to_fixed_signed(cn * ls));
-        atomicAdd(&accb[CW + b], to_fixed_signed(cd * ls));
+        atomicAdd(&accb[b], fixed_term_signed(cn, ls));
+        atomicAdd(&accb[CW + b], fixed_term_signed(cd, ls));
     } else {
         const double m = vmin_nonneg(en.lu, v);
         const double ls = __longlong_as_double((long long)en.q8);
