@@ -1291,21 +1291,27 @@ constexpr uint32_t SR_WAVES = 8;   // waves sharing one accumulator, each taking
 constexpr uint32_t SR_UNROLL = 4;  // postings per lane with their loads in flight together
 constexpr uint32_t SR_WIN = 1024;  // walk positions resolved to entries at a time (64 lanes x 16 bytes)
 
-// Sparse-node share for row a.  Weighted: sum over sparse nodes r held by
-// both a and b > a of l_r * min(u_ra, u_rb), written as FP32 into
-// out[a * n + b] for the tiled kernel's epilogue.  GRAM (unweighted): sum of
-// l_r over the sparse nodes a and b >= a share, added to the FP64 Gram entry
-// G(a, b) (the diagonal by the chunk that holds column a).  One workgroup of 8
-// waves per (row a, 4096-column chunk c), blockIdx = c * n + a.  Row a's list
+// Sparse-node share for row a.  Weighted: M = sum over sparse nodes r held by
+// both a and b > a of l_r * min(u_ra, u_rb); out[a * n + b] gets, for the
+// tiled kernel's epilogue, M itself (FP64 values) or, for FP32 values, the
+// sparse nodes' share of sum l |u_a - u_b|, ssp[a] + ssp[b] - 2 M, formed in
+// FP64 (ssp: each sample's sum of l u over its sparse nodes).  GEN
+// (generalized, FP32 values, 2048-column chunks): per shared sparse node the
+// signed corrections l (s^(alpha-1) |u - v| - u^alpha - v^alpha) and
+// l (s^alpha - u^alpha - v^alpha), s = u + v, summed into two accumulators;
+// out and out2 get ssp[a] + ssp[b] plus each (ssp: sums of l u^alpha).  GRAM
+// (unweighted): sum of l_r over the sparse nodes a and b >= a share, added to
+// the FP64 Gram entry G(a, b) (the diagonal by the chunk that holds column a).
+// One workgroup of 8 waves per (row a, column chunk c), blockIdx = c * n + a.  Row a's list
 // is taken 64 entries at a time (round q goes to wave q % 8); a round's
 // posting ranges inside the chunk (past a) are walked with the wave's 64 lanes
 // spread evenly over the concatenated postings, each position's entry read
 // from a per-wave byte map (entry starts marked, then a prefix max).  Terms
 // are accumulated in fixed point, integers in units of 2^(e - 51) (FP32
-// weighted values: the FP64 product l_r * min, rounded to nearest) or
-// 2^(e - 61) (FP64 weighted values: l_r * min, and GRAM: l_r, truncated) where
-// 2^e bounds the row's sum (S[a] weighted; the FP64 sum of the row list's
-// lengths unweighted).  Weighted partial sums go out in the value type.  Integer
+// values: the FP64 product rounded to nearest by one FMA) or 2^(e - 61) (FP64
+// weighted values: l_r * min, and GRAM: l_r, truncated) where 2^e bounds the
+// row's sum of term magnitudes (S[a] weighted; 3 (ssp[a] + max ssp) GEN; the
+// FP64 sum of the row list's lengths unweighted).  Weighted partial sums go out in the value type.  Integer
 // adds make the result independent of the order in which waves and lanes add,
 // and two samples holding the same nodes get bit-equal G(a, b) and G(a, a).
 // Posting slots are addressed with 32-bit offsets (fewer than 2^30 slots).
