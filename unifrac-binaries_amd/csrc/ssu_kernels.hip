@@ -1120,22 +1120,40 @@ __global__ void __launch_bounds__(64) k_node_split(uint32_t* __restrict__ dmask,
 // Posting lists of the sparse nodes: node r's samples in increasing order
 // (pidx) with their values (pval, when non-null: row urows[r] of U, or row r), in slots
 // [poff[r], poff[r + 1]); pc[p * K + r] = number of node r's samples below
-// 32 p, for p = 0..npan.  One thread per node.
+// 32 p, for p = 0..npan.  A workgroup holds 64 nodes (lane = node, so the
+// [panel][K] reads coalesce) in PST_WAVES waves, each wave a contiguous range
+// of panels: the waves count their ranges' samples, an exclusive sum over the
+// waves gives each range its first slot, then every wave writes its postings.
+// Slots and counts stop at the node's capacity (never reached otherwise: the
+// capacities are these same counts).
+constexpr uint32_t PST_WAVES = 8;
 template <class IX, class TV>
-__global__ void __launch_bounds__(64) k_postings(IX* __restrict__ pidx, TV* __restrict__ pval,
+__global__ void __launch_bounds__(64 * PST_WAVES) k_postings(IX* __restrict__ pidx, TV* __restrict__ pval,
                                                  uint32_t* __restrict__ pc, const uint32_t* __restrict__ smask,
                                                  const uint32_t* __restrict__ dmask, const TV* __restrict__ U,
                                                  const int32_t* __restrict__ urows, size_t ldu, uint32_t K,
                                                  uint32_t npan, const uint32_t* __restrict__ poff, uint32_t slots) {
-    const uint32_t r = blockIdx.x * 64 + threadIdx.x;
-    if (r >= K || ((dmask[r >> 5] >> (r & 31)) & 1u) || poff[K] > slots) return;   // (the last: never, by the bound)
+    const uint32_t lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const uint32_t r = blockIdx.x * 64 + lane;
+    __shared__ uint32_t cnt_s[PST_WAVES][64];
+    // (the last condition: never, by the bound)
+    const bool live = r < K && !((dmask[r >> 5] >> (r & 31)) & 1u) && poff[K] <= slots;
+    const uint32_t per = (npan + PST_WAVES - 1) / PST_WAVES;
+    const uint32_t p0 = min(npan, g * per), p1 = min(npan, p0 + per);
+    uint32_t c = 0;
+    if (live)
+        for (uint32_t p = p0; p < p1; ++p) c += __popc(smask[(size_t)p * K + r]);
+    cnt_s[g][lane] = c;
+    __syncthreads();
+    if (!live) return;
+    uint32_t cnt = 0;   // samples of node r below panel p0
+    for (uint32_t h = 0; h < g; ++h) cnt += cnt_s[h][lane];
     const TV* ur = U ? U + (size_t)(urows ? urows[r] : (int32_t)r) * ldu : nullptr;
     const uint32_t o = poff[r], cap = poff[r + 1] - o;
     IX* ix = pidx + o;
     TV* vx = pval ? pval + o : nullptr;
-    uint32_t cnt = 0;
-    for (uint32_t p = 0; p < npan; ++p) {
-        pc[(size_t)p * K + r] = cnt;
+    for (uint32_t p = p0; p < p1; ++p) {
+        pc[(size_t)p * K + r] = min(cnt, cap);
         uint32_t w = smask[(size_t)p * K + r];
         while (w && cnt < cap) {   // the slots hold every sample of the node; the bound guards memory
             const uint32_t s = p * 32 + (uint32_t)(__ffs(w) - 1);
@@ -1144,8 +1162,9 @@ __global__ void __launch_bounds__(64) k_postings(IX* __restrict__ pidx, TV* __re
             ++cnt;
             w &= w - 1;
         }
+        if (w) cnt += __popc(w);   // past the capacity: counted, not written
     }
-    pc[(size_t)npan * K + r] = cnt;
+    if (g == PST_WAVES - 1) pc[(size_t)npan * K + r] = min(cnt, cap);
 }
 
 // Fixed point for the LDS accumulators: 0 <= x < 2^62, truncated.  Integer
@@ -1946,7 +1965,7 @@ hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
     const void* U = a.gram ? nullptr : a.U;
     void* pval = a.gram ? nullptr : a.pval;
 #define SSU_POSTINGS(IX, TV)                                                                                   \
-    hipLaunchKernelGGL((k_postings<IX, TV>), dim3(gk), dim3(64), 0, st, (IX*)a.pidx, (TV*)pval, a.pc, a.smask, \
+    hipLaunchKernelGGL((k_postings<IX, TV>), dim3(gk), dim3(64 * PST_WAVES), 0, st, (IX*)a.pidx, (TV*)pval, a.pc, a.smask, \
                        a.dmask, (const TV*)U, a.urows, a.ldu, a.K, a.npan, (const uint32_t*)a.poff, a.slots)
     switch (key) {
     case 0: SSU_POSTINGS(uint32_t, float); break;
