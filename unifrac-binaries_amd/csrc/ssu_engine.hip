@@ -91,6 +91,9 @@ struct PlanImpl {
     bool gen = false;                // generalized on the shared-node path
     uint32_t split_thr = 0;          // FP32 shared-node path: nodes held by fewer samples use posting lists (0: off)
     uint32_t split_slots = 0;        // bound on their postings in any execution
+    // bit q: processed node q is held by fewer than split_thr samples of the plan's
+    // table, so it is never dense in any execution (its FP32 operand row is not needed)
+    std::vector<uint32_t> urare;
     // Gram path occupancy split: sparse nodes (held by few samples of the plan's
     // table) are summed from posting lists, the Gram keeps the others
     uint32_t gsplit_slots = 0;       // posting slots of all sparse nodes (0: off); d_poff holds their offsets
@@ -128,7 +131,7 @@ struct PlanImpl {
     DevMem d_indptr, d_indices, d_data, d_totals, d_leaf_k, d_leaf_row, d_level_nodes, d_child_ptr, d_child_idx;
     DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_order, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
         d_faith, d_stripes, d_smask, d_dmask, d_pc, d_pidx, d_pval, d_walked, d_rows, d_rcount, d_roff, d_rbsum,
-        d_sp_node, d_sp_len, d_umasks, d_B, d_poff, d_pcnt, d_obsum, d_Ssp, d_out2, d_pspmax;
+        d_sp_node, d_sp_len, d_umasks, d_B, d_poff, d_pcnt, d_obsum, d_Ssp, d_out2, d_pspmax, d_urare;
 
     hipStream_t own_st = nullptr;    // created with the plan; st may point at a caller's stream
     // per-execution inputs and output (nullptr: the plan's own table data / totals / result)
@@ -352,7 +355,10 @@ bool PlanImpl::split_weighted(std::string* err) {
     if (slots >= (1u << 30)) return true;
     split_thr = thr;
     split_slots = (uint32_t)std::max<uint64_t>(slots, 1);
-    return true;
+    urare.assign((K + 31) / 32, 0u);
+    for (uint32_t q = 0; q < K; ++q)
+        if (cnt[q] < thr) urare[q >> 5] |= 1u << (q & 31);
+    return upload(d_urare, urare, st, err);   // (a member: the host copy outlives the upload)
 }
 
 bool PlanImpl::embed_bits(DevMem& dst, uint32_t lb, bool structural, std::string* err) {
@@ -634,6 +640,9 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             SSU_HIP(launch_bits_smask(P.d_smask.as<uint32_t>(), P.d_B.as<uint32_t>(), P.ldb,
                                       P.d_sp_node.as<int32_t>(), Ks, npan, P.st));
     }
+    // one embedding block (E holds every sample): the postings of the FP32 split take their
+    // values from E, and the gather skips the operand rows of nodes that are never dense
+    const bool skip_rare = split && gather && single && !s.vaw && P.d_urare.p;
     for (size_t c0 = 0; c0 < P.ldn && !P.bits; c0 += P.blk) {
         const size_t w = std::min(P.blk, P.ldn - c0);
         if (!P.embed(P.d_E, s.normalize, c0, w, err)) return false;
@@ -651,7 +660,8 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             SSU_HIP(launch_gather_sums(gather ? P.d_U.as<float>() : nullptr, P.ldn, sums ? P.d_S.as<double>() : nullptr,
                                        P.d_Spart.as<double>(), P.sparse ? P.d_masks.as<uint32_t>() : nullptr,
                                        split ? P.d_smask.as<uint32_t>() : nullptr, P.d_E.as<double>(), w, (uint32_t)c0, P.d_knodes.as<int32_t>(),
-                                       P.d_klen_sum.as<double>(), K, P.n, P.st));
+                                       P.d_klen_sum.as<double>(), K, P.n, P.st,
+                                       skip_rare ? P.d_urare.as<uint32_t>() : nullptr));
             if (gather && s.vaw)
                 SSU_HIP(launch_gather_sums(P.d_Cu.as<float>(), P.ldn, nullptr, nullptr, nullptr, nullptr, P.d_C.as<double>(), w,
                                            (uint32_t)c0, P.d_knodes.as<int32_t>(), nullptr, K, P.n, P.st));
@@ -666,9 +676,10 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         b.pc = P.d_pc.as<uint32_t>();
         b.pidx = P.d_pidx.p;
         b.pval = P.d_pval.p;
-        b.U = s.fp64 ? P.d_E.p : P.d_U.p;                       // FP64: the embedding rows themselves
-        b.urows = s.fp64 ? P.d_knodes.as<int32_t>() : nullptr;
+        b.U = s.fp64 || skip_rare ? P.d_E.p : P.d_U.p;          // the embedding rows themselves
+        b.urows = s.fp64 || skip_rare ? P.d_knodes.as<int32_t>() : nullptr;
         b.values_fp64 = s.fp64;
+        b.src_fp64 = s.fp64 || skip_rare;
         b.ldu = P.ldn;
         b.len = P.d_klen_sum.as<double>();
         b.S = P.d_S.as<double>();

@@ -89,7 +89,8 @@ __global__ void __launch_bounds__(EMB_THREADS) k_gather_sums(float* __restrict__
                                                              const double* __restrict__ E, size_t lde,
                                                              const int32_t* __restrict__ knodes,
                                                              const double* __restrict__ klen, uint32_t K,
-                                                             uint32_t rows_per_seg, uint32_t n_valid) {
+                                                             uint32_t rows_per_seg, uint32_t n_valid,
+                                                             const uint32_t* __restrict__ skip_u) {
     const size_t s = (size_t)blockIdx.x * EMB_THREADS + threadIdx.x;
     if (s >= lde) return;
     const bool real = s < n_valid;   // columns past the last sample never enter a mask
@@ -103,7 +104,7 @@ __global__ void __launch_bounds__(EMB_THREADS) k_gather_sums(float* __restrict__
         double e = E[(size_t)knodes[r] * lde + s];
         if constexpr (GATHER) {
             const float f = (float)e;
-            U[(size_t)r * ldu + col0 + s] = f;
+            if (!skip_u || !((skip_u[r >> 5] >> (r & 31)) & 1u)) U[(size_t)r * ldu + col0 + s] = f;
             e = (double)f;
         }
         if constexpr (SUMS) acc += klen[r] * e;
@@ -1125,12 +1126,13 @@ __global__ void __launch_bounds__(64) k_node_split(uint32_t* __restrict__ dmask,
 // of panels: the waves count their ranges' samples, an exclusive sum over the
 // waves gives each range its first slot, then every wave writes its postings.
 // Slots and counts stop at the node's capacity (never reached otherwise: the
-// capacities are these same counts).
+// capacities are these same counts).  Values of type TU are stored as TV
+// (FP64 embedding rows rounded to FP32 the way the gathered operand is).
 constexpr uint32_t PST_WAVES = 8;
-template <class IX, class TV>
+template <class IX, class TV, class TU>
 __global__ void __launch_bounds__(64 * PST_WAVES) k_postings(IX* __restrict__ pidx, TV* __restrict__ pval,
                                                  uint32_t* __restrict__ pc, const uint32_t* __restrict__ smask,
-                                                 const uint32_t* __restrict__ dmask, const TV* __restrict__ U,
+                                                 const uint32_t* __restrict__ dmask, const TU* __restrict__ U,
                                                  const int32_t* __restrict__ urows, size_t ldu, uint32_t K,
                                                  uint32_t npan, const uint32_t* __restrict__ poff, uint32_t slots) {
     const uint32_t lane = threadIdx.x & 63, g = threadIdx.x >> 6;
@@ -1148,7 +1150,7 @@ __global__ void __launch_bounds__(64 * PST_WAVES) k_postings(IX* __restrict__ pi
     if (!live) return;
     uint32_t cnt = 0;   // samples of node r below panel p0
     for (uint32_t h = 0; h < g; ++h) cnt += cnt_s[h][lane];
-    const TV* ur = U ? U + (size_t)(urows ? urows[r] : (int32_t)r) * ldu : nullptr;
+    const TU* ur = U ? U + (size_t)(urows ? urows[r] : (int32_t)r) * ldu : nullptr;
     const uint32_t o = poff[r], cap = poff[r + 1] - o;
     IX* ix = pidx + o;
     TV* vx = pval ? pval + o : nullptr;
@@ -1158,7 +1160,7 @@ __global__ void __launch_bounds__(64 * PST_WAVES) k_postings(IX* __restrict__ pi
         while (w && cnt < cap) {   // the slots hold every sample of the node; the bound guards memory
             const uint32_t s = p * 32 + (uint32_t)(__ffs(w) - 1);
             ix[cnt] = (IX)s;
-            if (vx) vx[cnt] = ur[s];
+            if (vx) vx[cnt] = (TV)ur[s];
             ++cnt;
             w &= w - 1;
         }
@@ -1820,14 +1822,15 @@ uint32_t gather_segments(uint32_t K, size_t ldn) {
 template <bool G, bool S, bool M>
 static void gather_sums_t(dim3 grid, hipStream_t st, float* U, size_t ldu, uint32_t col0, double* Spart,
                           uint32_t* masks, uint32_t Kw, uint32_t* smask, const double* E, size_t lde,
-                          const int32_t* knodes, const double* klen, uint32_t K, uint32_t rps, uint32_t n_valid) {
+                          const int32_t* knodes, const double* klen, uint32_t K, uint32_t rps, uint32_t n_valid,
+                          const uint32_t* skip_u) {
     hipLaunchKernelGGL((k_gather_sums<G, S, M>), grid, dim3(EMB_THREADS), 0, st, U, ldu, col0, Spart, masks, Kw,
-                       smask, E, lde, knodes, klen, K, rps, n_valid);
+                       smask, E, lde, knodes, klen, K, rps, n_valid, skip_u);
 }
 
 hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, uint32_t* masks, uint32_t* smask,
                               const double* E, size_t lde, uint32_t col0, const int32_t* knodes, const double* klen,
-                              uint32_t K, uint32_t n, hipStream_t st) {
+                              uint32_t K, uint32_t n, hipStream_t st, const uint32_t* skip_u) {
     if (masks && (lde % 64 || col0 % 64)) return hipErrorInvalidValue;
     if (smask && !masks) return hipErrorInvalidValue;
     if (col0 >= n) return hipSuccess;
@@ -1846,12 +1849,12 @@ hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, ui
     const dim3 grid(cb, segs);
     const int key = (U ? 4 : 0) | (S ? 2 : 0) | (masks ? 1 : 0);
     switch (key) {
-    case 7: gather_sums_t<true, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
-    case 6: gather_sums_t<true, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
-    case 4: gather_sums_t<true, false, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
-    case 3: gather_sums_t<false, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
-    case 2: gather_sums_t<false, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
-    case 1: gather_sums_t<false, false, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt); break;
+    case 7: gather_sums_t<true, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt, skip_u); break;
+    case 6: gather_sums_t<true, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt, skip_u); break;
+    case 4: gather_sums_t<true, false, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt, skip_u); break;
+    case 3: gather_sums_t<false, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt, skip_u); break;
+    case 2: gather_sums_t<false, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt, skip_u); break;
+    case 1: gather_sums_t<false, false, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt, skip_u); break;
     case 0: return hipSuccess;
     default: return hipErrorInvalidValue;
     }
@@ -1961,17 +1964,20 @@ hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
         hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_T), 0, st, a.poff, a.pcnt, a.obsum, (size_t)a.K, nb);
     }
     const bool narrow = a.n <= 65536;   // sample indices fit 16 bits: 6 bytes per posting instead of 8
-    const int key = (narrow ? 1 : 0) | (a.values_fp64 ? 2 : 0);
+    if (a.values_fp64 && !a.src_fp64) return hipErrorInvalidValue;   // FP64 values come from FP64 rows
+    const int key = (narrow ? 1 : 0) | (a.values_fp64 ? 2 : 0) | (a.src_fp64 ? 4 : 0);
     const void* U = a.gram ? nullptr : a.U;
     void* pval = a.gram ? nullptr : a.pval;
-#define SSU_POSTINGS(IX, TV)                                                                                   \
-    hipLaunchKernelGGL((k_postings<IX, TV>), dim3(gk), dim3(64 * PST_WAVES), 0, st, (IX*)a.pidx, (TV*)pval, a.pc, a.smask, \
-                       a.dmask, (const TV*)U, a.urows, a.ldu, a.K, a.npan, (const uint32_t*)a.poff, a.slots)
+#define SSU_POSTINGS(IX, TV, TU)                                                                               \
+    hipLaunchKernelGGL((k_postings<IX, TV, TU>), dim3(gk), dim3(64 * PST_WAVES), 0, st, (IX*)a.pidx, (TV*)pval, a.pc, \
+                       a.smask, a.dmask, (const TU*)U, a.urows, a.ldu, a.K, a.npan, (const uint32_t*)a.poff, a.slots)
     switch (key) {
-    case 0: SSU_POSTINGS(uint32_t, float); break;
-    case 1: SSU_POSTINGS(uint16_t, float); break;
-    case 2: SSU_POSTINGS(uint32_t, double); break;
-    default: SSU_POSTINGS(uint16_t, double); break;
+    case 0: SSU_POSTINGS(uint32_t, float, float); break;
+    case 1: SSU_POSTINGS(uint16_t, float, float); break;
+    case 4: SSU_POSTINGS(uint32_t, float, double); break;
+    case 5: SSU_POSTINGS(uint16_t, float, double); break;
+    case 6: SSU_POSTINGS(uint32_t, double, double); break;
+    default: SSU_POSTINGS(uint16_t, double, double); break;
     }
 #undef SSU_POSTINGS
     // row lists: counts per (sample, node chunk), their exclusive scan, then the entries

@@ -60,12 +60,14 @@ hipError_t launch_bits_smask(uint32_t* smask, const uint32_t* B, uint32_t ldb, c
 //     segments in order, into S[col0 ..]; Spart holds gather_segments(K, lde) * lde doubles;
 //   masks (if non-null; lde, col0 multiples of 64): [ldn/32][ceil(K/32)] words, bit r%32 of
 //     word r/32 of panel p set iff some sample of p*32..p*32+31 has u_rs > 0;
-//   smask (if non-null, with masks): [ldn/32][K] words, bit s%32 of smask[s/32][r] = u_rs > 0.
+//   smask (if non-null, with masks): [ldn/32][K] words, bit s%32 of smask[s/32][r] = u_rs > 0;
+//   skip_u (optional, [ceil(K/32)] words): rows r with bit r set are not written to U (S and
+//     the masks still see their FP32-rounded values).
 uint32_t gather_segments(uint32_t K, size_t ldn);
 hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, uint32_t* masks, uint32_t* smask,
                               const double* E,
                               size_t lde, uint32_t col0, const int32_t* knodes, const double* klen, uint32_t K,
-                              uint32_t n, hipStream_t st);
+                              uint32_t n, hipStream_t st, const uint32_t* skip_u = nullptr);
 // Faith PD: out[s] = sum over all non-root nodes k in postorder of (E[k][s] > 0) * len[k]
 hipError_t launch_faith(double* out, const double* E, size_t ldn, const double* len, uint32_t m, uint32_t n,
                         hipStream_t st);
@@ -164,6 +166,7 @@ struct SparseNodesArgs {
     const void* U;          // (weighted) value rows: row urows[r] (or r) of U, stride ldu
     const int32_t* urows;   // nullptr: identity
     bool values_fp64;       // (weighted) FP64 values, FP64 output
+    bool src_fp64 = false;  // (weighted) U holds FP64 rows (FP32 values: rounded when copied)
     size_t ldu;
     const double* len;      // [K]
     const double* S;        // [n] (weighted)
