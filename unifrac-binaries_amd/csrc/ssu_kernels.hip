@@ -38,7 +38,10 @@ __global__ void __launch_bounds__(EMB_THREADS) k_embed_leaves(double* __restrict
     const uint32_t li = blockIdx.x;
     double* row = E + (size_t)leaf_k[li] * ld;
     double2* row2 = reinterpret_cast<double2*>(row);
-    for (size_t s = threadIdx.x; s < ld / 2; s += blockDim.x) row2[s] = make_double2(0.0, 0.0);
+    for (size_t s = threadIdx.x; s < ld / 2; s += blockDim.x) {   // streaming stores: read back from HBM later
+        __builtin_nontemporal_store(0.0, &row2[s].x);
+        __builtin_nontemporal_store(0.0, &row2[s].y);
+    }
     __syncthreads();
     const int32_t r = leaf_row[li];
     if (r < 0) return;
