@@ -136,7 +136,8 @@ __global__ void __launch_bounds__(EMB_THREADS) k_sum_segments(double* __restrict
     const uint32_t s = blockIdx.x * EMB_THREADS + threadIdx.x;
     if (s >= n) return;
     double acc = 0.0;
-    for (uint32_t g = 0; g < segs; ++g) acc += Spart[(size_t)g * ldn + s];
+#pragma unroll 16
+    for (uint32_t g = 0; g < segs; ++g) acc += Spart[(size_t)g * ldn + s];   // (loads ahead, adds in order)
     S[s] = acc;
 }
 
