@@ -1203,7 +1203,7 @@ __device__ __forceinline__ double vmin_nonneg(double a, double b) { return a < b
 // wave writes its per-sample counts to counts[a * nkc + chunk]; otherwise it
 // writes its entries from offsets[a * nkc + chunk] (the exclusive scan of the
 // counts in that order).
-constexpr uint32_t RL_KC = 1024;
+constexpr uint32_t RL_KC = 256;
 __global__ void __launch_bounds__(64) k_row_lists(unsigned long long* __restrict__ rows, uint32_t* __restrict__ counts,
                                                   const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ smask,
