@@ -9,17 +9,25 @@ held by at least 6% of the samples, the rarer ones from posting lists) and the f
 weighted_normalized matrix (nodes held by at least 25% of the samples through
 32x64 pair tiles summing l |u - v| in FP32, merged into FP64, visiting per
 tile only the nodes either sample panel holds; rarer nodes through per-node
-posting lists walked row by row into exact fixed-point sums of l min(u, v)),
-both with the table and tree already resident in HBM and the matrices left in
-HBM.
+posting lists walked row by row into exact fixed-point sums of l min(u, v)).
+
+Timed region of `value` (host to host): each step calls
+one_off_matrix_inmem_fp32_v3 once per method, from the CSR table and the tree
+arrays in host memory to the n x n matrix in host memory (the result buffer
+is passed back in, as src/api.cpp:674-677 allows, so its pages are mapped);
+it includes tree shear/collapse, uploads, plan building, the device work,
+near-duplicate refinement and the copy-back.  `kernel_only` repeats the step
+on device-resident plans (inputs in HBM, matrices left in HBM) with HIP-event
+timings per phase.
 
 metric: pair-node updates/sec = sum over both methods of P * N / t, with
 P = n(n-1)/2 and N the nodes the method processes after shear/collapse.
 
-Prints one JSON line (rank 0, N=1).  Extra fields: per-kernel HIP-event
-timings, the max relative error of both matrices against the FP64 oracle on a
-48-sample subset, the plain dense pair-tile kernels (unweighted and weighted)
-for comparison, a roofline entry for the dominant kernel and the CPU baseline.
+Prints one JSON line (rank 0, N=1).  Extra fields: the per-call phase
+breakdown, per-kernel HIP-event timings, the max relative error of both
+matrices against the FP64 oracle on a 48-sample subset, the plain dense
+pair-tile kernels (unweighted and weighted) for comparison, a roofline entry
+for the dominant kernel and the CPU baseline.
 """
 import argparse
 import json
@@ -35,7 +43,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 vector peak (packed FP32)
 PEAK_FP64_TFLOPS = 78.6     # MI355X published FP64 vector peak
-FLOPS_PER_TERM_SPARSE = 3   # min, fused multiply-add
+FLOPS_PER_TERM_SPARSE = 4   # sub, abs, fused multiply-add (the pair-node count of the stripe-kernel model)
 PEAK_INT8_TOPS = 5000.0     # MI355X published dense INT8 MFMA peak, TOP/s
 PEAK_HBM_GBS = 8000.0       # MI355X published HBM3E bandwidth, GB/s
 GRAM_TILE = 128             # Gram output tile edge
@@ -134,6 +142,37 @@ def cpu_baseline(tree, table, methods, budget_s):
         total_time += full
         log(f"cpu baseline {label}: {slope:.3f} s/stripe, setup {intercept:.2f} s, full {full:.0f} s")
     return kind, threads, total_time, per
+
+
+class OneOff:
+    """one_off_matrix_inmem_fp32_v3 / _v3 with the result buffer handed back in
+    on every later call (src/api.cpp:674-677 accepts a caller-preallocated
+    result), so the timed calls write into mapped pages."""
+
+    def __init__(self, ssu, inp, method, kw):
+        import ctypes as C
+        self.C, self.ssu, self.inp, self.method, self.kw = C, ssu, inp, method, kw
+        self.fp64 = method.endswith("_fp64")
+        self.res = C.POINTER(ssu.MatFullFP64 if self.fp64 else ssu.MatFullFP32)()
+
+    def __call__(self):
+        C, L, kw = self.C, self.ssu.lib(), self.kw
+        fn = L.one_off_matrix_inmem_v3 if self.fp64 else L.one_off_matrix_inmem_fp32_v3
+        st = fn(C.byref(self.inp.table), C.byref(self.inp.tree), self.method.encode(),
+                C.c_bool(kw.get("variance_adjust", False)), C.c_double(kw.get("alpha", 1.0)), C.c_bool(False),
+                C.c_bool(True), C.c_uint(1), C.c_uint(0), C.c_bool(True), None, C.byref(self.res))
+        if st != self.ssu.OKAY:
+            raise RuntimeError(f"{self.method}: status {st}: {self.ssu.last_error()}")
+        return self.ssu.last_call_timings()
+
+    def matrix(self):
+        n = self.res.contents.n_samples
+        return np.ctypeslib.as_array(self.res.contents.matrix, shape=(n * n,)).reshape(n, n)
+
+    def close(self):
+        if self.res:
+            L = self.ssu.lib()
+            (L.destroy_mat_full_fp64 if self.fp64 else L.destroy_mat_full_fp32)(self.C.byref(self.res))
 
 
 def copy_probe(nbytes):
@@ -296,57 +335,69 @@ def main():
     otree = O.Tree(tree[0].astype(np.uint8), tree[1], tree[2])
     errors = {}
 
-    def check(name, p):
-        """max relative error of the plan's matrix on a 48-sample subset, against the FP64 oracle"""
-        m = p.matrix()
-        ref = O.matrix(otree, O.Table(*sub), specs[name][0].replace("_fp64", ""), **specs[name][1])
-        errors[name] = max_rel_err(m[np.ix_(idx, idx)], ref)
-
     times = {k: [] for k in specs}
     plans_info = {}
+    k_steps = max(1, min(args.steps, 10))
+    # kernel-only: device-resident plans, inputs in HBM, matrices left in HBM
     if not args.sequential:
         # every matrix of a step computed back to back, all plans resident in HBM
         plans = {label: ssu.Plan(inp, m, **kw) for label, (m, kw) in specs.items()}
         for name, p in plans.items():
             log(f"plan {name}: main kernel {p.main_kernel()}, pair-nodes {p.pair_nodes():.4g}")
-        for _ in range(args.warmup):
+        for _ in range(max(args.warmup, 1)):
             for p in plans.values():
                 p.execute()
-        log("warmup done")
         t_start = time.perf_counter()
-        for step in range(args.steps):
+        for step in range(k_steps):
             for name, p in plans.items():
                 p.execute()
                 times[name].append(p.timings())
-            log(f"step {step + 1}/{args.steps} done")
-        elapsed = time.perf_counter() - t_start
+        k_elapsed = time.perf_counter() - t_start
         for name, p in plans.items():
             plans_info[name] = (p.main_kernel(), p.pair_nodes(), p.kernel_terms(), p.sparse_terms())
-            check(name, p)
-        for p in plans.values():
             p.close()
     else:
-        # one plan resident at a time (workloads whose plans do not fit in HBM together);
-        # a step is the sum of the per-plan step times
-        elapsed = 0.0
+        # one plan resident at a time (workloads whose plans do not fit in HBM together)
+        k_elapsed = 0.0
         for name, (m, kw) in specs.items():
             p = ssu.Plan(inp, m, **kw)
             log(f"plan {name}: main kernel {p.main_kernel()}, pair-nodes {p.pair_nodes():.4g}")
-            for _ in range(args.warmup):
+            for _ in range(max(args.warmup, 1)):
                 p.execute()
             t_start = time.perf_counter()
-            for step in range(args.steps):
+            for step in range(k_steps):
                 p.execute()
                 times[name].append(p.timings())
-            elapsed += time.perf_counter() - t_start
-            log(f"plan {name}: {args.steps} steps done")
+            k_elapsed += time.perf_counter() - t_start
             plans_info[name] = (p.main_kernel(), p.pair_nodes(), p.kernel_terms(), p.sparse_terms())
-            check(name, p)
             p.close()
+    log(f"kernel-only: {1000.0 * k_elapsed / k_steps:.1f} ms per step")
+
+    # host to host: one C-API call per method and step, CSR and tree in host memory -> matrix in host memory
+    calls = {label: OneOff(ssu, inp, m, kw) for label, (m, kw) in specs.items()}
+    phases = {k: [] for k in specs}
+    for _ in range(args.warmup):
+        for c in calls.values():
+            c()
+    log("warmup done")
+    t_start = time.perf_counter()
+    for step in range(args.steps):
+        for name, c in calls.items():
+            phases[name].append(c())
+        log(f"step {step + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t_start
+    for name, c in calls.items():
+        m = c.matrix()
+        ref = O.matrix(otree, O.Table(*sub), specs[name][0].replace("_fp64", ""), **specs[name][1])
+        errors[name] = max_rel_err(m[np.ix_(idx, idx)], ref)
+        c.close()
     log(f"max rel err vs FP64 oracle: {errors}")
     ms_per_step = 1000.0 * elapsed / args.steps
     pair_nodes = {k: v[1] for k, v in plans_info.items()}
     value = sum(pair_nodes.values()) / (elapsed / args.steps)
+    call_phases = {name: {k: float(np.mean([ph[k] for ph in v])) for k in v[0]} for name, v in phases.items()}
+    kernel_only = {"value": sum(pair_nodes.values()) / (k_elapsed / k_steps),
+                   "ms_per_step": 1000.0 * k_elapsed / k_steps, "steps": k_steps}
 
     def avg(name, key):
         return float(np.mean([t[key] for t in times[name]]))
@@ -375,7 +426,7 @@ def main():
         roofline = {"kernel": dom_kernel, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None}
     elif dom_kernel in ("k_pairs_sparse", "k_pairs_sparse32"):
-        # executed work: the (pair, shared node) terms the kernel evaluated, 3 flops each
+        # executed work: the (pair, shared node) terms the kernel evaluated, 4 flops each (sub, abs, FMA)
         peak = PEAK_FP64_TFLOPS if dom_kernel == "k_pairs_sparse" else PEAK_FP32_TFLOPS
         achieved = FLOPS_PER_TERM_SPARSE * dk["kernel_terms"] / (dom_ms / 1000.0) / 1e12
         roofline = {"kernel": dom_kernel, "bound": "valu", "achieved": achieved, "peak": peak,
@@ -394,6 +445,8 @@ def main():
         tr = json.load(open(prof)).get(f"{args.workload}:{dom_kernel}")
         if tr and not args.samples and not args.tips:
             roofline["traffic"] = tr
+            roofline["traffic_source"] = ("profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                                          "of this kernel on this workload, scaled by a 1 GiB copy's counts")
 
     plain = None
     if not args.no_plain:
@@ -437,6 +490,8 @@ def main():
                    "samples": n, "tree_nodes": len(tree[0]) // 2, "nnz": int(table[4].size),
                    "methods": list(specs)},
         "max_rel_err_vs_fp64": errors,
+        "call_phases_ms": call_phases,
+        "kernel_only": kernel_only,
         "kernels": kernels,
         "plain_dense_kernels": plain,
         "roofline": roofline,

@@ -395,6 +395,14 @@ SSU_EXTERN ComputeStatus ssu_plan_create_inmem(const support_biom_t* table_data,
                                                bool bypass_tips, bool normalize_sample_counts,
                                                int plain_kernel, ssu_plan_t** plan);
 SSU_EXTERN ComputeStatus ssu_plan_execute(ssu_plan_t* plan);
+/* execute computing only the pairs of stripes [start, stop) (start = stop = 0:
+ * every pair, as ssu_plan_execute); entries of other pairs are unspecified.
+ * Later ssu_plan_execute calls compute the same selection. */
+SSU_EXTERN ComputeStatus ssu_plan_execute_stripes(ssu_plan_t* plan, unsigned int start, unsigned int stop);
+/* stripes [start, stop) of the result as (stop - start) rows of n doubles:
+ * row s - start, element k = d(k, (k + s + 1) mod n) */
+SSU_EXTERN ComputeStatus ssu_plan_copy_stripes(const ssu_plan_t* plan, unsigned int start, unsigned int stop,
+                                               double* host_out);
 /* element size of the result: 8 (fp64 method names) or 4 */
 SSU_EXTERN int ssu_plan_result_bytes(const ssu_plan_t* plan);
 SSU_EXTERN ComputeStatus ssu_plan_copy_matrix(const ssu_plan_t* plan, void* host_out);
@@ -413,6 +421,15 @@ SSU_EXTERN uint64_t ssu_plan_kernel_terms(const ssu_plan_t* plan);
 /* (pair, node) terms of the last execute() summed from posting lists (FP32
  * weighted path: nodes held by few samples), not counted in kernel_terms */
 SSU_EXTERN uint64_t ssu_plan_sparse_terms(const ssu_plan_t* plan);
+/* pairs the last execute() recomputed in FP64 because their FP32 distance was
+ * small enough for the FP32 rounding of the proportions to reach 1e-6 relative
+ * (FP32 weighted and generalized methods; 0 otherwise) */
+SSU_EXTERN uint64_t ssu_plan_refined_pairs(const ssu_plan_t* plan);
+/* wall-clock milliseconds of the calling thread's last one_off_matrix_inmem*
+ * call: inputs, tree (shear / collapse / postorder), plan (uploads, schedules,
+ * occupancy splits), execute (device work and refinement), copy (result to the
+ * host), total.  Returns the number of values written (at most cap, 6). */
+SSU_EXTERN int ssu_last_call_timings(double* ms, int cap);
 SSU_EXTERN const char* ssu_plan_main_kernel(const ssu_plan_t* plan);
 SSU_EXTERN void ssu_plan_destroy(ssu_plan_t** plan);
 
@@ -467,6 +484,9 @@ SSU_EXTERN ComputeStatus ssu_rarefy_copy_matrix(const ssu_rarefy_t* batch, int m
 SSU_EXTERN double ssu_rarefy_last_ms(const ssu_rarefy_t* batch);     /* device time of the last execute */
 SSU_EXTERN int ssu_rarefy_graph_used(const ssu_rarefy_t* batch);     /* 1: the last execute replayed the graph */
 SSU_EXTERN uint64_t ssu_rarefy_pair_nodes(const ssu_rarefy_t* batch);
+/* pairs of the last execute recomputed in FP64 (near-duplicate refinement of
+ * the FP32 weighted and generalized methods), over every replicate */
+SSU_EXTERN uint64_t ssu_rarefy_refined_pairs(const ssu_rarefy_t* batch);
 SSU_EXTERN void ssu_rarefy_destroy(ssu_rarefy_t** batch);
 
 /* Host-side tree handling only (no device): sheared + collapsed postorder

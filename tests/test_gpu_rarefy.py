@@ -315,3 +315,49 @@ def test_generalized_fp32_replicates_and_replays(ssu, O, synth):
     assert r.graph_used()
     for rep in range(3):
         assert np.array_equal(r.matrix(0, rep), eager[rep]), rep
+
+
+def test_near_duplicate_refinement_in_replicates(ssu, O, synth):
+    """Near-duplicate samples survive the draws (samples 0..39 hold exactly
+    `depth` items and are taken whole without replacement; 40..79 repeat them
+    with one extra item, of which the draw leaves out one): the FP32 weighted
+    and generalized matrices of every replicate are refined to 1e-6 of the
+    FP64 oracle, in the eager pass and in graph replays alike."""
+    tips, n_base, depth = 300, 40, 2000
+    st, le, nm = synth.random_tree(tips, 71)
+    rng = np.random.default_rng(72)
+    dense = np.zeros((tips, 2 * n_base))
+    for s in range(n_base):
+        held = rng.choice(tips, 40, replace=False)
+        dense[held, s] = rng.multinomial(depth, rng.dirichlet(np.ones(held.size)))
+    dense[:, n_base:] = dense[:, :n_base]
+    for s in range(n_base, 2 * n_base):
+        dense[rng.choice(np.flatnonzero(dense[:, s])), s] += 1.0
+    rows = [np.flatnonzero(dense[r]) for r in range(tips)]
+    keep = [r for r in range(tips) if len(rows[r])]
+    table = ([f"t{r}" for r in keep], [f"s{i}" for i in range(2 * n_base)],
+             np.concatenate([[0], np.cumsum([len(rows[r]) for r in keep])]).astype(np.uint32),
+             np.concatenate([rows[r] for r in keep]).astype(np.uint32),
+             np.concatenate([dense[r, rows[r]] for r in keep]))
+    methods = ["unweighted", "weighted_normalized", "generalized"]
+    say("rarefy near duplicates")
+    r = ssu.Rarefaction(inputs(ssu, (st, le, nm), table), methods, depth, 3, with_replacement=False, seed=4,
+                        alpha=0.5)
+    r.execute()
+    assert not r.graph_used() and r.refined_pairs() > 0
+    si, oi, ip, ix = r.kept()
+    assert si.size == 2 * n_base
+    otree = O.Tree(np.asarray(st, np.uint8), le, nm)
+    eager = [[r.matrix(k, rep) for rep in range(3)] for k in range(len(methods))]
+    for rep in range(3):
+        t = O.Table([table[0][i] for i in oi], [table[1][s] for s in si], ip, ix, r.counts(rep))
+        for k, m in enumerate(methods):
+            kw = {"alpha": 0.5} if m == "generalized" else {}
+            err = rel_err(eager[k][rep], O.matrix(otree, t, m, **kw))
+            assert err <= TOL["fp32"], (m, rep, err)
+    refined_eager = r.refined_pairs()
+    r.execute()
+    assert r.graph_used() and r.refined_pairs() == refined_eager
+    for k in range(len(methods)):
+        for rep in range(3):
+            assert np.array_equal(r.matrix(k, rep), eager[k][rep]), (methods[k], rep)

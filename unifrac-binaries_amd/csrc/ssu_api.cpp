@@ -7,6 +7,7 @@
 #include <unistd.h>
 
 #include <charconv>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -47,6 +48,17 @@ const std::string& last_error() { return g_last_error; }
 namespace {
 
 std::mt19937_64 g_rng(5489u);
+
+// Wall-clock phases of the calling thread's last one-off matrix call, in ms:
+// inputs (table and tree structures), tree (shear, collapse, postorder), plan
+// (uploads, schedules, occupancy splits), execute (device work, refinement
+// included), copy (result to the host), total.
+enum CallPhase { CP_INPUT = 0, CP_TREE, CP_PLAN, CP_EXECUTE, CP_COPY, CP_TOTAL, CP_COUNT };
+thread_local double g_call_ms[CP_COUNT];
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
 using ssu::destroy_full;
 using ssu::dup_ids;
 using ssu::init_full;
@@ -106,9 +118,21 @@ ComputeSpec make_spec(Method m, bool fp64, bool vaw, double alpha, bool bypass, 
 template <class TOut>
 ComputeStatus run_full(const Table& t, const PhyloTree& tree, const ComputeSpec& spec, TOut* out) {
     std::string err;
+    auto t0 = Clock::now();
     ssu::PostorderNodes nodes = synced_nodes(t, tree);
+    g_call_ms[CP_TREE] = ms_since(t0);
+    t0 = Clock::now();
     std::unique_ptr<Plan> plan = Plan::create(t, nodes, spec, &err);
-    if (!plan || !plan->execute(&err)) { set_error(err); return output_error; }
+    if (!plan) { set_error(err); return output_error; }
+    g_call_ms[CP_PLAN] = ms_since(t0);
+    t0 = Clock::now();
+    if (!plan->execute(&err)) { set_error(err); return output_error; }
+    g_call_ms[CP_EXECUTE] = ms_since(t0);
+    t0 = Clock::now();
+    struct CopyTime {
+        Clock::time_point t0;
+        ~CopyTime() { g_call_ms[CP_COPY] = ms_since(t0); }
+    } copy_time{t0};
     const size_t nn = (size_t)t.n_samples * t.n_samples;
     if (plan->out_fp64() == (sizeof(TOut) == sizeof(double))) {
         if (!plan->copy_matrix(out, &err)) { set_error(err); return output_error; }
@@ -203,13 +227,18 @@ ComputeStatus matrix_inmem(const support_biom_t* td, const support_bptree_t* tr,
     if (!parse_method(method_name, m, fp64)) return unknown_method;
     if (fp64 != want_fp64) return invalid_method;
     if (!td || !tr || td->n_samples <= 0 || td->n_obs <= 0) return table_empty;
+    for (double& x : g_call_ms) x = 0.0;
+    const auto t0 = Clock::now();
     Table t = Table::borrow(td->obs_ids, td->sample_ids, td->indptr, td->indices, td->data, (uint32_t)td->n_obs,
                             (uint32_t)td->n_samples);
     PhyloTree tree = PhyloTree::from_arrays(tr->structure, tr->lengths, tr->names, tr->n_parens);
     ComputeStatus v = validate(t, tree);
     if (v != okay) return v;
-    return matrix_common<TReal, TMat>(t, tree, method_name, vaw, alpha, bypass, normalize, depth, with_repl, mmap_dir,
-                                      result);
+    g_call_ms[CP_INPUT] = ms_since(t0);
+    v = matrix_common<TReal, TMat>(t, tree, method_name, vaw, alpha, bypass, normalize, depth, with_repl, mmap_dir,
+                                   result);
+    g_call_ms[CP_TOTAL] = ms_since(t0);
+    return v;
 }
 
 bool file_exists(const char* f) {
@@ -657,7 +686,11 @@ ComputeStatus ssu_faith_pd_inmem(const support_biom_t* td, const support_bptree_
 
 // =================================================================== partial
 
-// src/api.cpp:527-567: stripes [stripe_start, stripe_stop) of the full matrix
+// src/api.cpp:527-567: stripes [stripe_start, stripe_stop) of the full matrix.
+// Only the pairs of those stripes are computed (the Gram tiles, pair blocks and
+// posting-list columns holding one).  n_substeps splits the reference's work
+// into sequential tasks without changing any value (src/api.cpp:447-491); the
+// range is computed here in one pass.
 ComputeStatus partial_v3(const char* biom_filename, const char* tree_filename, const char* unifrac_method,
                          bool variance_adjust, double alpha, bool bypass_tips, bool normalize_sample_count,
                          unsigned int, unsigned int stripe_start, unsigned int stripe_stop, partial_mat_t** result) {
@@ -683,7 +716,10 @@ ComputeStatus partial_v3(const char* biom_filename, const char* tree_filename, c
     ssu::PostorderNodes nodes = synced_nodes(t, tree);
     std::unique_ptr<Plan> plan =
         Plan::create(t, nodes, make_spec(m, fp64, variance_adjust, alpha, bypass_tips, normalize_sample_count), &err);
-    if (!plan || !plan->execute(&err)) { set_error(err); return output_error; }
+    if (!plan || !plan->set_stripes(stripe_start, stripe_stop, &err) || !plan->execute(&err)) {
+        set_error(err);
+        return output_error;
+    }
     partial_mat_t* r = (partial_mat_t*)malloc(sizeof(partial_mat_t));
     r->n_samples = n;
     r->sample_ids = dup_ids(t.sample_ids);
@@ -1355,6 +1391,8 @@ double ssu_rarefy_last_ms(const ssu_rarefy_t* r) { return r && r->batch ? r->bat
 int ssu_rarefy_graph_used(const ssu_rarefy_t* r) { return r && r->batch && r->batch->graph_used() ? 1 : 0; }
 uint64_t ssu_rarefy_pair_nodes(const ssu_rarefy_t* r) { return r && r->batch ? r->batch->pair_nodes() : 0; }
 
+uint64_t ssu_rarefy_refined_pairs(const ssu_rarefy_t* r) { return r && r->batch ? r->batch->refined_pairs() : 0; }
+
 void ssu_rarefy_destroy(ssu_rarefy_t** r) {
     if (!r || !*r) return;
     delete *r;
@@ -1441,6 +1479,38 @@ int ssu_plan_timings(const ssu_plan_t* plan, double* ms, int cap) {
 }
 
 uint64_t ssu_plan_sparse_terms(const ssu_plan_t* plan) { return plan ? plan->plan->sparse_terms() : 0; }
+
+uint64_t ssu_plan_refined_pairs(const ssu_plan_t* plan) { return plan ? plan->plan->refined_pairs() : 0; }
+
+ComputeStatus ssu_plan_execute_stripes(ssu_plan_t* plan, unsigned int start, unsigned int stop) {
+    std::string err;
+    if (!plan) return output_error;
+    if (!plan->plan->set_stripes(start, stop, &err) || !plan->plan->execute(&err)) {
+        set_error(err);
+        return output_error;
+    }
+    return okay;
+}
+
+ComputeStatus ssu_plan_copy_stripes(const ssu_plan_t* plan, unsigned int start, unsigned int stop, double* host_out) {
+    std::string err;
+    if (!plan) return output_error;
+    const uint32_t n = plan->plan->n_samples();
+    if (stop > (n + 1) / 2 || stop < start) {
+        set_error("ssu_plan_copy_stripes: stripes outside the matrix");
+        return output_error;
+    }
+    std::vector<double*> rows(stop - start);
+    for (uint32_t r = 0; r < stop - start; ++r) rows[r] = host_out + (size_t)r * n;
+    if (!plan->plan->copy_stripes(start, stop, rows.data(), &err)) { set_error(err); return output_error; }
+    return okay;
+}
+
+int ssu_last_call_timings(double* ms, int cap) {
+    const int k = std::min<int>(cap, CP_COUNT);
+    for (int i = 0; i < k; ++i) ms[i] = g_call_ms[i];
+    return k;
+}
 
 uint64_t ssu_plan_pair_nodes(const ssu_plan_t* plan) {
     if (!plan) return 0;

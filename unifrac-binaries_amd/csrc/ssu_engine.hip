@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <thread>
 
 namespace ssu {
 
@@ -71,6 +73,76 @@ bool upload_raw(DevMem& d, const void* h, size_t bytes, hipStream_t st, std::str
 
 }  // namespace
 
+// Pinned staging for device-to-host copies: two chunks, allocated on first use
+// and kept for the life of the process (pinned allocations are slow to make).
+namespace {
+constexpr size_t STAGE_CHUNK = size_t(64) << 20;
+struct Stage {
+    std::mutex mu;
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool failed = false;
+};
+Stage& stage() {
+    static Stage* s = new Stage();   // never destroyed: the runtime may already be gone at exit
+    return *s;
+}
+
+// dst[0..bytes) = src[0..bytes), split over up to `threads` threads
+void spread_copy(char* dst, const char* src, size_t bytes, unsigned threads) {
+    if (threads <= 1 || bytes < (size_t(4) << 20)) { std::memcpy(dst, src, bytes); return; }
+    std::vector<std::thread> pool;
+    const size_t part = (bytes / threads + 4095) / 4096 * 4096;
+    for (unsigned t = 1; t < threads; ++t) {
+        const size_t a = part * t;
+        if (a >= bytes) break;
+        pool.emplace_back([=] { std::memcpy(dst + a, src + a, std::min(part, bytes - a)); });
+    }
+    std::memcpy(dst, src, std::min(part, bytes));
+    for (auto& th : pool) th.join();
+}
+}  // namespace
+
+bool copy_to_host(void* host, const void* dev, size_t bytes, void* stream, std::string* err) {
+    if (!bytes) return true;
+    hipStream_t st = (hipStream_t)stream;
+    Stage& S = stage();
+    std::unique_lock<std::mutex> lock(S.mu);
+    if (bytes <= STAGE_CHUNK / 4 || S.failed) {
+        lock.unlock();
+        SSU_HIP(hipStreamSynchronize(st));
+        SSU_HIP(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+        return true;
+    }
+    if (!S.buf[0]) {
+        for (int b = 0; b < 2; ++b)
+            if (hipHostMalloc(&S.buf[b], STAGE_CHUNK, hipHostMallocDefault) != hipSuccess ||
+                hipEventCreateWithFlags(&S.done[b], hipEventDisableTiming) != hipSuccess) {
+                S.failed = true;   // no pinned memory: plain copies from now on
+                lock.unlock();
+                SSU_HIP(hipStreamSynchronize(st));
+                SSU_HIP(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+                return true;
+            }
+    }
+    const unsigned threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
+    auto len_of = [&](size_t c) { return std::min(STAGE_CHUNK, bytes - c * STAGE_CHUNK); };
+    SSU_HIP(hipMemcpyAsync(S.buf[0], dev, len_of(0), hipMemcpyDeviceToHost, st));
+    SSU_HIP(hipEventRecord(S.done[0], st));
+    for (size_t c = 0; c < nch; ++c) {
+        const int b = (int)(c & 1);
+        if (c + 1 < nch) {   // the other buffer was emptied in the previous iteration
+            SSU_HIP(hipMemcpyAsync(S.buf[b ^ 1], (const char*)dev + (c + 1) * STAGE_CHUNK, len_of(c + 1),
+                                   hipMemcpyDeviceToHost, st));
+            SSU_HIP(hipEventRecord(S.done[b ^ 1], st));
+        }
+        SSU_HIP(hipEventSynchronize(S.done[b]));
+        spread_copy((char*)host + c * STAGE_CHUNK, (const char*)S.buf[b], len_of(c), threads);
+    }
+    return true;
+}
+
 bool device_available(std::string* err) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
@@ -83,7 +155,13 @@ bool device_available(std::string* err) {
 
 struct PlanImpl {
     ComputeSpec spec;
-    uint32_t n = 0, m = 0;
+    uint32_t n = 0, m = 0, n_obs = 0;
+    PostorderNodes nodes;            // the plan's tree, kept for refinement sub-tables
+    // near-duplicate refinement (FP32 weighted / generalized): pairs below the
+    // bound are recomputed in FP64; refine_S: the bound scales with S_i + S_j
+    bool refine_on = false, refine_S = false;
+    float refine_tau = 0.0f;
+    uint64_t refined = 0;
     size_t ldn = 0;
     size_t blk = 0;                  // sample columns per embedding block (ldn: one block)
     bool gram = false;
@@ -106,6 +184,12 @@ struct PlanImpl {
     bool bits = false;
     uint32_t ldb = 0;
     std::vector<uint32_t> block_order;  // pair blocks / Gram tiles in launch order
+    // stripe selection (set_stripes): the pairs computed, and the blocks / tiles
+    // holding one (run_order, on the device in d_order_run; empty: block_order)
+    GapBand band;
+    bool band_all = true;
+    std::vector<uint32_t> run_order;
+    const std::vector<uint32_t>& order_used() const { return band_all ? block_order : run_order; }
     bool bad_lengths = false;
 
     // host schedule
@@ -113,6 +197,11 @@ struct PlanImpl {
     std::vector<int32_t> level_nodes;
     std::vector<uint32_t> level_off;
     std::vector<int32_t> child_ptr, child_idx;
+    // heavy-path schedule (internal nodes by rounds of heavy paths; used instead of
+    // the height levels when it takes fewer launches)
+    bool use_paths = false;
+    std::vector<int32_t> path_nodes, path_light, heavy;
+    std::vector<uint32_t> path_off, round_off;
     std::vector<int32_t> knodes;     // nodes entering the pair accumulation
     std::vector<double> klen_d;
     std::vector<float> klen_f;
@@ -128,10 +217,17 @@ struct PlanImpl {
     float times[PH_COUNT] = {};
     std::vector<double> totals;
 
-    DevMem d_indptr, d_indices, d_data, d_totals, d_leaf_k, d_leaf_row, d_level_nodes, d_child_ptr, d_child_idx;
+    DevMem d_indptr, d_indices, d_data, d_totals, d_leaf_k, d_leaf_row, d_level_nodes, d_child_ptr, d_child_idx,
+        d_path_nodes, d_path_light, d_path_off, d_heavy;
+    HeavyPaths hpaths() const {
+        return {d_path_nodes.as<int32_t>(), d_path_light.as<int32_t>(), d_path_off.as<uint32_t>(),
+                d_heavy.as<int32_t>(), d_child_ptr.as<int32_t>(), d_child_idx.as<int32_t>()};
+    }
+    void build_paths(const PostorderNodes& nodes);
     DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_order, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
         d_faith, d_stripes, d_smask, d_dmask, d_pc, d_pidx, d_pval, d_walked, d_rows, d_rcount, d_roff, d_rbsum,
-        d_sp_node, d_sp_len, d_umasks, d_B, d_poff, d_pcnt, d_obsum, d_Ssp, d_out2, d_pspmax, d_urare;
+        d_sp_node, d_sp_len, d_umasks, d_B, d_poff, d_pcnt, d_obsum, d_Ssp, d_out2, d_pspmax, d_urare, d_close,
+        d_ridx, d_order_run;
 
     hipStream_t own_st = nullptr;    // created with the plan; st may point at a caller's stream
     // per-execution inputs and output (nullptr: the plan's own table data / totals / result)
@@ -140,7 +236,9 @@ struct PlanImpl {
     void* x_out = nullptr;
     bool quiet = false;              // no events, no host synchronisation (stream capture)
 
+    double* x_S = nullptr;           // per-sample sums S (nullptr: the plan's own)
     const double* data_ptr() const { return x_data ? x_data : d_data.as<double>(); }
+    double* S_ptr() const { return x_S ? x_S : d_S.as<double>(); }
     const double* totals_ptr() const { return x_totals ? x_totals : d_totals.as<double>(); }
     void* out_ptr() const { return x_out ? x_out : d_out.p; }
     hipError_t mark(int i) { return quiet ? hipSuccess : hipEventRecord(ev[i], st); }
@@ -160,6 +258,7 @@ struct PlanImpl {
     bool embed_bits(DevMem& dst, uint32_t lb, bool structural, std::string* err);
     // node-major FP64 embedding of sample columns [c0, c0 + w) into dst (row stride w)
     bool embed(DevMem& dst, bool normalize, size_t c0, size_t w, std::string* err);
+    bool refine(const uint32_t* flags, std::string* err);
 };
 
 // Exact Gram column order.  Every non-zero length is an integer times a power
@@ -306,11 +405,13 @@ bool PlanImpl::split_gram(std::string* err) {
         }
         return total;
     };
-    // postings (4-byte slots) and row lists (8-byte slots), prefix counts and
-    // sample bits; fewer than 2^30 slots (32-bit slot offsets in the kernels)
+    // postings (4-byte slots) and row lists (8-byte slots), prefix counts, sample
+    // bits and the row lists' per-chunk counts and offsets; fewer than 2^30 slots
+    // (32-bit slot offsets in the kernels)
     auto fits = [&](uint64_t slots) {
         const double ks = (double)sp_node.size();
-        return slots * 12.0 + ks * (double)(npan + 1 + panels) * 4.0 <= GSPLIT_BUDGET && slots < (1u << 30);
+        const double rl = (double)n * row_list_chunks(n, (uint32_t)sp_node.size()) * 8.0;   // row-list counts, offsets
+        return slots * 12.0 + ks * (double)(npan + 1 + panels) * 4.0 + rl <= GSPLIT_BUDGET && slots < (1u << 30);
     };
     uint32_t thr = (uint32_t)std::max(1.0, std::ceil(theta * n));
     uint64_t slots = classify(thr);
@@ -366,6 +467,11 @@ bool PlanImpl::embed_bits(DevMem& dst, uint32_t lb, bool structural, std::string
     uint32_t* B = dst.as<uint32_t>();
     SSU_HIP(launch_bits_leaves(B, lb, d_leaf_k.as<int32_t>(), d_leaf_row.as<int32_t>(), (uint32_t)leaf_k.size(),
                                d_indptr.as<uint32_t>(), d_indices.as<uint32_t>(), data_ptr(), structural, st));
+    if (use_paths) {
+        for (size_t r = 0; r + 1 < round_off.size(); ++r)
+            SSU_HIP(launch_bits_paths(B, lb, hpaths(), round_off[r], round_off[r + 1], st));
+        return true;
+    }
     for (size_t h = 0; h + 1 < level_off.size(); ++h) {
         const uint32_t a = level_off[h], b = level_off[h + 1];
         SSU_HIP(launch_bits_level(B, lb, d_level_nodes.as<int32_t>() + a, b - a, d_child_ptr.as<int32_t>(),
@@ -380,12 +486,70 @@ bool PlanImpl::embed(DevMem& dst, bool normalize, size_t c0, size_t w, std::stri
     SSU_HIP(launch_embed_leaves(E, w, (uint32_t)c0, d_leaf_k.as<int32_t>(), d_leaf_row.as<int32_t>(),
                                 (uint32_t)leaf_k.size(), d_indptr.as<uint32_t>(), d_indices.as<uint32_t>(),
                                 data_ptr(), totals_ptr(), normalize, st));
+    if (use_paths) {
+        for (size_t r = 0; r + 1 < round_off.size(); ++r)
+            SSU_HIP(launch_embed_paths(E, w, hpaths(), round_off[r], round_off[r + 1], st));
+        return true;
+    }
     for (size_t h = 0; h + 1 < level_off.size(); ++h) {
         const uint32_t a = level_off[h], b = level_off[h + 1];
         SSU_HIP(launch_embed_level(E, w, d_level_nodes.as<int32_t>() + a, b - a, d_child_ptr.as<int32_t>(),
                                    d_child_idx.as<int32_t>(), st));
     }
     return true;
+}
+
+// Heavy-path decomposition: every internal node continues the path of its
+// largest child (the first of equal ones); a path runs from a leaf up to a node
+// that is not its parent's heavy child (or a child of the root).  A path's
+// round is one more than the latest round of the paths its other children head
+// (0 when they are all leaves), so at most log2(leaves) + 1 rounds, against one
+// launch per height level (a caterpillar tree of T tips has T - 1 levels).
+void PlanImpl::build_paths(const PostorderNodes& nodes) {
+    std::vector<int32_t> size(m, 1);
+    heavy.assign(std::max<uint32_t>(m, 1), -1);
+    for (uint32_t k = 0; k < m; ++k) {   // postorder: a node's subtree precedes it
+        for (int32_t c = child_ptr[k]; c < child_ptr[k + 1]; ++c) {
+            const int32_t ch = child_idx[c];
+            size[k] += size[ch];
+            if (heavy[k] < 0 || size[ch] > size[heavy[k]]) heavy[k] = ch;
+        }
+    }
+    // round of the path headed at each node (-1: a leaf); paths in head order
+    std::vector<int32_t> round(m, -1);
+    std::vector<std::vector<int32_t>> by_round;
+    std::vector<std::vector<uint32_t>> by_round_len;
+    std::vector<int32_t> path;
+    for (uint32_t h = 0; h < m; ++h) {
+        const int32_t p = nodes.parent[h];
+        const bool head = (uint32_t)p >= m || heavy[p] != (int32_t)h;
+        if (!head || nodes.is_leaf[h] || child_ptr[h] == child_ptr[h + 1]) continue;
+        path.clear();
+        for (int32_t k = (int32_t)h; child_ptr[k] < child_ptr[k + 1]; k = heavy[k]) path.push_back(k);
+        std::reverse(path.begin(), path.end());   // bottom to top
+        int32_t r = 0;
+        for (int32_t k : path)
+            for (int32_t c = child_ptr[k]; c < child_ptr[k + 1]; ++c)
+                if (child_idx[c] != heavy[k] && round[child_idx[c]] >= 0) r = std::max(r, round[child_idx[c]] + 1);
+        round[h] = r;
+        if ((size_t)r >= by_round.size()) { by_round.resize(r + 1); by_round_len.resize(r + 1); }
+        by_round[r].insert(by_round[r].end(), path.begin(), path.end());
+        by_round_len[r].push_back((uint32_t)path.size());
+    }
+    path_nodes.clear();
+    path_light.clear();
+    path_off.assign(1, 0u);
+    round_off.assign(1, 0u);
+    for (size_t r = 0; r < by_round.size(); ++r) {
+        for (int32_t k : by_round[r]) {
+            path_nodes.push_back(k);
+            const bool two = child_ptr[k + 1] - child_ptr[k] == 2;
+            const int32_t other = child_idx[child_ptr[k]] == heavy[k] ? child_idx[child_ptr[k] + 1] : child_idx[child_ptr[k]];
+            path_light.push_back(two ? other : -1);
+        }
+        for (uint32_t len : by_round_len[r]) path_off.push_back(path_off.back() + len);
+        round_off.push_back((uint32_t)path_off.size() - 1);
+    }
 }
 
 Plan::Plan() : p_(new PlanImpl) {}
@@ -409,6 +573,8 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     P.spec = spec;
     P.n = table.n_samples;
     P.m = nodes.m;
+    P.n_obs = table.n_obs;
+    P.nodes = nodes;
     P.ldn = round_up(std::max<uint32_t>(P.n, 1), PAIR_TILE);
 
     // schedule: leaves, height levels, children lists (postorder, left to right)
@@ -435,6 +601,10 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     for (uint32_t k = 0; k < P.m; ++k)
         if ((uint32_t)nodes.parent[k] < P.m) P.child_idx[fill[nodes.parent[k]]++] = (int32_t)k;
 
+    // heavy paths when they take fewer launches than the height levels (SSU_EMBED_PATHS=0 / 1 forces)
+    P.build_paths(nodes);
+    P.use_paths = P.round_off.size() < P.level_off.size();
+    if (const char* env = getenv("SSU_EMBED_PATHS")) P.use_paths = atoi(env) != 0;
     for (uint32_t k = 0; k < P.m; ++k) {
         P.len_all.push_back(nodes.length[k]);
         if (spec.bypass_tips && nodes.is_leaf[k]) continue;  // src/unifrac_cmp.cpp:123-124
@@ -479,6 +649,25 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
             if (!(table.data[b0 + q] >= 0.0)) P.sparse = false;
     }
     P.gen = P.gen && P.sparse;
+    // Near-duplicate refinement: an FP32 weighted or generalized distance is off
+    // by at most 2^-24 of its denominator through the FP32 rounding of the
+    // proportions (|u - v| from rounded values is off by at most 2^-24 (u + v)),
+    // a relative error of 2^-24 / d, plus the accumulation's own, well below
+    // 2e-7.  Pairs with d < 0.1 (d < 0.1 (S_i + S_j) unnormalised) are
+    // recomputed in FP64, which keeps every entry within 1e-6.  The
+    // unnormalised weighted distance needs S, computed on the shared-node path
+    // (tables with negative values or non-finite lengths run the dense kernel
+    // without it), and the unnormalised VAW distance has no per-pair
+    // denominator to bound it: those two are not refined.  SSU_REFINE=0 turns
+    // refinement off, SSU_REFINE_BELOW sets the bound.
+    if (!spec.fp64 && spec.plain_kernel == 0) {
+        P.refine_on = spec.method == Method::weighted_normalized || spec.method == Method::generalized ||
+                      (spec.method == Method::weighted_unnormalized && !spec.vaw && P.sparse);
+        P.refine_S = spec.method == Method::weighted_unnormalized;
+        P.refine_tau = 0.1f;
+        if (const char* env = getenv("SSU_REFINE_BELOW")) P.refine_tau = (float)atof(env);
+        if (const char* env = getenv("SSU_REFINE")) P.refine_on = P.refine_on && atoi(env) != 0;
+    }
     // Embedding blocks: the FP64 embedding of every node for every sample
     // (m x ldn doubles) is built in sample-column blocks when it would exceed
     // EMBED_BUDGET; each block feeds the FP32 operand, masks, sums or presence
@@ -517,11 +706,14 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
               upload(P.d_totals, P.totals, P.st, err) && upload(P.d_leaf_k, P.leaf_k, P.st, err) &&
               upload(P.d_leaf_row, P.leaf_row, P.st, err) && upload(P.d_level_nodes, P.level_nodes, P.st, err) &&
               upload(P.d_child_ptr, P.child_ptr, P.st, err) && upload(P.d_child_idx, P.child_idx, P.st, err) &&
+              upload(P.d_path_nodes, P.path_nodes, P.st, err) && upload(P.d_path_light, P.path_light, P.st, err) &&
+              upload(P.d_path_off, P.path_off, P.st, err) && upload(P.d_heavy, P.heavy, P.st, err) &&
               upload(P.d_knodes, P.knodes, P.st, err) && upload(P.d_len_all, P.len_all, P.st, err);
     if (ok) ok = spec.fp64 ? upload(P.d_klen, P.klen_d, P.st, err) : upload(P.d_klen, P.klen_f, P.st, err);
+    // lengths as the accumulation sees them, in FP64, for the per-sample denominators
+    // (declared here: the asynchronous upload reads it until the synchronisation below)
+    std::vector<double> ls(P.klen_f.size());
     if (ok) {
-        // lengths as the accumulation sees them, in FP64, for the per-sample denominators
-        std::vector<double> ls(P.klen_f.size());
         for (size_t q = 0; q < ls.size(); ++q) ls[q] = spec.fp64 ? P.klen_d[q] : (double)P.klen_f[q];
         ok = upload(P.d_klen_sum, ls, P.st, err);
     }
@@ -547,11 +739,13 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
 
 bool Plan::execute(std::string* err) { return execute_into(nullptr, nullptr, nullptr, false, err); }
 
-bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_out, bool quiet, std::string* err) {
+bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_out, bool quiet, std::string* err,
+                        uint32_t* d_flags, double* d_sums) {
     PlanImpl& P = *p_;
     P.x_data = d_data;
     P.x_totals = d_totals;
     P.x_out = d_out;
+    P.x_S = d_sums;
     P.quiet = quiet;
     const ComputeSpec& s = P.spec;
     const size_t nn = (size_t)P.n * P.n;
@@ -578,8 +772,10 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
     if (sums) {
         if (!P.d_S.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err)) return false;
         const size_t w_last = P.ldn - (P.ldn - 1) / P.blk * P.blk;
-        const size_t parts = std::max<size_t>({(size_t)gather_segments(K, P.blk) * P.blk,
-                                               (size_t)gather_segments(K, w_last) * w_last, 1});
+        size_t parts = std::max<size_t>({(size_t)gather_segments(K, P.blk) * P.blk,
+                                         (size_t)gather_segments(K, w_last) * w_last, 1});
+        if (s.fixed_segments && K)
+            parts = std::max(parts, (size_t)((K + fixed_rows_per_segment(K) - 1) / fixed_rows_per_segment(K)) * P.blk);
         if (!P.d_Spart.ensure(parts * sizeof(double), err)) return false;
     }
     if (P.sparse && !P.d_masks.ensure(std::max<size_t>((P.ldn / 32) * Kw, 1) * sizeof(uint32_t), err)) return false;
@@ -600,8 +796,8 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             !P.d_obsum.ensure((offset_scan_blocks(K) + 1) * sizeof(uint32_t), err) ||
             !P.d_walked.ensure((size_t)sparse_rows_blocks(P.n) * sizeof(uint32_t), err) ||
             !P.d_rows.ensure((size_t)P.split_slots * sizeof(unsigned long long), err) ||
-            !P.d_rcount.ensure((size_t)P.n * row_list_chunks(K) * sizeof(uint32_t), err) ||
-            !P.d_roff.ensure(((size_t)P.n * row_list_chunks(K) + 1) * sizeof(uint32_t), err) ||
+            !P.d_rcount.ensure((size_t)P.n * row_list_chunks(P.n, K) * sizeof(uint32_t), err) ||
+            !P.d_roff.ensure(((size_t)P.n * row_list_chunks(P.n, K) + 1) * sizeof(uint32_t), err) ||
             !P.d_rbsum.ensure((row_list_scan_blocks(P.n, K) + 1) * sizeof(uint32_t), err))
             return false;
     }
@@ -617,8 +813,8 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             !P.d_pidx.ensure((size_t)P.gsplit_slots * sizeof(uint32_t), err) ||
             !P.d_walked.ensure((size_t)sparse_rows_blocks(P.n) * sizeof(uint32_t), err) ||
             !P.d_rows.ensure((size_t)P.gsplit_slots * sizeof(unsigned long long), err) ||
-            !P.d_rcount.ensure((size_t)P.n * row_list_chunks(Ks) * sizeof(uint32_t), err) ||
-            !P.d_roff.ensure(((size_t)P.n * row_list_chunks(Ks) + 1) * sizeof(uint32_t), err) ||
+            !P.d_rcount.ensure((size_t)P.n * row_list_chunks(P.n, Ks) * sizeof(uint32_t), err) ||
+            !P.d_roff.ensure(((size_t)P.n * row_list_chunks(P.n, Ks) + 1) * sizeof(uint32_t), err) ||
             !P.d_rbsum.ensure((row_list_scan_blocks(P.n, Ks) + 1) * sizeof(uint32_t), err))
             return false;
     }
@@ -657,11 +853,12 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
                                            P.d_smask.as<uint32_t>(), P.d_E.as<double>(), w, (uint32_t)c0,
                                            P.d_sp_node.as<int32_t>(), nullptr, Ks, P.n, P.st));
         } else if (gather || sums) {
-            SSU_HIP(launch_gather_sums(gather ? P.d_U.as<float>() : nullptr, P.ldn, sums ? P.d_S.as<double>() : nullptr,
+            SSU_HIP(launch_gather_sums(gather ? P.d_U.as<float>() : nullptr, P.ldn, sums ? P.S_ptr() : nullptr,
                                        P.d_Spart.as<double>(), P.sparse ? P.d_masks.as<uint32_t>() : nullptr,
                                        split ? P.d_smask.as<uint32_t>() : nullptr, P.d_E.as<double>(), w, (uint32_t)c0, P.d_knodes.as<int32_t>(),
                                        P.d_klen_sum.as<double>(), K, P.n, P.st,
-                                       skip_rare ? P.d_urare.as<uint32_t>() : nullptr));
+                                       skip_rare ? P.d_urare.as<uint32_t>() : nullptr,
+                                       s.fixed_segments ? fixed_rows_per_segment(K) : 0u));
             if (gather && s.vaw)
                 SSU_HIP(launch_gather_sums(P.d_Cu.as<float>(), P.ldn, nullptr, nullptr, nullptr, nullptr, P.d_C.as<double>(), w,
                                            (uint32_t)c0, P.d_knodes.as<int32_t>(), nullptr, K, P.n, P.st));
@@ -682,7 +879,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         b.src_fp64 = s.fp64 || skip_rare;
         b.ldu = P.ldn;
         b.len = P.d_klen_sum.as<double>();
-        b.S = P.d_S.as<double>();
+        b.S = P.S_ptr();
         b.out = P.out_ptr();
         b.out_fp64 = s.fp64;
         b.walked = P.d_walked.as<uint32_t>();
@@ -703,6 +900,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         b.alpha = (float)s.alpha;
         b.out2 = P.gen ? P.d_out2.p : nullptr;
         b.pspmax = P.gen ? P.d_pspmax.as<double>() : nullptr;
+        b.band = P.band;
         SSU_HIP(launch_sparse_nodes(b, P.st));
     }
     SparseNodesArgs gb{};
@@ -725,6 +923,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         gb.slots = P.gsplit_slots;
         gb.gram = P.d_G.as<double>();
         gb.n_pad = (uint32_t)P.n_pad;
+        gb.band = P.band;
         SSU_HIP(launch_sparse_lists(gb, P.st));
     }
     SSU_HIP(P.mark(2));
@@ -733,7 +932,8 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         if (P.Kp) {
             SSU_HIP(launch_gram_i8(P.d_G.as<double>(), P.d_Pt.as<int8_t>(), P.Kp, P.d_digits.as<int8_t>(),
                                    P.d_chunks.as<GramChunk>(), (uint32_t)P.chunks.size(), (uint32_t)P.n_pad,
-                                   P.d_order.as<uint32_t>(), (uint32_t)P.block_order.size(), P.st));
+                                   P.band_all ? P.d_order.as<uint32_t>() : P.d_order_run.as<uint32_t>(),
+                                   (uint32_t)P.order_used().size(), P.st));
         } else {
             SSU_HIP(launch_fill(P.d_G.as<double>(), P.n_pad * P.n_pad, 0.0, P.st));   // no memset node in captures
         }
@@ -759,15 +959,15 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         a.masks = P.d_masks.as<uint32_t>();
         a.Kw = Kw;
         a.n = P.n;
-        a.S = P.d_S.as<double>();
+        a.S = P.S_ptr();
         a.normalized = s.method == Method::weighted_normalized;
         a.out = P.out_ptr();
         a.out_fp64 = s.fp64;
         const size_t nb = (P.n + 63) / 64;
         if (!P.d_visits.ensure(std::max<size_t>(2 * nb * nb, 1) * sizeof(uint32_t), err)) return false;
         a.visits = P.d_visits.as<uint32_t>();
-        a.order = P.d_order.as<uint32_t>();
-        a.n_blocks = (uint32_t)P.block_order.size();
+        a.order = P.band_all ? P.d_order.as<uint32_t>() : P.d_order_run.as<uint32_t>();
+        a.n_blocks = (uint32_t)P.order_used().size();
         if (split) a.dmask = P.d_dmask.as<uint32_t>();   // dense nodes only; adds the posting-list share
         if (s.fp64) SSU_HIP(launch_pairs_sparse(true, a, P.st));
         else if (P.gen) SSU_HIP(launch_pairs_gen32(a, (float)s.alpha, split ? P.d_out2.p : nullptr, P.st));
@@ -785,7 +985,8 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         a.out_fp64 = s.fp64;
         a.len = P.d_klen.p;
         a.normalized = normalized;
-        a.S = sums ? P.d_S.as<double>() : nullptr;
+        a.S = sums ? P.S_ptr() : nullptr;
+        a.band = P.band;
         if (s.fp64) {
             a.U = P.d_E.p;
             a.Cn = s.vaw ? P.d_C.p : nullptr;
@@ -799,6 +1000,15 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         SSU_HIP(P.mark(3));
         SSU_HIP(P.mark(4));
     }
+    // marks for the near-duplicate refinement (the plan's own buffer unless the caller brings one)
+    uint32_t* flags = d_flags;
+    if (P.refine_on && !flags && !quiet) {
+        if (!P.d_close.ensure(((size_t)P.n + 1) * sizeof(uint32_t), err)) return false;
+        flags = P.d_close.as<uint32_t>();
+    }
+    if (P.refine_on && flags)
+        SSU_HIP(launch_flag_close(flags, (const float*)P.out_ptr(), P.n, P.refine_S ? P.S_ptr() : nullptr,
+                                  P.refine_tau, P.band, P.st));
     SSU_HIP(P.mark(5));
     if (P.quiet) return true;
     SSU_HIP(hipStreamSynchronize(P.st));
@@ -823,14 +1033,145 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         SSU_HIP(hipEventElapsedTime(&P.times[PH_COMBINE], P.ev[EV_ROWS], P.ev[4]));
         P.times[PH_SPARSE] += rows_ms;
     }
+    P.refined = 0;
+    if (P.refine_on && flags && !P.refine(flags, err)) return false;
+    return true;
+}
+
+bool Plan::refine(const double* d_data, const double* d_totals, void* d_out, const uint32_t* d_flags, double* d_sums,
+                  std::string* err) {
+    PlanImpl& P = *p_;
+    if (!P.refine_on) return true;
+    P.x_data = d_data;
+    P.x_totals = d_totals;
+    P.x_out = d_out;
+    P.x_S = d_sums;
+    P.quiet = false;
+    P.refined = 0;
+    if (!d_flags && !P.d_close.p) return true;   // never marked
+    if (P.refine_S && !d_sums && !P.d_S.p) { if (err) *err = "refine: no per-sample sums"; return false; }
+    return P.refine(d_flags ? d_flags : P.d_close.as<uint32_t>(), err);
+}
+
+bool Plan::refines() const { return p_->refine_on; }
+
+bool Plan::set_stripes(uint32_t start, uint32_t stop, std::string* err) {
+    PlanImpl& P = *p_;
+    const uint32_t total = (P.n + 1) / 2;
+    if (stop > total) {
+        if (err) *err = "stripe range [" + std::to_string(start) + ", " + std::to_string(stop) + ") outside the " +
+                        std::to_string(total) + " stripes";
+        return false;
+    }
+    if (start == 0 && (stop == 0 || stop == total)) {
+        P.band = GapBand();
+        P.band_all = true;
+        P.run_order.clear();
+        return true;
+    }
+    P.band = stripe_band(P.n, start, stop);
+    P.band_all = false;
+    // the blocks / tiles of the kernel's order that hold a selected pair
+    P.run_order.clear();
+    uint32_t ei = 0, ej = 0;   // block edges (rows, columns)
+    if (P.gram) ei = ej = GRAM_BM;
+    else if (P.sparse && P.spec.fp64) ei = ej = 64;
+    else if (P.sparse) { ei = 32; ej = 64; }
+    for (uint32_t bij : P.block_order) {
+        const uint32_t bi = bij >> 16, bj = bij & 0xffffu;
+        const bool diag_tile = P.gram && bi == bj;   // diag(G) feeds every pair of the combine
+        if (diag_tile || P.band.meets_tile(bi * ei, std::min(P.n, bi * ei + ei), bj * ej, std::min(P.n, bj * ej + ej)))
+            P.run_order.push_back(bij);
+    }
+    if (!P.block_order.empty()) {
+        if (!upload(P.d_order_run, P.run_order.empty() ? std::vector<uint32_t>{0u} : P.run_order, P.st, err)) return false;
+        if (hipStreamSynchronize(P.st) != hipSuccess) { if (err) *err = "stripe order upload failed"; return false; }
+    }
+    return true;
+}
+uint64_t Plan::refined_pairs() const { return p_->refined; }
+
+// The marked pairs, recomputed by the FP64 method on the sub-table of the
+// samples involved: the same tree, the same per-sample values and totals, the
+// dense FP64 pair kernel (each pair's terms summed over every node in order)
+// and per-sample sums in segments fixed by the node count, so a pair's FP64
+// distance does not depend on which other samples the sub-table holds -- a
+// stripe range and the full matrix refine a pair to the same value.  Each pair
+// is marked by its own FP32 distance (and S), tested again when its FP64 value
+// is written.  The samples are taken in groups whose FP64
+// plan fits in REFINE_BUDGET; with more than one group, every pair of groups
+// runs as one sub-table and writes only its cross pairs.
+bool PlanImpl::refine(const uint32_t* flags, std::string* err) {
+    uint32_t count = 0;
+    SSU_HIP(hipStreamSynchronize(st));
+    SSU_HIP(hipMemcpy(&count, flags, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (!count) return true;
+    std::vector<uint32_t> mark(n);
+    SSU_HIP(hipMemcpy(mark.data(), flags + 1, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> F;
+    for (uint32_t s = 0; s < n; ++s)
+        if (mark[s]) F.push_back(s);
+    // the table as this execution sees it
+    std::vector<uint32_t> ip(n_obs + 1), ix(nnz);
+    std::vector<double> vals(nnz), tot(n);
+    SSU_HIP(hipMemcpy(ip.data(), d_indptr.p, ip.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (nnz) {
+        SSU_HIP(hipMemcpy(ix.data(), d_indices.p, nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        SSU_HIP(hipMemcpy(vals.data(), data_ptr(), nnz * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (n) SSU_HIP(hipMemcpy(tot.data(), totals_ptr(), (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+
+    constexpr double REFINE_BUDGET = 24.0 * (1u << 30);
+    // FP64 plan of g samples: the embedding m x g and the result g x g, 8 bytes each, twice over for scratch
+    auto fits = [&](double samples) { return 16.0 * ((double)std::max<uint32_t>(m, 1) * samples + samples * samples) <= REFINE_BUDGET; };
+    uint32_t g = 256;   // a sub-table holds up to 2 g samples
+    while (g < 16384 && fits(4.0 * g)) g *= 2;
+    std::vector<std::pair<uint32_t, uint32_t>> groups;   // [begin, end) in F
+    const uint32_t nf = (uint32_t)F.size();
+    if (nf <= 2 * g) groups.push_back({0, nf});
+    else
+        for (uint32_t b = 0; b < nf; b += g) groups.push_back({b, std::min(nf, b + g)});
+    ComputeSpec sp = spec;
+    sp.fp64 = true;
+    sp.plain_kernel = 1;
+    sp.fixed_segments = true;
+    std::vector<int64_t> local(n, -1);
+    for (size_t A = 0; A < groups.size(); ++A)
+        for (size_t B = A; B < groups.size(); ++B) {
+            std::vector<uint32_t> idx;
+            for (uint32_t q = groups[A].first; q < groups[A].second; ++q) idx.push_back(F[q]);
+            const uint32_t na = (uint32_t)idx.size();
+            if (B != A)
+                for (uint32_t q = groups[B].first; q < groups[B].second; ++q) idx.push_back(F[q]);
+            const uint32_t ns = (uint32_t)idx.size();
+            for (uint32_t a = 0; a < ns; ++a) local[idx[a]] = a;
+            std::vector<uint32_t> sip(1, 0), six;
+            std::vector<double> sdata;
+            for (uint32_t i = 0; i < n_obs; ++i) {
+                for (uint32_t j = ip[i]; j < ip[i + 1]; ++j)
+                    if (local[ix[j]] >= 0) { six.push_back((uint32_t)local[ix[j]]); sdata.push_back(vals[j]); }
+                sip.push_back((uint32_t)six.size());
+            }
+            for (uint32_t a = 0; a < ns; ++a) local[idx[a]] = -1;
+            Table sub = Table::own(std::vector<std::string>(n_obs), std::vector<std::string>(ns), std::move(sip),
+                                   std::move(six), std::move(sdata));
+            for (uint32_t a = 0; a < ns; ++a) sub.totals[a] = tot[idx[a]];
+            std::unique_ptr<Plan> sp_plan = Plan::create(sub, nodes, sp, err);
+            if (!sp_plan || !sp_plan->execute(err)) return false;
+            if (!upload(d_ridx, idx, st, err)) return false;
+            SSU_HIP(launch_refine_scatter((float*)out_ptr(), n, (const double*)sp_plan->device_output(), ns,
+                                          d_ridx.as<uint32_t>(), na, B != A, refine_S ? S_ptr() : nullptr,
+                                          refine_tau, st));
+            SSU_HIP(hipStreamSynchronize(st));   // idx and the sub-plan's result are read until here
+        }
+    refined = count;
     return true;
 }
 
 bool Plan::copy_matrix(void* host, std::string* err) const {
     const PlanImpl& P = *p_;
     const size_t bytes = (size_t)P.n * P.n * (P.spec.fp64 ? sizeof(double) : sizeof(float));
-    if (bytes) SSU_HIP(hipMemcpy(host, P.d_out.p, bytes, hipMemcpyDeviceToHost));
-    return true;
+    return copy_to_host(host, P.d_out.p, bytes, P.st, err);
 }
 
 bool Plan::copy_rows(uint32_t row0, uint32_t nrows, void* host, std::string* err) const {
@@ -841,9 +1182,7 @@ bool Plan::copy_rows(uint32_t row0, uint32_t nrows, void* host, std::string* err
         return false;
     }
     const size_t es = P.spec.fp64 ? sizeof(double) : sizeof(float);
-    if (nrows) SSU_HIP(hipMemcpy(host, P.d_out.as<char>() + (size_t)row0 * P.n * es, (size_t)nrows * P.n * es,
-                                 hipMemcpyDeviceToHost));
-    return true;
+    return copy_to_host(host, P.d_out.as<char>() + (size_t)row0 * P.n * es, (size_t)nrows * P.n * es, P.st, err);
 }
 
 bool Plan::copy_stripes(uint32_t start, uint32_t stop, double* const* host, std::string* err) const {
@@ -855,9 +1194,9 @@ bool Plan::copy_stripes(uint32_t start, uint32_t stop, double* const* host, std:
         SSU_HIP(launch_matrix_to_stripes<double>(P.d_stripes.as<double>(), P.d_out.as<double>(), P.n, start, stop, P.st));
     else
         SSU_HIP(launch_matrix_to_stripes<float>(P.d_stripes.as<double>(), P.d_out.as<float>(), P.n, start, stop, P.st));
-    SSU_HIP(hipStreamSynchronize(P.st));
-    for (size_t r = 0; r < rows; ++r)
-        SSU_HIP(hipMemcpy(host[r], P.d_stripes.as<double>() + r * P.n, P.n * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> tmp(rows * P.n);
+    if (!copy_to_host(tmp.data(), P.d_stripes.p, tmp.size() * sizeof(double), P.st, err)) return false;
+    for (size_t r = 0; r < rows; ++r) std::memcpy(host[r], tmp.data() + r * P.n, P.n * sizeof(double));
     return true;
 }
 
@@ -889,10 +1228,7 @@ uint64_t Plan::processed_nodes() const { return p_->knodes.size(); }
 uint64_t Plan::kernel_terms() const {
     const PlanImpl& P = *p_;
     const uint64_t K = P.knodes.size();
-    if (P.gram) {
-        const uint64_t t = P.n_pad / GRAM_BM;
-        return t * (t + 1) / 2 * GRAM_BM * GRAM_BM * (uint64_t)P.Kp;
-    }
+    if (P.gram) return (uint64_t)P.order_used().size() * GRAM_BM * GRAM_BM * (uint64_t)P.Kp;
     if (P.sparse) {
         // visited nodes per block, times the pairs a block evaluates
         const size_t nbi = P.spec.fp64 ? (P.n + 63) / 64 : (P.n + 31) / 32, nbj = (P.n + 63) / 64;
@@ -901,7 +1237,7 @@ uint64_t Plan::kernel_terms() const {
         if (hipMemcpy(v.data(), P.d_visits.p, v.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
             return 0;
         uint64_t sum = 0;
-        for (uint32_t bij : P.block_order) sum += v[(size_t)(bij >> 16) * nbj + (bij & 0xffffu)];
+        for (uint32_t bij : P.order_used()) sum += v[(size_t)(bij >> 16) * nbj + (bij & 0xffffu)];
         return sum * (P.spec.fp64 ? 64 * 64 : 32 * 64);
     }
     // dense tiles: 128 x 128 for weighted / plain unweighted, 64 x 64 for generalized and VAW

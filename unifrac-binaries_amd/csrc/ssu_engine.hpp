@@ -28,6 +28,9 @@ struct ComputeSpec {
     bool bypass_tips = false;
     bool normalize = true;         // normalize_sample_counts
     int plain_kernel = 0;     // 0: MFMA Gram (unweighted) / shared-node kernel (weighted); 1: dense pair tiles
+    // per-sample sums S in segments that depend on the node count alone, so that a
+    // sample's S does not depend on the other samples of the table (refinement sub-tables)
+    bool fixed_segments = false;
 };
 
 // Phases of one execution, timed with HIP events on the plan's stream during
@@ -50,7 +53,31 @@ public:
     // in caller-owned device buffers (nullptr: the plan's own).  quiet: records no
     // events and does not synchronise, so the work can be captured into a graph;
     // every buffer the plan needs must already exist (run it once non-quiet first).
-    bool execute_into(const double* d_data, const double* d_totals, void* d_out, bool quiet, std::string* err);
+    // FP32 weighted and generalized plans mark the pairs whose distance is small
+    // enough for the FP32 rounding of the proportions to exceed 1e-6 relative
+    // (refines()); a non-quiet call recomputes them in FP64 before it returns.
+    // d_flags (n + 1 words, optional) receives the marks instead of the plan's
+    // own buffer; after a quiet call, refine() with the same buffers finishes
+    // the job once the work has run.
+    // d_sums (n doubles, optional) receives the per-sample sums S in place of the
+    // plan's own buffer (the marks of the unnormalised weighted distance are tested
+    // against them again by refine()).
+    bool execute_into(const double* d_data, const double* d_totals, void* d_out, bool quiet, std::string* err,
+                      uint32_t* d_flags = nullptr, double* d_sums = nullptr);
+    // FP64 recomputation of the pairs marked in d_flags (nullptr: the plan's
+    // own marks) on the sub-table of the samples involved, written into d_out
+    // (nullptr: the plan's own result).  Synchronous; nothing to do when no
+    // pair is marked.
+    bool refine(const double* d_data, const double* d_totals, void* d_out, const uint32_t* d_flags, double* d_sums,
+                std::string* err);
+    bool refines() const;
+    uint64_t refined_pairs() const;   // pairs the last refine() recomputed
+    // Later executions compute only the pairs of stripes [start, stop) (stripe s
+    // holds the pairs (k, (k + s + 1) mod n), src/unifrac_task_impl.hpp:367):
+    // the tiles, pair blocks and posting-list columns that hold one.  Entries of
+    // other pairs in the result are left unspecified.  start = stop = 0: every
+    // pair again.  false (err set) when stop exceeds (n + 1) / 2.
+    bool set_stripes(uint32_t start, uint32_t stop, std::string* err);
     // Launch on a caller's HIP stream (nullptr: the plan's own stream).
     void set_stream(void* stream);
     void* stream() const;
@@ -77,5 +104,10 @@ private:
 
 // true iff a HIP device is usable; err explains why not
 bool device_available(std::string* err);
+
+// bytes of device memory into pageable host memory through a process-wide
+// pinned staging area: chunks copied over PCIe on `stream` while the previous
+// chunk is spread into `host` by several threads
+bool copy_to_host(void* host, const void* dev, size_t bytes, void* stream, std::string* err);
 
 }  // namespace ssu

@@ -75,6 +75,116 @@ __global__ void __launch_bounds__(EMB_THREADS) k_embed_level(double* __restrict_
     }
 }
 
+// Heavy paths (each internal node continues the path of its largest child):
+// path p holds internal nodes nodes[off[p] .. off[p + 1]) bottom to top, the
+// first one's heavy child a leaf.  Every other child of a path node heads a
+// path of an earlier round (or is a leaf), so the paths of one round are
+// independent and a thread walks its path keeping the running value in
+// registers: a node reads only its other children.  light[q] is the other
+// child of a two-child node (E = heavy + other: the same FP64 sum as
+// (0 + c0) + c1 in either order), or -1 for any other child count (children
+// added left to right from 0, the heavy child's value from the registers).
+// The other children of up to PATH_BATCH consecutive two-child nodes are
+// loaded before they are added, so the walk keeps that many loads in flight.
+constexpr int PATH_BATCH = 8;
+__global__ void __launch_bounds__(EMB_THREADS) k_embed_paths(double* __restrict__ E, size_t ldn,
+                                                             const int32_t* __restrict__ nodes,
+                                                             const int32_t* __restrict__ light,
+                                                             const uint32_t* __restrict__ off, uint32_t n_paths,
+                                                             const int32_t* __restrict__ heavy,
+                                                             const int32_t* __restrict__ child_ptr,
+                                                             const int32_t* __restrict__ child_idx) {
+    const size_t s2 = (size_t)blockIdx.x * EMB_THREADS + threadIdx.x;   // samples 2 s2, 2 s2 + 1
+    const size_t ld2 = ldn / 2;
+    if (s2 >= ld2) return;
+    double2* E2 = reinterpret_cast<double2*>(E);
+    for (uint32_t p = blockIdx.y; p < n_paths; p += gridDim.y) {
+        const uint32_t a = off[p], b = off[p + 1];
+        double2 run = E2[(size_t)heavy[nodes[a]] * ld2 + s2];
+        for (uint32_t q = a; q < b;) {
+            int32_t lt[PATH_BATCH];
+            int g = 0;
+            bool more = true;
+#pragma unroll
+            for (int u = 0; u < PATH_BATCH; ++u) {
+                lt[u] = q + u < b ? light[q + u] : -1;
+                more = more && lt[u] >= 0;
+                g += more ? 1 : 0;
+            }
+            double2 v[PATH_BATCH];
+#pragma unroll
+            for (int u = 0; u < PATH_BATCH; ++u)
+                if (u < g) v[u] = E2[(size_t)lt[u] * ld2 + s2];
+#pragma unroll
+            for (int u = 0; u < PATH_BATCH; ++u)
+                if (u < g) {
+                    run.x += v[u].x;
+                    run.y += v[u].y;
+                    E2[(size_t)nodes[q + u] * ld2 + s2] = run;
+                }
+            q += g;
+            if (g < PATH_BATCH && q < b) {   // a node with one or three or more children
+                const int32_t k = nodes[q], h = heavy[k];
+                double2 acc = make_double2(0.0, 0.0);
+                for (int32_t c = child_ptr[k]; c < child_ptr[k + 1]; ++c) {
+                    const int32_t ch = child_idx[c];
+                    const double2 w = ch == h ? run : E2[(size_t)ch * ld2 + s2];
+                    acc.x += w.x;
+                    acc.y += w.y;
+                }
+                run = acc;
+                E2[(size_t)k * ld2 + s2] = run;
+                ++q;
+            }
+        }
+    }
+}
+
+// Presence bits along heavy paths (as k_embed_paths, with OR): one thread per (path, word).
+__global__ void __launch_bounds__(256) k_bits_paths(uint32_t* __restrict__ B, uint32_t ldb,
+                                                    const int32_t* __restrict__ nodes,
+                                                    const int32_t* __restrict__ light,
+                                                    const uint32_t* __restrict__ off, uint32_t n_paths,
+                                                    const int32_t* __restrict__ heavy,
+                                                    const int32_t* __restrict__ child_ptr,
+                                                    const int32_t* __restrict__ child_idx) {
+    const uint32_t w = blockIdx.x * 256 + threadIdx.x;
+    if (w >= ldb) return;
+    for (uint32_t p = blockIdx.y; p < n_paths; p += gridDim.y) {
+        const uint32_t a = off[p], b = off[p + 1];
+        uint32_t run = B[(size_t)heavy[nodes[a]] * ldb + w];
+        for (uint32_t q = a; q < b;) {
+            int32_t lt[PATH_BATCH];
+            int g = 0;
+            bool more = true;
+#pragma unroll
+            for (int u = 0; u < PATH_BATCH; ++u) {
+                lt[u] = q + u < b ? light[q + u] : -1;
+                more = more && lt[u] >= 0;
+                g += more ? 1 : 0;
+            }
+            uint32_t v[PATH_BATCH];
+#pragma unroll
+            for (int u = 0; u < PATH_BATCH; ++u)
+                if (u < g) v[u] = B[(size_t)lt[u] * ldb + w];
+#pragma unroll
+            for (int u = 0; u < PATH_BATCH; ++u)
+                if (u < g) {
+                    run |= v[u];
+                    B[(size_t)nodes[q + u] * ldb + w] = run;
+                }
+            q += g;
+            if (g < PATH_BATCH && q < b) {
+                const int32_t k = nodes[q], h = heavy[k];
+                for (int32_t c = child_ptr[k]; c < child_ptr[k + 1]; ++c)
+                    if (child_idx[c] != h) run |= B[(size_t)child_idx[c] * ldb + w];
+                B[(size_t)k * ldb + w] = run;
+                ++q;
+            }
+        }
+    }
+}
+
 // Segmented pass over the processed rows: blockIdx.y owns rows [k0, k1), k0 a
 // multiple of 32.  Per thread (sample s) and row r, e = E[knodes[r]][s]:
 //   GATHER: U[r][s] = (float)e
@@ -572,7 +682,7 @@ __global__ void __launch_bounds__((TT / RI) * (TT / RJ))
 k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __restrict__ rows, size_t ldn,
         const T* __restrict__ len, uint32_t K,
         uint32_t n, const double* __restrict__ S, const double* __restrict__ totals, double alpha_d, int normalized,
-        void* __restrict__ out_v, int out_fp64) {
+        void* __restrict__ out_v, int out_fp64, GapBand band) {
     constexpr int NTI = TT / RI, NTJ = TT / RJ, NT = NTI * NTJ;
     constexpr bool TWO = PairTraits<OP>::two;
     constexpr bool CNT = PairTraits<OP>::counts;
@@ -586,7 +696,7 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
     constexpr int LPT = LOADS / NT;                          // per thread
     typedef __attribute__((ext_vector_type(VEC))) T vecT;
     const uint32_t bi = blockIdx.y, bj = blockIdx.x;
-    if (bi > bj) return;
+    if (bi > bj || !band.meets_tile(bi * TT, bi * TT + TT, bj * TT, bj * TT + TT)) return;
     const uint32_t t = threadIdx.x;
     const uint32_t ti = t % NTI, tj = t / NTI;
 
@@ -1198,17 +1308,21 @@ __device__ __forceinline__ double vmin_nonneg(double a, double b) { return a < b
 
 // Row lists of the sparse nodes: for sample a, the sparse nodes r that a holds,
 // in increasing r, each as r << 32 | (a's index in r's posting list).  One
-// wave per (32-sample panel P, chunk of RL_KC nodes) transposes the panel's
+// wave per (32-sample panel P, chunk of kcw nodes) transposes the panel's
 // bits with ballots, so the order is deterministic.  With rows == nullptr the
 // wave writes its per-sample counts to counts[a * nkc + chunk]; otherwise it
 // writes its entries from offsets[a * nkc + chunk] (the exclusive scan of the
 // counts in that order).
-constexpr uint32_t RL_KC = 256;
+// Chunks are RL_KC nodes wide unless that would make more than RL_MAX_CHUNKS
+// chunks or n * chunks above RL_MAX_ENTRIES (the count and offset arrays):
+// then the width doubles until both hold.
+constexpr uint32_t RL_KC = 256, RL_MAX_CHUNKS = 65535;
+constexpr size_t RL_MAX_ENTRIES = size_t(1) << 27;
 __global__ void __launch_bounds__(64) k_row_lists(unsigned long long* __restrict__ rows, uint32_t* __restrict__ counts,
                                                   const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ smask,
                                                   const uint32_t* __restrict__ dmask, const uint32_t* __restrict__ pc,
-                                                  uint32_t K, uint32_t n, uint32_t nkc,
+                                                  uint32_t K, uint32_t n, uint32_t nkc, uint32_t kcw,
                                                   const uint32_t* __restrict__ poff, uint32_t slots) {
     const uint32_t P = blockIdx.x, kc = blockIdx.y, lane = threadIdx.x;
     if (rows && poff[K] > slots) return;   // more postings than slots: never, by the plan's bound
@@ -1216,8 +1330,8 @@ __global__ void __launch_bounds__(64) k_row_lists(unsigned long long* __restrict
     uint32_t len = 0;   // lane s < 32: entries so far of sample 32 P + s in this chunk
     unsigned long long* dst =
         (rows && lane < 32 && a_lane < n) ? rows + offsets[(size_t)a_lane * nkc + kc] : nullptr;
-    const uint32_t k_end = min(K, (kc + 1) * RL_KC);
-    for (uint32_t k0 = kc * RL_KC; k0 < k_end; k0 += 64) {
+    const uint32_t k_end = (uint32_t)min((uint64_t)K, (uint64_t)(kc + 1) * kcw);
+    for (uint32_t k0 = kc * kcw; k0 < k_end; k0 += 64) {
         const uint32_t r = k0 + lane;
         uint32_t w = 0, base = 0;
         if (r < k_end && !((dmask[r >> 5] >> (r & 31)) & 1u)) {
@@ -1415,16 +1529,33 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                                                    uint32_t nchunks, uint32_t nkc, double* __restrict__ G,
                                                    uint32_t tiles, const double* __restrict__ ssp,
                                                    TOut* __restrict__ out2, const double* __restrict__ pspmax,
-                                                   float alpha) {
+                                                   float alpha, GapBand band) {
     // GEN keeps two accumulators (numerator, denominator) on half-width chunks
     constexpr uint32_t CW = GEN ? SR_CW / 2 : SR_CW;
     // chunk-major: the rows in flight share one column chunk of every posting
     // list, a quarter of the posting data at 16k samples (cache-resident)
     const uint32_t a = blockIdx.x % n, c = blockIdx.x / n;
     const uint32_t b0 = c * CW, b1 = min(n, b0 + CW);
-    // uniform: nothing at or above the diagonal here (weighted: strictly above); more
-    // postings than slots never happens (the plan's bound) and is skipped, not read
-    if (a >= n || (GRAM ? b1 <= a : b1 <= a + 1) || poff[K] > slots) {
+    // the selected columns b > a of this chunk: the hull [lo, hi) of the band's two gap
+    // windows, widened to whole 32-sample panels (inside the chunk: b0 is a panel start)
+    uint32_t lo = 0xffffffffu, hi = 0;
+    {
+        const uint64_t c_lo = max(b0, a + 1), c_hi = b1;
+        const uint64_t w[2][2] = {{(uint64_t)a + band.a0, (uint64_t)a + band.a1 + 1},
+                                  {(uint64_t)a + band.b0, (uint64_t)a + band.b1 + 1}};
+        for (int k = 0; k < 2; ++k) {
+            const uint64_t x = max(c_lo, w[k][0]), y = min(c_hi, w[k][1]);
+            if (x < y) { lo = min(lo, (uint32_t)x); hi = max(hi, (uint32_t)y); }
+        }
+        if (lo < hi) {
+            lo &= ~31u;
+            hi = min(b1, (hi + 31u) & ~31u);
+        }
+    }
+    const bool has_diag = GRAM && b0 <= a && a < b1;
+    // uniform: no selected column here (nor, unweighted, the diagonal); more postings
+    // than slots never happens (the plan's bound) and is skipped, not read
+    if (a >= n || (lo >= hi && !has_diag) || poff[K] > slots) {
         if (walked && threadIdx.x == 0) walked[blockIdx.x] = 0;
         return;
     }
@@ -1469,16 +1600,15 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     const int fb = FINE ? 61 : 51;
     const double scale = ldexp(1.0, fb - e2), inv = ldexp(1.0, e2 - fb);
     unsigned long long* const accb = acc - b0;   // accb[b]: column b (b0 <= b < b1)
-    const bool has_diag = GRAM && b0 <= a && a < b1;
     if (GRAM && has_diag) {   // G(a, a): the same fixed-point terms, every one of them
         unsigned long long x = 0ull;
         for (uint32_t e = e0 + threadIdx.x; e < e1; e += 64 * SR_WAVES)
             x += to_fixed(len[(uint32_t)(rows[e] >> 32)] * scale);
         if (x) atomicAdd(&diag_s, x);
     }
-    const uint32_t pb0 = b0 >> 5, pb1 = b1 == n ? npan : (b1 >> 5);
+    const uint32_t pb0 = lo >> 5, pb1 = lo >= hi ? pb0 : (hi == n ? npan : (hi >> 5));
     uint32_t total_walked = 0;
-    for (uint32_t r0 = e0 + 64 * wv; r0 < e1; r0 += 64 * SR_WAVES) {   // uniform per wave
+    for (uint32_t r0 = e0 + 64 * wv; r0 < e1 && pb0 < pb1; r0 += 64 * SR_WAVES) {   // uniform per wave
         uint32_t start = 0, end = 0, r = 0, slot0 = 0;
         TV ua = (TV)0;
         float uap = 0.0f;
@@ -1613,9 +1743,11 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         for (uint32_t w = 0; w < SR_WAVES; ++w) sum += wk[w];
         walked[blockIdx.x] = sum;
     }
+    // the walked panels' columns past the diagonal (every posting of theirs was added)
+    const uint32_t wlo = max(a + 1, pb0 * 32), whi = min(hi, pb1 * 32);
     if constexpr (GRAM) {
         if (has_diag && threadIdx.x == 0 && diag_s) G[gram_slot(a, a, tiles)] += (double)diag_s * inv;
-        for (uint32_t b = max(a + 1, b0) + threadIdx.x; b < b1; b += 64 * SR_WAVES) {
+        for (uint32_t b = wlo + threadIdx.x; b < whi; b += 64 * SR_WAVES) {
             const unsigned long long v = acc[b - b0];
             if (v) G[gram_slot(a, b, tiles)] += (double)v * inv;
         }
@@ -1623,7 +1755,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         // numerator and denominator shares: P_a + P_b plus the corrections of the
         // shared nodes (ssp = per-sample sum of l u^a over the sparse nodes)
         const double sa = ssp[a];
-        for (uint32_t b = max(a + 1, b0) + threadIdx.x; b < b1; b += 64 * SR_WAVES) {
+        for (uint32_t b = wlo + threadIdx.x; b < whi; b += 64 * SR_WAVES) {
             const double base2 = sa + ssp[b];
             out[(size_t)a * n + b] = (TOut)(base2 + (double)(long long)acc[b - b0] * inv);
             out2[(size_t)a * n + b] = (TOut)(base2 + (double)(long long)acc[CW + b - b0] * inv);
@@ -1632,10 +1764,10 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         // FP32 values: the sparse nodes' share of sum l |u_a - u_b| itself, formed in
         // FP64 before the rounding to FP32
         const double sa = ssp[a];
-        for (uint32_t b = max(a + 1, b0) + threadIdx.x; b < b1; b += 64 * SR_WAVES)
+        for (uint32_t b = wlo + threadIdx.x; b < whi; b += 64 * SR_WAVES)
             out[(size_t)a * n + b] = (TOut)((sa + ssp[b]) - 2.0 * ((double)acc[b - b0] * inv));
     } else {
-        for (uint32_t b = max(a + 1, b0) + threadIdx.x; b < b1; b += 64 * SR_WAVES)
+        for (uint32_t b = wlo + threadIdx.x; b < whi; b += 64 * SR_WAVES)
             out[(size_t)a * n + b] = (TOut)((double)acc[b - b0] * inv);
     }
 }
@@ -1717,7 +1849,8 @@ hipError_t launch_pairs_t(const PairArgs& a, hipStream_t st) {
     dim3 grid(tiles, tiles);
     constexpr int NT = (TT / RI) * (TT / RJ);
     hipLaunchKernelGGL((k_pairs<OP, T, RI, RJ, TT>), grid, dim3(NT), 0, st, (const T*)a.U, (const T*)a.Cn, a.rows, a.ldn,
-                       (const T*)a.len, a.K, a.n, a.S, a.totals, a.alpha, (int)a.normalized, a.out, (int)a.out_fp64);
+                       (const T*)a.len, a.K, a.n, a.S, a.totals, a.alpha, (int)a.normalized, a.out, (int)a.out_fp64,
+                       a.band);
     return hipGetLastError();
 }
 
@@ -1742,7 +1875,103 @@ __global__ void __launch_bounds__(256) k_fill(double* __restrict__ p, size_t n, 
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = v;
 }
 
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = v;
+}
+
+// ------------------------------------------------------------------ near-duplicate refinement
+
+// d (row i, column j) is below the refinement bound: tau, or tau * (S_i + S_j)
+// when the distance is not normalised (its denominator is S_i + S_j).  NaN
+// (an empty sample) is never below it.
+__device__ __forceinline__ bool close_pair(float d, const double* __restrict__ S, uint32_t i, uint32_t j, float tau) {
+    return S ? (double)d < (double)tau * (S[i] + S[j]) : d < tau;
+}
+
+// One workgroup per row i walks out[i][j], j > i, of the selected pairs:
+// flags[0] += pairs below the bound, flags[1 + s] = 1 for both samples of each.
+__global__ void __launch_bounds__(256) k_flag_close(uint32_t* __restrict__ flags, const float* __restrict__ out,
+                                                    uint32_t n, const double* __restrict__ S, float tau,
+                                                    GapBand band) {
+    __shared__ uint32_t part[4];
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        uint32_t c = 0;
+        const float* row = out + (size_t)i * n;
+        for (uint32_t j = i + 1 + threadIdx.x; j < n; j += 256)
+            if (band.has(j - i) && close_pair(row[j], S, i, j, tau)) {
+                ++c;
+                flags[1 + j] = 1u;
+            }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t t = part[0] + part[1] + part[2] + part[3];
+            if (t) {
+                atomicAdd(flags, t);
+                flags[1 + i] = 1u;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Sub-table samples a < b (original samples idx[a] < idx[b]) whose FP32
+// distance is below the bound take the FP64 distance sub[a][b] of the
+// sub-table's matrix (ns x ns), rounded once to FP32, in both triangles; with
+// cross, only the pairs a < na <= b.
+__global__ void __launch_bounds__(256) k_refine_scatter(float* __restrict__ out, uint32_t n,
+                                                        const double* __restrict__ sub, uint32_t ns,
+                                                        const uint32_t* __restrict__ idx, uint32_t na, int cross,
+                                                        const double* __restrict__ S, float tau) {
+    const uint32_t a = blockIdx.y;
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= ns || b <= a || (cross && (a >= na || b < na))) return;
+    const uint32_t i = idx[a], j = idx[b];
+    if (!close_pair(out[(size_t)i * n + j], S, i, j, tau)) return;
+    const float v = (float)sub[(size_t)a * ns + b];
+    out[(size_t)i * n + j] = v;
+    out[(size_t)j * n + i] = v;
+}
+
 // ------------------------------------------------------------------ launchers
+
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<size_t>(grid1(n, 256), 65536)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_flag_close(uint32_t* flags, const float* out, uint32_t n, const double* S, float tau,
+                             const GapBand& band, hipStream_t st) {
+    const hipError_t e = launch_fill_u32(flags, (size_t)n + 1, 0u, st);
+    if (e != hipSuccess || n < 2) return e;
+    hipLaunchKernelGGL(k_flag_close, dim3(std::min<uint32_t>(n, 65535)), dim3(256), 0, st, flags, out, n, S, tau, band);
+    return hipGetLastError();
+}
+
+GapBand stripe_band(uint32_t n, uint32_t s0, uint32_t s1) {
+    GapBand b;
+    if (s1 <= s0 || n < 2) {   // nothing selected
+        b.a0 = 1; b.a1 = 0; b.b0 = 1; b.b1 = 0;
+        return b;
+    }
+    b.a0 = s0 + 1;
+    b.a1 = s1;
+    b.b0 = n > s1 ? n - s1 : 1;
+    b.b1 = n >= s0 + 1 ? n - 1 - s0 : 0;
+    return b;
+}
+
+hipError_t launch_refine_scatter(float* out, uint32_t n, const double* sub, uint32_t ns, const uint32_t* idx,
+                                 uint32_t na, bool cross, const double* S, float tau, hipStream_t st) {
+    if (ns < 2) return hipSuccess;
+    if (ns > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_refine_scatter, dim3(grid1(ns, 256), ns), dim3(256), 0, st, out, n, sub, ns, idx, na,
+                       (int)cross, S, tau);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st) {
     if (!n) return hipSuccess;
@@ -1768,6 +1997,25 @@ hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint3
     const uint32_t gy = std::min<uint32_t>(n_nodes, 8192);
     hipLaunchKernelGGL(k_embed_level, dim3(cb, gy), dim3(EMB_THREADS), 0, st, E, ldn, nodes, n_nodes, child_ptr,
                        child_idx);
+    return hipGetLastError();
+}
+
+hipError_t launch_embed_paths(double* E, size_t ldn, const HeavyPaths& hp, uint32_t p0, uint32_t p1, hipStream_t st) {
+    if (p1 <= p0) return hipSuccess;
+    if (ldn % 2) return hipErrorInvalidValue;   // two samples per thread
+    const uint32_t cb = grid1(ldn / 2, EMB_THREADS);
+    const uint32_t gy = std::min<uint32_t>(p1 - p0, 8192);
+    hipLaunchKernelGGL(k_embed_paths, dim3(cb, gy), dim3(EMB_THREADS), 0, st, E, ldn, hp.nodes, hp.light, hp.off + p0,
+                       p1 - p0, hp.heavy, hp.child_ptr, hp.child_idx);
+    return hipGetLastError();
+}
+
+hipError_t launch_bits_paths(uint32_t* B, uint32_t ldb, const HeavyPaths& hp, uint32_t p0, uint32_t p1,
+                             hipStream_t st) {
+    if (p1 <= p0) return hipSuccess;
+    const uint32_t gy = std::min<uint32_t>(p1 - p0, 65535);
+    hipLaunchKernelGGL(k_bits_paths, dim3(grid1(ldb, 256), gy), dim3(256), 0, st, B, ldb, hp.nodes, hp.light,
+                       hp.off + p0, p1 - p0, hp.heavy, hp.child_ptr, hp.child_idx);
     return hipGetLastError();
 }
 
@@ -1815,6 +2063,8 @@ hipError_t launch_presence_T_bits(int8_t* Pt, size_t ldp, const uint32_t* B, uin
     return hipGetLastError();
 }
 
+uint32_t fixed_rows_per_segment(uint32_t K) { return std::max<uint32_t>(64, (K / 64 + 31) / 32 * 32); }
+
 uint32_t gather_segments(uint32_t K, size_t ldn) {
     if (!K) return 0;
     const uint32_t cb = grid1(ldn, EMB_THREADS);
@@ -1834,8 +2084,9 @@ static void gather_sums_t(dim3 grid, hipStream_t st, float* U, size_t ldu, uint3
 
 hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, uint32_t* masks, uint32_t* smask,
                               const double* E, size_t lde, uint32_t col0, const int32_t* knodes, const double* klen,
-                              uint32_t K, uint32_t n, hipStream_t st, const uint32_t* skip_u) {
+                              uint32_t K, uint32_t n, hipStream_t st, const uint32_t* skip_u, uint32_t rps_fixed) {
     if (masks && (lde % 64 || col0 % 64)) return hipErrorInvalidValue;
+    if (rps_fixed % 32) return hipErrorInvalidValue;
     if (smask && !masks) return hipErrorInvalidValue;
     if (col0 >= n) return hipSuccess;
     const uint32_t cnt = (uint32_t)std::min<size_t>(lde, n - col0);   // real samples in this block
@@ -1847,7 +2098,8 @@ hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, ui
     const size_t ldn = lde;
     const uint32_t cb = grid1(ldn, EMB_THREADS);
     const uint32_t segs0 = gather_segments(K, ldn);
-    const uint32_t rps = (((K + segs0 - 1) / segs0) + 31) / 32 * 32;   // mask words never straddle segments
+    const uint32_t rps = rps_fixed ? rps_fixed
+                                   : (((K + segs0 - 1) / segs0) + 31) / 32 * 32;   // mask words never straddle segments
     const uint32_t segs = (K + rps - 1) / rps;
     const uint32_t Kw = (K + 31) / 32;
     const dim3 grid(cb, segs);
@@ -1886,7 +2138,8 @@ hipError_t launch_gram_i8(double* G, const int8_t* Pt, size_t ldp, const int8_t*
                           uint32_t n_chunks, uint32_t n_pad, const uint32_t* order, uint32_t n_tiles, hipStream_t st) {
     if (n_pad % GRAM_BM || ldp % GRAM_BK) return hipErrorInvalidValue;
     const uint32_t tiles = n_pad / GRAM_BM;
-    if (tiles > 65535 || n_tiles != tiles * (tiles + 1) / 2 || !order) return hipErrorInvalidValue;
+    if (tiles > 65535 || n_tiles > tiles * (tiles + 1) / 2 || !order) return hipErrorInvalidValue;
+    if (!n_tiles) return hipSuccess;
     const uint32_t per_xcd = (n_tiles + N_XCDS - 1) / N_XCDS;
     hipLaunchKernelGGL(k_gram_i8, dim3(per_xcd * N_XCDS), dim3(GRAM_THREADS), 0, st, G, tiles, Pt, ldp, digits, chunks,
                        n_chunks, (uint32_t)(ldp / GRAM_BK), order, n_tiles, per_xcd);
@@ -1985,16 +2238,16 @@ hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
     }
 #undef SSU_POSTINGS
     // row lists: counts per (sample, node chunk), their exclusive scan, then the entries
-    const uint32_t nkc = row_list_chunks(a.K);
+    const uint32_t nkc = row_list_chunks(a.n, a.K), kcw = row_list_width(a.n, a.K);
     const size_t m = (size_t)a.n * nkc;
     const uint32_t nsb = (uint32_t)((m + SCAN_BLOCK - 1) / SCAN_BLOCK);
     hipLaunchKernelGGL(k_row_lists, dim3(a.npan, nkc), dim3(64), 0, st, nullptr, a.rcount, nullptr, a.smask, a.dmask,
-                       a.pc, a.K, a.n, nkc, (const uint32_t*)a.poff, a.slots);
+                       a.pc, a.K, a.n, nkc, kcw, (const uint32_t*)a.poff, a.slots);
     hipLaunchKernelGGL(k_scan_sums, dim3(nsb), dim3(SCAN_T), 0, st, a.rbsum, a.rcount, m);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, a.rbsum, nsb);
     hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(SCAN_T), 0, st, a.roff, a.rcount, a.rbsum, m, nsb);
     hipLaunchKernelGGL(k_row_lists, dim3(a.npan, nkc), dim3(64), 0, st, a.rows, nullptr, a.roff, a.smask, a.dmask,
-                       a.pc, a.K, a.n, nkc, (const uint32_t*)a.poff, a.slots);
+                       a.pc, a.K, a.n, nkc, kcw, (const uint32_t*)a.poff, a.slots);
     return hipGetLastError();
 }
 
@@ -2005,13 +2258,13 @@ static void pairs_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs&
                        GRAM ? nullptr : (TV*)a.out, a.walked, a.rows, a.roff, a.pc, (const IX*)a.pidx,
                        GRAM ? nullptr : (const TV*)a.pval, a.len, GRAM ? nullptr : a.S, a.K, a.n, a.npan,
                        (const uint32_t*)a.poff, a.slots, nchunks, nkc, a.gram, a.n_pad / GRAM_BM, a.ssp,
-                       (TV*)a.out2, a.pspmax, a.alpha);
+                       (TV*)a.out2, a.pspmax, a.alpha, a.band);
 }
 
 hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st) {
     if (!a.K || !a.n) return hipSuccess;
     if (!sparse_args_ok(a)) return hipErrorInvalidValue;
-    const uint32_t nkc = row_list_chunks(a.K);
+    const uint32_t nkc = row_list_chunks(a.n, a.K);
     const uint32_t cw = a.gen ? SR_CW / 2 : SR_CW;
     const uint32_t nchunks = (a.n + cw - 1) / cw;
     const uint64_t blocks = (uint64_t)a.n * nchunks;
@@ -2059,10 +2312,19 @@ uint32_t sparse_rows_launched(uint32_t n, bool gen) {
     const uint32_t cw = gen ? SR_CW / 2 : SR_CW;
     return n * ((n + cw - 1) / cw);
 }
-uint32_t row_list_chunks(uint32_t K) { return std::max<uint32_t>(1, (K + RL_KC - 1) / RL_KC); }
+uint32_t row_list_width(uint32_t n, uint32_t K) {
+    uint64_t w = RL_KC;
+    auto chunks = [&](uint64_t width) { return std::max<uint64_t>(1, (K + width - 1) / width); };
+    while (chunks(w) > RL_MAX_CHUNKS || (uint64_t)std::max<uint32_t>(n, 1) * chunks(w) > RL_MAX_ENTRIES) w *= 2;
+    return (uint32_t)std::min<uint64_t>(w, 1ull << 31);
+}
+uint32_t row_list_chunks(uint32_t n, uint32_t K) {
+    const uint64_t w = row_list_width(n, K);
+    return (uint32_t)std::max<uint64_t>(1, (K + w - 1) / w);
+}
 size_t offset_scan_blocks(uint32_t K) { return ((size_t)K + SCAN_BLOCK - 1) / SCAN_BLOCK; }
 size_t row_list_scan_blocks(uint32_t n, uint32_t K) {
-    return ((size_t)n * row_list_chunks(K) + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    return ((size_t)n * row_list_chunks(n, K) + SCAN_BLOCK - 1) / SCAN_BLOCK;
 }
 
 hipError_t launch_pairs_gen32(const SparseArgs& a, float alpha, void* out2, hipStream_t st) {

@@ -20,12 +20,52 @@ struct GramChunk {
     double scale;      // 2^e of digit 0 of this group
 };
 
+// Sample pairs (i, j), i < j, selected by their gap g = j - i: a0 <= g <= a1
+// or b0 <= g <= b1.  The stripes [s0, s1) of an n-sample matrix are such a
+// set (stripe s holds the pairs of gaps s + 1 and n - s - 1,
+// src/unifrac_task_impl.hpp:367); the default selects every pair.
+struct GapBand {
+    uint32_t a0 = 1, a1 = 0xffffffffu, b0 = 1, b1 = 0;
+    __host__ __device__ bool has(uint32_t g) const { return (g >= a0 && g <= a1) || (g >= b0 && g <= b1); }
+    // some gap in [lo, hi] is selected
+    __host__ __device__ bool meets(uint32_t lo, uint32_t hi) const {
+        return lo <= hi && ((lo <= a1 && hi >= a0) || (lo <= b1 && hi >= b0));
+    }
+    // the tile rows [r0, r1) x columns [c0, c1) holds a selected pair i < j
+    __host__ __device__ bool meets_tile(uint32_t r0, uint32_t r1, uint32_t c0, uint32_t c1) const {
+        if (c1 == 0 || r1 <= r0 || c1 <= c0 || c1 - 1 <= r0) return false;
+        const uint32_t lo = c0 + 1 > r1 ? c0 + 1 - r1 : 1u;
+        return meets(lo > 0 ? lo : 1u, c1 - 1 - r0);
+    }
+    bool all(uint32_t n) const { return a0 <= 1 && (uint64_t)a1 + 1 >= n; }
+};
+// the pairs of stripes [s0, s1) of an n-sample matrix (s1 <= (n + 1) / 2)
+GapBand stripe_band(uint32_t n, uint32_t s0, uint32_t s1);
+
 constexpr int GRAM_BM = 128;   // output tile rows per workgroup
 constexpr int GRAM_BK = 128;   // K' columns per LDS stage
 constexpr int PAIR_TILE = 128; // samples per side of a VALU pair tile
 
 // p[0..n) = v (a kernel, so that it can sit in a captured graph)
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st);
+
+// ---------------------------------------------------------------- near-duplicate refinement
+// FP32 weighted / generalized distances carry the FP32 rounding of the
+// proportions: |u - v| computed from rounded values is off by at most
+// 2^-24 (u + v), so d is off by at most 2^-24 of its denominator, a relative
+// error of up to 2^-24 / d.  Selected pairs (i < j in band) with d below tau --
+// below tau * (S_i + S_j) when S is given (unnormalised weighted) -- are
+// marked: flags[0] = their number, flags[1 + s] = 1 for every sample s of
+// one (flags holds n + 1 words and is cleared here first).
+hipError_t launch_flag_close(uint32_t* flags, const float* out, uint32_t n, const double* S, float tau,
+                             const GapBand& band, hipStream_t st);
+// Pairs among the sub-table samples idx[0..ns) that launch_flag_close marked
+// (the same test on the same FP32 value and S) take the FP64 distance of the
+// sub-table's ns x ns matrix, in both triangles; with cross, only pairs
+// a < na <= b.
+hipError_t launch_refine_scatter(float* out, uint32_t n, const double* sub, uint32_t ns, const uint32_t* idx,
+                                 uint32_t na, bool cross, const double* S, float tau, hipStream_t st);
 
 // ---------------------------------------------------------------- embedding
 // E holds the sample columns [col0, col0 + ld) of the node-major embedding
@@ -34,6 +74,24 @@ hipError_t launch_embed_leaves(double* E, size_t ld, uint32_t col0, const int32_
                                const double* totals, bool normalize, hipStream_t st);
 hipError_t launch_embed_level(double* E, size_t ldn, const int32_t* nodes, uint32_t n_nodes, const int32_t* child_ptr,
                               const int32_t* child_idx, hipStream_t st);
+
+// Internal nodes along heavy paths (each node continues the path of its
+// largest child; the paths of a round depend only on earlier rounds): path p
+// holds nodes[off[p] .. off[p + 1]), bottom to top, light[q] the node's other
+// child when it has two (-1 otherwise), heavy[k] every internal node's heavy
+// child.  The launches run the paths [p0, p1) of one round; the FP64 sums are
+// those of launch_embed_level (children left to right from 0).
+struct HeavyPaths {
+    const int32_t* nodes;
+    const int32_t* light;
+    const uint32_t* off;
+    const int32_t* heavy;
+    const int32_t* child_ptr;
+    const int32_t* child_idx;
+};
+hipError_t launch_embed_paths(double* E, size_t ldn, const HeavyPaths& hp, uint32_t p0, uint32_t p1, hipStream_t st);
+hipError_t launch_bits_paths(uint32_t* B, uint32_t ldb, const HeavyPaths& hp, uint32_t p0, uint32_t p1,
+                             hipStream_t st);
 
 // Presence bits (node-major, ldb words of 32 samples per row; every sample of
 // the table in one block): bit s%32 of B[k][s/32] = sample s holds node k.
@@ -64,10 +122,14 @@ hipError_t launch_bits_smask(uint32_t* smask, const uint32_t* B, uint32_t ldb, c
 //   skip_u (optional, [ceil(K/32)] words): rows r with bit r set are not written to U (S and
 //     the masks still see their FP32-rounded values).
 uint32_t gather_segments(uint32_t K, size_t ldn);
+//   rps_fixed (optional, a multiple of 32): rows per segment, in place of the
+//     default that depends on lde (S then depends on a sample's column alone).
 hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, uint32_t* masks, uint32_t* smask,
                               const double* E,
                               size_t lde, uint32_t col0, const int32_t* knodes, const double* klen, uint32_t K,
-                              uint32_t n, hipStream_t st, const uint32_t* skip_u = nullptr);
+                              uint32_t n, hipStream_t st, const uint32_t* skip_u = nullptr, uint32_t rps_fixed = 0);
+// rows per segment that depends on K alone (for launch_gather_sums)
+uint32_t fixed_rows_per_segment(uint32_t K);
 // Faith PD: out[s] = sum over all non-root nodes k in postorder of (E[k][s] > 0) * len[k]
 hipError_t launch_faith(double* out, const double* E, size_t ldn, const double* len, uint32_t m, uint32_t n,
                         hipStream_t st);
@@ -81,7 +143,8 @@ hipError_t launch_presence_T(int8_t* Pt, size_t ldp, const double* E, size_t lde
 hipError_t launch_presence_T_bits(int8_t* Pt, size_t ldp, const uint32_t* B, uint32_t ldb, const int32_t* col_node,
                                   uint32_t Kp, uint32_t n_pad, uint32_t n, hipStream_t st);
 // G = sum_c len_c * P[i][c] * P[j][c] for the upper-triangle 128x128 tiles, FP64,
-// stored tile by tile in MFMA fragment order (n_pad/128)^2 * 128^2 doubles
+// stored tile by tile in MFMA fragment order (n_pad/128)^2 * 128^2 doubles;
+// order lists the tiles computed (every tile, or a subset)
 hipError_t launch_gram_i8(double* G, const int8_t* Pt, size_t ldp, const int8_t* digits, const GramChunk* chunks,
                           uint32_t n_chunks, uint32_t n_pad, const uint32_t* order, uint32_t n_tiles, hipStream_t st);
 // out[i][j] from G and T = diag(G): (T_i+T_j-2G)/(T_i+T_j-G) or T_i+T_j-2G
@@ -113,6 +176,7 @@ struct PairArgs {
     bool normalized;        // divide by the accumulated / S-based total
     void* out;              // n x n output
     bool out_fp64;
+    GapBand band;           // tiles without a selected pair are skipped
 };
 hipError_t launch_pairs(PairOp op, bool fp64, const PairArgs& a, hipStream_t st);
 
@@ -174,8 +238,8 @@ struct SparseNodesArgs {
     bool out_fp64;
     uint32_t* walked;       // optional [sparse_rows_blocks(n)]: postings walked per wave
     unsigned long long* rows;   // [slots] row-list entries (node << 32 | position in its postings)
-    uint32_t* rcount;       // [n * row_list_chunks(K)]
-    uint32_t* roff;         // [n * row_list_chunks(K) + 1]
+    uint32_t* rcount;       // [n * row_list_chunks(n, K)]
+    uint32_t* roff;         // [n * row_list_chunks(n, K) + 1]
     uint32_t* rbsum;        // [row_list_scan_blocks(n, K) + 1]
     uint32_t K, n, npan;
     double* ssp = nullptr;  // (weighted, FP32 values) [n]: per-sample sum of l * u (GEN: l * u^alpha) over its sparse nodes
@@ -185,6 +249,7 @@ struct SparseNodesArgs {
     double* pspmax = nullptr;   // (GEN) [1] scratch: max of ssp
     double* gram = nullptr; // unweighted: the Gram tiles of launch_gram_i8
     uint32_t n_pad = 0;     // their padded edge
+    GapBand band;           // rows walk (and write) only the columns of selected pairs (G(a, a) always)
 };
 // posting lists, per-panel prefix counts and row lists
 hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st);
@@ -200,7 +265,9 @@ uint32_t sparse_rows_blocks(uint32_t n);
 // workgroups launch_sparse_rows launches (the entries of walked it writes)
 uint32_t sparse_rows_launched(uint32_t n, bool gen);
 size_t offset_scan_blocks(uint32_t K);
-uint32_t row_list_chunks(uint32_t K);
+// row lists are built per (sample, chunk of row_list_width(n, K) nodes)
+uint32_t row_list_width(uint32_t n, uint32_t K);
+uint32_t row_list_chunks(uint32_t n, uint32_t K);
 size_t row_list_scan_blocks(uint32_t n, uint32_t K);
 // The upper-triangle blocks (bi <= bj < nb) as bi << 16 | bj, super x super
 // super-tile by super-tile; kernels walk it in XCD-contiguous runs.

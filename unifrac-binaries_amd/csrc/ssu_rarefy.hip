@@ -156,12 +156,42 @@ struct RarefyImpl {
     void *d_col_ptr = nullptr, *d_col_pos = nullptr, *d_col_cum = nullptr, *d_rdata = nullptr, *d_depth = nullptr,
          *d_set = nullptr;
     std::vector<void*> d_out;   // per method: reps consecutive n x n matrices
+    // per (method, replicate): n + 1 words of near-duplicate marks (methods whose plan refines)
+    uint32_t* d_flags = nullptr;
+    double* d_sums = nullptr;   // per (method, replicate): the n per-sample sums S the marks were tested against
+    uint64_t refined = 0;
+
+    uint32_t* flags_of(size_t k, uint32_t r) const {
+        return d_flags + (k * reps + r) * ((size_t)table.n_samples + 1);
+    }
+    double* sums_of(size_t k, uint32_t r) const { return d_sums + (k * reps + r) * (size_t)table.n_samples; }
+    // after the replicates have run: FP64 recomputation of the marked pairs
+    bool refine(std::string* err) {
+        refined = 0;
+        bool any = false;
+        for (const auto& p : plans) any = any || p->refines();
+        if (!any) return true;
+        const size_t stride = (size_t)table.n_samples + 1;
+        std::vector<uint32_t> counts(plans.size() * reps);
+        RF_HIP(hipMemcpy2D(counts.data(), sizeof(uint32_t), d_flags, stride * sizeof(uint32_t), sizeof(uint32_t),
+                           counts.size(), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < plans.size(); ++k)
+            for (uint32_t r = 0; r < reps; ++r) {
+                if (!plans[k]->refines() || !counts[k * reps + r]) continue;
+                char* out = (char*)d_out[k] + out_bytes(k) * r;
+                if (!plans[k]->refine((const double*)d_rdata + (size_t)r * nnz, (const double*)d_depth, out,
+                                      flags_of(k, r), sums_of(k, r), err))
+                    return false;
+                refined += plans[k]->refined_pairs();
+            }
+        return true;
+    }
 
     ~RarefyImpl() {
         plans.clear();
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (void* p : {d_col_ptr, d_col_pos, d_col_cum, d_rdata, d_depth, d_set})
+        for (void* p : {d_col_ptr, d_col_pos, d_col_cum, d_rdata, d_depth, d_set, (void*)d_flags, (void*)d_sums})
             if (p) (void)hipFree(p);
         for (void* p : d_out)
             if (p) (void)hipFree(p);
@@ -190,9 +220,12 @@ struct RarefyImpl {
         for (uint32_t r = 0; r < reps; ++r)
             for (size_t k = 0; k < plans.size(); ++k) {
                 char* out = (char*)d_out[k] + out_bytes(k) * r;
+                // non-quiet (the eager pass) refines at once; captured work leaves it to refine()
+                const bool rf = plans[k]->refines();
                 if (!plans[k]->execute_into((const double*)d_rdata + (size_t)r * nnz, (const double*)d_depth, out,
-                                            quiet, err))
+                                            quiet, err, rf ? flags_of(k, r) : nullptr, rf ? sums_of(k, r) : nullptr))
                     return false;
+                if (!quiet) refined += plans[k]->refined_pairs();
             }
         return true;
     }
@@ -293,6 +326,9 @@ std::unique_ptr<RarefyBatch> RarefyBatch::create(const Table& table, const Phylo
         !dmalloc(&R.d_depth, n * 8))
         return nullptr;
     if (!with_replacement && !dmalloc(&R.d_set, (size_t)n * reps * R.hsize * 8)) return nullptr;
+    if (!specs.empty() && (!dmalloc((void**)&R.d_flags, specs.size() * reps * ((size_t)n + 1) * sizeof(uint32_t)) ||
+                           !dmalloc((void**)&R.d_sums, specs.size() * reps * (size_t)n * sizeof(double))))
+        return nullptr;
     for (size_t k = 0; k < specs.size(); ++k) {
         void* p = nullptr;
         if (!dmalloc(&p, R.out_bytes(k) * reps)) return nullptr;
@@ -312,6 +348,7 @@ bool RarefyBatch::execute(std::string* err) {
     RarefyImpl& R = *p_;
     if (!R.captured) {
         // eager pass: allocates every buffer the plans need, then the same work is captured
+        R.refined = 0;
         RF_HIP(hipEventRecord(R.e0, R.st));
         if (!R.enqueue(false, err)) return false;
         RF_HIP(hipEventRecord(R.e1, R.st));
@@ -338,7 +375,7 @@ bool RarefyBatch::execute(std::string* err) {
     RF_HIP(hipStreamSynchronize(R.st));
     RF_HIP(hipEventElapsedTime(&R.ms, R.e0, R.e1));
     R.used_graph = true;
-    return true;
+    return R.refine(err);
 }
 
 uint32_t RarefyBatch::n_samples() const { return p_->table.n_samples; }
@@ -351,12 +388,12 @@ const std::vector<uint32_t>& RarefyBatch::obs_index() const { return p_->obs_ind
 bool RarefyBatch::out_fp64(size_t k) const { return k < p_->specs.size() && p_->specs[k].fp64; }
 float RarefyBatch::last_ms() const { return p_->ms; }
 bool RarefyBatch::graph_used() const { return p_->used_graph; }
+uint64_t RarefyBatch::refined_pairs() const { return p_->refined; }
 
 bool RarefyBatch::copy_matrix(size_t k, uint32_t rep, void* host, std::string* err) const {
     const RarefyImpl& R = *p_;
     if (k >= R.specs.size() || rep >= R.reps) { if (err) *err = "method or replicate out of range"; return false; }
-    RF_HIP(hipMemcpy(host, (const char*)R.d_out[k] + R.out_bytes(k) * rep, R.out_bytes(k), hipMemcpyDeviceToHost));
-    return true;
+    return copy_to_host(host, (const char*)R.d_out[k] + R.out_bytes(k) * rep, R.out_bytes(k), R.st, err);
 }
 
 bool RarefyBatch::copy_counts(uint32_t rep, double* host, std::string* err) const {

@@ -40,6 +40,7 @@ public:
     float last_ms() const;                // device time of the last execute()
     bool graph_used() const;              // the last execute() replayed the captured graph
     uint64_t pair_nodes() const;          // sum over replicates and methods of n(n-1)/2 * processed nodes
+    uint64_t refined_pairs() const;       // pairs the last execute() recomputed in FP64 (all replicates)
 
 private:
     RarefyBatch();

@@ -84,13 +84,15 @@ EXPORTS = [
     "unifrac_multi_to_file_v3", "unifrac_multi_to_file_v2", "compute_permanova_fp64", "compute_permanova_fp32",
     "ssu_center_matrix_fp64", "ssu_center_matrix_fp32", "ssu_pcoa_fp64", "ssu_pcoa_fp32", "ssu_permanova_fp64",
     "ssu_permanova_fp32",
-    "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
+    "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_execute_stripes", "ssu_plan_copy_stripes",
+    "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
     "ssu_plan_copy_rows",
-    "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_sparse_terms", "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
+    "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_sparse_terms", "ssu_plan_refined_pairs", "ssu_last_call_timings",
+    "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
     "ssu_newick_to_bp", "ssu_biom_n_samples", "ssu_device_available", "ssu_last_error",
     "ssu_rarefy_create_inmem", "ssu_rarefy_execute", "ssu_rarefy_n_samples", "ssu_rarefy_n_obs", "ssu_rarefy_nnz",
     "ssu_rarefy_kept", "ssu_rarefy_copy_counts", "ssu_rarefy_copy_matrix", "ssu_rarefy_last_ms",
-    "ssu_rarefy_graph_used", "ssu_rarefy_pair_nodes", "ssu_rarefy_destroy",
+    "ssu_rarefy_graph_used", "ssu_rarefy_pair_nodes", "ssu_rarefy_refined_pairs", "ssu_rarefy_destroy",
 ]
 
 _lib = None
@@ -112,6 +114,7 @@ def lib():
         L.ssu_plan_pair_nodes.restype = C.c_uint64
         L.ssu_plan_kernel_terms.restype = C.c_uint64
         L.ssu_plan_sparse_terms.restype = C.c_uint64
+        L.ssu_plan_refined_pairs.restype = C.c_uint64
         L.ssu_plan_main_kernel.restype = C.c_char_p
         L.ssu_last_error.restype = C.c_char_p
         L.ssu_postorder.restype = C.c_int
@@ -140,6 +143,7 @@ def lib():
         L.ssu_rarefy_n_obs.restype = C.c_uint
         L.ssu_rarefy_nnz.restype = C.c_uint64
         L.ssu_rarefy_pair_nodes.restype = C.c_uint64
+        L.ssu_rarefy_refined_pairs.restype = C.c_uint64
         L.ssu_rarefy_last_ms.restype = C.c_double
         _lib = L
     return _lib
@@ -218,6 +222,14 @@ def matrix_inmem(inp: Inputs, method: str, variance_adjust=False, alpha=1.0, byp
     return st, m
 
 
+def last_call_timings():
+    """Wall-clock ms of this thread's last one_off_matrix_inmem* call, by phase."""
+    ms = (C.c_double * 6)()
+    k = lib().ssu_last_call_timings(ms, 6)
+    keys = ["input", "tree", "plan", "execute", "copy", "total"]
+    return {keys[i]: ms[i] for i in range(k)}
+
+
 def faith_pd_inmem(inp: Inputs):
     L = lib()
     res = C.POINTER(RVec)()
@@ -251,6 +263,20 @@ class Plan:
         if st != OKAY:
             raise RuntimeError(f"ssu_plan_execute failed: {last_error()}")
 
+    def execute_stripes(self, start, stop):
+        """compute only the pairs of stripes [start, stop)"""
+        st = lib().ssu_plan_execute_stripes(self.h, C.c_uint(start), C.c_uint(stop))
+        if st != OKAY:
+            raise RuntimeError(f"ssu_plan_execute_stripes failed: {last_error()}")
+
+    def stripes(self, start, stop):
+        """stripes [start, stop) of the result, (stop - start) x n doubles"""
+        out = np.zeros((stop - start, self.n))
+        st = lib().ssu_plan_copy_stripes(self.h, C.c_uint(start), C.c_uint(stop), out.ctypes.data_as(C.c_void_p))
+        if st != OKAY:
+            raise RuntimeError(last_error())
+        return out
+
     def timings(self):
         ms = (C.c_double * 8)()
         k = lib().ssu_plan_timings(self.h, ms, 8)
@@ -265,6 +291,9 @@ class Plan:
 
     def sparse_terms(self) -> int:
         return int(lib().ssu_plan_sparse_terms(self.h))
+
+    def refined_pairs(self) -> int:
+        return int(lib().ssu_plan_refined_pairs(self.h))
 
     def main_kernel(self) -> str:
         return lib().ssu_plan_main_kernel(self.h).decode()
@@ -401,6 +430,9 @@ class Rarefaction:
 
     def pair_nodes(self) -> int:
         return int(lib().ssu_rarefy_pair_nodes(self.h))
+
+    def refined_pairs(self) -> int:
+        return int(lib().ssu_rarefy_refined_pairs(self.h))
 
     def close(self):
         if self.h:

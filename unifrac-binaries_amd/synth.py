@@ -8,9 +8,13 @@ tips named t<i>.
 Table: each sample draws its nonzero tips without replacement, uniformly or
 with Zipf(s) tip popularity over a random rank permutation (Efraimidis-
 Spirakis keys, implemented as Gumbel top-k); counts lognormal or negative
-binomial.  Only observed tips become table rows.
+binomial.  Only observed tips become table rows.  Tables with more than 2^32
+key draws (65,536 samples x 200,000 tips) draw them from one generator per
+128-sample batch, spawned from the seed, so that they generate in parallel.
 """
 from __future__ import annotations
+
+import os
 
 import numpy as np
 
@@ -66,6 +70,52 @@ def random_tree(n_tips: int, seed: int, mean_len: float = 0.1):
     return structure, lengths, names
 
 
+def caterpillar_tree(n_tips: int, seed: int, mean_len: float = 0.1, shuffle: bool = True):
+    """A caterpillar (maximally deep) binary tree: every internal node joins one
+    tip and the subtree of the remaining tips, n_tips - 1 levels deep; with
+    shuffle the tip sits left or right at random.  Same arrays as random_tree."""
+    rng = np.random.default_rng(seed)
+    n_nodes = 2 * n_tips - 1
+    node_len = rng.exponential(mean_len, n_nodes).astype(np.float32).astype(np.float64)
+    tip_left = rng.random(n_tips) < 0.5 if shuffle else np.ones(n_tips, bool)
+    structure, lengths, names = [], [], []
+    # internal node i (0 = root) has tip i and internal node i + 1 (the last one: tips n-2, n-1)
+    closes = []
+    ln = 0
+    for i in range(n_tips - 1):
+        structure.append(True)
+        lengths.append(0.0 if i == 0 else node_len[n_tips + i])
+        names.append("")
+        tip = [(True, node_len[i], f"t{i}"), (False, 0.0, "")]
+        if i == n_tips - 2:
+            last = [(True, node_len[i + 1], f"t{i + 1}"), (False, 0.0, "")]
+            pair = tip + last if tip_left[i] else last + tip
+            for st_, le_, nm_ in pair:
+                structure.append(st_)
+                lengths.append(le_)
+                names.append(nm_)
+        elif tip_left[i]:
+            for st_, le_, nm_ in tip:
+                structure.append(st_)
+                lengths.append(le_)
+                names.append(nm_)
+        else:
+            closes.append(tip)
+            continue
+        closes.append(None)
+    # close the internal nodes from the deepest up, a right-hand tip before its parent's ')'
+    for tip in reversed(closes):
+        if tip is not None:
+            for st_, le_, nm_ in tip:
+                structure.append(st_)
+                lengths.append(le_)
+                names.append(nm_)
+        structure.append(False)
+        lengths.append(0.0)
+        names.append("")
+    return np.array(structure, bool), np.array(lengths), names
+
+
 def bp_to_newick(structure, lengths, names) -> str:
     """Newick text for parentheses arrays, lengths printed with %.9g."""
     out = []
@@ -95,6 +145,12 @@ def bp_to_newick(structure, lengths, names) -> str:
     return "".join(out) + ";"
 
 
+# tables needing more key draws than this take them from one generator per
+# sample batch (spawned from the seed), in parallel; smaller ones from the
+# single seeded generator
+PARALLEL_DRAWS = 1 << 32
+
+
 def random_table(n_tips: int, n_samples: int, density: float, seed: int, zipf_s: float | None = None,
                  counts: str = "lognormal", batch: int = 128):
     """Observation-major CSR: (obs_ids, sample_ids, indptr, indices, data)."""
@@ -104,17 +160,46 @@ def random_table(n_tips: int, n_samples: int, density: float, seed: int, zipf_s:
         ranks = rng.permutation(n_tips) + 1
         logw = (-zipf_s * np.log(ranks)).astype(np.float32)
     tips_all = np.empty(n_samples * k, np.int64)
-    for b0 in range(0, n_samples, batch):
-        b1 = min(n_samples, b0 + batch)
-        u = rng.random((b1 - b0, n_tips), dtype=np.float32)
+    inv_w = None if zipf_s is None else np.exp(-logw)[None, :]
+
+    def top_k(b0, u):
+        # the draws come from the one generator in batch order; the keys and the
+        # top-k of each batch are independent of the others (numpy releases the GIL)
+        b1 = b0 + u.shape[0]
         np.maximum(u, np.float32(1e-30), out=u)
-        if zipf_s is None:
+        if inv_w is None:
             keys = u
         else:
             # Efraimidis-Spirakis: top-k of u^(1/w)  <=>  top-k of log(u)/w
-            keys = np.log(u) * np.exp(-logw)[None, :]
-        top = np.argpartition(-keys, k - 1, axis=1)[:, :k]
+            keys = np.log(u)
+            keys *= inv_w
+        np.negative(keys, out=keys)
+        top = np.argpartition(keys, k - 1, axis=1)[:, :k]
         tips_all[b0 * k:b1 * k] = top.reshape(-1)
+
+    from concurrent.futures import ThreadPoolExecutor
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    starts = list(range(0, n_samples, batch))
+    with ThreadPoolExecutor(workers) as pool:
+        if n_samples * n_tips <= PARALLEL_DRAWS:
+            pending = []
+            for b0 in starts:
+                b1 = min(n_samples, b0 + batch)
+                pending.append(pool.submit(top_k, b0, rng.random((b1 - b0, n_tips), dtype=np.float32)))
+                if len(pending) >= 2 * workers:   # bounded memory: a few batches in flight
+                    pending.pop(0).result()
+            for f in pending:
+                f.result()
+        else:
+            # one generator per batch, spawned from the seed: the draws themselves run in parallel
+            subs = np.random.SeedSequence(seed).spawn(len(starts))
+
+            def batch_job(q):
+                b0 = starts[q]
+                g = np.random.default_rng(subs[q])
+                top_k(b0, g.random((min(n_samples, b0 + batch) - b0, n_tips), dtype=np.float32))
+
+            list(pool.map(batch_job, range(len(starts))))
     samples_all = np.repeat(np.arange(n_samples, dtype=np.int64), k)
     if counts == "lognormal":
         vals = np.maximum(1.0, np.ceil(rng.lognormal(3.0, 1.5, tips_all.size)))
