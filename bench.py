@@ -337,6 +337,7 @@ def main():
 
     times = {k: [] for k in specs}
     plans_info = {}
+    thetas = {}
     k_steps = max(1, min(args.steps, 10))
     # kernel-only: device-resident plans, inputs in HBM, matrices left in HBM
     if not args.sequential:
@@ -355,6 +356,7 @@ def main():
         k_elapsed = time.perf_counter() - t_start
         for name, p in plans.items():
             plans_info[name] = (p.main_kernel(), p.pair_nodes(), p.kernel_terms(), p.sparse_terms())
+            thetas[name] = p.split_theta()
             p.close()
     else:
         # one plan resident at a time (workloads whose plans do not fit in HBM together)
@@ -370,8 +372,9 @@ def main():
                 times[name].append(p.timings())
             k_elapsed += time.perf_counter() - t_start
             plans_info[name] = (p.main_kernel(), p.pair_nodes(), p.kernel_terms(), p.sparse_terms())
+            thetas[name] = p.split_theta()
             p.close()
-    log(f"kernel-only: {1000.0 * k_elapsed / k_steps:.1f} ms per step")
+    log(f"kernel-only: {1000.0 * k_elapsed / k_steps:.1f} ms per step; occupancy splits chosen {thetas}")
 
     # host to host: one C-API call per method and step, CSR and tree in host memory -> matrix in host memory
     calls = {label: OneOff(ssu, inp, m, kw) for label, (m, kw) in specs.items()}
@@ -410,6 +413,7 @@ def main():
                          "pair_node_per_s": pn / (avg(name, "total") / 1000.0),
                          "kernel_terms": terms, "sparse_terms": sterms}
         kernels[name]["terms_per_pair_node"] = (terms + sterms) / max(pair_nodes[name], 1)
+        kernels[name]["split_theta"] = thetas.get(name, 0.0)
     # dominant kernel: the largest of the main kernels and the posting-list phases (k_pairs_rows)
     cands = [(k["main_ms"], name, k["kernel"]) for name, k in kernels.items()]
     cands += [(k["sparse_ms"], name, "k_pairs_rows" + ("[gram]" if name.startswith("unweighted") else ""))

@@ -425,11 +425,27 @@ SSU_EXTERN uint64_t ssu_plan_sparse_terms(const ssu_plan_t* plan);
  * small enough for the FP32 rounding of the proportions to reach 1e-6 relative
  * (FP32 weighted and generalized methods; 0 otherwise) */
 SSU_EXTERN uint64_t ssu_plan_refined_pairs(const ssu_plan_t* plan);
+/* the occupancy split the plan chose (SSU_GRAM_SPLIT / SSU_SPARSE_SPLIT, or the
+ * cost model of ssu_choose_split): nodes held by fewer than this fraction of
+ * the samples are summed from posting lists; 0: no split */
+SSU_EXTERN double ssu_plan_split_theta(const ssu_plan_t* plan);
 /* wall-clock milliseconds of the calling thread's last one_off_matrix_inmem*
  * call: inputs, tree (shear / collapse / postorder), plan (uploads, schedules,
  * occupancy splits), execute (device work and refinement), copy (result to the
  * host), total.  Returns the number of values written (at most cap, 6). */
 SSU_EXTERN int ssu_last_call_timings(double* ms, int cap);
+/* Device buffers freed by finished calls stay cached in the process (mapping
+ * and unmapping multi-gigabyte buffers costs more than the kernels); this
+ * returns every cached buffer to the runtime.  SSU_POOL_GIB caps the cache
+ * (default 96 GiB; 0: no cache). */
+SSU_EXTERN void ssu_release_device_memory(void);
+/* The occupancy-split threshold a plan picks (host only): kind 0 FP32 weighted,
+ * 1 FP64 weighted, 2 FP32 generalized, 3 unweighted Gram; occupancy[k] = number
+ * of samples holding node k.  Returns the threshold (nodes held by fewer samples
+ * are summed from posting lists; 0: no split) and the modelled milliseconds
+ * with and without it. */
+SSU_EXTERN unsigned int ssu_choose_split(int kind, unsigned int n_samples, const uint32_t* occupancy,
+                                         unsigned int n_nodes, double* model_ms, double* model_ms_none);
 SSU_EXTERN const char* ssu_plan_main_kernel(const ssu_plan_t* plan);
 SSU_EXTERN void ssu_plan_destroy(ssu_plan_t** plan);
 

@@ -77,29 +77,6 @@ def test_caterpillar_faith_bit_exact(ssu, O, deep, oracle_tree):
     assert np.array_equal(vals, O.faith_pd(*oracle_tree))
 
 
-def contract(tree, frac, seed):
-    """Delete the parentheses of a random fraction of the internal non-root
-    nodes: their children join the parent, which gets three or more children."""
-    st, le, nm = tree
-    st = np.asarray(st, bool)
-    n = st.size
-    partner = np.zeros(n, np.int64)
-    stack = []
-    for i in range(n):
-        if st[i]:
-            stack.append(i)
-        else:
-            o = stack.pop()
-            partner[o], partner[i] = i, o
-    internal = [i for i in range(1, n) if st[i] and i + 1 < n and st[i + 1]]
-    rng = np.random.default_rng(seed)
-    drop = set(rng.choice(internal, int(frac * len(internal)), replace=False).tolist())
-    keep = np.ones(n, bool)
-    for o in drop:
-        keep[o] = keep[partner[o]] = False
-    return st[keep], np.asarray(le)[keep], [x for x, k in zip(nm, keep) if k]
-
-
 def test_paths_match_levels_on_random_trees(ssu, synth, monkeypatch):
     """A bushy random tree, and the same tree with a third of its internal
     nodes contracted (nodes of three or more children, summed left to right):
@@ -107,7 +84,7 @@ def test_paths_match_levels_on_random_trees(ssu, synth, monkeypatch):
     tree = synth.random_tree(3000, 9)
     table = synth.random_table(3000, 1200, 0.03, 10, zipf_s=1.1)
     inp = ssu.Inputs(*tree, *table)
-    minp = ssu.Inputs(*contract(tree, 0.33, 12), *table)
+    minp = ssu.Inputs(*synth.contract_tree(tree, 0.33, 12), *table)
     for case, x in (("random", inp), ("multifurcating", minp)):
         for name in ("unweighted", "weighted_normalized", "weighted_normalized_fp64", "generalized"):
             monkeypatch.setenv("SSU_EMBED_PATHS", "1")

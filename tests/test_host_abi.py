@@ -123,6 +123,26 @@ def test_postorder_matches_oracle(ssu, O, synth, seed, tips, keep):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("shape", ["multifurcating", "caterpillar"])
+@pytest.mark.parametrize("seed,keep", [(7, 0.05), (8, 0.4), (9, 1.0)])
+def test_postorder_shapes_match_oracle(ssu, O, synth, shape, seed, keep):
+    """Shear + collapse + postorder over multifurcating trees (nodes of three
+    or more children; shearing leaves chains of one-child nodes to collapse)
+    and a caterpillar tree, against the oracle's restatement."""
+    if shape == "multifurcating":
+        st, le, nm = synth.contract_tree(synth.random_tree(1500, seed), 0.4, seed + 1)
+    else:
+        st, le, nm = synth.caterpillar_tree(3000, seed)
+    rng = np.random.default_rng(seed)
+    tipnames = [n for n in nm if n]
+    obs = list(rng.choice(tipnames, max(1, int(keep * len(tipnames))), replace=False)) + ["not-a-tip"]
+    mine = ssu.postorder(st, le, nm, obs)
+    tab = O.Table(obs, ["s0"], [0] * (len(obs) + 1), [], [])
+    theirs = O.postorder(O.Tree(st.astype(np.uint8), le, nm), tab)
+    for a, b in zip(mine, theirs):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libssu_ref.so")),
                     reason="reference build not present")
 @pytest.mark.parametrize("seed,tips", [(5, 300), (6, 3000)])
@@ -186,3 +206,20 @@ def test_text_writers_match_reference_format(ssu, tmp_path):
     assert lines[0] == "\tA\tB\tC"
     assert lines[1] == "A\t0\t0.2\t0.5714285714285714"
     assert lines[2] == "B\t0.2\t0\t0.0000001"
+
+
+def test_split_model_choices(ssu):
+    """The occupancy-split model: rare nodes go to posting lists, common ones
+    stay on the tiles; the choice never models slower than no split, and a
+    heavier tile term (generalized) moves the threshold up."""
+    n = 16384
+    assert ssu.choose_split(0, n, np.full(1000, n))[0] == 0              # every node common: no split
+    thr, ms, ms0 = ssu.choose_split(0, n, np.full(1000, 3))
+    assert thr > 3 and ms < ms0                                           # every node rare: all to postings
+    rng = np.random.default_rng(1)
+    occ = np.minimum(n, (rng.pareto(1.2, 50000) * 40).astype(np.int64) + 1)
+    got = {k: ssu.choose_split(k, n, occ) for k in range(4)}
+    for k, (thr, ms, ms0) in got.items():
+        assert 0 < thr <= n and ms <= ms0
+    assert got[2][0] > got[0][0]                                          # generalized splits higher
+    assert got[3][0] < got[0][0]                                          # the Gram column is cheap

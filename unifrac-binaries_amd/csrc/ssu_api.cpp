@@ -24,6 +24,7 @@
 #include "ssu_rarefy.hpp"
 #include "ssu_stats.hpp"
 #include "ssu_tsv.hpp"
+#include "ssu_tune.hpp"
 #include "ssu_results.hpp"
 #include "ssu_table.hpp"
 #include "ssu_tree.hpp"
@@ -116,12 +117,9 @@ ComputeSpec make_spec(Method m, bool fp64, bool vaw, double alpha, bool bypass, 
 
 // Full n x n matrix of the given element type in host memory.
 template <class TOut>
-ComputeStatus run_full(const Table& t, const PhyloTree& tree, const ComputeSpec& spec, TOut* out) {
+ComputeStatus run_full(const Table& t, const ssu::PostorderNodes& nodes, const ComputeSpec& spec, TOut* out) {
     std::string err;
     auto t0 = Clock::now();
-    ssu::PostorderNodes nodes = synced_nodes(t, tree);
-    g_call_ms[CP_TREE] = ms_since(t0);
-    t0 = Clock::now();
     std::unique_ptr<Plan> plan = Plan::create(t, nodes, spec, &err);
     if (!plan) { set_error(err); return output_error; }
     g_call_ms[CP_PLAN] = ms_since(t0);
@@ -146,6 +144,14 @@ ComputeStatus run_full(const Table& t, const PhyloTree& tree, const ComputeSpec&
         for (size_t i = 0; i < nn; ++i) out[i] = (TOut)tmp[i];
     }
     return okay;
+}
+
+template <class TOut>
+ComputeStatus run_full(const Table& t, const PhyloTree& tree, const ComputeSpec& spec, TOut* out) {
+    const auto t0 = Clock::now();
+    ssu::PostorderNodes nodes = synced_nodes(t, tree);
+    g_call_ms[CP_TREE] = ms_since(t0);
+    return run_full<TOut>(t, nodes, spec, out);
 }
 
 // A rarefied table: the kept samples and rows of the batch with one
@@ -229,14 +235,38 @@ ComputeStatus matrix_inmem(const support_biom_t* td, const support_bptree_t* tr,
     if (!td || !tr || td->n_samples <= 0 || td->n_obs <= 0) return table_empty;
     for (double& x : g_call_ms) x = 0.0;
     const auto t0 = Clock::now();
+    if (depth > 0) {   // rarefied first: the sampler shears the tree to the kept table itself
+        Table t = Table::borrow(td->obs_ids, td->sample_ids, td->indptr, td->indices, td->data, (uint32_t)td->n_obs,
+                                (uint32_t)td->n_samples);
+        PhyloTree tree = PhyloTree::from_arrays(tr->structure, tr->lengths, tr->names, tr->n_parens);
+        ComputeStatus v = validate(t, tree);
+        if (v != okay) return v;
+        g_call_ms[CP_INPUT] = ms_since(t0);
+        v = matrix_common<TReal, TMat>(t, tree, method_name, vaw, alpha, bypass, normalize, depth, with_repl,
+                                       mmap_dir, result);
+        g_call_ms[CP_TOTAL] = ms_since(t0);
+        return v;
+    }
+    // the table borrowed without an id map; the tree sheared, collapsed and walked in
+    // postorder straight from its arrays (src/api.cpp:90-100 SYNC_TREE_TABLE, VALIDATE_TREE_TABLE)
     Table t = Table::borrow(td->obs_ids, td->sample_ids, td->indptr, td->indices, td->data, (uint32_t)td->n_obs,
-                            (uint32_t)td->n_samples);
-    PhyloTree tree = PhyloTree::from_arrays(tr->structure, tr->lengths, tr->names, tr->n_parens);
-    ComputeStatus v = validate(t, tree);
-    if (v != okay) return v;
+                            (uint32_t)td->n_samples, false);
     g_call_ms[CP_INPUT] = ms_since(t0);
-    v = matrix_common<TReal, TMat>(t, tree, method_name, vaw, alpha, bypass, normalize, depth, with_repl, mmap_dir,
-                                   result);
+    auto t1 = Clock::now();
+    ssu::NameIndex idx(td->obs_ids, (uint32_t)td->n_obs);
+    ssu::PostorderNodes nodes;
+    uint32_t matched = 0;
+    if (tr->n_parens <= 0 ||
+        !ssu::synced_postorder(tr->structure, tr->lengths, tr->names, (uint32_t)tr->n_parens, idx, nodes, matched))
+        return tree_missing;
+    if (matched < idx.distinct()) return table_and_tree_do_not_overlap;
+    g_call_ms[CP_TREE] = ms_since(t1);
+    if (*result == nullptr && !init_full<TReal, TMat>(*result, t.sample_ids, mmap_dir)) {
+        set_error("result allocation failed");
+        return output_error;
+    }
+    const ComputeStatus v =
+        run_full<TReal>(t, nodes, make_spec(m, fp64, vaw, alpha, bypass, normalize), (*result)->matrix);
     g_call_ms[CP_TOTAL] = ms_since(t0);
     return v;
 }
@@ -1482,6 +1512,8 @@ uint64_t ssu_plan_sparse_terms(const ssu_plan_t* plan) { return plan ? plan->pla
 
 uint64_t ssu_plan_refined_pairs(const ssu_plan_t* plan) { return plan ? plan->plan->refined_pairs() : 0; }
 
+double ssu_plan_split_theta(const ssu_plan_t* plan) { return plan ? plan->plan->split_theta() : 0.0; }
+
 ComputeStatus ssu_plan_execute_stripes(ssu_plan_t* plan, unsigned int start, unsigned int stop) {
     std::string err;
     if (!plan) return output_error;
@@ -1504,6 +1536,18 @@ ComputeStatus ssu_plan_copy_stripes(const ssu_plan_t* plan, unsigned int start, 
     for (uint32_t r = 0; r < stop - start; ++r) rows[r] = host_out + (size_t)r * n;
     if (!plan->plan->copy_stripes(start, stop, rows.data(), &err)) { set_error(err); return output_error; }
     return okay;
+}
+
+void ssu_release_device_memory(void) { ssu::pool_release_all(); }
+
+unsigned int ssu_choose_split(int kind, unsigned int n_samples, const uint32_t* occupancy, unsigned int n_nodes,
+                              double* model_ms, double* model_ms_none) {
+    if (kind < 0 || kind > 3) return 0;
+    std::vector<uint32_t> occ(occupancy, occupancy + n_nodes);
+    const ssu::SplitChoice c = ssu::choose_split((ssu::SplitKind)kind, n_samples, occ, 1ull << 30);
+    if (model_ms) *model_ms = c.model_ms;
+    if (model_ms_none) *model_ms_none = c.model_ms_none;
+    return c.thr;
 }
 
 int ssu_last_call_timings(double* ms, int cap) {
@@ -1531,13 +1575,12 @@ void ssu_plan_destroy(ssu_plan_t** plan) {
 
 int ssu_postorder(const support_bptree_t* tr, int n_obs, const char* const* obs_ids, int32_t* parent, double* length,
                   int32_t* is_leaf, int32_t* obs_row) {
-    PhyloTree tree = PhyloTree::from_arrays(tr->structure, tr->lengths, tr->names, tr->n_parens);
-    if (!tree.valid()) return -1;
-    std::unordered_set<std::string> keep;
-    std::unordered_map<std::string, uint32_t> idx;
-    for (int i = 0; i < n_obs; ++i) { keep.insert(obs_ids[i]); idx[obs_ids[i]] = (uint32_t)i; }
-    PhyloTree s = tree.shear(keep).collapse();
-    ssu::PostorderNodes p = ssu::postorder_nodes(s, idx);
+    ssu::NameIndex idx(obs_ids, (uint32_t)std::max(n_obs, 0));
+    ssu::PostorderNodes p;
+    uint32_t matched = 0;
+    if (tr->n_parens < 0 ||
+        !ssu::synced_postorder(tr->structure, tr->lengths, tr->names, (uint32_t)tr->n_parens, idx, p, matched))
+        return -1;
     for (uint32_t k = 0; k < p.m; ++k) {
         parent[k] = p.parent[k];
         length[k] = p.length[k];

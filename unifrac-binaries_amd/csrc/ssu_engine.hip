@@ -2,6 +2,7 @@
 // buffer management and the per-method kernel sequence.
 #include "ssu_engine.hpp"
 #include "ssu_kernels.hpp"
+#include "ssu_tune.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -9,6 +10,8 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <chrono>
+#include <cstdio>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -38,22 +41,15 @@ struct DevMem {
     size_t bytes = 0;
     ~DevMem() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) pool_free(p, bytes);
         p = nullptr;
         bytes = 0;
     }
     bool ensure(size_t b, std::string* err) {
         if (b <= bytes && p) return true;
         release();
-        if (b == 0) b = 16;
-        hipError_t e = hipMalloc(&p, b);
-        if (e != hipSuccess) {
-            p = nullptr;
-            if (err) *err = "hipMalloc(" + std::to_string(b) + " bytes) failed: " + hipGetErrorString(e);
-            return false;
-        }
-        bytes = b;
-        return true;
+        p = pool_alloc(std::max<size_t>(b, 16), &bytes, err);
+        return p != nullptr;
     }
     template <class T> T* as() const { return (T*)p; }
 };
@@ -65,13 +61,112 @@ bool upload(DevMem& d, const std::vector<T>& h, hipStream_t st, std::string* err
     return true;
 }
 
+// (through the pinned staging area: returns once the host buffer has been read)
 bool upload_raw(DevMem& d, const void* h, size_t bytes, hipStream_t st, std::string* err) {
     if (!d.ensure(bytes, err)) return false;
-    if (bytes) SSU_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, st));
-    return true;
+    return copy_to_device(d.p, h, bytes, st, err);
 }
 
 }  // namespace
+
+// Device memory cache.  hipMalloc and hipFree of the multi-gigabyte buffers a
+// plan holds cost far more than the plan's kernels (they map and unmap pages),
+// so freed blocks stay with the process and later plans of similar sizes reuse
+// them: sizes are rounded to 2 MiB, a request takes the smallest cached block
+// of its device that is large enough and at most a quarter larger, and the
+// cache keeps at most SSU_POOL_GIB GiB (default 96; 0 turns caching off),
+// releasing its largest blocks beyond that.  A failed hipMalloc empties the
+// cache and tries once more.  ssu_release_device_memory() empties it.
+namespace {
+struct Pool {
+    std::mutex mu;
+    std::multimap<size_t, std::pair<int, void*>> free_blocks;   // size -> (device, block)
+    size_t cached = 0;
+    size_t cap = 0;
+    bool cap_set = false;
+};
+Pool& pool() {
+    static Pool* p = new Pool();   // never destroyed: the runtime may already be gone at exit
+    return *p;
+}
+size_t pool_cap(Pool& P) {
+    if (!P.cap_set) {
+        double gib = 96.0;
+        if (const char* env = getenv("SSU_POOL_GIB")) gib = atof(env);
+        P.cap = gib > 0 ? (size_t)(gib * (double)(1ull << 30)) : 0;
+        P.cap_set = true;
+    }
+    return P.cap;
+}
+void pool_trim_locked(Pool& P, size_t keep) {
+    while (P.cached > keep && !P.free_blocks.empty()) {
+        auto it = std::prev(P.free_blocks.end());   // largest first
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        if (it->second.first != cur) (void)hipSetDevice(it->second.first);
+        (void)hipFree(it->second.second);
+        if (it->second.first != cur) (void)hipSetDevice(cur);
+        P.cached -= it->first;
+        P.free_blocks.erase(it);
+    }
+}
+}  // namespace
+
+void* pool_alloc(size_t bytes, size_t* got, std::string* err) {
+    constexpr size_t G = size_t(2) << 20;
+    const size_t rb = (bytes + G - 1) / G * G;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    Pool& P = pool();
+    {
+        std::lock_guard<std::mutex> lock(P.mu);
+        for (auto it = P.free_blocks.lower_bound(rb); it != P.free_blocks.end() && it->first <= rb + rb / 4; ++it) {
+            if (it->second.first != dev) continue;
+            void* q = it->second.second;
+            *got = it->first;
+            P.cached -= it->first;
+            P.free_blocks.erase(it);
+            return q;
+        }
+    }
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, rb);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> lock(P.mu);
+            pool_trim_locked(P, 0);
+        }
+        e = hipMalloc(&q, rb);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (err) *err = "hipMalloc(" + std::to_string(rb) + " bytes) failed: " + hipGetErrorString(e);
+        *got = 0;
+        return nullptr;
+    }
+    *got = rb;
+    return q;
+}
+
+void pool_free(void* p, size_t bytes) {
+    if (!p) return;
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lock(P.mu);
+    const size_t cap = pool_cap(P);
+    if (bytes > cap) { (void)hipFree(p); return; }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    P.free_blocks.emplace(bytes, std::make_pair(dev, p));
+    P.cached += bytes;
+    pool_trim_locked(P, cap);
+}
+
+void pool_release_all() {
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lock(P.mu);
+    pool_trim_locked(P, 0);
+}
 
 // Pinned staging for device-to-host copies: two chunks, allocated on first use
 // and kept for the life of the process (pinned allocations are slow to make).
@@ -86,6 +181,24 @@ struct Stage {
 Stage& stage() {
     static Stage* s = new Stage();   // never destroyed: the runtime may already be gone at exit
     return *s;
+}
+
+// threads for a host pass over n elements (one below a million elements)
+size_t host_threads(uint64_t n) {
+    return n < (1u << 20) ? 1 : std::max<size_t>(1, std::min<unsigned>(8, std::thread::hardware_concurrency()));
+}
+
+// fn(t, begin, end) over `parts` consecutive ranges of [0, n), range 0 on the calling thread
+template <class F>
+void parallel_chunks(uint64_t n, size_t parts, F fn) {
+    const uint64_t step = (n + parts - 1) / std::max<size_t>(parts, 1);
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < parts; ++t) {
+        const uint64_t a = std::min(n, step * t), b = std::min(n, a + step);
+        pool.emplace_back([=] { fn(t, a, b); });
+    }
+    fn(0, 0, std::min(n, step));
+    for (auto& th : pool) th.join();
 }
 
 // dst[0..bytes) = src[0..bytes), split over up to `threads` threads
@@ -143,6 +256,45 @@ bool copy_to_host(void* host, const void* dev, size_t bytes, void* stream, std::
     return true;
 }
 
+bool copy_to_device(void* dev, const void* host, size_t bytes, void* stream, std::string* err) {
+    if (!bytes) return true;
+    hipStream_t st = (hipStream_t)stream;
+    Stage& S = stage();
+    std::unique_lock<std::mutex> lock(S.mu);
+    if (bytes <= STAGE_CHUNK / 4 || S.failed) {
+        lock.unlock();
+        SSU_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st));
+        SSU_HIP(hipStreamSynchronize(st));   // the caller's buffer may go once this returns
+        return true;
+    }
+    if (!S.buf[0]) {
+        for (int b = 0; b < 2; ++b)
+            if (hipHostMalloc(&S.buf[b], STAGE_CHUNK, hipHostMallocDefault) != hipSuccess ||
+                hipEventCreateWithFlags(&S.done[b], hipEventDisableTiming) != hipSuccess) {
+                S.failed = true;
+                lock.unlock();
+                SSU_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st));
+                SSU_HIP(hipStreamSynchronize(st));
+                return true;
+            }
+    }
+    const unsigned threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
+    bool used[2] = {false, false};
+    for (size_t c = 0; c < nch; ++c) {
+        const int b = (int)(c & 1);
+        const size_t len = std::min(STAGE_CHUNK, bytes - c * STAGE_CHUNK);
+        if (used[b]) SSU_HIP(hipEventSynchronize(S.done[b]));   // its previous chunk has left for the device
+        spread_copy((char*)S.buf[b], (const char*)host + c * STAGE_CHUNK, len, threads);
+        SSU_HIP(hipMemcpyAsync((char*)dev + c * STAGE_CHUNK, S.buf[b], len, hipMemcpyHostToDevice, st));
+        SSU_HIP(hipEventRecord(S.done[b], st));
+        used[b] = true;
+    }
+    for (int b = 0; b < 2; ++b)
+        if (used[b]) SSU_HIP(hipEventSynchronize(S.done[b]));
+    return true;
+}
+
 bool device_available(std::string* err) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
@@ -168,6 +320,7 @@ struct PlanImpl {
     bool sparse = false;             // weighted (or generalized) through the shared-node kernels
     bool gen = false;                // generalized on the shared-node path
     uint32_t split_thr = 0;          // FP32 shared-node path: nodes held by fewer samples use posting lists (0: off)
+    double split_theta = 0.0;        // the occupancy split chosen (threshold / n; 0: none)
     uint32_t split_slots = 0;        // bound on their postings in any execution
     // bit q: processed node q is held by fewer than split_thr samples of the plan's
     // table, so it is never dense in any execution (its FP32 operand row is not needed)
@@ -375,14 +528,18 @@ bool PlanImpl::structural_counts(std::vector<uint32_t>& cnt, std::string* err) {
 // held by no sample leave it altogether.  SSU_GRAM_SPLIT sets theta (0: off),
 // SSU_GRAM_SPLIT_MIN_N the smallest table split (GSPLIT_MIN_N).
 bool PlanImpl::split_gram(std::string* err) {
-    double theta = 0.06;
-    if (const char* env = getenv("SSU_GRAM_SPLIT")) theta = atof(env);
+    const char* env_theta = getenv("SSU_GRAM_SPLIT");
+    const double theta = env_theta ? atof(env_theta) : 1.0;
     uint32_t min_n = GSPLIT_MIN_N;
     if (const char* env = getenv("SSU_GRAM_SPLIT_MIN_N")) min_n = (uint32_t)atol(env);
     const uint32_t K = (uint32_t)knodes.size();
     if (!(theta > 0.0) || K == 0 || n < std::max<uint32_t>(min_n, 2)) return true;
     std::vector<uint32_t> cnt;
     if (!structural_counts(cnt, err)) return false;
+    // the threshold: SSU_GRAM_SPLIT * n, or the cost model's choice on these occupancies
+    uint32_t thr0 = env_theta ? (uint32_t)std::max(1.0, std::ceil(theta * n))
+                              : choose_split(SplitKind::gram, n, cnt, 1ull << 30).thr;
+    if (thr0 == 0) return true;
 
     const size_t npan = (n + 31) / 32, panels = ldn / 32;
     std::vector<uint32_t> off;
@@ -413,9 +570,10 @@ bool PlanImpl::split_gram(std::string* err) {
         const double rl = (double)n * row_list_chunks(n, (uint32_t)sp_node.size()) * 8.0;   // row-list counts, offsets
         return slots * 12.0 + ks * (double)(npan + 1 + panels) * 4.0 + rl <= GSPLIT_BUDGET && slots < (1u << 30);
     };
-    uint32_t thr = (uint32_t)std::max(1.0, std::ceil(theta * n));
+    uint32_t thr = thr0;
     uint64_t slots = classify(thr);
     while (!fits(slots) && thr > 1) slots = classify(thr /= 2);
+    split_theta = (double)thr / n;
     if (!fits(slots)) slots = classify(0);   // no sparse nodes
     if (!build_gram_columns()) return false;
     gsplit_slots = sp_node.empty() ? 0 : (uint32_t)slots;
@@ -436,16 +594,20 @@ bool PlanImpl::split_gram(std::string* err) {
 // posting and row-list arrays and must stay below 2^30 (32-bit slot offsets in
 // the kernels): theta is lowered to fit.  SSU_SPARSE_SPLIT sets theta (0: off).
 bool PlanImpl::split_weighted(std::string* err) {
-    // generalized terms cost several times the weighted ones on the tiles, which moves the balance
-    double theta = gen ? 0.8 : 0.25;
-    if (const char* env = getenv("SSU_SPARSE_SPLIT")) theta = atof(env);
+    const char* env_theta = getenv("SSU_SPARSE_SPLIT");
+    const double theta = env_theta ? atof(env_theta) : 1.0;
     bool nonneg = true;
     for (size_t q = 0; q < knodes.size(); ++q) nonneg = nonneg && (spec.fp64 ? klen_d[q] >= 0.0 : klen_f[q] >= 0.0f);
     const uint32_t K = (uint32_t)knodes.size();
     if (!nonneg || !(theta > 0.0) || K == 0 || n < 2) return true;
     std::vector<uint32_t> cnt;
     if (!structural_counts(cnt, err)) return false;
-    uint32_t thr = (uint32_t)std::max(1.0, std::ceil(theta * n));
+    // the threshold: SSU_SPARSE_SPLIT * n, or the cost model's choice on these
+    // occupancies (generalized terms cost several times the weighted ones on the
+    // tiles, which moves the balance)
+    const SplitKind kind = gen ? SplitKind::generalized32 : spec.fp64 ? SplitKind::weighted64 : SplitKind::weighted32;
+    uint32_t thr = env_theta ? (uint32_t)std::max(1.0, std::ceil(theta * n)) : choose_split(kind, n, cnt, 1ull << 30).thr;
+    if (thr == 0) return true;
     auto bound = [&](uint32_t t) {
         uint64_t b = 0;
         for (uint32_t c : cnt) b += std::min<uint64_t>(c, t - 1);
@@ -558,11 +720,33 @@ Plan::~Plan() = default;
 std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nodes, const ComputeSpec& spec,
                                    std::string* err) {
     if (!device_available(err)) return nullptr;
+    // SSU_TIMING=1: wall-clock steps of the plan's creation on stderr
+    const bool timing = getenv("SSU_TIMING") && atoi(getenv("SSU_TIMING")) != 0;
+    auto tprev = std::chrono::steady_clock::now();
+    auto step = [&](const char* what) {
+        if (!timing) return;
+        const auto now = std::chrono::steady_clock::now();
+        fprintf(stderr, "[ssu plan] %-28s %8.2f ms\n", what, std::chrono::duration<double, std::milli>(now - tprev).count());
+        tprev = now;
+    };
     // the device kernels index with these values: reject malformed CSR up front
     for (uint32_t i = 0; i < table.n_obs; ++i)
         if (table.indptr[i + 1] < table.indptr[i]) { if (err) *err = "table indptr is not monotone"; return nullptr; }
-    for (uint64_t j = table.n_obs ? table.indptr[0] : 0; table.n_obs && j < table.indptr[table.n_obs]; ++j)
-        if (table.indices[j] >= table.n_samples) { if (err) *err = "table sample index out of range"; return nullptr; }
+    const uint64_t nz_all = table.nnz();
+    const uint32_t nz_base = table.n_obs ? table.indptr[0] : 0;
+    {
+        const uint32_t* ix = table.indices + nz_base;
+        std::vector<uint32_t> mx(host_threads(nz_all), 0u);
+        parallel_chunks(nz_all, mx.size(), [&](size_t t, uint64_t a, uint64_t b) {
+            uint32_t m = 0;   // a max reduction the compiler vectorises
+            for (uint64_t j = a; j < b; ++j) m = std::max(m, ix[j]);
+            mx[t] = m;
+        });
+        if (nz_all && *std::max_element(mx.begin(), mx.end()) >= table.n_samples) {
+            if (err) *err = "table sample index out of range";
+            return nullptr;
+        }
+    }
     for (uint32_t k = 0; k < nodes.m; ++k)
         if (nodes.obs_row[k] >= (int32_t)table.n_obs || nodes.parent[k] < 0 || (uint32_t)nodes.parent[k] > nodes.m) {
             if (err) *err = "inconsistent postorder arrays";
@@ -601,6 +785,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     for (uint32_t k = 0; k < P.m; ++k)
         if ((uint32_t)nodes.parent[k] < P.m) P.child_idx[fill[nodes.parent[k]]++] = (int32_t)k;
 
+    step("validate, levels, children");
     // heavy paths when they take fewer launches than the height levels (SSU_EMBED_PATHS=0 / 1 forces)
     P.build_paths(nodes);
     P.use_paths = P.round_off.size() < P.level_off.size();
@@ -613,20 +798,33 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         P.klen_f.push_back((float)nodes.length[k]);          // TFloat lengths, src/unifrac_cmp.cpp:127
     }
 
+    step("schedule");
     const bool unw = spec.method == Method::unweighted || spec.method == Method::unweighted_unnormalized;
     P.gram = unw && !spec.vaw && spec.plain_kernel == 0;
-    if (P.gram && !P.build_gram_columns()) P.gram = false;  // non-finite lengths: VALU path
+    if (P.gram)   // non-finite lengths: VALU path
+        for (size_t q = 0; q < P.knodes.size() && P.gram; ++q)
+            P.gram = std::isfinite(spec.fp64 ? P.klen_d[q] : (double)P.klen_f[q]);
     P.n_pad = round_up(std::max<uint32_t>(P.n, 1), GRAM_BM);
+    // every stored value finite and >= 0 (one pass, for the presence bits and the shared-node kernels)
+    const bool weighted_m = spec.method == Method::weighted_normalized || spec.method == Method::weighted_unnormalized;
+    bool vals_nonneg = true;
+    if (P.gram || weighted_m || spec.method == Method::generalized) {
+        const double* d = table.data + nz_base;
+        std::vector<uint8_t> bad(host_threads(nz_all), 0);
+        parallel_chunks(nz_all, bad.size(), [&](size_t t, uint64_t a, uint64_t b) {
+            bool x = false;   // a branch-free reduction the compiler vectorises
+            for (uint64_t q = a; q < b; ++q) x |= !(d[q] >= 0.0 && d[q] <= 1.7976931348623157e308);
+            bad[t] = x;
+        });
+        for (uint8_t x : bad) vals_nonneg = vals_nonneg && !x;
+    }
+    step("value checks");
     // Presence bits for the Gram path: with every value finite and >= 0 (and
     // positive totals), E[k][s] > 0 exactly when a stored value > 0 lies below
     // k, an OR up the tree.  SSU_GRAM_BITS=0 keeps the FP64 embedding.
     if (P.gram && P.ldn / 32 <= BITS_MAX_WORDS) {
-        bool ok_vals = true;
-        if (const char* env = getenv("SSU_GRAM_BITS")) ok_vals = atoi(env) != 0;
-        const uint64_t nz = table.nnz();
-        const uint32_t b0 = table.n_obs ? table.indptr[0] : 0;
-        for (uint64_t q = 0; q < nz && ok_vals; ++q)
-            ok_vals = std::isfinite(table.data[b0 + q]) && table.data[b0 + q] >= 0.0;
+        bool ok_vals = vals_nonneg;
+        if (const char* env = getenv("SSU_GRAM_BITS")) ok_vals = ok_vals && atoi(env) != 0;
         if (ok_vals) {
             P.bits = true;
             P.ldb = (uint32_t)(P.ldn / 32);
@@ -643,10 +841,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     if (P.sparse) {
         for (size_t q = 0; q < P.klen_d.size() && P.sparse; ++q)
             if (!std::isfinite(spec.fp64 ? P.klen_d[q] : (double)P.klen_f[q])) P.sparse = false;
-        const uint64_t nz = table.nnz();
-        const uint32_t b0 = table.n_obs ? table.indptr[0] : 0;
-        for (uint64_t q = 0; q < nz && P.sparse; ++q)
-            if (!(table.data[b0 + q] >= 0.0)) P.sparse = false;
+        P.sparse = P.sparse && vals_nonneg;
     }
     P.gen = P.gen && P.sparse;
     // Near-duplicate refinement: an FP32 weighted or generalized distance is off
@@ -688,6 +883,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
 
     std::string e2;
     if (!err) err = &e2;
+    step("method setup");
     hipError_t he = hipStreamCreateWithFlags(&P.own_st, hipStreamNonBlocking);
     if (he != hipSuccess) { *err = std::string("hipStreamCreate: ") + hipGetErrorString(he); return nullptr; }
     P.st = P.own_st;
@@ -727,13 +923,17 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         P.block_order = upper_block_order((uint32_t)(P.n_pad / GRAM_BM), 6);
         ok = upload(P.d_order, P.block_order, P.st, err);
     }
-    if (ok && P.gram) ok = P.split_gram(err);   // may narrow the Gram columns
+    step("stream, events, uploads");
+    if (ok && P.gram) ok = P.split_gram(err);   // builds the Gram columns (narrowed when it splits)
+    if (ok && P.gram && P.col_node.empty() && !P.build_gram_columns()) ok = false;
     if (ok && P.sparse) ok = P.split_weighted(err);
+    step("occupancy split");
     if (ok && P.gram && P.Kp)
         ok = upload(P.d_col, P.col_node, P.st, err) && upload(P.d_digits, P.digits, P.st, err) &&
              upload(P.d_chunks, P.chunks, P.st, err);
     if (!ok) return nullptr;
     if (hipStreamSynchronize(P.st) != hipSuccess) { *err = "upload failed"; return nullptr; }
+    step("gram columns, sync");
     return plan;
 }
 
@@ -1256,6 +1456,11 @@ uint64_t Plan::sparse_terms() const {
     return sum;
 }
 bool Plan::out_fp64() const { return p_->spec.fp64; }
+double Plan::split_theta() const {
+    const PlanImpl& P = *p_;
+    if (P.gram) return P.gsplit_slots ? P.split_theta : 0.0;
+    return P.split_thr ? (double)P.split_thr / std::max<uint32_t>(P.n, 1) : 0.0;
+}
 const char* Plan::main_kernel() const {
     if (p_->gram) return "k_gram_i8";
     if (p_->sparse) return p_->spec.fp64 ? "k_pairs_sparse" : (p_->gen ? "k_pairs_gen32" : "k_pairs_sparse32");

@@ -94,6 +94,9 @@ public:
     // the full diagonal tiles included (shared-node kernel: visited nodes only)
     uint64_t kernel_terms() const;
     bool out_fp64() const;
+    // occupancy split in use: nodes held by fewer than this fraction of the
+    // samples go to the posting lists (0: none)
+    double split_theta() const;
     const char* main_kernel() const;
     const void* device_output() const;
 
@@ -105,9 +108,18 @@ private:
 // true iff a HIP device is usable; err explains why not
 bool device_available(std::string* err);
 
+// Device memory blocks through the process-wide cache that plans allocate from
+// (got: the block's size, at least bytes); pool_release_all returns every
+// cached block to the runtime.
+void* pool_alloc(size_t bytes, size_t* got, std::string* err);
+void pool_free(void* p, size_t bytes);
+void pool_release_all();
+
 // bytes of device memory into pageable host memory through a process-wide
 // pinned staging area: chunks copied over PCIe on `stream` while the previous
 // chunk is spread into `host` by several threads
 bool copy_to_host(void* host, const void* dev, size_t bytes, void* stream, std::string* err);
+// the reverse, host to device; returns once the host buffer has been read
+bool copy_to_device(void* dev, const void* host, size_t bytes, void* stream, std::string* err);
 
 }  // namespace ssu

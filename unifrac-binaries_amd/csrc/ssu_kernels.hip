@@ -2332,6 +2332,8 @@ hipError_t launch_pairs_gen32(const SparseArgs& a, float alpha, void* out2, hipS
     const uint32_t nbj = (a.n + S3_BJ - 1) / S3_BJ;
     if (!nbj) return hipSuccess;
     if (a.dmask && !out2) return hipErrorInvalidValue;
+    if (!a.order) return hipErrorInvalidValue;
+    if (!a.n_blocks) return hipSuccess;   // a stripe selection without a block here
     const uint32_t per_xcd = (a.n_blocks + N_XCDS - 1) / N_XCDS;
     hipLaunchKernelGGL((k_pairs_gen32<float>), dim3(per_xcd * N_XCDS), dim3(64), 0, st, (const float*)a.U, a.ldn, a.len,
                        a.masks, a.Kw, a.n, (float*)a.out, (float*)out2, a.visits, a.order, a.n_blocks, per_xcd,
@@ -2343,7 +2345,8 @@ hipError_t launch_pairs_sparse32(const SparseArgs& a, hipStream_t st) {
     if (a.ldn % S3_BJ) return hipErrorInvalidValue;
     const uint32_t nbi = (a.n + S3_BI - 1) / S3_BI, nbj = (a.n + S3_BJ - 1) / S3_BJ;
     if (!nbi) return hipSuccess;
-    if ((size_t)nbj * S3_BJ > a.ldn || nbi > 65535 || !a.order || !a.n_blocks) return hipErrorInvalidValue;
+    if ((size_t)nbj * S3_BJ > a.ldn || nbi > 65535 || !a.order) return hipErrorInvalidValue;
+    if (!a.n_blocks) return hipSuccess;   // a stripe selection without a block here
     const uint32_t per_xcd = (a.n_blocks + N_XCDS - 1) / N_XCDS;
     if (a.out_fp64) pairs_sparse32_t<double>(per_xcd * N_XCDS, st, a, per_xcd);
     else pairs_sparse32_t<float>(per_xcd * N_XCDS, st, a, per_xcd);
@@ -2354,8 +2357,9 @@ hipError_t launch_pairs_sparse(bool in_fp64, const SparseArgs& a, hipStream_t st
     if (a.ldn % SP_B) return hipErrorInvalidValue;
     const uint32_t nb = (a.n + SP_B - 1) / SP_B;
     if (!nb) return hipSuccess;
-    if ((size_t)nb * SP_B > a.ldn || nb > 65535 || a.n_blocks != nb * (nb + 1) / 2 || !a.order)
+    if ((size_t)nb * SP_B > a.ldn || nb > 65535 || a.n_blocks > nb * (nb + 1) / 2 || !a.order)
         return hipErrorInvalidValue;
+    if (!a.n_blocks) return hipSuccess;   // a stripe selection without a block here
     const uint32_t per_xcd = (a.n_blocks + N_XCDS - 1) / N_XCDS;
     const uint32_t grid = per_xcd * N_XCDS;
     if (in_fp64) {

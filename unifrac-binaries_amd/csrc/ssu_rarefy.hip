@@ -142,7 +142,15 @@ struct RarefyImpl {
     Table table;
     std::vector<uint32_t> sample_index, obs_index;
     std::vector<ComputeSpec> specs;
+    // lanes x methods plans, plans[lane * specs.size() + k]: the replicates go round
+    // robin over the lanes, each lane on a stream of its own, so small tables'
+    // replicates run side by side (one lane's kernels alone leave most CUs idle)
     std::vector<std::unique_ptr<Plan>> plans;
+    uint32_t lanes = 1;
+    std::vector<hipStream_t> lane_st;
+    std::vector<hipEvent_t> lane_ev;
+    hipEvent_t fork_ev = nullptr;
+    Plan* plan_of(size_t k, uint32_t rep) const { return plans[(rep % lanes) * specs.size() + k].get(); }
     uint32_t depth = 0, reps = 0, hsize = 0;
     bool with = true;
     uint64_t seed = 0;
@@ -172,17 +180,18 @@ struct RarefyImpl {
         for (const auto& p : plans) any = any || p->refines();
         if (!any) return true;
         const size_t stride = (size_t)table.n_samples + 1;
-        std::vector<uint32_t> counts(plans.size() * reps);
+        std::vector<uint32_t> counts(specs.size() * reps);
         RF_HIP(hipMemcpy2D(counts.data(), sizeof(uint32_t), d_flags, stride * sizeof(uint32_t), sizeof(uint32_t),
                            counts.size(), hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < plans.size(); ++k)
+        for (size_t k = 0; k < specs.size(); ++k)
             for (uint32_t r = 0; r < reps; ++r) {
-                if (!plans[k]->refines() || !counts[k * reps + r]) continue;
+                Plan* pl = plan_of(k, r);
+                if (!pl->refines() || !counts[k * reps + r]) continue;
                 char* out = (char*)d_out[k] + out_bytes(k) * r;
-                if (!plans[k]->refine((const double*)d_rdata + (size_t)r * nnz, (const double*)d_depth, out,
-                                      flags_of(k, r), sums_of(k, r), err))
+                if (!pl->refine((const double*)d_rdata + (size_t)r * nnz, (const double*)d_depth, out, flags_of(k, r),
+                                sums_of(k, r), err))
                     return false;
-                refined += plans[k]->refined_pairs();
+                refined += pl->refined_pairs();
             }
         return true;
     }
@@ -197,6 +206,11 @@ struct RarefyImpl {
             if (p) (void)hipFree(p);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        for (hipEvent_t e : lane_ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t q : lane_st)
+            if (q) (void)hipStreamDestroy(q);
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -217,16 +231,25 @@ struct RarefyImpl {
                                reps, depth, seed, (uint64_t*)d_set, hsize);
         }
         RF_HIP(hipGetLastError());
+        // fork: every lane waits for the drawn tables
+        RF_HIP(hipEventRecord(fork_ev, st));
+        for (hipStream_t q : lane_st) RF_HIP(hipStreamWaitEvent(q, fork_ev, 0));
         for (uint32_t r = 0; r < reps; ++r)
-            for (size_t k = 0; k < plans.size(); ++k) {
+            for (size_t k = 0; k < specs.size(); ++k) {
+                Plan* pl = plan_of(k, r);
                 char* out = (char*)d_out[k] + out_bytes(k) * r;
                 // non-quiet (the eager pass) refines at once; captured work leaves it to refine()
-                const bool rf = plans[k]->refines();
-                if (!plans[k]->execute_into((const double*)d_rdata + (size_t)r * nnz, (const double*)d_depth, out,
-                                            quiet, err, rf ? flags_of(k, r) : nullptr, rf ? sums_of(k, r) : nullptr))
+                const bool rf = pl->refines();
+                if (!pl->execute_into((const double*)d_rdata + (size_t)r * nnz, (const double*)d_depth, out, quiet, err,
+                                      rf ? flags_of(k, r) : nullptr, rf ? sums_of(k, r) : nullptr))
                     return false;
-                if (!quiet) refined += plans[k]->refined_pairs();
+                if (!quiet) refined += pl->refined_pairs();
             }
+        // join: the batch's stream waits for every lane
+        for (uint32_t l = 0; l < (uint32_t)lane_st.size(); ++l) {
+            RF_HIP(hipEventRecord(lane_ev[l], lane_st[l]));
+            RF_HIP(hipStreamWaitEvent(st, lane_ev[l], 0));
+        }
         return true;
     }
 };
@@ -301,12 +324,33 @@ std::unique_ptr<RarefyBatch> RarefyBatch::create(const Table& table, const Phylo
         *err = "stream/event creation failed";
         return nullptr;
     }
-    for (const auto& sp : specs) {
-        if (sp.bypass_tips) { *err = "bypass_tips is not supported with batched rarefaction"; return nullptr; }
-        auto plan = Plan::create(t, nodes, sp, err);
-        if (!plan) return nullptr;
-        plan->set_stream(R.st);
-        R.plans.push_back(std::move(plan));
+    // lanes: SSU_RAREFY_LANES (default 4), at most one per replicate
+    R.lanes = 4;
+    if (const char* env = getenv("SSU_RAREFY_LANES")) R.lanes = (uint32_t)std::max(1L, atol(env));
+    R.lanes = std::max<uint32_t>(1, std::min(R.lanes, reps));
+    if (specs.empty()) R.lanes = 1;
+    if (hipEventCreateWithFlags(&R.fork_ev, hipEventDisableTiming) != hipSuccess) {
+        *err = "event creation failed";
+        return nullptr;
+    }
+    for (uint32_t l = 0; l < R.lanes; ++l) {
+        hipStream_t q = nullptr;
+        hipEvent_t e = nullptr;
+        if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            if (q) (void)hipStreamDestroy(q);
+            *err = "stream/event creation failed";
+            return nullptr;
+        }
+        R.lane_st.push_back(q);
+        R.lane_ev.push_back(e);
+        for (const auto& sp : specs) {
+            if (sp.bypass_tips) { *err = "bypass_tips is not supported with batched rarefaction"; return nullptr; }
+            auto plan = Plan::create(t, nodes, sp, err);
+            if (!plan) return nullptr;
+            plan->set_stream(q);
+            R.plans.push_back(std::move(plan));
+        }
     }
     auto dmalloc = [&](void** p, size_t bytes) {
         if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) {
@@ -406,7 +450,7 @@ bool RarefyBatch::copy_counts(uint32_t rep, double* host, std::string* err) cons
 uint64_t RarefyBatch::pair_nodes() const {
     const uint64_t n = p_->table.n_samples;
     uint64_t sum = 0;
-    for (const auto& pl : p_->plans) sum += n * (n - 1) / 2 * pl->processed_nodes();
+    for (size_t k = 0; k < p_->specs.size(); ++k) sum += n * (n - 1) / 2 * p_->plans[k]->processed_nodes();
     return sum * p_->reps;
 }
 

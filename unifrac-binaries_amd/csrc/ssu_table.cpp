@@ -2,29 +2,33 @@
 
 namespace ssu {
 
-void Table::finish() {
+void Table::finish(bool index_obs) {
     // src/biom_inmem.cpp:513-526: totals accumulate in observation order
     totals.assign(n_samples, 0.0);
+    double* tot = totals.data();
     for (uint32_t i = 0; i < n_obs; ++i)
-        for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j) totals[indices[j]] += data[j];
+        for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j) tot[indices[j]] += data[j];
     obs_index.clear();
+    if (!index_obs) return;
     obs_index.reserve(n_obs * 2 + 1);
     for (uint32_t i = 0; i < n_obs; ++i) obs_index[obs_ids[i]] = i;
 }
 
 Table Table::borrow(const char* const* oids, const char* const* sids, const uint32_t* ip, const uint32_t* ix,
-                    const double* d, uint32_t n_obs, uint32_t n_samples) {
+                    const double* d, uint32_t n_obs, uint32_t n_samples, bool index_obs) {
     Table t;
     t.n_obs = n_obs;
     t.n_samples = n_samples;
-    t.obs_ids.reserve(n_obs);
-    for (uint32_t i = 0; i < n_obs; ++i) t.obs_ids.emplace_back(oids[i]);
+    if (index_obs) {
+        t.obs_ids.reserve(n_obs);
+        for (uint32_t i = 0; i < n_obs; ++i) t.obs_ids.emplace_back(oids[i]);
+    }
     t.sample_ids.reserve(n_samples);
     for (uint32_t i = 0; i < n_samples; ++i) t.sample_ids.emplace_back(sids[i]);
     t.indptr = ip;
     t.indices = ix;
     t.data = d;
-    t.finish();
+    t.finish(index_obs);
     return t;
 }
 

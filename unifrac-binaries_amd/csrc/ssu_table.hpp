@@ -20,8 +20,10 @@ struct Table {
     std::unordered_map<std::string, uint32_t> obs_index;   // id -> row
 
     // Borrow caller arrays (src/biom_inmem.cpp:342-384 borrows them too).
+    // index_obs = false skips obs_ids and obs_index (callers that match rows to tips themselves).
     static Table borrow(const char* const* obs_ids, const char* const* sample_ids, const uint32_t* indptr,
-                        const uint32_t* indices, const double* data, uint32_t n_obs, uint32_t n_samples);
+                        const uint32_t* indices, const double* data, uint32_t n_obs, uint32_t n_samples,
+                        bool index_obs = true);
     // Own the arrays (file readers, filtered copies).
     static Table own(std::vector<std::string> obs_ids, std::vector<std::string> sample_ids,
                      std::vector<uint32_t> indptr, std::vector<uint32_t> indices, std::vector<double> data);
@@ -43,7 +45,7 @@ struct Table {
 private:
     std::vector<uint32_t> own_indptr_, own_indices_;
     std::vector<double> own_data_;
-    void finish();
+    void finish(bool index_obs = true);
 };
 
 }  // namespace ssu

@@ -268,4 +268,149 @@ PostorderNodes postorder_nodes(const PhyloTree& t, const std::unordered_map<std:
     return p;
 }
 
+namespace {
+inline uint64_t name_hash(const char* s) {   // FNV-1a
+    uint64_t h = 1469598103934665603ull;
+    for (; *s; ++s) h = (h ^ (unsigned char)*s) * 1099511628211ull;
+    return h ^ (h >> 29);
+}
+}  // namespace
+
+NameIndex::NameIndex(const char* const* ids, uint32_t n) : ids_(ids) {
+    uint32_t cap = 16;
+    while (cap < 2 * (uint64_t)n + 16) cap <<= 1;
+    mask_ = cap - 1;
+    slot_.assign(cap, -1);
+    for (uint32_t i = 0; i < n; ++i) {
+        const char* id = ids[i] ? ids[i] : "";
+        for (uint32_t h = (uint32_t)name_hash(id) & mask_;; h = (h + 1) & mask_) {
+            if (slot_[h] < 0) { slot_[h] = (int32_t)i; ++distinct_; break; }
+            const char* o = ids[slot_[h]] ? ids[slot_[h]] : "";
+            if (std::strcmp(o, id) == 0) { slot_[h] = (int32_t)i; break; }   // the last row wins
+        }
+    }
+}
+
+int32_t NameIndex::find(const char* name) const {
+    const char* nm = name ? name : "";
+    for (uint32_t h = (uint32_t)name_hash(nm) & mask_;; h = (h + 1) & mask_) {
+        const int32_t r = slot_[h];
+        if (r < 0) return -1;
+        const char* o = ids_[r] ? ids_[r] : "";
+        if (std::strcmp(o, nm) == 0) return r;
+    }
+}
+
+bool synced_postorder(const bool* structure, const double* lengths, const char* const* names, uint32_t n,
+                      const NameIndex& rows, PostorderNodes& out, uint32_t& matched) {
+    // pass 1, closes in postorder: kept nodes (leaves named by a row, and their
+    // ancestors) and each node's number of kept children
+    std::vector<uint32_t> kc(n, 0);
+    std::vector<int32_t> row(n, -1);
+    std::vector<uint8_t> kept(n, 0);
+    std::vector<uint32_t> stack;
+    stack.reserve(64);
+    std::vector<uint8_t> seen;
+    matched = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (structure[i]) { stack.push_back(i); continue; }
+        if (stack.empty()) return false;
+        const uint32_t o = stack.back();
+        stack.pop_back();
+        bool keep;
+        if (i == o + 1) {   // a leaf (src/tree.cpp:264-266)
+            const int32_t r = rows.find(names[o]);
+            row[o] = r;
+            keep = r >= 0;
+            if (keep) {
+                if (seen.size() <= (size_t)r) seen.resize((size_t)r + 1, 0);
+                if (!seen[r]) { seen[r] = 1; ++matched; }
+            }
+        } else {
+            keep = kc[o] > 0;
+        }
+        kept[o] = keep;
+        if (keep && !stack.empty()) ++kc[stack.back()];
+    }
+    if (!stack.empty()) return false;
+    // pass 2, opens in preorder: drop unkept nodes and collapse the kept internal
+    // nodes (not the root) with one kept child, whose lengths accumulate top-down
+    // into that child, l_child + (l_parent + ...) (src/tree.cpp:165-191)
+    struct Frame { uint32_t o; bool emit; bool pass; double chain; };
+    std::vector<Frame> fs;
+    fs.reserve(64);
+    std::vector<uint8_t> open2;
+    std::vector<double> len2;
+    std::vector<int32_t> row2;
+    open2.reserve(n);
+    len2.reserve(n);
+    row2.reserve(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (structure[i]) {
+            const bool has_chain = !fs.empty() && fs.back().pass;
+            const double up = has_chain ? fs.back().chain : 0.0;
+            Frame f{i, false, false, 0.0};
+            if (kept[i] && (fs.empty() || fs.back().emit || fs.back().pass)) {
+                const bool leaf = i + 1 < n && !structure[i + 1];
+                const bool collapse = !leaf && i != 0 && kc[i] == 1;
+                const double l = has_chain ? lengths[i] + up : lengths[i];
+                if (collapse) {
+                    f.pass = true;
+                    f.chain = l;
+                } else {
+                    f.emit = true;
+                    open2.push_back(1);
+                    len2.push_back(l);
+                    row2.push_back(leaf ? row[i] : -1);
+                }
+            }
+            fs.push_back(f);
+        } else {
+            const Frame f = fs.back();
+            fs.pop_back();
+            if (f.emit) {
+                open2.push_back(0);
+                len2.push_back(0.0);
+                row2.push_back(-1);
+            }
+        }
+    }
+    // pass 3: postorder of the sheared, collapsed tree (as postorder_nodes)
+    const uint32_t n2 = (uint32_t)open2.size();
+    const uint32_t total = n2 / 2;
+    PostorderNodes& p = out;
+    p.m = total > 0 ? total - 1 : 0;
+    p.parent.assign(p.m, (int32_t)p.m);
+    p.length.assign(p.m, 0.0);
+    p.is_leaf.assign(p.m, 0);
+    p.obs_row.assign(p.m, -1);
+    p.height.assign(p.m, 0);
+    std::vector<int32_t> h(total + 1, 0);
+    // postorder index of each open (numbered at its close), then parents, lengths, heights
+    std::vector<int32_t> post(n2, -1);
+    std::vector<uint32_t> s4;
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < n2; ++i) {
+        if (open2[i]) { s4.push_back(i); continue; }
+        post[s4.back()] = (int32_t)k++;
+        s4.pop_back();
+    }
+    for (uint32_t i = 0; i < n2; ++i) {
+        if (open2[i]) { s4.push_back(i); continue; }
+        const uint32_t o = s4.back();
+        s4.pop_back();
+        const uint32_t kk = (uint32_t)post[o];
+        if (kk >= p.m) continue;   // the root
+        const int32_t pk = s4.empty() ? (int32_t)p.m : post[s4.back()];
+        p.parent[kk] = pk;
+        p.length[kk] = len2[o];
+        const bool leaf = i == o + 1;
+        p.is_leaf[kk] = leaf ? 1 : 0;
+        if (leaf) p.obs_row[kk] = row2[o];
+        p.height[kk] = h[kk];
+        if ((uint32_t)pk < total && h[pk] < h[kk] + 1) h[pk] = h[kk] + 1;
+    }
+    return true;
+}
+
 }  // namespace ssu

@@ -63,4 +63,25 @@ struct PostorderNodes {
 
 PostorderNodes postorder_nodes(const PhyloTree& t, const std::unordered_map<std::string, uint32_t>& obs_index);
 
+// Observation id -> table row over the caller's strings (open addressing);
+// for repeated ids the last row wins, as in Table::obs_index.
+class NameIndex {
+public:
+    NameIndex(const char* const* ids, uint32_t n);
+    int32_t find(const char* name) const;   // -1: absent
+    uint32_t distinct() const { return distinct_; }
+private:
+    std::vector<int32_t> slot_;
+    const char* const* ids_;
+    uint32_t mask_ = 0, distinct_ = 0;
+};
+
+// The tree given as parentheses arrays, sheared to the leaves whose names are
+// table rows, collapsed and walked in postorder: the same nodes as
+// postorder_nodes(PhyloTree::from_arrays(...).shear(ids).collapse(), rows),
+// in two passes over the parentheses without building the intermediate trees.
+// matched: distinct table ids that name a leaf.  false: unbalanced parentheses.
+bool synced_postorder(const bool* structure, const double* lengths, const char* const* names, uint32_t n,
+                      const NameIndex& rows, PostorderNodes& out, uint32_t& matched);
+
 }  // namespace ssu

@@ -1,0 +1,80 @@
+#include "ssu_tune.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace ssu {
+
+namespace {
+
+// Marginal seconds per unit on one MI355X, fitted to whole-plan timings at
+// three thresholds each on the 16,384-sample (100k tips) and 8,192-sample
+// (50k tips) workloads (total = fixed + terms * term + postings * posting); the
+// modelled optimum lands within the measured one for every kind (weighted
+// 0.244 vs 0.25, FP64 weighted 0.229 vs 0.25, generalized 0.738 vs 0.8,
+// Gram 0.075 vs 0.06 - 0.1).  The per-term cost includes the list building
+// and row lists, whose work follows the postings.
+struct Rates {
+    double term;      // one (pair, node) term of the tiled kernel; Gram: one (pair, column) unit
+    double posting;   // one posting visited by the row walk
+};
+
+Rates rates_of(SplitKind k) {
+    switch (k) {
+    case SplitKind::weighted32: return {4.2e-14, 7.0e-13};
+    case SplitKind::weighted64: return {7.1e-14, 1.36e-12};
+    case SplitKind::generalized32: return {6.0e-13, 1.1e-12};
+    case SplitKind::gram: return {3.4e-15, 6.1e-13};
+    }
+    return {1.0, 1.0};
+}
+
+// expected tiled-kernel terms of a node held by c of n samples, placed at
+// random: the pair blocks visit a node either of their sample panels holds
+// (the kernels' counts agree within 1% on the workloads above); a Gram
+// column costs every pair
+double tile_terms(SplitKind k, uint32_t n, uint32_t c) {
+    const double pairs = 0.5 * (double)n * n;
+    if (k == SplitKind::gram) return pairs;
+    const double q = 1.0 - (double)c / n;   // a given sample lacks the node
+    return pairs * (1.0 - std::pow(q, 96.0));
+}
+
+}  // namespace
+
+SplitChoice choose_split(SplitKind kind, uint32_t n, const std::vector<uint32_t>& occupancy, uint64_t max_slots) {
+    SplitChoice best;
+    if (n < 2 || occupancy.empty()) return best;
+    const Rates r = rates_of(kind);
+    // nodes by occupancy (0: held by no sample, no cost either way)
+    std::vector<uint32_t> c(occupancy);
+    std::sort(c.begin(), c.end());
+    // cost with every node on the tiles; then move nodes to the posting lists in
+    // increasing occupancy (thr = c_i + 1 moves every node up to index i)
+    double tiles = 0.0;
+    for (uint32_t x : c)
+        if (x) tiles += tile_terms(kind, n, x) * r.term;
+    best.model_ms_none = best.model_ms = tiles * 1e3;
+    double cost = tiles, posts = 0.0;
+    uint64_t slots = 0;
+    for (size_t i = 0; i < c.size();) {
+        size_t j = i;
+        while (j < c.size() && c[j] == c[i]) ++j;   // all nodes of this occupancy move together
+        const double x = c[i];
+        if (x > 0) {
+            const double m = (double)(j - i);
+            cost -= m * tile_terms(kind, n, c[i]) * r.term;
+            posts += m * 0.5 * x * (x - 1.0) * r.posting;
+            slots += (uint64_t)(j - i) * c[i];
+        }
+        if (slots >= max_slots) break;
+        if (cost + posts < best.model_ms * 1e-3) {
+            best.model_ms = (cost + posts) * 1e3;
+            best.thr = c[i] + 1;
+        }
+        i = j;
+    }
+    return best;
+}
+
+}  // namespace ssu

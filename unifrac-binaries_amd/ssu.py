@@ -87,7 +87,7 @@ EXPORTS = [
     "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_execute_stripes", "ssu_plan_copy_stripes",
     "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
     "ssu_plan_copy_rows",
-    "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_sparse_terms", "ssu_plan_refined_pairs", "ssu_last_call_timings",
+    "ssu_plan_timings", "ssu_plan_pair_nodes", "ssu_plan_kernel_terms", "ssu_plan_sparse_terms", "ssu_plan_refined_pairs", "ssu_plan_split_theta", "ssu_last_call_timings", "ssu_release_device_memory", "ssu_choose_split",
     "ssu_plan_main_kernel", "ssu_plan_destroy", "ssu_postorder",
     "ssu_newick_to_bp", "ssu_biom_n_samples", "ssu_device_available", "ssu_last_error",
     "ssu_rarefy_create_inmem", "ssu_rarefy_execute", "ssu_rarefy_n_samples", "ssu_rarefy_n_obs", "ssu_rarefy_nnz",
@@ -115,6 +115,7 @@ def lib():
         L.ssu_plan_kernel_terms.restype = C.c_uint64
         L.ssu_plan_sparse_terms.restype = C.c_uint64
         L.ssu_plan_refined_pairs.restype = C.c_uint64
+        L.ssu_plan_split_theta.restype = C.c_double
         L.ssu_plan_main_kernel.restype = C.c_char_p
         L.ssu_last_error.restype = C.c_char_p
         L.ssu_postorder.restype = C.c_int
@@ -222,6 +223,18 @@ def matrix_inmem(inp: Inputs, method: str, variance_adjust=False, alpha=1.0, byp
     return st, m
 
 
+def choose_split(kind, n_samples, occupancy):
+    """(threshold, modelled ms, modelled ms without a split) of the occupancy split.
+    kind: 0 FP32 weighted, 1 FP64 weighted, 2 FP32 generalized, 3 unweighted Gram."""
+    L = lib()
+    L.ssu_choose_split.restype = C.c_uint
+    occ = np.ascontiguousarray(occupancy, dtype=np.uint32)
+    a, b = C.c_double(), C.c_double()
+    thr = L.ssu_choose_split(C.c_int(kind), C.c_uint(n_samples), occ.ctypes.data_as(C.c_void_p), C.c_uint(occ.size),
+                             C.byref(a), C.byref(b))
+    return int(thr), a.value, b.value
+
+
 def last_call_timings():
     """Wall-clock ms of this thread's last one_off_matrix_inmem* call, by phase."""
     ms = (C.c_double * 6)()
@@ -294,6 +307,9 @@ class Plan:
 
     def refined_pairs(self) -> int:
         return int(lib().ssu_plan_refined_pairs(self.h))
+
+    def split_theta(self) -> float:
+        return float(lib().ssu_plan_split_theta(self.h))
 
     def main_kernel(self) -> str:
         return lib().ssu_plan_main_kernel(self.h).decode()

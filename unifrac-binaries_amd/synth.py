@@ -116,6 +116,29 @@ def caterpillar_tree(n_tips: int, seed: int, mean_len: float = 0.1, shuffle: boo
     return np.array(structure, bool), np.array(lengths), names
 
 
+def contract_tree(tree, frac: float, seed: int):
+    """Delete the parentheses of a random fraction of the internal non-root
+    nodes: their children join the parent, which then has three or more."""
+    st, le, nm = tree
+    st = np.asarray(st, bool)
+    n = st.size
+    partner = np.zeros(n, np.int64)
+    stack = []
+    for i in range(n):
+        if st[i]:
+            stack.append(i)
+        else:
+            o = stack.pop()
+            partner[o], partner[i] = i, o
+    internal = [i for i in range(1, n) if st[i] and i + 1 < n and st[i + 1]]
+    rng = np.random.default_rng(seed)
+    drop = rng.choice(internal, int(frac * len(internal)), replace=False) if internal else []
+    keep = np.ones(n, bool)
+    for o in drop:
+        keep[o] = keep[partner[o]] = False
+    return st[keep], np.asarray(le)[keep], [x for x, k in zip(nm, keep) if k]
+
+
 def bp_to_newick(structure, lengths, names) -> str:
     """Newick text for parentheses arrays, lengths printed with %.9g."""
     out = []
