@@ -459,6 +459,18 @@ SSU_EXTERN void ssu_plan_destroy(ssu_plan_t** plan);
  * numbers follow the ssu_set_random_seed stream. */
 SSU_EXTERN ComputeStatus ssu_center_matrix_fp64(const double* mat, unsigned int n, double* centered);
 SSU_EXTERN ComputeStatus ssu_center_matrix_fp32(const float* mat, unsigned int n, float* centered);
+/* Allocation-free Gower centring: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (work under HIP graph capture,
+ * tight serving loops).  mat and centered are FP64 n x n row-major matrices in
+ * device memory (they may be the same buffer); the work is enqueued on stream
+ * (NULL: the null stream) and the call returns without waiting for it.  The
+ * row statistics live in a fixed device array of the library (512 KiB per
+ * GPU, held until the process ends), so n is at most 65,536 (larger n:
+ * output_error, nothing enqueued), and calls that overlap on one GPU race.
+ * The first call on a device looks the array up (hipGetSymbolAddress): make
+ * it outside a capture.  The result equals ssu_center_matrix_fp64's. */
+SSU_EXTERN ComputeStatus ssu_center_matrix_fp64_noalloc(const double* mat, unsigned int n, double* centered,
+                                                        void* stream);
 SSU_EXTERN ComputeStatus ssu_pcoa_fp64(const double* mat, unsigned int n, unsigned int dims, double* eigenvalues,
                                        double* samples, double* proportion_explained);
 SSU_EXTERN ComputeStatus ssu_pcoa_fp32(const float* mat, unsigned int n, unsigned int dims, float* eigenvalues,

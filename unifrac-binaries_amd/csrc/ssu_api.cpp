@@ -1133,6 +1133,13 @@ ComputeStatus ssu_center_matrix_fp64(const double* mat, unsigned int n, double* 
     return okay;
 }
 
+ComputeStatus ssu_center_matrix_fp64_noalloc(const double* mat, unsigned int n, double* centered, void* stream) {
+    if (!mat || !centered) return output_error;
+    std::string err;
+    if (!ssu::center_matrix_noalloc(mat, n, centered, stream, &err)) { set_error(err); return output_error; }
+    return okay;
+}
+
 ComputeStatus ssu_center_matrix_fp32(const float* mat, unsigned int n, float* centered) {
     if (!mat || !centered) return output_error;
     std::string err;

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <vector>
 
 namespace ssu {
@@ -72,7 +73,7 @@ __global__ void __launch_bounds__(ST_T) k_row_stats(double* __restrict__ rmean, 
     u = block_sum(u, part);
     if (threadIdx.x == 0) {
         rmean[i] = a / n;
-        upper[i] = u;
+        if (upper) upper[i] = u;
     }
 }
 
@@ -514,6 +515,78 @@ bool permanova_t(const void* mat, uint32_t n, const uint32_t* grouping, uint32_t
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------- allocation-free centring
+// The device scratch of center_matrix_noalloc: the row means of one call at
+// [0, n) and its grand mean at [NOALLOC_MAX_N].  Each GPU holds its own copy
+// for the life of the process; every call on a GPU overwrites it, so calls
+// that overlap on one GPU race.
+static __device__ __align__(256) double scratch[NOALLOC_MAX_N + 32];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[SCRATCH_MAX_DEVICES] = {};   // per device: the address of its copy of `scratch`
+
+// the current device's copy of `scratch` (looked up once per device)
+bool scratch_of_device(double** out, std::string* err) {
+    int dev = 0;
+    ST_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+        if (err) *err = "device index " + std::to_string(dev) + " beyond the scratch table";
+        return false;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* ptr = nullptr;
+        ST_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = (double*)ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return true;
+}
+
+// rm[n_max] = sum_i rm[i] / n, the row means added in index order (as center_on_device does on the host)
+__global__ void k_grand_mean(double* __restrict__ rm, uint32_t n) {
+    if (blockIdx.x || threadIdx.x) return;
+    double g = 0.0;
+    for (uint32_t i = 0; i < n; ++i) g += rm[i];
+    rm[NOALLOC_MAX_N] = g / n;
+}
+
+// C_ij = -D_ij^2 / 2 - rmean_i - rmean_j + grand (the arithmetic of k_center_apply), C may be D
+__global__ void __launch_bounds__(ST_T) k_center_copy(double* C, const double* D, const double* __restrict__ rm,
+                                                      uint32_t n) {
+    const size_t q = (size_t)blockIdx.x * ST_T + threadIdx.x;
+    if (q >= (size_t)n * n) return;
+    const uint32_t i = (uint32_t)(q / n), j = (uint32_t)(q % n);
+    const double d = D[q];
+    C[q] = -0.5 * d * d - rm[i] - rm[j] + rm[NOALLOC_MAX_N];
+}
+
+}  // namespace
+
+bool center_matrix_noalloc(const double* d_mat, uint32_t n, double* d_centred, void* stream, std::string* err) {
+    if (n > NOALLOC_MAX_N) {
+        if (err) *err = "ssu_center_matrix_fp64_noalloc: n = " + std::to_string(n) + " exceeds its maximum of " +
+                        std::to_string(NOALLOC_MAX_N) + " samples";
+        return false;
+    }
+    if (n == 0) return true;
+    double* rm = nullptr;
+    if (!scratch_of_device(&rm, err)) return false;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL((k_row_stats<double>), dim3(n), dim3(ST_T), 0, st, rm, nullptr, d_mat, n);
+    ST_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_grand_mean, dim3(1), dim3(64), 0, st, rm, n);
+    ST_HIP(hipGetLastError());
+    const size_t nn = (size_t)n * n;
+    hipLaunchKernelGGL(k_center_copy, dim3((uint32_t)((nn + ST_T - 1) / ST_T)), dim3(ST_T), 0, st, d_centred, d_mat,
+                       rm, n);
+    ST_HIP(hipGetLastError());
+    return true;
+}
 
 bool center_matrix(const void* mat, bool in_fp64, uint32_t n, double* centred, std::string* err) {
     if (n == 0) return true;

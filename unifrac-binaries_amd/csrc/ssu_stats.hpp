@@ -19,6 +19,13 @@ namespace ssu {
 // centred (n x n, row major) from mat; element types double or float
 bool center_matrix(const void* mat, bool in_fp64, uint32_t n, double* centred, std::string* err);
 
+// The same centring of an FP64 matrix already in device memory into another
+// (or the same) device buffer, enqueued on `stream` (nullptr: the null stream)
+// without allocating: its row means live in a fixed device array, so n is at
+// most NOALLOC_MAX_N.  The first call on a device looks the array up.
+constexpr uint32_t NOALLOC_MAX_N = 65536;
+bool center_matrix_noalloc(const double* d_mat, uint32_t n, double* d_centred, void* stream, std::string* err);
+
 // eigenvalues[dims], samples[n * dims] (sample-major: eigenvector * sqrt(eigenvalue)),
 // proportion[dims] = eigenvalue / trace(B).  Dimensions past n are zero.
 bool pcoa(const void* mat, bool in_fp64, uint32_t n, uint32_t dims, uint64_t seed, double* eigenvalues,

@@ -82,7 +82,8 @@ EXPORTS = [
     "ssu_faith_pd_inmem", "write_mat_from_matrix_hdf5_fp64_v2", "write_mat_from_matrix_hdf5_fp32_v2",
     "unifrac_to_file_v3", "unifrac_to_txt_file_v3", "unifrac_to_file_v2", "unifrac_to_file",
     "unifrac_multi_to_file_v3", "unifrac_multi_to_file_v2", "compute_permanova_fp64", "compute_permanova_fp32",
-    "ssu_center_matrix_fp64", "ssu_center_matrix_fp32", "ssu_pcoa_fp64", "ssu_pcoa_fp32", "ssu_permanova_fp64",
+    "ssu_center_matrix_fp64", "ssu_center_matrix_fp32", "ssu_center_matrix_fp64_noalloc", "ssu_pcoa_fp64",
+    "ssu_pcoa_fp32", "ssu_permanova_fp64",
     "ssu_permanova_fp32",
     "ssu_plan_create_inmem", "ssu_plan_execute", "ssu_plan_execute_stripes", "ssu_plan_copy_stripes",
     "ssu_plan_result_bytes", "ssu_plan_copy_matrix",
@@ -126,7 +127,7 @@ def lib():
                   "validate_partial", "merge_partial_to_mmap_matrix", "merge_partial_to_mmap_matrix_fp32"):
             getattr(L, n).restype = C.c_int
         for n in ("compute_permanova_fp64", "compute_permanova_fp32", "ssu_center_matrix_fp64",
-                  "ssu_center_matrix_fp32", "ssu_pcoa_fp64", "ssu_pcoa_fp32", "ssu_permanova_fp64",
+                  "ssu_center_matrix_fp32", "ssu_center_matrix_fp64_noalloc", "ssu_pcoa_fp64", "ssu_pcoa_fp32", "ssu_permanova_fp64",
                   "ssu_permanova_fp32"):
             getattr(L, n).restype = C.c_int
         for n in ("unifrac_to_file_v3", "unifrac_to_txt_file_v3", "unifrac_to_file_v2", "unifrac_to_file",
