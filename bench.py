@@ -56,7 +56,7 @@ def bytes_per_posting(name, n):
 
 
 def gram_units(n, kernel_terms):
-    """(sample pair i <= j, Gram column) units one k_gram_i8 launch processes:
+    """(sample pair i <= j, Gram column) units one k_gram_i8r launch processes:
     kernel_terms counts the padded upper-triangle tiles times the columns."""
     t = (n + GRAM_TILE - 1) // GRAM_TILE
     cols = kernel_terms / (t * (t + 1) // 2 * GRAM_TILE * GRAM_TILE)
@@ -420,7 +420,7 @@ def main():
               for name, k in kernels.items() if k["sparse_terms"]]
     dom_ms, dom, dom_kernel = max(cands)
     dk = kernels[dom]
-    if dom_kernel == "k_gram_i8":
+    if dom_kernel == "k_gram_i8r":
         achieved = OPS_PER_PAIR_NODE_GRAM * gram_units(n, dk["kernel_terms"]) / (dom_ms / 1000.0) / 1e12
         roofline = {"kernel": dom_kernel, "bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS,
                     "unit": "TOP/s", "frac": achieved / PEAK_INT8_TOPS, "traffic": None}
@@ -440,7 +440,7 @@ def main():
         roofline = {"kernel": dom_kernel, "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None}
     for name, k in kernels.items():
-        if k["kernel"] == "k_gram_i8":
+        if k["kernel"] == "k_gram_i8r":
             g_ops = OPS_PER_PAIR_NODE_GRAM * gram_units(n, k["kernel_terms"]) / (k["main_ms"] / 1000.0) / 1e12
             k["mfma_int8_tops"] = g_ops
             k["mfma_frac"] = g_ops / PEAK_INT8_TOPS

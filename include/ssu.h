@@ -431,8 +431,9 @@ SSU_EXTERN uint64_t ssu_plan_refined_pairs(const ssu_plan_t* plan);
 SSU_EXTERN double ssu_plan_split_theta(const ssu_plan_t* plan);
 /* wall-clock milliseconds of the calling thread's last one_off_matrix_inmem*
  * call: inputs, tree (shear / collapse / postorder), plan (uploads, schedules,
- * occupancy splits), execute (device work and refinement), copy (result to the
- * host), total.  Returns the number of values written (at most cap, 6). */
+ * occupancy splits), execute (device work and refinement; with the copy of the
+ * result when the two overlap, copy then 0), copy (result to the host), total.
+ * Returns the number of values written (at most cap, 6). */
 SSU_EXTERN int ssu_last_call_timings(double* ms, int cap);
 /* Device buffers freed by finished calls stay cached in the process (mapping
  * and unmapping multi-gigabyte buffers costs more than the kernels); this

@@ -220,7 +220,7 @@ def test_plain_fp32_unweighted_matches_gram(ssu, O, synth):
     p_plain = ssu.Plan(inp, "unweighted", plain_kernel=1)
     p_gram.execute()
     p_plain.execute()
-    assert p_gram.main_kernel() == "k_gram_i8" and p_plain.main_kernel() == "k_pairs"
+    assert p_gram.main_kernel() == "k_gram_i8r" and p_plain.main_kernel() == "k_pairs"
     ref = oracle_matrix(O, tree, table, "unweighted")
     assert rel_err(p_gram.matrix(), ref) <= 1e-6
     assert rel_err(p_plain.matrix(), ref) <= 1e-5

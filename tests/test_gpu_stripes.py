@@ -148,3 +148,25 @@ def test_partial_files_merge_bit_equal_to_one_off(ssu, synth, tmp_path):
     for dm in mats:
         L.destroy_partial_dyn_mat(C.byref(dm))
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("name,kw", [("weighted_normalized", {}), ("weighted_normalized_fp64", {}),
+                                     ("generalized", {"alpha": 0.5}), ("weighted_unnormalized", {})])
+def test_one_off_banded_copy_equals_plan(ssu, synth, name, kw):
+    """one_off_matrix_inmem* copies each finished band of rows while the rows
+    below compute; with near-duplicate samples refined afterwards (the copy is
+    redone): the matrix equals a resident plan's, bit for bit."""
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_gpu_parity import near_duplicate_table
+    tree, table = near_duplicate_table(synth, n_base=2100, tips=900, seed=23)
+    inp = ssu.Inputs(*tree, *table)
+    say("banded one-off", name, kw)
+    s, m = ssu.matrix_inmem(inp, name, **kw)
+    assert s == ssu.OKAY, ssu.last_error()
+    p = ssu.Plan(inp, name, **kw)
+    p.execute()
+    ref = p.matrix()
+    refined = p.refined_pairs()
+    p.close()
+    say("refined pairs", refined)
+    assert np.array_equal(m, ref)

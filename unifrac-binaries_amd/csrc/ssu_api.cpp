@@ -17,6 +17,7 @@
 #include <random>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ssu_biom.hpp"
@@ -124,6 +125,14 @@ ComputeStatus run_full(const Table& t, const ssu::PostorderNodes& nodes, const C
     if (!plan) { set_error(err); return output_error; }
     g_call_ms[CP_PLAN] = ms_since(t0);
     t0 = Clock::now();
+    const size_t nn = (size_t)t.n_samples * t.n_samples;
+    if (plan->out_fp64() == (sizeof(TOut) == sizeof(double))) {
+        // device work and the copy to the host together, overlapped where the path allows
+        if (!plan->execute_to_host(out, &err)) { set_error(err); return output_error; }
+        g_call_ms[CP_EXECUTE] = ms_since(t0);
+        g_call_ms[CP_COPY] = 0.0;
+        return okay;
+    }
     if (!plan->execute(&err)) { set_error(err); return output_error; }
     g_call_ms[CP_EXECUTE] = ms_since(t0);
     t0 = Clock::now();
@@ -131,10 +140,7 @@ ComputeStatus run_full(const Table& t, const ssu::PostorderNodes& nodes, const C
         Clock::time_point t0;
         ~CopyTime() { g_call_ms[CP_COPY] = ms_since(t0); }
     } copy_time{t0};
-    const size_t nn = (size_t)t.n_samples * t.n_samples;
-    if (plan->out_fp64() == (sizeof(TOut) == sizeof(double))) {
-        if (!plan->copy_matrix(out, &err)) { set_error(err); return output_error; }
-    } else if (plan->out_fp64()) {
+    if (plan->out_fp64()) {
         std::vector<double> tmp(nn);
         if (!plan->copy_matrix(tmp.data(), &err)) { set_error(err); return output_error; }
         for (size_t i = 0; i < nn; ++i) out[i] = (TOut)tmp[i];
@@ -247,20 +253,28 @@ ComputeStatus matrix_inmem(const support_biom_t* td, const support_bptree_t* tr,
         g_call_ms[CP_TOTAL] = ms_since(t0);
         return v;
     }
-    // the table borrowed without an id map; the tree sheared, collapsed and walked in
-    // postorder straight from its arrays (src/api.cpp:90-100 SYNC_TREE_TABLE, VALIDATE_TREE_TABLE)
+    // the tree sheared, collapsed and walked in postorder straight from its arrays
+    // (src/api.cpp:90-100 SYNC_TREE_TABLE, VALIDATE_TREE_TABLE) on a second thread,
+    // while this one borrows the table (sample totals) without an id map
+    ssu::PostorderNodes nodes;
+    uint32_t matched = 0, distinct = 0;
+    bool tree_ok = false;
+    double tree_ms = 0.0;
+    std::thread tree_thread([&] {
+        const auto t1 = Clock::now();
+        ssu::NameIndex idx(td->obs_ids, (uint32_t)td->n_obs);
+        distinct = idx.distinct();
+        tree_ok = tr->n_parens > 0 && ssu::synced_postorder(tr->structure, tr->lengths, tr->names,
+                                                            (uint32_t)tr->n_parens, idx, nodes, matched);
+        tree_ms = ms_since(t1);
+    });
     Table t = Table::borrow(td->obs_ids, td->sample_ids, td->indptr, td->indices, td->data, (uint32_t)td->n_obs,
                             (uint32_t)td->n_samples, false);
     g_call_ms[CP_INPUT] = ms_since(t0);
-    auto t1 = Clock::now();
-    ssu::NameIndex idx(td->obs_ids, (uint32_t)td->n_obs);
-    ssu::PostorderNodes nodes;
-    uint32_t matched = 0;
-    if (tr->n_parens <= 0 ||
-        !ssu::synced_postorder(tr->structure, tr->lengths, tr->names, (uint32_t)tr->n_parens, idx, nodes, matched))
-        return tree_missing;
-    if (matched < idx.distinct()) return table_and_tree_do_not_overlap;
-    g_call_ms[CP_TREE] = ms_since(t1);
+    tree_thread.join();
+    g_call_ms[CP_TREE] = tree_ms;   // (overlapped with the inputs)
+    if (!tree_ok) return tree_missing;
+    if (matched < distinct) return table_and_tree_do_not_overlap;
     if (*result == nullptr && !init_full<TReal, TMat>(*result, t.sample_ids, mmap_dir)) {
         set_error("result allocation failed");
         return output_error;

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <chrono>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -343,6 +344,17 @@ struct PlanImpl {
     bool band_all = true;
     std::vector<uint32_t> run_order;
     const std::vector<uint32_t>& order_used() const { return band_all ? block_order : run_order; }
+    // Output bands (execute_to_host): the tiled kernel runs as OUT_BANDS launches over
+    // consecutive row-panel ranges (the blocks of block_order in each, in order);
+    // after band k every row of band k is final (mirror entries come from the
+    // bands above), so its copy to the host overlaps the bands below.
+    static constexpr uint32_t OUT_BANDS = 8;
+    std::vector<uint32_t> band_row0, band_off;   // rows [band_row0[k], band_row0[k + 1]); blocks [band_off[k], ..)
+    std::vector<hipEvent_t> band_ev;
+    hipStream_t copy_st = nullptr;
+    bool banded_run = false;
+    std::function<bool(std::string*)> after_enqueue;   // host work between enqueue and the final wait
+    bool prepare_bands(std::string* err);
     bool bad_lengths = false;
 
     // host schedule
@@ -380,7 +392,7 @@ struct PlanImpl {
     DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_order, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
         d_faith, d_stripes, d_smask, d_dmask, d_pc, d_pidx, d_pval, d_walked, d_rows, d_rcount, d_roff, d_rbsum,
         d_sp_node, d_sp_len, d_umasks, d_B, d_poff, d_pcnt, d_obsum, d_Ssp, d_out2, d_pspmax, d_urare, d_close,
-        d_ridx, d_order_run;
+        d_ridx, d_order_run, d_order_band;
 
     hipStream_t own_st = nullptr;    // created with the plan; st may point at a caller's stream
     // per-execution inputs and output (nullptr: the plan's own table data / totals / result)
@@ -399,6 +411,9 @@ struct PlanImpl {
     ~PlanImpl() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : band_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (copy_st) (void)hipStreamDestroy(copy_st);
         if (own_st) (void)hipStreamDestroy(own_st);
     }
 
@@ -758,8 +773,29 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     P.n = table.n_samples;
     P.m = nodes.m;
     P.n_obs = table.n_obs;
-    P.nodes = nodes;
     P.ldn = round_up(std::max<uint32_t>(P.n, 1), PAIR_TILE);
+    std::string e2;
+    if (!err) err = &e2;
+    hipError_t he = hipStreamCreateWithFlags(&P.own_st, hipStreamNonBlocking);
+    if (he != hipSuccess) { *err = std::string("hipStreamCreate: ") + hipGetErrorString(he); return nullptr; }
+    P.st = P.own_st;
+    // the CSR goes up on a second thread while this one builds the schedules
+    P.nnz = nz_all;
+    std::string up_err;
+    bool up_ok = true;
+    std::thread uploader([&] {
+        std::vector<uint32_t> ip(table.indptr, table.indptr + table.n_obs + 1);
+        for (auto& x : ip) x -= nz_base;
+        up_ok = upload(P.d_indptr, ip, P.st, &up_err) &&
+                upload_raw(P.d_indices, table.indices + nz_base, nz_all * sizeof(uint32_t), P.st, &up_err) &&
+                upload_raw(P.d_data, table.data + nz_base, nz_all * sizeof(double), P.st, &up_err) &&
+                hipStreamSynchronize(P.st) == hipSuccess;   // `ip` is read until here
+    });
+    struct Join {
+        std::thread& t;
+        ~Join() { if (t.joinable()) t.join(); }
+    } join_uploader{uploader};
+    P.nodes = nodes;
 
     // schedule: leaves, height levels, children lists (postorder, left to right)
     int32_t max_h = 0;
@@ -881,25 +917,14 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         }
     }
 
-    std::string e2;
-    if (!err) err = &e2;
     step("method setup");
-    hipError_t he = hipStreamCreateWithFlags(&P.own_st, hipStreamNonBlocking);
-    if (he != hipSuccess) { *err = std::string("hipStreamCreate: ") + hipGetErrorString(he); return nullptr; }
-    P.st = P.own_st;
     for (auto& e : P.ev)
         if (hipEventCreate(&e) != hipSuccess) { *err = "hipEventCreate failed"; return nullptr; }
 
     P.totals = table.totals;
-    const uint64_t nnz = table.nnz();
-    P.nnz = nnz;
-    const uint32_t base = table.n_obs ? table.indptr[0] : 0;
-    std::vector<uint32_t> ip(table.indptr, table.indptr + table.n_obs + 1);
-    for (auto& x : ip) x -= base;
-    bool ok = upload(P.d_indptr, ip, P.st, err) &&
-              upload_raw(P.d_indices, table.indices + base, nnz * sizeof(uint32_t), P.st, err) &&
-              upload_raw(P.d_data, table.data + base, nnz * sizeof(double), P.st, err) &&
-              upload(P.d_totals, P.totals, P.st, err) && upload(P.d_leaf_k, P.leaf_k, P.st, err) &&
+    uploader.join();
+    if (!up_ok) { *err = up_err; return nullptr; }
+    bool ok = upload(P.d_totals, P.totals, P.st, err) && upload(P.d_leaf_k, P.leaf_k, P.st, err) &&
               upload(P.d_leaf_row, P.leaf_row, P.st, err) && upload(P.d_level_nodes, P.level_nodes, P.st, err) &&
               upload(P.d_child_ptr, P.child_ptr, P.st, err) && upload(P.d_child_idx, P.child_idx, P.st, err) &&
               upload(P.d_path_nodes, P.path_nodes, P.st, err) && upload(P.d_path_light, P.path_light, P.st, err) &&
@@ -1169,9 +1194,17 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         a.order = P.band_all ? P.d_order.as<uint32_t>() : P.d_order_run.as<uint32_t>();
         a.n_blocks = (uint32_t)P.order_used().size();
         if (split) a.dmask = P.d_dmask.as<uint32_t>();   // dense nodes only; adds the posting-list share
-        if (s.fp64) SSU_HIP(launch_pairs_sparse(true, a, P.st));
-        else if (P.gen) SSU_HIP(launch_pairs_gen32(a, (float)s.alpha, split ? P.d_out2.p : nullptr, P.st));
-        else SSU_HIP(launch_pairs_sparse32(a, P.st));
+        const uint32_t nbands = P.banded_run ? (uint32_t)P.band_ev.size() : 1;
+        for (uint32_t k = 0; k < nbands; ++k) {
+            if (P.banded_run) {
+                a.order = P.d_order_band.as<uint32_t>() + P.band_off[k];
+                a.n_blocks = P.band_off[k + 1] - P.band_off[k];
+            }
+            if (s.fp64) SSU_HIP(launch_pairs_sparse(true, a, P.st));
+            else if (P.gen) SSU_HIP(launch_pairs_gen32(a, (float)s.alpha, split ? P.d_out2.p : nullptr, P.st));
+            else SSU_HIP(launch_pairs_sparse32(a, P.st));
+            if (P.banded_run) SSU_HIP(hipEventRecord(P.band_ev[k], P.st));
+        }
         SSU_HIP(P.mark(3));
         SSU_HIP(P.mark(4));
     } else {
@@ -1211,6 +1244,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
                                   P.refine_tau, P.band, P.st));
     SSU_HIP(P.mark(5));
     if (P.quiet) return true;
+    if (P.after_enqueue && !P.after_enqueue(err)) return false;   // overlapping copies (execute_to_host)
     SSU_HIP(hipStreamSynchronize(P.st));
     for (int i = 0; i < PH_TOTAL; ++i) SSU_HIP(hipEventElapsedTime(&P.times[i], P.ev[i], P.ev[i + 1]));
     SSU_HIP(hipEventElapsedTime(&P.times[PH_TOTAL], P.ev[0], P.ev[5]));
@@ -1254,6 +1288,62 @@ bool Plan::refine(const double* d_data, const double* d_totals, void* d_out, con
 }
 
 bool Plan::refines() const { return p_->refine_on; }
+
+// Bands of about equal row counts, in whole block rows (32 FP32 / 64 FP64 samples).
+bool PlanImpl::prepare_bands(std::string* err) {
+    if (!band_ev.empty()) return true;
+    const uint32_t ei = spec.fp64 ? 64 : 32;
+    const uint32_t nbi = (n + ei - 1) / ei;
+    const uint32_t nb = std::min<uint32_t>(OUT_BANDS, nbi);
+    std::vector<uint32_t> cut(nb + 1);
+    for (uint32_t k = 0; k <= nb; ++k) cut[k] = (uint32_t)((uint64_t)nbi * k / nb);
+    std::vector<uint32_t> ord;
+    band_off.assign(1, 0u);
+    band_row0.clear();
+    for (uint32_t k = 0; k < nb; ++k) {
+        for (uint32_t bij : block_order) {
+            const uint32_t bi = bij >> 16;
+            if (bi >= cut[k] && bi < cut[k + 1]) ord.push_back(bij);
+        }
+        band_off.push_back((uint32_t)ord.size());
+        band_row0.push_back(std::min(n, cut[k] * ei));
+    }
+    band_row0.push_back(n);
+    if (!upload(d_order_band, ord.empty() ? std::vector<uint32_t>{0u} : ord, st, err)) return false;
+    SSU_HIP(hipStreamSynchronize(st));   // `ord` is read until here
+    if (!copy_st) SSU_HIP(hipStreamCreateWithFlags(&copy_st, hipStreamNonBlocking));
+    band_ev.assign(nb, nullptr);
+    for (auto& e : band_ev) SSU_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return true;
+}
+
+bool Plan::execute_to_host(void* host, std::string* err) {
+    PlanImpl& P = *p_;
+    // banded copies for the shared-node kernels over the full matrix (the other
+    // paths write their output in one final pass)
+    static const bool off = getenv("SSU_OUT_BANDS") && atoi(getenv("SSU_OUT_BANDS")) == 0;
+    if (off || !P.sparse || !P.band_all || P.n < 4096 || P.block_order.empty())
+        return execute(err) && copy_matrix(host, err);
+    if (!P.prepare_bands(err)) return false;
+    const size_t es = P.spec.fp64 ? sizeof(double) : sizeof(float);
+    P.banded_run = true;
+    P.after_enqueue = [&P, host, es](std::string* e) {
+        for (size_t k = 0; k < P.band_ev.size(); ++k) {
+            if (hipEventSynchronize(P.band_ev[k]) != hipSuccess) { if (e) *e = "band wait failed"; return false; }
+            const size_t r0 = P.band_row0[k], r1 = P.band_row0[k + 1];
+            if (r1 > r0 && !copy_to_host((char*)host + r0 * P.n * es, P.d_out.as<char>() + r0 * P.n * es,
+                                         (r1 - r0) * P.n * es, P.copy_st, e))
+                return false;
+        }
+        return true;
+    };
+    const bool ok = execute(err);
+    P.banded_run = false;
+    P.after_enqueue = nullptr;
+    if (!ok) return false;
+    // refinement rewrote entries after the bands were copied: copy again
+    return P.refined ? copy_matrix(host, err) : true;
+}
 
 bool Plan::set_stripes(uint32_t start, uint32_t stop, std::string* err) {
     PlanImpl& P = *p_;
@@ -1462,7 +1552,7 @@ double Plan::split_theta() const {
     return P.split_thr ? (double)P.split_thr / std::max<uint32_t>(P.n, 1) : 0.0;
 }
 const char* Plan::main_kernel() const {
-    if (p_->gram) return "k_gram_i8";
+    if (p_->gram) return "k_gram_i8r";
     if (p_->sparse) return p_->spec.fp64 ? "k_pairs_sparse" : (p_->gen ? "k_pairs_gen32" : "k_pairs_sparse32");
     return "k_pairs";
 }

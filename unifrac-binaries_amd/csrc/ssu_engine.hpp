@@ -82,6 +82,9 @@ public:
     void set_stream(void* stream);
     void* stream() const;
     bool copy_matrix(void* host, std::string* err) const;                 // n*n elements
+    // execute() and copy_matrix() in one, the copy of each finished band of rows
+    // overlapping the device work on the rows below (shared-node kernels)
+    bool execute_to_host(void* host, std::string* err);
     bool copy_rows(uint32_t row0, uint32_t nrows, void* host, std::string* err) const;   // nrows*n elements
     bool copy_stripes(uint32_t start, uint32_t stop, double* const* host, std::string* err) const;
     // Faith PD over all non-root nodes (needs only the embedding).

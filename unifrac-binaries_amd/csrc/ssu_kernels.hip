@@ -434,18 +434,18 @@ constexpr int GRAM_THREADS = 512;
 //   A operand: presence of the row samples, bytes 0x00 / 0xFF (0 / -1)
 //   B operand: presence of the column samples AND digit byte d_q of the column's length
 // so each MFMA accumulates -sum_c d_q(c) [i present][j present] exactly in int32.
-// Workgroup: 8 waves on a 128x128 output tile.  Wave w owns quadrant w&3
-// (rows ((w&3)>>1)*64, columns (w&1)*64, as 2x2 MFMA blocks of 32x32) and the
-// digit pair p = w>>2 (digits 2p, 2p+1): 128 accumulator registers per wave,
-// two waves per SIMD.  At the end of each K-chunk a wave merges its two
-// digits exactly, 2^(14p) (acc_2p + 2^7 acc_2p+1), and adds it times the
-// chunk scale into the fragment-ordered FP64 tile of G; the p = 1 waves add
-// first, then the p = 0 waves, separated by a barrier.
+// Workgroup: 8 waves on a 128x128 output tile.  Wave w owns rows (w>>2)*64 ..
+// +64 and columns (w&3)*32 .. +32 of the tile (two 32x32 MFMA blocks) for all
+// four digits: 128 accumulator registers, and per K step three operand
+// fragments for eight MFMAs.  At the end of each K-chunk the wave merges its
+// four digits exactly, acc_0 + 2^7 acc_1 + 2^14 acc_2 + 2^21 acc_3 (below
+// 2^45), and adds it times the chunk scale into its FP64 partial of the tile
+// held in registers (one FMA per chunk and element); G is written once, in the
+// fragment order of gram_slot.
 __global__ void __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_gram_i8(double* __restrict__ G, uint32_t tiles, const int8_t* __restrict__ Pt, size_t ldp,
-          const int8_t* __restrict__ digits, const GramChunk* __restrict__ chunks, uint32_t n_chunks, uint32_t n_stages,
-          const uint32_t* __restrict__ order, uint32_t n_tiles, uint32_t per_xcd) {
-    // tiles in XCD-contiguous runs of a super-tile order (see upper_block_order)
+k_gram_i8r(double* __restrict__ G, uint32_t tiles, const int8_t* __restrict__ Pt, size_t ldp,
+           const int8_t* __restrict__ digits, const GramChunk* __restrict__ chunks, uint32_t n_chunks,
+           uint32_t n_stages, const uint32_t* __restrict__ order, uint32_t n_tiles, uint32_t per_xcd) {
     const uint32_t seq = (blockIdx.x % N_XCDS) * per_xcd + blockIdx.x / N_XCDS;
     if (seq >= n_tiles) return;
     const uint32_t bi = order[seq] >> 16, bj = order[seq] & 0xffffu;
@@ -455,12 +455,10 @@ k_gram_i8(double* __restrict__ G, uint32_t tiles, const int8_t* __restrict__ Pt,
 
     const uint32_t t = threadIdx.x;
     const uint32_t lane = t & 63, wave = t >> 6;
-    const uint32_t quad = wave & 3, dp = wave >> 2;
-    const uint32_t wr = quad >> 1, wc = quad & 1;
+    const uint32_t rh = wave >> 2, cq = wave & 3;   // 64 rows, 32 columns
     const size_t Kp = ldp;
 
     static_assert(GRAM_BK == 128 && GRAM_THREADS == 512, "staging below assumes 128-column stages, 512 threads");
-    // thread t stages rows t/8 and t/8 + 64 of both operands, 16-byte column group t%8
     const uint32_t lrow = t >> 3, lc16 = (t & 7) * 16;
     const int8_t* gA0 = Pt + ((size_t)bi * GRAM_BM + lrow) * ldp + lc16;
     const int8_t* gB0 = Pt + ((size_t)bj * GRAM_BM + lrow) * ldp + lc16;
@@ -488,20 +486,20 @@ k_gram_i8(double* __restrict__ G, uint32_t tiles, const int8_t* __restrict__ Pt,
         if (t < 32) *reinterpret_cast<uint4*>(sD0) = rd;                              \
     } while (0)
 
-    v16i acc[2][2][2];  // [digit of the pair][mi][nj]
+    v16i acc[4][2];   // [digit][mi]
 #pragma unroll
-    for (int d = 0; d < 2; ++d)
+    for (int d = 0; d < 4; ++d)
 #pragma unroll
-        for (int mi = 0; mi < 2; ++mi)
+        for (int mi = 0; mi < 2; ++mi) acc[d][mi] = (v16i){0};
+    double gacc[2][16];   // the wave's FP64 partial of G, C-fragment order
 #pragma unroll
-            for (int nj = 0; nj < 2; ++nj) acc[d][mi][nj] = (v16i){0};
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) gacc[mi][r] = 0.0;
 
-    // this lane's slot in the fragment-ordered G tile
-    double* gl = G + ((size_t)bi * tiles + bj) * (GRAM_BM * GRAM_BM) + (size_t)quad * 4096 + lane;
-    const uint32_t arow = wr * 64 + (lane & 31);
-    const uint32_t brow = wc * 64 + (lane & 31);
+    const uint32_t arow = rh * 64 + (lane & 31);
+    const uint32_t brow = cq * 32 + (lane & 31);
     const uint32_t khalf = (lane >> 5) * 16;
-    const int8_t* sDp = sD + (2 * dp) * GRAM_BK;
 
     uint32_t ch = 0;
     uint32_t ch_end_stage = n_chunks ? chunks[0].k1 / GRAM_BK : 0;
@@ -514,46 +512,30 @@ k_gram_i8(double* __restrict__ G, uint32_t tiles, const int8_t* __restrict__ Pt,
             const uint32_t ko = kk * 32 + khalf;
             const v4i a0 = *reinterpret_cast<const v4i*>(sA + arow * GLDA + ko);
             const v4i a1 = *reinterpret_cast<const v4i*>(sA + (arow + 32) * GLDA + ko);
-            const v4i p0 = *reinterpret_cast<const v4i*>(sB + brow * GLDA + ko);
-            const v4i p1 = *reinterpret_cast<const v4i*>(sB + (brow + 32) * GLDA + ko);
+            const v4i pb = *reinterpret_cast<const v4i*>(sB + brow * GLDA + ko);
 #pragma unroll
-            for (int d = 0; d < 2; ++d) {
-                const v4i dw = *reinterpret_cast<const v4i*>(sDp + d * GRAM_BK + ko);
-                const v4i b0 = p0 & dw, b1 = p1 & dw;
-                acc[d][0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, acc[d][0][0], 0, 0, 0);
-                acc[d][0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, acc[d][0][1], 0, 0, 0);
-                acc[d][1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, acc[d][1][0], 0, 0, 0);
-                acc[d][1][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, acc[d][1][1], 0, 0, 0);
+            for (int d = 0; d < 4; ++d) {
+                const v4i b = pb & *reinterpret_cast<const v4i*>(sD + d * GRAM_BK + ko);
+                acc[d][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b, acc[d][0], 0, 0, 0);
+                acc[d][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b, acc[d][1], 0, 0, 0);
             }
         }
         if (s + 1 == ch_end_stage) {
-            // |acc| <= 127 * 2^17, so acc_lo + 2^7 acc_hi and its 2^14 / power-of-two scalings are
-            // exact in FP64; only the add into G rounds.
-            const double sc = -chunks[ch].scale * (dp ? 16384.0 : 1.0);  // A carries -1 for "present"
-            const bool first = (ch == 0);
-            double* gb = gl;
-            asm volatile("" : "+v"(gb));  // keep the 64 slot addresses out of the main loop's registers
-            for (uint32_t turn = 0; turn < 2; ++turn) {
-                if (turn == 1 - dp) {
+            // exact merge of the four digits (|acc| <= 127 * 2^17), then one FMA per element
+            const double sc = -chunks[ch].scale;   // A carries -1 for "present"
 #pragma unroll
-                    for (int mi = 0; mi < 2; ++mi)
+            for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-                        for (int nj = 0; nj < 2; ++nj)
-#pragma unroll
-                            for (int r = 0; r < 16; ++r) {
-                                const double v = fma((double)acc[1][mi][nj][r], 128.0, (double)acc[0][mi][nj][r]);
-                                double* gp = gb + ((mi * 2 + nj) * 16 + r) * 64;
-                                *gp = (first && dp == 1) ? v * sc : fma(v, sc, *gp);
-                            }
+                for (int r = 0; r < 16; ++r) {
+                    double v = fma((double)acc[3][mi][r], 128.0, (double)acc[2][mi][r]);
+                    v = fma(v, 128.0, (double)acc[1][mi][r]);
+                    v = fma(v, 128.0, (double)acc[0][mi][r]);
+                    gacc[mi][r] = fma(v, sc, gacc[mi][r]);
                 }
-                __syncthreads();
-            }
 #pragma unroll
-            for (int d = 0; d < 2; ++d)
+            for (int d = 0; d < 4; ++d)
 #pragma unroll
-                for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-                    for (int nj = 0; nj < 2; ++nj) acc[d][mi][nj] = (v16i){0};
+                for (int mi = 0; mi < 2; ++mi) acc[d][mi] = (v16i){0};
             ++ch;
             if (ch < n_chunks) ch_end_stage = chunks[ch].k1 / GRAM_BK;
         }
@@ -565,6 +547,13 @@ k_gram_i8(double* __restrict__ G, uint32_t tiles, const int8_t* __restrict__ Pt,
     }
 #undef SSU_GLOAD
 #undef SSU_SSTORE
+    // quadrant rh*2 + cq/2, block (mi, cq&1) of the fragment order of gram_slot
+    double* gl = G + ((size_t)bi * tiles + bj) * (GRAM_BM * GRAM_BM) + lane;
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            gl[(size_t)(((((rh * 2 + (cq >> 1)) * 2 + mi) * 2 + (cq & 1)) * 16 + r) * 64)] = gacc[mi][r];
 }
 
 template <class TOut>
@@ -2141,8 +2130,8 @@ hipError_t launch_gram_i8(double* G, const int8_t* Pt, size_t ldp, const int8_t*
     if (tiles > 65535 || n_tiles > tiles * (tiles + 1) / 2 || !order) return hipErrorInvalidValue;
     if (!n_tiles) return hipSuccess;
     const uint32_t per_xcd = (n_tiles + N_XCDS - 1) / N_XCDS;
-    hipLaunchKernelGGL(k_gram_i8, dim3(per_xcd * N_XCDS), dim3(GRAM_THREADS), 0, st, G, tiles, Pt, ldp, digits, chunks,
-                       n_chunks, (uint32_t)(ldp / GRAM_BK), order, n_tiles, per_xcd);
+    hipLaunchKernelGGL(k_gram_i8r, dim3(per_xcd * N_XCDS), dim3(GRAM_THREADS), 0, st, G, tiles, Pt, ldp, digits,
+                       chunks, n_chunks, (uint32_t)(ldp / GRAM_BK), order, n_tiles, per_xcd);
     return hipGetLastError();
 }
 
