@@ -151,11 +151,14 @@ def test_partial_files_merge_bit_equal_to_one_off(ssu, synth, tmp_path):
 
 
 @pytest.mark.parametrize("name,kw", [("weighted_normalized", {}), ("weighted_normalized_fp64", {}),
-                                     ("generalized", {"alpha": 0.5}), ("weighted_unnormalized", {})])
+                                     ("generalized", {"alpha": 0.5}), ("weighted_unnormalized", {}),
+                                     ("unweighted", {}), ("unweighted_fp64", {}), ("unweighted_unnormalized", {})])
 def test_one_off_banded_copy_equals_plan(ssu, synth, name, kw):
     """one_off_matrix_inmem* copies each finished band of rows while the rows
-    below compute; with near-duplicate samples refined afterwards (the copy is
-    redone): the matrix equals a resident plan's, bit for bit."""
+    below compute (weighted: the tiled kernel's bands alternating between two
+    streams; unweighted: Gram tiles, posting walk and combine per band); with
+    near-duplicate samples refined afterwards (the copy is redone): the matrix
+    equals a resident plan's, bit for bit."""
     sys.path.insert(0, os.path.dirname(__file__))
     from test_gpu_parity import near_duplicate_table
     tree, table = near_duplicate_table(synth, n_base=2100, tips=900, seed=23)
@@ -170,3 +173,49 @@ def test_one_off_banded_copy_equals_plan(ssu, synth, name, kw):
     p.close()
     say("refined pairs", refined)
     assert np.array_equal(m, ref)
+
+
+@pytest.mark.parametrize("name,kw", [("weighted_normalized", {}), ("generalized", {"alpha": 0.5})])
+def test_one_off_few_refined_samples_equals_plan(ssu, O, synth, name, kw):
+    """A few near-duplicate samples among many: after the banded copies only
+    the refined samples' square of entries is copied again; the matrix equals
+    a resident plan's, bit for bit, and the refined pairs stay within 1e-6 of
+    the FP64 oracle."""
+    st, le, nm = synth.random_tree(900, 31)
+    n, k = 4200, 40
+    obs, sids, ip, ix, d = synth.random_table(900, n, 0.08, 32)
+    dense = np.zeros((len(obs), n))
+    for r in range(len(obs)):
+        dense[r, ix[ip[r]:ip[r + 1]]] = d[ip[r]:ip[r + 1]]
+    dense[:, n - k:] = dense[:, :k]   # the last k samples repeat the first k, one count raised
+    rng = np.random.default_rng(33)
+    for s in range(n - k, n):
+        dense[rng.choice(np.flatnonzero(dense[:, s])), s] += 1.0
+    rows = [np.flatnonzero(dense[r]) for r in range(len(obs))]
+    keep = [r for r in range(len(obs)) if len(rows[r])]
+    table = ([obs[r] for r in keep], [f"s{i}" for i in range(n)],
+             np.concatenate([[0], np.cumsum([len(rows[r]) for r in keep])]).astype(np.uint32),
+             np.concatenate([rows[r] for r in keep]).astype(np.uint32),
+             np.concatenate([dense[r, rows[r]] for r in keep]))
+    inp = ssu.Inputs(st, le, nm, *table)
+    say("few refined", name, kw)
+    s, m = ssu.matrix_inmem(inp, name, **kw)
+    assert s == ssu.OKAY, ssu.last_error()
+    p = ssu.Plan(inp, name, **kw)
+    p.execute()
+    ref = p.matrix()
+    refined = p.refined_pairs()
+    p.close()
+    say("refined pairs", refined)
+    assert refined >= k and np.array_equal(m, ref)
+    # the near-duplicate pairs against the FP64 oracle, on the sub-table of the 2k samples
+    sel = list(range(k)) + list(range(n - k, n))
+    sub_rows = [np.array([j for j, c in enumerate(sel) if dense[r, c] > 0], dtype=np.uint32) for r in keep]
+    sub = ([obs[r] for r in keep], [f"s{i}" for i in sel],
+           np.concatenate([[0], np.cumsum([len(x) for x in sub_rows])]).astype(np.uint32),
+           np.concatenate(sub_rows).astype(np.uint32),
+           np.concatenate([dense[r, np.array(sel)[sub_rows[i]]] for i, r in enumerate(keep)]))
+    want = O.matrix(O.Tree(st.astype(np.uint8), le, nm), O.Table(*sub), name, **kw)
+    got = m[np.ix_(sel, sel)]
+    off = ~np.eye(2 * k, dtype=bool)
+    assert np.max(np.abs(got - want)[off] / np.maximum(np.abs(want[off]), 1e-300)) <= 1e-6

@@ -143,6 +143,29 @@ def test_postorder_shapes_match_oracle(ssu, O, synth, shape, seed, keep):
         assert np.array_equal(a, b)
 
 
+def test_postorder_large_threaded_matches_oracle(ssu, O, synth):
+    """Trees of 65,536 or more parentheses and tables of 65,536 or more ids
+    take the threaded name index and leaf lookups; repeated ids keep their
+    last row."""
+    st, le, nm = synth.random_tree(70000, 11)
+    rng = np.random.default_rng(11)
+    tipnames = [n for n in nm if n]
+    obs = list(rng.choice(tipnames, 66000, replace=False)) + ["not-a-tip"]
+    tab = O.Table(obs, ["s0"], [0] * (len(obs) + 1), [], [])
+    theirs = O.postorder(O.Tree(st.astype(np.uint8), le, nm), tab)
+    for a, b in zip(ssu.postorder(st, le, nm, obs), theirs):
+        assert np.array_equal(a, b)
+    # an id given twice maps to its last row, as the reference's id index does
+    # (src/biom_inmem.cpp:432-439, map[id] = row in row order)
+    rep = obs + obs[:50]
+    last = {o: i for i, o in enumerate(rep)}
+    mine = ssu.postorder(st, le, nm, rep)
+    for a, b in zip(mine[:3], theirs[:3]):
+        assert np.array_equal(a, b)
+    want = np.array([last[obs[r]] if r >= 0 else -1 for r in theirs[3]])
+    assert np.array_equal(mine[3], want) and (want >= len(obs)).sum() > 0
+
+
 @pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libssu_ref.so")),
                     reason="reference build not present")
 @pytest.mark.parametrize("seed,tips", [(5, 300), (6, 3000)])

@@ -118,10 +118,11 @@ ComputeSpec make_spec(Method m, bool fp64, bool vaw, double alpha, bool bypass, 
 
 // Full n x n matrix of the given element type in host memory.
 template <class TOut>
-ComputeStatus run_full(const Table& t, const ssu::PostorderNodes& nodes, const ComputeSpec& spec, TOut* out) {
+ComputeStatus run_full(const Table& t, const ssu::PostorderNodes& nodes, const ComputeSpec& spec, TOut* out,
+                       const std::shared_ptr<ssu::TableUpload>& pre = nullptr) {
     std::string err;
     auto t0 = Clock::now();
-    std::unique_ptr<Plan> plan = Plan::create(t, nodes, spec, &err);
+    std::unique_ptr<Plan> plan = Plan::create(t, nodes, spec, &err, pre);
     if (!plan) { set_error(err); return output_error; }
     g_call_ms[CP_PLAN] = ms_since(t0);
     t0 = Clock::now();
@@ -253,9 +254,13 @@ ComputeStatus matrix_inmem(const support_biom_t* td, const support_bptree_t* tr,
         g_call_ms[CP_TOTAL] = ms_since(t0);
         return v;
     }
-    // the tree sheared, collapsed and walked in postorder straight from its arrays
-    // (src/api.cpp:90-100 SYNC_TREE_TABLE, VALIDATE_TREE_TABLE) on a second thread,
-    // while this one borrows the table (sample totals) without an id map
+    // the table's CSR on its way to the device from the start; the tree sheared,
+    // collapsed and walked in postorder straight from its arrays (src/api.cpp:90-100
+    // SYNC_TREE_TABLE, VALIDATE_TREE_TABLE) on a second thread, while this one
+    // borrows the table (sample totals) without an id map
+    std::shared_ptr<ssu::TableUpload> pre;
+    if (ssu::device_available(nullptr))
+        pre = ssu::TableUpload::start(td->indptr, td->indices, td->data, (uint32_t)td->n_obs);
     ssu::PostorderNodes nodes;
     uint32_t matched = 0, distinct = 0;
     bool tree_ok = false;
@@ -280,7 +285,7 @@ ComputeStatus matrix_inmem(const support_biom_t* td, const support_bptree_t* tr,
         return output_error;
     }
     const ComputeStatus v =
-        run_full<TReal>(t, nodes, make_spec(m, fp64, vaw, alpha, bypass, normalize), (*result)->matrix);
+        run_full<TReal>(t, nodes, make_spec(m, fp64, vaw, alpha, bypass, normalize), (*result)->matrix, pre);
     g_call_ms[CP_TOTAL] = ms_since(t0);
     return v;
 }

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <functional>
 #include <map>
@@ -40,7 +41,17 @@ constexpr double GSPLIT_BUDGET = 8.0 * (1u << 30);
 struct DevMem {
     void* p = nullptr;
     size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
     ~DevMem() { release(); }
+    void take(DevMem& o) {   // o's memory, o left empty
+        release();
+        p = o.p;
+        bytes = o.bytes;
+        o.p = nullptr;
+        o.bytes = 0;
+    }
     void release() {
         if (p) pool_free(p, bytes);
         p = nullptr;
@@ -169,19 +180,32 @@ void pool_release_all() {
     pool_trim_locked(P, 0);
 }
 
-// Pinned staging for device-to-host copies: two chunks, allocated on first use
+// Pinned staging for host copies: STAGE_BUFS chunks, allocated on first use
 // and kept for the life of the process (pinned allocations are slow to make).
 namespace {
 constexpr size_t STAGE_CHUNK = size_t(64) << 20;
+constexpr int STAGE_BUFS = 4;
 struct Stage {
     std::mutex mu;
-    void* buf[2] = {nullptr, nullptr};
-    hipEvent_t done[2] = {nullptr, nullptr};
+    void* buf[STAGE_BUFS] = {};
+    hipEvent_t done[STAGE_BUFS] = {};
     bool failed = false;
 };
 Stage& stage() {
     static Stage* s = new Stage();   // never destroyed: the runtime may already be gone at exit
     return *s;
+}
+// the staging buffers, made on first use (false: pinned memory unavailable)
+bool stage_ready(Stage& S) {
+    if (S.failed) return false;
+    if (S.buf[0]) return true;
+    for (int b = 0; b < STAGE_BUFS; ++b)
+        if (hipHostMalloc(&S.buf[b], STAGE_CHUNK, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&S.done[b], hipEventDisableTiming) != hipSuccess) {
+            S.failed = true;   // plain copies from now on
+            return false;
+        }
+    return true;
 }
 
 // threads for a host pass over n elements (one below a million elements)
@@ -202,59 +226,126 @@ void parallel_chunks(uint64_t n, size_t parts, F fn) {
     for (auto& th : pool) th.join();
 }
 
-// dst[0..bytes) = src[0..bytes), split over up to `threads` threads
-void spread_copy(char* dst, const char* src, size_t bytes, unsigned threads) {
-    if (threads <= 1 || bytes < (size_t(4) << 20)) { std::memcpy(dst, src, bytes); return; }
-    std::vector<std::thread> pool;
-    const size_t part = (bytes / threads + 4095) / 4096 * 4096;
-    for (unsigned t = 1; t < threads; ++t) {
-        const size_t a = part * t;
-        if (a >= bytes) break;
-        pool.emplace_back([=] { std::memcpy(dst + a, src + a, std::min(part, bytes - a)); });
+// Host memcpy over a persistent pool of worker threads (started on first use,
+// up to 16): the calling thread takes one part and waits for the rest.
+class CopyPool {
+public:
+    static CopyPool& get() {
+        static CopyPool* p = new CopyPool();   // never joined: the workers wait for the process to end
+        return *p;
     }
-    std::memcpy(dst, src, std::min(part, bytes));
-    for (auto& th : pool) th.join();
-}
+    void copy(char* dst, const char* src, size_t bytes) {
+        if (workers_ == 0 || bytes < (size_t(4) << 20)) { std::memcpy(dst, src, bytes); return; }
+        std::lock_guard<std::mutex> one(call_mu_);   // one copy at a time on the pool
+        std::unique_lock<std::mutex> lk(mu_);
+        const unsigned parts = workers_ + 1;
+        part_ = (bytes / parts + 4095) / 4096 * 4096;
+        dst_ = dst;
+        src_ = src;
+        bytes_ = bytes;
+        pending_ = workers_;
+        ++gen_;
+        cv_.notify_all();
+        lk.unlock();
+        const size_t a = part_ * workers_;   // the caller's part: the last
+        if (a < bytes) std::memcpy(dst + a, src + a, bytes - a);
+        lk.lock();
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+private:
+    CopyPool() {
+        const unsigned hc = std::thread::hardware_concurrency();
+        workers_ = std::min(16u, std::max(hc, 1u)) - 1;
+        for (unsigned w = 0; w < workers_; ++w) std::thread([this, w] { run(w); }).detach();
+    }
+    void run(unsigned w) {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&] { return gen_ != seen; });
+            seen = gen_;
+            char* dst = dst_;
+            const char* src = src_;
+            const size_t a = part_ * w, bytes = bytes_, part = part_;
+            lk.unlock();
+            if (a < bytes) std::memcpy(dst + a, src + a, std::min(part, bytes - a));
+            lk.lock();
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_;
+    unsigned workers_ = 0, pending_ = 0;
+    uint64_t gen_ = 0;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t bytes_ = 0, part_ = 0;
+};
+
+// dst[0..bytes) = src[0..bytes) on the copy pool
+void spread_copy(char* dst, const char* src, size_t bytes, unsigned) { CopyPool::get().copy(dst, src, bytes); }
 }  // namespace
 
-bool copy_to_host(void* host, const void* dev, size_t bytes, void* stream, std::string* err) {
-    if (!bytes) return true;
+bool copy_ranges_to_host(char* host, const char* dev, const std::vector<CopyRange>& ranges, void* stream,
+                         std::string* err) {
     hipStream_t st = (hipStream_t)stream;
+    size_t bytes = 0;
+    for (const CopyRange& r : ranges) bytes += r.len;
+    if (!bytes) return true;
     Stage& S = stage();
     std::unique_lock<std::mutex> lock(S.mu);
-    if (bytes <= STAGE_CHUNK / 4 || S.failed) {
+    if (bytes <= STAGE_CHUNK / 4 || !stage_ready(S)) {
         lock.unlock();
-        SSU_HIP(hipStreamSynchronize(st));
-        SSU_HIP(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+        for (const CopyRange& r : ranges) {
+            for (void* e : r.ready) SSU_HIP(hipEventSynchronize((hipEvent_t)e));
+            SSU_HIP(hipStreamSynchronize(st));
+            if (r.len) SSU_HIP(hipMemcpy(host + r.off, dev + r.off, r.len, hipMemcpyDeviceToHost));
+        }
         return true;
     }
-    if (!S.buf[0]) {
-        for (int b = 0; b < 2; ++b)
-            if (hipHostMalloc(&S.buf[b], STAGE_CHUNK, hipHostMallocDefault) != hipSuccess ||
-                hipEventCreateWithFlags(&S.done[b], hipEventDisableTiming) != hipSuccess) {
-                S.failed = true;   // no pinned memory: plain copies from now on
-                lock.unlock();
-                SSU_HIP(hipStreamSynchronize(st));
-                SSU_HIP(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
-                return true;
-            }
-    }
-    const unsigned threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-    const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
-    auto len_of = [&](size_t c) { return std::min(STAGE_CHUNK, bytes - c * STAGE_CHUNK); };
-    SSU_HIP(hipMemcpyAsync(S.buf[0], dev, len_of(0), hipMemcpyDeviceToHost, st));
-    SSU_HIP(hipEventRecord(S.done[0], st));
-    for (size_t c = 0; c < nch; ++c) {
-        const int b = (int)(c & 1);
-        if (c + 1 < nch) {   // the other buffer was emptied in the previous iteration
-            SSU_HIP(hipMemcpyAsync(S.buf[b ^ 1], (const char*)dev + (c + 1) * STAGE_CHUNK, len_of(c + 1),
-                                   hipMemcpyDeviceToHost, st));
-            SSU_HIP(hipEventRecord(S.done[b ^ 1], st));
+    // chunks of at most STAGE_CHUNK bytes, each within one range
+    struct Piece { size_t off, len; uint32_t range; };
+    std::vector<Piece> pieces;
+    for (uint32_t r = 0; r < ranges.size(); ++r)
+        for (size_t o = 0; o < ranges[r].len; o += STAGE_CHUNK)
+            pieces.push_back({ranges[r].off + o, std::min(STAGE_CHUNK, ranges[r].len - o), r});
+    static const bool timing = getenv("SSU_TIMING") && atoi(getenv("SSU_TIMING")) != 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    double wait_ms = 0.0, host_ms = 0.0;
+    auto since = [](std::chrono::steady_clock::time_point a) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+    };
+    // a piece's copy waits on the device for its range's events, then fills buffer c % STAGE_BUFS
+    auto issue = [&](size_t c) -> hipError_t {
+        const Piece& q = pieces[c];
+        const int b = (int)(c % STAGE_BUFS);
+        for (void* e : ranges[q.range].ready) {
+            const hipError_t x = hipStreamWaitEvent(st, (hipEvent_t)e, 0);
+            if (x != hipSuccess) return x;
         }
+        hipError_t x = hipMemcpyAsync(S.buf[b], dev + q.off, q.len, hipMemcpyDeviceToHost, st);
+        return x != hipSuccess ? x : hipEventRecord(S.done[b], st);
+    };
+    const size_t np = pieces.size();
+    for (size_t c = 0; c < std::min<size_t>(np, STAGE_BUFS); ++c) SSU_HIP(issue(c));
+    for (size_t c = 0; c < np; ++c) {
+        const int b = (int)(c % STAGE_BUFS);
+        auto tw = std::chrono::steady_clock::now();
         SSU_HIP(hipEventSynchronize(S.done[b]));
-        spread_copy((char*)host + c * STAGE_CHUNK, (const char*)S.buf[b], len_of(c), threads);
+        wait_ms += since(tw);
+        tw = std::chrono::steady_clock::now();
+        spread_copy(host + pieces[c].off, (const char*)S.buf[b], pieces[c].len, 0);
+        host_ms += since(tw);
+        if (c + STAGE_BUFS < np) SSU_HIP(issue(c + STAGE_BUFS));   // into the buffer just emptied
     }
+    if (timing)
+        fprintf(stderr, "[ssu copy] %8.1f MB in %zu pieces, %7.2f ms (waiting %7.2f ms, host copies %7.2f ms)\n",
+                bytes / 1e6, np, since(t0), wait_ms, host_ms);
     return true;
+}
+
+bool copy_to_host(void* host, const void* dev, size_t bytes, void* stream, std::string* err) {
+    return copy_ranges_to_host((char*)host, (const char*)dev, {CopyRange{0, bytes, {}}}, stream, err);
 }
 
 bool copy_to_device(void* dev, const void* host, size_t bytes, void* stream, std::string* err) {
@@ -262,36 +353,24 @@ bool copy_to_device(void* dev, const void* host, size_t bytes, void* stream, std
     hipStream_t st = (hipStream_t)stream;
     Stage& S = stage();
     std::unique_lock<std::mutex> lock(S.mu);
-    if (bytes <= STAGE_CHUNK / 4 || S.failed) {
+    if (bytes <= STAGE_CHUNK / 4 || !stage_ready(S)) {
         lock.unlock();
         SSU_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st));
         SSU_HIP(hipStreamSynchronize(st));   // the caller's buffer may go once this returns
         return true;
     }
-    if (!S.buf[0]) {
-        for (int b = 0; b < 2; ++b)
-            if (hipHostMalloc(&S.buf[b], STAGE_CHUNK, hipHostMallocDefault) != hipSuccess ||
-                hipEventCreateWithFlags(&S.done[b], hipEventDisableTiming) != hipSuccess) {
-                S.failed = true;
-                lock.unlock();
-                SSU_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st));
-                SSU_HIP(hipStreamSynchronize(st));
-                return true;
-            }
-    }
-    const unsigned threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
     const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
-    bool used[2] = {false, false};
+    bool used[STAGE_BUFS] = {};
     for (size_t c = 0; c < nch; ++c) {
-        const int b = (int)(c & 1);
+        const int b = (int)(c % STAGE_BUFS);
         const size_t len = std::min(STAGE_CHUNK, bytes - c * STAGE_CHUNK);
         if (used[b]) SSU_HIP(hipEventSynchronize(S.done[b]));   // its previous chunk has left for the device
-        spread_copy((char*)S.buf[b], (const char*)host + c * STAGE_CHUNK, len, threads);
+        spread_copy((char*)S.buf[b], (const char*)host + c * STAGE_CHUNK, len, 0);
         SSU_HIP(hipMemcpyAsync((char*)dev + c * STAGE_CHUNK, S.buf[b], len, hipMemcpyHostToDevice, st));
         SSU_HIP(hipEventRecord(S.done[b], st));
         used[b] = true;
     }
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < STAGE_BUFS; ++b)
         if (used[b]) SSU_HIP(hipEventSynchronize(S.done[b]));
     return true;
 }
@@ -315,6 +394,7 @@ struct PlanImpl {
     bool refine_on = false, refine_S = false;
     float refine_tau = 0.0f;
     uint64_t refined = 0;
+    std::vector<uint32_t> refined_samples;   // the samples of the last refinement (its pairs lie among them)
     size_t ldn = 0;
     size_t blk = 0;                  // sample columns per embedding block (ldn: one block)
     bool gram = false;
@@ -345,13 +425,22 @@ struct PlanImpl {
     std::vector<uint32_t> run_order;
     const std::vector<uint32_t>& order_used() const { return band_all ? block_order : run_order; }
     // Output bands (execute_to_host): the tiled kernel runs as OUT_BANDS launches over
-    // consecutive row-panel ranges (the blocks of block_order in each, in order);
+    // consecutive row-panel ranges (the blocks of block_order in each, in order;
+    // the first two and the last two bands narrower, so that copies start early
+    // and little is left to copy after the last band; SSU_OUT_BANDS=<k> sets the
+    // count, 0 turns bands off);
     // after band k every row of band k is final (mirror entries come from the
-    // bands above), so its copy to the host overlaps the bands below.
-    static constexpr uint32_t OUT_BANDS = 8;
+    // bands above), so its copy to the host overlaps the bands below.  The bands
+    // alternate between st and band_st2, so that one band's last blocks share the
+    // device with the next band's first.  Unweighted: band k is the Gram tiles of
+    // tile rows in the band (band 0 also every diagonal tile), the posting walk of
+    // its rows (after band 0's tiles: the diagonal G(a, a) of every row) and the
+    // combine of its rows, in order on st.
+    static constexpr uint32_t OUT_BANDS = 10;
     std::vector<uint32_t> band_row0, band_off;   // rows [band_row0[k], band_row0[k + 1]); blocks [band_off[k], ..)
     std::vector<hipEvent_t> band_ev;
-    hipStream_t copy_st = nullptr;
+    hipEvent_t band_go = nullptr;                 // the bands' inputs are ready (band_st2 waits for it)
+    hipStream_t copy_st = nullptr, band_st2 = nullptr;
     bool banded_run = false;
     std::function<bool(std::string*)> after_enqueue;   // host work between enqueue and the final wait
     bool prepare_bands(std::string* err);
@@ -413,7 +502,9 @@ struct PlanImpl {
             if (e) (void)hipEventDestroy(e);
         for (auto& e : band_ev)
             if (e) (void)hipEventDestroy(e);
+        if (band_go) (void)hipEventDestroy(band_go);
         if (copy_st) (void)hipStreamDestroy(copy_st);
+        if (band_st2) (void)hipStreamDestroy(band_st2);
         if (own_st) (void)hipStreamDestroy(own_st);
     }
 
@@ -729,11 +820,48 @@ void PlanImpl::build_paths(const PostorderNodes& nodes) {
     }
 }
 
+struct TableUpload::Impl {
+    std::thread th;
+    DevMem ip, ix, d;
+    bool ok = false;
+    std::string err;
+    const uint32_t* indptr = nullptr;
+    uint32_t n_obs = 0;
+};
+
+std::shared_ptr<TableUpload> TableUpload::start(const uint32_t* indptr, const uint32_t* indices, const double* data,
+                                                uint32_t n_obs) {
+    auto u = std::make_shared<TableUpload>();
+    u->impl.reset(new Impl());
+    Impl* I = u->impl.get();
+    I->indptr = indptr;
+    I->n_obs = n_obs;
+    I->th = std::thread([I, indptr, indices, data, n_obs] {
+        const uint32_t base = n_obs ? indptr[0] : 0;
+        if (n_obs && indptr[n_obs] < base) { I->err = "table indptr is not monotone"; return; }
+        const uint64_t nz = n_obs ? (uint64_t)indptr[n_obs] - base : 0;
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) { I->err = "hipStreamCreate failed"; return; }
+        std::vector<uint32_t> ip(indptr, indptr + n_obs + 1);
+        for (auto& x : ip) x -= base;
+        I->ok = upload(I->ip, ip, st, &I->err) &&
+                upload_raw(I->ix, indices + base, nz * sizeof(uint32_t), st, &I->err) &&
+                upload_raw(I->d, data + base, nz * sizeof(double), st, &I->err) &&
+                hipStreamSynchronize(st) == hipSuccess;   // `ip` is read until here
+        (void)hipStreamDestroy(st);
+    });
+    return u;
+}
+
+TableUpload::~TableUpload() {
+    if (impl && impl->th.joinable()) impl->th.join();
+}
+
 Plan::Plan() : p_(new PlanImpl) {}
 Plan::~Plan() = default;
 
 std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nodes, const ComputeSpec& spec,
-                                   std::string* err) {
+                                   std::string* err, const std::shared_ptr<TableUpload>& pre) {
     if (!device_available(err)) return nullptr;
     // SSU_TIMING=1: wall-clock steps of the plan's creation on stderr
     const bool timing = getenv("SSU_TIMING") && atoi(getenv("SSU_TIMING")) != 0;
@@ -780,10 +908,24 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     if (he != hipSuccess) { *err = std::string("hipStreamCreate: ") + hipGetErrorString(he); return nullptr; }
     P.st = P.own_st;
     // the CSR goes up on a second thread while this one builds the schedules
+    // (or is already on its way, started by the caller on the same arrays)
     P.nnz = nz_all;
     std::string up_err;
     bool up_ok = true;
+    const bool pre_ok = pre && pre->impl && pre->impl->indptr == table.indptr && pre->impl->n_obs == table.n_obs;
     std::thread uploader([&] {
+        if (pre_ok) {
+            TableUpload::Impl& U = *pre->impl;
+            if (U.th.joinable()) U.th.join();
+            up_ok = U.ok;
+            up_err = U.err;
+            if (up_ok) {
+                P.d_indptr.take(U.ip);
+                P.d_indices.take(U.ix);
+                P.d_data.take(U.d);
+            }
+            return;
+        }
         std::vector<uint32_t> ip(table.indptr, table.indptr + table.n_obs + 1);
         for (auto& x : ip) x -= nz_base;
         up_ok = upload(P.d_indptr, ip, P.st, &up_err) &&
@@ -795,22 +937,40 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         std::thread& t;
         ~Join() { if (t.joinable()) t.join(); }
     } join_uploader{uploader};
-    P.nodes = nodes;
-
-    // schedule: leaves, height levels, children lists (postorder, left to right)
-    int32_t max_h = 0;
-    for (uint32_t k = 0; k < P.m; ++k) {
-        if (nodes.is_leaf[k]) { P.leaf_k.push_back((int32_t)k); P.leaf_row.push_back(nodes.obs_row[k]); }
-        max_h = std::max(max_h, nodes.height[k]);
-    }
-    std::vector<std::vector<int32_t>> by_level(max_h + 1);
-    for (uint32_t k = 0; k < P.m; ++k)
-        if (!nodes.is_leaf[k]) by_level[nodes.height[k]].push_back((int32_t)k);
-    P.level_off.push_back(0);
-    for (int32_t h = 1; h <= max_h; ++h) {
-        P.level_nodes.insert(P.level_nodes.end(), by_level[h].begin(), by_level[h].end());
-        P.level_off.push_back((uint32_t)P.level_nodes.size());
-    }
+    // schedule: leaves, height levels and the accumulated nodes on a second
+    // thread; children lists (postorder, left to right) and heavy paths here
+    std::thread levels([&P, &nodes, &spec] {
+        P.nodes = nodes;
+        int32_t max_h = 0;
+        for (uint32_t k = 0; k < P.m; ++k) {
+            if (nodes.is_leaf[k]) { P.leaf_k.push_back((int32_t)k); P.leaf_row.push_back(nodes.obs_row[k]); }
+            max_h = std::max(max_h, nodes.height[k]);
+        }
+        // internal nodes by height: a counting sort, postorder within a level
+        std::vector<uint32_t> cnt(max_h + 2, 0);
+        for (uint32_t k = 0; k < P.m; ++k)
+            if (!nodes.is_leaf[k] && nodes.height[k] >= 1) ++cnt[nodes.height[k] + 1];
+        for (int32_t h = 1; h <= max_h; ++h) cnt[h + 1] += cnt[h];
+        P.level_nodes.assign(cnt[max_h + 1], 0);
+        P.level_off.assign(std::max(max_h, 0) + 1, 0u);
+        for (int32_t h = 1; h <= max_h; ++h) P.level_off[h] = cnt[h + 1];
+        for (uint32_t k = 0; k < P.m; ++k)
+            if (!nodes.is_leaf[k] && nodes.height[k] >= 1) P.level_nodes[cnt[nodes.height[k]]++] = (int32_t)k;
+        P.len_all.assign(nodes.length.begin(), nodes.length.begin() + P.m);
+        P.knodes.reserve(P.m);
+        P.klen_d.reserve(P.m);
+        P.klen_f.reserve(P.m);
+        for (uint32_t k = 0; k < P.m; ++k) {
+            if (spec.bypass_tips && nodes.is_leaf[k]) continue;  // src/unifrac_cmp.cpp:123-124
+            P.knodes.push_back((int32_t)k);
+            P.klen_d.push_back(nodes.length[k]);
+            P.klen_f.push_back((float)nodes.length[k]);          // TFloat lengths, src/unifrac_cmp.cpp:127
+        }
+    });
+    struct JoinLevels {
+        std::thread& t;
+        ~JoinLevels() { if (t.joinable()) t.join(); }
+    } join_levels{levels};
     std::vector<int32_t> n_child(P.m + 1, 0);
     for (uint32_t k = 0; k < P.m; ++k)
         if ((uint32_t)nodes.parent[k] < P.m) n_child[nodes.parent[k]]++;
@@ -821,18 +981,12 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     for (uint32_t k = 0; k < P.m; ++k)
         if ((uint32_t)nodes.parent[k] < P.m) P.child_idx[fill[nodes.parent[k]]++] = (int32_t)k;
 
-    step("validate, levels, children");
-    // heavy paths when they take fewer launches than the height levels (SSU_EMBED_PATHS=0 / 1 forces)
+    step("validate, children");
     P.build_paths(nodes);
+    levels.join();
+    // heavy paths when they take fewer launches than the height levels (SSU_EMBED_PATHS=0 / 1 forces)
     P.use_paths = P.round_off.size() < P.level_off.size();
     if (const char* env = getenv("SSU_EMBED_PATHS")) P.use_paths = atoi(env) != 0;
-    for (uint32_t k = 0; k < P.m; ++k) {
-        P.len_all.push_back(nodes.length[k]);
-        if (spec.bypass_tips && nodes.is_leaf[k]) continue;  // src/unifrac_cmp.cpp:123-124
-        P.knodes.push_back((int32_t)k);
-        P.klen_d.push_back(nodes.length[k]);
-        P.klen_f.push_back((float)nodes.length[k]);          // TFloat lengths, src/unifrac_cmp.cpp:127
-    }
 
     step("schedule");
     const bool unw = spec.method == Method::unweighted || spec.method == Method::unweighted_unnormalized;
@@ -1153,7 +1307,34 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
     }
     SSU_HIP(P.mark(2));
 
-    if (P.gram) {
+    const bool norm = s.method == Method::unweighted;
+    auto combine = [&](uint32_t r0, uint32_t r1) {
+        return s.fp64 ? launch_unweighted_combine<double>((double*)P.out_ptr(), P.d_G.as<double>(), P.n_pad, P.n,
+                                                          norm, P.st, r0, r1)
+                      : launch_unweighted_combine<float>((float*)P.out_ptr(), P.d_G.as<double>(), P.n_pad, P.n, norm,
+                                                         P.st, r0, r1);
+    };
+    if (P.gram && P.banded_run) {
+        // bands in row order: the Gram tiles of the band's tile rows (band 0: every
+        // diagonal tile too), then (band 0) the diagonal shares of every row, the
+        // band's rows' posting walk and its rows of the output
+        for (uint32_t k = 0; k + 1 < (uint32_t)P.band_off.size(); ++k) {
+            const uint32_t r0 = P.band_row0[k], r1 = P.band_row0[k + 1];
+            SSU_HIP(launch_gram_i8(P.d_G.as<double>(), P.d_Pt.as<int8_t>(), P.Kp, P.d_digits.as<int8_t>(),
+                                   P.d_chunks.as<GramChunk>(), (uint32_t)P.chunks.size(), (uint32_t)P.n_pad,
+                                   P.d_order_band.as<uint32_t>() + P.band_off[k], P.band_off[k + 1] - P.band_off[k],
+                                   P.st));
+            if (gsplit) {
+                if (k == 0) SSU_HIP(launch_sparse_rows(gb, P.st, 0, P.n, ROWS_DIAGONAL));
+                SSU_HIP(launch_sparse_rows(gb, P.st, r0, r1, ROWS_OFF_DIAGONAL));
+            }
+            SSU_HIP(combine(r0, r1));
+            SSU_HIP(hipEventRecord(P.band_ev[k], P.st));
+        }
+        SSU_HIP(P.mark(3));
+        if (gsplit) SSU_HIP(P.mark(EV_ROWS));
+        SSU_HIP(P.mark(4));
+    } else if (P.gram) {
         if (P.Kp) {
             SSU_HIP(launch_gram_i8(P.d_G.as<double>(), P.d_Pt.as<int8_t>(), P.Kp, P.d_digits.as<int8_t>(),
                                    P.d_chunks.as<GramChunk>(), (uint32_t)P.chunks.size(), (uint32_t)P.n_pad,
@@ -1167,13 +1348,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             SSU_HIP(launch_sparse_rows(gb, P.st));   // adds the sparse nodes' share to G
             SSU_HIP(P.mark(EV_ROWS));
         }
-        const bool norm = s.method == Method::unweighted;
-        if (s.fp64)
-            SSU_HIP(launch_unweighted_combine<double>((double*)P.out_ptr(), P.d_G.as<double>(), P.n_pad, P.n, norm,
-                                                      P.st));
-        else
-            SSU_HIP(launch_unweighted_combine<float>((float*)P.out_ptr(), P.d_G.as<double>(), P.n_pad, P.n, norm,
-                                                     P.st));
+        SSU_HIP(combine(0, P.n));
         SSU_HIP(P.mark(4));
     } else if (P.sparse) {
         SparseArgs a{};
@@ -1195,16 +1370,24 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         a.n_blocks = (uint32_t)P.order_used().size();
         if (split) a.dmask = P.d_dmask.as<uint32_t>();   // dense nodes only; adds the posting-list share
         const uint32_t nbands = P.banded_run ? (uint32_t)P.band_ev.size() : 1;
+        if (P.banded_run && nbands > 1) {
+            SSU_HIP(hipEventRecord(P.band_go, P.st));
+            SSU_HIP(hipStreamWaitEvent(P.band_st2, P.band_go, 0));
+        }
         for (uint32_t k = 0; k < nbands; ++k) {
+            hipStream_t bs = P.st;
             if (P.banded_run) {
                 a.order = P.d_order_band.as<uint32_t>() + P.band_off[k];
                 a.n_blocks = P.band_off[k + 1] - P.band_off[k];
+                if (k & 1) bs = P.band_st2;
             }
-            if (s.fp64) SSU_HIP(launch_pairs_sparse(true, a, P.st));
-            else if (P.gen) SSU_HIP(launch_pairs_gen32(a, (float)s.alpha, split ? P.d_out2.p : nullptr, P.st));
-            else SSU_HIP(launch_pairs_sparse32(a, P.st));
-            if (P.banded_run) SSU_HIP(hipEventRecord(P.band_ev[k], P.st));
+            if (s.fp64) SSU_HIP(launch_pairs_sparse(true, a, bs));
+            else if (P.gen) SSU_HIP(launch_pairs_gen32(a, (float)s.alpha, split ? P.d_out2.p : nullptr, bs));
+            else SSU_HIP(launch_pairs_sparse32(a, bs));
+            if (P.banded_run) SSU_HIP(hipEventRecord(P.band_ev[k], bs));
         }
+        if (P.banded_run && nbands > 1)   // the later work on st reads every band
+            SSU_HIP(hipStreamWaitEvent(P.st, P.band_ev[(nbands - 2) | 1], 0));   // the last band on band_st2
         SSU_HIP(P.mark(3));
         SSU_HIP(P.mark(4));
     } else {
@@ -1289,21 +1472,32 @@ bool Plan::refine(const double* d_data, const double* d_totals, void* d_out, con
 
 bool Plan::refines() const { return p_->refine_on; }
 
-// Bands of about equal row counts, in whole block rows (32 FP32 / 64 FP64 samples).
+// Bands of about equal row counts, in whole block rows (32 FP32 / 64 FP64
+// samples; Gram: 128-sample tile rows, band 0 holding every diagonal tile).
 bool PlanImpl::prepare_bands(std::string* err) {
     if (!band_ev.empty()) return true;
-    const uint32_t ei = spec.fp64 ? 64 : 32;
+    const uint32_t ei = gram ? GRAM_BM : (spec.fp64 ? 64 : 32);
     const uint32_t nbi = (n + ei - 1) / ei;
-    const uint32_t nb = std::min<uint32_t>(OUT_BANDS, nbi);
+    uint32_t want = OUT_BANDS;
+    if (const char* env = getenv("SSU_OUT_BANDS")) want = std::max(1, atoi(env));
+    const uint32_t nb = std::min<uint32_t>(want, nbi);
+    // row shares 1 : 2 : 3 ... 3 : 2 : 1 (fewer than five bands: equal)
+    std::vector<double> wsum(nb + 1, 0.0);
+    for (uint32_t k = 0; k < nb; ++k) {
+        const uint32_t edge = std::min(k, nb - 1 - k);
+        wsum[k + 1] = wsum[k] + (nb < 5 ? 1.0 : std::min<double>(edge + 1, 3));
+    }
     std::vector<uint32_t> cut(nb + 1);
-    for (uint32_t k = 0; k <= nb; ++k) cut[k] = (uint32_t)((uint64_t)nbi * k / nb);
+    for (uint32_t k = 0; k <= nb; ++k) cut[k] = (uint32_t)std::llround(nbi * wsum[k] / wsum[nb]);
+    for (uint32_t k = 1; k <= nb; ++k) cut[k] = std::max(cut[k], cut[k - 1]);   // (monotone; empty bands allowed)
     std::vector<uint32_t> ord;
     band_off.assign(1, 0u);
     band_row0.clear();
     for (uint32_t k = 0; k < nb; ++k) {
         for (uint32_t bij : block_order) {
-            const uint32_t bi = bij >> 16;
-            if (bi >= cut[k] && bi < cut[k + 1]) ord.push_back(bij);
+            const uint32_t bi = bij >> 16, bj = bij & 0xffffu;
+            const bool in_band = bi >= cut[k] && bi < cut[k + 1];
+            if (gram ? (bi == bj ? k == 0 : in_band) : in_band) ord.push_back(bij);
         }
         band_off.push_back((uint32_t)ord.size());
         band_row0.push_back(std::min(n, cut[k] * ei));
@@ -1312,6 +1506,8 @@ bool PlanImpl::prepare_bands(std::string* err) {
     if (!upload(d_order_band, ord.empty() ? std::vector<uint32_t>{0u} : ord, st, err)) return false;
     SSU_HIP(hipStreamSynchronize(st));   // `ord` is read until here
     if (!copy_st) SSU_HIP(hipStreamCreateWithFlags(&copy_st, hipStreamNonBlocking));
+    if (!band_st2) SSU_HIP(hipStreamCreateWithFlags(&band_st2, hipStreamNonBlocking));
+    if (!band_go) SSU_HIP(hipEventCreateWithFlags(&band_go, hipEventDisableTiming));
     band_ev.assign(nb, nullptr);
     for (auto& e : band_ev) SSU_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return true;
@@ -1322,27 +1518,46 @@ bool Plan::execute_to_host(void* host, std::string* err) {
     // banded copies for the shared-node kernels over the full matrix (the other
     // paths write their output in one final pass)
     static const bool off = getenv("SSU_OUT_BANDS") && atoi(getenv("SSU_OUT_BANDS")) == 0;
-    if (off || !P.sparse || !P.band_all || P.n < 4096 || P.block_order.empty())
+    const bool gram_bands = P.gram && P.Kp;   // (no Gram columns: one fill, nothing to band)
+    if (off || !(P.sparse || gram_bands) || !P.band_all || P.n < 4096 || P.block_order.empty())
         return execute(err) && copy_matrix(host, err);
     if (!P.prepare_bands(err)) return false;
     const size_t es = P.spec.fp64 ? sizeof(double) : sizeof(float);
     P.banded_run = true;
     P.after_enqueue = [&P, host, es](std::string* e) {
+        // band k's rows are final once it and the band before it (the other stream) are done
+        std::vector<CopyRange> ranges;
         for (size_t k = 0; k < P.band_ev.size(); ++k) {
-            if (hipEventSynchronize(P.band_ev[k]) != hipSuccess) { if (e) *e = "band wait failed"; return false; }
             const size_t r0 = P.band_row0[k], r1 = P.band_row0[k + 1];
-            if (r1 > r0 && !copy_to_host((char*)host + r0 * P.n * es, P.d_out.as<char>() + r0 * P.n * es,
-                                         (r1 - r0) * P.n * es, P.copy_st, e))
-                return false;
+            CopyRange r{r0 * P.n * es, (r1 - r0) * P.n * es, {(void*)P.band_ev[k]}};
+            if (k > 0) r.ready.push_back((void*)P.band_ev[k - 1]);
+            ranges.push_back(r);
         }
-        return true;
+        return copy_ranges_to_host((char*)host, P.d_out.as<char>(), ranges, P.copy_st, e);
     };
     const bool ok = execute(err);
     P.banded_run = false;
     P.after_enqueue = nullptr;
     if (!ok) return false;
-    // refinement rewrote entries after the bands were copied: copy again
-    return P.refined ? copy_matrix(host, err) : true;
+    if (!P.refined) return true;
+    // refinement rewrote entries after the bands were copied, all among the refined
+    // samples: their square of entries again, or the whole matrix when that is
+    // more than a quarter of it (FP32 output: FP64 plans are not refined)
+    const std::vector<uint32_t>& F = P.refined_samples;
+    const size_t ns = F.size(), nn = (size_t)P.n * P.n;
+    if (P.spec.fp64 || ns > 65535 || ns * ns * 4 > nn) return copy_matrix(host, err);
+    DevMem sub;
+    std::vector<float> h(ns * ns);
+    if (!upload(P.d_ridx, F, P.st, err) || !sub.ensure(std::max<size_t>(ns * ns, 1) * sizeof(float), err))
+        return false;
+    SSU_HIP(launch_gather_square(sub.as<float>(), (const float*)P.d_out.p, P.n, P.d_ridx.as<uint32_t>(),
+                                 (uint32_t)ns, P.st));
+    if (!copy_to_host(h.data(), sub.p, ns * ns * sizeof(float), P.st, err)) return false;
+    SSU_HIP(hipStreamSynchronize(P.st));
+    float* out = (float*)host;
+    for (size_t a = 0; a < ns; ++a)
+        for (size_t b = 0; b < ns; ++b) out[(size_t)F[a] * P.n + F[b]] = h[a * ns + b];
+    return true;
 }
 
 bool Plan::set_stripes(uint32_t start, uint32_t stop, std::string* err) {
@@ -1398,7 +1613,8 @@ bool PlanImpl::refine(const uint32_t* flags, std::string* err) {
     if (!count) return true;
     std::vector<uint32_t> mark(n);
     SSU_HIP(hipMemcpy(mark.data(), flags + 1, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    std::vector<uint32_t> F;
+    std::vector<uint32_t>& F = refined_samples;
+    F.clear();
     for (uint32_t s = 0; s < n; ++s)
         if (mark[s]) F.push_back(s);
     // the table as this execution sees it

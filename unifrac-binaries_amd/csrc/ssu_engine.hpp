@@ -41,12 +41,26 @@ constexpr int EV_SPLIT = PH_COUNT, EV_ROWS = PH_COUNT + 1;   // extra events: po
 
 struct PlanImpl;
 
+// A table's CSR (entries indptr[0] .. indptr[n_obs]) copied to device memory
+// on a thread of its own, so that the copy overlaps the host work that comes
+// before a plan (the tree, the sample totals); Plan::create takes it over.
+class TableUpload {
+public:
+    static std::shared_ptr<TableUpload> start(const uint32_t* indptr, const uint32_t* indices, const double* data,
+                                              uint32_t n_obs);
+    ~TableUpload();
+    struct Impl;
+    std::unique_ptr<Impl> impl;
+};
+
 class Plan {
 public:
     ~Plan();
     // Uploads the table and postorder arrays; returns nullptr and sets err on failure.
+    // `pre`: the table's CSR already on its way to the device (TableUpload::start on
+    // the same arrays), taken over by the plan instead of uploading it again.
     static std::unique_ptr<Plan> create(const Table& table, const PostorderNodes& nodes, const ComputeSpec& spec,
-                                        std::string* err);
+                                        std::string* err, const std::shared_ptr<TableUpload>& pre = nullptr);
     // Full n x n distance matrix into device memory (FP64 if spec.fp64, FP32 otherwise).
     bool execute(std::string* err);
     // The same with the table's CSR values, the per-sample totals and the result
@@ -119,9 +133,18 @@ void pool_free(void* p, size_t bytes);
 void pool_release_all();
 
 // bytes of device memory into pageable host memory through a process-wide
-// pinned staging area: chunks copied over PCIe on `stream` while the previous
-// chunk is spread into `host` by several threads
+// pinned staging area: chunks copied over PCIe on `stream` while earlier
+// chunks are spread into `host` by a pool of threads
 bool copy_to_host(void* host, const void* dev, size_t bytes, void* stream, std::string* err);
+// The same for byte ranges [off, off + len) of dev into host at the same
+// offsets, in order; a range's copies wait on the device (on `stream`) for the
+// events in `ready`, so later ranges can still be computing when this starts.
+struct CopyRange {
+    size_t off, len;
+    std::vector<void*> ready;   // hipEvent_t
+};
+bool copy_ranges_to_host(char* host, const char* dev, const std::vector<CopyRange>& ranges, void* stream,
+                         std::string* err);
 // the reverse, host to device; returns once the host buffer has been read
 bool copy_to_device(void* dev, const void* host, size_t bytes, void* stream, std::string* err);
 

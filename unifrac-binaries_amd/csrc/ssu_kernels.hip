@@ -558,10 +558,11 @@ k_gram_i8r(double* __restrict__ G, uint32_t tiles, const int8_t* __restrict__ Pt
 
 template <class TOut>
 __global__ void __launch_bounds__(256) k_unweighted_combine(TOut* __restrict__ out, const double* __restrict__ G,
-                                                            uint32_t tiles, uint32_t n, int normalized) {
+                                                            uint32_t tiles, uint32_t n, int normalized, uint32_t row0,
+                                                            uint32_t row1) {
     const uint32_t j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const uint32_t i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= n || j >= n) return;
+    const uint32_t i = row0 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= row1 || j >= n) return;
     double d = 0.0;
     if (i != j) {
         const uint32_t a = i < j ? i : j, b = i < j ? j : i;
@@ -1418,6 +1419,7 @@ constexpr uint32_t SR_CW = 4096;   // output columns per workgroup (64-bit LDS a
 constexpr uint32_t SR_WAVES = 8;   // waves sharing one accumulator, each taking every 8th round of entries
 constexpr uint32_t SR_UNROLL = 4;  // postings per lane with their loads in flight together
 constexpr uint32_t SR_WIN = 1024;  // walk positions resolved to entries at a time (64 lanes x 16 bytes)
+constexpr uint32_t SR_WALK = 1, SR_DIAG = 2;   // k_pairs_rows modes: the columns past a, G(a, a)
 
 // Sparse-node share for row a.  Weighted: M = sum over sparse nodes r held by
 // both a and b > a of l_r * min(u_ra, u_rb); out[a * n + b] gets, for the
@@ -1518,12 +1520,17 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                                                    uint32_t nchunks, uint32_t nkc, double* __restrict__ G,
                                                    uint32_t tiles, const double* __restrict__ ssp,
                                                    TOut* __restrict__ out2, const double* __restrict__ pspmax,
-                                                   float alpha, GapBand band) {
+                                                   float alpha, GapBand band, uint32_t a0, uint32_t na,
+                                                   uint32_t mode) {
     // GEN keeps two accumulators (numerator, denominator) on half-width chunks
     constexpr uint32_t CW = GEN ? SR_CW / 2 : SR_CW;
     // chunk-major: the rows in flight share one column chunk of every posting
-    // list, a quarter of the posting data at 16k samples (cache-resident)
-    const uint32_t a = blockIdx.x % n, c = blockIdx.x / n;
+    // list, a quarter of the posting data at 16k samples (cache-resident).
+    // Rows a0 .. a0 + na - 1; mode SR_DIAG alone: one workgroup per row, on the
+    // chunk that holds its diagonal.
+    const uint32_t a = a0 + blockIdx.x % na;
+    const uint32_t c = mode == SR_DIAG ? a / CW : blockIdx.x / na;
+    const bool walk = (mode & SR_WALK) != 0;
     const uint32_t b0 = c * CW, b1 = min(n, b0 + CW);
     // the selected columns b > a of this chunk: the hull [lo, hi) of the band's two gap
     // windows, widened to whole 32-sample panels (inside the chunk: b0 is a panel start)
@@ -1540,12 +1547,13 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             lo &= ~31u;
             hi = min(b1, (hi + 31u) & ~31u);
         }
+        if (!walk) lo = 0xffffffffu, hi = 0;
     }
-    const bool has_diag = GRAM && b0 <= a && a < b1;
+    const bool has_diag = GRAM && (mode & SR_DIAG) && b0 <= a && a < b1;
     // uniform: no selected column here (nor, unweighted, the diagonal); more postings
     // than slots never happens (the plan's bound) and is skipped, not read
     if (a >= n || (lo >= hi && !has_diag) || poff[K] > slots) {
-        if (walked && threadIdx.x == 0) walked[blockIdx.x] = 0;
+        if (walked && walk && threadIdx.x == 0 && a < n) walked[(size_t)c * n + a] = 0;
         return;
     }
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1727,10 +1735,10 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     }
     if (lane == 0) wk[wv] = total_walked;
     __syncthreads();
-    if (walked && threadIdx.x == 0) {
+    if (walked && walk && threadIdx.x == 0) {
         uint32_t sum = 0;
         for (uint32_t w = 0; w < SR_WAVES; ++w) sum += wk[w];
-        walked[blockIdx.x] = sum;
+        walked[(size_t)c * n + a] = sum;
     }
     // the walked panels' columns past the diagonal (every posting of theirs was added)
     const uint32_t wlo = max(a + 1, pb0 * 32), whi = min(hi, pb1 * 32);
@@ -1962,6 +1970,22 @@ hipError_t launch_refine_scatter(float* out, uint32_t n, const double* sub, uint
     return hipGetLastError();
 }
 
+// sub[a * ns + b] = out[idx[a] * n + idx[b]]
+__global__ void __launch_bounds__(256) k_gather_square(float* __restrict__ sub, const float* __restrict__ out,
+                                                       uint32_t n, const uint32_t* __restrict__ idx, uint32_t ns) {
+    const uint32_t a = blockIdx.y, b = blockIdx.x * 256 + threadIdx.x;
+    if (a >= ns || b >= ns) return;
+    sub[(size_t)a * ns + b] = out[(size_t)idx[a] * n + idx[b]];
+}
+
+hipError_t launch_gather_square(float* sub, const float* out, uint32_t n, const uint32_t* idx, uint32_t ns,
+                                hipStream_t st) {
+    if (!ns) return hipSuccess;
+    if (ns > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gather_square, dim3(grid1(ns, 256), ns), dim3(256), 0, st, sub, out, n, idx, ns);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t st) {
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)std::min<size_t>(grid1(n, 256), 65536)), dim3(256), 0, st, p, n, v);
@@ -2137,13 +2161,17 @@ hipError_t launch_gram_i8(double* G, const int8_t* Pt, size_t ldp, const int8_t*
 
 template <class TOut>
 hipError_t launch_unweighted_combine(TOut* out, const double* G, uint32_t n_pad, uint32_t n, bool normalized,
-                                     hipStream_t st) {
-    hipLaunchKernelGGL((k_unweighted_combine<TOut>), dim3(grid1(n, 64), grid1(n, 4)), dim3(256), 0, st, out, G,
-                       n_pad / GRAM_BM, n, (int)normalized);
+                                     hipStream_t st, uint32_t row0, uint32_t row1) {
+    row1 = std::min(row1, n);
+    if (row0 >= row1) return hipSuccess;
+    hipLaunchKernelGGL((k_unweighted_combine<TOut>), dim3(grid1(n, 64), grid1(row1 - row0, 4)), dim3(256), 0, st, out,
+                       G, n_pad / GRAM_BM, n, (int)normalized, row0, row1);
     return hipGetLastError();
 }
-template hipError_t launch_unweighted_combine<float>(float*, const double*, uint32_t, uint32_t, bool, hipStream_t);
-template hipError_t launch_unweighted_combine<double>(double*, const double*, uint32_t, uint32_t, bool, hipStream_t);
+template hipError_t launch_unweighted_combine<float>(float*, const double*, uint32_t, uint32_t, bool, hipStream_t,
+                                                     uint32_t, uint32_t);
+template hipError_t launch_unweighted_combine<double>(double*, const double*, uint32_t, uint32_t, bool, hipStream_t,
+                                                      uint32_t, uint32_t);
 
 hipError_t launch_pairs(PairOp op, bool fp64, const PairArgs& a, hipStream_t st) {
     return fp64 ? launch_pairs_prec<double>(op, a, st) : launch_pairs_prec<float>(op, a, st);
@@ -2242,42 +2270,49 @@ hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
 
 // weighted: the partial sums go out in the value type (FP32 or FP64 output)
 template <class IX, bool GRAM, class TV, bool GEN = false>
-static void pairs_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs& a, uint32_t nchunks, uint32_t nkc) {
+static void pairs_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs& a, uint32_t nchunks, uint32_t nkc,
+                         uint32_t a0, uint32_t na, uint32_t mode) {
     hipLaunchKernelGGL((k_pairs_rows<TV, IX, GRAM, TV, GEN>), dim3(blocks), dim3(64 * SR_WAVES), 0, st,
                        GRAM ? nullptr : (TV*)a.out, a.walked, a.rows, a.roff, a.pc, (const IX*)a.pidx,
                        GRAM ? nullptr : (const TV*)a.pval, a.len, GRAM ? nullptr : a.S, a.K, a.n, a.npan,
                        (const uint32_t*)a.poff, a.slots, nchunks, nkc, a.gram, a.n_pad / GRAM_BM, a.ssp,
-                       (TV*)a.out2, a.pspmax, a.alpha, a.band);
+                       (TV*)a.out2, a.pspmax, a.alpha, a.band, a0, na, mode);
 }
 
-hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st) {
+hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st, uint32_t row0, uint32_t row1, int parts) {
     if (!a.K || !a.n) return hipSuccess;
     if (!sparse_args_ok(a)) return hipErrorInvalidValue;
+    row1 = std::min(row1, a.n);
+    if (row0 >= row1) return hipSuccess;
+    const uint32_t mode = (parts & ROWS_OFF_DIAGONAL ? SR_WALK : 0u) | (parts & ROWS_DIAGONAL ? SR_DIAG : 0u);
+    if (!mode || (mode == SR_DIAG && !a.gram)) return hipErrorInvalidValue;   // (the diagonal is the Gram's)
+    const uint32_t na = row1 - row0;
     const uint32_t nkc = row_list_chunks(a.n, a.K);
     const uint32_t cw = a.gen ? SR_CW / 2 : SR_CW;
     const uint32_t nchunks = (a.n + cw - 1) / cw;
-    const uint64_t blocks = (uint64_t)a.n * nchunks;
+    const uint64_t blocks = mode == SR_DIAG ? (uint64_t)na : (uint64_t)na * nchunks;
     if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
     const bool narrow = a.n <= 65536;
+    const bool whole = row0 == 0 && row1 == a.n;   // the per-sample sums are made for all rows at once
     if (a.gen) {   // generalized, FP32 values
-        if (a.gram || a.values_fp64 || !a.ssp || !a.out2 || !a.pspmax) return hipErrorInvalidValue;
+        if (a.gram || a.values_fp64 || !a.ssp || !a.out2 || !a.pspmax || !whole) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_sparse_sums<float, true>), dim3(a.n), dim3(64), 0, st, a.ssp, a.rows, a.roff, nkc,
                            (const uint32_t*)a.poff, (const float*)a.pval, a.len, a.K, a.n, a.slots, a.alpha);
         hipLaunchKernelGGL(k_max, dim3(1), dim3(256), 0, st, a.pspmax, a.ssp, a.n);
-        if (narrow) pairs_rows_t<uint16_t, false, float, true>((uint32_t)blocks, st, a, nchunks, nkc);
-        else pairs_rows_t<uint32_t, false, float, true>((uint32_t)blocks, st, a, nchunks, nkc);
+        if (narrow) pairs_rows_t<uint16_t, false, float, true>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode);
+        else pairs_rows_t<uint32_t, false, float, true>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode);
     } else if (a.gram) {
-        if (narrow) pairs_rows_t<uint16_t, true, float>((uint32_t)blocks, st, a, nchunks, nkc);
-        else pairs_rows_t<uint32_t, true, float>((uint32_t)blocks, st, a, nchunks, nkc);
+        if (narrow) pairs_rows_t<uint16_t, true, float>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode);
+        else pairs_rows_t<uint32_t, true, float>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode);
     } else if (a.values_fp64) {
-        if (narrow) pairs_rows_t<uint16_t, false, double>((uint32_t)blocks, st, a, nchunks, nkc);
-        else pairs_rows_t<uint32_t, false, double>((uint32_t)blocks, st, a, nchunks, nkc);
+        if (narrow) pairs_rows_t<uint16_t, false, double>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode);
+        else pairs_rows_t<uint32_t, false, double>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode);
     } else {
-        if (!a.ssp) return hipErrorInvalidValue;
+        if (!a.ssp || !whole) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_sparse_sums<float, false>), dim3(a.n), dim3(64), 0, st, a.ssp, a.rows, a.roff, nkc,
                            (const uint32_t*)a.poff, (const float*)a.pval, a.len, a.K, a.n, a.slots, 0.0f);
-        if (narrow) pairs_rows_t<uint16_t, false, float>((uint32_t)blocks, st, a, nchunks, nkc);
-        else pairs_rows_t<uint32_t, false, float>((uint32_t)blocks, st, a, nchunks, nkc);
+        if (narrow) pairs_rows_t<uint16_t, false, float>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode);
+        else pairs_rows_t<uint32_t, false, float>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode);
     }
     return hipGetLastError();
 }
@@ -2285,7 +2320,7 @@ hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st) {
 hipError_t launch_sparse_nodes(const SparseNodesArgs& a, hipStream_t st) {
     if (a.gram) return hipErrorInvalidValue;   // the Gram tiles must be written between the two halves
     const hipError_t e = launch_sparse_lists(a, st);
-    return e != hipSuccess ? e : launch_sparse_rows(a, st);
+    return e != hipSuccess ? e : launch_sparse_rows(a, st, 0, a.n, ROWS_OFF_DIAGONAL | ROWS_DIAGONAL);
 }
 
 hipError_t launch_count_present(uint32_t* cnt, const double* E, size_t lde, uint32_t col0, const int32_t* knodes,

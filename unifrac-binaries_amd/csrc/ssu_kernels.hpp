@@ -66,6 +66,9 @@ hipError_t launch_flag_close(uint32_t* flags, const float* out, uint32_t n, cons
 // a < na <= b.
 hipError_t launch_refine_scatter(float* out, uint32_t n, const double* sub, uint32_t ns, const uint32_t* idx,
                                  uint32_t na, bool cross, const double* S, float tau, hipStream_t st);
+// sub (ns x ns) = the entries of out (n x n) at rows and columns idx[0..ns)
+hipError_t launch_gather_square(float* sub, const float* out, uint32_t n, const uint32_t* idx, uint32_t ns,
+                                hipStream_t st);
 
 // ---------------------------------------------------------------- embedding
 // E holds the sample columns [col0, col0 + ld) of the node-major embedding
@@ -147,10 +150,10 @@ hipError_t launch_presence_T_bits(int8_t* Pt, size_t ldp, const uint32_t* B, uin
 // order lists the tiles computed (every tile, or a subset)
 hipError_t launch_gram_i8(double* G, const int8_t* Pt, size_t ldp, const int8_t* digits, const GramChunk* chunks,
                           uint32_t n_chunks, uint32_t n_pad, const uint32_t* order, uint32_t n_tiles, hipStream_t st);
-// out[i][j] from G and T = diag(G): (T_i+T_j-2G)/(T_i+T_j-G) or T_i+T_j-2G
+// out[i][j] from G and T = diag(G): (T_i+T_j-2G)/(T_i+T_j-G) or T_i+T_j-2G, rows i in [row0, row1)
 template <class TOut>
 hipError_t launch_unweighted_combine(TOut* out, const double* G, uint32_t n_pad, uint32_t n, bool normalized,
-                                     hipStream_t st);
+                                     hipStream_t st, uint32_t row0 = 0, uint32_t row1 = 0xffffffffu);
 
 // ---------------------------------------------------------------- VALU pair tiles
 enum class PairOp : int {
@@ -254,8 +257,12 @@ struct SparseNodesArgs {
 // posting lists, per-panel prefix counts and row lists
 hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st);
 // each row's share over the posting lists (after launch_sparse_lists; in the
-// unweighted case after launch_gram_i8 as well)
-hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st);
+// unweighted case after launch_gram_i8 as well), for rows [row0, row1): the
+// columns past each row, and (unweighted) the diagonal G(a, a), as `parts`
+// says.  Weighted and generalized sums need every row in one call.
+constexpr int ROWS_OFF_DIAGONAL = 1, ROWS_DIAGONAL = 2;
+hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st, uint32_t row0 = 0,
+                              uint32_t row1 = 0xffffffffu, int parts = ROWS_OFF_DIAGONAL | ROWS_DIAGONAL);
 // both, in order (weighted)
 hipError_t launch_sparse_nodes(const SparseNodesArgs& a, hipStream_t st);
 // cnt[r] += number of samples s in [col0, min(n, col0 + lde)) with E[knodes[r]][s - col0] > 0

@@ -1,13 +1,76 @@
 #include "ssu_table.hpp"
 
+#include <algorithm>
+#include <cmath>
+#include <thread>
+
 namespace ssu {
 
+namespace {
+// Sample totals over observation ranges on several threads, when that gives the
+// sequential sum bit for bit: every value an integer and the sum of all their
+// magnitudes below 2^53, so every partial sum, in any order, is an integer
+// below 2^53 and each FP64 add is exact (count tables).  false: not so.
+bool integer_totals(const uint32_t* indptr, const uint32_t* indices, const double* data, uint32_t n_obs,
+                    uint32_t n_samples, double* tot) {
+    const uint64_t nnz = n_obs ? (uint64_t)indptr[n_obs] - indptr[0] : 0;
+    const unsigned threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    if (nnz < (1u << 20) || threads < 2) return false;
+    // observation ranges of about equal entry counts
+    std::vector<uint32_t> cut(threads + 1, n_obs);
+    cut[0] = 0;
+    for (unsigned t = 1, i = 0; t < threads; ++t) {
+        const uint64_t goal = indptr[0] + nnz * t / threads;
+        while (i < n_obs && indptr[i] < goal) ++i;
+        cut[t] = i;
+    }
+    std::vector<std::vector<double>> part(threads);
+    std::vector<uint8_t> integral(threads, 1);
+    std::vector<double> mag(threads, 0.0);
+    std::vector<std::thread> pool;
+    auto run = [&](unsigned t) {
+        std::vector<double>& p = part[t];
+        p.assign(n_samples, 0.0);
+        bool ok = true;
+        double a = 0.0;   // sum of magnitudes (exact while below 2^53)
+        for (uint32_t i = cut[t]; i < cut[t + 1]; ++i)
+            for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j) {
+                const double v = data[j];
+                ok &= v == std::nearbyint(v) && std::fabs(v) < 0x1p52;
+                a += std::fabs(v);
+                p[indices[j]] += v;
+            }
+        integral[t] = ok;
+        mag[t] = a;
+    };
+    for (unsigned t = 1; t < threads; ++t) pool.emplace_back(run, t);
+    run(0);
+    for (auto& th : pool) th.join();
+    double a = 0.0;
+    for (unsigned t = 0; t < threads; ++t) {
+        if (!integral[t] || !(mag[t] < 0x1p52)) return false;
+        a += mag[t];
+    }
+    if (!(a < 0x1p52)) return false;
+    for (uint32_t s = 0; s < n_samples; ++s) {
+        double x = 0.0;
+        for (unsigned t = 0; t < threads; ++t) x += part[t][s];
+        tot[s] = x;
+    }
+    return true;
+}
+}  // namespace
+
 void Table::finish(bool index_obs) {
-    // src/biom_inmem.cpp:513-526: totals accumulate in observation order
+    // src/biom_inmem.cpp:513-526: totals accumulate in observation order (for
+    // integer counts any order gives the same sums)
     totals.assign(n_samples, 0.0);
     double* tot = totals.data();
-    for (uint32_t i = 0; i < n_obs; ++i)
-        for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j) tot[indices[j]] += data[j];
+    if (!integer_totals(indptr, indices, data, n_obs, n_samples, tot)) {
+        std::fill(totals.begin(), totals.end(), 0.0);
+        for (uint32_t i = 0; i < n_obs; ++i)
+            for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j) tot[indices[j]] += data[j];
+    }
     obs_index.clear();
     if (!index_obs) return;
     obs_index.reserve(n_obs * 2 + 1);

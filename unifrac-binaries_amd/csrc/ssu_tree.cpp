@@ -3,6 +3,8 @@
 #include <cctype>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <thread>
 
 namespace ssu {
 
@@ -274,30 +276,75 @@ inline uint64_t name_hash(const char* s) {   // FNV-1a
     for (; *s; ++s) h = (h ^ (unsigned char)*s) * 1099511628211ull;
     return h ^ (h >> 29);
 }
+
+// fn(t, begin, end) over `parts` consecutive ranges of [0, n), range 0 on the calling thread
+template <class F>
+void split_range(uint64_t n, unsigned parts, F fn) {
+    const uint64_t step = (n + parts - 1) / std::max(parts, 1u);
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < parts; ++t) {
+        const uint64_t a = std::min(n, step * t), b = std::min(n, a + step);
+        pool.emplace_back([=] { fn(t, a, b); });
+    }
+    fn(0, 0, std::min(n, step));
+    for (auto& th : pool) th.join();
+}
+
+unsigned threads_for(uint64_t n) {
+    return n < (1u << 16) ? 1u : std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+}
 }  // namespace
 
 NameIndex::NameIndex(const char* const* ids, uint32_t n) : ids_(ids) {
-    uint32_t cap = 16;
-    while (cap < 2 * (uint64_t)n + 16) cap <<= 1;
-    mask_ = cap - 1;
-    slot_.assign(cap, -1);
-    for (uint32_t i = 0; i < n; ++i) {
-        const char* id = ids[i] ? ids[i] : "";
-        for (uint32_t h = (uint32_t)name_hash(id) & mask_;; h = (h + 1) & mask_) {
-            if (slot_[h] < 0) { slot_[h] = (int32_t)i; ++distinct_; break; }
-            const char* o = ids[slot_[h]] ? ids[slot_[h]] : "";
-            if (std::strcmp(o, id) == 0) { slot_[h] = (int32_t)i; break; }   // the last row wins
-        }
+    const unsigned threads = threads_for(n);
+    std::vector<uint64_t> hv(n);
+    std::vector<uint32_t> count(PARTS, 0);
+    split_range(n, threads, [&](unsigned, uint64_t a, uint64_t b) {
+        for (uint64_t i = a; i < b; ++i) hv[i] = name_hash(ids[i] ? ids[i] : "");
+    });
+    for (uint32_t i = 0; i < n; ++i) ++count[hv[i] >> 58];
+    uint32_t total = 0;
+    for (uint32_t p = 0; p < PARTS; ++p) {
+        uint32_t cap = 8;
+        while (cap < 2 * (uint64_t)count[p] + 8) cap <<= 1;
+        base_[p] = total;
+        mask_[p] = cap - 1;
+        total += cap;
     }
+    slot_.assign(total, Slot{0, -1});
+    std::vector<uint32_t> distinct(threads, 0);
+    // thread t fills tables t, t + threads, ...: every id of a table in row order
+    split_range(threads, threads, [&](unsigned, uint64_t t0, uint64_t t1) {
+        for (uint64_t t = t0; t < t1; ++t)
+            for (uint32_t i = 0; i < n; ++i) {
+                const uint32_t p = (uint32_t)(hv[i] >> 58);
+                if (p % threads != t) continue;
+                Slot* tab = slot_.data() + base_[p];
+                const char* id = ids[i] ? ids[i] : "";
+                for (uint32_t h = (uint32_t)hv[i] & mask_[p];; h = (h + 1) & mask_[p]) {
+                    Slot& s = tab[h];
+                    if (s.row < 0) { s = Slot{hv[i], (int32_t)i}; ++distinct[t]; break; }
+                    if (s.hash == hv[i]) {
+                        const char* o = ids[s.row] ? ids[s.row] : "";
+                        if (std::strcmp(o, id) == 0) { s.row = (int32_t)i; break; }   // the last row wins
+                    }
+                }
+            }
+    });
+    for (uint32_t d : distinct) distinct_ += d;
 }
 
 int32_t NameIndex::find(const char* name) const {
     const char* nm = name ? name : "";
-    for (uint32_t h = (uint32_t)name_hash(nm) & mask_;; h = (h + 1) & mask_) {
-        const int32_t r = slot_[h];
-        if (r < 0) return -1;
-        const char* o = ids_[r] ? ids_[r] : "";
-        if (std::strcmp(o, nm) == 0) return r;
+    const uint64_t hv = name_hash(nm);
+    const uint32_t p = (uint32_t)(hv >> 58);
+    const Slot* tab = slot_.data() + base_[p];
+    for (uint32_t h = (uint32_t)hv & mask_[p];; h = (h + 1) & mask_[p]) {
+        const Slot& s = tab[h];
+        if (s.row < 0) return -1;
+        if (s.hash != hv) continue;
+        const char* o = ids_[s.row] ? ids_[s.row] : "";
+        if (std::strcmp(o, nm) == 0) return s.row;
     }
 }
 
@@ -312,15 +359,20 @@ bool synced_postorder(const bool* structure, const double* lengths, const char* 
     stack.reserve(64);
     std::vector<uint8_t> seen;
     matched = 0;
+    // the leaves' rows first, on several threads for large trees (a leaf: an open
+    // followed by its close, src/tree.cpp:264-266)
+    split_range(n, threads_for(n), [&](unsigned, uint64_t a, uint64_t b) {
+        for (uint64_t i = a; i < b; ++i)
+            if (structure[i] && i + 1 < n && !structure[i + 1]) row[i] = rows.find(names[i]);
+    });
     for (uint32_t i = 0; i < n; ++i) {
         if (structure[i]) { stack.push_back(i); continue; }
         if (stack.empty()) return false;
         const uint32_t o = stack.back();
         stack.pop_back();
         bool keep;
-        if (i == o + 1) {   // a leaf (src/tree.cpp:264-266)
-            const int32_t r = rows.find(names[o]);
-            row[o] = r;
+        if (i == o + 1) {   // a leaf
+            const int32_t r = row[o];
             keep = r >= 0;
             if (keep) {
                 if (seen.size() <= (size_t)r) seen.resize((size_t)r + 1, 0);
@@ -335,21 +387,33 @@ bool synced_postorder(const bool* structure, const double* lengths, const char* 
     if (!stack.empty()) return false;
     // pass 2, opens in preorder: drop unkept nodes and collapse the kept internal
     // nodes (not the root) with one kept child, whose lengths accumulate top-down
-    // into that child, l_child + (l_parent + ...) (src/tree.cpp:165-191)
-    struct Frame { uint32_t o; bool emit; bool pass; double chain; };
+    // into that child, l_child + (l_parent + ...) (src/tree.cpp:165-191); each
+    // kept node is numbered at its close, which is the postorder of the sheared,
+    // collapsed tree (as postorder_nodes), and takes the closed nodes pending
+    // since its open as its children
+    uint32_t total = 0;   // kept, uncollapsed nodes
+    for (uint32_t i = 0; i < n; ++i)
+        if (structure[i] && kept[i] && (i == 0 || !(kc[i] == 1 && !(i + 1 < n && !structure[i + 1])))) ++total;
+    // (a kept node inside a dropped subtree does not exist: kept means a kept leaf below)
+    PostorderNodes& p = out;
+    p.m = total > 0 ? total - 1 : 0;
+    p.parent.assign(p.m, (int32_t)p.m);
+    p.length.assign(p.m, 0.0);
+    p.is_leaf.assign(p.m, 0);
+    p.obs_row.assign(p.m, -1);
+    p.height.assign(p.m, 0);
+    std::vector<int32_t> h(total + 1, 0);
+    struct Frame { bool emit; bool pass; double chain; double len; uint32_t kids0; };
     std::vector<Frame> fs;
     fs.reserve(64);
-    std::vector<uint8_t> open2;
-    std::vector<double> len2;
-    std::vector<int32_t> row2;
-    open2.reserve(n);
-    len2.reserve(n);
-    row2.reserve(n);
+    std::vector<uint32_t> kids;   // numbers of closed nodes whose parent is still open
+    kids.reserve(64);
+    uint32_t k = 0;
     for (uint32_t i = 0; i < n; ++i) {
         if (structure[i]) {
             const bool has_chain = !fs.empty() && fs.back().pass;
             const double up = has_chain ? fs.back().chain : 0.0;
-            Frame f{i, false, false, 0.0};
+            Frame f{false, false, 0.0, 0.0, 0};
             if (kept[i] && (fs.empty() || fs.back().emit || fs.back().pass)) {
                 const bool leaf = i + 1 < n && !structure[i + 1];
                 const bool collapse = !leaf && i != 0 && kc[i] == 1;
@@ -359,56 +423,32 @@ bool synced_postorder(const bool* structure, const double* lengths, const char* 
                     f.chain = l;
                 } else {
                     f.emit = true;
-                    open2.push_back(1);
-                    len2.push_back(l);
-                    row2.push_back(leaf ? row[i] : -1);
+                    f.len = l;
+                    f.kids0 = (uint32_t)kids.size();
                 }
             }
             fs.push_back(f);
-        } else {
-            const Frame f = fs.back();
-            fs.pop_back();
-            if (f.emit) {
-                open2.push_back(0);
-                len2.push_back(0.0);
-                row2.push_back(-1);
-            }
+            continue;
         }
-    }
-    // pass 3: postorder of the sheared, collapsed tree (as postorder_nodes)
-    const uint32_t n2 = (uint32_t)open2.size();
-    const uint32_t total = n2 / 2;
-    PostorderNodes& p = out;
-    p.m = total > 0 ? total - 1 : 0;
-    p.parent.assign(p.m, (int32_t)p.m);
-    p.length.assign(p.m, 0.0);
-    p.is_leaf.assign(p.m, 0);
-    p.obs_row.assign(p.m, -1);
-    p.height.assign(p.m, 0);
-    std::vector<int32_t> h(total + 1, 0);
-    // postorder index of each open (numbered at its close), then parents, lengths, heights
-    std::vector<int32_t> post(n2, -1);
-    std::vector<uint32_t> s4;
-    uint32_t k = 0;
-    for (uint32_t i = 0; i < n2; ++i) {
-        if (open2[i]) { s4.push_back(i); continue; }
-        post[s4.back()] = (int32_t)k++;
-        s4.pop_back();
-    }
-    for (uint32_t i = 0; i < n2; ++i) {
-        if (open2[i]) { s4.push_back(i); continue; }
-        const uint32_t o = s4.back();
-        s4.pop_back();
-        const uint32_t kk = (uint32_t)post[o];
+        const Frame f = fs.back();
+        fs.pop_back();
+        if (!f.emit) continue;
+        const uint32_t kk = k++;
+        int32_t hk = 0;
+        for (size_t c = f.kids0; c < kids.size(); ++c) {
+            const uint32_t ch = kids[c];
+            p.parent[ch] = (int32_t)kk;
+            hk = std::max(hk, h[ch] + 1);
+        }
+        kids.resize(f.kids0);
+        kids.push_back(kk);
+        h[kk] = hk;
         if (kk >= p.m) continue;   // the root
-        const int32_t pk = s4.empty() ? (int32_t)p.m : post[s4.back()];
-        p.parent[kk] = pk;
-        p.length[kk] = len2[o];
-        const bool leaf = i == o + 1;
+        const bool leaf = structure[i - 1];   // its own open just before (an emitted close has i > 0)
+        p.length[kk] = f.len;
         p.is_leaf[kk] = leaf ? 1 : 0;
-        if (leaf) p.obs_row[kk] = row2[o];
-        p.height[kk] = h[kk];
-        if ((uint32_t)pk < total && h[pk] < h[kk] + 1) h[pk] = h[kk] + 1;
+        if (leaf) p.obs_row[kk] = row[i - 1];
+        p.height[kk] = hk;
     }
     return true;
 }

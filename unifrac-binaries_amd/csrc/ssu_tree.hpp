@@ -63,17 +63,23 @@ struct PostorderNodes {
 
 PostorderNodes postorder_nodes(const PhyloTree& t, const std::unordered_map<std::string, uint32_t>& obs_index);
 
-// Observation id -> table row over the caller's strings (open addressing);
-// for repeated ids the last row wins, as in Table::obs_index.
+// Observation id -> table row over the caller's strings: 64 open-addressing
+// tables picked by the top bits of a 64-bit hash, each slot holding the hash
+// (strings are compared only when hashes agree), built on several threads
+// for large tables (each table's ids inserted in row order by one thread); for
+// repeated ids the last row wins, as in Table::obs_index.
 class NameIndex {
 public:
     NameIndex(const char* const* ids, uint32_t n);
     int32_t find(const char* name) const;   // -1: absent
     uint32_t distinct() const { return distinct_; }
 private:
-    std::vector<int32_t> slot_;
+    static constexpr uint32_t PARTS = 64;
+    struct Slot { uint64_t hash; int32_t row; };   // row -1: empty
+    std::vector<Slot> slot_;
+    uint32_t base_[PARTS] = {}, mask_[PARTS] = {};
     const char* const* ids_;
-    uint32_t mask_ = 0, distinct_ = 0;
+    uint32_t distinct_ = 0;
 };
 
 // The tree given as parentheses arrays, sheared to the leaves whose names are

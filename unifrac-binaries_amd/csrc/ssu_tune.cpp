@@ -37,7 +37,8 @@ double tile_terms(SplitKind k, uint32_t n, uint32_t c) {
     const double pairs = 0.5 * (double)n * n;
     if (k == SplitKind::gram) return pairs;
     const double q = 1.0 - (double)c / n;   // a given sample lacks the node
-    return pairs * (1.0 - std::pow(q, 96.0));
+    const double q2 = q * q, q4 = q2 * q2, q8 = q4 * q4, q16 = q8 * q8, q32 = q16 * q16;
+    return pairs * (1.0 - q32 * q32 * q32);   // 1 - q^96
 }
 
 }  // namespace
@@ -46,33 +47,31 @@ SplitChoice choose_split(SplitKind kind, uint32_t n, const std::vector<uint32_t>
     SplitChoice best;
     if (n < 2 || occupancy.empty()) return best;
     const Rates r = rates_of(kind);
-    // nodes by occupancy (0: held by no sample, no cost either way)
-    std::vector<uint32_t> c(occupancy);
-    std::sort(c.begin(), c.end());
+    // nodes by occupancy (0: held by no sample, no cost either way); an
+    // occupancy above n (never, for counts of n samples) counts as n
+    std::vector<uint64_t> hist((size_t)n + 1, 0);
+    for (uint32_t x : occupancy) ++hist[std::min(x, n)];
     // cost with every node on the tiles; then move nodes to the posting lists in
-    // increasing occupancy (thr = c_i + 1 moves every node up to index i)
+    // increasing occupancy (thr = c + 1 moves every node held by at most c samples)
     double tiles = 0.0;
-    for (uint32_t x : c)
-        if (x) tiles += tile_terms(kind, n, x) * r.term;
+    for (uint32_t c = 1; c <= n; ++c)
+        if (hist[c]) tiles += (double)hist[c] * tile_terms(kind, n, c) * r.term;
     best.model_ms_none = best.model_ms = tiles * 1e3;
     double cost = tiles, posts = 0.0;
     uint64_t slots = 0;
-    for (size_t i = 0; i < c.size();) {
-        size_t j = i;
-        while (j < c.size() && c[j] == c[i]) ++j;   // all nodes of this occupancy move together
-        const double x = c[i];
-        if (x > 0) {
-            const double m = (double)(j - i);
-            cost -= m * tile_terms(kind, n, c[i]) * r.term;
+    for (uint32_t c = 0; c <= n; ++c) {
+        if (!hist[c]) continue;
+        if (c > 0) {
+            const double m = (double)hist[c], x = c;
+            cost -= m * tile_terms(kind, n, c) * r.term;
             posts += m * 0.5 * x * (x - 1.0) * r.posting;
-            slots += (uint64_t)(j - i) * c[i];
+            slots += hist[c] * c;
         }
         if (slots >= max_slots) break;
         if (cost + posts < best.model_ms * 1e-3) {
             best.model_ms = (cost + posts) * 1e3;
-            best.thr = c[i] + 1;
+            best.thr = c + 1;
         }
-        i = j;
     }
     return best;
 }
