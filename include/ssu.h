@@ -441,8 +441,8 @@ SSU_EXTERN int ssu_last_call_timings(double* ms, int cap);
  * (default 96 GiB; 0: no cache). */
 SSU_EXTERN void ssu_release_device_memory(void);
 /* The occupancy-split threshold a plan picks (host only): kind 0 FP32 weighted,
- * 1 FP64 weighted, 2 FP32 generalized, 3 unweighted Gram; occupancy[k] = number
- * of samples holding node k.  Returns the threshold (nodes held by fewer samples
+ * 1 FP64 weighted, 2 FP32 generalized, 3 unweighted Gram, 4 FP64 generalized;
+ * occupancy[k] = number of samples holding node k.  Returns the threshold (nodes held by fewer samples
  * are summed from posting lists; 0: no split) and the modelled milliseconds
  * with and without it. */
 SSU_EXTERN unsigned int ssu_choose_split(int kind, unsigned int n_samples, const uint32_t* occupancy,

@@ -622,6 +622,33 @@ def test_generalized_shared_node_path(ssu, O, synth, monkeypatch, theta, alpha):
     assert rel_err(m, dense) <= 2e-6
 
 
+@pytest.mark.parametrize("alpha", [0.0, 0.37, 1.0])
+@pytest.mark.parametrize("theta", ["0", "0.1", "0.5"])
+def test_generalized_fp64_shared_node_path(ssu, O, synth, monkeypatch, theta, alpha):
+    """generalized_fp64 (0 <= alpha <= 1) on the shared-node path: FP64
+    embedding rows on the 32 x 64 blocks, FP64 terms, the rare nodes' shares
+    from the posting lists with FP64 values (theta 0: no split).  Against the
+    FP64 oracle and the dense FP64 kernel (SSU_GEN_SPARSE=0), both at 1e-11."""
+    tree, table = _case(synth, 500, 300, 0.05, 76, zipf=1.1)
+    monkeypatch.setenv("SSU_SPARSE_SPLIT", theta)
+    inp = inputs_from(ssu, tree, table)
+    say("generalized_fp64 shared-node", theta, alpha)
+    p = ssu.Plan(inp, "generalized_fp64", alpha=alpha)
+    assert p.main_kernel() == "k_pairs_gen64"
+    p.execute()
+    m = p.matrix()
+    if theta != "0":
+        assert p.sparse_terms() > 0
+    p.close()
+    ref = oracle_matrix(O, tree, table, "generalized", alpha=alpha)
+    assert rel_err(m, ref) <= TOL["fp64"]
+    assert np.array_equal(m, m.T) and np.all(np.diag(m) == 0)
+    monkeypatch.setenv("SSU_GEN_SPARSE", "0")
+    s, dense = ssu.matrix_inmem(inp, "generalized_fp64", alpha=alpha, fp32=False)
+    assert s == ssu.OKAY
+    assert rel_err(m, dense) <= TOL["fp64"]
+
+
 def test_generalized_near_duplicates(ssu, O, synth):
     """Near-duplicate samples, generalized alpha 0.5: every entry within 1e-6
     of the FP64 oracle (the reference's own FP32 kernels on the same input are

@@ -39,7 +39,7 @@ def ranges(total, k=8, seed=3):
 
 PATHS = [("unweighted", {}), ("unweighted_fp64", {}), ("unweighted_unnormalized", {}),
          ("weighted_normalized", {}), ("weighted_unnormalized", {}), ("weighted_normalized_fp64", {}),
-         ("generalized", {"alpha": 0.5}), ("generalized", {"alpha": 1.5}),
+         ("generalized", {"alpha": 0.5}), ("generalized", {"alpha": 1.5}), ("generalized_fp64", {"alpha": 0.5}),
          ("weighted_normalized", {"variance_adjust": True})]
 
 
@@ -151,7 +151,8 @@ def test_partial_files_merge_bit_equal_to_one_off(ssu, synth, tmp_path):
 
 
 @pytest.mark.parametrize("name,kw", [("weighted_normalized", {}), ("weighted_normalized_fp64", {}),
-                                     ("generalized", {"alpha": 0.5}), ("weighted_unnormalized", {}),
+                                     ("generalized", {"alpha": 0.5}), ("generalized_fp64", {"alpha": 0.5}),
+                                     ("weighted_unnormalized", {}),
                                      ("unweighted", {}), ("unweighted_fp64", {}), ("unweighted_unnormalized", {})])
 def test_one_off_banded_copy_equals_plan(ssu, synth, name, kw):
     """one_off_matrix_inmem* copies each finished band of rows while the rows

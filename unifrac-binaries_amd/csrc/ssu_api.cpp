@@ -1568,7 +1568,7 @@ void ssu_release_device_memory(void) { ssu::pool_release_all(); }
 
 unsigned int ssu_choose_split(int kind, unsigned int n_samples, const uint32_t* occupancy, unsigned int n_nodes,
                               double* model_ms, double* model_ms_none) {
-    if (kind < 0 || kind > 3) return 0;
+    if (kind < 0 || kind > 4) return 0;
     std::vector<uint32_t> occ(occupancy, occupancy + n_nodes);
     const ssu::SplitChoice c = ssu::choose_split((ssu::SplitKind)kind, n_samples, occ, 1ull << 30);
     if (model_ms) *model_ms = c.model_ms;

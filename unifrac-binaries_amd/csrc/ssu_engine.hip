@@ -418,6 +418,9 @@ struct PlanImpl {
     bool bits = false;
     uint32_t ldb = 0;
     std::vector<uint32_t> block_order;  // pair blocks / Gram tiles in launch order
+    // shared-node pair blocks of 32 x 64 samples (FP32 operands, and generalized on
+    // either), else 64 x 64 (FP64 weighted)
+    bool rect() const { return sparse && (!spec.fp64 || gen); }
     // stripe selection (set_stripes): the pairs computed, and the blocks / tiles
     // holding one (run_order, on the device in d_order_run; empty: block_order)
     GapBand band;
@@ -711,7 +714,8 @@ bool PlanImpl::split_weighted(std::string* err) {
     // the threshold: SSU_SPARSE_SPLIT * n, or the cost model's choice on these
     // occupancies (generalized terms cost several times the weighted ones on the
     // tiles, which moves the balance)
-    const SplitKind kind = gen ? SplitKind::generalized32 : spec.fp64 ? SplitKind::weighted64 : SplitKind::weighted32;
+    const SplitKind kind = gen ? (spec.fp64 ? SplitKind::generalized64 : SplitKind::generalized32)
+                               : spec.fp64 ? SplitKind::weighted64 : SplitKind::weighted32;
     uint32_t thr = env_theta ? (uint32_t)std::max(1.0, std::ceil(theta * n)) : choose_split(kind, n, cnt, 1ull << 30).thr;
     if (thr == 0) return true;
     auto bound = [&](uint32_t t) {
@@ -1025,7 +1029,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     const bool weighted = spec.method == Method::weighted_normalized || spec.method == Method::weighted_unnormalized;
     // generalized (FP32, 0 <= alpha <= 1) shares the shared-node path: numerator and
     // denominator summed directly on the tiles, posting-list shares for the rare nodes
-    P.gen = spec.method == Method::generalized && !spec.vaw && !spec.fp64 && spec.plain_kernel == 0 &&
+    P.gen = spec.method == Method::generalized && !spec.vaw && spec.plain_kernel == 0 &&
             spec.alpha >= 0.0 && spec.alpha <= 1.0 && !(getenv("SSU_GEN_SPARSE") && atoi(getenv("SSU_GEN_SPARSE")) == 0);
     P.sparse = (weighted && !spec.vaw && spec.plain_kernel == 0) || P.gen;
     if (P.sparse) {
@@ -1095,7 +1099,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     // 16 x 16 super-tiles of 64-sample blocks, 6 x 6 of 128-sample Gram tiles:
     // about the number of blocks (tiles) resident on one XCD
     if (ok && P.sparse) {
-        P.block_order = spec.fp64 ? upper_block_order((P.n + 63) / 64, 16) : rect_block_order(P.n);
+        P.block_order = P.rect() ? rect_block_order(P.n) : upper_block_order((P.n + 63) / 64, 16);
         ok = upload(P.d_order, P.block_order, P.st, err);
     }
     if (ok && P.gram) {
@@ -1170,7 +1174,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             !P.d_poff.ensure((kk + 1) * sizeof(uint32_t), err) ||
             !P.d_pcnt.ensure(kk * sizeof(uint32_t), err) ||
             !P.d_Ssp.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err) ||
-            (P.gen && !P.d_out2.ensure(std::max<size_t>(nn, 1) * sizeof(float), err)) ||
+            (P.gen && !P.d_out2.ensure(std::max<size_t>(nn, 1) * osz, err)) ||
             (P.gen && !P.d_pspmax.ensure(sizeof(double), err)) ||
             !P.d_obsum.ensure((offset_scan_blocks(K) + 1) * sizeof(uint32_t), err) ||
             !P.d_walked.ensure((size_t)sparse_rows_blocks(P.n) * sizeof(uint32_t), err) ||
@@ -1276,7 +1280,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         b.thr = P.split_thr;
         b.ssp = P.d_Ssp.as<double>();
         b.gen = P.gen;
-        b.alpha = (float)s.alpha;
+        b.alpha = s.alpha;
         b.out2 = P.gen ? P.d_out2.p : nullptr;
         b.pspmax = P.gen ? P.d_pspmax.as<double>() : nullptr;
         b.band = P.band;
@@ -1381,8 +1385,9 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
                 a.n_blocks = P.band_off[k + 1] - P.band_off[k];
                 if (k & 1) bs = P.band_st2;
             }
-            if (s.fp64) SSU_HIP(launch_pairs_sparse(true, a, bs));
+            if (P.gen && s.fp64) SSU_HIP(launch_pairs_gen64(a, s.alpha, split ? P.d_out2.p : nullptr, bs));
             else if (P.gen) SSU_HIP(launch_pairs_gen32(a, (float)s.alpha, split ? P.d_out2.p : nullptr, bs));
+            else if (s.fp64) SSU_HIP(launch_pairs_sparse(true, a, bs));
             else SSU_HIP(launch_pairs_sparse32(a, bs));
             if (P.banded_run) SSU_HIP(hipEventRecord(P.band_ev[k], bs));
         }
@@ -1476,7 +1481,7 @@ bool Plan::refines() const { return p_->refine_on; }
 // samples; Gram: 128-sample tile rows, band 0 holding every diagonal tile).
 bool PlanImpl::prepare_bands(std::string* err) {
     if (!band_ev.empty()) return true;
-    const uint32_t ei = gram ? GRAM_BM : (spec.fp64 ? 64 : 32);
+    const uint32_t ei = gram ? GRAM_BM : (rect() ? 32 : 64);
     const uint32_t nbi = (n + ei - 1) / ei;
     uint32_t want = OUT_BANDS;
     if (const char* env = getenv("SSU_OUT_BANDS")) want = std::max(1, atoi(env));
@@ -1580,7 +1585,7 @@ bool Plan::set_stripes(uint32_t start, uint32_t stop, std::string* err) {
     P.run_order.clear();
     uint32_t ei = 0, ej = 0;   // block edges (rows, columns)
     if (P.gram) ei = ej = GRAM_BM;
-    else if (P.sparse && P.spec.fp64) ei = ej = 64;
+    else if (P.sparse && !P.rect()) ei = ej = 64;
     else if (P.sparse) { ei = 32; ej = 64; }
     for (uint32_t bij : P.block_order) {
         const uint32_t bi = bij >> 16, bj = bij & 0xffffu;
@@ -1737,14 +1742,14 @@ uint64_t Plan::kernel_terms() const {
     if (P.gram) return (uint64_t)P.order_used().size() * GRAM_BM * GRAM_BM * (uint64_t)P.Kp;
     if (P.sparse) {
         // visited nodes per block, times the pairs a block evaluates
-        const size_t nbi = P.spec.fp64 ? (P.n + 63) / 64 : (P.n + 31) / 32, nbj = (P.n + 63) / 64;
+        const size_t nbi = P.rect() ? (P.n + 31) / 32 : (P.n + 63) / 64, nbj = (P.n + 63) / 64;
         if (!nbj || !P.d_visits.p) return 0;
         std::vector<uint32_t> v(nbi * nbj);
         if (hipMemcpy(v.data(), P.d_visits.p, v.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
             return 0;
         uint64_t sum = 0;
         for (uint32_t bij : P.order_used()) sum += v[(size_t)(bij >> 16) * nbj + (bij & 0xffffu)];
-        return sum * (P.spec.fp64 ? 64 * 64 : 32 * 64);
+        return sum * (P.rect() ? 32 * 64 : 64 * 64);
     }
     // dense tiles: 128 x 128 for weighted / plain unweighted, 64 x 64 for generalized and VAW
     const bool small = P.spec.vaw || P.spec.method == Method::generalized;
@@ -1769,7 +1774,9 @@ double Plan::split_theta() const {
 }
 const char* Plan::main_kernel() const {
     if (p_->gram) return "k_gram_i8r";
-    if (p_->sparse) return p_->spec.fp64 ? "k_pairs_sparse" : (p_->gen ? "k_pairs_gen32" : "k_pairs_sparse32");
+    if (p_->sparse)
+        return p_->gen ? (p_->spec.fp64 ? "k_pairs_gen64" : "k_pairs_gen32")
+                       : (p_->spec.fp64 ? "k_pairs_sparse" : "k_pairs_sparse32");
     return "k_pairs";
 }
 const void* Plan::device_output() const { return p_->d_out.p; }

@@ -586,7 +586,67 @@ template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrt
 template <> __device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
 template <class T> __device__ __forceinline__ T dpow(T x, T a);
 template <> __device__ __forceinline__ float dpow<float>(float x, float a) { return powf(x, a); }
-template <> __device__ __forceinline__ double dpow<double>(double x, double a) { return pow(x, a); }
+
+// FP64 log2 of a positive normal x: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// log2 m = (2 / ln 2) atanh t with t = (m - 1) / (m + 1), |t| <= 0.1716, summed
+// as t (1 + t^2/3 + t^4/5 + ... + t^24/25) (the next term is below 2e-20 of the
+// sum): a few ulps, against the many branches of the general pow.
+__device__ __forceinline__ double log2_normal(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    int e = (int)((b >> 52) & 0x7ff) - 1022;
+    double m = __longlong_as_double((long long)((b & 0x000fffffffffffffull) | 0x3fe0000000000000ull));  // [0.5, 1)
+    if (m < 0.70710678118654752440) { m *= 2.0; --e; }
+    const double t = (m - 1.0) / (m + 1.0), t2 = t * t;
+    double p = 1.0 / 25.0;
+    p = fma(p, t2, 1.0 / 23.0);
+    p = fma(p, t2, 1.0 / 21.0);
+    p = fma(p, t2, 1.0 / 19.0);
+    p = fma(p, t2, 1.0 / 17.0);
+    p = fma(p, t2, 1.0 / 15.0);
+    p = fma(p, t2, 1.0 / 13.0);
+    p = fma(p, t2, 1.0 / 11.0);
+    p = fma(p, t2, 1.0 / 9.0);
+    p = fma(p, t2, 1.0 / 7.0);
+    p = fma(p, t2, 1.0 / 5.0);
+    p = fma(p, t2, 1.0 / 3.0);
+    p = fma(p, t2, 1.0);
+    return (double)e + (t * p) * 2.88539008177792681472;   // 2 / ln 2
+}
+
+// FP64 2^y for |y| < 1000: y = k + f with k = rint(y), |f| <= 1/2, and
+// 2^f = e^z, z = f ln 2, from the Taylor series to z^16/16! (|z| <= 0.347:
+// the next term is below 1e-21), scaled by 2^k in the exponent bits.
+__device__ __forceinline__ double exp2_finite(double y) {
+    const double k = rint(y);
+    const double z = (y - k) * 0.69314718055994530942;   // ln 2
+    double p = 1.0 / 20922789888000.0;                    // 1/16!
+    p = fma(p, z, 1.0 / 1307674368000.0);
+    p = fma(p, z, 1.0 / 87178291200.0);
+    p = fma(p, z, 1.0 / 6227020800.0);
+    p = fma(p, z, 1.0 / 479001600.0);
+    p = fma(p, z, 1.0 / 39916800.0);
+    p = fma(p, z, 1.0 / 3628800.0);
+    p = fma(p, z, 1.0 / 362880.0);
+    p = fma(p, z, 1.0 / 40320.0);
+    p = fma(p, z, 1.0 / 5040.0);
+    p = fma(p, z, 1.0 / 720.0);
+    p = fma(p, z, 1.0 / 120.0);
+    p = fma(p, z, 1.0 / 24.0);
+    p = fma(p, z, 1.0 / 6.0);
+    p = fma(p, z, 0.5);
+    p = fma(p, z, 1.0);
+    p = fma(p, z, 1.0);
+    return ldexp(p, (int)k);
+}
+
+// x^a for x > 0 in FP64: 2^(a log2 x) for the normal range, the general pow otherwise
+template <> __device__ __forceinline__ double dpow<double>(double x, double a) {
+    if (x >= 2.2250738585072014e-308 && x < 1.0e300) {
+        const double y = a * log2_normal(x);
+        if (y > -1000.0 && y < 1000.0) return exp2_finite(y);
+    }
+    return pow(x, a);
+}
 
 template <PairOp OP> struct PairTraits {
     static constexpr bool two = OP != PairOp::weighted;       // numerator and denominator accumulators
@@ -1035,16 +1095,31 @@ __device__ __forceinline__ void gen_term(float u, float v, float l, float alpha,
     num = __builtin_fmaf(q, __builtin_fabsf(u - v), num);
     den += w;
 }
+// FP64 as in the dense FP64 kernel: w = s^a l, num += w |u - v| / s, den += w
+__device__ __forceinline__ void gen_term(double u, double v, double l, double alpha, double& num, double& den) {
+    const double s = u + v;
+    const bool nz = s != 0.0;
+    const double ss = nz ? s : 1.0;
+    const double w = nz ? dpow<double>(ss, alpha) * l : 0.0;
+    num = fma(w, __builtin_fabs(u - v) / ss, num);
+    den += w;
+}
 
-template <class TOut, bool GEN>
-__device__ __forceinline__ void pairs_sparse32_body(const float* __restrict__ U, size_t ldn,
-                                                       const double* __restrict__ len,
+// V: the operand type.  FP32 operands are the gathered rows U[r] of the
+// processed nodes, summed in FP32 over S3_MERGE listed nodes and then merged
+// into FP64; FP64 operands (generalized only) are the embedding rows
+// U[rows[r]], summed in FP64.
+template <class TOut, bool GEN, class V = float>
+__device__ __forceinline__ void pairs_sparse32_body(const V* __restrict__ U, const int32_t* __restrict__ rows,
+                                                       size_t ldn, const double* __restrict__ len,
                                                        const uint32_t* __restrict__ masks, uint32_t Kw, uint32_t n,
                                                        const double* __restrict__ S, int normalized,
                                                        TOut* __restrict__ out, uint32_t* __restrict__ visits,
                                                        const uint32_t* __restrict__ order, uint32_t n_blocks,
                                                        uint32_t per_xcd, const uint32_t* __restrict__ dmask,
-                                                       int add_partial, float alpha, TOut* __restrict__ out2) {
+                                                       int add_partial, V alpha, TOut* __restrict__ out2) {
+    constexpr bool F64 = sizeof(V) == 8;
+    static_assert(!F64 || GEN, "FP64 operands take the generalized terms only");
     constexpr uint32_t EB = S3_EB;
     const uint32_t seq = (blockIdx.x % N_XCDS) * per_xcd + blockIdx.x / N_XCDS;
     if (seq >= n_blocks) return;
@@ -1054,20 +1129,20 @@ __device__ __forceinline__ void pairs_sparse32_body(const float* __restrict__ U,
     const uint32_t li = lane >> 3, lj = lane & 7;
 
     __shared__ uint16_t list[SP_WORDS * 32];
-    __shared__ __attribute__((aligned(16))) float sv[EB][S3_NODE];
-    __shared__ float sl[EB];
+    __shared__ __attribute__((aligned(16))) V sv[EB][S3_NODE];
+    __shared__ V sl[EB];
 
-    float acc[4][8];
+    V acc[4][8];
     double acc64[4][8];
-    float accd[GEN ? 4 : 1][GEN ? 8 : 1];        // GEN: the denominator
+    V accd[GEN ? 4 : 1][GEN ? 8 : 1];        // GEN: the denominator
     double accd64[GEN ? 4 : 1][GEN ? 8 : 1];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            acc[r][c] = 0.0f;
+            acc[r][c] = V(0);
             acc64[r][c] = 0.0;
-            if constexpr (GEN) { accd[r][c] = 0.0f; accd64[r][c] = 0.0; }
+            if constexpr (GEN) { accd[r][c] = V(0); accd64[r][c] = 0.0; }
         }
     uint32_t pending = 0;
 
@@ -1083,19 +1158,19 @@ __device__ __forceinline__ void pairs_sparse32_body(const float* __restrict__ U,
         const uint32_t total = build_list(list, word, lane);
         visited += total;
         const uint32_t base = w0 * 32;
-        float pA[EB], qA[EB], pB[EB], qB[EB];
-        float lA = 0.0f, lB = 0.0f;
+        V pA[EB], qA[EB], pB[EB], qB[EB];
+        V lA = V(0), lB = V(0);
         uint64_t vA = 0, vB = 0;
-        auto fetch = [&](float (&p)[EB], float (&q)[EB], float& l, uint64_t& valid, uint32_t t0)
+        auto fetch = [&](V (&p)[EB], V (&q)[EB], V& l, uint64_t& valid, uint32_t t0)
                          __attribute__((always_inline)) {
             const uint32_t idx = t0 + lane;
             const bool real = lane < EB && idx < total;
             int32_t row = 0;
-            l = 0.0f;
+            l = V(0);
             if (real) {
                 const uint32_t r = base + list[idx];
-                row = (int32_t)r;
-                l = (float)len[r];
+                row = F64 ? rows[r] : (int32_t)r;
+                l = (V)len[r];
             }
             valid = __ballot(real);
 #pragma unroll
@@ -1105,25 +1180,25 @@ __device__ __forceinline__ void pairs_sparse32_body(const float* __restrict__ U,
                 q[e] = U[off + offJ];
             }
         };
-        auto step = [&](float (&p)[EB], float (&q)[EB], float& l, uint64_t& valid, uint32_t t_next)
+        auto step = [&](V (&p)[EB], V (&q)[EB], V& l, uint64_t& valid, uint32_t t_next)
                         __attribute__((always_inline)) {
 #pragma unroll
             for (uint32_t e = 0; e < EB; ++e) {
                 const bool on = (valid >> e) & 1;
-                if (lane < S3_BI) sv[e][lane] = on ? p[e] : 0.0f;
-                sv[e][slotJ] = on ? q[e] : 0.0f;
+                if (lane < S3_BI) sv[e][lane] = on ? p[e] : V(0);
+                sv[e][slotJ] = on ? q[e] : V(0);
             }
             if (lane < EB) sl[lane] = l;
             __syncthreads();
             if (t_next < total) fetch(p, q, l, valid, t_next);
 #pragma unroll 2
             for (uint32_t e = 0; e < EB; ++e) {
-                float a[4], b[8];
+                V a[4], b[8];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) a[r] = sv[e][li * 4 + r];
 #pragma unroll
                 for (int c = 0; c < 8; ++c) b[c] = sv[e][S3_JOFF + lj * S3_GSJ + c];
-                const float ln = sl[e];
+                const V ln = sl[e];
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
 #pragma unroll
@@ -1134,17 +1209,19 @@ __device__ __forceinline__ void pairs_sparse32_body(const float* __restrict__ U,
                             acc[r][c] = __builtin_fmaf(__builtin_fabsf(a[r] - b[c]), ln, acc[r][c]);
                     }
             }
-            pending += EB;
-            if (pending >= S3_MERGE) {
+            if constexpr (!F64) {
+                pending += EB;
+                if (pending >= S3_MERGE) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
+                    for (int r = 0; r < 4; ++r)
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) {
-                        acc64[r][c] += (double)acc[r][c];
-                        acc[r][c] = 0.0f;
-                        if constexpr (GEN) { accd64[r][c] += (double)accd[r][c]; accd[r][c] = 0.0f; }
-                    }
-                pending = 0;
+                        for (int c = 0; c < 8; ++c) {
+                            acc64[r][c] += (double)acc[r][c];
+                            acc[r][c] = 0.0f;
+                            if constexpr (GEN) { accd64[r][c] += (double)accd[r][c]; accd[r][c] = 0.0f; }
+                        }
+                    pending = 0;
+                }
             }
             __syncthreads();
         };
@@ -1188,8 +1265,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         uint32_t Kw, uint32_t n, const double* __restrict__ S, int normalized, TOut* __restrict__ out,
         uint32_t* __restrict__ visits, const uint32_t* __restrict__ order, uint32_t n_blocks, uint32_t per_xcd,
         const uint32_t* __restrict__ dmask, int add_partial) {
-    pairs_sparse32_body<TOut, false>(U, ldn, len, masks, Kw, n, S, normalized, out, visits, order, n_blocks, per_xcd,
-                                     dmask, add_partial, 0.0f, nullptr);
+    pairs_sparse32_body<TOut, false>(U, nullptr, ldn, len, masks, Kw, n, S, normalized, out, visits, order, n_blocks,
+                                     per_xcd, dmask, add_partial, 0.0f, nullptr);
 }
 
 // generalized: numerator and denominator summed directly (two accumulator sets, 2 waves per SIMD)
@@ -1199,8 +1276,18 @@ __global__ void __launch_bounds__(64) k_pairs_gen32(
         uint32_t Kw, uint32_t n, TOut* __restrict__ out, TOut* __restrict__ out2, uint32_t* __restrict__ visits,
         const uint32_t* __restrict__ order, uint32_t n_blocks, uint32_t per_xcd, const uint32_t* __restrict__ dmask,
         int add_partial, float alpha) {
-    pairs_sparse32_body<TOut, true>(U, ldn, len, masks, Kw, n, nullptr, 1, out, visits, order, n_blocks, per_xcd,
-                                    dmask, add_partial, alpha, out2);
+    pairs_sparse32_body<TOut, true>(U, nullptr, ldn, len, masks, Kw, n, nullptr, 1, out, visits, order, n_blocks,
+                                    per_xcd, dmask, add_partial, alpha, out2);
+}
+
+// generalized on FP64 operands: the FP64 embedding rows, FP64 terms and sums (one wave per SIMD)
+__global__ void __launch_bounds__(64) k_pairs_gen64(
+        const double* __restrict__ U, const int32_t* __restrict__ rows, size_t ldn, const double* __restrict__ len,
+        const uint32_t* __restrict__ masks, uint32_t Kw, uint32_t n, double* __restrict__ out,
+        double* __restrict__ out2, uint32_t* __restrict__ visits, const uint32_t* __restrict__ order,
+        uint32_t n_blocks, uint32_t per_xcd, const uint32_t* __restrict__ dmask, int add_partial, double alpha) {
+    pairs_sparse32_body<double, true, double>(U, rows, ldn, len, masks, Kw, n, nullptr, 1, out, visits, order,
+                                              n_blocks, per_xcd, dmask, add_partial, alpha, out2);
 }
 
 // --------------------------------------------- sparse nodes: posting lists
@@ -1453,8 +1540,8 @@ template <class TV, bool GEN> struct alignas(16) RowEntry {
 };
 template <class TV> struct alignas(16) RowEntry<TV, true> {   // generalized
     uint32_t base;
-    float lu;                 // u_a
-    float uap;                // u_a^alpha
+    TV lu;                    // u_a
+    TV uap;                   // u_a^alpha
     unsigned long long q8;    // l * scale (FP64 bits)
 };
 
@@ -1462,6 +1549,9 @@ template <class TV> struct alignas(16) RowEntry<TV, true> {   // generalized
 __device__ __forceinline__ float pow_alpha(float x, float alpha) {
     return __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(x));
 }
+// the generalized terms' x^alpha in the value type: FP32 as above, FP64 as the FP64 kernels
+__device__ __forceinline__ float gen_pow(float x, double alpha) { return pow_alpha(x, (float)alpha); }
+__device__ __forceinline__ double gen_pow(double x, double alpha) { return dpow<double>(x, alpha); }
 
 // round(x * ls) as a signed integer (two's complement in the 64-bit
 // accumulators) for |x * ls| < 2^51: the FMA adds 1.5 * 2^52, so the sum lies
@@ -1485,14 +1575,26 @@ template <class T> __device__ __forceinline__ T ld_slot(const T* __restrict__ p,
 // array shifted so that accb[b] is column b (GEN: the denominator CW further on).
 template <class TV, bool GRAM, bool GEN, bool FINE, uint32_t CW>
 __device__ __forceinline__ void row_term(unsigned long long* accb, const RowEntry<TV, GEN>& en, uint32_t b, TV v,
-                                         float alpha) {
+                                         double alpha) {
     if constexpr (GRAM) {
         atomicAdd(&accb[b], en.q8);
+    } else if constexpr (GEN && sizeof(TV) == 8) {
+        // FP64 values: the same corrections in FP64, truncated to integers in units
+        // of 2^-61 of the row's bound (|x * ls| < 2^61)
+        const double uu = en.lu, vv = v, sm = uu + vv;
+        const double p1 = dpow<double>(sm, alpha - 1.0);
+        const double one = en.uap + dpow<double>(vv, alpha);
+        const double ls = __longlong_as_double((long long)en.q8);
+        const double cn = p1 * __builtin_fabs(uu - vv) - one;
+        const double cd = p1 * sm - one;
+        atomicAdd(&accb[b], (unsigned long long)(long long)(cn * ls));
+        atomicAdd(&accb[CW + b], (unsigned long long)(long long)(cd * ls));
     } else if constexpr (GEN) {
         // a pair's correction to u^a + v^a (held by both): numerator and denominator
+        const float af = (float)alpha;
         const float uu = (float)en.lu, vv = (float)v, sm = uu + vv;
-        const float p1 = __builtin_amdgcn_exp2f((alpha - 1.0f) * __builtin_amdgcn_logf(sm));
-        const double one = (double)en.uap + (double)pow_alpha(vv, alpha);
+        const float p1 = __builtin_amdgcn_exp2f((af - 1.0f) * __builtin_amdgcn_logf(sm));
+        const double one = (double)en.uap + (double)pow_alpha(vv, af);
         const double ls = __longlong_as_double((long long)en.q8);
         const double cn = (double)p1 * (double)__builtin_fabsf(uu - vv) - one;
         const double cd = (double)p1 * (double)sm - one;
@@ -1520,7 +1622,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                                                    uint32_t nchunks, uint32_t nkc, double* __restrict__ G,
                                                    uint32_t tiles, const double* __restrict__ ssp,
                                                    TOut* __restrict__ out2, const double* __restrict__ pspmax,
-                                                   float alpha, GapBand band, uint32_t a0, uint32_t na,
+                                                   double alpha, GapBand band, uint32_t a0, uint32_t na,
                                                    uint32_t mode) {
     // GEN keeps two accumulators (numerator, denominator) on half-width chunks
     constexpr uint32_t CW = GEN ? SR_CW / 2 : SR_CW;
@@ -1608,7 +1710,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     for (uint32_t r0 = e0 + 64 * wv; r0 < e1 && pb0 < pb1; r0 += 64 * SR_WAVES) {   // uniform per wave
         uint32_t start = 0, end = 0, r = 0, slot0 = 0;
         TV ua = (TV)0;
-        float uap = 0.0f;
+        TV uap = (TV)0;
         unsigned long long q = 0ull;
         if (r0 + lane < e1) {
             const unsigned long long ent = rows[r0 + lane];
@@ -1623,7 +1725,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             } else {
                 ua = pval[slot0 + pa];
                 q = (unsigned long long)__double_as_longlong(len[r] * scale);
-                if constexpr (GEN) uap = pow_alpha((float)ua, alpha);
+                if constexpr (GEN) uap = gen_pow(ua, alpha);
             }
         }
         const uint32_t cnt = end > start ? end - start : 0u;
@@ -1778,7 +1880,7 @@ __global__ void __launch_bounds__(64) k_sparse_sums(double* __restrict__ ssp,
                                                     const uint32_t* __restrict__ offsets, uint32_t nkc,
                                                     const uint32_t* __restrict__ poff, const TV* __restrict__ pval,
                                                     const double* __restrict__ len, uint32_t K, uint32_t n,
-                                                    uint32_t slots, float alpha) {
+                                                    uint32_t slots, double alpha) {
     const uint32_t a = blockIdx.x, lane = threadIdx.x;
     if (a >= n) return;
     double x = 0.0;
@@ -1789,7 +1891,7 @@ __global__ void __launch_bounds__(64) k_sparse_sums(double* __restrict__ ssp,
             const uint32_t r = (uint32_t)(ent >> 32);
             const TV v = pval[poff[r] + (uint32_t)ent];
             if constexpr (GEN)
-                x += len[r] * (double)pow_alpha((float)v, alpha);   // (GEN: l u^a)
+                x += len[r] * (double)gen_pow(v, alpha);   // (GEN: l u^a)
             else
                 x += len[r] * (double)v;
         }
@@ -2294,8 +2396,15 @@ hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st, uint32_t
     if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
     const bool narrow = a.n <= 65536;
     const bool whole = row0 == 0 && row1 == a.n;   // the per-sample sums are made for all rows at once
-    if (a.gen) {   // generalized, FP32 values
-        if (a.gram || a.values_fp64 || !a.ssp || !a.out2 || !a.pspmax || !whole) return hipErrorInvalidValue;
+    if (a.gen && a.values_fp64) {   // generalized, FP64 values
+        if (a.gram || !a.ssp || !a.out2 || !a.pspmax || !whole) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_sparse_sums<double, true>), dim3(a.n), dim3(64), 0, st, a.ssp, a.rows, a.roff, nkc,
+                           (const uint32_t*)a.poff, (const double*)a.pval, a.len, a.K, a.n, a.slots, a.alpha);
+        hipLaunchKernelGGL(k_max, dim3(1), dim3(256), 0, st, a.pspmax, a.ssp, a.n);
+        if (narrow) pairs_rows_t<uint16_t, false, double, true>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode);
+        else pairs_rows_t<uint32_t, false, double, true>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode);
+    } else if (a.gen) {   // generalized, FP32 values
+        if (a.gram || !a.ssp || !a.out2 || !a.pspmax || !whole) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_sparse_sums<float, true>), dim3(a.n), dim3(64), 0, st, a.ssp, a.rows, a.roff, nkc,
                            (const uint32_t*)a.poff, (const float*)a.pval, a.len, a.K, a.n, a.slots, a.alpha);
         hipLaunchKernelGGL(k_max, dim3(1), dim3(256), 0, st, a.pspmax, a.ssp, a.n);
@@ -2362,6 +2471,20 @@ hipError_t launch_pairs_gen32(const SparseArgs& a, float alpha, void* out2, hipS
     hipLaunchKernelGGL((k_pairs_gen32<float>), dim3(per_xcd * N_XCDS), dim3(64), 0, st, (const float*)a.U, a.ldn, a.len,
                        a.masks, a.Kw, a.n, (float*)a.out, (float*)out2, a.visits, a.order, a.n_blocks, per_xcd,
                        a.dmask, (int)(a.dmask != nullptr), alpha);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs_gen64(const SparseArgs& a, double alpha, void* out2, hipStream_t st) {
+    if (a.ldn % S3_BJ || !a.out_fp64 || !a.rows) return hipErrorInvalidValue;
+    const uint32_t nbj = (a.n + S3_BJ - 1) / S3_BJ;
+    if (!nbj) return hipSuccess;
+    if (a.dmask && !out2) return hipErrorInvalidValue;
+    if (!a.order) return hipErrorInvalidValue;
+    if (!a.n_blocks) return hipSuccess;   // a stripe selection without a block here
+    const uint32_t per_xcd = (a.n_blocks + N_XCDS - 1) / N_XCDS;
+    hipLaunchKernelGGL(k_pairs_gen64, dim3(per_xcd * N_XCDS), dim3(64), 0, st, (const double*)a.U, a.rows, a.ldn,
+                       a.len, a.masks, a.Kw, a.n, (double*)a.out, (double*)out2, a.visits, a.order, a.n_blocks,
+                       per_xcd, a.dmask, (int)(a.dmask != nullptr), alpha);
     return hipGetLastError();
 }
 

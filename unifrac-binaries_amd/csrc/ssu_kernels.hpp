@@ -246,8 +246,8 @@ struct SparseNodesArgs {
     uint32_t* rbsum;        // [row_list_scan_blocks(n, K) + 1]
     uint32_t K, n, npan;
     double* ssp = nullptr;  // (weighted, FP32 values) [n]: per-sample sum of l * u (GEN: l * u^alpha) over its sparse nodes
-    bool gen = false;       // generalized (FP32 values): numerator shares to out, denominator shares to out2
-    float alpha = 1.0f;
+    bool gen = false;       // generalized: numerator shares to out, denominator shares to out2
+    double alpha = 1.0;
     void* out2 = nullptr;
     double* pspmax = nullptr;   // (GEN) [1] scratch: max of ssp
     double* gram = nullptr; // unweighted: the Gram tiles of launch_gram_i8
@@ -289,6 +289,9 @@ hipError_t launch_pairs_sparse32(const SparseArgs& a, hipStream_t st);
 // directly over the nodes either panel holds; with dmask, adds the posting-list shares in
 // out (numerator) and out2 (denominator)
 hipError_t launch_pairs_gen32(const SparseArgs& a, float alpha, void* out2, hipStream_t st);
+// the same on FP64 operands (U: the FP64 embedding, row rows[r] for processed node r),
+// FP64 terms, sums and output
+hipError_t launch_pairs_gen64(const SparseArgs& a, double alpha, void* out2, hipStream_t st);
 
 // --------------------------------------------------------- output layouts
 // stripes[s - start][k] = M[k][(k+s+1) % n]   (double output)

@@ -20,6 +20,7 @@ enum class SplitKind : int {
     weighted64 = 1,     // FP64 weighted: 64 x 64 blocks over the nodes both panels hold
     generalized32 = 2,  // FP32 generalized: 32 x 64 blocks, both panels' nodes, two sums
     gram = 3,           // unweighted: int8 Gram columns (every pair, no masks)
+    generalized64 = 4,  // FP64 generalized: as generalized32 with FP64 operands and terms
 };
 
 struct SplitChoice {
