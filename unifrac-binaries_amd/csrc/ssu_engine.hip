@@ -86,9 +86,11 @@ bool upload_raw(DevMem& d, const void* h, size_t bytes, hipStream_t st, std::str
 // so freed blocks stay with the process and later plans of similar sizes reuse
 // them: sizes are rounded to 2 MiB, a request takes the smallest cached block
 // of its device that is large enough and at most a quarter larger, and the
-// cache keeps at most SSU_POOL_GIB GiB (default 96; 0 turns caching off),
-// releasing its largest blocks beyond that.  A failed hipMalloc empties the
-// cache and tries once more.  ssu_release_device_memory() empties it.
+// cache keeps at most SSU_POOL_GIB GiB (default: 90% of the first device's
+// memory, so that the plans of a 65,536-sample matrix stay cached between
+// calls; 0 turns caching off), releasing its largest blocks beyond that.  A
+// failed hipMalloc empties the cache and tries once more.
+// ssu_release_device_memory() empties it.
 namespace {
 struct Pool {
     std::mutex mu;
@@ -104,6 +106,9 @@ Pool& pool() {
 size_t pool_cap(Pool& P) {
     if (!P.cap_set) {
         double gib = 96.0;
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0) gib = 0.9 * (double)total_b / (1ull << 30);
+        else (void)hipGetLastError();
         if (const char* env = getenv("SSU_POOL_GIB")) gib = atof(env);
         P.cap = gib > 0 ? (size_t)(gib * (double)(1ull << 30)) : 0;
         P.cap_set = true;
@@ -226,49 +231,53 @@ void parallel_chunks(uint64_t n, size_t parts, F fn) {
     for (auto& th : pool) th.join();
 }
 
-// Host memcpy over a persistent pool of worker threads (started on first use,
-// up to 16): the calling thread takes one part and waits for the rest.
+// Host work over a persistent pool of worker threads (started on first use, up
+// to 16 with the caller): fn(part) for part = 0 .. parts() - 1, the calling
+// thread taking the last part and waiting for the rest.
 class CopyPool {
 public:
     static CopyPool& get() {
         static CopyPool* p = new CopyPool();   // never joined: the workers wait for the process to end
         return *p;
     }
-    void copy(char* dst, const char* src, size_t bytes) {
-        if (workers_ == 0 || bytes < (size_t(4) << 20)) { std::memcpy(dst, src, bytes); return; }
-        std::lock_guard<std::mutex> one(call_mu_);   // one copy at a time on the pool
+    unsigned parts() const { return workers_ + 1; }
+    void run(const std::function<void(unsigned)>& fn) {
+        if (workers_ == 0) { fn(0); return; }
+        std::lock_guard<std::mutex> one(call_mu_);   // one job at a time on the pool
         std::unique_lock<std::mutex> lk(mu_);
-        const unsigned parts = workers_ + 1;
-        part_ = (bytes / parts + 4095) / 4096 * 4096;
-        dst_ = dst;
-        src_ = src;
-        bytes_ = bytes;
+        fn_ = &fn;
         pending_ = workers_;
         ++gen_;
         cv_.notify_all();
         lk.unlock();
-        const size_t a = part_ * workers_;   // the caller's part: the last
-        if (a < bytes) std::memcpy(dst + a, src + a, bytes - a);
+        fn(workers_);
         lk.lock();
         done_.wait(lk, [&] { return pending_ == 0; });
+        fn_ = nullptr;
+    }
+    void copy(char* dst, const char* src, size_t bytes) {
+        if (workers_ == 0 || bytes < (size_t(4) << 20)) { std::memcpy(dst, src, bytes); return; }
+        const size_t part = (bytes / parts() + 4095) / 4096 * 4096;
+        run([&](unsigned w) {
+            const size_t a = part * w;
+            if (a < bytes) std::memcpy(dst + a, src + a, w + 1 == parts() ? bytes - a : std::min(part, bytes - a));
+        });
     }
 private:
     CopyPool() {
         const unsigned hc = std::thread::hardware_concurrency();
         workers_ = std::min(16u, std::max(hc, 1u)) - 1;
-        for (unsigned w = 0; w < workers_; ++w) std::thread([this, w] { run(w); }).detach();
+        for (unsigned w = 0; w < workers_; ++w) std::thread([this, w] { loop(w); }).detach();
     }
-    void run(unsigned w) {
+    void loop(unsigned w) {
         uint64_t seen = 0;
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
             cv_.wait(lk, [&] { return gen_ != seen; });
             seen = gen_;
-            char* dst = dst_;
-            const char* src = src_;
-            const size_t a = part_ * w, bytes = bytes_, part = part_;
+            const std::function<void(unsigned)>* fn = fn_;
             lk.unlock();
-            if (a < bytes) std::memcpy(dst + a, src + a, std::min(part, bytes - a));
+            (*fn)(w);
             lk.lock();
             if (--pending_ == 0) done_.notify_one();
         }
@@ -277,9 +286,7 @@ private:
     std::condition_variable cv_, done_;
     unsigned workers_ = 0, pending_ = 0;
     uint64_t gen_ = 0;
-    char* dst_ = nullptr;
-    const char* src_ = nullptr;
-    size_t bytes_ = 0, part_ = 0;
+    const std::function<void(unsigned)>* fn_ = nullptr;
 };
 
 // dst[0..bytes) = src[0..bytes) on the copy pool
@@ -287,7 +294,7 @@ void spread_copy(char* dst, const char* src, size_t bytes, unsigned) { CopyPool:
 }  // namespace
 
 bool copy_ranges_to_host(char* host, const char* dev, const std::vector<CopyRange>& ranges, void* stream,
-                         std::string* err) {
+                         std::string* err, bool prefault) {
     hipStream_t st = (hipStream_t)stream;
     size_t bytes = 0;
     for (const CopyRange& r : ranges) bytes += r.len;
@@ -328,6 +335,29 @@ bool copy_ranges_to_host(char* host, const char* dev, const std::vector<CopyRang
     };
     const size_t np = pieces.size();
     for (size_t c = 0; c < std::min<size_t>(np, STAGE_BUFS); ++c) SSU_HIP(issue(c));
+    // While the first piece is still on its way (its range computing), touch the
+    // destination's pages on the pool: first-touch page faults of a freshly
+    // allocated result would otherwise slow every host copy
+    size_t touched = 0, span = 0;
+    double touch_ms = 0.0;
+    if (prefault && !ranges.empty()) {
+        const size_t lo = ranges.front().off;
+        for (const CopyRange& r : ranges) span = std::max(span, r.off + r.len - lo);
+        const auto tt = std::chrono::steady_clock::now();
+        constexpr size_t TOUCH = size_t(256) << 20, PAGE = 4096;
+        CopyPool& pool = CopyPool::get();
+        while (touched < span && hipEventQuery(S.done[0]) == hipErrorNotReady) {
+            const size_t len = std::min(TOUCH, span - touched);
+            char* base = host + lo + touched;
+            const size_t pages = (len + PAGE - 1) / PAGE, parts = pool.parts();
+            pool.run([&](unsigned t) {
+                for (size_t q = pages * t / parts; q < pages * (t + 1) / parts; ++q)
+                    *reinterpret_cast<volatile char*>(base + q * PAGE) = 0;
+            });
+            touched += len;
+        }
+        touch_ms = since(tt);
+    }
     for (size_t c = 0; c < np; ++c) {
         const int b = (int)(c % STAGE_BUFS);
         auto tw = std::chrono::steady_clock::now();
@@ -339,8 +369,10 @@ bool copy_ranges_to_host(char* host, const char* dev, const std::vector<CopyRang
         if (c + STAGE_BUFS < np) SSU_HIP(issue(c + STAGE_BUFS));   // into the buffer just emptied
     }
     if (timing)
-        fprintf(stderr, "[ssu copy] %8.1f MB in %zu pieces, %7.2f ms (waiting %7.2f ms, host copies %7.2f ms)\n",
-                bytes / 1e6, np, since(t0), wait_ms, host_ms);
+        fprintf(stderr,
+                "[ssu copy] %8.1f MB in %zu pieces, %7.2f ms (waiting %7.2f ms, host copies %7.2f ms, %.1f MB of "
+                "pages touched first in %.2f ms)\n",
+                bytes / 1e6, np, since(t0), wait_ms, host_ms, touched / 1e6, touch_ms);
     return true;
 }
 
@@ -1538,7 +1570,7 @@ bool Plan::execute_to_host(void* host, std::string* err) {
             if (k > 0) r.ready.push_back((void*)P.band_ev[k - 1]);
             ranges.push_back(r);
         }
-        return copy_ranges_to_host((char*)host, P.d_out.as<char>(), ranges, P.copy_st, e);
+        return copy_ranges_to_host((char*)host, P.d_out.as<char>(), ranges, P.copy_st, e, true);
     };
     const bool ok = execute(err);
     P.banded_run = false;

@@ -139,12 +139,14 @@ bool copy_to_host(void* host, const void* dev, size_t bytes, void* stream, std::
 // The same for byte ranges [off, off + len) of dev into host at the same
 // offsets, in order; a range's copies wait on the device (on `stream`) for the
 // events in `ready`, so later ranges can still be computing when this starts.
+// prefault: while the first copy waits, touch the destination's pages (a
+// freshly allocated result) so that the host copies do not take the faults.
 struct CopyRange {
     size_t off, len;
     std::vector<void*> ready;   // hipEvent_t
 };
 bool copy_ranges_to_host(char* host, const char* dev, const std::vector<CopyRange>& ranges, void* stream,
-                         std::string* err);
+                         std::string* err, bool prefault = false);
 // the reverse, host to device; returns once the host buffer has been read
 bool copy_to_device(void* dev, const void* host, size_t bytes, void* stream, std::string* err);
 
