@@ -141,6 +141,9 @@ def cpu_baseline(tree, table, methods, budget_s):
         per[label] = {"stripe_s": slope, "setup_s": intercept, "full_matrix_s": full}
         total_time += full
         log(f"cpu baseline {label}: {slope:.3f} s/stripe, setup {intercept:.2f} s, full {full:.0f} s")
+    per["_host"] = {"isa": O.REF_ISA if kind == "reference" else "gcc -O3 (port)",
+                    "threads_from": "OMP_NUM_THREADS" if os.environ.get("OMP_NUM_THREADS") else "sched_getaffinity",
+                    "affinity_cpus": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
     return kind, threads, total_time, per
 
 
@@ -252,7 +255,7 @@ def bench_rarefy(args):
                "sample": f"one replicate of the {b1.n}-sample table: stripes [0,1) and [0,3) of each matrix "
                          f"(FP32 reference kernels, OpenMP), extrapolated to all {(b1.n + 1) // 2} stripes; "
                          f"host subsampling not timed",
-               "per_method": per}
+               "per_method": {k: v for k, v in per.items() if k != "_host"}, **per["_host"]}
     out = {
         "metric": "pair-node updates/sec, batched rarefaction (4 tables x 32 replicates, depth 1000)",
         "value": value,
@@ -474,7 +477,7 @@ def main():
                "kind": kind,
                "sample": f"stripes [0,1) and [0,3) of both matrices (FP32 reference kernels, OpenMP), "
                          f"extrapolated linearly to all {(n + 1) // 2} stripes",
-               "per_method": per}
+               "per_method": {k: v for k, v in per.items() if k != "_host"}, **per["_host"]}
 
     out = {
         "metric": "pair-node updates/sec, 16k samples (unweighted, weighted); max rel err vs FP64",

@@ -25,7 +25,26 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
-REF_PATH = os.path.join(HERE, "_ref", "libssu_ref.so")
+REF_PATH_V3 = os.path.join(HERE, "_ref", "libssu_ref.so")
+REF_PATH_V4 = os.path.join(HERE, "_ref", "libssu_ref_v4.so")
+
+
+def _host_has_v4() -> bool:
+    """x86-64-v4 needs AVX-512 F, BW, CD, DQ and VL."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    return {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags
+    except OSError:
+        pass
+    return False
+
+
+# the AVX-512 build where the host runs it, else the AVX2 one
+REF_ISA = "x86-64-v4" if os.path.exists(REF_PATH_V4) and _host_has_v4() else "x86-64-v3"
+REF_PATH = REF_PATH_V4 if REF_ISA == "x86-64-v4" else REF_PATH_V3
 
 METHODS = {
     "unweighted": 0,
