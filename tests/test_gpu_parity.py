@@ -183,6 +183,17 @@ def test_empty_sample_and_zero_lengths(ssu, O, synth):
         s, m = ssu.matrix_inmem(ssu.Inputs(st, le, nm, *table), method)
         ref = oracle_matrix(O, (st, le, nm), table, method.replace("_fp64", ""))
         assert s == ssu.OKAY and rel_err(m, ref) <= 1e-11
+    for method, tol in (("generalized_fp64", 1e-11), ("generalized", 1e-6)):   # the shared-node path
+        for theta in ("0", "0.5"):
+            say("empty sample", method, theta)
+            os.environ["SSU_SPARSE_SPLIT"] = theta
+            try:
+                s, m = ssu.matrix_inmem(ssu.Inputs(st, le, nm, *table), method, alpha=0.5)
+            finally:
+                del os.environ["SSU_SPARSE_SPLIT"]
+            ref = oracle_matrix(O, (st, le, nm), table, "generalized", alpha=0.5)
+            assert s == ssu.OKAY and rel_err(m, ref) <= tol
+            assert np.array_equal(np.isnan(m), np.isnan(ref))
     zero = np.zeros_like(le)
     s, m = ssu.matrix_inmem(ssu.Inputs(st, zero, nm, *table), "unweighted_fp64")
     off = ~np.eye(10, dtype=bool)
