@@ -438,10 +438,11 @@ SSU_EXTERN int ssu_last_call_timings(double* ms, int cap);
 /* Device buffers freed by finished calls stay cached in the process (mapping
  * and unmapping multi-gigabyte buffers costs more than the kernels); this
  * returns every cached buffer to the runtime.  SSU_POOL_GIB caps the cache
- * (default 96 GiB; 0: no cache). */
+ * (default 90% of device memory; 0: no cache). */
 SSU_EXTERN void ssu_release_device_memory(void);
 /* The occupancy-split threshold a plan picks (host only): kind 0 FP32 weighted,
- * 1 FP64 weighted, 2 FP32 generalized, 3 unweighted Gram, 4 FP64 generalized;
+ * 1 FP64 weighted, 2 FP32 generalized, 3 unweighted Gram, 4 FP64 generalized,
+ * 5 FP32 VAW, 6 FP64 VAW (one-sided terms priced at n_samples distinct totals);
  * occupancy[k] = number of samples holding node k.  Returns the threshold (nodes held by fewer samples
  * are summed from posting lists; 0: no split) and the modelled milliseconds
  * with and without it. */

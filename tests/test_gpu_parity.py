@@ -194,6 +194,18 @@ def test_empty_sample_and_zero_lengths(ssu, O, synth):
             ref = oracle_matrix(O, (st, le, nm), table, "generalized", alpha=0.5)
             assert s == ssu.OKAY and rel_err(m, ref) <= tol
             assert np.array_equal(np.isnan(m), np.isnan(ref))
+    for method in ("unweighted", "weighted_normalized", "weighted_unnormalized"):   # VAW, both paths
+        for prec, tol in (("_fp64", 1e-11), ("", 1e-6)):
+            for theta in ("0", "0.5"):
+                say("empty sample vaw", method + prec, theta)
+                os.environ["SSU_VAW_SPLIT"] = theta
+                try:
+                    s, m = ssu.matrix_inmem(ssu.Inputs(st, le, nm, *table), method + prec, variance_adjust=True)
+                finally:
+                    del os.environ["SSU_VAW_SPLIT"]
+                ref = oracle_matrix(O, (st, le, nm), table, method, variance_adjust=True)
+                assert s == ssu.OKAY and rel_err(m, ref) <= tol
+                assert np.array_equal(np.isnan(m), np.isnan(ref))
     zero = np.zeros_like(le)
     s, m = ssu.matrix_inmem(ssu.Inputs(st, zero, nm, *table), "unweighted_fp64")
     off = ~np.eye(10, dtype=bool)
@@ -658,6 +670,82 @@ def test_generalized_fp64_shared_node_path(ssu, O, synth, monkeypatch, theta, al
     s, dense = ssu.matrix_inmem(inp, "generalized_fp64", alpha=alpha, fp32=False)
     assert s == ssu.OKAY
     assert rel_err(m, dense) <= TOL["fp64"]
+
+
+VAW_SPLIT_METHODS = ["unweighted", "unweighted_unnormalized", "weighted_normalized", "weighted_unnormalized"]
+
+
+@pytest.mark.parametrize("theta", ["0.1", "0.5", "1"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("method", VAW_SPLIT_METHODS)
+def test_vaw_shared_node_path(ssu, O, synth, monkeypatch, method, prec, theta):
+    """VAW (weighted and presence terms) on the shared-node path: nodes held by
+    at least theta * n samples on the dense pair kernel, the others from the
+    posting lists as one-sided sums at the table's distinct totals plus the
+    walk's corrections.  Against the oracle at the precision's tolerance and
+    against the dense kernel (SSU_VAW_SPLIT=0)."""
+    tree, table = _case(synth, 500, 300, 0.05, 81, zipf=1.1)
+    monkeypatch.setenv("SSU_VAW_SPLIT", theta)
+    inp = inputs_from(ssu, tree, table)
+    name = method + ("_fp64" if prec == "fp64" else "")
+    say("vaw shared-node", name, theta)
+    p = ssu.Plan(inp, name, variance_adjust=True)
+    assert p.main_kernel() == "k_pairs"
+    assert abs(p.split_theta() - float(theta)) < 0.01
+    p.execute()
+    m = p.matrix()
+    assert p.sparse_terms() > 0
+    p.close()
+    ref = oracle_matrix(O, tree, table, method, variance_adjust=True)
+    err = rel_err(m, ref)
+    say("max rel err", err)
+    assert err <= TOL[prec]
+    assert np.array_equal(m, m.T) and np.all(np.diag(m) == 0)
+    monkeypatch.setenv("SSU_VAW_SPLIT", "0")
+    s, dense = ssu.matrix_inmem(inp, name, variance_adjust=True)
+    assert s == ssu.OKAY
+    assert rel_err(m, dense) <= 2 * TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_vaw_shared_node_stripes_and_equal_totals(ssu, O, synth, monkeypatch, prec):
+    """VAW on the shared-node path: stripe ranges give the full matrix's
+    stripes; a table whose samples all have one total (one distinct total for
+    the one-sided sums) and one of near-duplicate samples (where the one-sided
+    sums and the walk's corrections cancel almost entirely) against the oracle."""
+    monkeypatch.setenv("SSU_VAW_SPLIT", "0.5")
+    tree, table = _case(synth, 400, 200, 0.05, 83)
+    inp = inputs_from(ssu, tree, table)
+    name = "weighted_normalized" + ("_fp64" if prec == "fp64" else "")
+    p = ssu.Plan(inp, name, variance_adjust=True)
+    p.execute()
+    full = p.stripes(0, 100)
+    for a, b in ((0, 13), (13, 60), (60, 100)):
+        say("vaw stripes", prec, a, b)
+        p.execute_stripes(a, b)
+        assert np.array_equal(p.stripes(a, b), full[a:b])
+    p.close()
+    # every sample scaled to total 1000 (integer counts): one distinct total
+    obs, sids, ip, ix, d = table
+    tot = np.bincount(ix, weights=d, minlength=len(sids))
+    d2 = np.maximum(1.0, np.round(d * 1000.0 / tot[ix]))
+    tot2 = np.bincount(ix, weights=d2, minlength=len(sids))
+    for smp in range(len(sids)):   # the rounding's remainder onto each sample's largest count
+        at = np.nonzero(ix == smp)[0]
+        d2[at[np.argmax(d2[at])]] += 1000.0 - tot2[smp]
+    assert np.all(d2 > 0) and np.all(np.bincount(ix, weights=d2, minlength=len(sids)) == 1000.0)
+    table2 = (obs, sids, ip, ix, d2)
+    tree3, table3 = near_duplicate_table(synth)
+    for case, (tr, tb) in (("one total", (tree, table2)), ("near duplicates", (tree3, table3))):
+        for method in ("weighted_normalized", "unweighted"):
+            name = method + ("_fp64" if prec == "fp64" else "")
+            say("vaw", case, name)
+            s, m = ssu.matrix_inmem(inputs_from(ssu, tr, tb), name, variance_adjust=True)
+            assert s == ssu.OKAY
+            ref = oracle_matrix(O, tr, tb, method, variance_adjust=True)
+            err = rel_err(m, ref)
+            say("max rel err", err)
+            assert err <= TOL[prec]
 
 
 def test_generalized_near_duplicates(ssu, O, synth):

@@ -317,6 +317,33 @@ def test_generalized_fp32_replicates_and_replays(ssu, O, synth):
         assert np.array_equal(r.matrix(0, rep), eager[rep]), rep
 
 
+def test_vaw_shared_node_replicates_and_replays(ssu, O, synth, monkeypatch):
+    """VAW on the shared-node path in rarefied replicates (the executions bring
+    their own totals: one-sided sums at every sample's total): every replicate
+    against the oracle, graph replays equal eager."""
+    monkeypatch.setenv("SSU_VAW_SPLIT", "0.5")
+    tree, table = small_case(synth, seed=57)
+    depth = depth_dropping_some(table, 10)
+    methods = ["weighted_normalized_fp64", "unweighted", "weighted_unnormalized"]
+    say("rarefy vaw split", depth)
+    r = ssu.Rarefaction(inputs(ssu, tree, table), methods, depth, 3, seed=9, variance_adjust=True)
+    r.execute()
+    si, oi, ip, ix = r.kept()
+    st, le, nm = tree
+    otree = O.Tree(np.asarray(st, np.uint8), le, nm)
+    eager = {(k, rep): r.matrix(k, rep) for k in range(len(methods)) for rep in range(3)}
+    for rep in range(3):
+        t = O.Table([table[0][i] for i in oi], [table[1][s] for s in si], ip, ix, r.counts(rep))
+        for k, m in enumerate(methods):
+            base = m[:-5] if m.endswith("_fp64") else m
+            err = rel_err(eager[(k, rep)], O.matrix(otree, t, base, variance_adjust=True))
+            assert err <= TOL["fp64" if m.endswith("_fp64") else "fp32"], (m, rep, err)
+    r.execute()
+    assert r.graph_used()
+    for (k, rep), m in eager.items():
+        assert np.array_equal(r.matrix(k, rep), m), (methods[k], rep)
+
+
 def test_near_duplicate_refinement_in_replicates(ssu, O, synth):
     """Near-duplicate samples survive the draws (samples 0..39 hold exactly
     `depth` items and are taken whole without replacement; 40..79 repeat them

@@ -246,3 +246,11 @@ def test_split_model_choices(ssu):
         assert 0 < thr <= n and ms <= ms0
     assert got[2][0] > got[0][0]                                          # generalized splits higher
     assert got[3][0] < got[0][0]                                          # the Gram column is cheap
+    # VAW (5 FP32, 6 FP64): one-sided terms priced at n totals; the FP64 dense term
+    # costs more, which moves the threshold up; unknown kinds choose no split
+    n8 = 8192
+    occ8 = np.minimum(n8, (rng.pareto(1.2, 50000) * 20).astype(np.int64) + 1)
+    v32, v64 = ssu.choose_split(5, n8, occ8), ssu.choose_split(6, n8, occ8)
+    assert 0 < v32[0] <= n8 and v32[1] < v32[2] and 0 < v64[0] <= n8 and v64[1] < v64[2]
+    assert v64[0] >= v32[0]
+    assert ssu.choose_split(7, n8, occ8)[0] == 0

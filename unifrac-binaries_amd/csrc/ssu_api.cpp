@@ -1568,9 +1568,11 @@ void ssu_release_device_memory(void) { ssu::pool_release_all(); }
 
 unsigned int ssu_choose_split(int kind, unsigned int n_samples, const uint32_t* occupancy, unsigned int n_nodes,
                               double* model_ms, double* model_ms_none) {
-    if (kind < 0 || kind > 4) return 0;
+    if (kind < 0 || kind > 6) return 0;
     std::vector<uint32_t> occ(occupancy, occupancy + n_nodes);
-    const ssu::SplitChoice c = ssu::choose_split((ssu::SplitKind)kind, n_samples, occ, 1ull << 30);
+    // VAW (5, 6): one-sided terms at every sample's total (no two totals equal)
+    const uint32_t cols = kind >= 5 ? n_samples : 0u;
+    const ssu::SplitChoice c = ssu::choose_split((ssu::SplitKind)kind, n_samples, occ, 1ull << 30, cols);
     if (model_ms) *model_ms = c.model_ms;
     if (model_ms_none) *model_ms_none = c.model_ms_none;
     return c.thr;

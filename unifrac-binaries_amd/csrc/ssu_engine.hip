@@ -432,6 +432,16 @@ struct PlanImpl {
     bool gram = false;
     bool sparse = false;             // weighted (or generalized) through the shared-node kernels
     bool gen = false;                // generalized on the shared-node path
+    // VAW on the shared-node path: nodes held by at least split_thr samples of the
+    // plan's table (vdense_mask) stay with the dense pair kernel (its rows vrows,
+    // lengths vlen_*), the others go to the posting lists; the one-sided sums are
+    // taken at the table's distinct totals tau_h (sample s at kidx_h[s])
+    bool vsplit = false;
+    uint32_t n_vdense = 0;
+    std::vector<uint32_t> vdense_mask, kidx_h;
+    std::vector<int32_t> vrows;
+    std::vector<float> vlen_f;
+    std::vector<double> vlen_d, tau_h;
     uint32_t split_thr = 0;          // FP32 shared-node path: nodes held by fewer samples use posting lists (0: off)
     double split_theta = 0.0;        // the occupancy split chosen (threshold / n; 0: none)
     uint32_t split_slots = 0;        // bound on their postings in any execution
@@ -516,7 +526,7 @@ struct PlanImpl {
     DevMem d_knodes, d_klen, d_klen_sum, d_len_all, d_E, d_C, d_U, d_Cu, d_S, d_Spart, d_masks, d_visits, d_order, d_Pt, d_digits, d_chunks, d_G, d_col, d_out,
         d_faith, d_stripes, d_smask, d_dmask, d_pc, d_pidx, d_pval, d_walked, d_rows, d_rcount, d_roff, d_rbsum,
         d_sp_node, d_sp_len, d_umasks, d_B, d_poff, d_pcnt, d_obsum, d_Ssp, d_out2, d_pspmax, d_urare, d_close,
-        d_ridx, d_order_run, d_order_band;
+        d_ridx, d_order_run, d_order_band, d_vdense, d_vrows, d_vlen, d_tau, d_kidx, d_H, d_pcv, d_nbound;
 
     hipStream_t own_st = nullptr;    // created with the plan; st may point at a caller's stream
     // per-execution inputs and output (nullptr: the plan's own table data / totals / result)
@@ -546,6 +556,7 @@ struct PlanImpl {
     bool build_gram_columns();
     bool split_gram(std::string* err);
     bool split_weighted(std::string* err);
+    bool split_vaw(std::string* err);
     bool structural_counts(std::vector<uint32_t>& cnt, std::string* err);
     // presence bits of every node into dst, lb words per row (structural: every
     // stored entry counts as present)
@@ -764,6 +775,66 @@ bool PlanImpl::split_weighted(std::string* err) {
     for (uint32_t q = 0; q < K; ++q)
         if (cnt[q] < thr) urare[q >> 5] |= 1u << (q & 31);
     return upload(d_urare, urare, st, err);   // (a member: the host copy outlives the upload)
+}
+
+// VAW on the shared-node path: nodes held by fewer than theta * n samples of the
+// plan's table (structural occupancy: no execution holds more) are summed from
+// posting lists, the others by the dense pair kernel, decided once here.  The
+// one-sided sums are taken at the table's distinct totals (an execution with
+// totals of its own takes them at every sample's).  The [n][D] sums must fit
+// VAW_H_BUDGET; SSU_VAW_SPLIT sets theta (0: off).
+bool PlanImpl::split_vaw(std::string* err) {
+    constexpr size_t VAW_H_BUDGET = (size_t)16 << 30;
+    const char* env_theta = getenv("SSU_VAW_SPLIT");
+    const double theta = env_theta ? atof(env_theta) : 1.0;
+    const uint32_t K = (uint32_t)knodes.size();
+    vsplit = false;
+    if (!(theta > 0.0) || K == 0 || n < 2 || (size_t)n * n * sizeof(double) > VAW_H_BUDGET) return true;
+    // the distinct totals, increasing, and each sample's index among them
+    tau_h.assign(totals.begin(), totals.begin() + n);
+    std::sort(tau_h.begin(), tau_h.end());
+    tau_h.erase(std::unique(tau_h.begin(), tau_h.end()), tau_h.end());
+    kidx_h.resize(n);
+    for (uint32_t s = 0; s < n; ++s)
+        kidx_h[s] = (uint32_t)(std::lower_bound(tau_h.begin(), tau_h.end(), totals[s]) - tau_h.begin());
+    std::vector<uint32_t> cnt;
+    if (!structural_counts(cnt, err)) return false;
+    const SplitKind kind = spec.fp64 ? SplitKind::vaw64 : SplitKind::vaw32;
+    uint32_t thr = env_theta ? (uint32_t)std::max(1.0, std::ceil(theta * n))
+                             : choose_split(kind, n, cnt, 1ull << 30, (uint32_t)tau_h.size()).thr;
+    if (thr == 0) return true;
+    auto bound = [&](uint32_t t) {
+        uint64_t b = 0;
+        for (uint32_t c : cnt) b += c < t ? c : 0;
+        return b;
+    };
+    uint64_t slots = bound(thr);
+    while (slots >= (1u << 30) && thr > 1) slots = bound(thr /= 2);
+    if (slots >= (1u << 30)) return true;
+    split_thr = thr;
+    split_slots = (uint32_t)std::max<uint64_t>(slots, 1);
+    vdense_mask.assign((K + 31) / 32, 0u);
+    vrows.clear();
+    vlen_f.clear();
+    vlen_d.clear();
+    for (uint32_t q = 0; q < K; ++q) {
+        if (cnt[q] < thr) continue;
+        vdense_mask[q >> 5] |= 1u << (q & 31);
+        vrows.push_back(spec.fp64 ? knodes[q] : (int32_t)q);   // FP64: embedding rows; FP32: operand rows
+        if (spec.fp64) vlen_d.push_back(klen_d[q]);
+        else vlen_f.push_back(klen_f[q]);
+    }
+    n_vdense = (uint32_t)vrows.size();
+    if (vrows.empty()) {   // (uploads of one padding row; the kernel takes n_vdense = 0 rows)
+        vrows.push_back(0);
+        if (spec.fp64) vlen_d.push_back(0.0);
+        else vlen_f.push_back(0.0f);
+    }
+    vsplit = true;
+    // (members: the host copies outlive the uploads)
+    return upload(d_vdense, vdense_mask, st, err) && upload(d_vrows, vrows, st, err) &&
+           (spec.fp64 ? upload(d_vlen, vlen_d, st, err) : upload(d_vlen, vlen_f, st, err)) &&
+           upload(d_tau, tau_h, st, err) && upload(d_kidx, kidx_h, st, err);
 }
 
 bool PlanImpl::embed_bits(DevMem& dst, uint32_t lb, bool structural, std::string* err) {
@@ -1034,7 +1105,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     // every stored value finite and >= 0 (one pass, for the presence bits and the shared-node kernels)
     const bool weighted_m = spec.method == Method::weighted_normalized || spec.method == Method::weighted_unnormalized;
     bool vals_nonneg = true;
-    if (P.gram || weighted_m || spec.method == Method::generalized) {
+    if (P.gram || weighted_m || spec.method == Method::generalized || spec.vaw) {
         const double* d = table.data + nz_base;
         std::vector<uint8_t> bad(host_threads(nz_all), 0);
         parallel_chunks(nz_all, bad.size(), [&](size_t t, uint64_t a, uint64_t b) {
@@ -1070,6 +1141,14 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         P.sparse = P.sparse && vals_nonneg;
     }
     P.gen = P.gen && P.sparse;
+    // VAW with weighted or presence terms (generalized VAW stays on the dense kernel)
+    // takes the posting lists for its rare nodes when every value and length is
+    // finite and >= 0 (split_vaw decides whether it pays)
+    P.vsplit = spec.vaw && spec.method != Method::generalized && spec.plain_kernel == 0 && vals_nonneg;
+    for (size_t q = 0; q < P.klen_d.size() && P.vsplit; ++q) {
+        const double l = spec.fp64 ? P.klen_d[q] : (double)P.klen_f[q];
+        P.vsplit = std::isfinite(l) && l >= 0.0;
+    }
     // Near-duplicate refinement: an FP32 weighted or generalized distance is off
     // by at most 2^-24 of its denominator through the FP32 rounding of the
     // proportions (|u - v| from rounded values is off by at most 2^-24 (u + v)),
@@ -1142,6 +1221,7 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     if (ok && P.gram) ok = P.split_gram(err);   // builds the Gram columns (narrowed when it splits)
     if (ok && P.gram && P.col_node.empty() && !P.build_gram_columns()) ok = false;
     if (ok && P.sparse) ok = P.split_weighted(err);
+    if (ok && P.vsplit) ok = P.split_vaw(err);
     step("occupancy split");
     if (ok && P.gram && P.Kp)
         ok = upload(P.d_col, P.col_node, P.st, err) && upload(P.d_digits, P.digits, P.st, err) &&
@@ -1193,8 +1273,13 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             parts = std::max(parts, (size_t)((K + fixed_rows_per_segment(K) - 1) / fixed_rows_per_segment(K)) * P.blk);
         if (!P.d_Spart.ensure(parts * sizeof(double), err)) return false;
     }
-    if (P.sparse && !P.d_masks.ensure(std::max<size_t>((P.ldn / 32) * Kw, 1) * sizeof(uint32_t), err)) return false;
-    const bool split = P.sparse && P.split_thr > 0 && K > 0;
+    const bool shared = P.sparse || P.vsplit;   // sample masks and (with a split) posting lists
+    if (shared && !P.d_masks.ensure(std::max<size_t>((P.ldn / 32) * Kw, 1) * sizeof(uint32_t), err)) return false;
+    const bool split = shared && P.split_thr > 0 && K > 0;
+    const bool vsplit = P.vsplit && split;
+    // VAW one-sided sums: at the plan's distinct totals, or at every sample's when the
+    // execution brings totals of its own
+    const uint32_t vD = P.x_totals ? P.n : (uint32_t)P.tau_h.size();
     const uint32_t npan = (P.n + 31) / 32;
     if (split) {
         const size_t kk = std::max<size_t>(K, 1);
@@ -1206,8 +1291,11 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             !P.d_poff.ensure((kk + 1) * sizeof(uint32_t), err) ||
             !P.d_pcnt.ensure(kk * sizeof(uint32_t), err) ||
             !P.d_Ssp.ensure(std::max<uint32_t>(P.n, 1) * sizeof(double), err) ||
-            (P.gen && !P.d_out2.ensure(std::max<size_t>(nn, 1) * osz, err)) ||
+            ((P.gen || vsplit) && !P.d_out2.ensure(std::max<size_t>(nn, 1) * osz, err)) ||
             (P.gen && !P.d_pspmax.ensure(sizeof(double), err)) ||
+            (vsplit && !P.d_pcv.ensure((size_t)P.split_slots * (s.fp64 ? sizeof(double) : sizeof(float)), err)) ||
+            (vsplit && !P.d_H.ensure(std::max<size_t>((size_t)P.n * vD, 1) * sizeof(double), err)) ||
+            (vsplit && !P.d_nbound.ensure(kk * sizeof(double), err)) ||
             !P.d_obsum.ensure((offset_scan_blocks(K) + 1) * sizeof(uint32_t), err) ||
             !P.d_walked.ensure((size_t)sparse_rows_blocks(P.n) * sizeof(uint32_t), err) ||
             !P.d_rows.ensure((size_t)P.split_slots * sizeof(unsigned long long), err) ||
@@ -1267,9 +1355,9 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
                 SSU_HIP(launch_gather_sums(nullptr, 0, nullptr, nullptr, P.d_umasks.as<uint32_t>(),
                                            P.d_smask.as<uint32_t>(), P.d_E.as<double>(), w, (uint32_t)c0,
                                            P.d_sp_node.as<int32_t>(), nullptr, Ks, P.n, P.st));
-        } else if (gather || sums) {
+        } else if (gather || sums || shared) {
             SSU_HIP(launch_gather_sums(gather ? P.d_U.as<float>() : nullptr, P.ldn, sums ? P.S_ptr() : nullptr,
-                                       P.d_Spart.as<double>(), P.sparse ? P.d_masks.as<uint32_t>() : nullptr,
+                                       P.d_Spart.as<double>(), shared ? P.d_masks.as<uint32_t>() : nullptr,
                                        split ? P.d_smask.as<uint32_t>() : nullptr, P.d_E.as<double>(), w, (uint32_t)c0, P.d_knodes.as<int32_t>(),
                                        P.d_klen_sum.as<double>(), K, P.n, P.st,
                                        skip_rare ? P.d_urare.as<uint32_t>() : nullptr,
@@ -1313,9 +1401,22 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         b.ssp = P.d_Ssp.as<double>();
         b.gen = P.gen;
         b.alpha = s.alpha;
-        b.out2 = P.gen ? P.d_out2.p : nullptr;
+        b.out2 = P.gen || vsplit ? P.d_out2.p : nullptr;
         b.pspmax = P.gen ? P.d_pspmax.as<double>() : nullptr;
         b.band = P.band;
+        if (vsplit) {
+            const bool unw = s.method == Method::unweighted || s.method == Method::unweighted_unnormalized;
+            b.vaw = unw ? 2 : 1;
+            b.fixed_dense = P.d_vdense.as<uint32_t>();
+            b.Cv = s.fp64 ? P.d_C.p : P.d_Cu.p;
+            b.pcv = P.d_pcv.p;
+            b.tot = P.totals_ptr();
+            b.tau = P.x_totals ? P.x_totals : P.d_tau.as<double>();
+            b.kidx = P.x_totals ? nullptr : P.d_kidx.as<uint32_t>();
+            b.D = vD;
+            b.H = P.d_H.as<double>();
+            b.nbound = P.d_nbound.as<double>();
+        }
         SSU_HIP(launch_sparse_nodes(b, P.st));
     }
     SparseNodesArgs gb{};
@@ -1448,6 +1549,12 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             a.U = P.d_U.p;
             a.Cn = s.vaw ? P.d_Cu.p : nullptr;
             a.rows = nullptr;
+        }
+        if (vsplit) {   // the dense nodes only, plus the posting-list shares
+            a.rows = P.d_vrows.as<int32_t>();
+            a.len = P.d_vlen.p;
+            a.K = P.n_vdense;
+            a.out2 = P.d_out2.p;
         }
         SSU_HIP(launch_pairs(op, s.fp64, a, P.st));
         SSU_HIP(P.mark(3));
@@ -1787,12 +1894,12 @@ uint64_t Plan::kernel_terms() const {
     const bool small = P.spec.vaw || P.spec.method == Method::generalized;
     const uint64_t edge = small ? 64 : PAIR_TILE;
     const uint64_t t = (P.n + edge - 1) / edge;
-    return t * (t + 1) / 2 * edge * edge * K;
+    return t * (t + 1) / 2 * edge * edge * (P.vsplit ? (uint64_t)P.n_vdense : K);
 }
 uint64_t Plan::sparse_terms() const {
     const PlanImpl& P = *p_;
-    if (!((P.sparse && P.split_thr) || (P.gram && P.gsplit_slots)) || !P.d_walked.p) return 0;
-    std::vector<uint32_t> wk(sparse_rows_launched(P.n, P.gen && !P.gram));
+    if (!(((P.sparse || P.vsplit) && P.split_thr) || (P.gram && P.gsplit_slots)) || !P.d_walked.p) return 0;
+    std::vector<uint32_t> wk(sparse_rows_launched(P.n, (P.gen || P.vsplit) && !P.gram));
     if (hipMemcpy(wk.data(), P.d_walked.p, wk.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) return 0;
     uint64_t sum = 0;
     for (uint32_t x : wk) sum += x;

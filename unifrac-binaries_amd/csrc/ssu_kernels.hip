@@ -648,6 +648,33 @@ template <> __device__ __forceinline__ double dpow<double>(double x, double a) {
     return pow(x, a);
 }
 
+// 1 / sqrt(x) for normal x > 0 in FP64: the hardware FP64 reciprocal square
+// root approximation refined by STEPS Newton steps r += r (1 - x r^2) / 2, each
+// of which squares the relative error (one step: below 1e-12; two: FP64
+// rounding).  Straight-line code, no branch.
+template <int STEPS> __device__ __forceinline__ double rsqrt_nr(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int i = 0; i < STEPS; ++i) {
+        const double e = fma(-x * r, r, 1.0);
+        r = fma(0.5 * r, e, r);
+    }
+    return r;
+}
+
+// One-sided VAW term (src/unifrac_task_impl.hpp:481-494 with one count zero):
+// a sample holding a node with count c > 0 and value term k (l u, or l for
+// presence), e = max(T - c, 0) the rest of its total, against a sample that
+// lacks the node, of total tau: mi = c, m - mi = e + tau, so
+// k / sqrt(c (e + tau)), 0 where c (e + tau) is 0.  The one-sided sums and the
+// posting walk call this same function on the same operands, so the terms the
+// walk takes back out are bit-equal to those the sums hold.
+template <int STEPS> __device__ __forceinline__ double vaw_one(double k, double c, double e, double tau) {
+    const double x = c * (e + tau);
+    const double r = rsqrt_nr<STEPS>(x);   // (x = 0: not finite, and not taken)
+    return x > 0.0 ? __dmul_rn(k, r) : 0.0;
+}
+
 template <PairOp OP> struct PairTraits {
     static constexpr bool two = OP != PairOp::weighted;       // numerator and denominator accumulators
     static constexpr bool counts = OP == PairOp::vaw_weighted || OP == PairOp::vaw_generalized ||
@@ -727,12 +754,15 @@ template <class T> struct PairStage { static constexpr int EB = sizeof(T) == 4 ?
 // of both 128-sample panels (and of the count panels for VAW) are staged in
 // LDS per step; the next step's rows are prefetched into registers while the
 // current one is consumed, and LDS is double-buffered (one barrier per step).
+// add_partial (two-accumulator ops): out[i * n + j] and out2[i * n + j], i < j,
+// hold the posting-list shares of the numerator and the denominator, added
+// before the division; each pair is then written by its i < j thread alone.
 template <PairOp OP, class T, int RI, int RJ, int TT>
 __global__ void __launch_bounds__((TT / RI) * (TT / RJ))
 k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __restrict__ rows, size_t ldn,
         const T* __restrict__ len, uint32_t K,
         uint32_t n, const double* __restrict__ S, const double* __restrict__ totals, double alpha_d, int normalized,
-        void* __restrict__ out_v, int out_fp64, GapBand band) {
+        void* __restrict__ out_v, int out_fp64, GapBand band, int add_partial, const void* __restrict__ out2_v) {
     constexpr int NTI = TT / RI, NTJ = TT / RJ, NT = NTI * NTJ;
     constexpr bool TWO = PairTraits<OP>::two;
     constexpr bool CNT = PairTraits<OP>::counts;
@@ -873,6 +903,21 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
             double nm, dn = 0.0;
             if constexpr (MIXED) { nm = num64[r][c]; if constexpr (TWO) dn = den64[r][c]; }
             else { nm = (double)num[r][c]; if constexpr (TWO) dn = (double)den[r][c]; }
+            if constexpr (TWO) {
+                if (add_partial) {
+                    if (i > j) continue;   // (the pair's i < j thread writes both entries)
+                    if (i < j) {
+                        if (out_fp64) {
+                            nm += ((const double*)out_v)[i * n + j];
+                            dn += ((const double*)out2_v)[i * n + j];
+                        } else {
+                            nm += (double)((const float*)out_v)[i * n + j];
+                            dn += (double)((const float*)out2_v)[i * n + j];
+                        }
+                        nm = nm < 0.0 ? 0.0 : nm;   // the shares' rounding can undershoot 0
+                    }
+                }
+            }
             double d;
             if (i == j) d = 0.0;
             else if (!normalized) d = nm;
@@ -1295,22 +1340,24 @@ __global__ void __launch_bounds__(64) k_pairs_gen64(
 // Node occupancy split: a node held by at least `thr` samples is dense (its
 // bit set in dmask, it stays with the tiled shared-node kernel); the rest go
 // to the posting-list kernels, pcnt[r] = their sample counts (0 for dense
-// nodes).  One thread per node; smask is [npan][K].
+// nodes).  With `fixed` ([ceil(K/32)] words), the dense nodes are its set bits
+// instead.  One thread per node; smask is [npan][K].
 __global__ void __launch_bounds__(64) k_node_split(uint32_t* __restrict__ dmask, uint32_t* __restrict__ pcnt,
                                                    const uint32_t* __restrict__ smask, uint32_t K, uint32_t npan,
-                                                   uint32_t thr) {
+                                                   uint32_t thr, const uint32_t* __restrict__ fixed) {
     const uint32_t r = blockIdx.x * 64 + threadIdx.x;
     uint32_t cnt = 0;
     if (r < K)
         for (uint32_t p = 0; p < npan; ++p) cnt += __popc(smask[(size_t)p * K + r]);
-    if (r < K) pcnt[r] = cnt < thr ? cnt : 0u;
-    const uint64_t b = __ballot(r < K && cnt >= thr);
+    const bool dense = r < K && (fixed ? ((fixed[r >> 5] >> (r & 31)) & 1u) != 0u : cnt >= thr);
+    if (r < K) pcnt[r] = dense ? 0u : cnt;
+    const uint64_t b = __ballot(dense);
     const uint32_t w = blockIdx.x * 2 + (threadIdx.x >> 5);
     if ((threadIdx.x & 31) == 0 && w * 32 < K) dmask[w] = (threadIdx.x < 32) ? (uint32_t)b : (uint32_t)(b >> 32);
 }
 
 // Posting lists of the sparse nodes: node r's samples in increasing order
-// (pidx) with their values (pval, when non-null: row urows[r] of U, or row r), in slots
+// (pidx, when non-null) with their values (pval, when non-null: row urows[r] of U, or row r), in slots
 // [poff[r], poff[r + 1]); pc[p * K + r] = number of node r's samples below
 // 32 p, for p = 0..npan.  A workgroup holds 64 nodes (lane = node, so the
 // [panel][K] reads coalesce) in PST_WAVES waves, each wave a contiguous range
@@ -1343,14 +1390,14 @@ __global__ void __launch_bounds__(64 * PST_WAVES) k_postings(IX* __restrict__ pi
     for (uint32_t h = 0; h < g; ++h) cnt += cnt_s[h][lane];
     const TU* ur = U ? U + (size_t)(urows ? urows[r] : (int32_t)r) * ldu : nullptr;
     const uint32_t o = poff[r], cap = poff[r + 1] - o;
-    IX* ix = pidx + o;
+    IX* ix = pidx ? pidx + o : nullptr;
     TV* vx = pval ? pval + o : nullptr;
     for (uint32_t p = p0; p < p1; ++p) {
         pc[(size_t)p * K + r] = min(cnt, cap);
         uint32_t w = smask[(size_t)p * K + r];
         while (w && cnt < cap) {   // the slots hold every sample of the node; the bound guards memory
             const uint32_t s = p * 32 + (uint32_t)(__ffs(w) - 1);
-            ix[cnt] = (IX)s;
+            if (ix) ix[cnt] = (IX)s;
             if (vx) vx[cnt] = (TV)ur[s];
             ++cnt;
             w &= w - 1;
@@ -1610,7 +1657,40 @@ __device__ __forceinline__ void row_term(unsigned long long* accb, const RowEntr
     }
 }
 
-template <class TOut, class IX, bool GRAM, class TV, bool GEN>
+// VAW (1: weighted values, 2: presence): one posting's corrections for the
+// pair (a, b) holding node r, the both-held terms less the one-sided terms the
+// sums H hold, numerator and denominator; en.lu = u_a, en.uap = c_a, en.q8 = l_r
+// (FP64 bits); fixed point in units of 1 / scale, rounded (|x * scale| < 2^61
+// for FP64 values, 2^51 for FP32 values).
+template <class TV, int VAW, uint32_t CW>
+__device__ __forceinline__ void vaw_row_term(unsigned long long* accb, const RowEntry<TV, true>& en, uint32_t b,
+                                             TV v, TV cv, double Ta, double Tb, double scale) {
+    constexpr int ST = sizeof(TV) == 8 ? 2 : 1;
+    const double l = __longlong_as_double((long long)en.q8);
+    const double ua = (double)en.lu, ca = (double)en.uap, ub = (double)v, cb = (double)cv;
+    const double ea = Ta - ca > 0.0 ? Ta - ca : 0.0, eb = Tb - cb > 0.0 ? Tb - cb : 0.0;
+    const double x = (ca + cb) * (ea + eb);
+    const double r = rsqrt_nr<ST>(x);   // (x = 0: not finite, and not taken)
+    const double rb = x > 0.0 ? r : 0.0;
+    const double one = vaw_one<ST>(VAW == 2 ? l : l * ua, ca, ea, Tb) + vaw_one<ST>(VAW == 2 ? l : l * ub, cb, eb, Ta);
+    double cn, cd;
+    if constexpr (VAW == 2) {   // xor = 0, or = 1 for a node both hold
+        cn = -one;
+        cd = l * rb - one;
+    } else {
+        cn = l * __builtin_fabs(ua - ub) * rb - one;
+        cd = l * (ua + ub) * rb - one;
+    }
+    if constexpr (sizeof(TV) == 8) {   // FP64 values: 61-bit units, rounded
+        atomicAdd(&accb[b], (unsigned long long)(long long)rint(cn * scale));
+        atomicAdd(&accb[CW + b], (unsigned long long)(long long)rint(cd * scale));
+    } else {                           // FP32 values: 51-bit units, rounded by one FMA
+        atomicAdd(&accb[b], fixed_term_signed(cn, scale));
+        atomicAdd(&accb[CW + b], fixed_term_signed(cd, scale));
+    }
+}
+
+template <class TOut, class IX, bool GRAM, class TV, bool GEN, int VAW = 0>
 __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__ out, uint32_t* __restrict__ walked,
                                                    const unsigned long long* __restrict__ rows,
                                                    const uint32_t* __restrict__ offsets,
@@ -1623,9 +1703,12 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                                                    uint32_t tiles, const double* __restrict__ ssp,
                                                    TOut* __restrict__ out2, const double* __restrict__ pspmax,
                                                    double alpha, GapBand band, uint32_t a0, uint32_t na,
-                                                   uint32_t mode) {
-    // GEN keeps two accumulators (numerator, denominator) on half-width chunks
-    constexpr uint32_t CW = GEN ? SR_CW / 2 : SR_CW;
+                                                   uint32_t mode, const TV* __restrict__ pcv,
+                                                   const double* __restrict__ tot, const double* __restrict__ Hm,
+                                                   uint32_t D, const uint32_t* __restrict__ kidx) {
+    // GEN and VAW keep two accumulators (numerator, denominator) on half-width chunks
+    constexpr bool TWO = GEN || VAW != 0;
+    constexpr uint32_t CW = TWO ? SR_CW / 2 : SR_CW;
     // chunk-major: the rows in flight share one column chunk of every posting
     // list, a quarter of the posting data at 16k samples (cache-resident).
     // Rows a0 .. a0 + na - 1; mode SR_DIAG alone: one workgroup per row, on the
@@ -1661,8 +1744,9 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     __shared__ unsigned long long acc[SR_CW];
     __shared__ uint32_t wk[SR_WAVES];
-    using Ent = RowEntry<TV, GEN>;
+    using Ent = RowEntry<TV, TWO>;
     __shared__ Ent ent_s[SR_WAVES][65];   // per wave: the round's 64 entries at 1..64
+    __shared__ double tot_s[VAW ? CW : 1];   // VAW: the chunk's sample totals
     // per entry: GRAM the fixed-point term, weighted l_r * scale (FP64 bits)
     __shared__ unsigned long long diag_s;
     __shared__ double bsum_s[SR_WAVES];
@@ -1671,6 +1755,8 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     uint8_t* own = reinterpret_cast<uint8_t*>(own4);
     Ent* ents = ent_s[wv];
     for (uint32_t q = threadIdx.x; q < SR_CW; q += 64 * SR_WAVES) acc[q] = 0ull;
+    if constexpr (VAW != 0)
+        for (uint32_t q = threadIdx.x; q < CW; q += 64 * SR_WAVES) tot_s[q] = b0 + q < n ? tot[b0 + q] : 0.0;
     const uint32_t e0 = offsets[(size_t)a * nkc], e1 = offsets[(size_t)(a + 1) * nkc];
     double bound;
     if constexpr (GRAM) {
@@ -1685,6 +1771,9 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         __syncthreads();
         bound = 0.0;
         for (uint32_t w = 0; w < SR_WAVES; ++w) bound += bsum_s[w];
+    } else if constexpr (VAW != 0) {
+        bound = ssp[a];   // the sum of the row's node bounds
+        bound = bound < 1e300 ? bound : 1e300;   // (extreme inputs: coarser units, no overflow)
     } else if constexpr (GEN) {
         // |l (s^(a-1) |u - v| - u^a - v^a)| and |l (s^a - u^a - v^a)| are at most 3 l (u^a + v^a)
         bound = 3.0 * (ssp[a] + pspmax[0]);
@@ -1694,10 +1783,12 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
     __syncthreads();
     int e2 = 0;
     (void)frexp(bound, &e2);   // bound < 2^e2 bounds every partial sum of the row
-    // FP32 weighted terms: 51-bit rounding (one FMA); FP64 values and GRAM: 61-bit truncation
+    // FP32 values: 51-bit rounding (one FMA); FP64 values and GRAM: 61 bits (truncated;
+    // VAW: rounded)
     constexpr bool FINE = GRAM || sizeof(TV) == 8;
     const int fb = FINE ? 61 : 51;
     const double scale = ldexp(1.0, fb - e2), inv = ldexp(1.0, e2 - fb);
+    const double Ta = VAW != 0 ? tot[a] : 0.0;
     unsigned long long* const accb = acc - b0;   // accb[b]: column b (b0 <= b < b1)
     if (GRAM && has_diag) {   // G(a, a): the same fixed-point terms, every one of them
         unsigned long long x = 0ull;
@@ -1722,6 +1813,10 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
             slot0 = poff[r];
             if constexpr (GRAM) {
                 q = to_fixed(len[r] * scale);
+            } else if constexpr (VAW != 0) {
+                ua = pval[slot0 + pa];
+                uap = pcv[slot0 + pa];   // c_a
+                q = (unsigned long long)__double_as_longlong(len[r]);
             } else {
                 ua = pval[slot0 + pa];
                 q = (unsigned long long)__double_as_longlong(len[r] * scale);
@@ -1741,7 +1836,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         me.base = slot0 + start - (incl - cnt);   // slot of walk position t: base + t (mod 2^32)
         me.lu = ua;
         me.q8 = q;
-        if constexpr (GEN) me.uap = uap;
+        if constexpr (TWO) me.uap = uap;
         ents[lane + 1] = me;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1795,7 +1890,7 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                 for (uint32_t g = 0; g < SR_WIN / 64; g += SR_UNROLL) {
                     Ent en[SR_UNROLL];
                     uint32_t bq[SR_UNROLL];
-                    TV vq[SR_UNROLL];
+                    TV vq[SR_UNROLL], cq[VAW ? SR_UNROLL : 1];
 #pragma unroll
                     for (uint32_t u = 0; u < SR_UNROLL; ++u) en[u] = ents[own[lane + 64 * (g + u)]];
 #pragma unroll
@@ -1803,16 +1898,21 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                         const uint32_t qs = en[u].base + wb + 64 * (g + u);
                         bq[u] = (uint32_t)ld_slot(pidx, qs);
                         if constexpr (!GRAM) vq[u] = ld_slot(pval, qs);
+                        if constexpr (VAW != 0) cq[u] = ld_slot(pcv, qs);
                     }
 #pragma unroll
-                    for (uint32_t u = 0; u < SR_UNROLL; ++u)
-                        row_term<TV, GRAM, GEN, FINE, CW>(accb, en[u], bq[u], vq[u], alpha);
+                    for (uint32_t u = 0; u < SR_UNROLL; ++u) {
+                        if constexpr (VAW != 0)
+                            vaw_row_term<TV, VAW, CW>(accb, en[u], bq[u], vq[u], cq[u], Ta, tot_s[bq[u] - b0], scale);
+                        else
+                            row_term<TV, GRAM, GEN, FINE, CW>(accb, en[u], bq[u], vq[u], alpha);
+                    }
                 }
             } else {
                 for (uint32_t j0 = lane; j0 < wn; j0 += 64 * SR_UNROLL) {
                     Ent en[SR_UNROLL];
                     uint32_t bq[SR_UNROLL];
-                    TV vq[SR_UNROLL];
+                    TV vq[SR_UNROLL], cq[VAW ? SR_UNROLL : 1];
 #pragma unroll
                     for (uint32_t u = 0; u < SR_UNROLL; ++u) {
                         const uint32_t j = min(j0 + 64 * u, wn - 1);
@@ -1820,10 +1920,16 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                         const uint32_t qs = en[u].base + w0 + j;
                         bq[u] = (uint32_t)ld_slot(pidx, qs);
                         if constexpr (!GRAM) vq[u] = ld_slot(pval, qs);
+                        if constexpr (VAW != 0) cq[u] = ld_slot(pcv, qs);
                     }
 #pragma unroll
-                    for (uint32_t u = 0; u < SR_UNROLL; ++u)
-                        if (j0 + 64 * u < wn) row_term<TV, GRAM, GEN, FINE, CW>(accb, en[u], bq[u], vq[u], alpha);
+                    for (uint32_t u = 0; u < SR_UNROLL; ++u) {
+                        if (j0 + 64 * u >= wn) continue;
+                        if constexpr (VAW != 0)
+                            vaw_row_term<TV, VAW, CW>(accb, en[u], bq[u], vq[u], cq[u], Ta, tot_s[bq[u] - b0], scale);
+                        else
+                            row_term<TV, GRAM, GEN, FINE, CW>(accb, en[u], bq[u], vq[u], alpha);
+                    }
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1849,6 +1955,16 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         for (uint32_t b = wlo + threadIdx.x; b < whi; b += 64 * SR_WAVES) {
             const unsigned long long v = acc[b - b0];
             if (v) G[gram_slot(a, b, tiles)] += (double)v * inv;
+        }
+    } else if constexpr (VAW != 0) {
+        // numerator and denominator shares: the one-sided sums of both samples at
+        // the other's total plus the corrections of the shared nodes
+        const uint32_t ka = kidx ? kidx[a] : a;
+        for (uint32_t b = wlo + threadIdx.x; b < whi; b += 64 * SR_WAVES) {
+            const uint32_t kb = kidx ? kidx[b] : b;
+            const double base2 = Hm[(size_t)a * D + kb] + Hm[(size_t)b * D + ka];
+            out[(size_t)a * n + b] = (TOut)(base2 + (double)(long long)acc[b - b0] * inv);
+            out2[(size_t)a * n + b] = (TOut)(base2 + (double)(long long)acc[CW + b - b0] * inv);
         }
     } else if constexpr (GEN) {
         // numerator and denominator shares: P_a + P_b plus the corrections of the
@@ -1901,6 +2017,134 @@ __global__ void __launch_bounds__(64) k_sparse_sums(double* __restrict__ ssp,
     if (lane == 0) ssp[a] = x;
 }
 
+// ------------------------------------------------ VAW on the shared-node path
+//
+// VAW terms (src/unifrac_task_impl.hpp:481-494, 897-911, 1817-1829) divide
+// each node's term by sqrt(mi (m - mi)), mi = c_a + c_b, m = T_a + T_b, so a
+// node held by a alone still adds l u_a / sqrt(c_a (T_a - c_a + T_b)), which
+// depends on b's total.  Over the sparse nodes the pair's share is therefore
+//   H(a, T_b) + H(b, T_a) + sum over the nodes both hold of (both-term - h_a - h_b),
+// H(a, t) = sum over a's sparse nodes of the one-sided term at total t (taken
+// at each distinct total t once), the sum over shared nodes from the posting
+// walk.  m - mi is taken as e_a + e_b, e = max(T - c, 0) (equal for integer
+// counts; it keeps FP32-rounded counts from producing a tiny positive m - mi).
+
+// Per sparse node r, a bound on any one correction the walk adds for a pair
+// holding r (numerator or denominator): |both-term - h_a - h_b| <= den_both +
+// h_a + h_b, den_both = l (u_a + u_b) / sqrt((c_a + c_b)(e_a + e_b)) and
+// h = l u / sqrt(c (e + T')), so with, over r's postings, umax the largest
+// value (1 for presence), cmin the smallest positive count, emin the smallest
+// positive e, tmin the smallest positive total, g = min(emin, tmin):
+//   B_r = l umax (2 / sqrt(cmin emin) + 2 / sqrt(cmin g)).
+// One thread per node.
+template <class IX, class TV, bool UNW>
+__global__ void __launch_bounds__(64) k_vaw_node_bound(double* __restrict__ nb, const IX* __restrict__ pidx,
+                                                       const TV* __restrict__ pval, const TV* __restrict__ pcv,
+                                                       const uint32_t* __restrict__ poff,
+                                                       const double* __restrict__ tot, const double* __restrict__ len,
+                                                       uint32_t K, uint32_t slots) {
+    const uint32_t r = blockIdx.x * 64 + threadIdx.x;
+    if (r >= K) return;
+    double b = 0.0;
+    if (poff[K] <= slots) {   // (never otherwise, by the plan's bound)
+        constexpr double INF = __builtin_huge_val();
+        double umax = 0.0, cmin = INF, emin = INF, tmin = INF;
+        for (uint32_t q = poff[r]; q < poff[r + 1]; ++q) {
+            const double t = tot[pidx[q]], c = (double)pcv[q], u = (double)pval[q];
+            const double e = t - c > 0.0 ? t - c : 0.0;
+            umax = u > umax ? u : umax;
+            if (c > 0.0) cmin = c < cmin ? c : cmin;
+            if (e > 0.0) emin = e < emin ? e : emin;
+            if (t > 0.0) tmin = t < tmin ? t : tmin;
+        }
+        if (cmin < INF) {
+            const double g = emin < tmin ? emin : tmin;
+            const double uu = UNW ? 1.0 : umax;
+            b = len[r] * uu * ((emin < INF ? 2.0 / sqrt(cmin * emin) : 0.0) + (g < INF ? 2.0 / sqrt(cmin * g) : 0.0));
+        }
+    }
+    nb[r] = b;
+}
+
+// out[a] = sum of w[r] over sample a's row list (the sparse nodes it holds), in
+// FP64 in a fixed order: one wavefront per sample, lane-strided partial sums
+// then a butterfly.
+__global__ void __launch_bounds__(64) k_row_weights(double* __restrict__ out, const unsigned long long* __restrict__ rows,
+                                                    const uint32_t* __restrict__ offsets, uint32_t nkc,
+                                                    const double* __restrict__ w, const uint32_t* __restrict__ poff,
+                                                    uint32_t K, uint32_t n, uint32_t slots) {
+    const uint32_t a = blockIdx.x, lane = threadIdx.x;
+    if (a >= n) return;
+    double x = 0.0;
+    if (poff[K] <= slots) {
+        const uint32_t e0 = offsets[(size_t)a * nkc], e1 = offsets[(size_t)(a + 1) * nkc];
+        for (uint32_t e = e0 + lane; e < e1; e += 64) x += w[(uint32_t)(rows[e] >> 32)];
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+    if (lane == 0) out[a] = x;
+}
+
+// One-sided sums H[a][k] = sum over sample a's sparse nodes r (its row list, in
+// order) of vaw_one(l_r u_ar, c_ar, max(T_a - c_ar, 0), tau[k]) (UNW: l_r in
+// place of l_r u_ar), in FP64.  One wavefront per (sample a, 256 totals): the
+// row's entries are staged 64 at a time in LDS and read by every lane, each
+// lane summing VH_COLS totals.
+constexpr uint32_t VH_COLS = 4;
+template <class TV, bool UNW>
+__global__ void __launch_bounds__(64) k_vaw_onesided(double* __restrict__ H, uint32_t D, const double* __restrict__ tau,
+                                                     const unsigned long long* __restrict__ rows,
+                                                     const uint32_t* __restrict__ offsets, uint32_t nkc,
+                                                     const uint32_t* __restrict__ poff, const TV* __restrict__ pval,
+                                                     const TV* __restrict__ pcv, const double* __restrict__ len,
+                                                     const double* __restrict__ tot, uint32_t K, uint32_t n,
+                                                     uint32_t slots) {
+    constexpr int ST = sizeof(TV) == 8 ? 2 : 1;
+    const uint32_t a = blockIdx.x, lane = threadIdx.x;
+    const uint32_t k0 = blockIdx.y * 64 * VH_COLS + lane;
+    __shared__ double sk[64], sc[64], se[64];
+    double tk[VH_COLS], acc[VH_COLS];
+#pragma unroll
+    for (uint32_t j = 0; j < VH_COLS; ++j) {
+        const uint32_t k = k0 + 64 * j;
+        tk[j] = k < D ? tau[k] : 0.0;
+        acc[j] = 0.0;
+    }
+    if (a >= n) return;
+    if (poff[K] <= slots) {   // (never otherwise, by the plan's bound)
+        const double Ta = tot[a];
+        const uint32_t e0 = offsets[(size_t)a * nkc], e1 = offsets[(size_t)(a + 1) * nkc];
+        for (uint32_t q0 = e0; q0 < e1; q0 += 64) {   // uniform
+            double kk = 0.0, cc = 0.0, ee = 0.0;
+            if (q0 + lane < e1) {
+                const unsigned long long ent = rows[q0 + lane];
+                const uint32_t r = (uint32_t)(ent >> 32);
+                const uint32_t slot = poff[r] + (uint32_t)ent;
+                const double l = len[r];
+                cc = (double)pcv[slot];
+                kk = UNW ? l : l * (double)pval[slot];
+                ee = Ta - cc > 0.0 ? Ta - cc : 0.0;
+            }
+            __syncthreads();   // the previous round's entries are read
+            sk[lane] = kk;
+            sc[lane] = cc;
+            se[lane] = ee;
+            __syncthreads();
+            const uint32_t cnt = min(64u, e1 - q0);
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const double kv = sk[i], cv = sc[i], ev = se[i];
+#pragma unroll
+                for (uint32_t j = 0; j < VH_COLS; ++j) acc[j] += vaw_one<ST>(kv, cv, ev, tk[j]);
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < VH_COLS; ++j) {
+        const uint32_t k = k0 + 64 * j;
+        if (k < D) H[(size_t)a * D + k] = acc[j];
+    }
+}
+
 // mx[0] = max of v[0..n): one workgroup
 __global__ void __launch_bounds__(256) k_max(double* __restrict__ mx, const double* __restrict__ v, uint32_t n) {
     __shared__ double part[256];
@@ -1949,7 +2193,7 @@ hipError_t launch_pairs_t(const PairArgs& a, hipStream_t st) {
     constexpr int NT = (TT / RI) * (TT / RJ);
     hipLaunchKernelGGL((k_pairs<OP, T, RI, RJ, TT>), grid, dim3(NT), 0, st, (const T*)a.U, (const T*)a.Cn, a.rows, a.ldn,
                        (const T*)a.len, a.K, a.n, a.S, a.totals, a.alpha, (int)a.normalized, a.out, (int)a.out_fp64,
-                       a.band);
+                       a.band, a.out2 ? 1 : 0, a.out2);
     return hipGetLastError();
 }
 
@@ -2222,6 +2466,7 @@ hipError_t launch_gather_sums(float* U, size_t ldu, double* S, double* Spart, ui
     switch (key) {
     case 7: gather_sums_t<true, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt, skip_u); break;
     case 6: gather_sums_t<true, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt, skip_u); break;
+    case 5: gather_sums_t<true, false, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt, skip_u); break;
     case 4: gather_sums_t<true, false, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt, skip_u); break;
     case 3: gather_sums_t<false, true, true>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt, skip_u); break;
     case 2: gather_sums_t<false, true, false>(grid, st, U, ldu, col0, Spart, masks, Kw, smask, E, lde, knodes, klen, K, rps, cnt, skip_u); break;
@@ -2276,6 +2521,7 @@ template hipError_t launch_unweighted_combine<double>(double*, const double*, ui
                                                       uint32_t, uint32_t);
 
 hipError_t launch_pairs(PairOp op, bool fp64, const PairArgs& a, hipStream_t st) {
+    if (a.out2 && op == PairOp::weighted) return hipErrorInvalidValue;   // (one accumulator: no denominator)
     return fp64 ? launch_pairs_prec<double>(op, a, st) : launch_pairs_prec<float>(op, a, st);
 }
 
@@ -2321,8 +2567,8 @@ static bool sparse_args_ok(const SparseNodesArgs& a) {
                         a.rbsum && a.npan == (a.n + 31) / 32 && a.slots < (1u << 30);
     if (!common) return false;
     if (a.gram) return a.n_pad >= a.n && a.n_pad % GRAM_BM == 0;   // unweighted: values, sums and out unused
-    return a.pcnt && a.obsum && a.thr && a.pval && a.U && a.S && a.out && a.out_fp64 == a.values_fp64 &&
-           a.ldu >= (size_t)a.npan * 32;
+    return a.pcnt && a.obsum && a.thr && a.pval && a.U && (a.S || a.vaw) && a.out && a.out_fp64 == a.values_fp64 &&
+           a.ldu >= (size_t)a.npan * 32 && (!a.vaw || (a.Cv && a.pcv));
 }
 
 hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
@@ -2334,7 +2580,8 @@ hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
     // sparse (dmask zero) with offsets fixed since plan creation
     if (!a.gram) {
         const uint32_t nb = (uint32_t)((a.K + SCAN_BLOCK - 1) / SCAN_BLOCK);
-        hipLaunchKernelGGL(k_node_split, dim3(gk), dim3(64), 0, st, a.dmask, a.pcnt, a.smask, a.K, a.npan, a.thr);
+        hipLaunchKernelGGL(k_node_split, dim3(gk), dim3(64), 0, st, a.dmask, a.pcnt, a.smask, a.K, a.npan, a.thr,
+                           a.fixed_dense);
         hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(SCAN_T), 0, st, a.obsum, a.pcnt, (size_t)a.K);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, a.obsum, nb);
         hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_T), 0, st, a.poff, a.pcnt, a.obsum, (size_t)a.K, nb);
@@ -2356,6 +2603,18 @@ hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
     default: SSU_POSTINGS(uint16_t, double, double); break;
     }
 #undef SSU_POSTINGS
+    if (a.vaw) {   // the counts, from the count rows, in the same slots (indices already written)
+#define SSU_COUNTS(IX, TV, TU)                                                                                  \
+    hipLaunchKernelGGL((k_postings<IX, TV, TU>), dim3(gk), dim3(64 * PST_WAVES), 0, st, (IX*)nullptr, (TV*)a.pcv,  \
+                       a.pc, a.smask, a.dmask, (const TU*)a.Cv, a.urows, a.ldu, a.K, a.npan, (const uint32_t*)a.poff, \
+                       a.slots)
+        switch (key) {
+        case 0: case 1: SSU_COUNTS(uint32_t, float, float); break;
+        case 4: case 5: SSU_COUNTS(uint32_t, float, double); break;
+        default: SSU_COUNTS(uint32_t, double, double); break;
+        }
+#undef SSU_COUNTS
+    }
     // row lists: counts per (sample, node chunk), their exclusive scan, then the entries
     const uint32_t nkc = row_list_chunks(a.n, a.K), kcw = row_list_width(a.n, a.K);
     const size_t m = (size_t)a.n * nkc;
@@ -2371,14 +2630,31 @@ hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st) {
 }
 
 // weighted: the partial sums go out in the value type (FP32 or FP64 output)
-template <class IX, bool GRAM, class TV, bool GEN = false>
+template <class IX, bool GRAM, class TV, bool GEN = false, int VAW = 0>
 static void pairs_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs& a, uint32_t nchunks, uint32_t nkc,
                          uint32_t a0, uint32_t na, uint32_t mode) {
-    hipLaunchKernelGGL((k_pairs_rows<TV, IX, GRAM, TV, GEN>), dim3(blocks), dim3(64 * SR_WAVES), 0, st,
+    hipLaunchKernelGGL((k_pairs_rows<TV, IX, GRAM, TV, GEN, VAW>), dim3(blocks), dim3(64 * SR_WAVES), 0, st,
                        GRAM ? nullptr : (TV*)a.out, a.walked, a.rows, a.roff, a.pc, (const IX*)a.pidx,
                        GRAM ? nullptr : (const TV*)a.pval, a.len, GRAM ? nullptr : a.S, a.K, a.n, a.npan,
                        (const uint32_t*)a.poff, a.slots, nchunks, nkc, a.gram, a.n_pad / GRAM_BM, a.ssp,
-                       (TV*)a.out2, a.pspmax, a.alpha, a.band, a0, na, mode);
+                       (TV*)a.out2, a.pspmax, a.alpha, a.band, a0, na, mode, (const TV*)a.pcv, a.tot, a.H, a.D,
+                       a.kidx);
+}
+
+// VAW: node bounds, row bounds, one-sided sums, then the walk
+template <class IX, class TV, int VAW>
+static void vaw_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs& a, uint32_t nchunks, uint32_t nkc,
+                       uint32_t a0, uint32_t na, uint32_t mode) {
+    constexpr bool UNW = VAW == 2;
+    hipLaunchKernelGGL((k_vaw_node_bound<IX, TV, UNW>), dim3((a.K + 63) / 64), dim3(64), 0, st, a.nbound,
+                       (const IX*)a.pidx, (const TV*)a.pval, (const TV*)a.pcv, (const uint32_t*)a.poff, a.tot, a.len,
+                       a.K, a.slots);
+    hipLaunchKernelGGL(k_row_weights, dim3(a.n), dim3(64), 0, st, a.ssp, a.rows, a.roff, nkc, a.nbound,
+                       (const uint32_t*)a.poff, a.K, a.n, a.slots);
+    hipLaunchKernelGGL((k_vaw_onesided<TV, UNW>), dim3(a.n, (a.D + 64 * VH_COLS - 1) / (64 * VH_COLS)), dim3(64), 0,
+                       st, a.H, a.D, a.tau, a.rows, a.roff, nkc, (const uint32_t*)a.poff, (const TV*)a.pval,
+                       (const TV*)a.pcv, a.len, a.tot, a.K, a.n, a.slots);
+    pairs_rows_t<IX, false, TV, false, VAW>(blocks, st, a, nchunks, nkc, a0, na, mode);
 }
 
 hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st, uint32_t row0, uint32_t row1, int parts) {
@@ -2390,12 +2666,29 @@ hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st, uint32_t
     if (!mode || (mode == SR_DIAG && !a.gram)) return hipErrorInvalidValue;   // (the diagonal is the Gram's)
     const uint32_t na = row1 - row0;
     const uint32_t nkc = row_list_chunks(a.n, a.K);
-    const uint32_t cw = a.gen ? SR_CW / 2 : SR_CW;
+    const uint32_t cw = a.gen || a.vaw ? SR_CW / 2 : SR_CW;
     const uint32_t nchunks = (a.n + cw - 1) / cw;
     const uint64_t blocks = mode == SR_DIAG ? (uint64_t)na : (uint64_t)na * nchunks;
     if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
     const bool narrow = a.n <= 65536;
     const bool whole = row0 == 0 && row1 == a.n;   // the per-sample sums are made for all rows at once
+    if (a.vaw) {
+        if (a.gram || a.gen || a.vaw > 2 || !a.ssp || !a.out2 || !a.pcv || !a.tot || !a.tau || !a.H || !a.nbound ||
+            !a.D || a.D > a.n || (!a.kidx && a.D != a.n) || !whole)
+            return hipErrorInvalidValue;
+        const int key = (narrow ? 1 : 0) | (a.values_fp64 ? 2 : 0) | (a.vaw == 2 ? 4 : 0);
+        switch (key) {
+        case 0: vaw_rows_t<uint32_t, float, 1>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
+        case 1: vaw_rows_t<uint16_t, float, 1>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
+        case 2: vaw_rows_t<uint32_t, double, 1>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
+        case 3: vaw_rows_t<uint16_t, double, 1>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
+        case 4: vaw_rows_t<uint32_t, float, 2>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
+        case 5: vaw_rows_t<uint16_t, float, 2>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
+        case 6: vaw_rows_t<uint32_t, double, 2>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
+        default: vaw_rows_t<uint16_t, double, 2>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
+        }
+        return hipGetLastError();
+    }
     if (a.gen && a.values_fp64) {   // generalized, FP64 values
         if (a.gram || !a.ssp || !a.out2 || !a.pspmax || !whole) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_sparse_sums<double, true>), dim3(a.n), dim3(64), 0, st, a.ssp, a.rows, a.roff, nkc,

@@ -180,6 +180,9 @@ struct PairArgs {
     void* out;              // n x n output
     bool out_fp64;
     GapBand band;           // tiles without a selected pair are skipped
+    // two-accumulator ops (VAW): with out2, out[i * n + j] and out2[i * n + j] (i < j) hold
+    // the posting-list shares of the numerator and denominator, added before the division
+    const void* out2 = nullptr;
 };
 hipError_t launch_pairs(PairOp op, bool fp64, const PairArgs& a, hipStream_t st);
 
@@ -253,6 +256,19 @@ struct SparseNodesArgs {
     double* gram = nullptr; // unweighted: the Gram tiles of launch_gram_i8
     uint32_t n_pad = 0;     // their padded edge
     GapBand band;           // rows walk (and write) only the columns of selected pairs (G(a, a) always)
+    // dense nodes fixed in advance ([ceil(K/32)] words, in place of the occupancy test)
+    const uint32_t* fixed_dense = nullptr;
+    // VAW (1: weighted values, 2: presence): numerator shares to out, denominator shares
+    // to out2, each H(a, T_b) + H(b, T_a) plus the walk's corrections of the shared nodes
+    int vaw = 0;
+    const void* Cv = nullptr;       // count rows, indexed as U
+    void* pcv = nullptr;            // [slots] counts, in the value type
+    const double* tot = nullptr;    // [n] sample totals
+    const double* tau = nullptr;    // [D] the totals the one-sided sums are taken at
+    const uint32_t* kidx = nullptr; // [n] sample -> its total's index in tau (nullptr: identity, D = n)
+    uint32_t D = 0;
+    double* H = nullptr;            // [n][D] one-sided sums
+    double* nbound = nullptr;       // [K] per-node correction bounds (ssp: their row sums)
 };
 // posting lists, per-panel prefix counts and row lists
 hipError_t launch_sparse_lists(const SparseNodesArgs& a, hipStream_t st);

@@ -17,6 +17,7 @@ namespace {
 struct Rates {
     double term;      // one (pair, node) term of the tiled kernel; Gram: one (pair, column) unit
     double posting;   // one posting visited by the row walk
+    double onesided = 0.0;   // VAW: one (posting slot, total) one-sided term
 };
 
 Rates rates_of(SplitKind k) {
@@ -29,6 +30,12 @@ Rates rates_of(SplitKind k) {
     // posting): fitted to the 8,192-sample workload at theta 0.5 / 0.707 / 0.9
     // (within 2% of each phase's measured time)
     case SplitKind::generalized64: return {7.7e-12, 7.5e-12};
+    // VAW: the dense kernel's term over the dense nodes' rows; a posting (three
+    // FP64 reciprocal square roots); a one-sided term at one total.  Fitted to the
+    // 8,192-sample workload at theta 0.25 / 0.5 / 1 (weighted and presence terms
+    // within 15% of each other; the weighted rates are used for both)
+    case SplitKind::vaw32: return {4.4e-13, 2.0e-12, 5.15e-13};
+    case SplitKind::vaw64: return {2.26e-12, 2.84e-12, 6.2e-13};
     }
     return {1.0, 1.0};
 }
@@ -39,7 +46,7 @@ Rates rates_of(SplitKind k) {
 // column costs every pair
 double tile_terms(SplitKind k, uint32_t n, uint32_t c) {
     const double pairs = 0.5 * (double)n * n;
-    if (k == SplitKind::gram) return pairs;
+    if (k == SplitKind::gram || k == SplitKind::vaw32 || k == SplitKind::vaw64) return pairs;
     const double q = 1.0 - (double)c / n;   // a given sample lacks the node
     const double q2 = q * q, q4 = q2 * q2, q8 = q4 * q4, q16 = q8 * q8, q32 = q16 * q16;
     return pairs * (1.0 - q32 * q32 * q32);   // 1 - q^96
@@ -47,7 +54,8 @@ double tile_terms(SplitKind k, uint32_t n, uint32_t c) {
 
 }  // namespace
 
-SplitChoice choose_split(SplitKind kind, uint32_t n, const std::vector<uint32_t>& occupancy, uint64_t max_slots) {
+SplitChoice choose_split(SplitKind kind, uint32_t n, const std::vector<uint32_t>& occupancy, uint64_t max_slots,
+                         uint32_t cols) {
     SplitChoice best;
     if (n < 2 || occupancy.empty()) return best;
     const Rates r = rates_of(kind);
@@ -68,7 +76,7 @@ SplitChoice choose_split(SplitKind kind, uint32_t n, const std::vector<uint32_t>
         if (c > 0) {
             const double m = (double)hist[c], x = c;
             cost -= m * tile_terms(kind, n, c) * r.term;
-            posts += m * 0.5 * x * (x - 1.0) * r.posting;
+            posts += m * (0.5 * x * (x - 1.0) * r.posting + x * (double)cols * r.onesided);
             slots += hist[c] * c;
         }
         if (slots >= max_slots) break;

@@ -21,6 +21,8 @@ enum class SplitKind : int {
     generalized32 = 2,  // FP32 generalized: 32 x 64 blocks, both panels' nodes, two sums
     gram = 3,           // unweighted: int8 Gram columns (every pair, no masks)
     generalized64 = 4,  // FP64 generalized: as generalized32 with FP64 operands and terms
+    vaw32 = 5,          // FP32 VAW: dense pair tiles over every pair, one-sided sums per (posting, total)
+    vaw64 = 6,          // FP64 VAW: the same with FP64 operands and terms
 };
 
 struct SplitChoice {
@@ -31,6 +33,8 @@ struct SplitChoice {
 
 // occupancy: structural occupancy of each processed node (0: held by no
 // sample).  max_slots bounds the postings (sum of min(occupancy, thr - 1)).
-SplitChoice choose_split(SplitKind kind, uint32_t n, const std::vector<uint32_t>& occupancy, uint64_t max_slots);
+// cols (VAW): the number of totals each posting's one-sided term is summed at.
+SplitChoice choose_split(SplitKind kind, uint32_t n, const std::vector<uint32_t>& occupancy, uint64_t max_slots,
+                         uint32_t cols = 0);
 
 }  // namespace ssu

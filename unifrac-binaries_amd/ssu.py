@@ -226,7 +226,8 @@ def matrix_inmem(inp: Inputs, method: str, variance_adjust=False, alpha=1.0, byp
 
 def choose_split(kind, n_samples, occupancy):
     """(threshold, modelled ms, modelled ms without a split) of the occupancy split.
-    kind: 0 FP32 weighted, 1 FP64 weighted, 2 FP32 generalized, 3 unweighted Gram, 4 FP64 generalized."""
+    kind: 0 FP32 weighted, 1 FP64 weighted, 2 FP32 generalized, 3 unweighted Gram, 4 FP64 generalized,
+    5 FP32 VAW, 6 FP64 VAW."""
     L = lib()
     L.ssu_choose_split.restype = C.c_uint
     occ = np.ascontiguousarray(occupancy, dtype=np.uint32)
