@@ -194,16 +194,17 @@ def test_empty_sample_and_zero_lengths(ssu, O, synth):
             ref = oracle_matrix(O, (st, le, nm), table, "generalized", alpha=0.5)
             assert s == ssu.OKAY and rel_err(m, ref) <= tol
             assert np.array_equal(np.isnan(m), np.isnan(ref))
-    for method in ("unweighted", "weighted_normalized", "weighted_unnormalized"):   # VAW, both paths
+    for method in ("unweighted", "weighted_normalized", "weighted_unnormalized", "generalized"):   # VAW, both paths
         for prec, tol in (("_fp64", 1e-11), ("", 1e-6)):
             for theta in ("0", "0.5"):
                 say("empty sample vaw", method + prec, theta)
                 os.environ["SSU_VAW_SPLIT"] = theta
                 try:
-                    s, m = ssu.matrix_inmem(ssu.Inputs(st, le, nm, *table), method + prec, variance_adjust=True)
+                    s, m = ssu.matrix_inmem(ssu.Inputs(st, le, nm, *table), method + prec, variance_adjust=True,
+                                            alpha=0.5)
                 finally:
                     del os.environ["SSU_VAW_SPLIT"]
-                ref = oracle_matrix(O, (st, le, nm), table, method, variance_adjust=True)
+                ref = oracle_matrix(O, (st, le, nm), table, method, variance_adjust=True, alpha=0.5)
                 assert s == ssu.OKAY and rel_err(m, ref) <= tol
                 assert np.array_equal(np.isnan(m), np.isnan(ref))
     zero = np.zeros_like(le)
@@ -672,37 +673,55 @@ def test_generalized_fp64_shared_node_path(ssu, O, synth, monkeypatch, theta, al
     assert rel_err(m, dense) <= TOL["fp64"]
 
 
-VAW_SPLIT_METHODS = ["unweighted", "unweighted_unnormalized", "weighted_normalized", "weighted_unnormalized"]
+VAW_SPLIT_METHODS = [("unweighted", {}), ("unweighted_unnormalized", {}), ("weighted_normalized", {}),
+                     ("weighted_unnormalized", {}), ("generalized", {"alpha": 0.5})]
 
 
 @pytest.mark.parametrize("theta", ["0.1", "0.5", "1"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("method", VAW_SPLIT_METHODS)
-def test_vaw_shared_node_path(ssu, O, synth, monkeypatch, method, prec, theta):
-    """VAW (weighted and presence terms) on the shared-node path: nodes held by
-    at least theta * n samples on the dense pair kernel, the others from the
-    posting lists as one-sided sums at the table's distinct totals plus the
-    walk's corrections.  Against the oracle at the precision's tolerance and
-    against the dense kernel (SSU_VAW_SPLIT=0)."""
+@pytest.mark.parametrize("method,kw", VAW_SPLIT_METHODS)
+def test_vaw_shared_node_path(ssu, O, synth, monkeypatch, method, kw, prec, theta):
+    """VAW on the shared-node path: nodes held by at least theta * n samples
+    on the dense pair kernel, the others from the posting lists as one-sided
+    sums at the table's distinct totals plus the walk's corrections.  Against
+    the oracle at the precision's tolerance and against the dense kernel
+    (SSU_VAW_SPLIT=0)."""
     tree, table = _case(synth, 500, 300, 0.05, 81, zipf=1.1)
+    _vaw_split_case(ssu, O, monkeypatch, tree, table, method, kw, prec, theta)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("alpha", [0.0, 1.0])
+def test_vaw_generalized_alpha_ends(ssu, O, synth, monkeypatch, alpha, prec):
+    """Generalized VAW at alpha 0 and 1 on the shared-node path; alpha 1.5
+    keeps the dense kernel (no split)."""
+    tree, table = _case(synth, 400, 260, 0.05, 85)
+    _vaw_split_case(ssu, O, monkeypatch, tree, table, "generalized", {"alpha": alpha}, prec, "0.5")
+    monkeypatch.setenv("SSU_VAW_SPLIT", "0.5")
+    p = ssu.Plan(inputs_from(ssu, tree, table), "generalized", variance_adjust=True, alpha=1.5)
+    assert p.split_theta() == 0.0
+    p.close()
+
+
+def _vaw_split_case(ssu, O, monkeypatch, tree, table, method, kw, prec, theta):
     monkeypatch.setenv("SSU_VAW_SPLIT", theta)
     inp = inputs_from(ssu, tree, table)
     name = method + ("_fp64" if prec == "fp64" else "")
-    say("vaw shared-node", name, theta)
-    p = ssu.Plan(inp, name, variance_adjust=True)
+    say("vaw shared-node", name, kw, theta)
+    p = ssu.Plan(inp, name, variance_adjust=True, **kw)
     assert p.main_kernel() == "k_pairs"
     assert abs(p.split_theta() - float(theta)) < 0.01
     p.execute()
     m = p.matrix()
     assert p.sparse_terms() > 0
     p.close()
-    ref = oracle_matrix(O, tree, table, method, variance_adjust=True)
+    ref = oracle_matrix(O, tree, table, method, variance_adjust=True, **kw)
     err = rel_err(m, ref)
     say("max rel err", err)
     assert err <= TOL[prec]
     assert np.array_equal(m, m.T) and np.all(np.diag(m) == 0)
     monkeypatch.setenv("SSU_VAW_SPLIT", "0")
-    s, dense = ssu.matrix_inmem(inp, name, variance_adjust=True)
+    s, dense = ssu.matrix_inmem(inp, name, variance_adjust=True, **kw)
     assert s == ssu.OKAY
     assert rel_err(m, dense) <= 2 * TOL[prec]
 

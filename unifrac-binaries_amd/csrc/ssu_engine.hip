@@ -1141,10 +1141,11 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         P.sparse = P.sparse && vals_nonneg;
     }
     P.gen = P.gen && P.sparse;
-    // VAW with weighted or presence terms (generalized VAW stays on the dense kernel)
-    // takes the posting lists for its rare nodes when every value and length is
-    // finite and >= 0 (split_vaw decides whether it pays)
-    P.vsplit = spec.vaw && spec.method != Method::generalized && spec.plain_kernel == 0 && vals_nonneg;
+    // VAW (weighted, presence, or generalized with 0 <= alpha <= 1 terms) takes the
+    // posting lists for its rare nodes when every value and length is finite and
+    // >= 0 (split_vaw decides whether it pays)
+    P.vsplit = spec.vaw && spec.plain_kernel == 0 && vals_nonneg &&
+               (spec.method != Method::generalized || (spec.alpha >= 0.0 && spec.alpha <= 1.0));
     for (size_t q = 0; q < P.klen_d.size() && P.vsplit; ++q) {
         const double l = spec.fp64 ? P.klen_d[q] : (double)P.klen_f[q];
         P.vsplit = std::isfinite(l) && l >= 0.0;
@@ -1406,7 +1407,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         b.band = P.band;
         if (vsplit) {
             const bool unw = s.method == Method::unweighted || s.method == Method::unweighted_unnormalized;
-            b.vaw = unw ? 2 : 1;
+            b.vaw = s.method == Method::generalized ? 3 : unw ? 2 : 1;
             b.fixed_dense = P.d_vdense.as<uint32_t>();
             b.Cv = s.fp64 ? P.d_C.p : P.d_Cu.p;
             b.pcv = P.d_pcv.p;

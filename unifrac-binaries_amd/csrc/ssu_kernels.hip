@@ -1657,14 +1657,26 @@ __device__ __forceinline__ void row_term(unsigned long long* accb, const RowEntr
     }
 }
 
-// VAW (1: weighted values, 2: presence): one posting's corrections for the
-// pair (a, b) holding node r, the both-held terms less the one-sided terms the
-// sums H hold, numerator and denominator; en.lu = u_a, en.uap = c_a, en.q8 = l_r
-// (FP64 bits); fixed point in units of 1 / scale, rounded (|x * scale| < 2^61
-// for FP64 values, 2^51 for FP32 values).
+// Generalized VAW one-sided term (src/unifrac_task_impl.hpp:1097-1109 with one
+// count zero): sum1 = sub1 = u / vaw, so the numerator and the denominator
+// both gain l (u / vaw)^alpha, the power in the value type (FP32: the
+// hardware log2 / exp2; FP64: the series power); shared by the one-sided sums
+// and the walk as vaw_one is.
+template <int STEPS, class TV>
+__device__ __forceinline__ double vaw_one_gen(double l, double u, double c, double e, double tau, double alpha) {
+    const double x = c * (e + tau);
+    const double r = rsqrt_nr<STEPS>(x);   // (x = 0: not finite, and not taken)
+    return x > 0.0 ? __dmul_rn(l, (double)gen_pow((TV)(u * r), alpha)) : 0.0;
+}
+
+// VAW (1: weighted values, 2: presence, 3: generalized): one posting's
+// corrections for the pair (a, b) holding node r, the both-held terms less the
+// one-sided terms the sums H hold, numerator and denominator; en.lu = u_a,
+// en.uap = c_a, en.q8 = l_r (FP64 bits); fixed point in units of 1 / scale,
+// rounded (|x * scale| < 2^61 for FP64 values, 2^51 for FP32 values).
 template <class TV, int VAW, uint32_t CW>
 __device__ __forceinline__ void vaw_row_term(unsigned long long* accb, const RowEntry<TV, true>& en, uint32_t b,
-                                             TV v, TV cv, double Ta, double Tb, double scale) {
+                                             TV v, TV cv, double Ta, double Tb, double scale, double alpha) {
     constexpr int ST = sizeof(TV) == 8 ? 2 : 1;
     const double l = __longlong_as_double((long long)en.q8);
     const double ua = (double)en.lu, ca = (double)en.uap, ub = (double)v, cb = (double)cv;
@@ -1672,11 +1684,21 @@ __device__ __forceinline__ void vaw_row_term(unsigned long long* accb, const Row
     const double x = (ca + cb) * (ea + eb);
     const double r = rsqrt_nr<ST>(x);   // (x = 0: not finite, and not taken)
     const double rb = x > 0.0 ? r : 0.0;
-    const double one = vaw_one<ST>(VAW == 2 ? l : l * ua, ca, ea, Tb) + vaw_one<ST>(VAW == 2 ? l : l * ub, cb, eb, Ta);
+    double one;
+    if constexpr (VAW == 3)
+        one = vaw_one_gen<ST, TV>(l, ua, ca, ea, Tb, alpha) + vaw_one_gen<ST, TV>(l, ub, cb, eb, Ta, alpha);
+    else
+        one = vaw_one<ST>(VAW == 2 ? l : l * ua, ca, ea, Tb) + vaw_one<ST>(VAW == 2 ? l : l * ub, cb, eb, Ta);
     double cn, cd;
     if constexpr (VAW == 2) {   // xor = 0, or = 1 for a node both hold
         cn = -one;
         cd = l * rb - one;
+    } else if constexpr (VAW == 3) {   // w = l ((u + v) / vaw)^alpha, num += w |u - v| / (u + v)
+        const double s1 = (ua + ub) * rb;
+        const double w = x > 0.0 ? l * (double)gen_pow((TV)s1, alpha) : 0.0;
+        const double q = x > 0.0 ? __builtin_fabs(ua - ub) / (ua + ub) : 0.0;
+        cn = w * q - one;
+        cd = w - one;
     } else {
         cn = l * __builtin_fabs(ua - ub) * rb - one;
         cd = l * (ua + ub) * rb - one;
@@ -1903,7 +1925,8 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
 #pragma unroll
                     for (uint32_t u = 0; u < SR_UNROLL; ++u) {
                         if constexpr (VAW != 0)
-                            vaw_row_term<TV, VAW, CW>(accb, en[u], bq[u], vq[u], cq[u], Ta, tot_s[bq[u] - b0], scale);
+                            vaw_row_term<TV, VAW, CW>(accb, en[u], bq[u], vq[u], cq[u], Ta, tot_s[bq[u] - b0], scale,
+                                                      alpha);
                         else
                             row_term<TV, GRAM, GEN, FINE, CW>(accb, en[u], bq[u], vq[u], alpha);
                     }
@@ -1926,7 +1949,8 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
                     for (uint32_t u = 0; u < SR_UNROLL; ++u) {
                         if (j0 + 64 * u >= wn) continue;
                         if constexpr (VAW != 0)
-                            vaw_row_term<TV, VAW, CW>(accb, en[u], bq[u], vq[u], cq[u], Ta, tot_s[bq[u] - b0], scale);
+                            vaw_row_term<TV, VAW, CW>(accb, en[u], bq[u], vq[u], cq[u], Ta, tot_s[bq[u] - b0], scale,
+                                                      alpha);
                         else
                             row_term<TV, GRAM, GEN, FINE, CW>(accb, en[u], bq[u], vq[u], alpha);
                     }
@@ -2037,12 +2061,14 @@ __global__ void __launch_bounds__(64) k_sparse_sums(double* __restrict__ ssp,
 // positive e, tmin the smallest positive total, g = min(emin, tmin):
 //   B_r = l umax (2 / sqrt(cmin emin) + 2 / sqrt(cmin g)).
 // One thread per node.
-template <class IX, class TV, bool UNW>
+// Generalized (GEN): the terms are l s^alpha with s at most the quotients above,
+// so B_r = l ((2 umax / sqrt(cmin emin))^alpha + 2 (umax / sqrt(cmin g))^alpha).
+template <class IX, class TV, bool UNW, bool GEN>
 __global__ void __launch_bounds__(64) k_vaw_node_bound(double* __restrict__ nb, const IX* __restrict__ pidx,
                                                        const TV* __restrict__ pval, const TV* __restrict__ pcv,
                                                        const uint32_t* __restrict__ poff,
                                                        const double* __restrict__ tot, const double* __restrict__ len,
-                                                       uint32_t K, uint32_t slots) {
+                                                       uint32_t K, uint32_t slots, double alpha) {
     const uint32_t r = blockIdx.x * 64 + threadIdx.x;
     if (r >= K) return;
     double b = 0.0;
@@ -2060,7 +2086,11 @@ __global__ void __launch_bounds__(64) k_vaw_node_bound(double* __restrict__ nb, 
         if (cmin < INF) {
             const double g = emin < tmin ? emin : tmin;
             const double uu = UNW ? 1.0 : umax;
-            b = len[r] * uu * ((emin < INF ? 2.0 / sqrt(cmin * emin) : 0.0) + (g < INF ? 2.0 / sqrt(cmin * g) : 0.0));
+            if constexpr (GEN)
+                b = len[r] * ((emin < INF ? pow(2.0 * uu / sqrt(cmin * emin), alpha) : 0.0) +
+                              (g < INF ? 2.0 * pow(uu / sqrt(cmin * g), alpha) : 0.0));
+            else
+                b = len[r] * uu * ((emin < INF ? 2.0 / sqrt(cmin * emin) : 0.0) + (g < INF ? 2.0 / sqrt(cmin * g) : 0.0));
         }
     }
     nb[r] = b;
@@ -2086,23 +2116,25 @@ __global__ void __launch_bounds__(64) k_row_weights(double* __restrict__ out, co
 }
 
 // One-sided sums H[a][k] = sum over sample a's sparse nodes r (its row list, in
-// order) of vaw_one(l_r u_ar, c_ar, max(T_a - c_ar, 0), tau[k]) (UNW: l_r in
-// place of l_r u_ar), in FP64.  One wavefront per (sample a, 256 totals): the
-// row's entries are staged 64 at a time in LDS and read by every lane, each
-// lane summing VH_COLS totals.
+// order) of vaw_one(l_r u_ar, c_ar, max(T_a - c_ar, 0), tau[k]) (VAW 2,
+// presence: l_r in place of l_r u_ar; VAW 3, generalized: vaw_one_gen), in
+// FP64.  One wavefront per (sample a, 256 totals): the row's entries are
+// staged 64 at a time in LDS and read by every lane, each lane summing
+// VH_COLS totals.
 constexpr uint32_t VH_COLS = 4;
-template <class TV, bool UNW>
+template <class TV, int VAW>
 __global__ void __launch_bounds__(64) k_vaw_onesided(double* __restrict__ H, uint32_t D, const double* __restrict__ tau,
                                                      const unsigned long long* __restrict__ rows,
                                                      const uint32_t* __restrict__ offsets, uint32_t nkc,
                                                      const uint32_t* __restrict__ poff, const TV* __restrict__ pval,
                                                      const TV* __restrict__ pcv, const double* __restrict__ len,
                                                      const double* __restrict__ tot, uint32_t K, uint32_t n,
-                                                     uint32_t slots) {
+                                                     uint32_t slots, double alpha) {
     constexpr int ST = sizeof(TV) == 8 ? 2 : 1;
+    constexpr bool GEN = VAW == 3;
     const uint32_t a = blockIdx.x, lane = threadIdx.x;
     const uint32_t k0 = blockIdx.y * 64 * VH_COLS + lane;
-    __shared__ double sk[64], sc[64], se[64];
+    __shared__ double sk[64], sc[64], se[64], su[GEN ? 64 : 1];
     double tk[VH_COLS], acc[VH_COLS];
 #pragma unroll
     for (uint32_t j = 0; j < VH_COLS; ++j) {
@@ -2115,26 +2147,34 @@ __global__ void __launch_bounds__(64) k_vaw_onesided(double* __restrict__ H, uin
         const double Ta = tot[a];
         const uint32_t e0 = offsets[(size_t)a * nkc], e1 = offsets[(size_t)(a + 1) * nkc];
         for (uint32_t q0 = e0; q0 < e1; q0 += 64) {   // uniform
-            double kk = 0.0, cc = 0.0, ee = 0.0;
+            double kk = 0.0, cc = 0.0, ee = 0.0, uu = 0.0;
             if (q0 + lane < e1) {
                 const unsigned long long ent = rows[q0 + lane];
                 const uint32_t r = (uint32_t)(ent >> 32);
                 const uint32_t slot = poff[r] + (uint32_t)ent;
                 const double l = len[r];
                 cc = (double)pcv[slot];
-                kk = UNW ? l : l * (double)pval[slot];
+                uu = (double)pval[slot];
+                kk = VAW == 1 ? l * uu : l;
                 ee = Ta - cc > 0.0 ? Ta - cc : 0.0;
             }
             __syncthreads();   // the previous round's entries are read
             sk[lane] = kk;
             sc[lane] = cc;
             se[lane] = ee;
+            if constexpr (GEN) su[lane] = uu;
             __syncthreads();
             const uint32_t cnt = min(64u, e1 - q0);
             for (uint32_t i = 0; i < cnt; ++i) {
                 const double kv = sk[i], cv = sc[i], ev = se[i];
+                if constexpr (GEN) {
+                    const double uv = su[i];
 #pragma unroll
-                for (uint32_t j = 0; j < VH_COLS; ++j) acc[j] += vaw_one<ST>(kv, cv, ev, tk[j]);
+                    for (uint32_t j = 0; j < VH_COLS; ++j) acc[j] += vaw_one_gen<ST, TV>(kv, uv, cv, ev, tk[j], alpha);
+                } else {
+#pragma unroll
+                    for (uint32_t j = 0; j < VH_COLS; ++j) acc[j] += vaw_one<ST>(kv, cv, ev, tk[j]);
+                }
             }
         }
     }
@@ -2645,16 +2685,26 @@ static void pairs_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs&
 template <class IX, class TV, int VAW>
 static void vaw_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs& a, uint32_t nchunks, uint32_t nkc,
                        uint32_t a0, uint32_t na, uint32_t mode) {
-    constexpr bool UNW = VAW == 2;
-    hipLaunchKernelGGL((k_vaw_node_bound<IX, TV, UNW>), dim3((a.K + 63) / 64), dim3(64), 0, st, a.nbound,
-                       (const IX*)a.pidx, (const TV*)a.pval, (const TV*)a.pcv, (const uint32_t*)a.poff, a.tot, a.len,
-                       a.K, a.slots);
+    hipLaunchKernelGGL((k_vaw_node_bound<IX, TV, VAW == 2, VAW == 3>), dim3((a.K + 63) / 64), dim3(64), 0, st,
+                       a.nbound, (const IX*)a.pidx, (const TV*)a.pval, (const TV*)a.pcv, (const uint32_t*)a.poff,
+                       a.tot, a.len, a.K, a.slots, a.alpha);
     hipLaunchKernelGGL(k_row_weights, dim3(a.n), dim3(64), 0, st, a.ssp, a.rows, a.roff, nkc, a.nbound,
                        (const uint32_t*)a.poff, a.K, a.n, a.slots);
-    hipLaunchKernelGGL((k_vaw_onesided<TV, UNW>), dim3(a.n, (a.D + 64 * VH_COLS - 1) / (64 * VH_COLS)), dim3(64), 0,
+    hipLaunchKernelGGL((k_vaw_onesided<TV, VAW>), dim3(a.n, (a.D + 64 * VH_COLS - 1) / (64 * VH_COLS)), dim3(64), 0,
                        st, a.H, a.D, a.tau, a.rows, a.roff, nkc, (const uint32_t*)a.poff, (const TV*)a.pval,
-                       (const TV*)a.pcv, a.len, a.tot, a.K, a.n, a.slots);
+                       (const TV*)a.pcv, a.len, a.tot, a.K, a.n, a.slots, a.alpha);
     pairs_rows_t<IX, false, TV, false, VAW>(blocks, st, a, nchunks, nkc, a0, na, mode);
+}
+template <int VAW>
+static void vaw_rows_v(bool narrow, bool fp64, uint32_t blocks, hipStream_t st, const SparseNodesArgs& a,
+                       uint32_t nchunks, uint32_t nkc, uint32_t a0, uint32_t na, uint32_t mode) {
+    if (fp64) {
+        if (narrow) vaw_rows_t<uint16_t, double, VAW>(blocks, st, a, nchunks, nkc, a0, na, mode);
+        else vaw_rows_t<uint32_t, double, VAW>(blocks, st, a, nchunks, nkc, a0, na, mode);
+    } else {
+        if (narrow) vaw_rows_t<uint16_t, float, VAW>(blocks, st, a, nchunks, nkc, a0, na, mode);
+        else vaw_rows_t<uint32_t, float, VAW>(blocks, st, a, nchunks, nkc, a0, na, mode);
+    }
 }
 
 hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st, uint32_t row0, uint32_t row1, int parts) {
@@ -2673,20 +2723,14 @@ hipError_t launch_sparse_rows(const SparseNodesArgs& a, hipStream_t st, uint32_t
     const bool narrow = a.n <= 65536;
     const bool whole = row0 == 0 && row1 == a.n;   // the per-sample sums are made for all rows at once
     if (a.vaw) {
-        if (a.gram || a.gen || a.vaw > 2 || !a.ssp || !a.out2 || !a.pcv || !a.tot || !a.tau || !a.H || !a.nbound ||
-            !a.D || a.D > a.n || (!a.kidx && a.D != a.n) || !whole)
+        if (a.gram || a.gen || a.vaw < 1 || a.vaw > 3 || !a.ssp || !a.out2 || !a.pcv || !a.tot || !a.tau || !a.H ||
+            !a.nbound || !a.D || a.D > a.n || (!a.kidx && a.D != a.n) || !whole ||
+            (a.vaw == 3 && !(a.alpha >= 0.0 && a.alpha <= 1.0)))
             return hipErrorInvalidValue;
-        const int key = (narrow ? 1 : 0) | (a.values_fp64 ? 2 : 0) | (a.vaw == 2 ? 4 : 0);
-        switch (key) {
-        case 0: vaw_rows_t<uint32_t, float, 1>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
-        case 1: vaw_rows_t<uint16_t, float, 1>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
-        case 2: vaw_rows_t<uint32_t, double, 1>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
-        case 3: vaw_rows_t<uint16_t, double, 1>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
-        case 4: vaw_rows_t<uint32_t, float, 2>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
-        case 5: vaw_rows_t<uint16_t, float, 2>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
-        case 6: vaw_rows_t<uint32_t, double, 2>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
-        default: vaw_rows_t<uint16_t, double, 2>((uint32_t)blocks, st, a, nchunks, nkc, row0, na, mode); break;
-        }
+        const uint32_t nb = (uint32_t)blocks;
+        if (a.vaw == 1) vaw_rows_v<1>(narrow, a.values_fp64, nb, st, a, nchunks, nkc, row0, na, mode);
+        else if (a.vaw == 2) vaw_rows_v<2>(narrow, a.values_fp64, nb, st, a, nchunks, nkc, row0, na, mode);
+        else vaw_rows_v<3>(narrow, a.values_fp64, nb, st, a, nchunks, nkc, row0, na, mode);
         return hipGetLastError();
     }
     if (a.gen && a.values_fp64) {   // generalized, FP64 values

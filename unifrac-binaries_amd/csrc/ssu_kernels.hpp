@@ -258,8 +258,9 @@ struct SparseNodesArgs {
     GapBand band;           // rows walk (and write) only the columns of selected pairs (G(a, a) always)
     // dense nodes fixed in advance ([ceil(K/32)] words, in place of the occupancy test)
     const uint32_t* fixed_dense = nullptr;
-    // VAW (1: weighted values, 2: presence): numerator shares to out, denominator shares
-    // to out2, each H(a, T_b) + H(b, T_a) plus the walk's corrections of the shared nodes
+    // VAW (1: weighted values, 2: presence, 3: generalized, 0 <= alpha <= 1): numerator shares
+    // to out, denominator shares to out2, each H(a, T_b) + H(b, T_a) plus the walk's
+    // corrections of the shared nodes
     int vaw = 0;
     const void* Cv = nullptr;       // count rows, indexed as U
     void* pcv = nullptr;            // [slots] counts, in the value type
