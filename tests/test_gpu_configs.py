@@ -5,10 +5,11 @@ their real sizes, on the device (C ABI of libssu.so):
   full 256 x 256 matrices against the FP64 oracle -- FP64 names within 1e-11,
   FP32 names within 1e-6 -- and Faith PD bit-exact;
 * w_50k_8192 (50,000 tips, 8,192 samples, 2% density, negative-binomial
-  counts): weighted_normalized, generalized (alpha 0.5) and VAW weighted
-  through their default paths: symmetric, zero diagonal, within [0, 1]; the
-  first 64 rows against the plain dense pair tiles (2e-6 relative); a 48-sample
-  subset within 1e-6 of the FP64 oracle;
+  counts): weighted_normalized, generalized (alpha 0.5), VAW weighted and VAW
+  generalized (alpha 0.5) through their default paths (occupancy splits):
+  symmetric, zero diagonal, within [0, 1]; the first 64 rows against the plain
+  dense pair tiles (2e-6 relative); a 48-sample subset within 1e-6 of the FP64
+  oracle;
 * rarefy_20k_4tables (20,000 tips, tables of 512 / 1,536 / 3,000 / 6,000
   samples, 32 replicates at depth 1,000 with replacement, unweighted +
   weighted_normalized): graph replays equal the eager pass, replicate 0 of
@@ -91,8 +92,10 @@ def cfg3(synth):
 
 
 @pytest.mark.parametrize("name,kw", [("weighted_normalized", {}), ("generalized", {"alpha": 0.5}),
-                                     ("weighted_normalized", {"variance_adjust": True})],
-                         ids=["weighted_normalized", "generalized-0.5", "vaw-weighted_normalized"])
+                                     ("weighted_normalized", {"variance_adjust": True}),
+                                     ("generalized", {"alpha": 0.5, "variance_adjust": True})],
+                         ids=["weighted_normalized", "generalized-0.5", "vaw-weighted_normalized",
+                              "vaw-generalized-0.5"])
 def test_cfg3_w_50k_8192(ssu, O, cfg3, name, kw):
     tree, table = cfg3
     n = len(table[1])
@@ -103,17 +106,18 @@ def test_cfg3_w_50k_8192(ssu, O, cfg3, name, kw):
     p.execute()
     m = p.matrix()
     say("main kernel", p.main_kernel(), "timings", p.timings())
+    p_theta = p.split_theta()
+    assert p_theta > 0.0   # every one of these takes an occupancy split at this size
     p.close()
     assert np.all(np.isfinite(m))
     assert np.array_equal(m, m.T) and np.all(np.diag(m) == 0)
     assert m.min() >= 0.0 and m.max() <= 1.0 + 1e-6
-    if not kw.get("variance_adjust"):   # VAW runs the dense tiles already
-        say("cfg3", name, "plain dense tiles, rows 0..63")
-        q = ssu.Plan(inp, name, plain_kernel=1, **kw)
-        q.execute()
-        band = q.rows(0, 64)
-        q.close()
-        assert rel_err(m[:64], band) <= 2e-6
+    say("cfg3", name, "split theta", p_theta, "plain dense tiles, rows 0..63")
+    q = ssu.Plan(inp, name, plain_kernel=1, **kw)
+    q.execute()
+    band = q.rows(0, 64)
+    q.close()
+    assert rel_err(m[:64], band) <= 2e-6
     idx = subset(n)
     ref = oracle_on(O, tree, table, idx, name, **kw)
     err = rel_err(m[np.ix_(idx, idx)], ref)
