@@ -518,6 +518,21 @@ SSU_EXTERN uint64_t ssu_rarefy_pair_nodes(const ssu_rarefy_t* batch);
  * the FP32 weighted and generalized methods), over every replicate */
 SSU_EXTERN uint64_t ssu_rarefy_refined_pairs(const ssu_rarefy_t* batch);
 SSU_EXTERN void ssu_rarefy_destroy(ssu_rarefy_t** batch);
+/* Several rarefaction batches (tables of uneven sample counts) in one HIP graph
+ * launch.  Extension, no counterpart in the reference, whose tables run one call
+ * at a time.  Each batch's captured graph is a child node of one parent graph,
+ * with no dependence between them, so the tables run side by side on the device.
+ * Batches not yet executed run their first, eager execute inside create.  The
+ * batches stay owned by the caller and must outlive the group.  After a group
+ * execute, each batch's matrices, counts and refined-pair count read as after
+ * its own execute; its last_ms reads 0, and the group's last_ms times the
+ * whole launch. */
+typedef struct ssu_rarefy_group ssu_rarefy_group_t;
+SSU_EXTERN ComputeStatus ssu_rarefy_group_create(ssu_rarefy_t* const* batches, int n_batches,
+                                                 ssu_rarefy_group_t** out);
+SSU_EXTERN ComputeStatus ssu_rarefy_group_execute(ssu_rarefy_group_t* group);
+SSU_EXTERN double ssu_rarefy_group_last_ms(const ssu_rarefy_group_t* group);
+SSU_EXTERN void ssu_rarefy_group_destroy(ssu_rarefy_group_t** group);
 
 /* Host-side tree handling only (no device): sheared + collapsed postorder
  * arrays of tree_data restricted to obs_ids.  Arrays hold n_parens/2 entries.

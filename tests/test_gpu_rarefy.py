@@ -317,6 +317,45 @@ def test_generalized_fp32_replicates_and_replays(ssu, O, synth):
         assert np.array_equal(r.matrix(0, rep), eager[rep]), rep
 
 
+def test_group_launch_equals_each_batch(ssu, O, synth):
+    """Three tables of uneven sample counts in one group launch (every batch's
+    graph a child of one parent graph): every replicate matrix equal to the
+    batch's own eager pass, on two launches in a row; replicate 0 against the
+    oracle."""
+    tree = synth.random_tree(300, 61)
+    tables = [synth.random_table(300, n, 0.1, 62 + n, None, "lognormal") for n in (40, 90, 150)]
+    methods = ["unweighted", "weighted_normalized", "weighted_unnormalized_fp64"]
+    batches, eager = [], []
+    for k, t in enumerate(tables):
+        depth = depth_dropping_some(t, 10)
+        say("group batch", k, len(t[1]), depth)
+        b = ssu.Rarefaction(inputs(ssu, tree, t), methods, depth, 3, seed=20 + k)
+        b.execute()   # eager, captures the graph
+        eager.append({(m, rep): b.matrix(m, rep) for m in range(len(methods)) for rep in range(3)})
+        batches.append(b)
+    g = ssu.RarefactionGroup(batches)
+    st, le, nm = tree
+    otree = O.Tree(np.asarray(st, np.uint8), le, nm)
+    for launch in range(2):
+        say("group launch", launch)
+        g.execute()
+        assert g.last_ms() > 0.0
+        for k, b in enumerate(batches):
+            assert b.graph_used()
+            for (m, rep), x in eager[k].items():
+                assert np.array_equal(b.matrix(m, rep), x), (k, methods[m], rep)
+    for k, (b, t) in enumerate(zip(batches, tables)):
+        si, oi, ip, ix = b.kept()
+        ot = O.Table([t[0][i] for i in oi], [t[1][s] for s in si], ip, ix, b.counts(0))
+        for m, name in enumerate(methods):
+            base = name[:-5] if name.endswith("_fp64") else name
+            err = rel_err(b.matrix(m, 0), O.matrix(otree, ot, base))
+            assert err <= TOL["fp64" if name.endswith("_fp64") else "fp32"], (k, name, err)
+    g.close()
+    for b in batches:
+        b.close()
+
+
 def test_vaw_shared_node_replicates_and_replays(ssu, O, synth, monkeypatch):
     """VAW on the shared-node path in rarefied replicates (the executions bring
     their own totals: one-sided sums at every sample's total): every replicate

@@ -1455,6 +1455,40 @@ void ssu_rarefy_destroy(ssu_rarefy_t** r) {
     *r = nullptr;
 }
 
+struct ssu_rarefy_group {
+    std::unique_ptr<ssu::RarefyGroup> group;
+};
+
+ComputeStatus ssu_rarefy_group_create(ssu_rarefy_t* const* batches, int n_batches, ssu_rarefy_group_t** out) {
+    if (!out || !batches || n_batches <= 0) return output_error;
+    *out = nullptr;
+    std::vector<ssu::RarefyBatch*> v;
+    for (int i = 0; i < n_batches; ++i) {
+        if (!batches[i] || !batches[i]->batch) { set_error("null rarefaction batch"); return output_error; }
+        v.push_back(batches[i]->batch.get());
+    }
+    std::string err;
+    auto g = ssu::RarefyGroup::create(v, &err);
+    if (!g) { set_error(err); return output_error; }
+    *out = new ssu_rarefy_group{std::move(g)};
+    return okay;
+}
+
+ComputeStatus ssu_rarefy_group_execute(ssu_rarefy_group_t* g) {
+    if (!g || !g->group) return output_error;
+    std::string err;
+    if (!g->group->execute(&err)) { set_error(err); return output_error; }
+    return okay;
+}
+
+double ssu_rarefy_group_last_ms(const ssu_rarefy_group_t* g) { return g && g->group ? g->group->last_ms() : 0.0; }
+
+void ssu_rarefy_group_destroy(ssu_rarefy_group_t** g) {
+    if (!g || !*g) return;
+    delete *g;
+    *g = nullptr;
+}
+
 long ssu_biom_n_samples(const char* biom_filename) {
     uint64_t n = 0;
     std::string err;

@@ -422,6 +422,69 @@ bool RarefyBatch::execute(std::string* err) {
     return R.refine(err);
 }
 
+struct RarefyGroupImpl {
+    std::vector<RarefyBatch*> batches;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.0f;
+    ~RarefyGroupImpl() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+RarefyGroup::RarefyGroup() : p_(new RarefyGroupImpl) {}
+RarefyGroup::~RarefyGroup() = default;
+
+std::unique_ptr<RarefyGroup> RarefyGroup::create(const std::vector<RarefyBatch*>& batches, std::string* err) {
+    std::unique_ptr<RarefyGroup> g(new RarefyGroup);
+    RarefyGroupImpl& G = *g->p_;
+    auto fail = [&](const char* what) {
+        *err = std::string(what) + " failed";
+        return nullptr;
+    };
+    if (batches.empty()) { *err = "a rarefaction group needs at least one batch"; return nullptr; }
+    for (RarefyBatch* b : batches) {
+        if (!b) { *err = "a rarefaction group holds a null batch"; return nullptr; }
+        if (!b->p_->captured && !b->execute(err)) return nullptr;   // the eager pass, then the capture
+        if (!b->p_->graph) { *err = "batch without a captured graph"; return nullptr; }
+    }
+    G.batches = batches;
+    if (hipStreamCreateWithFlags(&G.st, hipStreamNonBlocking) != hipSuccess) return fail("stream creation");
+    if (hipEventCreate(&G.e0) != hipSuccess || hipEventCreate(&G.e1) != hipSuccess) return fail("event creation");
+    if (hipGraphCreate(&G.graph, 0) != hipSuccess) return fail("hipGraphCreate");
+    for (RarefyBatch* b : batches) {
+        hipGraphNode_t node = nullptr;
+        if (hipGraphAddChildGraphNode(&node, G.graph, nullptr, 0, b->p_->graph) != hipSuccess)
+            return fail("hipGraphAddChildGraphNode");
+    }
+    if (hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0) != hipSuccess) return fail("hipGraphInstantiate");
+    return g;
+}
+
+bool RarefyGroup::execute(std::string* err) {
+    RarefyGroupImpl& G = *p_;
+    RF_HIP(hipEventRecord(G.e0, G.st));
+    RF_HIP(hipGraphLaunch(G.exec, G.st));
+    RF_HIP(hipEventRecord(G.e1, G.st));
+    RF_HIP(hipStreamSynchronize(G.st));
+    RF_HIP(hipEventElapsedTime(&G.ms, G.e0, G.e1));
+    for (RarefyBatch* b : G.batches) {
+        RarefyImpl& R = *b->p_;
+        R.used_graph = true;
+        R.ms = 0.0f;   // (timed for the group as a whole)
+        if (!R.refine(err)) return false;
+    }
+    return true;
+}
+
+float RarefyGroup::last_ms() const { return p_->ms; }
+
 uint32_t RarefyBatch::n_samples() const { return p_->table.n_samples; }
 uint32_t RarefyBatch::n_obs() const { return p_->table.n_obs; }
 uint64_t RarefyBatch::nnz() const { return p_->nnz; }

@@ -43,8 +43,29 @@ public:
     uint64_t refined_pairs() const;       // pairs the last execute() recomputed in FP64 (all replicates)
 
 private:
+    friend class RarefyGroup;
     RarefyBatch();
     std::unique_ptr<RarefyImpl> p_;
+};
+
+struct RarefyGroupImpl;
+
+// Several batches (tables of uneven sample counts) in one HIP graph launch:
+// every batch's captured graph is a child node of one parent graph, with no
+// dependence between them, so the device runs the tables side by side.  The
+// batches stay owned by the caller and must outlive the group; each keeps its
+// own outputs, as after its own execute().
+class RarefyGroup {
+public:
+    ~RarefyGroup();
+    // batches not yet captured run their first (eager) execute here
+    static std::unique_ptr<RarefyGroup> create(const std::vector<RarefyBatch*>& batches, std::string* err);
+    bool execute(std::string* err);   // one launch of the parent graph, then each batch's refinement
+    float last_ms() const;            // device time of the last execute(), launch to last kernel
+
+private:
+    RarefyGroup();
+    std::unique_ptr<RarefyGroupImpl> p_;
 };
 
 }  // namespace ssu

@@ -94,6 +94,7 @@ EXPORTS = [
     "ssu_rarefy_create_inmem", "ssu_rarefy_execute", "ssu_rarefy_n_samples", "ssu_rarefy_n_obs", "ssu_rarefy_nnz",
     "ssu_rarefy_kept", "ssu_rarefy_copy_counts", "ssu_rarefy_copy_matrix", "ssu_rarefy_last_ms",
     "ssu_rarefy_graph_used", "ssu_rarefy_pair_nodes", "ssu_rarefy_refined_pairs", "ssu_rarefy_destroy",
+    "ssu_rarefy_group_create", "ssu_rarefy_group_execute", "ssu_rarefy_group_last_ms", "ssu_rarefy_group_destroy",
 ]
 
 _lib = None
@@ -139,8 +140,9 @@ def lib():
         L.ssu_biom_n_samples.restype = C.c_long
         L.read_partial_one_stripe.argtypes = [C.POINTER(PartialDynMat), C.c_uint32]
         for n in ("ssu_rarefy_create_inmem", "ssu_rarefy_execute", "ssu_rarefy_copy_counts", "ssu_rarefy_copy_matrix",
-                  "ssu_rarefy_graph_used"):
+                  "ssu_rarefy_graph_used", "ssu_rarefy_group_create", "ssu_rarefy_group_execute"):
             getattr(L, n).restype = C.c_int
+        L.ssu_rarefy_group_last_ms.restype = C.c_double
         L.ssu_rarefy_n_samples.restype = C.c_uint
         L.ssu_rarefy_n_obs.restype = C.c_uint
         L.ssu_rarefy_nnz.restype = C.c_uint64
@@ -455,6 +457,38 @@ class Rarefaction:
     def close(self):
         if self.h:
             lib().ssu_rarefy_destroy(C.byref(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RarefactionGroup:
+    """Several Rarefaction batches in one HIP graph launch (extension): each
+    batch's captured graph is a child node of one parent graph, so the tables
+    run side by side.  The batches must stay open while the group is."""
+
+    def __init__(self, batches):
+        self.batches = list(batches)
+        arr = (C.c_void_p * len(self.batches))(*[b.h.value for b in self.batches])
+        self.h = C.c_void_p()
+        st = lib().ssu_rarefy_group_create(arr, C.c_int(len(self.batches)), C.byref(self.h))
+        if st != OKAY:
+            raise RuntimeError(f"ssu_rarefy_group_create failed: status {st}: {last_error()}")
+
+    def execute(self):
+        if lib().ssu_rarefy_group_execute(self.h) != OKAY:
+            raise RuntimeError(f"ssu_rarefy_group_execute failed: {last_error()}")
+
+    def last_ms(self) -> float:
+        return float(lib().ssu_rarefy_group_last_ms(self.h))
+
+    def close(self):
+        if self.h:
+            lib().ssu_rarefy_group_destroy(C.byref(self.h))
             self.h = C.c_void_p()
 
     def __del__(self):
