@@ -34,7 +34,7 @@ Rates rates_of(SplitKind k) {
     // FP64 reciprocal square roots); a one-sided term at one total.  Fitted to the
     // 8,192-sample workload at theta 0.25 / 0.5 / 1 (weighted and presence terms
     // within 15% of each other; the weighted rates are used for both)
-    case SplitKind::vaw32: return {4.4e-13, 2.0e-12, 5.15e-13};
+    case SplitKind::vaw32: return {4.4e-13, 1.72e-12, 4.13e-13};
     case SplitKind::vaw64: return {2.26e-12, 2.84e-12, 6.2e-13};
     }
     return {1.0, 1.0};
