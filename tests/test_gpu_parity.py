@@ -703,7 +703,30 @@ def test_vaw_generalized_alpha_ends(ssu, O, synth, monkeypatch, alpha, prec):
     p.close()
 
 
-def _vaw_split_case(ssu, O, monkeypatch, tree, table, method, kw, prec, theta):
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("method,kw", [("weighted_normalized", {"bypass_tips": True}),
+                                       ("weighted_normalized", {"normalize": False}),
+                                       ("unweighted", {"bypass_tips": True}),
+                                       ("generalized", {"alpha": 0.3, "normalize": False})])
+def test_vaw_shared_node_options(ssu, O, synth, monkeypatch, method, kw, prec):
+    """VAW on the shared-node path with bypass_tips (tips leave the node set)
+    and normalize_sample_counts=False (raw values in place of proportions)."""
+    tree, table = _case(synth, 400, 240, 0.05, 87, zipf=1.1)
+    _vaw_split_case(ssu, O, monkeypatch, tree, table, method, kw, prec, "0.5")
+
+
+@pytest.mark.parametrize("n", [2, 3, 65])
+def test_vaw_shared_node_small_tables(ssu, O, synth, monkeypatch, n):
+    """VAW on the shared-node path at 2, 3 and 65 samples (every node rare at
+    theta 1), both precisions."""
+    tree, table = _case(synth, 120, n, 0.2, 900 + n)
+    for prec in ("fp64", "fp32"):
+        for method in ("weighted_normalized", "unweighted_unnormalized"):
+            _vaw_split_case(ssu, O, monkeypatch, tree, table, method, {}, prec, "1", walk=n > 3)
+
+
+def _vaw_split_case(ssu, O, monkeypatch, tree, table, method, kw, prec, theta, walk=True):
+    """walk: the samples share some rare node (the posting walk has terms)"""
     monkeypatch.setenv("SSU_VAW_SPLIT", theta)
     inp = inputs_from(ssu, tree, table)
     name = method + ("_fp64" if prec == "fp64" else "")
@@ -713,7 +736,7 @@ def _vaw_split_case(ssu, O, monkeypatch, tree, table, method, kw, prec, theta):
     assert abs(p.split_theta() - float(theta)) < 0.01
     p.execute()
     m = p.matrix()
-    assert p.sparse_terms() > 0
+    assert p.sparse_terms() > 0 or not walk
     p.close()
     ref = oracle_matrix(O, tree, table, method, variance_adjust=True, **kw)
     err = rel_err(m, ref)
