@@ -2066,22 +2066,22 @@ __global__ void __launch_bounds__(64) k_sparse_sums(double* __restrict__ ssp,
 // value (1 for presence), cmin the smallest positive count, emin the smallest
 // positive e, tmin the smallest positive total, g = min(emin, tmin):
 //   B_r = l umax (2 / sqrt(cmin emin) + 2 / sqrt(cmin g)).
-// One thread per node.
 // Generalized (GEN): the terms are l s^alpha with s at most the quotients above,
 // so B_r = l ((2 umax / sqrt(cmin emin))^alpha + 2 (umax / sqrt(cmin g))^alpha).
+// One wavefront per node: lane-strided maxima and minima over the postings,
+// then a butterfly.
 template <class IX, class TV, bool UNW, bool GEN>
 __global__ void __launch_bounds__(64) k_vaw_node_bound(double* __restrict__ nb, const IX* __restrict__ pidx,
                                                        const TV* __restrict__ pval, const TV* __restrict__ pcv,
                                                        const uint32_t* __restrict__ poff,
                                                        const double* __restrict__ tot, const double* __restrict__ len,
                                                        uint32_t K, uint32_t slots, double alpha) {
-    const uint32_t r = blockIdx.x * 64 + threadIdx.x;
+    const uint32_t r = blockIdx.x, lane = threadIdx.x;
     if (r >= K) return;
-    double b = 0.0;
+    constexpr double INF = __builtin_huge_val();
+    double umax = 0.0, cmin = INF, emin = INF, tmin = INF;
     if (poff[K] <= slots) {   // (never otherwise, by the plan's bound)
-        constexpr double INF = __builtin_huge_val();
-        double umax = 0.0, cmin = INF, emin = INF, tmin = INF;
-        for (uint32_t q = poff[r]; q < poff[r + 1]; ++q) {
+        for (uint32_t q = poff[r] + lane; q < poff[r + 1]; q += 64) {
             const double t = tot[pidx[q]], c = (double)pcv[q], u = (double)pval[q];
             const double e = t - c > 0.0 ? t - c : 0.0;
             umax = u > umax ? u : umax;
@@ -2089,15 +2089,24 @@ __global__ void __launch_bounds__(64) k_vaw_node_bound(double* __restrict__ nb, 
             if (e > 0.0) emin = e < emin ? e : emin;
             if (t > 0.0) tmin = t < tmin ? t : tmin;
         }
-        if (cmin < INF) {
-            const double g = emin < tmin ? emin : tmin;
-            const double uu = UNW ? 1.0 : umax;
-            if constexpr (GEN)
-                b = len[r] * ((emin < INF ? pow(2.0 * uu / sqrt(cmin * emin), alpha) : 0.0) +
-                              (g < INF ? 2.0 * pow(uu / sqrt(cmin * g), alpha) : 0.0));
-            else
-                b = len[r] * uu * ((emin < INF ? 2.0 / sqrt(cmin * emin) : 0.0) + (g < INF ? 2.0 / sqrt(cmin * g) : 0.0));
-        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        umax = fmax(umax, __shfl_xor(umax, d, 64));
+        cmin = fmin(cmin, __shfl_xor(cmin, d, 64));
+        emin = fmin(emin, __shfl_xor(emin, d, 64));
+        tmin = fmin(tmin, __shfl_xor(tmin, d, 64));
+    }
+    if (lane) return;
+    double b = 0.0;
+    if (cmin < INF) {
+        const double g = emin < tmin ? emin : tmin;
+        const double uu = UNW ? 1.0 : umax;
+        if constexpr (GEN)
+            b = len[r] * ((emin < INF ? pow(2.0 * uu / sqrt(cmin * emin), alpha) : 0.0) +
+                          (g < INF ? 2.0 * pow(uu / sqrt(cmin * g), alpha) : 0.0));
+        else
+            b = len[r] * uu * ((emin < INF ? 2.0 / sqrt(cmin * emin) : 0.0) + (g < INF ? 2.0 / sqrt(cmin * g) : 0.0));
     }
     nb[r] = b;
 }
@@ -2691,7 +2700,7 @@ static void pairs_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs&
 template <class IX, class TV, int VAW>
 static void vaw_rows_t(uint32_t blocks, hipStream_t st, const SparseNodesArgs& a, uint32_t nchunks, uint32_t nkc,
                        uint32_t a0, uint32_t na, uint32_t mode) {
-    hipLaunchKernelGGL((k_vaw_node_bound<IX, TV, VAW == 2, VAW == 3>), dim3((a.K + 63) / 64), dim3(64), 0, st,
+    hipLaunchKernelGGL((k_vaw_node_bound<IX, TV, VAW == 2, VAW == 3>), dim3(a.K), dim3(64), 0, st,
                        a.nbound, (const IX*)a.pidx, (const TV*)a.pval, (const TV*)a.pcv, (const uint32_t*)a.poff,
                        a.tot, a.len, a.K, a.slots, a.alpha);
     hipLaunchKernelGGL(k_row_weights, dim3(a.n), dim3(64), 0, st, a.ssp, a.rows, a.roff, nkc, a.nbound,
