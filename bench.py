@@ -51,8 +51,12 @@ GRAM_TILE = 128             # Gram output tile edge
 
 def bytes_per_posting(name, n):
     """Bytes a walked posting reads: its sample index (16-bit up to 65,536
-    samples, else 32-bit) and, for the weighted methods, its FP32 value."""
-    return (2 if n <= 65536 else 4) + (0 if name.startswith("unweighted") else 4)
+    samples, else 32-bit), its value (FP32, or FP64 for "_fp64" names; none for
+    the Gram path's walk) and, for VAW, its count in the value's type."""
+    vaw = ":vaw" in name
+    val = 8 if "_fp64" in name.split(":")[0] else 4
+    gram_walk = name.startswith("unweighted") and not vaw
+    return (2 if n <= 65536 else 4) + (0 if gram_walk else val) + (val if vaw else 0)
 
 
 def gram_units(n, kernel_terms):
