@@ -1556,6 +1556,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             a.len = P.d_vlen.p;
             a.K = P.n_vdense;
             a.out2 = P.d_out2.p;
+            a.len_nonneg = true;   // (the split needs every length finite and >= 0)
         }
         SSU_HIP(launch_pairs(op, s.fp64, a, P.st));
         SSU_HIP(P.mark(3));

@@ -688,8 +688,11 @@ template <PairOp OP> struct PairTraits {
 };
 
 // Per-node term for one pair.  u, v: embedded values; cu, cv: embedded counts
-// (VAW); m: c_k + c_l sample totals (VAW); l: branch length.
-template <PairOp OP, class T>
+// (VAW); m: c_k + c_l sample totals (VAW); l: branch length.  LPRE (VAW
+// presence terms, and FP64 weighted terms; lengths >= 0): u and v arrive
+// multiplied by l, so l |u - v| = |lu - lv| and l (u + v) = lu + lv
+// (presence: l max(u, v) = max(lu, lv), exact for 0/1 values).
+template <PairOp OP, class T, bool LPRE = false>
 __device__ __forceinline__ void pair_term(T u, T v, T cu, T cv, T m, T l, T alpha, T& num, T& den) {
     if constexpr (OP == PairOp::weighted) {
         // src/unifrac_task_impl.hpp:117-127
@@ -725,7 +728,10 @@ __device__ __forceinline__ void pair_term(T u, T v, T cu, T cv, T m, T l, T alph
         T rv;
         if constexpr (sizeof(T) == 4) rv = ok ? __builtin_amdgcn_rsqf(vv) : 0.0f;
         else rv = ok ? T(1) / dsqrt<T>(vv) : T(0);
-        if constexpr (OP == PairOp::vaw_weighted) {
+        if constexpr (OP == PairOp::vaw_weighted && LPRE) {
+            num = fma(dabs(u - v), rv, num);
+            den = fma(u + v, rv, den);
+        } else if constexpr (OP == PairOp::vaw_weighted) {
             num = fma(dabs(u - v) * l, rv, num);
             den = fma((u + v) * l, rv, den);
         } else if constexpr (OP == PairOp::vaw_generalized) {
@@ -743,6 +749,9 @@ __device__ __forceinline__ void pair_term(T u, T v, T cu, T cv, T m, T l, T alph
                 num = fma(w, d1 / ss, num);
                 den += w;
             }
+        } else if constexpr (LPRE) {   // vaw_unweighted: u, v are l or 0
+            num = fma(dabs(u - v), rv, num);
+            den = fma(u > v ? u : v, rv, den);
         } else {  // vaw_unweighted: u, v are 0/1 presence values
             const T lv = l * rv;
             num = fma(dabs(u - v), lv, num);
@@ -763,7 +772,7 @@ template <class T> struct PairStage { static constexpr int EB = sizeof(T) == 4 ?
 // add_partial (two-accumulator ops): out[i * n + j] and out2[i * n + j], i < j,
 // hold the posting-list shares of the numerator and the denominator, added
 // before the division; each pair is then written by its i < j thread alone.
-template <PairOp OP, class T, int RI, int RJ, int TT>
+template <PairOp OP, class T, int RI, int RJ, int TT, bool LPRE = false>
 __global__ void __launch_bounds__((TT / RI) * (TT / RJ))
 k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __restrict__ rows, size_t ldn,
         const T* __restrict__ len, uint32_t K,
@@ -815,6 +824,7 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
 
     vecT pre[LPT];
     T pre_l = T(0);
+    T pre_lq[LPRE ? LPT : 1];   // LPRE: each load's node length
     // load step k0 into registers: load q -> panel, node e, 16-byte column group x
     auto fetch = [&](uint32_t k0) __attribute__((always_inline)) {
 #pragma unroll
@@ -831,6 +841,7 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
                 v = *reinterpret_cast<const vecT*>(src);
             }
             pre[q] = v;
+            if constexpr (LPRE) pre_lq[q] = k < K ? len[k] : T(0);
         }
         if (t < EB) pre_l = (k0 + t < K) ? len[k0 + t] : T(0);
     };
@@ -846,6 +857,12 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
                 if (panel < 2) {
 #pragma unroll
                     for (int z = 0; z < VEC; ++z) v[z] = v[z] > T(0) ? T(1) : T(0);
+                }
+            }
+            if constexpr (LPRE) {
+                if (panel < 2) {
+#pragma unroll
+                    for (int z = 0; z < VEC; ++z) v[z] *= pre_lq[q];
                 }
             }
             *reinterpret_cast<vecT*>(&sp[buf][panel][e][x]) = v;
@@ -879,7 +896,8 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
 #pragma unroll
                 for (int c = 0; c < RJ; ++c) {
                     if constexpr (TWO)
-                        pair_term<OP, T>(u[r], v[c], cu[r], cv[c], CNT ? mrow[r][c] : T(0), l, alpha, num[r][c], den[r][c]);
+                        pair_term<OP, T, LPRE>(u[r], v[c], cu[r], cv[c], CNT ? mrow[r][c] : T(0), l, alpha, num[r][c],
+                                               den[r][c]);
                     else
                         pair_term<OP, T>(u[r], v[c], cu[r], cv[c], T(0), l, alpha, num[r][c], dummy);
                 }
@@ -2240,13 +2258,13 @@ __global__ void __launch_bounds__(256) k_matrix_to_stripes(double* __restrict__ 
     st[(size_t)blockIdx.y * n + k] = (double)M[(size_t)k * n + l];
 }
 
-template <PairOp OP, class T, int RI, int RJ, int TT = PAIR_TILE>
+template <PairOp OP, class T, int RI, int RJ, int TT = PAIR_TILE, bool LPRE = false>
 hipError_t launch_pairs_t(const PairArgs& a, hipStream_t st) {
     static_assert(PAIR_TILE % TT == 0, "the row stride is padded to PAIR_TILE");
     const uint32_t tiles = (a.n + TT - 1) / TT;
     dim3 grid(tiles, tiles);
     constexpr int NT = (TT / RI) * (TT / RJ);
-    hipLaunchKernelGGL((k_pairs<OP, T, RI, RJ, TT>), grid, dim3(NT), 0, st, (const T*)a.U, (const T*)a.Cn, a.rows, a.ldn,
+    hipLaunchKernelGGL((k_pairs<OP, T, RI, RJ, TT, LPRE>), grid, dim3(NT), 0, st, (const T*)a.U, (const T*)a.Cn, a.rows, a.ldn,
                        (const T*)a.len, a.K, a.n, a.S, a.totals, a.alpha, (int)a.normalized, a.out, (int)a.out_fp64,
                        a.band, a.out2 ? 1 : 0, a.out2);
     return hipGetLastError();
@@ -2260,9 +2278,15 @@ hipError_t launch_pairs_prec(PairOp op, const PairArgs& a, hipStream_t st) {
     case PairOp::weighted: return launch_pairs_t<PairOp::weighted, T, 8, 8>(a, st);
     case PairOp::unweighted_plain: return launch_pairs_t<PairOp::unweighted_plain, T, 4, 8>(a, st);
     case PairOp::generalized: return launch_pairs_t<PairOp::generalized, T, 4, 4, 64>(a, st);
-    case PairOp::vaw_weighted: return launch_pairs_t<PairOp::vaw_weighted, T, 4, 4, 64>(a, st);
+    case PairOp::vaw_weighted:
+        // FP64 only: in FP32, rounding l u and l v separately doubles the worst-case error of
+        // l |u - v| (2^-23 (u + v) l), past what the near-duplicate bound allows
+        return a.len_nonneg && sizeof(T) == 8 ? launch_pairs_t<PairOp::vaw_weighted, T, 4, 4, 64, true>(a, st)
+                                              : launch_pairs_t<PairOp::vaw_weighted, T, 4, 4, 64>(a, st);
     case PairOp::vaw_generalized: return launch_pairs_t<PairOp::vaw_generalized, T, 4, 4, 64>(a, st);
-    case PairOp::vaw_unweighted: return launch_pairs_t<PairOp::vaw_unweighted, T, 4, 4, 64>(a, st);
+    case PairOp::vaw_unweighted:
+        return a.len_nonneg ? launch_pairs_t<PairOp::vaw_unweighted, T, 4, 4, 64, true>(a, st)
+                            : launch_pairs_t<PairOp::vaw_unweighted, T, 4, 4, 64>(a, st);
     }
     return hipErrorInvalidValue;
 }

@@ -183,6 +183,8 @@ struct PairArgs {
     // two-accumulator ops (VAW): with out2, out[i * n + j] and out2[i * n + j] (i < j) hold
     // the posting-list shares of the numerator and denominator, added before the division
     const void* out2 = nullptr;
+    // every length >= 0 (VAW weighted / presence terms stage l u instead of u)
+    bool len_nonneg = false;
 };
 hipError_t launch_pairs(PairOp op, bool fp64, const PairArgs& a, hipStream_t st);
 
