@@ -2067,10 +2067,11 @@ __global__ void __launch_bounds__(64) k_sparse_sums(double* __restrict__ ssp,
 
 // ------------------------------------------------ VAW on the shared-node path
 //
-// VAW terms (src/unifrac_task_impl.hpp:481-494, 897-911, 1817-1829) divide
-// each node's term by sqrt(mi (m - mi)), mi = c_a + c_b, m = T_a + T_b, so a
-// node held by a alone still adds l u_a / sqrt(c_a (T_a - c_a + T_b)), which
-// depends on b's total.  Over the sparse nodes the pair's share is therefore
+// VAW terms (src/unifrac_task_impl.hpp:481-494, 897-911, 1097-1109,
+// 1817-1829) divide each node's term by sqrt(mi (m - mi)), mi = c_a + c_b,
+// m = T_a + T_b, so a node held by a alone still adds
+// l u_a / sqrt(c_a (T_a - c_a + T_b)) (generalized: l (u_a / sqrt(...))^alpha),
+// which depends on b's total.  Over the sparse nodes the pair's share is therefore
 //   H(a, T_b) + H(b, T_a) + sum over the nodes both hold of (both-term - h_a - h_b),
 // H(a, t) = sum over a's sparse nodes of the one-sided term at total t (taken
 // at each distinct total t once), the sum over shared nodes from the posting
