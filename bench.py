@@ -126,12 +126,14 @@ def cpu_baseline(tree, table, methods, budget_s):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     total_time, per = 0.0, {}
     for label, (method, kw) in methods.items():
+        base, fp32 = method.replace("_fp64", ""), not method.endswith("_fp64")
+
         def run(k):
             t0 = time.perf_counter()
             if kind == "reference":
-                O.ref_stripes(otree, otab, method, fp32=True, start=0, stop=k, **kw)
+                O.ref_stripes(otree, otab, base, fp32=fp32, start=0, stop=k, **kw)
             else:
-                O.stripes(otree, otab, method, start=0, stop=k, threads=threads, **kw)
+                O.stripes(otree, otab, base, start=0, stop=k, threads=threads, **kw)
             return time.perf_counter() - t0
         t1 = run(1)
         if t1 * 4 < budget_s / len(methods):
@@ -493,8 +495,8 @@ def main():
         kind, threads, t_full, per = cpu_baseline(tree, table, specs, args.cpu_budget)
         cpu = {"value": sum(pair_nodes.values()) / t_full, "unit": "pair-node updates/s", "cores": threads,
                "kind": kind,
-               "sample": f"stripes [0,1) and [0,3) of both matrices (FP32 reference kernels, OpenMP), "
-                         f"extrapolated linearly to all {(n + 1) // 2} stripes",
+               "sample": f"stripes [0,1) and [0,3) of each matrix (the reference kernels in each method's "
+                         f"precision, OpenMP), extrapolated linearly to all {(n + 1) // 2} stripes",
                "per_method": {k: v for k, v in per.items() if k != "_host"}, **per["_host"]}
 
     out = {
