@@ -297,7 +297,7 @@ bool copy_ranges_to_host(char* host, const char* dev, const std::vector<CopyRang
                          std::string* err, bool prefault) {
     hipStream_t st = (hipStream_t)stream;
     size_t bytes = 0;
-    for (const CopyRange& r : ranges) bytes += r.len;
+    for (const CopyRange& r : ranges) bytes += r.len * r.rows;
     if (!bytes) return true;
     Stage& S = stage();
     std::unique_lock<std::mutex> lock(S.mu);
@@ -306,16 +306,32 @@ bool copy_ranges_to_host(char* host, const char* dev, const std::vector<CopyRang
         for (const CopyRange& r : ranges) {
             for (void* e : r.ready) SSU_HIP(hipEventSynchronize((hipEvent_t)e));
             SSU_HIP(hipStreamSynchronize(st));
-            if (r.len) SSU_HIP(hipMemcpy(host + r.off, dev + r.off, r.len, hipMemcpyDeviceToHost));
+            if (!r.len || !r.rows) continue;
+            if (r.rows > 1)
+                SSU_HIP(hipMemcpy2D(host + r.off, r.pitch, dev + r.off, r.pitch, r.len, r.rows, hipMemcpyDeviceToHost));
+            else
+                SSU_HIP(hipMemcpy(host + r.off, dev + r.off, r.len, hipMemcpyDeviceToHost));
         }
         return true;
     }
-    // chunks of at most STAGE_CHUNK bytes, each within one range
-    struct Piece { size_t off, len; uint32_t range; };
+    // pieces of at most STAGE_CHUNK bytes, each within one range: runs of a 1-D
+    // range, or whole rows of a rectangle (a row wider than a staging buffer is
+    // taken as 1-D runs)
+    struct Piece { size_t off, len, rows, pitch; uint32_t range; };
     std::vector<Piece> pieces;
-    for (uint32_t r = 0; r < ranges.size(); ++r)
-        for (size_t o = 0; o < ranges[r].len; o += STAGE_CHUNK)
-            pieces.push_back({ranges[r].off + o, std::min(STAGE_CHUNK, ranges[r].len - o), r});
+    for (uint32_t r = 0; r < ranges.size(); ++r) {
+        const CopyRange& R = ranges[r];
+        if (!R.len || !R.rows) continue;
+        if (R.rows > 1 && R.len <= STAGE_CHUNK) {
+            const size_t per = STAGE_CHUNK / R.len;
+            for (size_t q = 0; q < R.rows; q += per)
+                pieces.push_back({R.off + q * R.pitch, R.len, std::min(per, R.rows - q), R.pitch, r});
+            continue;
+        }
+        for (size_t row = 0; row < R.rows; ++row)
+            for (size_t o = 0; o < R.len; o += STAGE_CHUNK)
+                pieces.push_back({R.off + row * R.pitch + o, std::min(STAGE_CHUNK, R.len - o), 1, 0, r});
+    }
     static const bool timing = getenv("SSU_TIMING") && atoi(getenv("SSU_TIMING")) != 0;
     const auto t0 = std::chrono::steady_clock::now();
     double wait_ms = 0.0, host_ms = 0.0;
@@ -330,7 +346,9 @@ bool copy_ranges_to_host(char* host, const char* dev, const std::vector<CopyRang
             const hipError_t x = hipStreamWaitEvent(st, (hipEvent_t)e, 0);
             if (x != hipSuccess) return x;
         }
-        hipError_t x = hipMemcpyAsync(S.buf[b], dev + q.off, q.len, hipMemcpyDeviceToHost, st);
+        hipError_t x = q.rows > 1 ? hipMemcpy2DAsync(S.buf[b], q.len, dev + q.off, q.pitch, q.len, q.rows,
+                                                     hipMemcpyDeviceToHost, st)
+                                  : hipMemcpyAsync(S.buf[b], dev + q.off, q.len, hipMemcpyDeviceToHost, st);
         return x != hipSuccess ? x : hipEventRecord(S.done[b], st);
     };
     const size_t np = pieces.size();
@@ -341,8 +359,10 @@ bool copy_ranges_to_host(char* host, const char* dev, const std::vector<CopyRang
     size_t touched = 0, span = 0;
     double touch_ms = 0.0;
     if (prefault && !ranges.empty()) {
-        const size_t lo = ranges.front().off;
-        for (const CopyRange& r : ranges) span = std::max(span, r.off + r.len - lo);
+        size_t lo = ranges.front().off;
+        for (const CopyRange& r : ranges) lo = std::min(lo, r.off);
+        for (const CopyRange& r : ranges)
+            if (r.len && r.rows) span = std::max(span, r.off + (r.rows - 1) * r.pitch + r.len - lo);
         const auto tt = std::chrono::steady_clock::now();
         constexpr size_t TOUCH = size_t(256) << 20, PAGE = 4096;
         CopyPool& pool = CopyPool::get();
@@ -364,7 +384,18 @@ bool copy_ranges_to_host(char* host, const char* dev, const std::vector<CopyRang
         SSU_HIP(hipEventSynchronize(S.done[b]));
         wait_ms += since(tw);
         tw = std::chrono::steady_clock::now();
-        spread_copy(host + pieces[c].off, (const char*)S.buf[b], pieces[c].len, 0);
+        const Piece& q = pieces[c];
+        if (q.rows > 1) {   // the rows, packed in the buffer, to their places
+            CopyPool& pool = CopyPool::get();
+            const size_t parts = pool.parts();
+            const char* src = (const char*)S.buf[b];
+            pool.run([&](unsigned t) {
+                for (size_t r = q.rows * t / parts; r < q.rows * (t + 1) / parts; ++r)
+                    memcpy(host + q.off + r * q.pitch, src + r * q.len, q.len);
+            });
+        } else {
+            spread_copy(host + q.off, (const char*)S.buf[b], q.len, 0);
+        }
         host_ms += since(tw);
         if (c + STAGE_BUFS < np) SSU_HIP(issue(c + STAGE_BUFS));   // into the buffer just emptied
     }
@@ -1452,6 +1483,14 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
                       : launch_unweighted_combine<float>((float*)P.out_ptr(), P.d_G.as<double>(), P.n_pad, P.n, norm,
                                                          P.st, r0, r1);
     };
+    // banded: each band's pairs from its first row's column on, at (i, j) and (j, i) (an
+    // L of final entries per band, for the overlapped copies)
+    auto combine_band = [&](uint32_t r0, uint32_t r1) {
+        return s.fp64 ? launch_unweighted_combine_band<double>((double*)P.out_ptr(), P.d_G.as<double>(), P.n_pad,
+                                                               P.n, norm, P.st, r0, r1)
+                      : launch_unweighted_combine_band<float>((float*)P.out_ptr(), P.d_G.as<double>(), P.n_pad, P.n,
+                                                              norm, P.st, r0, r1);
+    };
     if (P.gram && P.banded_run) {
         // bands in row order: the Gram tiles of the band's tile rows (band 0: every
         // diagonal tile too), then (band 0) the diagonal shares of every row, the
@@ -1466,7 +1505,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
                 if (k == 0) SSU_HIP(launch_sparse_rows(gb, P.st, 0, P.n, ROWS_DIAGONAL));
                 SSU_HIP(launch_sparse_rows(gb, P.st, r0, r1, ROWS_OFF_DIAGONAL));
             }
-            SSU_HIP(combine(r0, r1));
+            SSU_HIP(combine_band(r0, r1));
             SSU_HIP(hipEventRecord(P.band_ev[k], P.st));
         }
         SSU_HIP(P.mark(3));
@@ -1671,11 +1710,23 @@ bool Plan::execute_to_host(void* host, std::string* err) {
     const size_t es = P.spec.fp64 ? sizeof(double) : sizeof(float);
     P.banded_run = true;
     P.after_enqueue = [&P, host, es](std::string* e) {
-        // band k's rows are final once it and the band before it (the other stream) are done
         std::vector<CopyRange> ranges;
+        const size_t n = P.n, pitch = n * es;
         for (size_t k = 0; k < P.band_ev.size(); ++k) {
             const size_t r0 = P.band_row0[k], r1 = P.band_row0[k + 1];
-            CopyRange r{r0 * P.n * es, (r1 - r0) * P.n * es, {(void*)P.band_ev[k]}};
+            if (P.sparse || P.gram) {
+                // band k writes every pair (i, j), i < j, of its rows i and the mirror
+                // (j, i) (the tiled kernel's blocks; the Gram path's banded combine):
+                // once it is done, its rows from column r0 on and the columns [r0, r1)
+                // of the rows below are final, an L whose size follows the band's work
+                // (the earlier columns of its rows came with earlier bands)
+                ranges.push_back(CopyRange{(r0 * n + r0) * es, (n - r0) * es, {(void*)P.band_ev[k]}, r1 - r0, pitch});
+                if (r1 < n)
+                    ranges.push_back(CopyRange{(r1 * n + r0) * es, (r1 - r0) * es, {(void*)P.band_ev[k]}, n - r1, pitch});
+                continue;
+            }
+            // band k's rows are final once it and the band before it are done
+            CopyRange r{r0 * pitch, (r1 - r0) * pitch, {(void*)P.band_ev[k]}};
             if (k > 0) r.ready.push_back((void*)P.band_ev[k - 1]);
             ranges.push_back(r);
         }

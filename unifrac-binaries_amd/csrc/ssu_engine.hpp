@@ -141,9 +141,12 @@ bool copy_to_host(void* host, const void* dev, size_t bytes, void* stream, std::
 // events in `ready`, so later ranges can still be computing when this starts.
 // prefault: while the first copy waits, touch the destination's pages (a
 // freshly allocated result) so that the host copies do not take the faults.
+// Bytes [off, off + len) of the device buffer, or, with rows > 1, a rectangle of
+// `rows` rows of len bytes each, pitch bytes apart (the same layout on both sides)
 struct CopyRange {
     size_t off, len;
     std::vector<void*> ready;   // hipEvent_t
+    size_t rows = 1, pitch = 0;
 };
 bool copy_ranges_to_host(char* host, const char* dev, const std::vector<CopyRange>& ranges, void* stream,
                          std::string* err, bool prefault = false);

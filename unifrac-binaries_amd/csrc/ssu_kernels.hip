@@ -575,6 +575,41 @@ __global__ void __launch_bounds__(256) k_unweighted_combine(TOut* __restrict__ o
     out[(size_t)i * n + j] = (TOut)d;
 }
 
+// The same distances for the pairs of rows [row0, row1) (multiples of 64) with
+// every column from row0 on, each written at (i, j) and (j, i): one workgroup
+// per 64 x 64 upper block (bi <= bj), the block transposed through LDS so that
+// both stores are row-contiguous.  After it, rows [row0, row1) from column
+// row0 on and columns [row0, row1) of the rows below are final.
+template <class TOut>
+__global__ void __launch_bounds__(256) k_unweighted_combine_band(TOut* __restrict__ out, const double* __restrict__ G,
+                                                                 uint32_t tiles, uint32_t n, int normalized,
+                                                                 uint32_t row0, uint32_t row1) {
+    const uint32_t bi = row0 / 64 + blockIdx.y, bj = blockIdx.x;
+    if (bj < bi || bi * 64 >= row1) return;   // (uniform)
+    const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    __shared__ TOut tile[64][65];
+    for (uint32_t r = ty; r < 64; r += 4) {
+        const uint32_t i = bi * 64 + r, j = bj * 64 + tx;
+        TOut v = (TOut)0;
+        if (i < n && j < n && i != j) {
+            const uint32_t a = i < j ? i : j, b = i < j ? j : i;
+            const double g = G[gram_slot(a, b, tiles)];
+            const double ta = G[gram_slot(a, a, tiles)], tb = G[gram_slot(b, b, tiles)];
+            const double x = (ta + tb) - 2.0 * g;
+            const double o = (ta + tb) - g;
+            v = (TOut)(normalized ? x / o : x);
+        }
+        if (i < n && j < n) out[(size_t)i * n + j] = v;
+        tile[r][tx] = v;
+    }
+    if (bi == bj) return;   // (uniform) a diagonal block holds both triangles itself
+    __syncthreads();
+    for (uint32_t c = ty; c < 64; c += 4) {
+        const uint32_t j = bj * 64 + c, i = bi * 64 + tx;
+        if (i < n && j < n) out[(size_t)j * n + i] = tile[tx][c];
+    }
+}
+
 // ------------------------------------------------------------------ VALU pair tiles
 
 // |x| through the IEEE fabs builtins, so the compiler folds it into a source modifier
@@ -2595,6 +2630,21 @@ hipError_t launch_unweighted_combine(TOut* out, const double* G, uint32_t n_pad,
                        G, n_pad / GRAM_BM, n, (int)normalized, row0, row1);
     return hipGetLastError();
 }
+template <class TOut>
+hipError_t launch_unweighted_combine_band(TOut* out, const double* G, uint32_t n_pad, uint32_t n, bool normalized,
+                                          hipStream_t st, uint32_t row0, uint32_t row1) {
+    row1 = std::min(row1, n);
+    if (row0 >= row1) return hipSuccess;
+    if (row0 % 64 || (row1 % 64 && row1 != n) || n_pad < n) return hipErrorInvalidValue;
+    const uint32_t nb = (n + 63) / 64, rb = (row1 - row0 + 63) / 64;
+    hipLaunchKernelGGL((k_unweighted_combine_band<TOut>), dim3(nb, rb), dim3(256), 0, st, out, G, n_pad / GRAM_BM, n,
+                       (int)normalized, row0, row1);
+    return hipGetLastError();
+}
+template hipError_t launch_unweighted_combine_band<float>(float*, const double*, uint32_t, uint32_t, bool,
+                                                          hipStream_t, uint32_t, uint32_t);
+template hipError_t launch_unweighted_combine_band<double>(double*, const double*, uint32_t, uint32_t, bool,
+                                                           hipStream_t, uint32_t, uint32_t);
 template hipError_t launch_unweighted_combine<float>(float*, const double*, uint32_t, uint32_t, bool, hipStream_t,
                                                      uint32_t, uint32_t);
 template hipError_t launch_unweighted_combine<double>(double*, const double*, uint32_t, uint32_t, bool, hipStream_t,

@@ -154,6 +154,13 @@ hipError_t launch_gram_i8(double* G, const int8_t* Pt, size_t ldp, const int8_t*
 template <class TOut>
 hipError_t launch_unweighted_combine(TOut* out, const double* G, uint32_t n_pad, uint32_t n, bool normalized,
                                      hipStream_t st, uint32_t row0 = 0, uint32_t row1 = 0xffffffffu);
+// the same values for the pairs (i, j), i in [row0, row1) (multiples of 64, or row1 = n),
+// j >= row0, written at (i, j) and (j, i): afterwards rows [row0, row1) from column row0 on
+// and columns [row0, row1) of the rows below are final (G's tiles of those rows and its
+// diagonal must be complete)
+template <class TOut>
+hipError_t launch_unweighted_combine_band(TOut* out, const double* G, uint32_t n_pad, uint32_t n, bool normalized,
+                                          hipStream_t st, uint32_t row0, uint32_t row1);
 
 // ---------------------------------------------------------------- VALU pair tiles
 enum class PairOp : int {
