@@ -500,19 +500,21 @@ struct PlanImpl {
     bool band_all = true;
     std::vector<uint32_t> run_order;
     const std::vector<uint32_t>& order_used() const { return band_all ? block_order : run_order; }
-    // Output bands (execute_to_host): the tiled kernel runs as OUT_BANDS launches over
-    // consecutive row-panel ranges (the blocks of block_order in each, in order;
-    // the first two and the last two bands narrower, so that copies start early
-    // and little is left to copy after the last band; SSU_OUT_BANDS=<k> sets the
-    // count, 0 turns bands off);
-    // after band k every row of band k is final (mirror entries come from the
-    // bands above), so its copy to the host overlaps the bands below.  The bands
-    // alternate between st and band_st2, so that one band's last blocks share the
-    // device with the next band's first.  Unweighted: band k is the Gram tiles of
-    // tile rows in the band (band 0 also every diagonal tile), the posting walk of
-    // its rows (after band 0's tiles: the diagonal G(a, a) of every row) and the
-    // combine of its rows, in order on st.
-    static constexpr uint32_t OUT_BANDS = 10;
+    // Output bands (execute_to_host): the tiled kernel runs as OUT_BANDS launches
+    // (the Gram path OUT_BANDS_GRAM) over consecutive row-panel ranges (the blocks
+    // of block_order in each, in order; the first two and the last two bands
+    // narrower, so that copies start early and little is left to copy after the
+    // last band; SSU_OUT_BANDS=<k> sets the count, 0 turns bands off).  Band k
+    // writes every pair of its rows with the columns from its first row on, and
+    // the mirrors, so after it its rows from that column on and those columns of
+    // the rows below are final (an L whose size follows the band's work), and
+    // their copy to the host overlaps the bands below.  The bands alternate
+    // between st and band_st2, so that one band's last blocks share the device
+    // with the next band's first.  Unweighted: band k is the Gram tiles of tile
+    // rows in the band (band 0 also every diagonal tile), the posting walk of its
+    // rows (after band 0's tiles: the diagonal G(a, a) of every row) and the
+    // banded combine of its rows, in order on st.
+    static constexpr uint32_t OUT_BANDS = 10, OUT_BANDS_GRAM = 16;
     std::vector<uint32_t> band_row0, band_off;   // rows [band_row0[k], band_row0[k + 1]); blocks [band_off[k], ..)
     std::vector<hipEvent_t> band_ev;
     hipEvent_t band_go = nullptr;                 // the bands' inputs are ready (band_st2 waits for it)
@@ -1663,7 +1665,10 @@ bool PlanImpl::prepare_bands(std::string* err) {
     if (!band_ev.empty()) return true;
     const uint32_t ei = gram ? GRAM_BM : (rect() ? 32 : 64);
     const uint32_t nbi = (n + ei - 1) / ei;
-    uint32_t want = OUT_BANDS;
+    // the Gram path's bands are cheaper to compute than to copy: more, smaller ones
+    // (16 bands measured 27.0 ms against 28.1 ms for 10 at 16,384 samples; the
+    // tiled weighted kernel 137.9 against 137.6 ms)
+    uint32_t want = gram ? OUT_BANDS_GRAM : OUT_BANDS;
     if (const char* env = getenv("SSU_OUT_BANDS")) want = std::max(1, atoi(env));
     const uint32_t nb = std::min<uint32_t>(want, nbi);
     // row shares 1 : 2 : 3 ... 3 : 2 : 1 (fewer than five bands: equal)
