@@ -4,6 +4,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <thread>
 
 namespace ssu {
@@ -293,6 +294,29 @@ void split_range(uint64_t n, unsigned parts, F fn) {
 unsigned threads_for(uint64_t n) {
     return n < (1u << 16) ? 1u : std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
 }
+
+// Scratch arrays of the postorder passes, handed from call to call (their pages
+// stay mapped: first-touch faults of fresh arrays cost as much as a pass).  The
+// list keeps one set per caller that ran at the same time, 9 bytes per tree node
+// each, until the process ends.
+struct PassScratch {
+    std::vector<uint32_t> kc;
+    std::vector<int32_t> row;
+    std::vector<uint8_t> kept;
+};
+std::mutex g_scratch_mu;
+std::vector<PassScratch> g_scratch;
+struct ScratchLease {
+    PassScratch s;
+    ScratchLease() {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        if (!g_scratch.empty()) { s = std::move(g_scratch.back()); g_scratch.pop_back(); }
+    }
+    ~ScratchLease() {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        g_scratch.push_back(std::move(s));
+    }
+};
 }  // namespace
 
 NameIndex::NameIndex(const char* const* ids, uint32_t n) : ids_(ids) {
@@ -352,9 +376,13 @@ bool synced_postorder(const bool* structure, const double* lengths, const char* 
                       const NameIndex& rows, PostorderNodes& out, uint32_t& matched) {
     // pass 1, closes in postorder: kept nodes (leaves named by a row, and their
     // ancestors) and each node's number of kept children
-    std::vector<uint32_t> kc(n, 0);
-    std::vector<int32_t> row(n, -1);
-    std::vector<uint8_t> kept(n, 0);
+    ScratchLease lease;
+    lease.s.kc.assign(n, 0);
+    lease.s.row.assign(n, -1);
+    lease.s.kept.assign(n, 0);
+    uint32_t* const kc = lease.s.kc.data();
+    int32_t* const row = lease.s.row.data();
+    uint8_t* const kept = lease.s.kept.data();
     std::vector<uint32_t> stack;
     stack.reserve(64);
     std::vector<uint8_t> seen;
