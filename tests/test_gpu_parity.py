@@ -426,6 +426,47 @@ def test_weighted_negative_counts_use_dense_kernel(ssu, O, synth):
     p.close()
 
 
+# 200 tips x 50 samples: the totals pass runs on one thread; 4,000 x 512 at 60%
+# (1,228,800 entries, above 2^20): on several
+CHECK_SIZES = [(200, 50, 0.1), (4000, 512, 0.6)]
+
+
+@pytest.mark.parametrize("tips,n,dens", CHECK_SIZES)
+def test_sample_index_out_of_range_is_an_error(ssu, synth, tips, n, dens):
+    """A CSR sample index at or past n_samples fails the call with an error and no device work."""
+    tree, table = _case(synth, tips, n, dens, 95)
+    obs, sids, ip, ix, d = table
+    assert (len(d) >= 1 << 20) == (tips > 1000)
+    ix = ix.copy()
+    ix[len(ix) // 2] = n
+    inp = inputs_from(ssu, tree, (obs, sids, ip, ix, d))
+    say("index out of range", tips, n)
+    s, m = ssu.matrix_inmem(inp, "unweighted")
+    assert s == ssu.OUTPUT_ERROR and m is None
+    assert "sample index out of range" in ssu.last_error()
+
+
+@pytest.mark.parametrize("tips,n,dens", CHECK_SIZES)
+@pytest.mark.parametrize("bad", [-1.0, float("nan"), float("inf")])
+def test_values_outside_shared_node_domain_use_dense_kernel(ssu, synth, tips, n, dens, bad):
+    """One value that is negative or not finite anywhere in the table keeps the weighted
+    metrics off the shared-node kernels; the same table with that value set to 1 takes them."""
+    tree, table = _case(synth, tips, n, dens, 97)
+    obs, sids, ip, ix, d = table
+    d = d.copy()
+    d[len(d) // 3] = bad
+    inp = inputs_from(ssu, tree, (obs, sids, ip, ix, d))
+    say("values outside domain", tips, n, bad)
+    p = ssu.Plan(inp, "weighted_unnormalized")
+    assert p.main_kernel() == "k_pairs"
+    p.close()
+    d[len(d) // 3] = 1.0
+    inp = inputs_from(ssu, tree, (obs, sids, ip, ix, d))
+    p = ssu.Plan(inp, "weighted_unnormalized")
+    assert p.main_kernel() != "k_pairs"
+    p.close()
+
+
 def test_file_entry_points(ssu, O, ref_trees, ref_tables, goldens, tmp_path):
     L = ssu.lib()
     tre = tmp_path / "test.tre"

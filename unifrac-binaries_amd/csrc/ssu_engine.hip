@@ -213,24 +213,6 @@ bool stage_ready(Stage& S) {
     return true;
 }
 
-// threads for a host pass over n elements (one below a million elements)
-size_t host_threads(uint64_t n) {
-    return n < (1u << 20) ? 1 : std::max<size_t>(1, std::min<unsigned>(8, std::thread::hardware_concurrency()));
-}
-
-// fn(t, begin, end) over `parts` consecutive ranges of [0, n), range 0 on the calling thread
-template <class F>
-void parallel_chunks(uint64_t n, size_t parts, F fn) {
-    const uint64_t step = (n + parts - 1) / std::max<size_t>(parts, 1);
-    std::vector<std::thread> pool;
-    for (size_t t = 1; t < parts; ++t) {
-        const uint64_t a = std::min(n, step * t), b = std::min(n, a + step);
-        pool.emplace_back([=] { fn(t, a, b); });
-    }
-    fn(0, 0, std::min(n, step));
-    for (auto& th : pool) th.join();
-}
-
 // Host work over a persistent pool of worker threads (started on first use, up
 // to 16 with the caller): fn(part) for part = 0 .. parts() - 1, the calling
 // thread taking the last part and waiting for the rest.
@@ -1017,18 +999,9 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         if (table.indptr[i + 1] < table.indptr[i]) { if (err) *err = "table indptr is not monotone"; return nullptr; }
     const uint64_t nz_all = table.nnz();
     const uint32_t nz_base = table.n_obs ? table.indptr[0] : 0;
-    {
-        const uint32_t* ix = table.indices + nz_base;
-        std::vector<uint32_t> mx(host_threads(nz_all), 0u);
-        parallel_chunks(nz_all, mx.size(), [&](size_t t, uint64_t a, uint64_t b) {
-            uint32_t m = 0;   // a max reduction the compiler vectorises
-            for (uint64_t j = a; j < b; ++j) m = std::max(m, ix[j]);
-            mx[t] = m;
-        });
-        if (nz_all && *std::max_element(mx.begin(), mx.end()) >= table.n_samples) {
-            if (err) *err = "table sample index out of range";
-            return nullptr;
-        }
+    if (!table.index_ok) {   // (checked by the pass that formed the table's totals)
+        if (err) *err = "table sample index out of range";
+        return nullptr;
     }
     for (uint32_t k = 0; k < nodes.m; ++k)
         if (nodes.obs_row[k] >= (int32_t)table.n_obs || nodes.parent[k] < 0 || (uint32_t)nodes.parent[k] > nodes.m) {
@@ -1135,19 +1108,9 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
         for (size_t q = 0; q < P.knodes.size() && P.gram; ++q)
             P.gram = std::isfinite(spec.fp64 ? P.klen_d[q] : (double)P.klen_f[q]);
     P.n_pad = round_up(std::max<uint32_t>(P.n, 1), GRAM_BM);
-    // every stored value finite and >= 0 (one pass, for the presence bits and the shared-node kernels)
-    const bool weighted_m = spec.method == Method::weighted_normalized || spec.method == Method::weighted_unnormalized;
-    bool vals_nonneg = true;
-    if (P.gram || weighted_m || spec.method == Method::generalized || spec.vaw) {
-        const double* d = table.data + nz_base;
-        std::vector<uint8_t> bad(host_threads(nz_all), 0);
-        parallel_chunks(nz_all, bad.size(), [&](size_t t, uint64_t a, uint64_t b) {
-            bool x = false;   // a branch-free reduction the compiler vectorises
-            for (uint64_t q = a; q < b; ++q) x |= !(d[q] >= 0.0 && d[q] <= 1.7976931348623157e308);
-            bad[t] = x;
-        });
-        for (uint8_t x : bad) vals_nonneg = vals_nonneg && !x;
-    }
+    // every stored value finite and >= 0 (for the presence bits and the shared-node
+    // kernels), found by the pass that formed the table's totals
+    const bool vals_nonneg = table.values_nonneg;
     step("value checks");
     // Presence bits for the Gram path: with every value finite and >= 0 (and
     // positive totals), E[k][s] > 0 exactly when a stored value > 0 lies below

@@ -7,15 +7,35 @@
 namespace ssu {
 
 namespace {
+// What one read of the entries establishes besides the totals
+struct EntryScan {
+    bool index_ok = true;   // every sample index < n_samples
+    bool nonneg = true;     // every value finite and >= 0
+};
+
 // Sample totals over observation ranges on several threads, when that gives the
 // sequential sum bit for bit: every value an integer and the sum of all their
 // magnitudes below 2^53, so every partial sum, in any order, is an integer
-// below 2^53 and each FP64 add is exact (count tables).  false: not so.
+// below 2^53 and each FP64 add is exact (count tables).  The same read checks
+// the sample indices and the values.  false: the totals are not so (the checks
+// are filled either way).
 bool integer_totals(const uint32_t* indptr, const uint32_t* indices, const double* data, uint32_t n_obs,
-                    uint32_t n_samples, double* tot) {
+                    uint32_t n_samples, double* tot, EntryScan& scan) {
     const uint64_t nnz = n_obs ? (uint64_t)indptr[n_obs] - indptr[0] : 0;
     const unsigned threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-    if (nnz < (1u << 20) || threads < 2) return false;
+    if (nnz < (1u << 20) || threads < 2) {
+        // in observation order on this thread (the reference's order)
+        std::fill(tot, tot + n_samples, 0.0);
+        for (uint32_t i = 0; i < n_obs; ++i)
+            for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j) {
+                const uint32_t s = indices[j];
+                const double v = data[j];
+                scan.nonneg &= v >= 0.0 && v <= 1.7976931348623157e308;
+                if (s < n_samples) tot[s] += v;
+                else scan.index_ok = false;
+            }
+        return true;
+    }
     // observation ranges of about equal entry counts
     std::vector<uint32_t> cut(threads + 1, n_obs);
     cut[0] = 0;
@@ -25,33 +45,41 @@ bool integer_totals(const uint32_t* indptr, const uint32_t* indices, const doubl
         cut[t] = i;
     }
     std::vector<std::vector<double>> part(threads);
-    std::vector<uint8_t> integral(threads, 1);
+    std::vector<uint8_t> integral(threads, 1), index_ok(threads, 1), nonneg(threads, 1);
     std::vector<double> mag(threads, 0.0);
     std::vector<std::thread> pool;
     auto run = [&](unsigned t) {
         std::vector<double>& p = part[t];
-        p.assign(n_samples, 0.0);
-        bool ok = true;
+        p.assign((size_t)n_samples + 1, 0.0);   // slot n_samples takes out-of-range indices
+        bool ok = true, ix = true, nn = true;
         double a = 0.0;   // sum of magnitudes (exact while below 2^53)
         for (uint32_t i = cut[t]; i < cut[t + 1]; ++i)
             for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j) {
+                const uint32_t s = indices[j];
                 const double v = data[j];
                 ok &= v == std::nearbyint(v) && std::fabs(v) < 0x1p52;
+                nn &= v >= 0.0 && v <= 1.7976931348623157e308;
+                ix &= s < n_samples;
                 a += std::fabs(v);
-                p[indices[j]] += v;
+                p[s < n_samples ? s : n_samples] += v;
             }
         integral[t] = ok;
+        index_ok[t] = ix;
+        nonneg[t] = nn;
         mag[t] = a;
     };
     for (unsigned t = 1; t < threads; ++t) pool.emplace_back(run, t);
     run(0);
     for (auto& th : pool) th.join();
     double a = 0.0;
+    bool whole = true;
     for (unsigned t = 0; t < threads; ++t) {
-        if (!integral[t] || !(mag[t] < 0x1p52)) return false;
+        scan.index_ok &= index_ok[t] != 0;
+        scan.nonneg &= nonneg[t] != 0;
+        whole = whole && integral[t] && mag[t] < 0x1p52;
         a += mag[t];
     }
-    if (!(a < 0x1p52)) return false;
+    if (!whole || !(a < 0x1p52)) return false;
     for (uint32_t s = 0; s < n_samples; ++s) {
         double x = 0.0;
         for (unsigned t = 0; t < threads; ++t) x += part[t][s];
@@ -66,11 +94,15 @@ void Table::finish(bool index_obs) {
     // integer counts any order gives the same sums)
     totals.assign(n_samples, 0.0);
     double* tot = totals.data();
-    if (!integer_totals(indptr, indices, data, n_obs, n_samples, tot)) {
+    EntryScan scan;
+    if (!integer_totals(indptr, indices, data, n_obs, n_samples, tot, scan)) {
         std::fill(totals.begin(), totals.end(), 0.0);
         for (uint32_t i = 0; i < n_obs; ++i)
-            for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j) tot[indices[j]] += data[j];
+            for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j)
+                if (indices[j] < n_samples) tot[indices[j]] += data[j];
     }
+    index_ok = scan.index_ok;
+    values_nonneg = scan.nonneg;
     obs_index.clear();
     if (!index_obs) return;
     obs_index.reserve(n_obs * 2 + 1);
@@ -139,7 +171,7 @@ Table Table::filtered(double min_total) const {
         for (uint32_t i = 0; i < n_obs; ++i) {
             const size_t before = ix.size();
             for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j) {
-                const int64_t ni = new_index[indices[j]];
+                const int64_t ni = indices[j] < n_samples ? new_index[indices[j]] : -1;
                 if (ni >= 0) { ix.push_back((uint32_t)ni); d.push_back(data[j]); }
             }
             if (ix.size() > before) { oids.push_back(obs_ids[i]); ip.push_back((uint32_t)ix.size()); }

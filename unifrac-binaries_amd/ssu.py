@@ -208,6 +208,8 @@ def matrix_inmem(inp: Inputs, method: str, variance_adjust=False, alpha=1.0, byp
                                             C.c_bool(normalize), C.c_uint(1), C.c_uint(subsample_depth),
                                             C.c_bool(with_replacement), md, C.byref(res))
         if st != OKAY:
+            if res:   # allocated before the call failed
+                L.destroy_mat_full_fp32(C.byref(res))
             return st, None
         n = res.contents.n_samples
         m = np.ctypeslib.as_array(res.contents.matrix, shape=(n * n,)).reshape(n, n).copy()
@@ -219,6 +221,8 @@ def matrix_inmem(inp: Inputs, method: str, variance_adjust=False, alpha=1.0, byp
                                        C.c_bool(normalize), C.c_uint(1), C.c_uint(subsample_depth),
                                        C.c_bool(with_replacement), md, C.byref(res))
         if st != OKAY:
+            if res:   # allocated before the call failed
+                L.destroy_mat_full_fp64(C.byref(res))
             return st, None
         n = res.contents.n_samples
         m = np.ctypeslib.as_array(res.contents.matrix, shape=(n * n,)).reshape(n, n).copy()
