@@ -238,7 +238,8 @@ def test_reads_files_written_like_the_reference(ssu, tmp_path):
     ssu.lib().destroy_partial_mat(C.byref(obs))
 
 
-def _merge(ssu, tmp_path, ids, stripes, splits, fp32=False, mmap_dir=None):
+def _merge(ssu, tmp_path, ids, stripes, splits, fp32=False, mmap_dir=None, preload=()):
+    """preload: (partial, stripe) pairs read with read_partial_one_stripe before the merge."""
     L = ssu.lib()
     total = len(stripes)
     mats = []
@@ -251,6 +252,8 @@ def _merge(ssu, tmp_path, ids, stripes, splits, fp32=False, mmap_dir=None):
         mats.append(dyn)
     arr = (C.POINTER(ssu.PartialDynMat) * len(mats))(*mats)
     assert L.validate_partial(arr, len(mats)) == ssu.MERGE_OKAY
+    for p, q in preload:
+        assert L.read_partial_one_stripe(mats[p], q) == ssu.READ_OKAY
     n = len(ids)
     md = None if mmap_dir is None else str(mmap_dir).encode()
     if fp32:
@@ -302,6 +305,55 @@ def test_merge_of_random_matrix_stripes(ssu, O, tmp_path):
         for splits in ([0, total], [0, 1, total], [0, total // 2, total - 1, total]):
             splits = sorted(set(splits))
             assert np.array_equal(_merge(ssu, tmp_path, ids, stripes, splits), full)
+
+
+@pytest.mark.parametrize("n", [3000, 3001])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_merge_on_several_threads(ssu, tmp_path, n, fp32):
+    """1,500 stripes in 24 blocks of 64, placed by several threads, with stripes a caller
+    loaded beforehand (first, middle and last of a partial), into memory and into an mmap'd file."""
+    rng = np.random.default_rng(n)
+    a = rng.random((n, n))
+    full = a + a.T
+    np.fill_diagonal(full, 0)
+    total = (n + 1) // 2
+    k = np.arange(n)
+    stripes = [full[k, (k + s + 1) % n] for s in range(total)]
+    ids = [f"x{i}" for i in range(n)]
+    splits = [0, 130, 700, 701, total]
+    pre = [(0, 0), (1, 63), (1, 64), (3, total - 702)]
+    exp = full.astype(np.float32) if fp32 else full
+    assert np.array_equal(_merge(ssu, tmp_path, ids, stripes, splits, fp32, preload=pre), exp)
+    mm = tmp_path / "mm"
+    mm.mkdir()
+    assert np.array_equal(_merge(ssu, tmp_path, ids, stripes, splits, fp32, mmap_dir=mm), exp)
+
+
+def test_merge_of_a_file_cut_after_its_header(ssu, tmp_path):
+    """A partial cut short after read_partial_header: the merge reports an incomplete
+    stripe set and hands back no result."""
+    L = ssu.lib()
+    n = 1000
+    rng = np.random.default_rng(11)
+    total = (n + 1) // 2
+    stripes = [rng.random(n) for _ in range(total)]
+    ids = [f"x{i}" for i in range(n)]
+    mats = []
+    for a, b in ((0, 200), (200, total)):
+        pm, keep = make_partial(ssu, ids, stripes[a:b], a, total)
+        path = tmp_path / f"p{a}.partial"
+        assert L.write_partial(str(path).encode(), C.byref(pm)) == ssu.WRITE_OKAY
+        st, dyn = read_dyn(ssu, path)
+        assert st == ssu.READ_OKAY
+        mats.append(dyn)
+    last = tmp_path / "p200.partial"
+    last.write_bytes(last.read_bytes()[: last.stat().st_size * 3 // 5])
+    arr = (C.POINTER(ssu.PartialDynMat) * 2)(*mats)
+    res = C.POINTER(ssu.MatFullFP64)()
+    assert L.merge_partial_to_mmap_matrix(arr, 2, None, C.byref(res)) == ssu.INCOMPLETE_STRIPE_SET
+    assert not res
+    for d in mats:
+        L.destroy_partial_dyn_mat(C.byref(d))
 
 
 def test_validate_and_merge_error_codes(ssu, tmp_path):

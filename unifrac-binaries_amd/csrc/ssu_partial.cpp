@@ -11,10 +11,12 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ssu.h"
@@ -184,46 +186,115 @@ MergeStatus check_partials(const TPMat* const* mats, int n_partials, bool verbos
 // entry (i, j), d = j - i, takes stripe d-1 at i when d-1 < n_stripes and
 // stripe n-d-1 at j otherwise (src/unifrac.cpp:228-340).  So a wrapped
 // position (k > l) is used only when its pair has no unwrapped copy.
+//
+// Stripes are decompressed and placed in blocks of SB on several threads: each
+// stripe holds its own pairs, so the blocks write disjoint entries.  Each
+// partial's stripe offsets are learned first from the size words (pread, one
+// thread per partial); stripes a caller already loaded are used as they are,
+// and every loaded stripe is freed once placed, as the one-thread merge did.
 template <class TReal>
 bool stripes_into_matrix(partial_dyn_mat_t** mats, int n_partials, uint32_t n, uint32_t total, TReal* M) {
     for (uint64_t i = 0; i < n; ++i) M[i * n + i] = 0;
     std::vector<int> owner(total, -1);
     for (int p = 0; p < n_partials; ++p)
         for (uint32_t s = mats[p]->stripe_start; s < mats[p]->stripe_stop; ++s) owner[s] = p;
-    constexpr uint32_t SB = 64;   // stripes resident at once
-    constexpr uint32_t KB = 64;   // samples per tile
-    std::vector<const double*> rows(SB);
-    for (uint32_t s0 = 0; s0 < total; s0 += SB) {
-        const uint32_t s1 = std::min(total, s0 + SB);
-        for (uint32_t s = s0; s < s1; ++s) {
-            partial_dyn_mat_t* pm = mats[owner[s]];
-            const uint32_t idx = s - pm->stripe_start;
-            if (!pm->stripes[idx] && read_partial_one_stripe(pm, idx) != read_okay) return false;
-            rows[s - s0] = pm->stripes[idx];
+    std::vector<int> fds(n_partials, -1);
+    struct CloseAll {
+        std::vector<int>& f;
+        ~CloseAll() {
+            for (int x : f)
+                if (x >= 0) close(x);
         }
-        for (uint32_t k0 = 0; k0 < n; k0 += KB) {
-            const uint32_t k1 = std::min(n, k0 + KB);
+    } close_all{fds};
+    for (int p = 0; p < n_partials; ++p)
+        if ((fds[p] = open(mats[p]->filename, O_RDONLY)) < 0) return false;
+    // offsets of every stripe record, walking forward from the first
+    std::vector<uint8_t> offsets_ok(n_partials, 1);
+    {
+        std::vector<std::thread> th;
+        for (int p = 0; p < n_partials; ++p)
+            th.emplace_back([&, p] {
+                partial_dyn_mat_t* pm = mats[p];
+                const uint32_t ns = pm->stripe_stop - pm->stripe_start;
+                for (uint32_t c = 0; c + 1 < ns; ++c) {
+                    if (pm->offsets[c + 1]) continue;
+                    uint32_t size = 0;
+                    if (pread(fds[p], &size, sizeof(size), (off_t)pm->offsets[c]) != (ssize_t)sizeof(size)) {
+                        offsets_ok[p] = 0;
+                        return;
+                    }
+                    pm->offsets[c + 1] = pm->offsets[c] + sizeof(uint32_t) + size;
+                }
+            });
+        for (auto& t : th) t.join();
+    }
+    for (uint8_t ok : offsets_ok)
+        if (!ok) return false;
+    constexpr uint32_t SB = 64;   // stripes placed together
+    constexpr uint32_t KB = 64;   // samples per tile
+    const size_t raw = sizeof(double) * (size_t)n;
+    const uint32_t blocks = (total + SB - 1) / SB;
+    const unsigned threads =
+        std::max(1u, std::min({16u, std::thread::hardware_concurrency(), blocks}));
+    std::atomic<uint32_t> next{0};
+    std::atomic<bool> failed{false};
+    auto work = [&] {
+        std::vector<double> buf((size_t)SB * n);
+        std::vector<uint8_t> cbuf;
+        std::vector<const double*> rows(SB);
+        for (uint32_t blk; !failed.load(std::memory_order_relaxed) && (blk = next.fetch_add(1)) < blocks;) {
+            const uint32_t s0 = blk * SB, s1 = std::min(total, s0 + SB);
             for (uint32_t s = s0; s < s1; ++s) {
-                const double* v = rows[s - s0];
-                const uint32_t d = s + 1;
-                const bool wrapped_dup = n - d - 1 < total;   // the pair's distance n-d is also stored unwrapped
-                for (uint32_t k = k0; k < k1; ++k) {
-                    const uint32_t l = (uint32_t)(((uint64_t)k + d) % n);
-                    if (k > l && wrapped_dup) continue;
-                    const TReal x = (TReal)v[k];
-                    M[(uint64_t)k * n + l] = x;
-                    M[(uint64_t)l * n + k] = x;
+                partial_dyn_mat_t* pm = mats[owner[s]];
+                const uint32_t idx = s - pm->stripe_start;
+                if (pm->stripes[idx]) {
+                    rows[s - s0] = pm->stripes[idx];
+                    continue;
+                }
+                const int fd = fds[owner[s]];
+                uint32_t size = 0;
+                double* v = buf.data() + (size_t)(s - s0) * n;
+                if (pread(fd, &size, sizeof(size), (off_t)pm->offsets[idx]) != (ssize_t)sizeof(size) || size == 0 ||
+                    size > ssu::lz4_bound(raw)) {
+                    failed = true;
+                    return;
+                }
+                cbuf.resize(size);
+                if (pread(fd, cbuf.data(), size, (off_t)(pm->offsets[idx] + sizeof(uint32_t))) != (ssize_t)size ||
+                    ssu::lz4_decompress(cbuf.data(), size, (uint8_t*)v, raw) != (long)raw) {
+                    failed = true;
+                    return;
+                }
+                rows[s - s0] = v;
+            }
+            for (uint32_t k0 = 0; k0 < n; k0 += KB) {
+                const uint32_t k1 = std::min(n, k0 + KB);
+                for (uint32_t s = s0; s < s1; ++s) {
+                    const double* v = rows[s - s0];
+                    const uint32_t d = s + 1;
+                    const bool wrapped_dup = n - d - 1 < total;   // the pair's distance n-d is also stored unwrapped
+                    for (uint32_t k = k0; k < k1; ++k) {
+                        const uint32_t l = (uint32_t)(((uint64_t)k + d) % n);
+                        if (k > l && wrapped_dup) continue;
+                        const TReal x = (TReal)v[k];
+                        M[(uint64_t)k * n + l] = x;
+                        M[(uint64_t)l * n + k] = x;
+                    }
                 }
             }
+            for (uint32_t s = s0; s < s1; ++s) {
+                partial_dyn_mat_t* pm = mats[owner[s]];
+                const uint32_t idx = s - pm->stripe_start;
+                free(pm->stripes[idx]);
+                pm->stripes[idx] = nullptr;
+            }
         }
-        for (uint32_t s = s0; s < s1; ++s) {
-            partial_dyn_mat_t* pm = mats[owner[s]];
-            const uint32_t idx = s - pm->stripe_start;
-            free(pm->stripes[idx]);
-            pm->stripes[idx] = nullptr;
-        }
-    }
-    return true;
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    return !failed;
 }
 
 // src/api.cpp:2448-2475 (merge_partial_to_matrix_T)
