@@ -166,6 +166,25 @@ def test_postorder_large_threaded_matches_oracle(ssu, O, synth):
     assert np.array_equal(mine[3], want) and (want >= len(obs)).sum() > 0
 
 
+def test_postorder_concurrent_calls(ssu, synth):
+    """Eight threads walking different trees at once (each call takes its own set of the
+    passes' reused scratch arrays) give what each gives alone."""
+    from concurrent.futures import ThreadPoolExecutor
+    cases = []
+    for seed in range(8):
+        st, le, nm = synth.random_tree(3000 + 4000 * seed, 40 + seed)
+        tips = [x for x in nm if x]
+        rng = np.random.default_rng(seed)
+        obs = list(rng.choice(tips, len(tips) // (seed + 2), replace=False))
+        cases.append((st, le, nm, obs))
+    alone = [ssu.postorder(*c) for c in cases]
+    with ThreadPoolExecutor(8) as ex:
+        for rep in range(3):
+            together = list(ex.map(lambda c: ssu.postorder(*c), cases))
+            for a, b in zip(alone, together):
+                assert all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
 @pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libssu_ref.so")),
                     reason="reference build not present")
 @pytest.mark.parametrize("seed,tips", [(5, 300), (6, 3000)])
