@@ -241,6 +241,98 @@ __global__ void __launch_bounds__(EMB_THREADS) k_gather_sums(float* __restrict__
     if constexpr (SUMS) Spart[(size_t)blockIdx.y * lde + s] = acc;
 }
 
+// bits 0..15 of x to the even bits 0, 2, ..., 30
+__device__ __forceinline__ uint32_t spread16(uint32_t x) {
+    x &= 0xffffu;
+    x = (x | (x << 8)) & 0x00ff00ffu;
+    x = (x | (x << 4)) & 0x0f0f0f0fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    return (x | (x << 1)) & 0x55555555u;
+}
+
+// k_gather_sums with two samples per thread: 16-byte loads of E (8-byte U
+// stores), so a wave covers 128 samples, four 32-sample panels; the outputs,
+// and the order of every sum, are those of k_gather_sums.  The samples' words
+// (smask) of 32 rows are collected in two registers, lane j (j < 32) holding
+// the word of the group's row j for panel 0 (2), lane 32 + j that for panel 1
+// (3), and stored as one coalesced store per panel pair, not a one-lane store
+// per row and panel.
+// Needs lde, col0 and ldu even; with masks, lde and col0 multiples of 64, so a
+// wave that runs past lde does so by exactly its upper two panels: all its
+// lanes stay (for the stores), those past lde neither load nor store samples.
+template <bool GATHER, bool SUMS, bool MASKS>
+__global__ void __launch_bounds__(EMB_THREADS) k_gather_sums2(float* __restrict__ U, size_t ldu, uint32_t col0,
+                                                              double* __restrict__ Spart,
+                                                              uint32_t* __restrict__ masks, uint32_t Kw,
+                                                              uint32_t* __restrict__ smask,
+                                                              const double* __restrict__ E, size_t lde,
+                                                              const int32_t* __restrict__ knodes,
+                                                              const double* __restrict__ klen, uint32_t K,
+                                                              uint32_t rows_per_seg, uint32_t n_valid,
+                                                              const uint32_t* __restrict__ skip_u) {
+    const size_t s = 2 * ((size_t)blockIdx.x * EMB_THREADS + threadIdx.x);   // samples s, s + 1
+    const size_t wave0 = s & ~(size_t)127;            // the wave's first sample
+    if (wave0 >= lde) return;                         // (whole waves only)
+    const bool in = s < lde;
+    const bool real0 = s < n_valid, real1 = s + 1 < n_valid;   // (n_valid <= lde)
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t k0 = blockIdx.y * rows_per_seg;
+    const uint32_t k1 = min(K, k0 + rows_per_seg);
+    const size_t panel = (col0 + wave0) / 32;         // the wave's first 32-sample panel
+    const uint32_t npan = lde - wave0 >= 128 ? 4u : 2u;   // panels of the wave inside [0, lde)
+    double acc0 = 0.0, acc1 = 0.0;
+    uint32_t pb[4] = {0u, 0u, 0u, 0u};
+    uint32_t w01 = 0u, w23 = 0u;
+#pragma unroll 8
+    for (uint32_t r = k0; r < k1; ++r) {
+        double e0 = 0.0, e1 = 0.0;
+        if (in) {
+            const double2 e2 = *reinterpret_cast<const double2*>(E + (size_t)knodes[r] * lde + s);
+            e0 = e2.x;
+            e1 = e2.y;
+        }
+        if constexpr (GATHER) {
+            const float f0 = (float)e0, f1 = (float)e1;
+            if (in && (!skip_u || !((skip_u[r >> 5] >> (r & 31)) & 1u)))
+                *reinterpret_cast<float2*>(U + (size_t)r * ldu + col0 + s) = make_float2(f0, f1);
+            e0 = (double)f0;
+            e1 = (double)f1;
+        }
+        if constexpr (SUMS) {
+            acc0 += klen[r] * e0;
+            acc1 += klen[r] * e1;
+        }
+        if constexpr (MASKS) {
+            // lane l holds samples wave0 + 2l (b0) and wave0 + 2l + 1 (b1): panel q is lanes 16q .. 16q + 15
+            const uint64_t b0 = __ballot(real0 && e0 > 0.0);
+            const uint64_t b1 = __ballot(real1 && e1 > 0.0);
+            const uint64_t bo = b0 | b1;
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q)
+                pb[q] |= (((bo >> (16 * q)) & 0xffffu) != 0u ? 1u : 0u) << (r & 31);
+            if (smask && (lane & 31) == (r & 31)) {   // row r's words go to lanes r % 32 and 32 + r % 32
+                const uint32_t sh = 16 * (lane >> 5);
+                w01 = spread16((uint32_t)(b0 >> sh)) | (spread16((uint32_t)(b1 >> sh)) << 1);
+                w23 = spread16((uint32_t)(b0 >> (32 + sh))) | (spread16((uint32_t)(b1 >> (32 + sh))) << 1);
+            }
+            if ((r & 31) == 31 || r + 1 == k1) {
+                const uint32_t rc = r & ~31u, rows = (r & 31) + 1;   // this group's first row, its rows
+                if (smask && (lane & 31) < rows) {
+                    const uint32_t q = lane >> 5;
+                    smask[(panel + q) * K + rc + (lane & 31)] = w01;
+                    if (npan == 4) smask[(panel + 2 + q) * K + rc + (lane & 31)] = w23;
+                }
+                if (lane == 0)
+                    for (uint32_t q = 0; q < npan; ++q) masks[(panel + q) * Kw + (r >> 5)] = pb[q];
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) pb[q] = 0u;
+            }
+        }
+    }
+    if constexpr (SUMS)
+        if (in) *reinterpret_cast<double2*>(Spart + (size_t)blockIdx.y * lde + s) = make_double2(acc0, acc1);
+}
+
 __global__ void __launch_bounds__(EMB_THREADS) k_sum_segments(double* __restrict__ S, const double* __restrict__ Spart,
                                                               size_t ldn, uint32_t segs, uint32_t n) {
     const uint32_t s = blockIdx.x * EMB_THREADS + threadIdx.x;
@@ -2552,6 +2644,14 @@ static void gather_sums_t(dim3 grid, hipStream_t st, float* U, size_t ldu, uint3
                           uint32_t* masks, uint32_t Kw, uint32_t* smask, const double* E, size_t lde,
                           const int32_t* knodes, const double* klen, uint32_t K, uint32_t rps, uint32_t n_valid,
                           const uint32_t* skip_u) {
+    // two samples per thread where the layout allows 16-byte loads (SSU_GATHER2=0: one)
+    static const bool two_ok = !(getenv("SSU_GATHER2") && atoi(getenv("SSU_GATHER2")) == 0);
+    if (two_ok && lde % 2 == 0 && col0 % 2 == 0 && ldu % 2 == 0) {
+        const dim3 g2(grid1(lde / 2, EMB_THREADS), grid.y);
+        hipLaunchKernelGGL((k_gather_sums2<G, S, M>), g2, dim3(EMB_THREADS), 0, st, U, ldu, col0, Spart, masks, Kw,
+                           smask, E, lde, knodes, klen, K, rps, n_valid, skip_u);
+        return;
+    }
     hipLaunchKernelGGL((k_gather_sums<G, S, M>), grid, dim3(EMB_THREADS), 0, st, U, ldu, col0, Spart, masks, Kw,
                        smask, E, lde, knodes, klen, K, rps, n_valid, skip_u);
 }
