@@ -269,9 +269,14 @@ ComputeStatus matrix_inmem(const support_biom_t* td, const support_bptree_t* tr,
         const auto t1 = Clock::now();
         ssu::NameIndex idx(td->obs_ids, (uint32_t)td->n_obs);
         distinct = idx.distinct();
+        const double idx_ms = ms_since(t1);
         tree_ok = tr->n_parens > 0 && ssu::synced_postorder(tr->structure, tr->lengths, tr->names,
                                                             (uint32_t)tr->n_parens, idx, nodes, matched);
         tree_ms = ms_since(t1);
+        static const bool timing = getenv("SSU_TIMING") && atoi(getenv("SSU_TIMING")) != 0;
+        if (timing)   // SSU_TIMING=1: the tree thread's two steps on stderr
+            fprintf(stderr, "[ssu tree] name index %7.2f ms, shear and postorder %7.2f ms\n", idx_ms,
+                    tree_ms - idx_ms);
     });
     Table t = Table::borrow(td->obs_ids, td->sample_ids, td->indptr, td->indices, td->data, (uint32_t)td->n_obs,
                             (uint32_t)td->n_samples, false);
