@@ -766,6 +766,56 @@ def test_vaw_shared_node_small_tables(ssu, O, synth, monkeypatch, n):
             _vaw_split_case(ssu, O, monkeypatch, tree, table, method, {}, prec, "1", walk=n > 3)
 
 
+@pytest.mark.parametrize("alpha", [-0.5, 1.5, 2.5])
+def test_generalized_fp32_alpha_outside_unit_interval(ssu, O, synth, alpha):
+    """alpha outside [0, 1] runs the dense kernel, whose FP32 powers keep the exponent of
+    u + v exact: exp2(alpha log2 s) alone is off by |alpha log2 s| ulp, up to 2.4e-6 for s
+    near 1e-8, and with alpha < 0 the terms of small s weigh the most.  A sparse table (1% of
+    3,000 tips per sample, Zipf) gives many such terms; 1e-6 per entry."""
+    tree, table = _case(synth, 3000, 300, 0.01, 41, zipf=1.1)
+    say("generalized fp32 alpha", alpha)
+    s, m = ssu.matrix_inmem(inputs_from(ssu, tree, table), "generalized", alpha=alpha)
+    assert s == ssu.OKAY, ssu.last_error()
+    err = rel_err(m, oracle_matrix(O, tree, table, "generalized", alpha=alpha))
+    say("max rel err", err)
+    assert err <= 1e-6
+
+
+VAW_FP32 = [("weighted_normalized", {"variance_adjust": True}), ("weighted_unnormalized", {"variance_adjust": True}),
+            ("unweighted", {"variance_adjust": True}), ("generalized", {"alpha": 0.5, "variance_adjust": True})]
+
+
+@pytest.mark.parametrize("counts", ["fractional", "large integers"])
+@pytest.mark.parametrize("method,kw", VAW_FP32)
+def test_vaw_fp32_counts_not_exact_in_fp32(ssu, O, synth, counts, method, kw):
+    """FP32 VAW divides by sqrt(mi (m - mi)) over node counts and sample totals; with
+    counts FP32 cannot hold (fractional, or two totals summing past 2^24) m - mi cancels
+    near the root, so such tables compute in FP64 and round to FP32: 1e-6 per entry
+    (2.6e-5 before, on the fractional table).  Integer counts keep the FP32 kernels."""
+    tree, table = _case(synth, 565, 65, 0.01, 5, zipf=1.1)
+    obs, sids, ip, ix, d = table
+    rng = np.random.default_rng(7)
+    if counts == "fractional":
+        d = np.asarray(d, dtype=np.float64) * rng.random(len(d)) + 0.01
+    else:
+        d = np.asarray(d, dtype=np.float64) * 123457.0   # integers, totals past 2^23
+    table = (obs, sids, ip, ix, d)
+    inp = inputs_from(ssu, tree, table)
+    say("vaw fp32 counts", counts, method, kw)
+    s, m = ssu.matrix_inmem(inp, method, **kw)
+    assert s == ssu.OKAY, ssu.last_error()
+    assert m.dtype == np.float32
+    ref = oracle_matrix(O, tree, table, method, **kw)
+    assert rel_err(m, ref) <= 1e-6
+    p = ssu.Plan(inp, method, **kw)
+    assert ssu.lib().ssu_plan_result_bytes(p.h) == 8
+    p.close()
+    p = ssu.Plan(inputs_from(ssu, tree, (obs, sids, ip, ix, np.rint(np.asarray(table[4]) / 123457.0 + 1.0))),
+                 method, **kw)
+    assert ssu.lib().ssu_plan_result_bytes(p.h) == 4   # integer counts, small totals: FP32 kernels
+    p.close()
+
+
 def _vaw_split_case(ssu, O, monkeypatch, tree, table, method, kw, prec, theta, walk=True):
     """walk: the samples share some rare node (the posting walk has terms)"""
     monkeypatch.setenv("SSU_VAW_SPLIT", theta)

@@ -122,7 +122,7 @@ ComputeStatus run_full(const Table& t, const ssu::PostorderNodes& nodes, const C
                        const std::shared_ptr<ssu::TableUpload>& pre = nullptr) {
     std::string err;
     auto t0 = Clock::now();
-    std::unique_ptr<Plan> plan = Plan::create(t, nodes, spec, &err, pre);
+    std::unique_ptr<Plan> plan = Plan::create(t, nodes, ssu::spec_for_table(t, spec), &err, pre);
     if (!plan) { set_error(err); return output_error; }
     g_call_ms[CP_PLAN] = ms_since(t0);
     t0 = Clock::now();
@@ -769,7 +769,9 @@ ComputeStatus partial_v3(const char* biom_filename, const char* tree_filename, c
     std::string err;
     ssu::PostorderNodes nodes = synced_nodes(t, tree);
     std::unique_ptr<Plan> plan =
-        Plan::create(t, nodes, make_spec(m, fp64, variance_adjust, alpha, bypass_tips, normalize_sample_count), &err);
+        Plan::create(t, nodes,
+                     ssu::spec_for_table(t, make_spec(m, fp64, variance_adjust, alpha, bypass_tips, normalize_sample_count)),
+                     &err);
     if (!plan || !plan->set_stripes(stripe_start, stripe_stop, &err) || !plan->execute(&err)) {
         set_error(err);
         return output_error;
@@ -1538,7 +1540,7 @@ ComputeStatus ssu_plan_create_inmem(const support_biom_t* td, const support_bptr
     spec.plain_kernel = plain_kernel;
     std::string err;
     ssu::PostorderNodes nodes = synced_nodes(p->table, tree);
-    p->plan = Plan::create(p->table, nodes, spec, &err);
+    p->plan = Plan::create(p->table, nodes, ssu::spec_for_table(p->table, spec), &err);
     if (!p->plan) { set_error(err); return output_error; }
     *out = p.release();
     return okay;

@@ -982,6 +982,17 @@ TableUpload::~TableUpload() {
 Plan::Plan() : p_(new PlanImpl) {}
 Plan::~Plan() = default;
 
+ComputeSpec spec_for_table(const Table& t, ComputeSpec s) {
+    if (!s.vaw || s.fp64) return s;
+    double t1 = 0.0, t2 = 0.0;   // the two largest totals
+    for (double x : t.totals) {
+        if (x > t1) { t2 = t1; t1 = x; }
+        else if (x > t2) t2 = x;
+    }
+    if (!t.values_integral || !(t1 + t2 < 16777216.0)) s.fp64 = true;
+    return s;
+}
+
 std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nodes, const ComputeSpec& spec,
                                    std::string* err, const std::shared_ptr<TableUpload>& pre) {
     if (!device_available(err)) return nullptr;

@@ -33,6 +33,14 @@ struct ComputeSpec {
     bool fixed_segments = false;
 };
 
+// The spec a table's own values run with.  FP32 VAW terms hold the node counts
+// and the sample totals in FP32, and divide by sqrt(mi (m - mi)) with m - mi
+// small near the root: exact when every value is an integer and any two sample
+// totals sum below 2^24, off by up to 2.6e-5 relative otherwise (fractional
+// counts, measured).  Such tables compute in FP64, and the caller rounds the
+// FP64 matrix to FP32.  Other specs are returned as given.
+ComputeSpec spec_for_table(const Table& t, ComputeSpec s);
+
 // Phases of one execution, timed with HIP events on the plan's stream during
 // execute(); PH_SPARSE is the posting-list share of an occupancy split (zero
 // without one), PH_MAIN the main kernel alone.

@@ -795,6 +795,34 @@ template <int STEPS> __device__ __forceinline__ double rsqrt_nr(double x) {
 // below the FP32 rounding the reference's FP32 terms carry)
 template <class TV> constexpr int vaw_steps() { return sizeof(TV) == 8 ? 2 : 0; }
 
+// x^alpha for x > 0 from the hardware log2 / exp2, as the generalized terms of
+// the shared-node path use them (0 <= alpha <= 1 there): exp2(a log2 x) carries
+// the rounding of a log2 x, |a log2 x| ulp, which the weights (x + y)^alpha
+// and the ratios |u - v| / (u + v) keep below 1e-6 of a distance for alpha in
+// [0, 1] (every FP32 test, 1.1e-7 to 2.3e-7 at 8,192 samples).
+__device__ __forceinline__ float pow_alpha(float x, float alpha) {
+    return __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(x));
+}
+
+// x^a and x^(a - 1) for x > 0 with the exponent of x kept exact, for the dense
+// kernel's alpha outside [0, 1], where terms of small x can carry a distance
+// (alpha < 0) and exp2(a log2 x) is off by up to 2.4e-6 (a = 1.5, x near 1e-8;
+// simulated with correctly rounded log2 / exp2).  x = m 2^e (m in [0.5, 1)),
+// a e = n + r with n the nearest integer and r exact (an FMA gives the
+// product's rounding error), x^a = 2^n exp2(r + a log2 m): exp2 sees
+// |argument| <= 1/2 + |a| and ldexp applies 2^n exactly; x^(a-1) = x^a / x.
+// About 1.3e-7 relative for x^a, 2.5e-7 for x^(a-1).
+__device__ __forceinline__ void pow_alpha2_split(float x, float a, float& pa, float& pa1) {
+    const float lm = __builtin_amdgcn_logf(__builtin_amdgcn_frexp_mantf(x));
+    const float fe = (float)__builtin_amdgcn_frexp_expf(x);
+    const float ae = a * fe;
+    const float lo = __builtin_fmaf(a, fe, -ae);   // a e - ae, exactly
+    const float nr = __builtin_rintf(ae);
+    const float f = __builtin_fmaf(a, lm, (ae - nr) + lo);
+    pa = __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(f), (int)nr);
+    pa1 = pa * __builtin_amdgcn_rcpf(x);
+}
+
 // One-sided VAW term (src/unifrac_task_impl.hpp:481-494 with one count zero):
 // a sample holding a node with count c > 0 and value term k (l u, or l for
 // presence), e = max(T - c, 0) the rest of its total, against a sample that
@@ -819,7 +847,7 @@ template <PairOp OP> struct PairTraits {
 // presence terms, and FP64 weighted terms; lengths >= 0): u and v arrive
 // multiplied by l, so l |u - v| = |lu - lv| and l (u + v) = lu + lv
 // (presence: l max(u, v) = max(lu, lv), exact for 0/1 values).
-template <PairOp OP, class T, bool LPRE = false>
+template <PairOp OP, class T, bool LPRE = false, bool ACC = false>
 __device__ __forceinline__ void pair_term(T u, T v, T cu, T cv, T m, T l, T alpha, T& num, T& den) {
     if constexpr (OP == PairOp::weighted) {
         // src/unifrac_task_impl.hpp:117-127
@@ -834,11 +862,18 @@ __device__ __forceinline__ void pair_term(T u, T v, T cu, T cv, T m, T l, T alph
         const bool nz = s != T(0);
         const T ss = nz ? s : T(1);
         if constexpr (sizeof(T) == 4) {
-            // FP32: s^a and s^(a-1) from the hardware log2 / exp2 (about 1 ulp each);
-            // the FP64 merges and the FP64 oracle comparison bound the result.
-            const float lg = __builtin_amdgcn_logf(ss);
-            const float w = nz ? __builtin_amdgcn_exp2f(alpha * lg) * l : 0.0f;
-            const float q = nz ? __builtin_amdgcn_exp2f((alpha - 1.0f) * lg) * l : 0.0f;
+            // FP32: s^a and s^(a-1) from the hardware log2 / exp2, or, for alpha outside
+            // [0, 1] (ACC), with the exponent of s kept exact (pow_alpha2_split)
+            float pa, pa1;
+            if constexpr (ACC) {
+                pow_alpha2_split(ss, alpha, pa, pa1);
+            } else {
+                const float lg = __builtin_amdgcn_logf(ss);
+                pa = __builtin_amdgcn_exp2f(alpha * lg);
+                pa1 = __builtin_amdgcn_exp2f((alpha - 1.0f) * lg);
+            }
+            const float w = nz ? pa * l : 0.0f;
+            const float q = nz ? pa1 * l : 0.0f;
             num = fma(q, dabs(u - v), num);
             den += w;
         } else {
@@ -866,9 +901,16 @@ __device__ __forceinline__ void pair_term(T u, T v, T cu, T cv, T m, T l, T alph
             const bool on = ok && s1 != T(0);
             const T ss = on ? s1 : T(1);
             if constexpr (sizeof(T) == 4) {
-                const float lg = __builtin_amdgcn_logf(ss);
-                const float w = on ? __builtin_amdgcn_exp2f(alpha * lg) * l : 0.0f;
-                const float q = on ? __builtin_amdgcn_exp2f((alpha - 1.0f) * lg) * l : 0.0f;
+                float pa, pa1;
+                if constexpr (ACC) {
+                    pow_alpha2_split(ss, alpha, pa, pa1);
+                } else {
+                    const float lg = __builtin_amdgcn_logf(ss);
+                    pa = __builtin_amdgcn_exp2f(alpha * lg);
+                    pa1 = __builtin_amdgcn_exp2f((alpha - 1.0f) * lg);
+                }
+                const float w = on ? pa * l : 0.0f;
+                const float q = on ? pa1 * l : 0.0f;
                 num = fma(q, d1, num);
                 den += w;
             } else {
@@ -899,7 +941,7 @@ template <class T> struct PairStage { static constexpr int EB = sizeof(T) == 4 ?
 // add_partial (two-accumulator ops): out[i * n + j] and out2[i * n + j], i < j,
 // hold the posting-list shares of the numerator and the denominator, added
 // before the division; each pair is then written by its i < j thread alone.
-template <PairOp OP, class T, int RI, int RJ, int TT, bool LPRE = false>
+template <PairOp OP, class T, int RI, int RJ, int TT, bool LPRE = false, bool ACC = false>
 __global__ void __launch_bounds__((TT / RI) * (TT / RJ))
 k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __restrict__ rows, size_t ldn,
         const T* __restrict__ len, uint32_t K,
@@ -1023,8 +1065,8 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
 #pragma unroll
                 for (int c = 0; c < RJ; ++c) {
                     if constexpr (TWO)
-                        pair_term<OP, T, LPRE>(u[r], v[c], cu[r], cv[c], CNT ? mrow[r][c] : T(0), l, alpha, num[r][c],
-                                               den[r][c]);
+                        pair_term<OP, T, LPRE, ACC>(u[r], v[c], cu[r], cv[c], CNT ? mrow[r][c] : T(0), l, alpha,
+                                                    num[r][c], den[r][c]);
                     else
                         pair_term<OP, T>(u[r], v[c], cu[r], cv[c], T(0), l, alpha, num[r][c], dummy);
                 }
@@ -1743,10 +1785,6 @@ template <class TV> struct alignas(16) RowEntry<TV, true> {   // generalized
     unsigned long long q8;    // l * scale (FP64 bits)
 };
 
-// x^alpha for x > 0 from the hardware log2 / exp2, as the generalized terms use them
-__device__ __forceinline__ float pow_alpha(float x, float alpha) {
-    return __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(x));
-}
 // the generalized terms' x^alpha in the value type: FP32 as above, FP64 as the FP64 kernels
 __device__ __forceinline__ float gen_pow(float x, double alpha) { return pow_alpha(x, (float)alpha); }
 __device__ __forceinline__ double gen_pow(double x, double alpha) { return dpow<double>(x, alpha); }
@@ -2386,13 +2424,13 @@ __global__ void __launch_bounds__(256) k_matrix_to_stripes(double* __restrict__ 
     st[(size_t)blockIdx.y * n + k] = (double)M[(size_t)k * n + l];
 }
 
-template <PairOp OP, class T, int RI, int RJ, int TT = PAIR_TILE, bool LPRE = false>
+template <PairOp OP, class T, int RI, int RJ, int TT = PAIR_TILE, bool LPRE = false, bool ACC = false>
 hipError_t launch_pairs_t(const PairArgs& a, hipStream_t st) {
     static_assert(PAIR_TILE % TT == 0, "the row stride is padded to PAIR_TILE");
     const uint32_t tiles = (a.n + TT - 1) / TT;
     dim3 grid(tiles, tiles);
     constexpr int NT = (TT / RI) * (TT / RJ);
-    hipLaunchKernelGGL((k_pairs<OP, T, RI, RJ, TT, LPRE>), grid, dim3(NT), 0, st, (const T*)a.U, (const T*)a.Cn, a.rows, a.ldn,
+    hipLaunchKernelGGL((k_pairs<OP, T, RI, RJ, TT, LPRE, ACC>), grid, dim3(NT), 0, st, (const T*)a.U, (const T*)a.Cn, a.rows, a.ldn,
                        (const T*)a.len, a.K, a.n, a.S, a.totals, a.alpha, (int)a.normalized, a.out, (int)a.out_fp64,
                        a.band, a.out2 ? 1 : 0, a.out2);
     return hipGetLastError();
@@ -2405,13 +2443,20 @@ hipError_t launch_pairs_prec(PairOp op, const PairArgs& a, hipStream_t st) {
     switch (op) {
     case PairOp::weighted: return launch_pairs_t<PairOp::weighted, T, 8, 8>(a, st);
     case PairOp::unweighted_plain: return launch_pairs_t<PairOp::unweighted_plain, T, 4, 8>(a, st);
-    case PairOp::generalized: return launch_pairs_t<PairOp::generalized, T, 4, 4, 64>(a, st);
+    // FP32 generalized powers with the exponent kept exact for alpha outside [0, 1] (pow_alpha2_split)
+    case PairOp::generalized:
+        return sizeof(T) == 4 && !(a.alpha >= 0.0 && a.alpha <= 1.0)
+                   ? launch_pairs_t<PairOp::generalized, T, 4, 4, 64, false, true>(a, st)
+                   : launch_pairs_t<PairOp::generalized, T, 4, 4, 64>(a, st);
     case PairOp::vaw_weighted:
         // FP64 only: in FP32, rounding l u and l v separately doubles the worst-case error of
         // l |u - v| (2^-23 (u + v) l), past what the near-duplicate bound allows
         return a.len_nonneg && sizeof(T) == 8 ? launch_pairs_t<PairOp::vaw_weighted, T, 4, 4, 64, true>(a, st)
                                               : launch_pairs_t<PairOp::vaw_weighted, T, 4, 4, 64>(a, st);
-    case PairOp::vaw_generalized: return launch_pairs_t<PairOp::vaw_generalized, T, 4, 4, 64>(a, st);
+    case PairOp::vaw_generalized:
+        return sizeof(T) == 4 && !(a.alpha >= 0.0 && a.alpha <= 1.0)
+                   ? launch_pairs_t<PairOp::vaw_generalized, T, 4, 4, 64, false, true>(a, st)
+                   : launch_pairs_t<PairOp::vaw_generalized, T, 4, 4, 64>(a, st);
     case PairOp::vaw_unweighted:
         return a.len_nonneg ? launch_pairs_t<PairOp::vaw_unweighted, T, 4, 4, 64, true>(a, st)
                             : launch_pairs_t<PairOp::vaw_unweighted, T, 4, 4, 64>(a, st);

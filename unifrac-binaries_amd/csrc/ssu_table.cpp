@@ -11,6 +11,7 @@ namespace {
 struct EntryScan {
     bool index_ok = true;   // every sample index < n_samples
     bool nonneg = true;     // every value finite and >= 0
+    bool integral = true;   // every value an integer
 };
 
 // Sample totals over observation ranges on several threads, when that gives the
@@ -31,6 +32,7 @@ bool integer_totals(const uint32_t* indptr, const uint32_t* indices, const doubl
                 const uint32_t s = indices[j];
                 const double v = data[j];
                 scan.nonneg &= v >= 0.0 && v <= 1.7976931348623157e308;
+                scan.integral &= v == std::nearbyint(v);
                 if (s < n_samples) tot[s] += v;
                 else scan.index_ok = false;
             }
@@ -45,19 +47,21 @@ bool integer_totals(const uint32_t* indptr, const uint32_t* indices, const doubl
         cut[t] = i;
     }
     std::vector<std::vector<double>> part(threads);
-    std::vector<uint8_t> integral(threads, 1), index_ok(threads, 1), nonneg(threads, 1);
+    std::vector<uint8_t> integral(threads, 1), index_ok(threads, 1), nonneg(threads, 1), whole_vals(threads, 1);
     std::vector<double> mag(threads, 0.0);
     std::vector<std::thread> pool;
     auto run = [&](unsigned t) {
         std::vector<double>& p = part[t];
         p.assign((size_t)n_samples + 1, 0.0);   // slot n_samples takes out-of-range indices
-        bool ok = true, ix = true, nn = true;
+        bool ok = true, ix = true, nn = true, wv = true;
         double a = 0.0;   // sum of magnitudes (exact while below 2^53)
         for (uint32_t i = cut[t]; i < cut[t + 1]; ++i)
             for (uint32_t j = indptr[i]; j < indptr[i + 1]; ++j) {
                 const uint32_t s = indices[j];
                 const double v = data[j];
-                ok &= v == std::nearbyint(v) && std::fabs(v) < 0x1p52;
+                const bool w = v == std::nearbyint(v);
+                wv &= w;
+                ok &= w && std::fabs(v) < 0x1p52;
                 nn &= v >= 0.0 && v <= 1.7976931348623157e308;
                 ix &= s < n_samples;
                 a += std::fabs(v);
@@ -66,6 +70,7 @@ bool integer_totals(const uint32_t* indptr, const uint32_t* indices, const doubl
         integral[t] = ok;
         index_ok[t] = ix;
         nonneg[t] = nn;
+        whole_vals[t] = wv;
         mag[t] = a;
     };
     for (unsigned t = 1; t < threads; ++t) pool.emplace_back(run, t);
@@ -76,6 +81,7 @@ bool integer_totals(const uint32_t* indptr, const uint32_t* indices, const doubl
     for (unsigned t = 0; t < threads; ++t) {
         scan.index_ok &= index_ok[t] != 0;
         scan.nonneg &= nonneg[t] != 0;
+        scan.integral &= whole_vals[t] != 0;
         whole = whole && integral[t] && mag[t] < 0x1p52;
         a += mag[t];
     }
@@ -103,6 +109,7 @@ void Table::finish(bool index_obs) {
     }
     index_ok = scan.index_ok;
     values_nonneg = scan.nonneg;
+    values_integral = scan.integral;
     obs_index.clear();
     if (!index_obs) return;
     obs_index.reserve(n_obs * 2 + 1);

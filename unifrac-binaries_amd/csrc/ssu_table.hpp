@@ -18,8 +18,9 @@ struct Table {
     const double* data = nullptr;
     std::vector<double> totals;                            // per-sample sum of values
     // Found by the pass that forms the totals: every sample index < n_samples
-    // (entries outside are left out of the totals), every value finite and >= 0
-    bool index_ok = true, values_nonneg = true;
+    // (entries outside are left out of the totals), every value finite and >= 0,
+    // every value an integer
+    bool index_ok = true, values_nonneg = true, values_integral = true;
     std::unordered_map<std::string, uint32_t> obs_index;   // id -> row
 
     // Borrow caller arrays (src/biom_inmem.cpp:342-384 borrows them too).
