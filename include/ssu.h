@@ -403,7 +403,9 @@ SSU_EXTERN ComputeStatus ssu_plan_execute_stripes(ssu_plan_t* plan, unsigned int
  * row s - start, element k = d(k, (k + s + 1) mod n) */
 SSU_EXTERN ComputeStatus ssu_plan_copy_stripes(const ssu_plan_t* plan, unsigned int start, unsigned int stop,
                                                double* host_out);
-/* element size of the result: 8 (fp64 method names) or 4 */
+/* element size of the result: 8 (fp64 method names, and variance-adjusted fp32
+ * names on tables whose counts FP32 cannot hold: a value that is not an integer,
+ * or two sample totals summing to 2^24 or more) or 4 */
 SSU_EXTERN int ssu_plan_result_bytes(const ssu_plan_t* plan);
 SSU_EXTERN ComputeStatus ssu_plan_copy_matrix(const ssu_plan_t* plan, void* host_out);
 /* rows [row0, row0 + nrows) of the n x n result (nrows * n elements), for
