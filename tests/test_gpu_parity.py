@@ -781,6 +781,41 @@ def test_generalized_fp32_alpha_outside_unit_interval(ssu, O, synth, alpha):
     assert err <= 1e-6
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("method,kw", [("weighted_normalized", {"variance_adjust": True}),
+                                       ("unweighted", {"variance_adjust": True}),
+                                       ("generalized", {"alpha": 0.5, "variance_adjust": True})])
+def test_vaw_single_tip_samples_on_shared_node_path(ssu, O, synth, monkeypatch, method, kw, prec):
+    """1,030 samples of one tip each (40 tips, integer counts): two samples on the same tip
+    hold every node with mi = m, so vaw = 0 on all of them and the reference returns 0 / 0.
+    The shared-node path forms that pair's shares as one-sided sums less the same terms;
+    what the cancellation leaves must come out as 0, so the entry is NaN, as in the oracle."""
+    tree = synth.random_tree(40, 47)
+    rng = np.random.default_rng(48)
+    n = 1030
+    tips = [x for x in tree[2] if x]
+    own = rng.integers(0, len(tips), n)
+    obs = tips
+    ip, ix, d = [0], [], []
+    for t in range(len(tips)):
+        who = np.nonzero(own == t)[0]
+        ix.extend(who.tolist()); d.extend(rng.integers(1, 200, len(who)).astype(float).tolist()); ip.append(len(ix))
+    table = (obs, [f"s{i}" for i in range(n)], np.array(ip), np.array(ix), np.array(d))
+    say("single-tip samples", method, kw, prec)
+    monkeypatch.setenv("SSU_VAW_SPLIT", "0.5")   # every tip rare: the pairs on one tip meet in the walk
+    p = ssu.Plan(inputs_from(ssu, tree, table), method + ("_fp64" if prec == "fp64" else ""), **kw)
+    assert abs(p.split_theta() - 0.5) < 0.01
+    p.execute()
+    assert p.sparse_terms() > 0
+    m = p.matrix()
+    p.close()
+    ref = oracle_matrix(O, tree, table, method, **kw)
+    same = own[:, None] == own[None, :]
+    np.fill_diagonal(same, False)
+    assert np.all(np.isnan(ref[same])) and np.array_equal(np.isnan(m), np.isnan(ref))
+    assert rel_err(m, ref) <= TOL[prec]
+
+
 VAW_FP32 = [("weighted_normalized", {"variance_adjust": True}), ("weighted_unnormalized", {"variance_adjust": True}),
             ("unweighted", {"variance_adjust": True}), ("generalized", {"alpha": 0.5, "variance_adjust": True})]
 

@@ -9,7 +9,8 @@ Each case draws:
 * lognormal or negative-binomial counts, made fractional in some cases;
 * a method with its options (every row of the parity list) and a precision.
 
-Each case also computes Faith PD, which must be bit-exact. Tolerances are those of the parity
+Each case also computes Faith PD, which must be bit-exact (over the samples whose total is at
+least 1, which the reference keeps). Tolerances are those of the parity
 tests: FP64 1e-11, FP32 1e-6 relative per matrix entry against the FP64 oracle.
 """
 import sys
@@ -88,6 +89,25 @@ def test_random_case_against_oracle(ssu, O, synth, seed):
     err = rel_err(m, ref)
     say("max rel err", err)
     assert err <= TOL[prec]
-    s, vals, _ = ssu.faith_pd_inmem(inp)
+    # Faith PD drops samples whose total is below 1 first, as the reference does
+    # (src/api.cpp:577-578), then the observations left without entries
+    s, vals, ids = ssu.faith_pd_inmem(inp)
     assert s == ssu.OKAY, ssu.last_error()
-    assert np.array_equal(vals, O.faith_pd(ot, otab))
+    obs, sids, ip, ix, d = table
+    ix, d = np.asarray(ix), np.asarray(d, dtype=np.float64)
+    tot = np.bincount(ix, weights=d, minlength=len(sids))
+    keep = np.nonzero(tot >= 1.0)[0]
+    if len(keep) == len(sids):
+        assert np.array_equal(vals, O.faith_pd(ot, otab))
+    else:
+        new_ix = -np.ones(len(sids), dtype=np.int64)
+        new_ix[keep] = np.arange(len(keep))
+        fobs, fip, fix, fd = [], [0], [], []
+        for r in range(len(obs)):
+            q = np.arange(ip[r], ip[r + 1])
+            q = q[new_ix[ix[q]] >= 0]
+            if len(q):
+                fobs.append(obs[r]); fix.extend(new_ix[ix[q]]); fd.extend(d[q]); fip.append(len(fix))
+        ftab = O.Table(fobs, [sids[k] for k in keep], np.array(fip), np.array(fix), np.array(fd))
+        assert ids == [sids[k] for k in keep]
+        assert np.array_equal(vals, O.faith_pd(ot, ftab))

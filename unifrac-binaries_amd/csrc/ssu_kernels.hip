@@ -1901,6 +1901,17 @@ __device__ __forceinline__ void vaw_row_term(unsigned long long* accb, const Row
     }
 }
 
+// A pair's share formed as base2 (the two samples' one-sided or per-sample sums,
+// >= 0) plus the walk's corrections, which take back the terms of the nodes both
+// samples hold.  When the pair's true share is 0 (for VAW: every node both hold
+// has mi = m, so vaw = 0 and the reference adds no term) what remains is the
+// rounding of that cancellation, FP64 sums and fixed-point units, below 1e-11 of
+// base2: shares within 2^-30 of base2 are 0, so that such a pair is 0 / 0 as in
+// the reference.  (A true share that small would be that rounding's anyway.)
+__device__ __forceinline__ double snap_share(double x, double base2) {
+    return __builtin_fabs(x) <= 0x1p-30 * base2 ? 0.0 : x;
+}
+
 template <class TOut, class IX, bool GRAM, class TV, bool GEN, int VAW = 0>
 __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__ out, uint32_t* __restrict__ walked,
                                                    const unsigned long long* __restrict__ rows,
@@ -2176,8 +2187,8 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         for (uint32_t b = wlo + threadIdx.x; b < whi; b += 64 * SR_WAVES) {
             const uint32_t kb = kidx ? kidx[b] : b;
             const double base2 = Hm[(size_t)a * D + kb] + Hm[(size_t)b * D + ka];
-            out[(size_t)a * n + b] = (TOut)(base2 + (double)(long long)acc[b - b0] * inv);
-            out2[(size_t)a * n + b] = (TOut)(base2 + (double)(long long)acc[CW + b - b0] * inv);
+            out[(size_t)a * n + b] = (TOut)snap_share(base2 + (double)(long long)acc[b - b0] * inv, base2);
+            out2[(size_t)a * n + b] = (TOut)snap_share(base2 + (double)(long long)acc[CW + b - b0] * inv, base2);
         }
     } else if constexpr (GEN) {
         // numerator and denominator shares: P_a + P_b plus the corrections of the
@@ -2185,8 +2196,8 @@ __global__ void __launch_bounds__(64 * SR_WAVES) k_pairs_rows(TOut* __restrict__
         const double sa = ssp[a];
         for (uint32_t b = wlo + threadIdx.x; b < whi; b += 64 * SR_WAVES) {
             const double base2 = sa + ssp[b];
-            out[(size_t)a * n + b] = (TOut)(base2 + (double)(long long)acc[b - b0] * inv);
-            out2[(size_t)a * n + b] = (TOut)(base2 + (double)(long long)acc[CW + b - b0] * inv);
+            out[(size_t)a * n + b] = (TOut)snap_share(base2 + (double)(long long)acc[b - b0] * inv, base2);
+            out2[(size_t)a * n + b] = (TOut)snap_share(base2 + (double)(long long)acc[CW + b - b0] * inv, base2);
         }
     } else if constexpr (sizeof(TV) == 4) {
         // FP32 values: the sparse nodes' share of sum l |u_a - u_b| itself, formed in
