@@ -595,6 +595,27 @@ def near_duplicate_table(synth, n_base=40, tips=400, seed=95):
     return (st, le, nm), table
 
 
+@pytest.mark.parametrize("normalize_name", ["weighted_unnormalized", "weighted_normalized"])
+def test_vaw_weighted_near_duplicate_samples(ssu, O, synth, normalize_name):
+    """VAW weighted on near-duplicate samples (distances 1e-4..1e-2 of their VAW denominator):
+    FP32 entries within 1e-6 of the FP64 oracle.  The unnormalised VAW distance has no
+    per-sample scale for the refinement's test; the pair kernel marks the pairs below 0.1
+    of their own VAW denominator, and every pair of the marked samples is recomputed in FP64."""
+    tree, table = near_duplicate_table(synth)
+    say("vaw near duplicates", normalize_name)
+    inp = inputs_from(ssu, tree, table)
+    s, m = ssu.matrix_inmem(inp, normalize_name, variance_adjust=True)
+    assert s == ssu.OKAY, ssu.last_error()
+    ref = oracle_matrix(O, tree, table, normalize_name, variance_adjust=True)
+    err = rel_err(m, ref)
+    say("max rel err", err)
+    assert err <= TOL["fp32"]
+    p = ssu.Plan(inp, normalize_name, variance_adjust=True)
+    p.execute()
+    assert p.refined_pairs() > 0
+    p.close()
+
+
 @pytest.mark.parametrize("name", ["weighted_normalized", "weighted_unnormalized", "weighted_normalized_fp64",
                                   "weighted_unnormalized_fp64"])
 def test_weighted_near_duplicate_samples(ssu, O, synth, name):

@@ -437,6 +437,9 @@ struct PlanImpl {
     // near-duplicate refinement (FP32 weighted / generalized): pairs below the
     // bound are recomputed in FP64; refine_S: the bound scales with S_i + S_j
     bool refine_on = false, refine_S = false;
+    // refine_vaw (unnormalised VAW weighted): the pair kernel marks the pairs, against
+    // each pair's VAW denominator, and every pair of the marked samples is recomputed
+    bool refine_vaw = false;
     float refine_tau = 0.0f;
     uint64_t refined = 0;
     std::vector<uint32_t> refined_samples;   // the samples of the last refinement (its pairs lie among them)
@@ -1165,13 +1168,15 @@ std::unique_ptr<Plan> Plan::create(const Table& table, const PostorderNodes& nod
     // recomputed in FP64, which keeps every entry within 1e-6.  The
     // unnormalised weighted distance needs S, computed on the shared-node path
     // (tables with negative values or non-finite lengths run the dense kernel
-    // without it), and the unnormalised VAW distance has no per-pair
-    // denominator to bound it: those two are not refined.  SSU_REFINE=0 turns
-    // refinement off, SSU_REFINE_BELOW sets the bound.
+    // without it, and are not refined).  The unnormalised VAW distance is
+    // bounded by its own VAW denominator, which only the pair kernel holds: it
+    // marks those pairs itself (refine_vaw).  SSU_REFINE=0 turns refinement
+    // off, SSU_REFINE_BELOW sets the bound.
     if (!spec.fp64 && spec.plain_kernel == 0) {
+        P.refine_vaw = spec.method == Method::weighted_unnormalized && spec.vaw;
         P.refine_on = spec.method == Method::weighted_normalized || spec.method == Method::generalized ||
-                      (spec.method == Method::weighted_unnormalized && !spec.vaw && P.sparse);
-        P.refine_S = spec.method == Method::weighted_unnormalized;
+                      (spec.method == Method::weighted_unnormalized && (P.sparse || spec.vaw));
+        P.refine_S = spec.method == Method::weighted_unnormalized && !spec.vaw;
         P.refine_tau = 0.1f;
         if (const char* env = getenv("SSU_REFINE_BELOW")) P.refine_tau = (float)atof(env);
         if (const char* env = getenv("SSU_REFINE")) P.refine_on = P.refine_on && atoi(env) != 0;
@@ -1573,6 +1578,18 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
             a.out2 = P.d_out2.p;
             a.len_nonneg = true;   // (the split needs every length finite and >= 0)
         }
+        if (P.refine_vaw) {   // the kernel marks the near-duplicates itself
+            uint32_t* vf = d_flags;
+            if (!vf && !quiet) {
+                if (!P.d_close.ensure(((size_t)P.n + 1) * sizeof(uint32_t), err)) return false;
+                vf = P.d_close.as<uint32_t>();
+            }
+            if (vf) {
+                SSU_HIP(launch_fill_u32(vf, (size_t)P.n + 1, 0u, P.st));
+                a.flags = vf;
+                a.flag_tau = P.refine_tau;
+            }
+        }
         SSU_HIP(launch_pairs(op, s.fp64, a, P.st));
         SSU_HIP(P.mark(3));
         SSU_HIP(P.mark(4));
@@ -1583,7 +1600,7 @@ bool Plan::execute_into(const double* d_data, const double* d_totals, void* d_ou
         if (!P.d_close.ensure(((size_t)P.n + 1) * sizeof(uint32_t), err)) return false;
         flags = P.d_close.as<uint32_t>();
     }
-    if (P.refine_on && flags)
+    if (P.refine_on && flags && !P.refine_vaw)
         SSU_HIP(launch_flag_close(flags, (const float*)P.out_ptr(), P.n, P.refine_S ? P.S_ptr() : nullptr,
                                   P.refine_tau, P.band, P.st));
     SSU_HIP(P.mark(5));
@@ -1841,9 +1858,10 @@ bool PlanImpl::refine(const uint32_t* flags, std::string* err) {
             std::unique_ptr<Plan> sp_plan = Plan::create(sub, nodes, sp, err);
             if (!sp_plan || !sp_plan->execute(err)) return false;
             if (!upload(d_ridx, idx, st, err)) return false;
+            // (refine_vaw: every pair of the marked samples, the FP64 value being the better one)
             SSU_HIP(launch_refine_scatter((float*)out_ptr(), n, (const double*)sp_plan->device_output(), ns,
                                           d_ridx.as<uint32_t>(), na, B != A, refine_S ? S_ptr() : nullptr,
-                                          refine_tau, st));
+                                          refine_vaw ? INFINITY : refine_tau, st));
             SSU_HIP(hipStreamSynchronize(st));   // idx and the sub-plan's result are read until here
         }
     refined = count;

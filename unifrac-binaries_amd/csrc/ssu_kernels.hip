@@ -946,7 +946,8 @@ __global__ void __launch_bounds__((TT / RI) * (TT / RJ))
 k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __restrict__ rows, size_t ldn,
         const T* __restrict__ len, uint32_t K,
         uint32_t n, const double* __restrict__ S, const double* __restrict__ totals, double alpha_d, int normalized,
-        void* __restrict__ out_v, int out_fp64, GapBand band, int add_partial, const void* __restrict__ out2_v) {
+        void* __restrict__ out_v, int out_fp64, GapBand band, int add_partial, const void* __restrict__ out2_v,
+        uint32_t* __restrict__ flags, float flag_tau) {
     constexpr int NTI = TT / RI, NTJ = TT / RJ, NT = NTI * NTJ;
     constexpr bool TWO = PairTraits<OP>::two;
     constexpr bool CNT = PairTraits<OP>::counts;
@@ -1116,6 +1117,15 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
             else if (!normalized) d = nm;
             else if (TWO) d = nm / dn;
             else d = nm / (S[i] + S[j]);
+            if constexpr (OP == PairOp::vaw_weighted) {
+                // unnormalised: near-duplicates (distance below flag_tau of the VAW
+                // denominator) marked for the FP64 recomputation
+                if (flags && !normalized && i < j && nm < (double)flag_tau * dn) {
+                    flags[1 + i] = 1u;
+                    flags[1 + j] = 1u;
+                    atomicAdd(flags, 1u);
+                }
+            }
             if (out_fp64) {
                 double* o = (double*)out_v;
                 o[i * n + j] = d; o[j * n + i] = d;
@@ -2443,7 +2453,7 @@ hipError_t launch_pairs_t(const PairArgs& a, hipStream_t st) {
     constexpr int NT = (TT / RI) * (TT / RJ);
     hipLaunchKernelGGL((k_pairs<OP, T, RI, RJ, TT, LPRE, ACC>), grid, dim3(NT), 0, st, (const T*)a.U, (const T*)a.Cn, a.rows, a.ldn,
                        (const T*)a.len, a.K, a.n, a.S, a.totals, a.alpha, (int)a.normalized, a.out, (int)a.out_fp64,
-                       a.band, a.out2 ? 1 : 0, a.out2);
+                       a.band, a.out2 ? 1 : 0, a.out2, a.flags, a.flag_tau);
     return hipGetLastError();
 }
 

@@ -192,6 +192,10 @@ struct PairArgs {
     const void* out2 = nullptr;
     // every length >= 0 (VAW weighted / presence terms stage l u instead of u)
     bool len_nonneg = false;
+    // unnormalised VAW weighted: with flags, a pair (i < j) whose distance is below
+    // flag_tau of its VAW denominator sets flags[1 + i], flags[1 + j] and adds 1 to flags[0]
+    uint32_t* flags = nullptr;
+    float flag_tau = 0.0f;
 };
 hipError_t launch_pairs(PairOp op, bool fp64, const PairArgs& a, hipStream_t st);
 
