@@ -847,7 +847,10 @@ template <PairOp OP> struct PairTraits {
 // presence terms, and FP64 weighted terms; lengths >= 0): u and v arrive
 // multiplied by l, so l |u - v| = |lu - lv| and l (u + v) = lu + lv
 // (presence: l max(u, v) = max(lu, lv), exact for 0/1 values).
-template <PairOp OP, class T, bool LPRE = false, bool ACC = false>
+// PW (FP32 generalized powers): 0 the hardware exp2(a log2 s), 1 the exponent of s kept
+// exact (pow_alpha2_split, alpha outside [0, 1]), 2 alpha = 1/2 by the hardware square
+// root and reciprocal square root (1 ulp each, and one instruction fewer than 0)
+template <PairOp OP, class T, bool LPRE = false, int PW = 0>
 __device__ __forceinline__ void pair_term(T u, T v, T cu, T cv, T m, T l, T alpha, T& num, T& den) {
     if constexpr (OP == PairOp::weighted) {
         // src/unifrac_task_impl.hpp:117-127
@@ -862,11 +865,13 @@ __device__ __forceinline__ void pair_term(T u, T v, T cu, T cv, T m, T l, T alph
         const bool nz = s != T(0);
         const T ss = nz ? s : T(1);
         if constexpr (sizeof(T) == 4) {
-            // FP32: s^a and s^(a-1) from the hardware log2 / exp2, or, for alpha outside
-            // [0, 1] (ACC), with the exponent of s kept exact (pow_alpha2_split)
+            // FP32: s^a and s^(a-1) as PW selects
             float pa, pa1;
-            if constexpr (ACC) {
+            if constexpr (PW == 1) {
                 pow_alpha2_split(ss, alpha, pa, pa1);
+            } else if constexpr (PW == 2) {
+                pa = __builtin_amdgcn_sqrtf(ss);
+                pa1 = __builtin_amdgcn_rsqf(ss);
             } else {
                 const float lg = __builtin_amdgcn_logf(ss);
                 pa = __builtin_amdgcn_exp2f(alpha * lg);
@@ -902,8 +907,11 @@ __device__ __forceinline__ void pair_term(T u, T v, T cu, T cv, T m, T l, T alph
             const T ss = on ? s1 : T(1);
             if constexpr (sizeof(T) == 4) {
                 float pa, pa1;
-                if constexpr (ACC) {
+                if constexpr (PW == 1) {
                     pow_alpha2_split(ss, alpha, pa, pa1);
+                } else if constexpr (PW == 2) {
+                    pa = __builtin_amdgcn_sqrtf(ss);
+                    pa1 = __builtin_amdgcn_rsqf(ss);
                 } else {
                     const float lg = __builtin_amdgcn_logf(ss);
                     pa = __builtin_amdgcn_exp2f(alpha * lg);
@@ -941,7 +949,7 @@ template <class T> struct PairStage { static constexpr int EB = sizeof(T) == 4 ?
 // add_partial (two-accumulator ops): out[i * n + j] and out2[i * n + j], i < j,
 // hold the posting-list shares of the numerator and the denominator, added
 // before the division; each pair is then written by its i < j thread alone.
-template <PairOp OP, class T, int RI, int RJ, int TT, bool LPRE = false, bool ACC = false>
+template <PairOp OP, class T, int RI, int RJ, int TT, bool LPRE = false, int PW = 0>
 __global__ void __launch_bounds__((TT / RI) * (TT / RJ))
 k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __restrict__ rows, size_t ldn,
         const T* __restrict__ len, uint32_t K,
@@ -1066,7 +1074,7 @@ k_pairs(const T* __restrict__ U, const T* __restrict__ Cn, const int32_t* __rest
 #pragma unroll
                 for (int c = 0; c < RJ; ++c) {
                     if constexpr (TWO)
-                        pair_term<OP, T, LPRE, ACC>(u[r], v[c], cu[r], cv[c], CNT ? mrow[r][c] : T(0), l, alpha,
+                        pair_term<OP, T, LPRE, PW>(u[r], v[c], cu[r], cv[c], CNT ? mrow[r][c] : T(0), l, alpha,
                                                     num[r][c], den[r][c]);
                     else
                         pair_term<OP, T>(u[r], v[c], cu[r], cv[c], T(0), l, alpha, num[r][c], dummy);
@@ -1796,7 +1804,11 @@ template <class TV> struct alignas(16) RowEntry<TV, true> {   // generalized
 };
 
 // the generalized terms' x^alpha in the value type: FP32 as above, FP64 as the FP64 kernels
-__device__ __forceinline__ float gen_pow(float x, double alpha) { return pow_alpha(x, (float)alpha); }
+// (FP32: alpha = 1/2 by the hardware square root, as the dense kernel's PW 2; the
+// test is uniform, alpha being the launch's)
+__device__ __forceinline__ float gen_pow(float x, double alpha) {
+    return alpha == 0.5 ? __builtin_amdgcn_sqrtf(x) : pow_alpha(x, (float)alpha);
+}
 __device__ __forceinline__ double gen_pow(double x, double alpha) { return dpow<double>(x, alpha); }
 
 // round(x * ls) as a signed integer (two's complement in the 64-bit
@@ -1840,7 +1852,7 @@ __device__ __forceinline__ void row_term(unsigned long long* accb, const RowEntr
         const float af = (float)alpha;
         const float uu = (float)en.lu, vv = (float)v, sm = uu + vv;
         const float p1 = __builtin_amdgcn_exp2f((af - 1.0f) * __builtin_amdgcn_logf(sm));
-        const double one = (double)en.uap + (double)pow_alpha(vv, af);
+        const double one = (double)en.uap + (double)gen_pow(vv, alpha);   // (as the sums' terms)
         const double ls = __longlong_as_double((long long)en.q8);
         const double cn = (double)p1 * (double)__builtin_fabsf(uu - vv) - one;
         const double cd = (double)p1 * (double)sm - one;
@@ -2445,13 +2457,13 @@ __global__ void __launch_bounds__(256) k_matrix_to_stripes(double* __restrict__ 
     st[(size_t)blockIdx.y * n + k] = (double)M[(size_t)k * n + l];
 }
 
-template <PairOp OP, class T, int RI, int RJ, int TT = PAIR_TILE, bool LPRE = false, bool ACC = false>
+template <PairOp OP, class T, int RI, int RJ, int TT = PAIR_TILE, bool LPRE = false, int PW = 0>
 hipError_t launch_pairs_t(const PairArgs& a, hipStream_t st) {
     static_assert(PAIR_TILE % TT == 0, "the row stride is padded to PAIR_TILE");
     const uint32_t tiles = (a.n + TT - 1) / TT;
     dim3 grid(tiles, tiles);
     constexpr int NT = (TT / RI) * (TT / RJ);
-    hipLaunchKernelGGL((k_pairs<OP, T, RI, RJ, TT, LPRE, ACC>), grid, dim3(NT), 0, st, (const T*)a.U, (const T*)a.Cn, a.rows, a.ldn,
+    hipLaunchKernelGGL((k_pairs<OP, T, RI, RJ, TT, LPRE, PW>), grid, dim3(NT), 0, st, (const T*)a.U, (const T*)a.Cn, a.rows, a.ldn,
                        (const T*)a.len, a.K, a.n, a.S, a.totals, a.alpha, (int)a.normalized, a.out, (int)a.out_fp64,
                        a.band, a.out2 ? 1 : 0, a.out2, a.flags, a.flag_tau);
     return hipGetLastError();
@@ -2464,10 +2476,12 @@ hipError_t launch_pairs_prec(PairOp op, const PairArgs& a, hipStream_t st) {
     switch (op) {
     case PairOp::weighted: return launch_pairs_t<PairOp::weighted, T, 8, 8>(a, st);
     case PairOp::unweighted_plain: return launch_pairs_t<PairOp::unweighted_plain, T, 4, 8>(a, st);
-    // FP32 generalized powers with the exponent kept exact for alpha outside [0, 1] (pow_alpha2_split)
+    // FP32 generalized powers (pair_term's PW): alpha = 1/2 by square roots, outside [0, 1]
+    // with the exponent kept exact, otherwise the hardware exp2(a log2 s)
     case PairOp::generalized:
+        if (sizeof(T) == 4 && a.alpha == 0.5) return launch_pairs_t<PairOp::generalized, T, 4, 4, 64, false, 2>(a, st);
         return sizeof(T) == 4 && !(a.alpha >= 0.0 && a.alpha <= 1.0)
-                   ? launch_pairs_t<PairOp::generalized, T, 4, 4, 64, false, true>(a, st)
+                   ? launch_pairs_t<PairOp::generalized, T, 4, 4, 64, false, 1>(a, st)
                    : launch_pairs_t<PairOp::generalized, T, 4, 4, 64>(a, st);
     case PairOp::vaw_weighted:
         // FP64 only: in FP32, rounding l u and l v separately doubles the worst-case error of
@@ -2475,8 +2489,10 @@ hipError_t launch_pairs_prec(PairOp op, const PairArgs& a, hipStream_t st) {
         return a.len_nonneg && sizeof(T) == 8 ? launch_pairs_t<PairOp::vaw_weighted, T, 4, 4, 64, true>(a, st)
                                               : launch_pairs_t<PairOp::vaw_weighted, T, 4, 4, 64>(a, st);
     case PairOp::vaw_generalized:
+        if (sizeof(T) == 4 && a.alpha == 0.5)
+            return launch_pairs_t<PairOp::vaw_generalized, T, 4, 4, 64, false, 2>(a, st);
         return sizeof(T) == 4 && !(a.alpha >= 0.0 && a.alpha <= 1.0)
-                   ? launch_pairs_t<PairOp::vaw_generalized, T, 4, 4, 64, false, true>(a, st)
+                   ? launch_pairs_t<PairOp::vaw_generalized, T, 4, 4, 64, false, 1>(a, st)
                    : launch_pairs_t<PairOp::vaw_generalized, T, 4, 4, 64>(a, st);
     case PairOp::vaw_unweighted:
         return a.len_nonneg ? launch_pairs_t<PairOp::vaw_unweighted, T, 4, 4, 64, true>(a, st)
