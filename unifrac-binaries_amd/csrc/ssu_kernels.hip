@@ -1341,17 +1341,27 @@ constexpr uint32_t S3_NODE = S3_JOFF + 8 * S3_GSJ;
 
 
 // generalized (GEN) terms, FP32 as in the dense kernel: w = s^a l, q = s^(a-1) l
-// from the hardware log2 / exp2 (s = u + v > 0), num += q |u - v|, den += w
+// from the hardware log2 / exp2 (s = u + v > 0), or, HALF (a = 1/2), from the
+// hardware square root and reciprocal square root; num += q |u - v|, den += w
+template <bool HALF = false>
 __device__ __forceinline__ void gen_term(float u, float v, float l, float alpha, float& num, float& den) {
     const float s = u + v;
     const bool nz = s != 0.0f;
-    const float lg = __builtin_amdgcn_logf(nz ? s : 1.0f);
-    const float w = nz ? __builtin_amdgcn_exp2f(alpha * lg) * l : 0.0f;
-    const float q = nz ? __builtin_amdgcn_exp2f((alpha - 1.0f) * lg) * l : 0.0f;
+    float w, q;
+    if constexpr (HALF) {
+        const float ss = nz ? s : 1.0f;
+        w = nz ? __builtin_amdgcn_sqrtf(ss) * l : 0.0f;
+        q = nz ? __builtin_amdgcn_rsqf(ss) * l : 0.0f;
+    } else {
+        const float lg = __builtin_amdgcn_logf(nz ? s : 1.0f);
+        w = nz ? __builtin_amdgcn_exp2f(alpha * lg) * l : 0.0f;
+        q = nz ? __builtin_amdgcn_exp2f((alpha - 1.0f) * lg) * l : 0.0f;
+    }
     num = __builtin_fmaf(q, __builtin_fabsf(u - v), num);
     den += w;
 }
 // FP64 as in the dense FP64 kernel: w = s^a l, num += w |u - v| / s, den += w
+template <bool HALF = false>   // (FP64: dpow for every alpha)
 __device__ __forceinline__ void gen_term(double u, double v, double l, double alpha, double& num, double& den) {
     const double s = u + v;
     const bool nz = s != 0.0;
@@ -1365,7 +1375,7 @@ __device__ __forceinline__ void gen_term(double u, double v, double l, double al
 // processed nodes, summed in FP32 over S3_MERGE listed nodes and then merged
 // into FP64; FP64 operands (generalized only) are the embedding rows
 // U[rows[r]], summed in FP64.
-template <class TOut, bool GEN, class V = float>
+template <class TOut, bool GEN, class V = float, bool HALF = false>
 __device__ __forceinline__ void pairs_sparse32_body(const V* __restrict__ U, const int32_t* __restrict__ rows,
                                                        size_t ldn, const double* __restrict__ len,
                                                        const uint32_t* __restrict__ masks, uint32_t Kw, uint32_t n,
@@ -1460,7 +1470,7 @@ __device__ __forceinline__ void pairs_sparse32_body(const V* __restrict__ U, con
 #pragma unroll
                     for (int c = 0; c < 8; ++c) {
                         if constexpr (GEN)
-                            gen_term(a[r], b[c], ln, alpha, acc[r][c], accd[r][c]);
+                            gen_term<HALF>(a[r], b[c], ln, alpha, acc[r][c], accd[r][c]);
                         else
                             acc[r][c] = __builtin_fmaf(__builtin_fabsf(a[r] - b[c]), ln, acc[r][c]);
                     }
@@ -1526,13 +1536,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 }
 
 // generalized: numerator and denominator summed directly (two accumulator sets, 2 waves per SIMD)
-template <class TOut>
+template <class TOut, bool HALF = false>
 __global__ void __launch_bounds__(64) k_pairs_gen32(
         const float* __restrict__ U, size_t ldn, const double* __restrict__ len, const uint32_t* __restrict__ masks,
         uint32_t Kw, uint32_t n, TOut* __restrict__ out, TOut* __restrict__ out2, uint32_t* __restrict__ visits,
         const uint32_t* __restrict__ order, uint32_t n_blocks, uint32_t per_xcd, const uint32_t* __restrict__ dmask,
         int add_partial, float alpha) {
-    pairs_sparse32_body<TOut, true>(U, nullptr, ldn, len, masks, Kw, n, nullptr, 1, out, visits, order, n_blocks,
+    pairs_sparse32_body<TOut, true, float, HALF>(U, nullptr, ldn, len, masks, Kw, n, nullptr, 1, out, visits, order, n_blocks,
                                     per_xcd, dmask, add_partial, alpha, out2);
 }
 
@@ -1851,7 +1861,8 @@ __device__ __forceinline__ void row_term(unsigned long long* accb, const RowEntr
         // a pair's correction to u^a + v^a (held by both): numerator and denominator
         const float af = (float)alpha;
         const float uu = (float)en.lu, vv = (float)v, sm = uu + vv;
-        const float p1 = __builtin_amdgcn_exp2f((af - 1.0f) * __builtin_amdgcn_logf(sm));
+        const float p1 = alpha == 0.5 ? __builtin_amdgcn_rsqf(sm)
+                                      : __builtin_amdgcn_exp2f((af - 1.0f) * __builtin_amdgcn_logf(sm));
         const double one = (double)en.uap + (double)gen_pow(vv, alpha);   // (as the sums' terms)
         const double ls = __longlong_as_double((long long)en.q8);
         const double cn = (double)p1 * (double)__builtin_fabsf(uu - vv) - one;
@@ -3077,9 +3088,14 @@ hipError_t launch_pairs_gen32(const SparseArgs& a, float alpha, void* out2, hipS
     if (!a.order) return hipErrorInvalidValue;
     if (!a.n_blocks) return hipSuccess;   // a stripe selection without a block here
     const uint32_t per_xcd = (a.n_blocks + N_XCDS - 1) / N_XCDS;
-    hipLaunchKernelGGL((k_pairs_gen32<float>), dim3(per_xcd * N_XCDS), dim3(64), 0, st, (const float*)a.U, a.ldn, a.len,
-                       a.masks, a.Kw, a.n, (float*)a.out, (float*)out2, a.visits, a.order, a.n_blocks, per_xcd,
-                       a.dmask, (int)(a.dmask != nullptr), alpha);
+    if (alpha == 0.5f)   // square roots in place of log2 / exp2 (gen_term's HALF)
+        hipLaunchKernelGGL((k_pairs_gen32<float, true>), dim3(per_xcd * N_XCDS), dim3(64), 0, st, (const float*)a.U,
+                           a.ldn, a.len, a.masks, a.Kw, a.n, (float*)a.out, (float*)out2, a.visits, a.order,
+                           a.n_blocks, per_xcd, a.dmask, (int)(a.dmask != nullptr), alpha);
+    else
+        hipLaunchKernelGGL((k_pairs_gen32<float>), dim3(per_xcd * N_XCDS), dim3(64), 0, st, (const float*)a.U, a.ldn,
+                           a.len, a.masks, a.Kw, a.n, (float*)a.out, (float*)out2, a.visits, a.order, a.n_blocks,
+                           per_xcd, a.dmask, (int)(a.dmask != nullptr), alpha);
     return hipGetLastError();
 }
 
